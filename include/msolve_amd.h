@@ -1,0 +1,217 @@
+/*
+ * msolve_amd.h -- C ABI of the MI355X (gfx950) implementation of MSolve's
+ * linear-elastic Hexa8 stiffness -> assembly -> PCG path, driven per Monte
+ * Carlo sample.  Plain C types only; every entry point returns an int status
+ * (MSA_OK = 0) and never throws.  Host arrays are owned by the caller and only
+ * read/written during the call; device memory is owned by the msa_* objects.
+ *
+ * One msa_ctx per host thread (or per stream); objects created from a ctx are
+ * used with that ctx only.  Calls on different contexts are independent.
+ *
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the isaar/MSolve tree).
+ */
+#ifndef MSOLVE_AMD_H
+#define MSOLVE_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+enum {
+    MSA_OK = 0,
+    MSA_NOT_CONVERGED = 1, /* PCG hit max iterations (the reference is silent
+                              here, SolverPCG.cs:117-138) */
+    MSA_BREAKDOWN = 2,     /* p.q <= 0 or non-finite scalar in PCG */
+    MSA_NEG_DETJ = 3,      /* Jacobian determinant < 1e-8 (rElHexa8.f90:203) */
+    MSA_NAN_INPUT = 4,     /* non-finite modulus / load */
+    MSA_ERR_ARG = -1,
+    MSA_ERR_HIP = -2,
+    MSA_ERR_NO_DEVICE = -3,
+    MSA_ERR_OOM = -4,
+    MSA_ERR_STATE = -5
+};
+
+enum { MSA_FP64 = 0, MSA_MIXED = 1 };
+
+typedef struct msa_ctx msa_ctx;
+typedef struct msa_mesh msa_mesh;
+typedef struct msa_batch msa_batch;
+
+/* Last error message of the calling thread (static storage). */
+const char *msa_last_error(void);
+int msa_version(void);
+
+/* ------------------------------------------------- Fortran drop-in (ABI)
+ * Same names, argument lists and layouts as the femelements.dll imports of
+ * ISAAR.MSolve.PreProcessor/Elements/Hexa8.cs:27-55 (cdecl, everything by
+ * reference), replacing FEMElements/rElHexa8.f90:422-464 (CalcH8GaussMatrices)
+ * and :489-543 (CalcH8K).  Computed on the GPU of a lazily created default
+ * context (device 0); one small launch per call.  Only iInt == 2 is
+ * supported (the reference's Hexa8 uses 2, Hexa8.cs:17).  On error the
+ * outputs are filled with NaN (the Fortran has no error channel).
+ *   xyz[8][3]; w[8]; S[8][8]; DS[8][24]; J[8] Fortran (3,3); detJ[8];
+ *   B[8][24][6] (= Fortran faB(6,24,8)); E[8][6][6]; K[300] packed upper.  */
+void CALCH8GAUSSMATRICES(const int *iInt, const double *xyz, double *w,
+                         double *S, double *DS, double *J, double *detJ,
+                         double *B);
+void CALCH8K(const int *iInt, const double *E, const double *B,
+             const double *w, double *K);
+
+/* --------------------------------------------------------------- context */
+int msa_ctx_create(int device, msa_ctx **out);
+int msa_ctx_destroy(msa_ctx *ctx);
+/* The HIP stream every kernel of this context runs on (hipStream_t). */
+void *msa_ctx_stream(msa_ctx *ctx);
+int msa_ctx_synchronize(msa_ctx *ctx);
+
+/* ---------------------------------------------------- model / numbering
+ * Free-DOF numbering of Model.EnumerateGlobalDOFs / Subdomain.EnumerateDOFs
+ * (ISAAR.MSolve.PreProcessor/Model.cs:165-215, Subdomain.cs:77-168): nodes in
+ * ascending index, DOFs X, Y, Z, constrained -> -1 (eliminated).
+ * constrained[3*node + d] != 0 marks a constraint. */
+int msa_enumerate_dofs(int n_nodes, const unsigned char *constrained, int *dof,
+                       int *n_free);
+
+/* A Hexa8 mesh: node coordinates xyz[n_nodes][3], 0-based connectivity
+ * conn[n_elem][8] in the Hexa8 node order (rElHexa8.f90:25-117), the free-DOF
+ * map dof[n_nodes][3] from msa_enumerate_dofs (or the reference's own
+ * NodalDOFsDictionary).  Builds the node-block sparsity of the global K, the
+ * element -> block-slot map and an element colouring on the host, and
+ * uploads mesh data to the device.  Replaces the per-sample skyline profile
+ * + scatter bookkeeping of GlobalMatrixAssemblerSkyline.cs:14-103. */
+int msa_mesh_create(msa_ctx *ctx, int n_nodes, const double *xyz, int n_elem,
+                    const int *conn, const int *dof, int n_free,
+                    msa_mesh **out);
+/* info[0] free node blocks, [1] block nonzeros, [2] scalar nonzeros of the
+ * free-DOF K (full symmetric), [3] colours, [4] max blocks per row,
+ * [5] n_free, [6] n_elem, [7] n_nodes. */
+int msa_mesh_info(const msa_mesh *mesh, long long *info);
+int msa_mesh_destroy(msa_mesh *mesh);
+
+/* ------------------------------------------------- element stiffness
+ * Batched Hexa8WithStochasticMaterial.StiffnessMatrix / Hexa8.StiffnessMatrix
+ * behind IElementMatrixProvider.Matrix (ISAAR.MSolve.PreProcessor/Elements/
+ * Hexa8WithStochasticMaterial.cs:61-110, Hexa8.cs:368-401,
+ * Providers/ElementStructuralStiffnessProvider.cs:11-38): 2x2x2 Gauss
+ * B^T D B with isotropic D(E[e], nu).  xyz8[n_elem][8][3] element node
+ * coordinates; ke300[n_elem][300] packed upper triangle row-major
+ * (SymmetricMatrix2D.cs:22-49).  elem_status[n_elem] (may be NULL) gets
+ * MSA_OK or MSA_NEG_DETJ.  FP64 on the GPU. */
+int msa_ke_packed(msa_ctx *ctx, int n_elem, const double *xyz8,
+                  const double *E, double nu, double *ke300, int *elem_status);
+
+/* ----------------------------------------------------------- MC batch
+ * A batch solves many stochastic-modulus samples at once: n_classes meshes
+ * (uneven sizes allowed), n_samples[c] samples of mesh c.  Samples of a class
+ * share the sparsity and the load vector; each has its own modulus field,
+ * stiffness values and solution.  Replaces the per-sample loop of
+ * MonteCarloAnalyzerWithStochasticMaterial.SolveNormal
+ * (ISAAR.MSolve.Analyzers/MonteCarloAnalyzerWithStochasticMaterial.cs:
+ * 232-296) with its StaticAnalyzer -> LinearAnalyzer -> SolverPCG chain.
+ *   precision   MSA_FP64: FP64 element integration, assembly and PCG.
+ *               MSA_MIXED: FP32 integration/assembly/inner PCG, FP64 outer
+ *               residual f - K u (matrix-free), update and stopping test.
+ *   tile        samples per lane tile: 1, 2, 4, 8, 16, 32 or 64 (0 = pick:
+ *               64 when a class has more than 32 samples, else 1). */
+int msa_batch_create(msa_ctx *ctx, int n_classes, msa_mesh *const *meshes,
+                     const int *n_samples, int precision, int tile,
+                     msa_batch **out);
+int msa_batch_destroy(msa_batch *b);
+/* Device bytes held by the batch. */
+long long msa_batch_bytes(const msa_batch *b);
+
+/* Modulus per element per sample, E[n_samples][n_elem] (host). */
+int msa_batch_set_modulus(msa_batch *b, int cls, const double *E);
+
+/* Lognormal Karhunen-Loeve modulus field of
+ * LognormalFileStochasticCoefficientsProvider.GetCoefficient
+ * (ISAAR.MSolve.PreProcessor/Stochastic/
+ * LognormalFileStochasticCoefficientsProvider.cs:134-193), evaluated at the
+ * element centroid rule of Hexa8WithStochasticMaterial.cs:112-139.  Table in
+ * the reference's KL-file fields (:90-109): w1d[3][kmax], a1d[3][kmax] per
+ * 1-based 1-D mode, evpPos[M][3] (1-based), eigs[M]; domain[3] the
+ * stochastic domain lengths (coordinates are centred by domain/2).
+ * cov replaces the reference's hard-coded 0.5 (:115-117).  The per-element
+ * basis is built once per class on the device. */
+int msa_batch_set_kl(msa_batch *b, int cls, int M, int kmax,
+                     const double *w1d, const double *a1d, const int *evpPos,
+                     const double *eigs, const double *domain, double meanE,
+                     double cov);
+/* Standard-normal KL coefficients xi[n_samples][M] of class cls (the
+ * RandomVariables of MC...Material.cs:261). */
+int msa_batch_set_xi(msa_batch *b, int cls, const double *xi);
+/* E = exp(ln meanE - sigma^2/2 + sum_i xi_i phi_i(centroid)) on the device,
+ * all classes with a KL basis. */
+int msa_batch_eval_kl(msa_batch *b);
+
+/* Load vector f[n_free] of class cls, shared by its samples
+ * (Subdomain.Forces, Model.AssignNodalLoads, Model.cs:227-238). */
+int msa_batch_set_rhs(msa_batch *b, int cls, const double *f);
+
+/* Element stiffness + coloured scatter into the block image of K for every
+ * sample, plus the Jacobi diagonal (ProblemStructural.BuildKs ->
+ * GlobalMatrixAssemblerSkyline.CalculateGlobalMatrix,
+ * ISAAR.MSolve.Problems/ProblemStructural.cs:63-89, GMAS.cs:51-103). */
+int msa_batch_assemble(msa_batch *b, double nu);
+
+/* Solve K u = f for every sample (SolverPCG.Solve, ISAAR.MSolve.Solvers/PCG/
+ * SolverPCG.cs:105-139, with the simple search vector calculator
+ * SolverPCGSimpleSearchVectorCalculator.cs:18-41; x0 = 0, stop when
+ * |r|/|f| < tol).  Preconditioner: Jacobi.  FP64: tol on the PCG residual,
+ * max_it PCG iterations.  MIXED: tol on the FP64 residual |f - K u|/|f|,
+ * inner FP32 PCG to inner_tol (relative) with at most max_it iterations,
+ * at most max_outer refinements.  graph_iters > 0 captures that many PCG
+ * iterations per hipGraph launch (even number), 0 launches eagerly. */
+int msa_batch_solve(msa_batch *b, double tol, int max_it, double inner_tol,
+                    int max_outer, int graph_iters);
+
+/* Modulus field -> assembly -> solve in one call (the whole MC job with all
+ * inputs resident on the device). */
+int msa_batch_run(msa_batch *b, double nu, double tol, int max_it,
+                  double inner_tol, int max_outer, int graph_iters);
+
+/* Solution of one sample in the reference free-DOF numbering, u[n_free]. */
+int msa_batch_get_solution(msa_batch *b, int cls, int sample, double *u);
+/* Per-sample results of class cls: iters[n] PCG iterations (matrix-vector
+ * products; mixed: summed over inner solves), outer[n] refinements (FP64:
+ * 0), relres[n] final relative residual (mixed: the FP64 |f-Ku|/|f|),
+ * status[n].  Any pointer may be NULL. */
+int msa_batch_get_stats(msa_batch *b, int cls, int *iters, int *outer,
+                        double *relres, int *status);
+/* The modulus field of one sample, E[n_elem]. */
+int msa_batch_get_modulus(msa_batch *b, int cls, int sample, double *E);
+
+/* Assembled K of one sample in the reference free-DOF numbering as full
+ * symmetric CSR: rowptr[n_free+1], col[nnz], val[nnz] (nnz from
+ * msa_mesh_info info[2]); MIXED batches return the FP32 values widened. */
+int msa_batch_get_matrix_csr(msa_batch *b, int cls, int sample,
+                             long long *rowptr, int *col, double *val);
+/* y = K x for one sample with the batch's SpMV kernel (x, y in the
+ * reference numbering, FP64 on the host). */
+int msa_batch_spmv(msa_batch *b, int cls, int sample, const double *x,
+                   double *y);
+
+/* Kernel timing with HIP events on the context stream (eager launches while
+ * enabled).  kind: 0 SpMV+dot kernel, 1 update kernel, 2 assembly,
+ * 3 FP64 residual, 4 other.  ms_total[5], launches[5]. */
+int msa_batch_set_kernel_timing(msa_batch *b, int enable);
+int msa_batch_get_kernel_timing(msa_batch *b, double *ms_total,
+                                long long *launches);
+/* Algorithmic bytes of one launch of the SpMV+dot kernel over every sample
+ * of the batch (values + shared indices + vectors), and its element count. */
+int msa_batch_spmv_bytes(msa_batch *b, double *bytes_per_launch);
+
+/* ----------------------------------------------------------- autotuner
+ * Times the SpMV+dot kernel of mesh `mesh` for every lane tile
+ * (1..64 samples per wave) with n_samples synthetic samples and writes the
+ * fastest tile to *best_tile; ms_per_tile[7] receives the timings (tiles
+ * 1, 2, 4, 8, 16, 32, 64; negative = skipped). */
+int msa_autotune_tile(msa_ctx *ctx, msa_mesh *mesh, int n_samples,
+                      int precision, int *best_tile, double *ms_per_tile);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,224 @@
+"""Object wrappers over the C ABI: Context, Mesh, Batch."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import check, f64, i32, lib, ptr
+
+
+class Context:
+    """One HIP device + stream (msa_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().msa_ctx_create(device, C.byref(h)))
+        self.h = h
+        self.device = device
+
+    @property
+    def stream(self) -> int:
+        return lib().msa_ctx_stream(self.h)
+
+    def synchronize(self):
+        check(lib().msa_ctx_synchronize(self.h))
+
+    def close(self):
+        if self.h:
+            lib().msa_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def enumerate_dofs(constrained):
+    """Free-DOF numbering of Model.EnumerateGlobalDOFs (Model.cs:165-215):
+    nodes ascending, X/Y/Z, constrained -> -1.  Returns (dof, n_free)."""
+    c = np.ascontiguousarray(constrained, np.uint8).ravel()
+    dof = np.zeros(c.size, np.int32)
+    n = C.c_int()
+    check(lib().msa_enumerate_dofs(c.size // 3, c.ctypes.data_as(C.c_void_p), ptr(dof, C.c_int),
+                                   C.byref(n)))
+    return dof, n.value
+
+
+class Mesh:
+    """Hexa8 mesh + free-DOF map uploaded to the device (msa_mesh)."""
+
+    def __init__(self, ctx: Context, xyz, conn, dof, n_free: int):
+        self.ctx = ctx
+        self.xyz = f64(xyz).reshape(-1, 3)
+        self.conn = i32(conn).reshape(-1, 8)
+        self.dof = i32(dof).ravel()
+        self.n_free = int(n_free)
+        h = C.c_void_p()
+        check(lib().msa_mesh_create(ctx.h, self.xyz.shape[0], ptr(self.xyz, C.c_double),
+                                    self.conn.shape[0], ptr(self.conn, C.c_int),
+                                    ptr(self.dof, C.c_int), self.n_free, C.byref(h)))
+        self.h = h
+        info = np.zeros(8, np.int64)
+        check(lib().msa_mesh_info(self.h, ptr(info, C.c_longlong)))
+        self.info = dict(blocks=int(info[0]), block_nnz=int(info[1]), nnz=int(info[2]),
+                         colours=int(info[3]), max_row=int(info[4]), n_free=int(info[5]),
+                         n_elem=int(info[6]), n_nodes=int(info[7]))
+
+    @property
+    def n_elem(self):
+        return self.conn.shape[0]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().msa_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ke_packed(ctx: Context, xyz8, E, nu: float):
+    """Batched element stiffness (packed 300 upper entries per element) and
+    per-element status (MSA_OK / MSA_NEG_DETJ)."""
+    x = f64(xyz8).reshape(-1, 24)
+    E = f64(E).ravel()
+    n = x.shape[0]
+    ke = np.zeros((n, 300))
+    st = np.zeros(n, np.int32)
+    check(lib().msa_ke_packed(ctx.h, n, ptr(x, C.c_double), ptr(E, C.c_double), nu,
+                              ptr(ke, C.c_double), ptr(st, C.c_int)))
+    return ke, st
+
+
+class Batch:
+    """Many stochastic-modulus samples over one or more meshes (msa_batch)."""
+
+    def __init__(self, ctx: Context, meshes, n_samples, precision=L.MSA_FP64, tile=0):
+        self.ctx = ctx
+        self.meshes = list(meshes)
+        self.n_samples = [int(s) for s in n_samples]
+        self.precision = precision
+        arr = (C.c_void_p * len(self.meshes))(*[m.h for m in self.meshes])
+        ns = np.array(self.n_samples, np.int32)
+        h = C.c_void_p()
+        check(lib().msa_batch_create(ctx.h, len(self.meshes), arr, ptr(ns, C.c_int), precision,
+                                     tile, C.byref(h)))
+        self.h = h
+
+    @property
+    def device_bytes(self) -> int:
+        return int(lib().msa_batch_bytes(self.h))
+
+    def set_modulus(self, cls: int, E):
+        E = f64(E)
+        assert E.size == self.n_samples[cls] * self.meshes[cls].n_elem
+        check(lib().msa_batch_set_modulus(self.h, cls, ptr(E, C.c_double)))
+
+    def set_kl(self, cls: int, table, domain, meanE: float, cov: float):
+        w = f64(table["w1d"])
+        a = f64(table["a1d"])
+        pos = i32(table["evpPos"])
+        eig = f64(table["eigs"])
+        dom = f64(domain)
+        check(lib().msa_batch_set_kl(self.h, cls, int(table["M"]), int(table["kmax"]),
+                                     ptr(w, C.c_double), ptr(a, C.c_double), ptr(pos, C.c_int),
+                                     ptr(eig, C.c_double), ptr(dom, C.c_double), meanE, cov))
+
+    def set_xi(self, cls: int, xi):
+        xi = f64(xi)
+        check(lib().msa_batch_set_xi(self.h, cls, ptr(xi, C.c_double)))
+
+    def eval_kl(self):
+        check(lib().msa_batch_eval_kl(self.h))
+
+    def set_rhs(self, cls: int, f):
+        f = f64(f)
+        assert f.size == self.meshes[cls].n_free
+        check(lib().msa_batch_set_rhs(self.h, cls, ptr(f, C.c_double)))
+
+    def assemble(self, nu: float = 0.3):
+        check(lib().msa_batch_assemble(self.h, nu))
+
+    def solve(self, tol=1e-10, max_it=20000, inner_tol=1e-4, max_outer=20, graph_iters=0):
+        check(lib().msa_batch_solve(self.h, tol, max_it, inner_tol, max_outer, graph_iters))
+
+    def run(self, nu=0.3, tol=1e-10, max_it=20000, inner_tol=1e-4, max_outer=20, graph_iters=0):
+        check(lib().msa_batch_run(self.h, nu, tol, max_it, inner_tol, max_outer, graph_iters))
+
+    def solution(self, cls: int, sample: int):
+        u = np.zeros(self.meshes[cls].n_free)
+        check(lib().msa_batch_get_solution(self.h, cls, sample, ptr(u, C.c_double)))
+        return u
+
+    def stats(self, cls: int):
+        n = self.n_samples[cls]
+        it = np.zeros(n, np.int32)
+        outer = np.zeros(n, np.int32)
+        rel = np.zeros(n)
+        st = np.zeros(n, np.int32)
+        check(lib().msa_batch_get_stats(self.h, cls, ptr(it, C.c_int), ptr(outer, C.c_int),
+                                        ptr(rel, C.c_double), ptr(st, C.c_int)))
+        return dict(iters=it, outer=outer, relres=rel, status=st)
+
+    def modulus(self, cls: int, sample: int):
+        E = np.zeros(self.meshes[cls].n_elem)
+        check(lib().msa_batch_get_modulus(self.h, cls, sample, ptr(E, C.c_double)))
+        return E
+
+    def matrix_csr(self, cls: int, sample: int):
+        m = self.meshes[cls]
+        nnz = m.info["nnz"]
+        rp = np.zeros(m.n_free + 1, np.int64)
+        col = np.zeros(nnz, np.int32)
+        val = np.zeros(nnz)
+        check(lib().msa_batch_get_matrix_csr(self.h, cls, sample, ptr(rp, C.c_longlong),
+                                             ptr(col, C.c_int), ptr(val, C.c_double)))
+        return rp, col, val
+
+    def spmv(self, cls: int, sample: int, x):
+        x = f64(x)
+        y = np.zeros(self.meshes[cls].n_free)
+        check(lib().msa_batch_spmv(self.h, cls, sample, ptr(x, C.c_double), ptr(y, C.c_double)))
+        return y
+
+    def set_kernel_timing(self, on: bool):
+        check(lib().msa_batch_set_kernel_timing(self.h, 1 if on else 0))
+
+    def kernel_timing(self):
+        ms = np.zeros(5)
+        n = np.zeros(5, np.int64)
+        check(lib().msa_batch_get_kernel_timing(self.h, ptr(ms, C.c_double),
+                                                ptr(n, C.c_longlong)))
+        names = ["spmv_dot", "update", "assemble", "residual64", "other"]
+        return {k: dict(ms=float(ms[i]), launches=int(n[i])) for i, k in enumerate(names)}
+
+    def spmv_bytes(self):
+        b = np.zeros(2)
+        check(lib().msa_batch_spmv_bytes(self.h, ptr(b, C.c_double)))
+        return dict(kernel=float(b[0]), spmv=float(b[1]))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().msa_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune_tile(ctx: Context, mesh: Mesh, n_samples: int, precision=L.MSA_MIXED):
+    best = C.c_int()
+    ms = np.zeros(7)
+    check(lib().msa_autotune_tile(ctx.h, mesh.h, n_samples, precision, C.byref(best),
+                                  ptr(ms, C.c_double)))
+    return best.value, dict(zip([1, 2, 4, 8, 16, 32, 64], ms.tolist()))

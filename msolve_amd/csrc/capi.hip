@@ -1,0 +1,1104 @@
+// C ABI of msolve_amd (include/msolve_amd.h): contexts, meshes, Monte Carlo
+// batches and their solve drivers.  All device work runs on the context's
+// stream; host syncs happen only where a result must reach the host (the
+// convergence poll between captured iteration blocks, downloads).
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "msa_common.hpp"
+
+namespace msa {
+
+static thread_local std::string g_last_error;
+
+void set_error(const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+}
+
+}  // namespace msa
+
+using namespace msa;
+
+// ------------------------------------------------------------------ batch
+
+namespace {
+
+struct ClassHost {
+    msa_mesh *mesh = nullptr;
+    int nsamp = 0, ntiles = 0, nslices = 0, nchunks = 0, chunk = 0;
+    std::vector<int> slice_ptr, cidx;
+    long long tile_vals = 0, tile_vec = 0;
+    ClassDev dev{};
+    bool has_rhs = false, has_E = false, has_kl = false, has_xi = false;
+    int M = 0;
+    double fnorm = 0;
+};
+
+struct Timing {
+    bool on = false;
+    double ms[5] = {0, 0, 0, 0, 0};
+    long long n[5] = {0, 0, 0, 0, 0};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+}  // namespace
+
+struct msa_batch {
+    msa_ctx *ctx = nullptr;
+    int precision = MSA_FP64, T = 1, R = 64;
+    std::vector<ClassHost> cls;
+    ClassDev *d_cls = nullptr;
+    std::vector<int4> items;
+    int4 *d_items = nullptr;
+    int *d_counts = nullptr;  // [0] PCG active samples, [1] refinement active samples
+    std::vector<void *> allocs;
+    long long bytes = 0;
+    bool assembled = false;
+    double nu = 0.3;
+    // captured PCG iteration block
+    hipGraphExec_t graph = nullptr;
+    int graph_iters = 0;
+    double graph_tol = -1;
+    int graph_maxit = -1;
+    Timing timing;
+};
+
+namespace {
+
+bool is_mixed(const msa_batch *b) { return b->precision == MSA_MIXED; }
+size_t real_size(const msa_batch *b) { return is_mixed(b) ? sizeof(float) : sizeof(double); }
+
+int dalloc(msa_batch *b, void **p, size_t bytes)
+{
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("hipMalloc(%zu bytes) failed: %s (batch holds %lld bytes)", bytes,
+                  hipGetErrorString(e), b->bytes);
+        *p = nullptr;
+        return MSA_ERR_OOM;
+    }
+    b->allocs.push_back(*p);
+    b->bytes += (long long)bytes;
+    return MSA_OK;
+}
+
+#define MSA_TRY(x)                      \
+    do {                                \
+        int rc_ = (x);                  \
+        if (rc_ != MSA_OK) return rc_;  \
+    } while (0)
+
+// Timed launch helper: with timing on, bracket the launch with events on the
+// context stream and wait for it.
+template <typename F>
+int timed(msa_batch *b, int kind, F &&launch)
+{
+    hipStream_t s = b->ctx->stream;
+    if (b->timing.on) MSA_HIP(hipEventRecord(b->timing.e0, s));
+    hipError_t e = launch();
+    if (e != hipSuccess) {
+        set_error("kernel launch failed: %s", hipGetErrorString(e));
+        return MSA_ERR_HIP;
+    }
+    if (b->timing.on) {
+        MSA_HIP(hipEventRecord(b->timing.e1, s));
+        MSA_HIP(hipEventSynchronize(b->timing.e1));
+        float ms = 0;
+        MSA_HIP(hipEventElapsedTime(&ms, b->timing.e0, b->timing.e1));
+        b->timing.ms[kind] += ms;
+        b->timing.n[kind] += 1;
+    }
+    return MSA_OK;
+}
+
+int upload_class_table(msa_batch *b)
+{
+    std::vector<ClassDev> h(b->cls.size());
+    for (size_t c = 0; c < b->cls.size(); c++) h[c] = b->cls[c].dev;
+    MSA_HIP(hipMemcpyAsync(b->d_cls, h.data(), sizeof(ClassDev) * h.size(),
+                           hipMemcpyHostToDevice, b->ctx->stream));
+    MSA_HIP(hipStreamSynchronize(b->ctx->stream));
+    return MSA_OK;
+}
+
+int read_count(msa_batch *b, int which, int *out)
+{
+    MSA_HIP(hipMemcpyAsync(b->ctx->h_pinned, b->d_counts + which, sizeof(int),
+                           hipMemcpyDeviceToHost, b->ctx->stream));
+    MSA_HIP(hipStreamSynchronize(b->ctx->stream));
+    *out = *b->ctx->h_pinned;
+    return MSA_OK;
+}
+
+int launch_iteration_pair(msa_batch *b, int pbuf, double tol, int max_it)
+{
+    const bool mx = is_mixed(b);
+    hipStream_t s = b->ctx->stream;
+    MSA_TRY(timed(b, KK_SPMV, [&] {
+        return launch_pcg_a(s, mx, b->T, b->d_cls, b->d_items, (int)b->items.size(), pbuf,
+                            b->d_counts);
+    }));
+    MSA_TRY(timed(b, KK_UPDATE, [&] {
+        return launch_pcg_b(s, mx, b->T, b->d_cls, b->d_items, (int)b->items.size(), pbuf, tol,
+                            max_it, b->d_counts);
+    }));
+    return MSA_OK;
+}
+
+int build_graph(msa_batch *b, int iters, double tol, int max_it)
+{
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    hipStream_t s = b->ctx->stream;
+    MSA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = MSA_OK;
+    for (int k = 0; k < iters && rc == MSA_OK; k++) rc = launch_iteration_pair(b, k & 1, tol, max_it);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc != MSA_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    MSA_HIP(e);
+    e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    MSA_HIP(e);
+    b->graph_iters = iters;
+    b->graph_tol = tol;
+    b->graph_maxit = max_it;
+    return MSA_OK;
+}
+
+// Iterate the PCG kernels until no sample of the batch is active.
+int pcg_loop(msa_batch *b, double tol, int max_it, int graph_iters)
+{
+    const bool use_graph = graph_iters > 0 && !b->timing.on;
+    const int block = use_graph ? graph_iters : 8;
+    if (use_graph && (b->graph == nullptr || b->graph_iters != graph_iters ||
+                      b->graph_tol != tol || b->graph_maxit != max_it))
+        MSA_TRY(build_graph(b, graph_iters, tol, max_it));
+    long long done = 0;
+    for (;;) {
+        int active = 0;
+        MSA_TRY(read_count(b, 0, &active));
+        if (active <= 0) break;
+        if (done > (long long)max_it + block) {
+            set_error("PCG driver: samples still active after %lld iterations", done);
+            return MSA_ERR_STATE;
+        }
+        if (use_graph) {
+            MSA_HIP(hipGraphLaunch(b->graph, b->ctx->stream));
+        } else {
+            for (int k = 0; k < block; k++) MSA_TRY(launch_iteration_pair(b, k & 1, tol, max_it));
+        }
+        done += block;
+    }
+    return MSA_OK;
+}
+
+ColourLaunch colour_launch(const msa_batch *b, int colour, int *nblocks)
+{
+    ColourLaunch cl{};
+    cl.ncls = (int)b->cls.size();
+    cl.colour = colour;
+    cl.prefix[0] = 0;
+    for (size_t c = 0; c < b->cls.size(); c++) {
+        const MeshHost &m = b->cls[c].mesh->h;
+        int cnt = colour < m.ncolours ? m.colour_ptr[colour + 1] - m.colour_ptr[colour] : 0;
+        cl.prefix[c + 1] = cl.prefix[c] + cnt;
+    }
+    *nblocks = cl.prefix[cl.ncls];
+    return cl;
+}
+
+int max_colours(const msa_batch *b)
+{
+    int m = 0;
+    for (auto &c : b->cls) m = std::max(m, c.mesh->h.ncolours);
+    return m;
+}
+
+msa_ctx *g_default_ctx = nullptr;
+std::once_flag g_default_once;
+std::mutex g_default_mutex;
+double *g_drop_buf = nullptr;  // device scratch of the Fortran drop-ins
+
+}  // namespace
+
+// ================================================================ C ABI
+
+extern "C" {
+
+const char *msa_last_error(void) { return g_last_error.c_str(); }
+int msa_version(void) { return 100; }
+
+int msa_ctx_create(int device, msa_ctx **out)
+{
+    MSA_CHECK(out, MSA_ERR_ARG, "msa_ctx_create: out is null");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        set_error("no HIP device available (%s)", hipGetErrorString(e));
+        return MSA_ERR_NO_DEVICE;
+    }
+    MSA_CHECK(device >= 0 && device < n, MSA_ERR_ARG, "device %d out of range (%d devices)",
+              device, n);
+    MSA_HIP(hipSetDevice(device));
+    msa_ctx *c = new msa_ctx();
+    c->device = device;
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_pinned, 64, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        set_error("context setup failed: %s", hipGetErrorString(e));
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        delete c;
+        return MSA_ERR_HIP;
+    }
+    *out = c;
+    return MSA_OK;
+}
+
+int msa_ctx_destroy(msa_ctx *ctx)
+{
+    if (!ctx) return MSA_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return MSA_OK;
+}
+
+void *msa_ctx_stream(msa_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int msa_ctx_synchronize(msa_ctx *ctx)
+{
+    MSA_CHECK(ctx, MSA_ERR_ARG, "null context");
+    MSA_HIP(hipStreamSynchronize(ctx->stream));
+    return MSA_OK;
+}
+
+int msa_enumerate_dofs(int n_nodes, const unsigned char *constrained, int *dof, int *n_free)
+{
+    MSA_CHECK(n_nodes >= 0 && constrained && dof && n_free, MSA_ERR_ARG,
+              "msa_enumerate_dofs: bad arguments");
+    int total = 0;
+    for (int n = 0; n < n_nodes; n++)
+        for (int d = 0; d < 3; d++) dof[3 * n + d] = constrained[3 * n + d] ? -1 : total++;
+    *n_free = total;
+    return MSA_OK;
+}
+
+// ---------------------------------------------------------------- mesh
+
+int msa_mesh_create(msa_ctx *ctx, int n_nodes, const double *xyz, int n_elem, const int *conn,
+                    const int *dof, int n_free, msa_mesh **out)
+{
+    MSA_CHECK(ctx && out, MSA_ERR_ARG, "msa_mesh_create: null argument");
+    *out = nullptr;
+    msa_mesh *m = new msa_mesh();
+    m->ctx = ctx;
+    int rc = build_mesh_host(m->h, n_nodes, xyz, n_elem, conn, dof, n_free);
+    if (rc != MSA_OK) {
+        delete m;
+        return rc;
+    }
+    const MeshHost &h = m->h;
+    hipError_t e = hipSetDevice(ctx->device);
+    auto up = [&](void **dst, const void *src, size_t bytes) {
+        if (e != hipSuccess) return;
+        e = hipMalloc(dst, bytes ? bytes : 16);
+        if (e == hipSuccess && bytes)
+            e = hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+    };
+    up((void **)&m->d_xyz, h.xyz.data(), h.xyz.size() * sizeof(double));
+    up((void **)&m->d_conn, h.conn.data(), h.conn.size() * sizeof(int));
+    up((void **)&m->d_node_block, h.node_block.data(), h.node_block.size() * sizeof(int));
+    up((void **)&m->d_node_free, h.node_free.data(), h.node_free.size());
+    up((void **)&m->d_emap, h.emap.data(), h.emap.size());
+    up((void **)&m->d_colour_elems, h.colour_elems.data(), h.colour_elems.size() * sizeof(int));
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        set_error("mesh upload failed: %s", hipGetErrorString(e));
+        msa_mesh_destroy(m);
+        return MSA_ERR_HIP;
+    }
+    *out = m;
+    return MSA_OK;
+}
+
+int msa_mesh_info(const msa_mesh *mesh, long long *info)
+{
+    MSA_CHECK(mesh && info, MSA_ERR_ARG, "msa_mesh_info: null argument");
+    const MeshHost &h = mesh->h;
+    info[0] = h.nb;
+    info[1] = (long long)h.bcol.size();
+    info[2] = h.nnz_scalar;
+    info[3] = h.ncolours;
+    info[4] = h.max_row;
+    info[5] = h.n_free;
+    info[6] = h.n_elem;
+    info[7] = h.n_nodes;
+    return MSA_OK;
+}
+
+int msa_mesh_destroy(msa_mesh *m)
+{
+    if (!m) return MSA_OK;
+    (void)hipSetDevice(m->ctx->device);
+    for (void *p : {(void *)m->d_xyz, (void *)m->d_conn, (void *)m->d_node_block,
+                    (void *)m->d_node_free, (void *)m->d_emap, (void *)m->d_colour_elems})
+        if (p) (void)hipFree(p);
+    delete m;
+    return MSA_OK;
+}
+
+// ---------------------------------------------------------------- elements
+
+int msa_ke_packed(msa_ctx *ctx, int n_elem, const double *xyz8, const double *E, double nu,
+                  double *ke300, int *elem_status)
+{
+    MSA_CHECK(ctx && xyz8 && E && ke300 && n_elem >= 0, MSA_ERR_ARG, "msa_ke_packed: bad args");
+    if (n_elem == 0) return MSA_OK;
+    MSA_HIP(hipSetDevice(ctx->device));
+    double *dx = nullptr, *dE = nullptr, *dk = nullptr;
+    int *ds = nullptr;
+    hipStream_t s = ctx->stream;
+    hipError_t e = hipMalloc(&dx, sizeof(double) * 24 * (size_t)n_elem);
+    if (e == hipSuccess) e = hipMalloc(&dE, sizeof(double) * (size_t)n_elem);
+    if (e == hipSuccess) e = hipMalloc(&dk, sizeof(double) * 300 * (size_t)n_elem);
+    if (e == hipSuccess) e = hipMalloc(&ds, sizeof(int) * (size_t)n_elem);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dx, xyz8, sizeof(double) * 24 * (size_t)n_elem, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dE, E, sizeof(double) * (size_t)n_elem, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = launch_ke_packed(s, n_elem, dx, dE, nu, dk, ds);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(ke300, dk, sizeof(double) * 300 * (size_t)n_elem, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && elem_status)
+        e = hipMemcpyAsync(elem_status, ds, sizeof(int) * (size_t)n_elem, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    for (void *p : {(void *)dx, (void *)dE, (void *)dk, (void *)ds})
+        if (p) (void)hipFree(p);
+    if (e != hipSuccess) {
+        set_error("msa_ke_packed: %s", hipGetErrorString(e));
+        return MSA_ERR_HIP;
+    }
+    return MSA_OK;
+}
+
+static int default_ctx(msa_ctx **out)
+{
+    std::call_once(g_default_once, [] {
+        msa_ctx *c = nullptr;
+        if (msa_ctx_create(0, &c) == MSA_OK) {
+            if (hipMalloc(&g_drop_buf, sizeof(double) * 4096) == hipSuccess)
+                g_default_ctx = c;
+            else
+                msa_ctx_destroy(c);
+        }
+    });
+    *out = g_default_ctx;
+    if (!g_default_ctx) {
+        set_error("no default HIP context for the Fortran drop-ins");
+        return MSA_ERR_NO_DEVICE;
+    }
+    return MSA_OK;
+}
+
+static void fill_nan(double *p, size_t n)
+{
+    if (!p) return;
+    for (size_t i = 0; i < n; i++) p[i] = NAN;
+}
+
+void CALCH8GAUSSMATRICES(const int *iInt, const double *xyz, double *w, double *S, double *DS,
+                         double *J, double *detJ, double *B)
+{
+    msa_ctx *ctx = nullptr;
+    bool ok = iInt && *iInt == 2 && xyz && default_ctx(&ctx) == MSA_OK;
+    if (ok) {
+        std::lock_guard<std::mutex> lock(g_default_mutex);
+        double out[1496];
+        hipError_t e = hipSetDevice(ctx->device);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(g_drop_buf + 2048, xyz, 24 * sizeof(double), hipMemcpyHostToDevice,
+                               ctx->stream);
+        if (e == hipSuccess) e = launch_gauss_matrices(ctx->stream, g_drop_buf + 2048, g_drop_buf);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(out, g_drop_buf, sizeof(out), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess) {
+            memcpy(w, out, 8 * sizeof(double));
+            memcpy(S, out + 8, 64 * sizeof(double));
+            memcpy(DS, out + 72, 192 * sizeof(double));
+            memcpy(J, out + 264, 72 * sizeof(double));
+            memcpy(detJ, out + 336, 8 * sizeof(double));
+            memcpy(B, out + 344, 1152 * sizeof(double));
+            return;
+        }
+        set_error("CALCH8GAUSSMATRICES: %s", hipGetErrorString(e));
+    } else if (iInt && *iInt != 2) {
+        set_error("CALCH8GAUSSMATRICES: iInt=%d unsupported (only 2)", *iInt);
+    }
+    fill_nan(w, 8);
+    fill_nan(S, 64);
+    fill_nan(DS, 192);
+    fill_nan(J, 72);
+    fill_nan(detJ, 8);
+    fill_nan(B, 1152);
+}
+
+void CALCH8K(const int *iInt, const double *E, const double *B, const double *w, double *K)
+{
+    msa_ctx *ctx = nullptr;
+    bool ok = iInt && *iInt == 2 && E && B && w && default_ctx(&ctx) == MSA_OK;
+    if (ok) {
+        std::lock_guard<std::mutex> lock(g_default_mutex);
+        double *dE = g_drop_buf, *dB = g_drop_buf + 288, *dw = g_drop_buf + 1440,
+               *dK = g_drop_buf + 1448;
+        hipError_t e = hipSetDevice(ctx->device);
+        if (e == hipSuccess) e = hipMemcpyAsync(dE, E, 288 * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(dB, B, 1152 * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(dw, w, 8 * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = launch_h8k(ctx->stream, dE, dB, dw, dK);
+        if (e == hipSuccess) e = hipMemcpyAsync(K, dK, 300 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess) return;
+        set_error("CALCH8K: %s", hipGetErrorString(e));
+    } else if (iInt && *iInt != 2) {
+        set_error("CALCH8K: iInt=%d unsupported (only 2)", *iInt);
+    }
+    fill_nan(K, 300);
+}
+
+// ---------------------------------------------------------------- batch
+
+int msa_batch_destroy(msa_batch *b)
+{
+    if (!b) return MSA_OK;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    if (b->timing.e0) (void)hipEventDestroy(b->timing.e0);
+    if (b->timing.e1) (void)hipEventDestroy(b->timing.e1);
+    for (void *p : b->allocs) (void)hipFree(p);
+    delete b;
+    return MSA_OK;
+}
+
+int msa_batch_create(msa_ctx *ctx, int n_classes, msa_mesh *const *meshes, const int *n_samples,
+                     int precision, int tile, msa_batch **out)
+{
+    MSA_CHECK(ctx && meshes && n_samples && out, MSA_ERR_ARG, "msa_batch_create: null argument");
+    *out = nullptr;
+    MSA_CHECK(n_classes >= 1 && n_classes <= 32, MSA_ERR_ARG, "n_classes=%d (1..32)", n_classes);
+    MSA_CHECK(precision == MSA_FP64 || precision == MSA_MIXED, MSA_ERR_ARG, "precision=%d",
+              precision);
+    int maxs = 0;
+    for (int c = 0; c < n_classes; c++) {
+        MSA_CHECK(meshes[c] && meshes[c]->ctx == ctx, MSA_ERR_ARG, "mesh %d invalid", c);
+        MSA_CHECK(n_samples[c] >= 1, MSA_ERR_ARG, "n_samples[%d]=%d", c, n_samples[c]);
+        maxs = std::max(maxs, n_samples[c]);
+    }
+    if (tile == 0) tile = maxs > 32 ? 64 : 1;
+    MSA_CHECK(tile == 1 || tile == 2 || tile == 4 || tile == 8 || tile == 16 || tile == 32 ||
+                  tile == 64,
+              MSA_ERR_ARG, "tile=%d (1,2,4,...,64)", tile);
+    MSA_HIP(hipSetDevice(ctx->device));
+    msa_batch *b = new msa_batch();
+    b->ctx = ctx;
+    b->precision = precision;
+    b->T = tile;
+    b->R = 64 / tile;
+    const int T = b->T, R = b->R;
+    const size_t rs = real_size(b);
+    b->cls.resize(n_classes);
+    int rc = MSA_OK;
+    auto fail = [&](int code) {
+        msa_batch_destroy(b);
+        return code;
+    };
+    for (int c = 0; c < n_classes && rc == MSA_OK; c++) {
+        ClassHost &C = b->cls[c];
+        const MeshHost &h = meshes[c]->h;
+        C.mesh = meshes[c];
+        C.nsamp = n_samples[c];
+        C.ntiles = (C.nsamp + T - 1) / T;
+        C.nslices = (h.nb + R - 1) / R;
+        C.chunk = kWaves * std::max(1, 16 / R);
+        C.nchunks = (C.nslices + C.chunk - 1) / C.chunk;
+        C.slice_ptr.assign(C.nslices + 1, 0);
+        for (int sl = 0; sl < C.nslices; sl++) {
+            int wmax = 0;
+            for (int rr = 0; rr < R; rr++) {
+                int br = sl * R + rr;
+                if (br < h.nb) wmax = std::max(wmax, h.brow_ptr[br + 1] - h.brow_ptr[br]);
+            }
+            C.slice_ptr[sl + 1] = C.slice_ptr[sl] + wmax;
+        }
+        const long long nslots = C.slice_ptr[C.nslices];
+        C.cidx.assign((size_t)nslots * R, -1);
+        for (int br = 0; br < h.nb; br++) {
+            int sl = br / R, rr = br % R;
+            for (int j = 0; j < h.brow_ptr[br + 1] - h.brow_ptr[br]; j++)
+                C.cidx[(size_t)(C.slice_ptr[sl] + j) * R + rr] = h.bcol[h.brow_ptr[br] + j];
+        }
+        std::vector<uint8_t> bfree(h.nb);
+        for (int br = 0; br < h.nb; br++) bfree[br] = h.node_free[h.block_node[br]];
+        C.tile_vals = nslots * 9 * R * T;
+        C.tile_vec = (long long)h.nb * 3 * T;
+
+        ClassDev &D = C.dev;
+        memset(&D, 0, sizeof(D));
+        D.nb = h.nb;
+        D.nslices = C.nslices;
+        D.ntiles = C.ntiles;
+        D.nsamp = C.nsamp;
+        D.n_elem = h.n_elem;
+        D.nchunks = C.nchunks;
+        D.chunk = C.chunk;
+        D.tile_vals = C.tile_vals;
+        D.tile_vec = C.tile_vec;
+        D.xyz = meshes[c]->d_xyz;
+        D.conn = meshes[c]->d_conn;
+        D.node_block = meshes[c]->d_node_block;
+        D.node_free = meshes[c]->d_node_free;
+        D.emap = meshes[c]->d_emap;
+        D.colour_elems = meshes[c]->d_colour_elems;
+        MSA_CHECK(h.ncolours <= kMaxColours, fail(MSA_ERR_ARG), "too many colours");
+        for (int k = 0; k <= h.ncolours; k++) D.colour_ptr[k] = h.colour_ptr[k];
+        auto A = [&](void **p, size_t bytes) {
+            if (rc == MSA_OK) rc = dalloc(b, p, bytes);
+        };
+        void *p = nullptr;
+        A(&p, C.slice_ptr.size() * sizeof(int));
+        D.slice_ptr = (const int *)p;
+        if (rc == MSA_OK) MSA_HIP(hipMemcpy(p, C.slice_ptr.data(), C.slice_ptr.size() * sizeof(int), hipMemcpyHostToDevice));
+        A(&p, C.cidx.size() * sizeof(int));
+        D.cidx = (const int *)p;
+        if (rc == MSA_OK) MSA_HIP(hipMemcpy(p, C.cidx.data(), C.cidx.size() * sizeof(int), hipMemcpyHostToDevice));
+        A(&p, h.diag_pos.size() * sizeof(int));
+        D.diag_pos = (const int *)p;
+        if (rc == MSA_OK) MSA_HIP(hipMemcpy(p, h.diag_pos.data(), h.diag_pos.size() * sizeof(int), hipMemcpyHostToDevice));
+        A(&p, bfree.size());
+        D.block_free = (const uint8_t *)p;
+        if (rc == MSA_OK) MSA_HIP(hipMemcpy(p, bfree.data(), bfree.size(), hipMemcpyHostToDevice));
+        const size_t nvec = (size_t)C.ntiles * C.tile_vec;
+        A(&D.val, (size_t)C.ntiles * C.tile_vals * rs);
+        A(&D.x, nvec * rs);
+        A(&D.r, nvec * rs);
+        A(&D.z, nvec * rs);
+        A(&D.p0, nvec * rs);
+        A(&D.p1, nvec * rs);
+        A(&D.q, nvec * rs);
+        A(&D.dinv, nvec * rs);
+        if (is_mixed(b)) {
+            A((void **)&D.u64, nvec * sizeof(double));
+            A((void **)&D.r64, nvec * sizeof(double));
+        }
+        A(&p, (size_t)h.nb * 3 * sizeof(double));
+        D.f = (const double *)p;
+        A((void **)&D.E, (size_t)C.ntiles * h.n_elem * T * sizeof(double));
+        A((void **)&D.st, (size_t)C.ntiles * T * sizeof(SampleState));
+        A((void **)&D.partial, (size_t)C.ntiles * C.nchunks * 3 * T * sizeof(double));
+        A((void **)&D.counter, (size_t)C.ntiles * 8 * sizeof(unsigned));
+        A((void **)&D.negdet, sizeof(int));
+        if (rc != MSA_OK) break;
+        hipStream_t s = ctx->stream;
+        MSA_HIP(hipMemsetAsync(D.st, 0, (size_t)C.ntiles * T * sizeof(SampleState), s));
+        MSA_HIP(hipMemsetAsync(D.counter, 0, (size_t)C.ntiles * 8 * sizeof(unsigned), s));
+        MSA_HIP(hipMemsetAsync(D.negdet, 0, sizeof(int), s));
+        MSA_HIP(hipMemsetAsync((void *)D.f, 0, (size_t)h.nb * 3 * sizeof(double), s));
+        MSA_HIP(hipMemsetAsync(D.E, 0, (size_t)C.ntiles * h.n_elem * T * sizeof(double), s));
+        for (int t = 0; t < C.ntiles; t++)
+            for (int k = 0; k < C.nchunks; k++) b->items.push_back(make_int4(c, t, k, 0));
+    }
+    if (rc != MSA_OK) return fail(rc);
+    void *p = nullptr;
+    rc = dalloc(b, &p, sizeof(ClassDev) * n_classes);
+    b->d_cls = (ClassDev *)p;
+    if (rc == MSA_OK) rc = dalloc(b, &p, sizeof(int4) * b->items.size());
+    b->d_items = (int4 *)p;
+    if (rc == MSA_OK) rc = dalloc(b, &p, 4 * sizeof(int));
+    b->d_counts = (int *)p;
+    if (rc != MSA_OK) return fail(rc);
+    if (hipMemcpy(b->d_items, b->items.data(), sizeof(int4) * b->items.size(),
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(b->d_counts, 0, 4 * sizeof(int)) != hipSuccess) {
+        set_error("batch upload failed");
+        return fail(MSA_ERR_HIP);
+    }
+    if (hipEventCreate(&b->timing.e0) != hipSuccess || hipEventCreate(&b->timing.e1) != hipSuccess) {
+        set_error("event creation failed");
+        return fail(MSA_ERR_HIP);
+    }
+    rc = upload_class_table(b);
+    if (rc != MSA_OK) return fail(rc);
+    *out = b;
+    return MSA_OK;
+}
+
+long long msa_batch_bytes(const msa_batch *b) { return b ? b->bytes : 0; }
+
+int msa_batch_set_modulus(msa_batch *b, int cls, const double *E)
+{
+    MSA_CHECK(b && E && cls >= 0 && cls < (int)b->cls.size(), MSA_ERR_ARG,
+              "msa_batch_set_modulus: bad arguments");
+    ClassHost &C = b->cls[cls];
+    const int T = b->T, ne = C.mesh->h.n_elem;
+    std::vector<double> h((size_t)C.ntiles * ne * T, 0.0);
+    for (int s = 0; s < C.nsamp; s++) {
+        const int tile = s / T, lane = s % T;
+        for (int e = 0; e < ne; e++) h[((size_t)tile * ne + e) * T + lane] = E[(size_t)s * ne + e];
+    }
+    MSA_HIP(hipSetDevice(b->ctx->device));
+    MSA_HIP(hipMemcpy(C.dev.E, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    C.has_E = true;
+    b->assembled = false;
+    return MSA_OK;
+}
+
+int msa_batch_set_kl(msa_batch *b, int cls, int M, int kmax, const double *w1d, const double *a1d,
+                     const int *evpPos, const double *eigs, const double *domain, double meanE,
+                     double cov)
+{
+    MSA_CHECK(b && cls >= 0 && cls < (int)b->cls.size() && M > 0 && kmax > 0 && w1d && a1d &&
+                  evpPos && eigs && domain && meanE > 0 && cov >= 0,
+              MSA_ERR_ARG, "msa_batch_set_kl: bad arguments");
+    for (int i = 0; i < M; i++)
+        for (int d = 0; d < 3; d++)
+            MSA_CHECK(evpPos[3 * i + d] >= 1 && evpPos[3 * i + d] <= kmax, MSA_ERR_ARG,
+                      "evpPos[%d][%d]=%d outside 1..%d", i, d, evpPos[3 * i + d], kmax);
+    ClassHost &C = b->cls[cls];
+    MSA_HIP(hipSetDevice(b->ctx->device));
+    const int ne = C.mesh->h.n_elem;
+    if (C.M != M) {
+        MSA_CHECK(C.M == 0, MSA_ERR_STATE, "KL order of class %d already set to %d", cls, C.M);
+        MSA_TRY(dalloc(b, (void **)&C.dev.phi, (size_t)ne * M * sizeof(double)));
+        MSA_TRY(dalloc(b, (void **)&C.dev.xi, (size_t)C.ntiles * b->T * M * sizeof(double)));
+        MSA_HIP(hipMemset(C.dev.xi, 0, (size_t)C.ntiles * b->T * M * sizeof(double)));
+        C.M = M;
+        C.dev.M = M;
+    }
+    const double sigma = sqrt(log(pow(cov / 1.0, 2) + 1.0));
+    C.dev.c0 = log(meanE) - 0.5 * pow(sigma, 2);
+    MSA_TRY(upload_class_table(b));
+    std::vector<double> tab;
+    tab.insert(tab.end(), w1d, w1d + 3 * kmax);
+    tab.insert(tab.end(), a1d, a1d + 3 * kmax);
+    tab.insert(tab.end(), eigs, eigs + M);
+    tab.insert(tab.end(), domain, domain + 3);
+    double *dt = nullptr;
+    int *dp = nullptr;
+    MSA_HIP(hipMalloc(&dt, tab.size() * sizeof(double)));
+    hipError_t e = hipMalloc(&dp, 3 * (size_t)M * sizeof(int));
+    if (e == hipSuccess) e = hipMemcpy(dt, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dp, evpPos, 3 * (size_t)M * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = launch_kl_basis(b->ctx->stream, b->d_cls + cls, ne, M, kmax, dt, dt + 3 * kmax, dp,
+                            dt + 6 * kmax, dt + 6 * kmax + M, sigma);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->ctx->stream);
+    (void)hipFree(dt);
+    if (dp) (void)hipFree(dp);
+    MSA_HIP(e);
+    C.has_kl = true;
+    return MSA_OK;
+}
+
+int msa_batch_set_xi(msa_batch *b, int cls, const double *xi)
+{
+    MSA_CHECK(b && xi && cls >= 0 && cls < (int)b->cls.size(), MSA_ERR_ARG,
+              "msa_batch_set_xi: bad arguments");
+    ClassHost &C = b->cls[cls];
+    MSA_CHECK(C.M > 0, MSA_ERR_STATE, "class %d has no KL table (msa_batch_set_kl)", cls);
+    MSA_HIP(hipSetDevice(b->ctx->device));
+    MSA_HIP(hipMemcpy(C.dev.xi, xi, (size_t)C.nsamp * C.M * sizeof(double), hipMemcpyHostToDevice));
+    C.has_xi = true;
+    return MSA_OK;
+}
+
+int msa_batch_eval_kl(msa_batch *b)
+{
+    MSA_CHECK(b, MSA_ERR_ARG, "null batch");
+    MSA_HIP(hipSetDevice(b->ctx->device));
+    for (size_t c = 0; c < b->cls.size(); c++) {
+        ClassHost &C = b->cls[c];
+        if (!C.has_kl) continue;
+        MSA_CHECK(C.has_xi, MSA_ERR_STATE, "class %zu: KL coefficients not set", c);
+        MSA_TRY(timed(b, KK_OTHER, [&] {
+            return launch_kl_eval(b->ctx->stream, b->T, b->d_cls + c, C.ntiles, C.mesh->h.n_elem,
+                                  C.M);
+        }));
+        C.has_E = true;
+    }
+    b->assembled = false;
+    return MSA_OK;
+}
+
+int msa_batch_set_rhs(msa_batch *b, int cls, const double *f)
+{
+    MSA_CHECK(b && f && cls >= 0 && cls < (int)b->cls.size(), MSA_ERR_ARG,
+              "msa_batch_set_rhs: bad arguments");
+    ClassHost &C = b->cls[cls];
+    const MeshHost &h = C.mesh->h;
+    std::vector<double> fi((size_t)h.nb * 3, 0.0);
+    double ss = 0;
+    for (int g = 0; g < h.n_free; g++) {
+        fi[h.ref2int[g]] = f[g];
+        ss += f[g] * f[g];
+    }
+    C.fnorm = sqrt(ss);
+    C.dev.fnorm = C.fnorm;
+    MSA_HIP(hipSetDevice(b->ctx->device));
+    MSA_HIP(hipMemcpy((void *)C.dev.f, fi.data(), fi.size() * sizeof(double), hipMemcpyHostToDevice));
+    C.has_rhs = true;
+    return upload_class_table(b);
+}
+
+int msa_batch_assemble(msa_batch *b, double nu)
+{
+    MSA_CHECK(b, MSA_ERR_ARG, "null batch");
+    MSA_CHECK(nu > -1.0 && nu < 0.5, MSA_ERR_ARG, "Poisson ratio %g outside (-1, 0.5)", nu);
+    for (size_t c = 0; c < b->cls.size(); c++)
+        MSA_CHECK(b->cls[c].has_E, MSA_ERR_STATE, "class %zu: no modulus field", c);
+    MSA_HIP(hipSetDevice(b->ctx->device));
+    hipStream_t s = b->ctx->stream;
+    const bool mx = is_mixed(b);
+    for (auto &C : b->cls) {
+        const size_t n = (size_t)C.ntiles * C.tile_vals;
+        MSA_TRY(timed(b, KK_OTHER, [&] {
+            return mx ? launch_fill_f32(s, (float *)C.dev.val, n, 0.f)
+                      : launch_fill_f64(s, (double *)C.dev.val, n, 0.0);
+        }));
+        MSA_HIP(hipMemsetAsync(C.dev.negdet, 0, sizeof(int), s));
+    }
+    const int ncol = max_colours(b);
+    for (int k = 0; k < ncol; k++) {
+        int nblk = 0;
+        ColourLaunch cl = colour_launch(b, k, &nblk);
+        MSA_TRY(timed(b, KK_ASSEMBLE, [&] { return launch_assemble(s, mx, b->T, b->d_cls, cl, nblk, nu); }));
+    }
+    MSA_TRY(timed(b, KK_OTHER, [&] {
+        return launch_dinv(s, mx, b->T, b->d_cls, b->d_items, (int)b->items.size());
+    }));
+    b->nu = nu;
+    b->assembled = true;
+    return MSA_OK;
+}
+
+int msa_batch_solve(msa_batch *b, double tol, int max_it, double inner_tol, int max_outer,
+                    int graph_iters)
+{
+    MSA_CHECK(b, MSA_ERR_ARG, "null batch");
+    MSA_CHECK(b->assembled, MSA_ERR_STATE, "batch not assembled");
+    for (size_t c = 0; c < b->cls.size(); c++)
+        MSA_CHECK(b->cls[c].has_rhs, MSA_ERR_STATE, "class %zu: no load vector", c);
+    MSA_CHECK(tol > 0 && max_it > 0, MSA_ERR_ARG, "tol=%g max_it=%d", tol, max_it);
+    MSA_CHECK(graph_iters >= 0 && graph_iters % 2 == 0, MSA_ERR_ARG,
+              "graph_iters=%d must be even", graph_iters);
+    MSA_HIP(hipSetDevice(b->ctx->device));
+    hipStream_t s = b->ctx->stream;
+    const bool mx = is_mixed(b);
+    const int T = b->T, ni = (int)b->items.size();
+    if (!mx) {
+        MSA_HIP(hipMemsetAsync(b->d_counts, 0, sizeof(int), s));
+        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_pcg_init(s, false, T, b->d_cls, b->d_items, ni, 0, b->d_counts); }));
+        return pcg_loop(b, tol, max_it, graph_iters);
+    }
+    MSA_CHECK(inner_tol > 0 && inner_tol < 1 && max_outer >= 1, MSA_ERR_ARG,
+              "inner_tol=%g max_outer=%d", inner_tol, max_outer);
+    MSA_HIP(hipMemsetAsync(b->d_counts + 1, 0, sizeof(int), s));
+    MSA_TRY(timed(b, KK_OTHER, [&] { return launch_ir_begin(s, T, b->d_cls, b->d_items, ni, b->d_counts + 1); }));
+    const int ncol = max_colours(b);
+    for (int outer = 0; outer < max_outer; outer++) {
+        int left = 0;
+        MSA_TRY(read_count(b, 1, &left));
+        if (left <= 0) break;
+        MSA_HIP(hipMemsetAsync(b->d_counts, 0, sizeof(int), s));
+        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_pcg_init(s, true, T, b->d_cls, b->d_items, ni, 1, b->d_counts); }));
+        MSA_TRY(pcg_loop(b, inner_tol, max_it, graph_iters));
+        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_ir_update(s, T, b->d_cls, b->d_items, ni); }));
+        for (int k = 0; k < ncol; k++) {
+            int nblk = 0;
+            ColourLaunch cl = colour_launch(b, k, &nblk);
+            MSA_TRY(timed(b, KK_RES64, [&] { return launch_residual64(s, T, b->d_cls, cl, nblk, b->nu); }));
+        }
+        MSA_TRY(timed(b, KK_OTHER, [&] {
+            return launch_ir_norm(s, T, b->d_cls, b->d_items, ni, tol, max_outer, b->d_counts + 1);
+        }));
+    }
+    MSA_HIP(hipStreamSynchronize(s));
+    return MSA_OK;
+}
+
+int msa_batch_run(msa_batch *b, double nu, double tol, int max_it, double inner_tol, int max_outer,
+                  int graph_iters)
+{
+    MSA_CHECK(b, MSA_ERR_ARG, "null batch");
+    bool any_kl = false;
+    for (auto &C : b->cls) any_kl |= C.has_kl;
+    if (any_kl) MSA_TRY(msa_batch_eval_kl(b));
+    MSA_TRY(msa_batch_assemble(b, nu));
+    return msa_batch_solve(b, tol, max_it, inner_tol, max_outer, graph_iters);
+}
+
+static int extract_lane(msa_batch *b, bool real_is_float, const void *tile_base, size_t n, int lane,
+                        std::vector<double> &host)
+{
+    double *d = nullptr;
+    MSA_HIP(hipMalloc(&d, std::max<size_t>(n, 1) * sizeof(double)));
+    host.resize(n);
+    hipError_t e = launch_extract_sample(b->ctx->stream, real_is_float, b->T, tile_base, n, lane, d);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(host.data(), d, n * sizeof(double), hipMemcpyDeviceToHost, b->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->ctx->stream);
+    (void)hipFree(d);
+    MSA_HIP(e);
+    return MSA_OK;
+}
+
+int msa_batch_get_solution(msa_batch *b, int cls, int sample, double *u)
+{
+    MSA_CHECK(b && u && cls >= 0 && cls < (int)b->cls.size(), MSA_ERR_ARG, "bad arguments");
+    ClassHost &C = b->cls[cls];
+    MSA_CHECK(sample >= 0 && sample < C.nsamp, MSA_ERR_ARG, "sample %d out of range", sample);
+    MSA_HIP(hipSetDevice(b->ctx->device));
+    const int tile = sample / b->T, lane = sample % b->T;
+    std::vector<double> h;
+    const size_t n = (size_t)C.mesh->h.nb * 3;
+    if (is_mixed(b))
+        MSA_TRY(extract_lane(b, false, C.dev.u64 + (size_t)tile * C.tile_vec, n, lane, h));
+    else
+        MSA_TRY(extract_lane(b, false, (double *)C.dev.x + (size_t)tile * C.tile_vec, n, lane, h));
+    const MeshHost &m = C.mesh->h;
+    for (int g = 0; g < m.n_free; g++) u[g] = h[m.ref2int[g]];
+    return MSA_OK;
+}
+
+int msa_batch_get_stats(msa_batch *b, int cls, int *iters, int *outer, double *relres, int *status)
+{
+    MSA_CHECK(b && cls >= 0 && cls < (int)b->cls.size(), MSA_ERR_ARG, "bad arguments");
+    ClassHost &C = b->cls[cls];
+    MSA_HIP(hipSetDevice(b->ctx->device));
+    std::vector<SampleState> st((size_t)C.ntiles * b->T);
+    int negdet = 0;
+    MSA_HIP(hipMemcpyAsync(st.data(), C.dev.st, st.size() * sizeof(SampleState), hipMemcpyDeviceToHost, b->ctx->stream));
+    MSA_HIP(hipMemcpyAsync(b->ctx->h_pinned, C.dev.negdet, sizeof(int), hipMemcpyDeviceToHost, b->ctx->stream));
+    MSA_HIP(hipStreamSynchronize(b->ctx->stream));
+    negdet = *b->ctx->h_pinned;
+    const bool mx = is_mixed(b);
+    for (int s = 0; s < C.nsamp; s++) {
+        const SampleState &S = st[s];
+        if (iters) iters[s] = mx ? S.total_iters : S.iters;
+        if (outer) outer[s] = mx ? S.outer : 0;
+        if (relres) relres[s] = S.relres;
+        if (status) status[s] = (negdet && S.status == MSA_OK) ? MSA_NEG_DETJ : S.status;
+    }
+    return MSA_OK;
+}
+
+int msa_batch_get_modulus(msa_batch *b, int cls, int sample, double *E)
+{
+    MSA_CHECK(b && E && cls >= 0 && cls < (int)b->cls.size(), MSA_ERR_ARG, "bad arguments");
+    ClassHost &C = b->cls[cls];
+    MSA_CHECK(sample >= 0 && sample < C.nsamp, MSA_ERR_ARG, "sample %d out of range", sample);
+    MSA_HIP(hipSetDevice(b->ctx->device));
+    const int tile = sample / b->T, lane = sample % b->T, ne = C.mesh->h.n_elem;
+    std::vector<double> h;
+    MSA_TRY(extract_lane(b, false, C.dev.E + (size_t)tile * ne * b->T, ne, lane, h));
+    memcpy(E, h.data(), ne * sizeof(double));
+    return MSA_OK;
+}
+
+int msa_batch_get_matrix_csr(msa_batch *b, int cls, int sample, long long *rowptr, int *col,
+                             double *val)
+{
+    MSA_CHECK(b && rowptr && col && val && cls >= 0 && cls < (int)b->cls.size(), MSA_ERR_ARG,
+              "bad arguments");
+    ClassHost &C = b->cls[cls];
+    MSA_CHECK(sample >= 0 && sample < C.nsamp, MSA_ERR_ARG, "sample %d out of range", sample);
+    MSA_CHECK(b->assembled, MSA_ERR_STATE, "batch not assembled");
+    MSA_HIP(hipSetDevice(b->ctx->device));
+    const int tile = sample / b->T, lane = sample % b->T, R = b->R;
+    const size_t rs = real_size(b);
+    std::vector<double> hv;
+    const size_t n = (size_t)C.slice_ptr[C.nslices] * 9 * R;
+    MSA_TRY(extract_lane(b, is_mixed(b), (const char *)C.dev.val + (size_t)tile * C.tile_vals * rs, n,
+                         lane, hv));
+    const MeshHost &m = C.mesh->h;
+    long long k = 0;
+    rowptr[0] = 0;
+    // rows in reference order: free DOF g = (node, p)
+    std::vector<std::pair<int, int>> rows(m.n_free);
+    for (int nd = 0; nd < m.n_nodes; nd++)
+        for (int p = 0; p < 3; p++)
+            if (m.dof[3 * nd + p] >= 0) rows[m.dof[3 * nd + p]] = {nd, p};
+    for (int g = 0; g < m.n_free; g++) {
+        const int nd = rows[g].first, p = rows[g].second;
+        const int br = m.node_block[nd], sl = br / R, rr = br % R;
+        for (int j = 0; j < m.brow_ptr[br + 1] - m.brow_ptr[br]; j++) {
+            const int bc = m.bcol[m.brow_ptr[br] + j];
+            const int mc = m.block_node[bc];
+            for (int q = 0; q < 3; q++) {
+                const int gc = m.dof[3 * mc + q];
+                if (gc < 0) continue;
+                col[k] = gc;
+                val[k] = hv[((size_t)(C.slice_ptr[sl] + j) * 9 + 3 * p + q) * R + rr];
+                k++;
+            }
+        }
+        rowptr[g + 1] = k;
+    }
+    return MSA_OK;
+}
+
+int msa_batch_spmv(msa_batch *b, int cls, int sample, const double *x, double *y)
+{
+    MSA_CHECK(b && x && y && cls >= 0 && cls < (int)b->cls.size(), MSA_ERR_ARG, "bad arguments");
+    ClassHost &C = b->cls[cls];
+    MSA_CHECK(sample >= 0 && sample < C.nsamp, MSA_ERR_ARG, "sample %d out of range", sample);
+    MSA_CHECK(b->assembled, MSA_ERR_STATE, "batch not assembled");
+    MSA_HIP(hipSetDevice(b->ctx->device));
+    const MeshHost &m = C.mesh->h;
+    const int tile = sample / b->T, lane = sample % b->T;
+    const size_t n = (size_t)m.nb * 3;
+    std::vector<double> xi(n, 0.0);
+    for (int g = 0; g < m.n_free; g++) xi[m.ref2int[g]] = x[g];
+    std::vector<int4> its;
+    for (auto &it : b->items)
+        if (it.x == cls && it.y == tile) its.push_back(it);
+    double *dx = nullptr, *dy = nullptr, *din = nullptr;
+    int4 *di = nullptr;
+    hipStream_t s = b->ctx->stream;
+    hipError_t e = hipMalloc(&dx, C.tile_vec * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&dy, C.tile_vec * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&din, n * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&di, its.size() * sizeof(int4));
+    if (e == hipSuccess) e = hipMemsetAsync(dx, 0, C.tile_vec * sizeof(double), s);
+    if (e == hipSuccess) e = hipMemsetAsync(dy, 0, C.tile_vec * sizeof(double), s);
+    if (e == hipSuccess) e = hipMemcpyAsync(din, xi.data(), n * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(di, its.data(), its.size() * sizeof(int4), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = launch_insert_sample(s, b->T, din, n, lane, dx);
+    if (e == hipSuccess) e = launch_spmv(s, is_mixed(b), b->T, b->d_cls, di, (int)its.size(), dx, dy);
+    if (e == hipSuccess) e = launch_extract_sample(s, false, b->T, dy, n, lane, din);
+    if (e == hipSuccess) e = hipMemcpyAsync(xi.data(), din, n * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    for (void *p : {(void *)dx, (void *)dy, (void *)din, (void *)di})
+        if (p) (void)hipFree(p);
+    MSA_HIP(e);
+    for (int g = 0; g < m.n_free; g++) y[g] = xi[m.ref2int[g]];
+    return MSA_OK;
+}
+
+int msa_batch_set_kernel_timing(msa_batch *b, int enable)
+{
+    MSA_CHECK(b, MSA_ERR_ARG, "null batch");
+    b->timing.on = enable != 0;
+    for (int k = 0; k < 5; k++) {
+        b->timing.ms[k] = 0;
+        b->timing.n[k] = 0;
+    }
+    return MSA_OK;
+}
+
+int msa_batch_get_kernel_timing(msa_batch *b, double *ms_total, long long *launches)
+{
+    MSA_CHECK(b && ms_total && launches, MSA_ERR_ARG, "bad arguments");
+    for (int k = 0; k < 5; k++) {
+        ms_total[k] = b->timing.ms[k];
+        launches[k] = b->timing.n[k];
+    }
+    return MSA_OK;
+}
+
+int msa_batch_spmv_bytes(msa_batch *b, double *bytes)
+{
+    MSA_CHECK(b && bytes, MSA_ERR_ARG, "bad arguments");
+    const double rs = (double)real_size(b);
+    double kernel = 0, survey = 0;
+    for (auto &C : b->cls) {
+        const MeshHost &m = C.mesh->h;
+        const double S = C.nsamp, nnz = (double)m.nnz_scalar, n = 3.0 * m.nb;
+        const double idx = 4.0 * (double)m.bcol.size() + 4.0 * (C.nslices + 1);
+        kernel += S * rs * nnz + idx + S * 4.0 * rs * n;
+        survey += S * rs * nnz + idx + S * 2.0 * rs * n;
+    }
+    bytes[0] = kernel;
+    bytes[1] = survey;
+    return MSA_OK;
+}
+
+int msa_autotune_tile(msa_ctx *ctx, msa_mesh *mesh, int n_samples, int precision, int *best_tile,
+                      double *ms_per_tile)
+{
+    MSA_CHECK(ctx && mesh && best_tile && n_samples >= 1, MSA_ERR_ARG, "bad arguments");
+    const int tiles[7] = {1, 2, 4, 8, 16, 32, 64};
+    double best = 1e300;
+    *best_tile = 0;
+    const MeshHost &h = mesh->h;
+    std::vector<double> E((size_t)n_samples * h.n_elem, 1.0);
+    std::vector<double> f(h.n_free, 1.0);
+    for (int k = 0; k < 7; k++) {
+        if (ms_per_tile) ms_per_tile[k] = -1;
+        msa_batch *b = nullptr;
+        int rc = msa_batch_create(ctx, 1, &mesh, &n_samples, precision, tiles[k], &b);
+        if (rc != MSA_OK) continue;
+        rc = msa_batch_set_modulus(b, 0, E.data());
+        if (rc == MSA_OK) rc = msa_batch_set_rhs(b, 0, f.data());
+        if (rc == MSA_OK) rc = msa_batch_assemble(b, 0.3);
+        hipStream_t s = ctx->stream;
+        const int ni = (int)b->items.size();
+        if (rc == MSA_OK) {
+            (void)hipMemsetAsync(b->d_counts, 0, sizeof(int), s);
+            if (launch_pcg_init(s, precision == MSA_MIXED, b->T, b->d_cls, b->d_items, ni, 0,
+                                b->d_counts) != hipSuccess)
+                rc = MSA_ERR_HIP;
+        }
+        if (rc == MSA_OK) {
+            // warm-up, then 10 timed products (pbuf 0 keeps every sample active)
+            const int reps = 10;
+            hipError_t e = launch_pcg_a(s, precision == MSA_MIXED, b->T, b->d_cls, b->d_items, ni, 0,
+                                        b->d_counts);
+            if (e == hipSuccess) e = hipEventRecord(b->timing.e0, s);
+            for (int r = 0; r < reps && e == hipSuccess; r++)
+                e = launch_pcg_a(s, precision == MSA_MIXED, b->T, b->d_cls, b->d_items, ni, 0,
+                                 b->d_counts);
+            if (e == hipSuccess) e = hipEventRecord(b->timing.e1, s);
+            if (e == hipSuccess) e = hipEventSynchronize(b->timing.e1);
+            float ms = 0;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, b->timing.e0, b->timing.e1);
+            if (e == hipSuccess) {
+                const double per = ms / reps;
+                if (ms_per_tile) ms_per_tile[k] = per;
+                if (per < best) {
+                    best = per;
+                    *best_tile = tiles[k];
+                }
+            }
+        }
+        msa_batch_destroy(b);
+    }
+    MSA_CHECK(*best_tile > 0, MSA_ERR_STATE, "autotune: no tile could be timed (%s)",
+              g_last_error.c_str());
+    return MSA_OK;
+}
+
+}  // extern "C"

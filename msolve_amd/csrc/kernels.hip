@@ -1,0 +1,1192 @@
+// HIP kernels for gfx950 (CDNA4): Hexa8 element stiffness, coloured scatter
+// into the sliced block-ELL image of K, fused Jacobi-PCG steps over many
+// samples at once, mixed-precision iterative refinement and the lognormal
+// Karhunen-Loeve modulus field.  Layouts are described in msa_common.hpp.
+//
+// Wavefront mapping of the PCG kernels: lane = rr*T + s covers R = 64/T block
+// rows (rr) times T samples (s).  Every value load reads 64 consecutive words;
+// every gathered vector load reads T consecutive words of one block column.
+// Per-sample dot products are reduced wave -> workgroup -> a partial per
+// work item; the last workgroup of a (class, tile) to finish (atomic ticket)
+// sums the partials in a fixed order, so results do not depend on scheduling.
+#include <cmath>
+
+#include "msa_common.hpp"
+
+namespace msa {
+
+// ------------------------------------------------------------ helpers
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give every XCD a
+// contiguous range of work items so neighbouring slices share its L2.
+__device__ __forceinline__ int xcd_item(int nitems)
+{
+    const int per = gridDim.x >> 3;
+    const int item = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    return item < nitems ? item : -1;
+}
+
+static inline int xcd_grid(int nitems) { return ((nitems + 7) / 8) * 8; }
+
+template <int T, typename V>
+__device__ __forceinline__ V reduce_rows(V v)
+{
+#pragma unroll
+    for (int off = T; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Wave + workgroup reduction of NV per-sample sums; writes NV*T partials.
+template <int T, int NV>
+__device__ __forceinline__ void block_partials(double (&v)[NV], double *out)
+{
+    __shared__ double red[kWaves][NV][T];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] = reduce_rows<T>(v[k]);
+    if (lane < T) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) red[wave][k][lane] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < T) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            double acc = red[0][k][threadIdx.x];
+#pragma unroll
+            for (int w = 1; w < kWaves; w++) acc += red[w][k][threadIdx.x];
+            out[k * T + threadIdx.x] = acc;
+        }
+    }
+}
+
+// Ticket: true in exactly one workgroup, the last of `total` to arrive.
+__device__ __forceinline__ bool last_block(unsigned *counter, unsigned total)
+{
+    __shared__ unsigned is_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) is_last = (atomicAdd(counter, 1u) == total - 1u);
+    __syncthreads();
+    if (is_last) {
+        __threadfence();
+        if (threadIdx.x == 0) atomicExch(counter, 0u);
+    }
+    return is_last;
+}
+
+__device__ __forceinline__ double coherent_load(const double *p)
+{
+    return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sum of partial k of sample lane t over all work items of a tile.
+template <int T>
+__device__ __forceinline__ double sum_partials(const ClassDev &C, int tile, int k, int t)
+{
+    double acc = 0;
+    const double *p = C.partial + ((size_t)tile * C.nchunks * 3 + k) * T + t;
+    for (int c = 0; c < C.nchunks; c++) acc += coherent_load(p + (size_t)c * 3 * T);
+    return acc;
+}
+
+// ------------------------------------------------------------ Hexa8 stiffness
+
+// 13-digit Gauss abscissa of the reference (GaussQuadrature.cs:73-81,
+// rElHexa8.f90:435-439) and sqrt(1/8) of CalcH8NablaShape (rElHexa8.f90:30).
+constexpr double kGauss = 0.5773502691896;
+constexpr double kSq125 = 0.35355339059327376220042218105242;
+
+// One wavefront computes the unit-modulus element stiffness
+// Khat = sum_gp w_gp B^T Dhat B (2x2x2 Gauss, Dhat = D(E = 1, nu)) and stages
+// it in LDS as 64 node-pair blocks Kh[(a*8 + b)*9 + 3p + q].
+// Phase 1, lane = gp*8 + a: shape derivatives at Gauss point gp for node a,
+//   Jacobian by an 8-lane reduction, determinant/inverse as CalcH8JDetJ and
+//   CalcH8B (rElHexa8.f90:141-250), gradient g_a of N_a -> LDS.
+// Phase 2, lane = a*8 + b: the 3x3 block (a, b) of B^T D B for isotropic D:
+//   lam g_a,p g_b,q + mu g_a,q g_b,p + mu delta_pq (g_a . g_b), summed over
+//   the Gauss points with weight detJ (all Gauss weights are 1).
+// xa is the coordinate of node a = lane & 7 (every lane loads its own node).
+// Coordinates are taken relative to node 0 (the Jacobian is translation-
+// invariant; this keeps the FP32 variant accurate on small elements).
+// Returns true in lanes whose Gauss point has detJ < 1e-8.
+template <typename kreal>
+__device__ bool wave_khat(const double xa[3], double nu, kreal *Kh, kreal *g, kreal *w)
+{
+    const int lane = threadIdx.x & 63;
+    const int gp = lane >> 3, a = lane & 7;
+    const double x0 = __shfl(xa[0], lane & 56, 64);
+    const double y0 = __shfl(xa[1], lane & 56, 64);
+    const double z0 = __shfl(xa[2], lane & 56, 64);
+    const kreal X[3] = {(kreal)(xa[0] - x0), (kreal)(xa[1] - y0), (kreal)(xa[2] - z0)};
+    const kreal sx = ((a + 1) & 2) ? 1 : -1;
+    const kreal sy = (a & 2) ? 1 : -1;
+    const kreal sz = (a & 4) ? 1 : -1;
+    const kreal xi = ((gp >> 2) & 1) ? (kreal)kGauss : (kreal)-kGauss;
+    const kreal eta = ((gp >> 1) & 1) ? (kreal)kGauss : (kreal)-kGauss;
+    const kreal zeta = (gp & 1) ? (kreal)kGauss : (kreal)-kGauss;
+    const kreal s = (kreal)kSq125;
+    const kreal fx = (1 + sx * xi) * s, fy = (1 + sy * eta) * s, fz = (1 + sz * zeta) * s;
+    const kreal dn[3] = {sx * (fy * fz), sy * (fx * fz), sz * (fx * fy)};
+    kreal J[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            kreal v = dn[i] * X[j];
+            v += __shfl_xor(v, 1, 64);
+            v += __shfl_xor(v, 2, 64);
+            v += __shfl_xor(v, 4, 64);
+            J[i][j] = v;
+        }
+    const kreal det = J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2]) -
+                      J[0][1] * (J[1][0] * J[2][2] - J[2][0] * J[1][2]) +
+                      J[0][2] * (J[1][0] * J[2][1] - J[2][0] * J[1][1]);
+    const kreal inv = (kreal)1 / det;
+    kreal Ji[3][3];
+    Ji[0][0] = (J[1][1] * J[2][2] - J[2][1] * J[1][2]) * inv;
+    Ji[1][0] = (J[2][0] * J[1][2] - J[1][0] * J[2][2]) * inv;
+    Ji[2][0] = (J[1][0] * J[2][1] - J[2][0] * J[1][1]) * inv;
+    Ji[0][1] = (J[2][1] * J[0][2] - J[0][1] * J[2][2]) * inv;
+    Ji[1][1] = (J[0][0] * J[2][2] - J[2][0] * J[0][2]) * inv;
+    Ji[2][1] = (J[2][0] * J[0][1] - J[2][1] * J[0][0]) * inv;
+    Ji[0][2] = (J[0][1] * J[1][2] - J[1][1] * J[0][2]) * inv;
+    Ji[1][2] = (J[1][0] * J[0][2] - J[0][0] * J[1][2]) * inv;
+    Ji[2][2] = (J[0][0] * J[1][1] - J[1][0] * J[0][1]) * inv;
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+        g[lane * 3 + j] = Ji[j][0] * dn[0] + Ji[j][1] * dn[1] + Ji[j][2] * dn[2];
+    if (a == 0) w[gp] = det;
+    __syncthreads();
+
+    const int pa = lane >> 3, pb = lane & 7;
+    const kreal e1 = (kreal)(1.0 / (1.0 + nu));
+    const kreal lam = e1 * (kreal)nu / (kreal)(1.0 - 2.0 * nu);
+    const kreal mu = e1 * (kreal)0.5;
+    kreal K[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) K[k] = 0;
+#pragma unroll
+    for (int q8 = 0; q8 < 8; q8++) {
+        const kreal wq = w[q8];
+        const kreal *ga = g + (q8 * 8 + pa) * 3;
+        const kreal *gb = g + (q8 * 8 + pb) * 3;
+        const kreal A0 = ga[0], A1 = ga[1], A2 = ga[2];
+        const kreal B0 = gb[0], B1 = gb[1], B2 = gb[2];
+        const kreal mdot = mu * (A0 * B0 + A1 * B1 + A2 * B2);
+        const kreal A[3] = {A0, A1, A2}, B[3] = {B0, B1, B2};
+#pragma unroll
+        for (int p = 0; p < 3; p++)
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                kreal t = lam * A[p] * B[q] + mu * A[q] * B[p];
+                if (p == q) t += mdot;
+                K[p * 3 + q] += wq * t;
+            }
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) Kh[lane * 9 + k] = K[k];
+    __syncthreads();
+    return det < (kreal)0.00000001;
+}
+
+__device__ __forceinline__ int sym_index(int r, int c)
+{
+    if (r > c) {
+        int t = r;
+        r = c;
+        c = t;
+    }
+    return 300 - ((24 - r + 1) * (24 - r)) / 2 + c - r;
+}
+
+// Packed 300-entry Ke of independent elements (IElementMatrixProvider.Matrix).
+__global__ __launch_bounds__(64) void k_ke_packed(int n_elem, const double *__restrict__ xyz8,
+                                                  const double *__restrict__ E, double nu,
+                                                  double *__restrict__ ke300,
+                                                  int *__restrict__ status)
+{
+    __shared__ double Kh[576];
+    __shared__ double g[192];
+    __shared__ double w[8];
+    const int e = blockIdx.x;
+    if (e >= n_elem) return;
+    const int lane = threadIdx.x & 63, a = lane & 7;
+    double xa[3] = {xyz8[(size_t)e * 24 + a * 3], xyz8[(size_t)e * 24 + a * 3 + 1],
+                    xyz8[(size_t)e * 24 + a * 3 + 2]};
+    const bool neg = wave_khat<double>(xa, nu, Kh, g, w);
+    const unsigned long long anyneg = __ballot(neg);
+    const double Ee = E[e];
+    const int pa = lane >> 3, pb = lane & 7;
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const int I = 3 * pa + p, Jc = 3 * pb + q;
+            if (Jc >= I) ke300[(size_t)e * 300 + sym_index(I, Jc)] = Ee * Kh[lane * 9 + p * 3 + q];
+        }
+    if (status && lane == 0) status[e] = anyneg ? MSA_NEG_DETJ : MSA_OK;
+}
+
+// CALCH8GAUSSMATRICES drop-in: one lane per Gauss point, the Fortran formulas
+// with absolute coordinates (rElHexa8.f90:25-464).  out layout per Gauss
+// point M: w | S[8] | DS[24] | J[9] (Fortran column-major) | detJ | B[144].
+__global__ void k_gauss_matrices(const double *__restrict__ xyz, double *__restrict__ out)
+{
+    const int M = threadIdx.x;
+    if (M >= 8) return;
+    const double xi = ((M >> 2) & 1) ? kGauss : -kGauss;
+    const double eta = ((M >> 1) & 1) ? kGauss : -kGauss;
+    const double zeta = (M & 1) ? kGauss : -kGauss;
+    double *w = out + 0, *S = out + 8, *DS = out + 72, *J = out + 264, *dJ = out + 336,
+           *B = out + 344;
+    double xiP = (1.0 + xi) * kSq125, etaP = (1.0 + eta) * kSq125, zetaP = (1.0 + zeta) * kSq125;
+    double xiM = (1.0 - xi) * kSq125, etaM = (1.0 - eta) * kSq125, zetaM = (1.0 - zeta) * kSq125;
+    double d[24];
+    d[0] = -etaM * zetaM; d[1] = -d[0]; d[2] = etaP * zetaM; d[3] = -d[2];
+    d[4] = -etaM * zetaP; d[5] = -d[4]; d[6] = etaP * zetaP; d[7] = -d[6];
+    d[8] = -xiM * zetaM; d[9] = -xiP * zetaM; d[10] = -d[9]; d[11] = -d[8];
+    d[12] = -xiM * zetaP; d[13] = -xiP * zetaP; d[14] = -d[13]; d[15] = -d[12];
+    d[16] = -xiM * etaM; d[17] = -xiP * etaM; d[18] = -xiP * etaP; d[19] = -xiM * etaP;
+    d[20] = -d[16]; d[21] = -d[17]; d[22] = -d[18]; d[23] = -d[19];
+    for (int k = 0; k < 24; k++) DS[M * 24 + k] = d[k];
+    const double h = 0.5;
+    double xp = (1.0 + xi) * h, ep = (1.0 + eta) * h, zp = (1.0 + zeta) * h;
+    double xm = (1.0 - xi) * h, em = (1.0 - eta) * h, zm = (1.0 - zeta) * h;
+    double sh[8] = {xm * em * zm, xp * em * zm, xp * ep * zm, xm * ep * zm,
+                    xm * em * zp, xp * em * zp, xp * ep * zp, xm * ep * zp};
+    for (int k = 0; k < 8; k++) S[M * 8 + k] = sh[k];
+    double Jm[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double v = 0;
+            for (int a = 0; a < 8; a++) v += d[8 * i + a] * xyz[3 * a + j];
+            Jm[i][j] = v;
+            J[M * 9 + i + 3 * j] = v;
+        }
+    double det = Jm[0][0] * (Jm[1][1] * Jm[2][2] - Jm[2][1] * Jm[1][2]) -
+                 Jm[0][1] * (Jm[1][0] * Jm[2][2] - Jm[2][0] * Jm[1][2]) +
+                 Jm[0][2] * (Jm[1][0] * Jm[2][1] - Jm[2][0] * Jm[1][1]);
+    dJ[M] = det;
+    double inv = 1.0 / det, Ji[3][3];
+    Ji[0][0] = (Jm[1][1] * Jm[2][2] - Jm[2][1] * Jm[1][2]) * inv;
+    Ji[1][0] = (Jm[2][0] * Jm[1][2] - Jm[1][0] * Jm[2][2]) * inv;
+    Ji[2][0] = (Jm[1][0] * Jm[2][1] - Jm[2][0] * Jm[1][1]) * inv;
+    Ji[0][1] = (Jm[2][1] * Jm[0][2] - Jm[0][1] * Jm[2][2]) * inv;
+    Ji[1][1] = (Jm[0][0] * Jm[2][2] - Jm[2][0] * Jm[0][2]) * inv;
+    Ji[2][1] = (Jm[2][0] * Jm[0][1] - Jm[2][1] * Jm[0][0]) * inv;
+    Ji[0][2] = (Jm[0][1] * Jm[1][2] - Jm[1][1] * Jm[0][2]) * inv;
+    Ji[1][2] = (Jm[1][0] * Jm[0][2] - Jm[0][0] * Jm[1][2]) * inv;
+    Ji[2][2] = (Jm[0][0] * Jm[1][1] - Jm[1][0] * Jm[0][1]) * inv;
+    double *Bm = B + M * 144;
+    for (int k = 0; k < 144; k++) Bm[k] = 0.0;
+    for (int a = 0; a < 8; a++) {
+        double gx = Ji[0][0] * d[a] + Ji[0][1] * d[8 + a] + Ji[0][2] * d[16 + a];
+        double gy = Ji[1][0] * d[a] + Ji[1][1] * d[8 + a] + Ji[1][2] * d[16 + a];
+        double gz = Ji[2][0] * d[a] + Ji[2][1] * d[8 + a] + Ji[2][2] * d[16 + a];
+        int u = 3 * a;
+        Bm[0 + 6 * u] = gx;
+        Bm[1 + 6 * (u + 1)] = gy;
+        Bm[2 + 6 * (u + 2)] = gz;
+        Bm[3 + 6 * u] = gy; Bm[3 + 6 * (u + 1)] = gx;
+        Bm[4 + 6 * (u + 1)] = gz; Bm[4 + 6 * (u + 2)] = gy;
+        Bm[5 + 6 * u] = gz; Bm[5 + 6 * (u + 2)] = gx;
+    }
+    w[M] = 1.0 * 1.0 * 1.0 * det;
+}
+
+// CALCH8K drop-in (rElHexa8.f90:489-543): one lane per packed entry, the
+// Fortran summation order over Gauss points and strain components.
+__global__ void k_h8k(const double *__restrict__ E, const double *__restrict__ B,
+                      const double *__restrict__ w, double *__restrict__ K)
+{
+    const int ks = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ks >= 300) return;
+    int I = 0, rem = ks;
+    while (rem >= 24 - I) {
+        rem -= 24 - I;
+        I++;
+    }
+    const int Jc = I + rem;
+    double acc = 0;
+    for (int M = 0; M < 8; M++) {
+        const double *Em = E + 36 * M, *Bm = B + 144 * M;
+        double st = 0;
+        double EB[6];
+        for (int r = 0; r < 6; r++)
+            EB[r] = Em[r] * Bm[6 * I] + Em[r + 6] * Bm[1 + 6 * I] + Em[r + 12] * Bm[2 + 6 * I] +
+                    Em[r + 18] * Bm[3 + 6 * I] + Em[r + 24] * Bm[4 + 6 * I] +
+                    Em[r + 30] * Bm[5 + 6 * I];
+        st = Bm[6 * Jc] * EB[0] + Bm[1 + 6 * Jc] * EB[1] + Bm[2 + 6 * Jc] * EB[2] +
+             Bm[3 + 6 * Jc] * EB[3] + Bm[4 + 6 * Jc] * EB[4] + Bm[5 + 6 * Jc] * EB[5];
+        acc = acc + st * w[M];
+    }
+    K[ks] = acc;
+}
+
+__device__ __forceinline__ int colour_class(const ColourLaunch &cl, int item)
+{
+    int c = 0;
+    while (c + 1 < cl.ncls && item >= cl.prefix[c + 1]) c++;
+    return c;
+}
+
+// Assembly: one wavefront per element of the current colour (all classes),
+// Khat staged in LDS, then val += E_s * Khat for every sample of every tile.
+// Elements of one colour share no node, so the read-modify-writes are
+// race-free.  kreal = precision of the element integration.
+template <typename kreal, typename real, int T>
+__global__ __launch_bounds__(64) void k_assemble(const ClassDev *__restrict__ cls, ColourLaunch cl,
+                                                 double nu)
+{
+    constexpr int R = 64 / T;
+    __shared__ kreal Kh[576];
+    __shared__ kreal g[192];
+    __shared__ kreal w[8];
+    const int item = blockIdx.x;
+    const int c = colour_class(cl, item);
+    if (item >= cl.prefix[cl.ncls]) return;
+    const ClassDev &C = cls[c];
+    const int e = C.colour_elems[C.colour_ptr[cl.colour] + (item - cl.prefix[c])];
+    const int lane = threadIdx.x & 63;
+    const int *en = C.conn + (size_t)e * 8;
+    const int na = en[lane & 7];
+    const double xa[3] = {C.xyz[3 * na], C.xyz[3 * na + 1], C.xyz[3 * na + 2]};
+    const bool neg = wave_khat<kreal>(xa, nu, Kh, g, w);
+    if (__ballot(neg) && lane == 0) atomicAdd(C.negdet, 1);
+    const uint8_t *em = C.emap + (size_t)e * 64;
+    const int s = lane % T;
+    for (int tile = 0; tile < C.ntiles; tile++) {
+        if (tile * T + s >= C.nsamp) continue;
+        real *val = (real *)C.val + (size_t)tile * C.tile_vals;
+        const real Es = (real)C.E[((size_t)tile * C.n_elem + e) * T + s];
+        for (int pe = lane / T; pe < 576; pe += R) {
+            const int pair = pe / 9, ent = pe - 9 * pair;
+            const int j = em[pair];
+            if (j == 255) continue;
+            const int a = pair >> 3, b = pair & 7;
+            const int p = ent / 3, q = ent - 3 * (ent / 3);
+            const int nA = en[a], nB = en[b];
+            if (!((C.node_free[nA] >> p) & 1) || !((C.node_free[nB] >> q) & 1)) continue;
+            const int br = C.node_block[nA];
+            const int slot = C.slice_ptr[br / R] + j;
+            const size_t idx = (((size_t)slot * 9 + ent) * R + (br % R)) * T + s;
+            val[idx] += Es * (real)Kh[pe];
+        }
+    }
+}
+
+// ------------------------------------------------------------ PCG kernels
+
+template <typename real, int T>
+__global__ __launch_bounds__(256) void k_dinv(const ClassDev *__restrict__ cls,
+                                              const int4 *__restrict__ items, int nitems)
+{
+    constexpr int R = 64 / T;
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const int tile = it.y, chunk = it.z;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rr = lane / T, s = lane % T;
+    if (tile * T + s >= C.nsamp) return;
+    const real *val = (const real *)C.val + (size_t)tile * C.tile_vals;
+    real *dinv = (real *)C.dinv + (size_t)tile * C.tile_vec;
+    const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+    for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+        const int br = slice * R + rr;
+        if (br >= C.nb) continue;
+        const int slot = C.slice_ptr[slice] + C.diag_pos[br];
+        const uint8_t fr = C.block_free[br];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const real v = val[(((size_t)slot * 9 + 4 * d) * R + rr) * T + s];
+            dinv[((size_t)br * 3 + d) * T + s] =
+                (((fr >> d) & 1) && v != (real)0) ? (real)1 / v : (real)0;
+        }
+    }
+}
+
+// Start of a PCG solve with x0 = 0 (SolverPCGSkyline.Initialize +
+// SolverPCG.Solve prologue, SolverPCGSkyline.cs:75-100, SolverPCG.cs:112-114):
+// r = b, z = D^-1 r, p = 0, |b|.  mode 0: b = f (FP64 solve);
+// mode 1: b = r64 / |r64| (inner solve of the refinement).
+template <typename real, int T>
+__global__ __launch_bounds__(256) void k_pcg_init(const ClassDev *__restrict__ cls,
+                                                  const int4 *__restrict__ items, int nitems,
+                                                  int mode, int *active_count)
+{
+    constexpr int R = 64 / T;
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const int tile = it.y, chunk = it.z;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rr = lane / T, s = lane % T;
+    const int sample = tile * T + s;
+    const bool act = sample < C.nsamp && (mode == 0 || C.st[sample].ir_active);
+    const double sc = (act && mode) ? 1.0 / C.st[sample].scale : 1.0;
+    const size_t vb = (size_t)tile * C.tile_vec;
+    real *x = (real *)C.x + vb, *r = (real *)C.r + vb, *z = (real *)C.z + vb;
+    real *p0 = (real *)C.p0 + vb, *p1 = (real *)C.p1 + vb;
+    const real *dinv = (const real *)C.dinv + vb;
+    const double *r64 = C.r64 ? C.r64 + vb : nullptr;
+    double acc[2] = {0, 0};
+    const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+    if (act) {
+        for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+            const int br = slice * R + rr;
+            if (br >= C.nb) continue;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const size_t o = ((size_t)br * 3 + d) * T + s;
+                const real b = mode ? (real)(r64[o] * sc) : (real)C.f[br * 3 + d];
+                const real zv = dinv[o] * b;
+                r[o] = b;
+                x[o] = 0;
+                z[o] = zv;
+                p0[o] = 0;
+                p1[o] = 0;
+                acc[0] += (double)(b * zv);
+                acc[1] += (double)(b * b);
+            }
+        }
+    }
+    block_partials<T, 2>(acc, C.partial + ((size_t)tile * C.nchunks + chunk) * 3 * T);
+    if (last_block(C.counter + tile * 8 + 0, C.nchunks) && threadIdx.x < T) {
+        const int t = threadIdx.x, smp = tile * T + t;
+        if (smp < C.nsamp) {
+            SampleState &S = C.st[smp];
+            const bool on = (mode == 0) || S.ir_active;
+            S.iters = 0;
+            S.beta = 0;
+            if (!on) {
+                S.active = 0;
+            } else {
+                const double rz = sum_partials<T>(C, tile, 0, t);
+                const double bb = sum_partials<T>(C, tile, 1, t);
+                S.rz = rz;
+                S.bnorm = sqrt(bb);
+                S.relres = 1.0;
+                if (!isfinite(bb) || !isfinite(rz)) {
+                    S.status = MSA_NAN_INPUT;
+                    S.active = 0;
+                } else if (S.bnorm > 1e-25) {
+                    S.status = MSA_NOT_CONVERGED;
+                    S.active = 1;
+                    atomicAdd(active_count, 1);
+                } else {
+                    S.status = MSA_OK;
+                    S.relres = 0;
+                    S.active = 0;
+                }
+            }
+        }
+    }
+}
+
+// Search direction + product + curvature (SolverPCGSimpleSearchVectorCalculator
+// .cs:18-41, MCPCGDirect.cs:48-51): p = z + beta p_old on the fly for every
+// gathered column, q = K p, own rows of p written to the other buffer,
+// pq = p.q per sample; the last workgroup of each tile sets alpha = rz/pq.
+template <typename real, int T>
+__global__ __launch_bounds__(256) void k_pcg_a(const ClassDev *__restrict__ cls,
+                                               const int4 *__restrict__ items, int nitems,
+                                               int pbuf, int *active_count)
+{
+    constexpr int R = 64 / T;
+    constexpr int ES = R * T;  // stride between the 9 entries of a block = 64
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const int tile = it.y, chunk = it.z;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int rr = lane / T, s = lane % T;
+    const int sample = tile * T + s;
+    const bool act = sample < C.nsamp && C.st[sample].active;
+    const real beta = act ? (real)C.st[sample].beta : (real)0;
+    const size_t vb = (size_t)tile * C.tile_vec;
+    const real *__restrict__ z = (const real *)C.z + vb;
+    const real *__restrict__ po = (const real *)(pbuf ? C.p1 : C.p0) + vb;
+    real *__restrict__ pn = (real *)(pbuf ? C.p0 : C.p1) + vb;
+    real *__restrict__ q = (real *)C.q + vb;
+    const real *__restrict__ val = (const real *)C.val + (size_t)tile * C.tile_vals;
+    real pq = 0;
+    const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+    if (__ballot(act)) {
+        for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+            const int br = slice * R + rr;
+            const int base = C.slice_ptr[slice];
+            const int W = C.slice_ptr[slice + 1] - base;
+            if (act && br < C.nb) {
+                real a0 = 0, a1 = 0, a2 = 0;
+#pragma unroll 3
+                for (int j = 0; j < W; j++) {
+                    const int slot = base + j;
+                    const int col = C.cidx[(size_t)slot * R + rr];
+                    if (col < 0) continue;
+                    const size_t xo = (size_t)col * 3 * T + s;
+                    const real v0 = z[xo] + beta * po[xo];
+                    const real v1 = z[xo + T] + beta * po[xo + T];
+                    const real v2 = z[xo + 2 * T] + beta * po[xo + 2 * T];
+                    const real *m = val + ((size_t)slot * 9 * R + rr) * T + s;
+                    a0 += m[0] * v0 + m[ES] * v1 + m[2 * ES] * v2;
+                    a1 += m[3 * ES] * v0 + m[4 * ES] * v1 + m[5 * ES] * v2;
+                    a2 += m[6 * ES] * v0 + m[7 * ES] * v1 + m[8 * ES] * v2;
+                }
+                const size_t o = (size_t)br * 3 * T + s;
+                const real n0 = z[o] + beta * po[o];
+                const real n1 = z[o + T] + beta * po[o + T];
+                const real n2 = z[o + 2 * T] + beta * po[o + 2 * T];
+                pn[o] = n0;
+                pn[o + T] = n1;
+                pn[o + 2 * T] = n2;
+                q[o] = a0;
+                q[o + T] = a1;
+                q[o + 2 * T] = a2;
+                pq += n0 * a0 + n1 * a1 + n2 * a2;
+            }
+        }
+    }
+    double acc[1] = {(double)pq};
+    block_partials<T, 1>(acc, C.partial + ((size_t)tile * C.nchunks + chunk) * 3 * T);
+    if (last_block(C.counter + tile * 8 + 1, C.nchunks) && threadIdx.x < T) {
+        const int t = threadIdx.x, smp = tile * T + t;
+        if (smp < C.nsamp && C.st[smp].active) {
+            SampleState &S = C.st[smp];
+            const double v = sum_partials<T>(C, tile, 0, t);
+            S.pq = v;
+            if (!(v > 0) || !isfinite(v)) {
+                S.status = MSA_BREAKDOWN;
+                S.active = 0;
+                atomicSub(active_count, 1);
+            } else {
+                S.alpha = S.rz / v;
+            }
+        }
+    }
+}
+
+// Update + preconditioner + stopping test (SolverPCG.cs:122-136):
+// x += alpha p, r -= alpha q, z = D^-1 r, rz = r.z, rr = r.r; the last
+// workgroup of each tile checks |r|/|b| < tol and forms beta.
+template <typename real, int T>
+__global__ __launch_bounds__(256) void k_pcg_b(const ClassDev *__restrict__ cls,
+                                               const int4 *__restrict__ items, int nitems,
+                                               int pbuf, double tol, int max_it,
+                                               int *active_count)
+{
+    constexpr int R = 64 / T;
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const int tile = it.y, chunk = it.z;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rr = lane / T, s = lane % T;
+    const int sample = tile * T + s;
+    const bool act = sample < C.nsamp && C.st[sample].active;
+    const real alpha = act ? (real)C.st[sample].alpha : (real)0;
+    const size_t vb = (size_t)tile * C.tile_vec;
+    real *__restrict__ x = (real *)C.x + vb;
+    real *__restrict__ r = (real *)C.r + vb;
+    real *__restrict__ z = (real *)C.z + vb;
+    const real *__restrict__ pn = (const real *)(pbuf ? C.p0 : C.p1) + vb;
+    const real *__restrict__ q = (const real *)C.q + vb;
+    const real *__restrict__ dinv = (const real *)C.dinv + vb;
+    real rz = 0, rr2 = 0;
+    const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+    if (act) {
+        for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+            const int br = slice * R + rr;
+            if (br >= C.nb) continue;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const size_t o = ((size_t)br * 3 + d) * T + s;
+                const real xv = x[o] + alpha * pn[o];
+                const real rv = r[o] - alpha * q[o];
+                const real zv = dinv[o] * rv;
+                x[o] = xv;
+                r[o] = rv;
+                z[o] = zv;
+                rz += rv * zv;
+                rr2 += rv * rv;
+            }
+        }
+    }
+    double acc[2] = {(double)rz, (double)rr2};
+    block_partials<T, 2>(acc, C.partial + ((size_t)tile * C.nchunks + chunk) * 3 * T);
+    if (last_block(C.counter + tile * 8 + 2, C.nchunks) && threadIdx.x < T) {
+        const int t = threadIdx.x, smp = tile * T + t;
+        if (smp < C.nsamp && C.st[smp].active) {
+            SampleState &S = C.st[smp];
+            const double rzn = sum_partials<T>(C, tile, 0, t);
+            const double rrn = sum_partials<T>(C, tile, 1, t);
+            S.iters += 1;
+            S.rr = rrn;
+            const double err = sqrt(rrn) / S.bnorm;
+            S.relres = err;
+            if (!isfinite(err) || !isfinite(rzn)) {
+                S.status = MSA_BREAKDOWN;
+                S.active = 0;
+                atomicSub(active_count, 1);
+            } else if (err < tol) {
+                S.status = MSA_OK;
+                S.active = 0;
+                atomicSub(active_count, 1);
+            } else if (S.iters >= max_it) {
+                S.status = MSA_NOT_CONVERGED;
+                S.active = 0;
+                atomicSub(active_count, 1);
+            } else {
+                S.beta = rzn / S.rz;
+                S.rz = rzn;
+            }
+        }
+    }
+}
+
+// y = K x for every lane of one tile (tests and the single-product entry).
+template <typename real, int T>
+__global__ __launch_bounds__(256) void k_spmv(const ClassDev *__restrict__ cls,
+                                              const int4 *__restrict__ items, int nitems,
+                                              const double *__restrict__ xin,
+                                              double *__restrict__ yout)
+{
+    constexpr int R = 64 / T;
+    constexpr int ES = R * T;
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const int tile = it.y, chunk = it.z;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rr = lane / T, s = lane % T;
+    const real *val = (const real *)C.val + (size_t)tile * C.tile_vals;
+    const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+    for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+        const int br = slice * R + rr;
+        if (br >= C.nb) continue;
+        const int base = C.slice_ptr[slice], W = C.slice_ptr[slice + 1] - base;
+        double a0 = 0, a1 = 0, a2 = 0;
+        for (int j = 0; j < W; j++) {
+            const int slot = base + j;
+            const int col = C.cidx[(size_t)slot * R + rr];
+            if (col < 0) continue;
+            const size_t xo = (size_t)col * 3 * T + s;
+            const double v0 = xin[xo], v1 = xin[xo + T], v2 = xin[xo + 2 * T];
+            const real *m = val + ((size_t)slot * 9 * R + rr) * T + s;
+            a0 += (double)m[0] * v0 + (double)m[ES] * v1 + (double)m[2 * ES] * v2;
+            a1 += (double)m[3 * ES] * v0 + (double)m[4 * ES] * v1 + (double)m[5 * ES] * v2;
+            a2 += (double)m[6 * ES] * v0 + (double)m[7 * ES] * v1 + (double)m[8 * ES] * v2;
+        }
+        const size_t o = (size_t)br * 3 * T + s;
+        yout[o] = a0;
+        yout[o + T] = a1;
+        yout[o + 2 * T] = a2;
+    }
+}
+
+// ------------------------------------------------------------ refinement
+
+// u64 = 0, r64 = f, outer state (mixed precision only).
+template <int T>
+__global__ __launch_bounds__(256) void k_ir_begin(const ClassDev *__restrict__ cls,
+                                                  const int4 *__restrict__ items, int nitems,
+                                                  int *ir_count)
+{
+    constexpr int R = 64 / T;
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const int tile = it.y, chunk = it.z;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rr = lane / T, s = lane % T;
+    const int sample = tile * T + s;
+    const size_t vb = (size_t)tile * C.tile_vec;
+    if (sample < C.nsamp) {
+        const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+        for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+            const int br = slice * R + rr;
+            if (br >= C.nb) continue;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const size_t o = vb + ((size_t)br * 3 + d) * T + s;
+                C.u64[o] = 0;
+                C.r64[o] = C.f[br * 3 + d];
+            }
+        }
+    }
+    if (chunk == 0 && threadIdx.x < T) {
+        const int smp = tile * T + threadIdx.x;
+        if (smp < C.nsamp) {
+            SampleState &S = C.st[smp];
+            S.r64norm = C.fnorm;
+            S.scale = C.fnorm;
+            S.outer = 0;
+            S.total_iters = 0;
+            S.relres = 1.0;
+            S.active = 0;
+            if (!isfinite(C.fnorm)) {
+                S.status = MSA_NAN_INPUT;
+                S.ir_active = 0;
+            } else if (C.fnorm > 1e-25) {
+                S.status = MSA_NOT_CONVERGED;
+                S.ir_active = 1;
+                atomicAdd(ir_count, 1);
+            } else {
+                S.status = MSA_OK;
+                S.relres = 0;
+                S.ir_active = 0;
+            }
+        }
+    }
+}
+
+// u64 += |r64| * x (FP64 update of the refinement), r64 = f.
+template <int T>
+__global__ __launch_bounds__(256) void k_ir_update(const ClassDev *__restrict__ cls,
+                                                   const int4 *__restrict__ items, int nitems)
+{
+    constexpr int R = 64 / T;
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const int tile = it.y, chunk = it.z;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rr = lane / T, s = lane % T;
+    const int sample = tile * T + s;
+    if (sample >= C.nsamp || !C.st[sample].ir_active) return;
+    const double sc = C.st[sample].scale;
+    const size_t vb = (size_t)tile * C.tile_vec;
+    const float *x = (const float *)C.x + vb;
+    const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+    for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+        const int br = slice * R + rr;
+        if (br >= C.nb) continue;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const size_t o = ((size_t)br * 3 + d) * T + s;
+            C.u64[vb + o] += sc * (double)x[o];
+            C.r64[vb + o] = C.f[br * 3 + d];
+        }
+    }
+}
+
+// FP64 residual r64 -= K u64, matrix-free: one wavefront per element of the
+// current colour recomputes Khat in FP64 (LDS) and applies E_s Khat to the
+// element's displacements of every active sample.
+template <int T>
+__global__ __launch_bounds__(64) void k_residual64(const ClassDev *__restrict__ cls,
+                                                   ColourLaunch cl, double nu)
+{
+    constexpr int R = 64 / T;
+    __shared__ double Kh[576];
+    __shared__ double g[192];
+    __shared__ double w[8];
+    const int item = blockIdx.x;
+    const int c = colour_class(cl, item);
+    if (item >= cl.prefix[cl.ncls]) return;
+    const ClassDev &C = cls[c];
+    const int e = C.colour_elems[C.colour_ptr[cl.colour] + (item - cl.prefix[c])];
+    const int lane = threadIdx.x & 63;
+    const int *en = C.conn + (size_t)e * 8;
+    const int na = en[lane & 7];
+    const double xa[3] = {C.xyz[3 * na], C.xyz[3 * na + 1], C.xyz[3 * na + 2]};
+    (void)wave_khat<double>(xa, nu, Kh, g, w);
+    const int s = lane % T, sub = lane / T;
+    for (int tile = 0; tile < C.ntiles; tile++) {
+        const int smp = tile * T + s;
+        if (smp >= C.nsamp || !C.st[smp].ir_active) continue;
+        const size_t vb = (size_t)tile * C.tile_vec;
+        double u[24];
+#pragma unroll
+        for (int jn = 0; jn < 8; jn++) {
+            const int nd = en[jn];
+            const int b = C.node_block[nd];
+            const uint8_t fr = C.node_free[nd];
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+                u[3 * jn + d] = (b >= 0 && ((fr >> d) & 1))
+                                    ? C.u64[vb + ((size_t)b * 3 + d) * T + s]
+                                    : 0.0;
+        }
+        const double Es = C.E[((size_t)tile * C.n_elem + e) * T + s];
+        for (int i = sub; i < 24; i += R) {
+            const int ia = i / 3, p = i - 3 * (i / 3);
+            const int nd = en[ia];
+            const int b = C.node_block[nd];
+            if (b < 0 || !((C.node_free[nd] >> p) & 1)) continue;
+            double y = 0;
+#pragma unroll
+            for (int jn = 0; jn < 8; jn++) {
+                const double *blk = Kh + (ia * 8 + jn) * 9 + p * 3;
+                y += blk[0] * u[3 * jn] + blk[1] * u[3 * jn + 1] + blk[2] * u[3 * jn + 2];
+            }
+            C.r64[vb + ((size_t)b * 3 + p) * T + s] -= Es * y;
+        }
+    }
+}
+
+// |r64| per sample and the refinement stopping test |f - K u|/|f| < tol.
+template <int T>
+__global__ __launch_bounds__(256) void k_ir_norm(const ClassDev *__restrict__ cls,
+                                                 const int4 *__restrict__ items, int nitems,
+                                                 double tol, int max_outer, int *ir_count)
+{
+    constexpr int R = 64 / T;
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const int tile = it.y, chunk = it.z;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rr = lane / T, s = lane % T;
+    const int sample = tile * T + s;
+    const bool act = sample < C.nsamp && C.st[sample].ir_active;
+    const size_t vb = (size_t)tile * C.tile_vec;
+    double acc[1] = {0};
+    if (act) {
+        const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+        for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+            const int br = slice * R + rr;
+            if (br >= C.nb) continue;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const double v = C.r64[vb + ((size_t)br * 3 + d) * T + s];
+                acc[0] += v * v;
+            }
+        }
+    }
+    block_partials<T, 1>(acc, C.partial + ((size_t)tile * C.nchunks + chunk) * 3 * T);
+    if (last_block(C.counter + tile * 8 + 3, C.nchunks) && threadIdx.x < T) {
+        const int t = threadIdx.x, smp = tile * T + t;
+        if (smp < C.nsamp && C.st[smp].ir_active) {
+            SampleState &S = C.st[smp];
+            const double nr = sqrt(sum_partials<T>(C, tile, 0, t));
+            S.total_iters += S.iters;
+            S.outer += 1;
+            S.r64norm = nr;
+            const double err = nr / C.fnorm;
+            S.relres = err;
+            if (!isfinite(err)) {
+                S.status = MSA_BREAKDOWN;
+                S.ir_active = 0;
+                atomicSub(ir_count, 1);
+            } else if (err < tol) {
+                S.status = MSA_OK;
+                S.ir_active = 0;
+                atomicSub(ir_count, 1);
+            } else if (S.outer >= max_outer) {
+                S.status = MSA_NOT_CONVERGED;
+                S.ir_active = 0;
+                atomicSub(ir_count, 1);
+            } else {
+                S.scale = nr;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------ KL field
+
+// phi[e][i] = sigma sqrt(eigs_i) prod_d a * (index even ? sin : cos)
+// (w (c_d - L_d/2)) at the element centroid (Lognormal...cs:134-160,
+// Hexa8WithStochasticMaterial.cs:112-139).
+__global__ void k_kl_basis(const ClassDev *__restrict__ cp, int n_elem, int M, int kmax,
+                           const double *__restrict__ w1d, const double *__restrict__ a1d,
+                           const int *__restrict__ evpPos, const double *__restrict__ eigs,
+                           const double *__restrict__ domain, double sigma)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_elem) return;
+    const ClassDev &C = *cp;
+    double x8[24];
+    for (int a = 0; a < 8; a++)
+        for (int j = 0; j < 3; j++) x8[3 * a + j] = C.xyz[3 * C.conn[(size_t)e * 8 + a] + j];
+    double X = 0, Y = 0, Z = 0, mnx = x8[0], mny = x8[1], mnz = x8[2];
+    for (int i = 0; i < 8; i++) {
+        mnx = mnx > x8[3 * i] ? x8[3 * i] : mnx;
+        mny = mny > x8[3 * i + 1] ? x8[3 * i + 1] : mny;
+        mnz = mnz > x8[3 * i + 2] ? x8[3 * i + 2] : mnz;
+        for (int j = i + 1; j < 8; j++) {
+            double ax = fabs(x8[3 * j] - x8[3 * i]);
+            double ay = fabs(x8[3 * j + 1] - x8[3 * i + 1]);
+            double az = fabs(x8[3 * j + 2] - x8[3 * i + 2]);
+            X = X < ax ? ax : X;
+            Y = Y < ay ? ay : Y;
+            Z = Z < az ? az : Z;
+        }
+    }
+    const double cxyz[3] = {mnx + X / 2, mny + Y / 2, mnz + Z / 2};
+    for (int i = 0; i < M; i++) {
+        double f = sigma * sqrt(eigs[i]);
+        for (int d = 0; d < 3; d++) {
+            const int index = evpPos[3 * i + d];
+            const double wv = w1d[d * kmax + index - 1];
+            const double av = a1d[d * kmax + index - 1];
+            const double x = cxyz[d] - domain[d] / 2;
+            f *= (index % 2 == 0) ? av * sin(wv * x) : av * cos(wv * x);
+        }
+        C.phi[(size_t)e * M + i] = f;
+    }
+}
+
+// E[tile][e][s] = exp(c0 + sum_i xi_s,i phi_e,i) (Lognormal...cs:173-193).
+template <int T>
+__global__ void k_kl_eval(const ClassDev *__restrict__ cp, int ntiles, int n_elem, int M)
+{
+    const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t total = (size_t)ntiles * n_elem * T;
+    if (id >= total) return;
+    const ClassDev &C = *cp;
+    const int s = (int)(id % T);
+    const size_t te = id / T;
+    const int e = (int)(te % n_elem);
+    const int tile = (int)(te / n_elem);
+    const int smp = tile * T + s;
+    if (smp >= C.nsamp) {
+        C.E[id] = 0.0;
+        return;
+    }
+    double res = C.c0;
+    const double *xs = C.xi + (size_t)smp * M;
+    const double *ph = C.phi + (size_t)e * M;
+    for (int i = 0; i < M; i++) res += xs[i] * ph[i];
+    C.E[id] = exp(res);
+}
+
+// ------------------------------------------------------------ utilities
+
+template <typename V>
+__global__ void k_fill(V *p, size_t n, V v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+template <typename real>
+__global__ void k_extract(const real *__restrict__ src, size_t n, int T, int lane,
+                          double *__restrict__ dst)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (double)src[i * T + lane];
+}
+
+__global__ void k_insert(const double *__restrict__ src, size_t n, int T, int lane,
+                         double *__restrict__ dst)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i * T + lane] = src[i];
+}
+
+// ------------------------------------------------------------ launchers
+
+static inline unsigned grid1d(size_t n, unsigned block)
+{
+    size_t g = (n + block - 1) / block;
+    if (g > 1u << 20) g = 1u << 20;
+    return (unsigned)(g ? g : 1);
+}
+
+hipError_t launch_fill_f64(hipStream_t s, double *p, size_t n, double v)
+{
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_fill<double>, dim3(grid1d(n, 256)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+hipError_t launch_fill_f32(hipStream_t s, float *p, size_t n, float v)
+{
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_fill<float>, dim3(grid1d(n, 256)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+hipError_t launch_fill_u32(hipStream_t s, unsigned *p, size_t n, unsigned v)
+{
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_fill<unsigned>, dim3(grid1d(n, 256)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_ke_packed(hipStream_t s, int n_elem, const double *xyz8, const double *E,
+                            double nu, double *ke300, int *status)
+{
+    if (n_elem <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ke_packed, dim3(n_elem), dim3(64), 0, s, n_elem, xyz8, E, nu, ke300,
+                       status);
+    return hipGetLastError();
+}
+
+hipError_t launch_gauss_matrices(hipStream_t s, const double *xyz, double *out)
+{
+    hipLaunchKernelGGL(k_gauss_matrices, dim3(1), dim3(64), 0, s, xyz, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_h8k(hipStream_t s, const double *E, const double *B, const double *w, double *K)
+{
+    hipLaunchKernelGGL(k_h8k, dim3(5), dim3(64), 0, s, E, B, w, K);
+    return hipGetLastError();
+}
+
+#define MSA_DISPATCH_T(T, BODY)                                                   \
+    switch (T) {                                                                  \
+    case 1: { constexpr int TT = 1; BODY; } break;                                \
+    case 2: { constexpr int TT = 2; BODY; } break;                                \
+    case 4: { constexpr int TT = 4; BODY; } break;                                \
+    case 8: { constexpr int TT = 8; BODY; } break;                                \
+    case 16: { constexpr int TT = 16; BODY; } break;                              \
+    case 32: { constexpr int TT = 32; BODY; } break;                              \
+    case 64: { constexpr int TT = 64; BODY; } break;                              \
+    default: return hipErrorInvalidValue;                                         \
+    }
+
+hipError_t launch_assemble(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
+                           const ColourLaunch &cl, int nblocks, double nu)
+{
+    if (nblocks <= 0) return hipSuccess;
+    if (mixed) {
+        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_assemble<float, float, TT>), dim3(nblocks),
+                                             dim3(64), 0, s, d_cls, cl, nu));
+    } else {
+        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_assemble<double, double, TT>), dim3(nblocks),
+                                             dim3(64), 0, s, d_cls, cl, nu));
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_residual64(hipStream_t s, int T, const ClassDev *d_cls, const ColourLaunch &cl,
+                             int nblocks, double nu)
+{
+    if (nblocks <= 0) return hipSuccess;
+    MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_residual64<TT>), dim3(nblocks), dim3(64), 0, s,
+                                         d_cls, cl, nu));
+    return hipGetLastError();
+}
+
+#define MSA_PCG_LAUNCH(KERNEL, ...)                                                       \
+    do {                                                                                  \
+        if (nitems <= 0) return hipSuccess;                                               \
+        const dim3 grid(xcd_grid(nitems)), block(256);                                    \
+        if (mixed) {                                                                      \
+            MSA_DISPATCH_T(T, hipLaunchKernelGGL((KERNEL<float, TT>), grid, block, 0, s,  \
+                                                 __VA_ARGS__));                           \
+        } else {                                                                          \
+            MSA_DISPATCH_T(T, hipLaunchKernelGGL((KERNEL<double, TT>), grid, block, 0, s, \
+                                                 __VA_ARGS__));                           \
+        }                                                                                 \
+        return hipGetLastError();                                                         \
+    } while (0)
+
+hipError_t launch_dinv(hipStream_t s, bool mixed, int T, const ClassDev *d_cls, const int4 *items,
+                       int nitems)
+{
+    MSA_PCG_LAUNCH(k_dinv, d_cls, items, nitems);
+}
+
+hipError_t launch_pcg_init(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
+                           const int4 *items, int nitems, int mode, int *active_count)
+{
+    MSA_PCG_LAUNCH(k_pcg_init, d_cls, items, nitems, mode, active_count);
+}
+
+hipError_t launch_pcg_a(hipStream_t s, bool mixed, int T, const ClassDev *d_cls, const int4 *items,
+                        int nitems, int pbuf, int *active_count)
+{
+    MSA_PCG_LAUNCH(k_pcg_a, d_cls, items, nitems, pbuf, active_count);
+}
+
+hipError_t launch_pcg_b(hipStream_t s, bool mixed, int T, const ClassDev *d_cls, const int4 *items,
+                        int nitems, int pbuf, double tol, int max_it, int *active_count)
+{
+    MSA_PCG_LAUNCH(k_pcg_b, d_cls, items, nitems, pbuf, tol, max_it, active_count);
+}
+
+hipError_t launch_spmv(hipStream_t s, bool mixed, int T, const ClassDev *d_cls, const int4 *items,
+                       int nitems, const double *x, double *y)
+{
+    MSA_PCG_LAUNCH(k_spmv, d_cls, items, nitems, x, y);
+}
+
+#define MSA_T_LAUNCH(KERNEL, ...)                                                          \
+    do {                                                                                   \
+        if (nitems <= 0) return hipSuccess;                                                \
+        const dim3 grid(xcd_grid(nitems)), block(256);                                     \
+        MSA_DISPATCH_T(T, hipLaunchKernelGGL((KERNEL<TT>), grid, block, 0, s, __VA_ARGS__)); \
+        return hipGetLastError();                                                          \
+    } while (0)
+
+hipError_t launch_ir_begin(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,
+                           int nitems, int *ir_count)
+{
+    MSA_T_LAUNCH(k_ir_begin, d_cls, items, nitems, ir_count);
+}
+
+hipError_t launch_ir_update(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,
+                            int nitems)
+{
+    MSA_T_LAUNCH(k_ir_update, d_cls, items, nitems);
+}
+
+hipError_t launch_ir_norm(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,
+                          int nitems, double tol, int max_outer, int *ir_count)
+{
+    MSA_T_LAUNCH(k_ir_norm, d_cls, items, nitems, tol, max_outer, ir_count);
+}
+
+hipError_t launch_kl_basis(hipStream_t s, const ClassDev *d_cls, int n_elem, int M, int kmax,
+                           const double *w1d, const double *a1d, const int *evpPos,
+                           const double *eigs, const double *domain, double sigma)
+{
+    hipLaunchKernelGGL(k_kl_basis, dim3((n_elem + 127) / 128), dim3(128), 0, s, d_cls, n_elem, M,
+                       kmax, w1d, a1d, evpPos, eigs, domain, sigma);
+    return hipGetLastError();
+}
+
+hipError_t launch_kl_eval(hipStream_t s, int T, const ClassDev *d_cls, int ntiles, int n_elem,
+                          int M)
+{
+    const size_t total = (size_t)ntiles * n_elem * T;
+    const unsigned g = (unsigned)((total + 255) / 256);
+    MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_kl_eval<TT>), dim3(g), dim3(256), 0, s, d_cls, ntiles,
+                                         n_elem, M));
+    return hipGetLastError();
+}
+
+hipError_t launch_extract_sample(hipStream_t s, bool mixed, int T, const void *src, size_t n_rows,
+                                 int lane, double *dst)
+{
+    if (!n_rows) return hipSuccess;
+    const unsigned g = (unsigned)((n_rows + 255) / 256);
+    if (mixed)
+        hipLaunchKernelGGL(k_extract<float>, dim3(g), dim3(256), 0, s, (const float *)src, n_rows,
+                           T, lane, dst);
+    else
+        hipLaunchKernelGGL(k_extract<double>, dim3(g), dim3(256), 0, s, (const double *)src,
+                           n_rows, T, lane, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_insert_sample(hipStream_t s, int T, const double *src, size_t n_rows, int lane,
+                                double *dst)
+{
+    if (!n_rows) return hipSuccess;
+    const unsigned g = (unsigned)((n_rows + 255) / 256);
+    hipLaunchKernelGGL(k_insert, dim3(g), dim3(256), 0, s, src, n_rows, T, lane, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_extract_values(hipStream_t s, bool mixed, int T, const void *val, size_t n,
+                                 int lane, double *dst)
+{
+    return launch_extract_sample(s, mixed, T, val, n, lane, dst);
+}
+
+}  // namespace msa

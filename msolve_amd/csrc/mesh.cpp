@@ -1,0 +1,152 @@
+// Host-side analysis of a Hexa8 mesh, run once per mesh (not per sample).
+//
+// The reference rebuilds its skyline profile for every sample
+// (GlobalMatrixAssemblerSkyline.CalculateRowIndex, GMAS.cs:14-44) and finds
+// each scatter target through two dictionary lookups and
+// SymmetricMatrix2D.IndexOf (GMAS.cs:68-98).  Here the sparsity is fixed per
+// mesh: node blocks (3x3) of the free-DOF K, sorted block columns per block
+// row, the slot of every element node pair (a, b) inside block row
+// block(a), and an element colouring in which no two elements of a colour
+// share a node, so the device scatter is race-free and deterministic.
+#include <algorithm>
+#include <cstring>
+
+#include "msa_common.hpp"
+
+namespace msa {
+
+int build_mesh_host(MeshHost &m, int n_nodes, const double *xyz, int n_elem,
+                    const int *conn, const int *dof, int n_free)
+{
+    MSA_CHECK(n_nodes > 0 && n_elem > 0 && n_free >= 0, MSA_ERR_ARG,
+              "mesh: n_nodes=%d n_elem=%d n_free=%d", n_nodes, n_elem, n_free);
+    MSA_CHECK(xyz && conn && dof, MSA_ERR_ARG, "mesh: null array");
+    m.n_nodes = n_nodes;
+    m.n_elem = n_elem;
+    m.n_free = n_free;
+    m.xyz.assign(xyz, xyz + 3 * (size_t)n_nodes);
+    m.conn.assign(conn, conn + 8 * (size_t)n_elem);
+    m.dof.assign(dof, dof + 3 * (size_t)n_nodes);
+    for (size_t i = 0; i < m.conn.size(); i++)
+        MSA_CHECK(m.conn[i] >= 0 && m.conn[i] < n_nodes, MSA_ERR_ARG,
+                  "mesh: conn[%zu]=%d out of range", i, m.conn[i]);
+    for (int e = 0; e < n_elem; e++)
+        for (int a = 0; a < 8; a++)
+            for (int b = a + 1; b < 8; b++)
+                MSA_CHECK(m.conn[8 * e + a] != m.conn[8 * e + b], MSA_ERR_ARG,
+                          "mesh: element %d repeats node %d", e, m.conn[8 * e + a]);
+
+    // blocks and the reference -> internal permutation
+    m.node_free.assign(n_nodes, 0);
+    m.node_block.assign(n_nodes, -1);
+    m.block_node.clear();
+    m.ref2int.assign(n_free, -1);
+    for (int n = 0; n < n_nodes; n++) {
+        uint8_t mask = 0;
+        for (int d = 0; d < 3; d++) {
+            int g = m.dof[3 * n + d];
+            MSA_CHECK(g >= -1 && g < n_free, MSA_ERR_ARG, "mesh: dof[%d]=%d out of range",
+                      3 * n + d, g);
+            if (g >= 0) mask |= (uint8_t)(1u << d);
+        }
+        m.node_free[n] = mask;
+        if (mask) {
+            m.node_block[n] = (int)m.block_node.size();
+            m.block_node.push_back(n);
+        }
+    }
+    m.nb = (int)m.block_node.size();
+    for (int n = 0; n < n_nodes; n++)
+        for (int d = 0; d < 3; d++) {
+            int g = m.dof[3 * n + d];
+            if (g < 0) continue;
+            MSA_CHECK(m.ref2int[g] < 0, MSA_ERR_ARG, "mesh: free DOF %d assigned twice", g);
+            m.ref2int[g] = 3 * m.node_block[n] + d;
+        }
+    for (int g = 0; g < n_free; g++)
+        MSA_CHECK(m.ref2int[g] >= 0, MSA_ERR_ARG, "mesh: free DOF %d not assigned", g);
+
+    // node -> incident elements
+    std::vector<int> inc_ptr(n_nodes + 1, 0), inc;
+    for (int e = 0; e < n_elem; e++)
+        for (int a = 0; a < 8; a++) inc_ptr[m.conn[8 * e + a] + 1]++;
+    for (int n = 0; n < n_nodes; n++) inc_ptr[n + 1] += inc_ptr[n];
+    inc.resize(inc_ptr[n_nodes]);
+    {
+        std::vector<int> fill(inc_ptr.begin(), inc_ptr.end() - 1);
+        for (int e = 0; e < n_elem; e++)
+            for (int a = 0; a < 8; a++) inc[fill[m.conn[8 * e + a]]++] = e;
+    }
+
+    // block adjacency
+    m.brow_ptr.assign(m.nb + 1, 0);
+    m.bcol.clear();
+    m.diag_pos.assign(m.nb, -1);
+    m.max_row = 0;
+    m.nnz_scalar = 0;
+    std::vector<int> tmp;
+    for (int br = 0; br < m.nb; br++) {
+        int n = m.block_node[br];
+        tmp.clear();
+        for (int p = inc_ptr[n]; p < inc_ptr[n + 1]; p++)
+            for (int a = 0; a < 8; a++) {
+                int bc = m.node_block[m.conn[8 * inc[p] + a]];
+                if (bc >= 0) tmp.push_back(bc);
+            }
+        std::sort(tmp.begin(), tmp.end());
+        tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+        for (size_t j = 0; j < tmp.size(); j++) {
+            if (tmp[j] == br) m.diag_pos[br] = (int)j;
+            m.nnz_scalar += (long long)__builtin_popcount(m.node_free[n]) *
+                            __builtin_popcount(m.node_free[m.block_node[tmp[j]]]);
+        }
+        m.bcol.insert(m.bcol.end(), tmp.begin(), tmp.end());
+        m.brow_ptr[br + 1] = (int)m.bcol.size();
+        m.max_row = std::max<int>(m.max_row, (int)tmp.size());
+        MSA_CHECK(m.diag_pos[br] >= 0, MSA_ERR_STATE, "mesh: block %d lacks a diagonal", br);
+    }
+    MSA_CHECK(m.max_row < 255, MSA_ERR_ARG, "mesh: a node couples to %d nodes (max 254)",
+              m.max_row);
+
+    // element pair -> slot
+    m.emap.assign((size_t)n_elem * 64, 255);
+    for (int e = 0; e < n_elem; e++)
+        for (int a = 0; a < 8; a++) {
+            int br = m.node_block[m.conn[8 * e + a]];
+            if (br < 0) continue;
+            const int *row = m.bcol.data() + m.brow_ptr[br];
+            int len = m.brow_ptr[br + 1] - m.brow_ptr[br];
+            for (int b = 0; b < 8; b++) {
+                int bc = m.node_block[m.conn[8 * e + b]];
+                if (bc < 0) continue;
+                int j = (int)(std::lower_bound(row, row + len, bc) - row);
+                m.emap[(size_t)e * 64 + a * 8 + b] = (uint8_t)j;
+            }
+        }
+
+    // greedy colouring in element order: smallest colour unused at all 8 nodes
+    std::vector<uint64_t> used(n_nodes, 0);
+    std::vector<int> colour(n_elem);
+    m.ncolours = 0;
+    for (int e = 0; e < n_elem; e++) {
+        uint64_t u = 0;
+        for (int a = 0; a < 8; a++) u |= used[m.conn[8 * e + a]];
+        MSA_CHECK(~u != 0, MSA_ERR_ARG, "mesh: more than 64 element colours needed");
+        int c = __builtin_ctzll(~u);
+        MSA_CHECK(c < kMaxColours, MSA_ERR_ARG, "mesh: more than %d colours", kMaxColours);
+        colour[e] = c;
+        for (int a = 0; a < 8; a++) used[m.conn[8 * e + a]] |= (1ull << c);
+        m.ncolours = std::max(m.ncolours, c + 1);
+    }
+    m.colour_ptr.assign(m.ncolours + 1, 0);
+    for (int e = 0; e < n_elem; e++) m.colour_ptr[colour[e] + 1]++;
+    for (int c = 0; c < m.ncolours; c++) m.colour_ptr[c + 1] += m.colour_ptr[c];
+    m.colour_elems.resize(n_elem);
+    {
+        std::vector<int> fill(m.colour_ptr.begin(), m.colour_ptr.end() - 1);
+        for (int e = 0; e < n_elem; e++) m.colour_elems[fill[colour[e]]++] = e;
+    }
+    return MSA_OK;
+}
+
+}  // namespace msa

@@ -1,0 +1,175 @@
+// Internal declarations shared by the msolve_amd sources (not part of the
+// C ABI).  Device data layout of a batch class:
+//
+//   block rows      node blocks with at least one free DOF, in node order;
+//                   internal vector index = 3*block + component
+//   sliced ELL      slices of R = 64/T block rows; slice s owns slots
+//                   [slice_ptr[s], slice_ptr[s+1]) per row (width = longest
+//                   row of the slice); cidx[slot*R + rr] = block column or -1
+//   values          val[tile][((slot*9 + e)*R + rr)*T + s], e = 3p+q of the
+//                   3x3 block, s = sample lane within the tile
+//   vectors         v[tile][(block*3 + d)*T + s]
+//
+// One wavefront covers R block rows x T samples, so a value load is always
+// 64 consecutive words and a gathered vector load is T consecutive words.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/msolve_amd.h"
+
+namespace msa {
+
+constexpr int kMaxColours = 64;
+constexpr int kWaves = 4;  // waves per workgroup of the PCG kernels
+
+void set_error(const char *fmt, ...);
+
+#define MSA_HIP(call)                                                         \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess) {                                               \
+            ::msa::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,       \
+                             hipGetErrorString(e_));                          \
+            return MSA_ERR_HIP;                                               \
+        }                                                                     \
+    } while (0)
+
+#define MSA_CHECK(cond, code, ...)                                            \
+    do {                                                                      \
+        if (!(cond)) {                                                        \
+            ::msa::set_error(__VA_ARGS__);                                    \
+            return (code);                                                    \
+        }                                                                     \
+    } while (0)
+
+// Per-sample solver state (device).  One per sample lane of every tile.
+struct SampleState {
+    double rz, pq, alpha, beta;  // PCG scalars
+    double rr, bnorm, relres;    // |r|^2, |b|, |r|/|b|
+    double r64norm, scale;       // mixed: |f - K u|, inner right-hand-side scale
+    int iters, status, active;   // inner PCG
+    int outer, total_iters, ir_active, negdet;
+};
+
+// Host-side mesh analysis (mesh.cpp).
+struct MeshHost {
+    int n_nodes = 0, n_elem = 0, n_free = 0;
+    int nb = 0;                          // free node blocks
+    std::vector<double> xyz;             // [n_nodes*3]
+    std::vector<int> conn;               // [n_elem*8]
+    std::vector<int> dof;                // [n_nodes*3] reference free-DOF ids
+    std::vector<int> node_block;         // [n_nodes] block or -1
+    std::vector<uint8_t> node_free;      // [n_nodes] bit d = DOF d free
+    std::vector<int> block_node;         // [nb]
+    std::vector<int> ref2int;            // [n_free] -> 3*block + d
+    std::vector<int> brow_ptr, bcol;     // block adjacency, sorted
+    std::vector<int> diag_pos;           // [nb] position of the diagonal
+    std::vector<uint8_t> emap;           // [n_elem*64] slot of (a,b) or 255
+    int max_row = 0;
+    int ncolours = 0;
+    std::vector<int> colour_ptr;         // [ncolours+1]
+    std::vector<int> colour_elems;       // elements sorted by colour
+    long long nnz_scalar = 0;
+};
+
+int build_mesh_host(MeshHost &m, int n_nodes, const double *xyz, int n_elem,
+                    const int *conn, const int *dof, int n_free);
+
+// Device view of one class of a batch (copied to device memory).
+struct ClassDev {
+    int nb, nslices, ntiles, nsamp;
+    int n_elem, nchunks, chunk, M;
+    long long tile_vals, tile_vec;
+    double fnorm, c0;
+    const int *slice_ptr, *cidx, *diag_pos;
+    const uint8_t *block_free;
+    const double *xyz;
+    const int *conn, *node_block;
+    const uint8_t *node_free, *emap;
+    const int *colour_elems;
+    void *val;
+    void *x, *r, *z, *p0, *p1, *q, *dinv;
+    double *u64, *r64;
+    const double *f;      // [nb*3] internal numbering, shared by samples
+    double *E;            // [ntiles][n_elem][T]
+    double *phi;          // [n_elem][M]
+    double *xi;           // [ntiles*T][M]
+    SampleState *st;      // [ntiles*T]
+    double *partial;      // [ntiles][nchunks][3][T]
+    unsigned *counter;    // [ntiles][8]
+    int *negdet;          // [1]
+    int colour_ptr[kMaxColours + 1];
+};
+
+// Elementwise launch over one colour of every class: blocks [prefix[c],
+// prefix[c+1]) belong to class c.
+struct ColourLaunch {
+    int ncls;
+    int colour;
+    int prefix[33];
+};
+
+enum KernelKind { KK_SPMV = 0, KK_UPDATE = 1, KK_ASSEMBLE = 2, KK_RES64 = 3, KK_OTHER = 4 };
+
+// Launch wrappers (kernels.hip).  T is the lane tile of the batch.
+hipError_t launch_fill_f64(hipStream_t s, double *p, size_t n, double v);
+hipError_t launch_fill_f32(hipStream_t s, float *p, size_t n, float v);
+hipError_t launch_fill_u32(hipStream_t s, unsigned *p, size_t n, unsigned v);
+hipError_t launch_ke_packed(hipStream_t s, int n_elem, const double *xyz8, const double *E,
+                            double nu, double *ke300, int *status);
+hipError_t launch_gauss_matrices(hipStream_t s, const double *xyz, double *out);
+hipError_t launch_h8k(hipStream_t s, const double *E, const double *B, const double *w, double *K);
+hipError_t launch_assemble(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
+                           const ColourLaunch &cl, int nblocks, double nu);
+hipError_t launch_residual64(hipStream_t s, int T, const ClassDev *d_cls, const ColourLaunch &cl,
+                             int nblocks, double nu);
+hipError_t launch_dinv(hipStream_t s, bool mixed, int T, const ClassDev *d_cls, const int4 *items,
+                       int nitems);
+hipError_t launch_pcg_init(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
+                           const int4 *items, int nitems, int mode, int *active_count);
+hipError_t launch_pcg_a(hipStream_t s, bool mixed, int T, const ClassDev *d_cls, const int4 *items,
+                        int nitems, int pbuf, int *active_count);
+hipError_t launch_pcg_b(hipStream_t s, bool mixed, int T, const ClassDev *d_cls, const int4 *items,
+                        int nitems, int pbuf, double tol, int max_it, int *active_count);
+hipError_t launch_ir_begin(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,
+                           int nitems, int *ir_count);
+hipError_t launch_ir_update(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,
+                            int nitems);
+hipError_t launch_ir_norm(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,
+                          int nitems, double tol, int max_outer, int *ir_count);
+hipError_t launch_spmv(hipStream_t s, bool mixed, int T, const ClassDev *d_cls, const int4 *items,
+                       int nitems, const double *x, double *y);
+hipError_t launch_kl_basis(hipStream_t s, const ClassDev *d_cls, int n_elem, int M, int kmax,
+                           const double *w1d, const double *a1d, const int *evpPos,
+                           const double *eigs, const double *domain, double sigma);
+hipError_t launch_kl_eval(hipStream_t s, int T, const ClassDev *d_cls, int ntiles, int n_elem,
+                          int M);
+hipError_t launch_extract_sample(hipStream_t s, bool mixed, int T, const void *src, size_t n_rows,
+                                 int lane, double *dst);
+hipError_t launch_insert_sample(hipStream_t s, int T, const double *src, size_t n_rows, int lane,
+                                double *dst);
+hipError_t launch_extract_values(hipStream_t s, bool mixed, int T, const void *val, size_t nslots9R,
+                                 int lane, double *dst);
+
+}  // namespace msa
+
+// Opaque handle types of the C ABI.
+struct msa_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int *h_pinned = nullptr;  // small pinned host scratch for polling
+};
+
+struct msa_mesh {
+    msa_ctx *ctx = nullptr;
+    msa::MeshHost h;
+    double *d_xyz = nullptr;
+    int *d_conn = nullptr, *d_node_block = nullptr, *d_colour_elems = nullptr;
+    uint8_t *d_node_free = nullptr, *d_emap = nullptr;
+};

@@ -1,0 +1,249 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the
+same seeded inputs.  Bars: element/assembly values 1e-13 relative (FP64),
+FP32 assembly 2e-6; displacements 1e-10 relative for FP64 vs the skyline
+direct solve, 1e-8 for mixed precision."""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+import msolve_amd as msa
+from msolve_amd import _lib as L
+from msolve_amd.stochastic import kl_table, lognormal_sigma, standard_normal
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "ke_golden.json")
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+
+
+def direct_solution(model, E, nu=0.3):
+    """Oracle: Ke -> skyline -> LDL^T solve (SolverSkyline semantics)."""
+    ke, bad = O.ke_mesh(model.conn, model.xyz, E, nu)
+    assert bad == 0
+    ri, data = O.skyline_assemble(model.conn, model.dof, model.n_free, ke)
+    fac, rc = O.skyline_factorize(ri, data, 1e-15)
+    assert rc == 0
+    return O.skyline_solve(ri, fac, model.f)
+
+
+def lognormal_E(rng, n, mean=2.1e5, cov=0.3):
+    s = lognormal_sigma(cov)
+    return mean * np.exp(-0.5 * s * s + s * rng.standard_normal(n))
+
+
+# ------------------------------------------------------------------ elements
+def test_ke_packed_matches_golden_and_oracle(ctx):
+    cases = json.load(open(GOLDEN))["cases"]
+    for c in cases:
+        x = np.array(c["xyz"])
+        ke, st = msa.ke_packed(ctx, x[None], [c["E"]], c["nu"])
+        assert st[0] == msa.MSA_OK
+        gold = np.array(c["ke"])
+        ref = O.h8_k(np.tile(O.elastic_D(c["E"], c["nu"]), (8, 1, 1)),
+                     O.gauss_matrices(x)["B"], O.gauss_matrices(x)["w"])
+        assert np.abs(ke[0] - gold).max() / np.abs(gold).max() < 1e-13, c["name"]
+        assert np.abs(ke[0] - ref).max() / np.abs(ref).max() < 1e-13, c["name"]
+
+
+def test_fortran_dropins_match_oracle(ctx):
+    x = np.array(json.load(open(GOLDEN))["cases"][3]["xyz"])
+    it = C.c_int(2)
+    w, S, DS = np.zeros(8), np.zeros(64), np.zeros(192)
+    J, dJ, B = np.zeros(72), np.zeros(8), np.zeros(1152)
+    p = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    L.lib().CALCH8GAUSSMATRICES(C.byref(it), p(np.ascontiguousarray(x.ravel())), p(w), p(S),
+                                p(DS), p(J), p(dJ), p(B))
+    ref = O.gauss_matrices(x)
+    assert np.allclose(w, ref["w"], rtol=1e-14, atol=0)
+    assert np.allclose(S, ref["S"].ravel(), rtol=1e-14, atol=1e-16)
+    assert np.allclose(DS, ref["DS"].ravel(), rtol=1e-14, atol=1e-16)
+    assert np.allclose(J, ref["J"].ravel(), rtol=1e-13, atol=1e-15)
+    assert np.allclose(dJ, ref["detJ"], rtol=1e-13)
+    assert np.abs(B - ref["B"].ravel()).max() <= 1e-13 * np.abs(ref["B"]).max()
+    E = np.tile(O.elastic_D(2.1e5, 0.3), (8, 1, 1)).ravel()
+    K = np.zeros(300)
+    L.lib().CALCH8K(C.byref(it), p(E), p(np.ascontiguousarray(ref["B"].ravel())),
+                    p(np.ascontiguousarray(ref["w"])), p(K))
+    Kref = O.h8_k(E, ref["B"], ref["w"])
+    assert np.abs(K - Kref).max() <= 1e-14 * np.abs(Kref).max()
+    bad = C.c_int(3)
+    L.lib().CALCH8K(C.byref(bad), p(E), p(np.ascontiguousarray(ref["B"].ravel())),
+                    p(np.ascontiguousarray(ref["w"])), p(K))
+    assert np.isnan(K).all()
+
+
+def test_neg_detj_status(ctx):
+    x = np.array(json.load(open(GOLDEN))["cases"][0]["xyz"])[[4, 5, 6, 7, 0, 1, 2, 3]]
+    _, st = msa.ke_packed(ctx, x[None], [1.0], 0.3)
+    assert st[0] == msa.MSA_NEG_DETJ
+
+
+# ------------------------------------------------------------------ assembly
+def partial_bc_model(n=3):
+    """Box with x = 0 fixed in X only, y = 0 in Y, z = 0 in Z (symmetry
+    planes): partially constrained nodes exercise the padded blocks."""
+    m = msa.cantilever(n)
+    xyz = m.xyz
+    cons = np.zeros((xyz.shape[0], 3), np.uint8)
+    cons[xyz[:, 0] == 0, 0] = 1
+    cons[xyz[:, 1] == 0, 1] = 1
+    cons[xyz[:, 2] == 0, 2] = 1
+    dof, n_free = msa.enumerate_dofs(cons)
+    f = np.zeros(n_free)
+    top = np.nonzero(xyz[:, 0] == xyz[:, 0].max())[0]
+    for nd in top:
+        f[dof[3 * nd]] = 1.0 / top.size
+    m.constrained, m.dof, m.n_free, m.f = cons.ravel(), dof, n_free, f
+    return m
+
+
+@pytest.mark.parametrize("precision,tile", [(L.MSA_FP64, 1), (L.MSA_FP64, 4), (L.MSA_FP64, 64),
+                                            (L.MSA_MIXED, 64), (L.MSA_MIXED, 16)])
+@pytest.mark.parametrize("bc", ["cantilever", "partial"])
+def test_assembly_matches_oracle(ctx, precision, tile, bc):
+    print(f"\n[case] assembly precision={precision} tile={tile} bc={bc}", flush=True)
+    model = msa.cantilever(4, 3, 2) if bc == "cantilever" else partial_bc_model(3)
+    rng = np.random.default_rng(7)
+    ns = 3
+    E = lognormal_E(rng, ns * model.n_elem).reshape(ns, -1)
+    mesh = msa.Mesh(ctx, model.xyz, model.conn, model.dof, model.n_free)
+    b = msa.Batch(ctx, [mesh], [ns], precision, tile)
+    b.set_modulus(0, E)
+    b.set_rhs(0, model.f)
+    b.assemble(0.3)
+    bar = 1e-13 if precision == L.MSA_FP64 else 2e-6
+    for s in range(ns):
+        ke, _ = O.ke_mesh(model.conn, model.xyz, E[s], 0.3)
+        rp, col, val = O.csr_assemble(model.conn, model.dof, model.n_free, ke)
+        grp, gcol, gval = b.matrix_csr(0, s)
+        assert np.array_equal(grp, rp) and np.array_equal(gcol, col)
+        assert np.abs(gval - val).max() <= bar * np.abs(val).max()
+        x = rng.standard_normal(model.n_free)
+        y = b.spmv(0, s, x)
+        yref = O.csr_multiply(rp, col, val, x)
+        assert rel(y, yref) < (1e-13 if precision == L.MSA_FP64 else 2e-6)
+    b.close()
+    mesh.close()
+
+
+# ------------------------------------------------------------------ solves
+def test_plain_fp64_pcg_vs_skyline_direct(ctx):
+    """Configs[0]: 10^3 cantilever, E = 2.1e5, nu = 0.3."""
+    print("\n[case] plain fp64 10^3", flush=True)
+    model = msa.cantilever(10)
+    E = np.full(model.n_elem, 2.1e5)
+    solver = msa.SolverPCG(ctx, model, E, 0.3)
+    solver.Initialize()
+    u = solver.Solve(20000, 1e-14)
+    assert solver.Status == msa.MSA_OK
+    om = O.Cantilever(10)
+    assert np.array_equal(om.dof, model.dof) and np.allclose(om.f, model.f, rtol=0, atol=0)
+    ud = direct_solution(om, E)
+    assert rel(u, ud) < 1e-10
+    # stopped at the reference's test |r|/|f| < 1e-10: same iteration count as the
+    # oracle PCG (Jacobi) within 1, solutions within the stopping accuracy
+    u10 = solver.Solve(20000, 1e-10)
+    ke, _ = O.ke_mesh(om.conn, om.xyz, E, 0.3)
+    rp, col, val = O.csr_assemble(om.conn, om.dof, om.n, ke)
+    r = O.pcg_csr(rp, col, val, om.f, O.PRECOND_JACOBI, tol=1e-10)
+    assert abs(solver.Iterations - r["iters"]) <= 1
+    assert rel(u10, r["x"]) < 1e-9
+    assert rel(u10, ud) < 1e-8
+
+
+@pytest.mark.parametrize("graph_iters", [0, 8])
+def test_batched_fp64_two_classes_kl(ctx, graph_iters):
+    """Uneven sizes + KL lognormal modulus, tile 4 (R = 16 rows per wave)."""
+    print(f"\n[case] batched fp64 kl graph={graph_iters}", flush=True)
+    models = [msa.cantilever(4), msa.cantilever(6, 5, 4)]
+    ns = [5, 3]
+    table = kl_table([1.0, 1.0, 1.0], 0.25, 12)
+    xi = standard_normal(123, sum(ns), 12)
+    meshes = [msa.Mesh(ctx, m.xyz, m.conn, m.dof, m.n_free) for m in models]
+    b = msa.Batch(ctx, meshes, ns, L.MSA_FP64, 4)
+    off = 0
+    for c, m in enumerate(models):
+        b.set_kl(c, table, m.domain, 2.1e5, 0.2)
+        b.set_xi(c, xi[off:off + ns[c]])
+        b.set_rhs(c, m.f)
+        off += ns[c]
+    b.run(0.3, tol=1e-13, max_it=5000, graph_iters=graph_iters)
+    off = 0
+    for c, m in enumerate(models):
+        st = b.stats(c)
+        assert (st["status"] == msa.MSA_OK).all(), st
+        for s in range(ns[c]):
+            Eg = b.modulus(c, s)
+            Eo, bad = O.kl_field(m.conn, m.xyz, table, xi[off + s], m.domain, 2.1e5, 0.2)
+            assert bad == 0 and rel(Eg, Eo) < 1e-13
+            ud = direct_solution(m, Eo)
+            assert rel(b.solution(c, s), ud) < 1e-10
+        off += ns[c]
+
+
+def test_graph_equals_eager(ctx):
+    print("\n[case] graph vs eager", flush=True)
+    model = msa.cantilever(5)
+    rng = np.random.default_rng(3)
+    E = lognormal_E(rng, 4 * model.n_elem).reshape(4, -1)
+    mesh = msa.Mesh(ctx, model.xyz, model.conn, model.dof, model.n_free)
+    out = []
+    for g in (0, 6):
+        b = msa.Batch(ctx, [mesh], [4], L.MSA_FP64, 64)
+        b.set_modulus(0, E)
+        b.set_rhs(0, model.f)
+        b.assemble()
+        b.solve(1e-10, 2000, graph_iters=g)
+        out.append(([b.solution(0, s) for s in range(4)], b.stats(0)["iters"].copy()))
+        b.close()
+    for s in range(4):
+        assert np.array_equal(out[0][0][s], out[1][0][s])
+    assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_mixed_precision_refinement(ctx):
+    print("\n[case] mixed IR 8^3", flush=True)
+    model = msa.cantilever(8)
+    rng = np.random.default_rng(11)
+    E = lognormal_E(rng, 3 * model.n_elem, cov=0.3).reshape(3, -1)
+    mesh = msa.Mesh(ctx, model.xyz, model.conn, model.dof, model.n_free)
+    b = msa.Batch(ctx, [mesh], [3], L.MSA_MIXED, 64)
+    b.set_modulus(0, E)
+    b.set_rhs(0, model.f)
+    b.assemble()
+    b.solve(tol=1e-10, max_it=5000, inner_tol=1e-4, max_outer=30, graph_iters=8)
+    st = b.stats(0)
+    assert (st["status"] == msa.MSA_OK).all(), st
+    assert (st["outer"] >= 2).all()
+    for s in range(3):
+        ud = direct_solution(model, E[s])
+        assert rel(b.solution(0, s), ud) < 1e-8
+        # the reported residual is the FP64 one
+        ke, _ = O.ke_mesh(model.conn, model.xyz, E[s], 0.3)
+        rp, col, val = O.csr_assemble(model.conn, model.dof, model.n_free, ke)
+        r = model.f - O.csr_multiply(rp, col, val, b.solution(0, s))
+        assert abs(np.linalg.norm(r) / np.linalg.norm(model.f) - st["relres"][s]) < 1e-12
+
+
+def test_status_codes(ctx):
+    print("\n[case] status codes", flush=True)
+    model = msa.cantilever(3)
+    E = np.full((2, model.n_elem), 2.1e5)
+    E[1, 5] = np.nan
+    mesh = msa.Mesh(ctx, model.xyz, model.conn, model.dof, model.n_free)
+    b = msa.Batch(ctx, [mesh], [2], L.MSA_FP64, 1)
+    b.set_modulus(0, E)
+    b.set_rhs(0, model.f)
+    b.assemble()
+    b.solve(1e-12, 3)
+    st = b.stats(0)
+    assert st["status"][0] == msa.MSA_NOT_CONVERGED and st["iters"][0] == 3
+    assert st["status"][1] == msa.MSA_NAN_INPUT
+    with pytest.raises(msa.MsaError):
+        b.solve(1e-12, 10, graph_iters=3)   # odd graph length is rejected
