@@ -1,0 +1,270 @@
+#!/usr/bin/env python3
+"""Monte Carlo solves/sec of the Hexa8 cantilever on one MI355X.
+
+Workload (one "step"): the batched mixed-precision Monte Carlo job of
+1024 samples whose cube-mesh sizes are drawn uniformly from
+n in {16, 24, 32, 40, 48} elements per side; per sample a lognormal
+Karhunen-Loeve modulus field (40 terms, exponential covariance, correlation
+length 0.25, mean 2.1e5, COV 0.3) evaluated at element centroids, FP32
+element integration + coloured assembly, FP32 Jacobi-PCG inner solves under
+FP64 iterative refinement (FP64 residual f - K u).  Inputs (meshes, KL
+tables, KL coefficients, loads) are resident on the device before the timed
+region; each step re-evaluates the modulus, re-assembles and re-solves every
+sample.
+
+Reported beside the headline: the same job in plain FP64 (batched), the
+single-sample FP64 GPU PCG, each sample's error against a tightly converged
+FP64 solve, the SpMV kernel's roofline (HIP events) and a CPU baseline (the
+oracle's CSR/Jacobi restatement of the reference loop, one sample per mesh
+size, one core).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MC solves/sec at 1e-8 rel. err, batched mixed vs plain FP64, 1 MI355X"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=2)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--samples", type=int, default=1024)
+    p.add_argument("--sizes", type=str, default="16,24,32,40,48")
+    p.add_argument("--cov", type=float, default=0.3)
+    p.add_argument("--terms", type=int, default=40)
+    p.add_argument("--corr", type=float, default=0.25)
+    p.add_argument("--tol", type=float, default=1e-9, help="FP64 residual tolerance |f-Ku|/|f|")
+    p.add_argument("--inner-tol", type=float, default=1e-4)
+    p.add_argument("--oracle-tol", type=float, default=1e-13)
+    p.add_argument("--max-it", type=int, default=20000)
+    p.add_argument("--max-outer", type=int, default=30)
+    p.add_argument("--graph-iters", type=int, default=16)
+    p.add_argument("--tile", type=int, default=64)
+    p.add_argument("--seed", type=int, default=20261202)
+    p.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison runs")
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--no-single", action="store_true", help="skip the single-sample FP64 runs")
+    return p.parse_args()
+
+
+def workload(args):
+    sizes = [int(s) for s in args.sizes.split(",")]
+    rng = np.random.Generator(np.random.Philox(args.seed))
+    draw = rng.integers(0, len(sizes), args.samples)
+    counts = [int((draw == k).sum()) for k in range(len(sizes))]
+    keep = [k for k in range(len(sizes)) if counts[k] > 0]
+    return [sizes[k] for k in keep], [counts[k] for k in keep]
+
+
+def make_batch(msa, ctx, meshes, models, counts, precision, tile, table, xi, args):
+    b = msa.Batch(ctx, meshes, counts, precision, tile)
+    off = 0
+    for c, m in enumerate(models):
+        b.set_kl(c, table, m.domain, 2.1e5, args.cov)
+        b.set_xi(c, xi[off:off + counts[c]])
+        b.set_rhs(c, m.f)
+        off += counts[c]
+    ctx.synchronize()
+    return b
+
+
+def kernel_a_bytes(b, models, counts, rs, launches):
+    """Algorithmic bytes moved by all SpMV+dot launches of one solve:
+    per active sample and iteration the matrix values and the four vectors
+    (z, p_old read; p_new, q written); the shared block-column indices and
+    slice pointers once per launch per class with active samples."""
+    total = 0.0
+    for c, m in enumerate(models):
+        st = b.stats(c)
+        mesh = b.meshes[c]
+        nnz, n = mesh.info["nnz"], 3 * mesh.info["blocks"]
+        its = st["iters"].astype(np.float64)
+        total += its.sum() * (rs * nnz + 4 * rs * n)
+        idx = 4.0 * mesh.info["block_nnz"] + 4.0 * (mesh.info["blocks"] + 1)
+        total += float(its.max()) * idx
+    return total
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of the "
+                                   "Monte Carlo samples is described in DESIGN.md",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import msolve_amd as msa
+    from msolve_amd.stochastic import kl_table, standard_normal
+
+    sizes, counts = workload(args)
+    N = sum(counts)
+    log(f"[bench] sizes={sizes} counts={counts} samples={N}")
+    ctx = msa.Context(0)
+    models = [msa.cantilever(n) for n in sizes]
+    meshes = [msa.Mesh(ctx, m.xyz, m.conn, m.dof, m.n_free) for m in models]
+    table = kl_table([1.0, 1.0, 1.0], args.corr, args.terms)
+    xi = standard_normal(args.seed, N, args.terms)
+    run_kw = dict(nu=0.3, tol=args.tol, max_it=args.max_it, inner_tol=args.inner_tol,
+                  max_outer=args.max_outer, graph_iters=args.graph_iters)
+
+    # ------------------------------------------------ headline: mixed, batched
+    bm = make_batch(msa, ctx, meshes, models, counts, msa.MSA_MIXED, args.tile, table, xi, args)
+    log(f"[bench] mixed batch: {bm.device_bytes / 1e9:.1f} GB on device")
+    for w in range(args.warmup):
+        t = time.perf_counter()
+        bm.run(**run_kw)
+        ctx.synchronize()
+        log(f"[bench] warmup {w}: {time.perf_counter() - t:.2f} s")
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        bm.run(**run_kw)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    value = N * args.steps / elapsed
+    log(f"[bench] mixed: {args.steps} steps in {elapsed:.2f} s -> {value:.1f} solves/s")
+    stats = [bm.stats(c) for c in range(len(models))]
+    all_ok = all((s["status"] == msa.MSA_OK).all() for s in stats)
+
+    # roofline of the dominant kernel: one more step with HIP events around
+    # every launch on the context stream
+    bm.set_kernel_timing(True)
+    bm.run(**run_kw)
+    kt = bm.kernel_timing()
+    bm.set_kernel_timing(False)
+    launches = kt["spmv_dot"]["launches"]
+    a_ms = kt["spmv_dot"]["ms"]
+    bytes_total = kernel_a_bytes(bm, models, counts, 4, launches)
+    avg_bytes = bytes_total / max(launches, 1)
+    avg_s = a_ms / 1e3 / max(launches, 1)
+    achieved = avg_bytes / avg_s / 1e9 if avg_s > 0 else 0.0
+    log(f"[bench] kernel timing: {json.dumps(kt)}")
+    u_mixed = [[bm.solution(c, s) for s in range(counts[c])] for c in range(len(models))]
+    iters_mixed = [s["iters"].tolist() for s in stats]
+    outer_mixed = [s["outer"].tolist() for s in stats]
+    bm.close()
+
+    extra = {}
+    # ------------------------------------------------ FP64: batched + oracle
+    max_err = None
+    if not args.no_fp64:
+        b64 = make_batch(msa, ctx, meshes, models, counts, msa.MSA_FP64, args.tile, table, xi, args)
+        ctx.synchronize()
+        t = time.perf_counter()
+        b64.run(**run_kw)
+        ctx.synchronize()
+        t64 = time.perf_counter() - t
+        extra["fp64_batched_solves_per_s"] = N / t64
+        log(f"[bench] fp64 batched: {t64:.2f} s -> {N / t64:.1f} solves/s")
+        kw = dict(run_kw)
+        kw["tol"] = args.oracle_tol
+        b64.run(**kw)
+        ctx.synchronize()
+        errs = []
+        for c in range(len(models)):
+            st = b64.stats(c)
+            assert (st["status"] == msa.MSA_OK).all(), "FP64 oracle solve did not converge"
+            for s in range(counts[c]):
+                ur = b64.solution(c, s)
+                errs.append(np.linalg.norm(u_mixed[c][s] - ur) / np.linalg.norm(ur))
+        max_err = float(np.max(errs))
+        extra["rel_err_vs_fp64"] = dict(max=max_err, median=float(np.median(errs)),
+                                        oracle_tol=args.oracle_tol)
+        log(f"[bench] mixed vs FP64 oracle: max rel err {max_err:.3e}")
+        b64.close()
+
+    # ------------------------------------------------ single-sample FP64 GPU PCG
+    if not args.no_single:
+        per = []
+        off = 0
+        for c, m in enumerate(models):
+            b1 = make_batch(msa, ctx, [meshes[c]], [m], [1], msa.MSA_FP64, 1, table,
+                            xi[off:off + 1], args)
+            b1.run(**run_kw)      # warm-up (graph capture)
+            ctx.synchronize()
+            t = time.perf_counter()
+            b1.run(**run_kw)
+            ctx.synchronize()
+            per.append(time.perf_counter() - t)
+            b1.close()
+            off += counts[c]
+        t_single = sum(counts[c] * per[c] for c in range(len(models)))
+        extra["fp64_single_solves_per_s"] = N / t_single
+        extra["fp64_single_s_per_size"] = dict(zip(map(str, sizes), per))
+        log(f"[bench] fp64 single-sample: {N / t_single:.1f} solves/s ({per})")
+
+    # ------------------------------------------------ CPU baseline
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        otab = O.kl_table([1.0, 1.0, 1.0], args.corr, args.terms)
+        per, t_meas = [], 0.0
+        off = 0
+        for c, n in enumerate(sizes):
+            out = O.mc_reference(n, otab, xi[off:off + 1], cov=args.cov, tol=args.tol,
+                                 keep_u=False, storage="csr")
+            per.append(out["times"]["all"])
+            t_meas += out["times"]["all"]
+            off += counts[c]
+            log(f"[bench] cpu {n}^3: {out['times']['all']:.2f} s, {int(out['iters'][0])} its")
+        t_cpu = sum(counts[c] * per[c] for c in range(len(sizes)))
+        cpu = dict(value=N / t_cpu, unit="solves/s", cores=1, kind="port",
+                   sample=f"one sample per mesh size {sizes} (first of each class), oracle "
+                          f"CSR/Jacobi-PCG FP64 restatement of the reference per-sample loop "
+                          f"(KL modulus, Ke, assembly, PCG to {args.tol}), serial; rate = "
+                          f"{N} / sum(class count x measured time), {t_meas:.1f} s measured")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"batched mixed-precision MC: {N} samples, unit-cube Hexa8 cantilevers "
+                         f"n in {sizes} (counts {counts}), lognormal KL modulus (mean 2.1e5, "
+                         f"COV {args.cov}, {args.terms} terms, corr. length {args.corr}), FP32 "
+                         f"inner Jacobi-PCG (tol {args.inner_tol}) + FP64 refinement to "
+                         f"|f-Ku|/|f| < {args.tol}"),
+            "samples": N, "sizes": sizes, "counts": counts, "tile": args.tile,
+            "graph_iters": args.graph_iters, "all_converged": bool(all_ok),
+            "max_rel_err_vs_fp64": max_err,
+        },
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_pcg_a<float,64> (SpMV + p update + p.q)",
+                     "launches": launches, "avg_launch_ms": 1e3 * avg_s,
+                     "avg_bytes_per_launch": avg_bytes},
+        "cpu_baseline": cpu,
+        "extra": extra,
+        "iters_mean_per_size": {str(n): float(np.mean(iters_mixed[c])) for c, n in enumerate(sizes)},
+        "outer_mean_per_size": {str(n): float(np.mean(outer_mixed[c])) for c, n in enumerate(sizes)},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

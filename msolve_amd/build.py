@@ -58,5 +58,24 @@ def build(verbose: bool = True) -> str:
     return LIB
 
 
+def build_host_check(verbose: bool = True) -> str:
+    """Compile tests/cpp/host_api_check.cpp (the C++ host layer over the C ABI,
+    include/msolve_amd.hpp) with the host compiler against the library."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "host_api_check.cpp")
+    out_dir = os.path.join(root, "tests", "cpp", "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "host_api_check")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(root, "include"), src,
+           "-o", out, "-L" + LIB_DIR, "-lmsolve_amd", "-Wl,-rpath,$ORIGIN/../../../msolve_amd/_lib"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("host check build failed:\n" + res.stderr)
+    if verbose:
+        print("built", out)
+    return out
+
+
 if __name__ == "__main__":
     build()
+    build_host_check()

@@ -1006,3 +1006,69 @@ int orc_mc_reference(int nex, int ney, int nez, double nu, int M, int kmax,
     free(E); free(ke); free(ri); free(P); free(Pri); free(K); free(x);
     return rc_all;
 }
+
+/* The Monte Carlo loop of orc_mc_reference with the storage and
+ * preconditioner swapped for the CSR image and Jacobi (the deviation used
+ * where the skyline profile is infeasible, 32^3 and up): per sample KL
+ * modulus -> Ke (memoized w, B) -> CSR scatter -> Jacobi PCG to `tol`.
+ * The CSR pattern is built once.  Same outputs as orc_mc_reference. */
+int orc_mc_csr_jacobi(int nex, int ney, int nez, double nu, int M, int kmax,
+                      const double *w1d, const double *a1d, const int *evpPos,
+                      const double *eigs, double meanE, double cov,
+                      int n_samples, const double *xi, double tol, int max_it,
+                      double *u_out, int *iters, double *times)
+{
+    const int nX = nex + 1, nY = ney + 1, nZ = nez + 1;
+    const int n_nodes = nX * nY * nZ, n_elem = nex * ney * nez;
+    double *xyz = (double *)malloc(sizeof(double) * 3 * (size_t)n_nodes);
+    int *conn = (int *)malloc(sizeof(int) * 8 * (size_t)n_elem);
+    unsigned char *cons = (unsigned char *)malloc(3 * (size_t)n_nodes);
+    int *dof = (int *)malloc(sizeof(int) * 3 * (size_t)n_nodes);
+    orc_mesh_box(nex, ney, nez, 1.0 / nex, 1.0 / ney, 1.0 / nez, xyz, conn);
+    orc_cantilever_bc(nex, ney, nez, cons);
+    int n = orc_enumerate_dofs(n_nodes, cons, dof);
+    double *f = (double *)malloc(sizeof(double) * (size_t)n);
+    orc_cantilever_loads(nex, ney, nez, dof, -100.0, f);
+    const double domain[3] = {1.0, 1.0, 1.0};
+    int64_t *rp = (int64_t *)malloc(sizeof(int64_t) * ((size_t)n + 1));
+    int64_t nnz = orc_csr_count(n_elem, conn, dof, n, rp);
+    int *col = (int *)malloc(sizeof(int) * (size_t)nnz);
+    orc_csr_fill(n_elem, conn, dof, n, rp, col);
+    double *val = (double *)malloc(sizeof(double) * (size_t)nnz);
+    double *Wm = (double *)malloc(sizeof(double) * 8 * (size_t)n_elem);
+    double *Bm = (double *)malloc(sizeof(double) * 1152 * (size_t)n_elem);
+    double *E = (double *)malloc(sizeof(double) * (size_t)n_elem);
+    double *ke = (double *)malloc(sizeof(double) * 300 * (size_t)n_elem);
+    double *x = (double *)malloc(sizeof(double) * (size_t)n);
+    times[0] = times[1] = times[2] = times[3] = 0;
+    double t_all = now_s();
+    for (int e = 0; e < n_elem; e++) {
+        double x8[24], S[64], DS[192], J[72], dj[8];
+        for (int a = 0; a < 8; a++)
+            for (int j = 0; j < 3; j++) x8[3 * a + j] = xyz[3 * conn[8 * e + a] + j];
+        orc_h8_gauss_matrices(2, x8, Wm + 8 * (size_t)e, S, DS, J, dj, Bm + 1152 * (size_t)e);
+    }
+    for (int s = 0; s < n_samples; s++) {
+        double t0 = now_s();
+        orc_kl_field(n_elem, conn, xyz, M, kmax, w1d, a1d, evpPos, eigs, domain,
+                     xi + (size_t)s * M, meanE, cov, E);
+        for (int e = 0; e < n_elem; e++) {
+            double D[36], Eg[288];
+            orc_elastic_D(E[e], nu, D);
+            for (int m = 0; m < 8; m++) memcpy(Eg + 36 * m, D, sizeof(D));
+            orc_h8_k(2, Eg, Bm + 1152 * (size_t)e, Wm + 8 * (size_t)e, ke + 300 * (size_t)e);
+        }
+        orc_csr_assemble(n_elem, conn, dof, n, ke, rp, col, val);
+        double t1 = now_s();
+        times[0] += t1 - t0;
+        int it = 0;
+        orc_pcg_csr(n, rp, col, val, ORC_PRECOND_JACOBI, f, x, max_it, tol, &it, NULL);
+        times[2] += now_s() - t1;
+        if (iters) iters[s] = it;
+        if (u_out) memcpy(u_out + (size_t)s * n, x, sizeof(double) * (size_t)n);
+    }
+    times[3] = now_s() - t_all;
+    free(xyz); free(conn); free(cons); free(dof); free(f); free(rp); free(col); free(val);
+    free(Wm); free(Bm); free(E); free(ke); free(x);
+    return n;
+}

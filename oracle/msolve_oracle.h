@@ -94,6 +94,12 @@ int orc_mc_reference(int nex, int ney, int nez, double nu, int M, int kmax,
                      int n_samples, const double *xi, double tol, int max_it,
                      double *u_out, int *iters, double *times);
 
+int orc_mc_csr_jacobi(int nex, int ney, int nez, double nu, int M, int kmax,
+                      const double *w1d, const double *a1d, const int *evpPos,
+                      const double *eigs, double meanE, double cov,
+                      int n_samples, const double *xi, double tol, int max_it,
+                      double *u_out, int *iters, double *times);
+
 #ifdef __cplusplus
 }
 #endif

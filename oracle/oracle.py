@@ -72,6 +72,8 @@ def lib():
                              _f64p, D, D, _f64p]),
         "orc_mc_reference": (I, [I, I, I, D, I, I, _f64p, _f64p, _i32p, _f64p, D, D, I,
                                  _f64p, D, I, C.c_void_p, C.c_void_p, _f64p]),
+        "orc_mc_csr_jacobi": (I, [I, I, I, D, I, I, _f64p, _f64p, _i32p, _f64p, D, D, I,
+                                  _f64p, D, I, C.c_void_p, C.c_void_p, _f64p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -269,13 +271,13 @@ def kl_1d_modes(length, corr_len, kmax):
             m = k // 2
             lo, hi = m * math.pi / half, (m + 0.5) * math.pi / half
             g = lambda x: c * math.cos(x * half) - x * math.sin(x * half)
-            om = brentq(g, lo + 1e-14, hi - 1e-14, xtol=1e-15, rtol=4e-16, maxiter=500)
+            om = brentq(g, lo + 1e-14, hi - 1e-14, xtol=1e-15, rtol=1e-15, maxiter=500)
             nrm = half + math.sin(2 * om * half) / (2 * om)
         else:  # sine: w + c tan(w half) = 0 on ((m + 1/2) pi, (m + 1) pi)/half
             m = k // 2
             lo, hi = (m + 0.5) * math.pi / half, (m + 1.0) * math.pi / half
             g = lambda x: x * math.cos(x * half) + c * math.sin(x * half)
-            om = brentq(g, lo + 1e-14, hi - 1e-14, xtol=1e-15, rtol=4e-16, maxiter=500)
+            om = brentq(g, lo + 1e-14, hi - 1e-14, xtol=1e-15, rtol=1e-15, maxiter=500)
             nrm = half - math.sin(2 * om * half) / (2 * om)
         w.append(om)
         lam.append(2 * c / (om * om + c * c))
@@ -325,15 +327,17 @@ def kl_field(conn, xyz, table, xi, domain, meanE, cov):
 
 
 def mc_reference(nex, table, xi, meanE=2.1e5, cov=0.2, nu=0.3, tol=1e-10, max_it=20000,
-                 keep_u=True):
-    """Reference Monte Carlo loop (orc_mc_reference).  xi: [n_samples, M]."""
+                 keep_u=True, storage="skyline"):
+    """Reference Monte Carlo loop (orc_mc_reference; storage="csr" runs
+    orc_mc_csr_jacobi).  xi: [n_samples, M]."""
     xi = np.ascontiguousarray(xi, np.float64)
     ns = xi.shape[0]
     cl = Cantilever(nex)
     u = np.zeros((ns, cl.n)) if keep_u else None
     iters = np.zeros(ns, np.int32)
     times = np.zeros(4)
-    rc = lib().orc_mc_reference(nex, nex, nex, nu, table["M"], table["kmax"],
+    fn = lib().orc_mc_reference if storage == "skyline" else lib().orc_mc_csr_jacobi
+    rc = fn(nex, nex, nex, nu, table["M"], table["kmax"],
                                 np.ascontiguousarray(table["w1d"]).ravel(),
                                 np.ascontiguousarray(table["a1d"]).ravel(),
                                 np.ascontiguousarray(table["evpPos"], np.int32).ravel(),

@@ -1,0 +1,83 @@
+// Exercises the C++ host layer (include/msolve_amd.hpp) end to end on the
+// GPU: a plain SolverPCG solve and a two-class Monte Carlo batch, printing
+// the results as JSON for tests/test_gpu_host_cpp.py to compare with the
+// oracle.  Usage: host_api_check <kl_table.txt> (M, then rows of w a per
+// axis, evpPos, eigs) ; xi and moduli are derived deterministically.
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <vector>
+
+#include "../../include/msolve_amd.hpp"
+
+static void dump(const char *key, const std::vector<double> &v, bool last = false)
+{
+    std::printf("\"%s\": [", key);
+    for (size_t i = 0; i < v.size(); i++) std::printf(i ? ",%.17g" : "%.17g", v[i]);
+    std::printf("]%s\n", last ? "" : ",");
+}
+
+int main(int argc, char **argv)
+{
+    if (argc < 2) {
+        std::fprintf(stderr, "usage: host_api_check kl_table.txt\n");
+        return 2;
+    }
+    try {
+        std::ifstream in(argv[1]);
+        msolve::KLTable t;
+        in >> t.M >> t.kmax;
+        t.w1d.resize(3 * t.kmax);
+        t.a1d.resize(3 * t.kmax);
+        t.evpPos.resize(3 * t.M);
+        t.eigs.resize(t.M);
+        for (auto &v : t.w1d) in >> v;
+        for (auto &v : t.a1d) in >> v;
+        for (auto &v : t.evpPos) in >> v;
+        for (auto &v : t.eigs) in >> v;
+        std::vector<double> xi;
+        double v;
+        while (in >> v) xi.push_back(v);
+
+        msolve::Context ctx(0);
+        msolve::Model m6 = msolve::Hexa8MeshGenerator::Cantilever(6);
+        std::vector<double> E(m6.ElementCount(), 2.1e5);
+        msolve::SolverPCG solver(ctx, m6, E, 0.3);
+        solver.Initialize();
+        solver.Solve(20000, 1e-12);
+
+        msolve::Model m4 = msolve::Hexa8MeshGenerator::Cantilever(4);
+        msolve::MonteCarloAnalyzerWithStochasticMaterial mc(ctx, {&m4, &m6}, {3, 2}, t, xi, 2.1e5,
+                                                            0.3, 0.3, MSA_FP64);
+        mc.Initialize();
+        mc.Solve(1e-12, 20000, 1e-4, 30, 16);
+
+        msolve::ElementStructuralStiffnessProvider prov(ctx);
+        std::vector<double> E4(m4.ElementCount(), 1.0);
+        std::vector<double> ke = prov.Matrix(m4, E4, 0.3);
+
+        std::printf("{\n\"status\": %d, \"current_iteration\": %d, \"products\": %d,\n",
+                    solver.Status, solver.CurrentIteration, solver.Products);
+        dump("u_plain", solver.VectorX);
+        std::vector<int> st4 = mc.Status(0), st6 = mc.Status(1);
+        int bad = 0;
+        for (int s : st4) bad += s != MSA_OK;
+        for (int s : st6) bad += s != MSA_OK;
+        std::printf("\"mc_bad\": %d,\n", bad);
+        for (int s = 0; s < 3; s++) {
+            std::string k = "u4_" + std::to_string(s);
+            dump(k.c_str(), mc.Solution(0, s));
+        }
+        for (int s = 0; s < 2; s++) {
+            std::string k = "u6_" + std::to_string(s);
+            dump(k.c_str(), mc.Solution(1, s));
+        }
+        std::vector<double> ke0(ke.begin(), ke.begin() + 300);
+        dump("ke0", ke0, true);
+        std::printf("}\n");
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}
