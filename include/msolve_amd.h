@@ -34,6 +34,7 @@ enum {
 };
 
 enum { MSA_FP64 = 0, MSA_MIXED = 1 };
+enum { MSA_REFINE_RESTART = 0, MSA_REFINE_RELIABLE = 1 };
 
 typedef struct msa_ctx msa_ctx;
 typedef struct msa_mesh msa_mesh;
@@ -113,6 +114,7 @@ int msa_ke_packed(msa_ctx *ctx, int n_elem, const double *xyz8,
  *   precision   MSA_FP64: FP64 element integration, assembly and PCG.
  *               MSA_MIXED: FP32 integration/assembly/inner PCG, FP64 outer
  *               residual f - K u (matrix-free), update and stopping test.
+ *   (mixed refinement scheme: msa_batch_set_refinement)
  *   tile        samples per lane tile: 1, 2, 4, 8, 16, 32 or 64 (0 = pick:
  *               64 when a class has more than 32 samples, else 1). */
 int msa_batch_create(msa_ctx *ctx, int n_classes, msa_mesh *const *meshes,
@@ -166,6 +168,16 @@ int msa_batch_assemble(msa_batch *b, double nu);
  * iterations per hipGraph launch (even number), 0 launches eagerly. */
 int msa_batch_solve(msa_batch *b, double tol, int max_it, double inner_tol,
                     int max_outer, int graph_iters);
+
+/* Refinement scheme of MIXED solves (no counterpart in the reference, which
+ * is FP64 only).  MSA_REFINE_RELIABLE (default): the FP32 PCG never restarts;
+ * every `period` iterations (even) the FP32 correction is folded into the FP64
+ * solution, the residual is replaced by the FP64 f - K u, the stopping test
+ * |f - K u|/|f| < tol runs on it and the FP32 recurrence continues from it
+ * (search direction rescaled); inner_tol and max_outer are unused.
+ * MSA_REFINE_RESTART: classic refinement, each outer step solves
+ * K d = r/|r| in FP32 to inner_tol from zero. */
+int msa_batch_set_refinement(msa_batch *b, int method, int period);
 
 /* Modulus field -> assembly -> solve in one call (the whole MC job with all
  * inputs resident on the device). */

@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "_lib", "libmsolve_amd.so")
 MSA_OK, MSA_NOT_CONVERGED, MSA_BREAKDOWN, MSA_NEG_DETJ, MSA_NAN_INPUT = 0, 1, 2, 3, 4
 MSA_ERR_ARG, MSA_ERR_HIP, MSA_ERR_NO_DEVICE, MSA_ERR_OOM, MSA_ERR_STATE = -1, -2, -3, -4, -5
 MSA_FP64, MSA_MIXED = 0, 1
+MSA_REFINE_RESTART, MSA_REFINE_RELIABLE = 0, 1
 STATUS_NAMES = {0: "OK", 1: "NOT_CONVERGED", 2: "BREAKDOWN", 3: "NEG_DETJ", 4: "NAN_INPUT"}
 
 # every symbol include/msolve_amd.h declares
@@ -30,6 +31,7 @@ EXPORTS = [
     "msa_batch_get_solution", "msa_batch_get_stats", "msa_batch_get_modulus",
     "msa_batch_get_matrix_csr", "msa_batch_spmv", "msa_batch_set_kernel_timing",
     "msa_batch_get_kernel_timing", "msa_batch_spmv_bytes", "msa_autotune_tile",
+    "msa_batch_set_refinement",
 ]
 
 
@@ -98,6 +100,7 @@ def lib():
         "msa_batch_get_kernel_timing": [_vp, _dp, _llp],
         "msa_batch_spmv_bytes": [_vp, _dp],
         "msa_autotune_tile": [_vp, _vp, _i, _i, _ip, _dp],
+        "msa_batch_set_refinement": [_vp, _i, _i],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

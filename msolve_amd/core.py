@@ -188,6 +188,9 @@ class Batch:
         check(lib().msa_batch_spmv(self.h, cls, sample, ptr(x, C.c_double), ptr(y, C.c_double)))
         return y
 
+    def set_refinement(self, method=L.MSA_REFINE_RELIABLE, period: int = 8):
+        check(lib().msa_batch_set_refinement(self.h, method, period))
+
     def set_kernel_timing(self, on: bool):
         check(lib().msa_batch_set_kernel_timing(self.h, 1 if on else 0))
 

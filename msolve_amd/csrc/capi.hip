@@ -66,9 +66,13 @@ struct msa_batch {
     long long bytes = 0;
     bool assembled = false;
     double nu = 0.3;
-    // captured PCG iteration block
+    // mixed-precision refinement: restarted inner solves or periodic
+    // residual replacement
+    int refine = MSA_REFINE_RELIABLE;
+    int period = 8;
+    // captured block of PCG iterations (kind 0 plain, 1 with replacement)
     hipGraphExec_t graph = nullptr;
-    int graph_iters = 0;
+    int graph_iters = 0, graph_kind = -1;
     double graph_tol = -1;
     int graph_maxit = -1;
     Timing timing;
@@ -158,7 +162,42 @@ int launch_iteration_pair(msa_batch *b, int pbuf, double tol, int max_it)
     return MSA_OK;
 }
 
-int build_graph(msa_batch *b, int iters, double tol, int max_it)
+ColourLaunch colour_launch(const msa_batch *b, int colour, int *nblocks);
+int max_colours(const msa_batch *b);
+
+// Fold the FP32 correction into u64, form the FP64 residual, test it and
+// restart the FP32 recurrence from it (reliable-update refinement).
+int launch_replacement(msa_batch *b, double tol, int max_it)
+{
+    hipStream_t s = b->ctx->stream;
+    const int T = b->T, ni = (int)b->items.size();
+    MSA_TRY(timed(b, KK_OTHER, [&] { return launch_ru_fold(s, T, b->d_cls, b->d_items, ni); }));
+    const int ncol = max_colours(b);
+    for (int k = 0; k < ncol; k++) {
+        int nblk = 0;
+        ColourLaunch cl = colour_launch(b, k, &nblk);
+        MSA_TRY(timed(b, KK_RES64, [&] { return launch_residual64(s, T, b->d_cls, cl, nblk, b->nu); }));
+    }
+    MSA_TRY(timed(b, KK_OTHER, [&] {
+        return launch_ru_norm(s, T, b->d_cls, b->d_items, ni, tol, max_it, b->d_counts + 1,
+                              b->d_counts);
+    }));
+    MSA_TRY(timed(b, KK_OTHER, [&] { return launch_ru_restart(s, T, b->d_cls, b->d_items, ni); }));
+    return MSA_OK;
+}
+
+// One block of `iters` PCG iterations; kind 1 adds a residual replacement
+// after every b->period iterations (inner stopping test disabled).
+int launch_block(msa_batch *b, int kind, int iters, double tol, int max_it)
+{
+    for (int k = 0; k < iters; k++) {
+        MSA_TRY(launch_iteration_pair(b, k & 1, kind == 1 ? 0.0 : tol, max_it));
+        if (kind == 1 && (k + 1) % b->period == 0) MSA_TRY(launch_replacement(b, tol, max_it));
+    }
+    return MSA_OK;
+}
+
+int build_graph(msa_batch *b, int kind, int iters, double tol, int max_it)
 {
     if (b->graph) {
         (void)hipGraphExecDestroy(b->graph);
@@ -166,8 +205,7 @@ int build_graph(msa_batch *b, int iters, double tol, int max_it)
     }
     hipStream_t s = b->ctx->stream;
     MSA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int rc = MSA_OK;
-    for (int k = 0; k < iters && rc == MSA_OK; k++) rc = launch_iteration_pair(b, k & 1, tol, max_it);
+    int rc = launch_block(b, kind, iters, tol, max_it);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(s, &g);
     if (rc != MSA_OK) {
@@ -179,23 +217,27 @@ int build_graph(msa_batch *b, int iters, double tol, int max_it)
     (void)hipGraphDestroy(g);
     MSA_HIP(e);
     b->graph_iters = iters;
+    b->graph_kind = kind;
     b->graph_tol = tol;
     b->graph_maxit = max_it;
     return MSA_OK;
 }
 
-// Iterate the PCG kernels until no sample of the batch is active.
-int pcg_loop(msa_batch *b, double tol, int max_it, int graph_iters)
+// Iterate until the polled counter (0: PCG active samples, 1: samples still
+// refining) reaches zero.  kind 1 interleaves residual replacements.
+int pcg_loop(msa_batch *b, int kind, double tol, int max_it, int graph_iters)
 {
     const bool use_graph = graph_iters > 0 && !b->timing.on;
-    const int block = use_graph ? graph_iters : 8;
-    if (use_graph && (b->graph == nullptr || b->graph_iters != graph_iters ||
+    int block = graph_iters > 0 ? graph_iters : 8;
+    if (kind == 1) block = ((block + b->period - 1) / b->period) * b->period;
+    if (use_graph && (b->graph == nullptr || b->graph_iters != block || b->graph_kind != kind ||
                       b->graph_tol != tol || b->graph_maxit != max_it))
-        MSA_TRY(build_graph(b, graph_iters, tol, max_it));
+        MSA_TRY(build_graph(b, kind, block, tol, max_it));
+    const int which = kind == 1 ? 1 : 0;
     long long done = 0;
     for (;;) {
         int active = 0;
-        MSA_TRY(read_count(b, 0, &active));
+        MSA_TRY(read_count(b, which, &active));
         if (active <= 0) break;
         if (done > (long long)max_it + block) {
             set_error("PCG driver: samples still active after %lld iterations", done);
@@ -204,7 +246,7 @@ int pcg_loop(msa_batch *b, double tol, int max_it, int graph_iters)
         if (use_graph) {
             MSA_HIP(hipGraphLaunch(b->graph, b->ctx->stream));
         } else {
-            for (int k = 0; k < block; k++) MSA_TRY(launch_iteration_pair(b, k & 1, tol, max_it));
+            MSA_TRY(launch_block(b, kind, block, tol, max_it));
         }
         done += block;
     }
@@ -821,7 +863,15 @@ int msa_batch_solve(msa_batch *b, double tol, int max_it, double inner_tol, int 
     if (!mx) {
         MSA_HIP(hipMemsetAsync(b->d_counts, 0, sizeof(int), s));
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_pcg_init(s, false, T, b->d_cls, b->d_items, ni, 0, b->d_counts); }));
-        return pcg_loop(b, tol, max_it, graph_iters);
+        return pcg_loop(b, 0, tol, max_it, graph_iters);
+    }
+    if (b->refine == MSA_REFINE_RELIABLE) {
+        MSA_HIP(hipMemsetAsync(b->d_counts, 0, 2 * sizeof(int), s));
+        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_ir_begin(s, T, b->d_cls, b->d_items, ni, b->d_counts + 1); }));
+        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_pcg_init(s, true, T, b->d_cls, b->d_items, ni, 1, b->d_counts); }));
+        MSA_TRY(pcg_loop(b, 1, tol, max_it, graph_iters));
+        MSA_HIP(hipStreamSynchronize(s));
+        return MSA_OK;
     }
     MSA_CHECK(inner_tol > 0 && inner_tol < 1 && max_outer >= 1, MSA_ERR_ARG,
               "inner_tol=%g max_outer=%d", inner_tol, max_outer);
@@ -834,7 +884,7 @@ int msa_batch_solve(msa_batch *b, double tol, int max_it, double inner_tol, int 
         if (left <= 0) break;
         MSA_HIP(hipMemsetAsync(b->d_counts, 0, sizeof(int), s));
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_pcg_init(s, true, T, b->d_cls, b->d_items, ni, 1, b->d_counts); }));
-        MSA_TRY(pcg_loop(b, inner_tol, max_it, graph_iters));
+        MSA_TRY(pcg_loop(b, 0, inner_tol, max_it, graph_iters));
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_ir_update(s, T, b->d_cls, b->d_items, ni); }));
         for (int k = 0; k < ncol; k++) {
             int nblk = 0;
@@ -1005,6 +1055,18 @@ int msa_batch_spmv(msa_batch *b, int cls, int sample, const double *x, double *y
         if (p) (void)hipFree(p);
     MSA_HIP(e);
     for (int g = 0; g < m.n_free; g++) y[g] = xi[m.ref2int[g]];
+    return MSA_OK;
+}
+
+int msa_batch_set_refinement(msa_batch *b, int method, int period)
+{
+    MSA_CHECK(b, MSA_ERR_ARG, "null batch");
+    MSA_CHECK(method == MSA_REFINE_RESTART || method == MSA_REFINE_RELIABLE, MSA_ERR_ARG,
+              "refinement method %d", method);
+    MSA_CHECK(period >= 2 && period % 2 == 0 && period <= 4096, MSA_ERR_ARG,
+              "replacement period %d must be even, 2..4096", period);
+    b->refine = method;
+    b->period = period;
     return MSA_OK;
 }
 

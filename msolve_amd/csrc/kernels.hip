@@ -796,6 +796,9 @@ __global__ __launch_bounds__(64) void k_residual64(const ClassDev *__restrict__ 
     const ClassDev &C = cls[c];
     const int e = C.colour_elems[C.colour_ptr[cl.colour] + (item - cl.prefix[c])];
     const int lane = threadIdx.x & 63;
+    bool any = false;
+    for (int smp = lane; smp < C.nsamp; smp += 64) any |= C.st[smp].ir_active != 0;
+    if (!__ballot(any)) return;
     const int *en = C.conn + (size_t)e * 8;
     const int na = en[lane & 7];
     const double xa[3] = {C.xyz[3 * na], C.xyz[3 * na + 1], C.xyz[3 * na + 2]};
@@ -891,6 +894,159 @@ __global__ __launch_bounds__(256) void k_ir_norm(const ClassDev *__restrict__ cl
                 S.scale = nr;
             }
         }
+    }
+}
+
+// ------------------------------------------------------------ reliable updates
+// Mixed precision without restarts: the FP32 PCG keeps its search direction
+// while every `period` iterations the accumulated FP32 correction is folded
+// into the FP64 solution, the residual is replaced by the FP64 f - K u, the
+// stopping test runs on that FP64 residual and the FP32 recurrence restarts
+// from it (search direction rescaled to the new residual scale).
+
+// u64 += scale * x32, x32 = 0, r64 = f for every sample still refining.
+template <int T>
+__global__ __launch_bounds__(256) void k_ru_fold(const ClassDev *__restrict__ cls,
+                                                 const int4 *__restrict__ items, int nitems)
+{
+    constexpr int R = 64 / T;
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const int tile = it.y, chunk = it.z;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rr = lane / T, s = lane % T;
+    const int sample = tile * T + s;
+    if (sample >= C.nsamp || !C.st[sample].ir_active) return;
+    const double sc = C.st[sample].scale;
+    const size_t vb = (size_t)tile * C.tile_vec;
+    float *x = (float *)C.x + vb;
+    const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+    for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+        const int br = slice * R + rr;
+        if (br >= C.nb) continue;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const size_t o = ((size_t)br * 3 + d) * T + s;
+            C.u64[vb + o] += sc * (double)x[o];
+            x[o] = 0.f;
+            C.r64[vb + o] = C.f[br * 3 + d];
+        }
+    }
+}
+
+// |r64| and the FP64 stopping test; sets the rescale ratio for the restart.
+template <int T>
+__global__ __launch_bounds__(256) void k_ru_norm(const ClassDev *__restrict__ cls,
+                                                 const int4 *__restrict__ items, int nitems,
+                                                 double tol, int max_it, int *ir_count,
+                                                 int *active_count)
+{
+    constexpr int R = 64 / T;
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const int tile = it.y, chunk = it.z;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rr = lane / T, s = lane % T;
+    const int sample = tile * T + s;
+    const bool act = sample < C.nsamp && C.st[sample].ir_active;
+    const size_t vb = (size_t)tile * C.tile_vec;
+    double acc[1] = {0};
+    if (act) {
+        const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+        for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+            const int br = slice * R + rr;
+            if (br >= C.nb) continue;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const double v = C.r64[vb + ((size_t)br * 3 + d) * T + s];
+                acc[0] += v * v;
+            }
+        }
+    }
+    block_partials<T, 1>(acc, C.partial + ((size_t)tile * C.nchunks + chunk) * 3 * T);
+    if (last_block(C.counter + tile * 8 + 4, C.nchunks) && threadIdx.x < T) {
+        const int t = threadIdx.x, smp = tile * T + t;
+        if (smp < C.nsamp && C.st[smp].ir_active) {
+            SampleState &S = C.st[smp];
+            const double nr = sqrt(sum_partials<T>(C, tile, 0, t));
+            S.outer += 1;
+            S.total_iters = S.iters;
+            S.r64norm = nr;
+            const double err = nr / C.fnorm;
+            S.relres = err;
+            int done = 0;
+            if (!isfinite(err)) {
+                S.status = MSA_BREAKDOWN;
+                done = 1;
+            } else if (err < tol) {
+                S.status = MSA_OK;
+                done = 1;
+            } else if (S.iters >= max_it || !S.active) {
+                if (S.status != MSA_BREAKDOWN) S.status = MSA_NOT_CONVERGED;
+                done = 1;
+            }
+            if (done) {
+                S.ir_active = 0;
+                atomicSub(ir_count, 1);
+                if (S.active) {
+                    S.active = 0;
+                    atomicSub(active_count, 1);
+                }
+            } else {
+                S.pq = S.scale / nr;  // rescale ratio of the search direction
+                S.scale = nr;
+            }
+        }
+    }
+}
+
+// Restart of the FP32 recurrence from the FP64 residual: r = r64/|r64|,
+// z = D^-1 r, p_old *= old_scale/new_scale, rz = r.z.
+template <int T>
+__global__ __launch_bounds__(256) void k_ru_restart(const ClassDev *__restrict__ cls,
+                                                    const int4 *__restrict__ items, int nitems)
+{
+    constexpr int R = 64 / T;
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const int tile = it.y, chunk = it.z;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rr = lane / T, s = lane % T;
+    const int sample = tile * T + s;
+    const bool act = sample < C.nsamp && C.st[sample].ir_active;
+    const size_t vb = (size_t)tile * C.tile_vec;
+    double acc[1] = {0};
+    if (act) {
+        const double inv = 1.0 / C.st[sample].scale;
+        const float ratio = (float)C.st[sample].pq;
+        float *r = (float *)C.r + vb, *z = (float *)C.z + vb, *p0 = (float *)C.p0 + vb;
+        const float *dinv = (const float *)C.dinv + vb;
+        const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+        for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+            const int br = slice * R + rr;
+            if (br >= C.nb) continue;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const size_t o = ((size_t)br * 3 + d) * T + s;
+                const float rv = (float)(C.r64[vb + o] * inv);
+                const float zv = dinv[o] * rv;
+                r[o] = rv;
+                z[o] = zv;
+                p0[o] *= ratio;
+                acc[0] += (double)(rv * zv);
+            }
+        }
+    }
+    block_partials<T, 1>(acc, C.partial + ((size_t)tile * C.nchunks + chunk) * 3 * T);
+    if (last_block(C.counter + tile * 8 + 5, C.nchunks) && threadIdx.x < T) {
+        const int t = threadIdx.x, smp = tile * T + t;
+        if (smp < C.nsamp && C.st[smp].ir_active) C.st[smp].rz = sum_partials<T>(C, tile, 0, t);
     }
 }
 
@@ -1139,6 +1295,24 @@ hipError_t launch_ir_norm(hipStream_t s, int T, const ClassDev *d_cls, const int
                           int nitems, double tol, int max_outer, int *ir_count)
 {
     MSA_T_LAUNCH(k_ir_norm, d_cls, items, nitems, tol, max_outer, ir_count);
+}
+
+hipError_t launch_ru_fold(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,
+                          int nitems)
+{
+    MSA_T_LAUNCH(k_ru_fold, d_cls, items, nitems);
+}
+
+hipError_t launch_ru_norm(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,
+                          int nitems, double tol, int max_it, int *ir_count, int *active_count)
+{
+    MSA_T_LAUNCH(k_ru_norm, d_cls, items, nitems, tol, max_it, ir_count, active_count);
+}
+
+hipError_t launch_ru_restart(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,
+                             int nitems)
+{
+    MSA_T_LAUNCH(k_ru_restart, d_cls, items, nitems);
 }
 
 hipError_t launch_kl_basis(hipStream_t s, const ClassDev *d_cls, int n_elem, int M, int kmax,

@@ -143,6 +143,12 @@ hipError_t launch_ir_update(hipStream_t s, int T, const ClassDev *d_cls, const i
                             int nitems);
 hipError_t launch_ir_norm(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,
                           int nitems, double tol, int max_outer, int *ir_count);
+hipError_t launch_ru_fold(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,
+                          int nitems);
+hipError_t launch_ru_norm(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,
+                          int nitems, double tol, int max_it, int *ir_count, int *active_count);
+hipError_t launch_ru_restart(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,
+                             int nitems);
 hipError_t launch_spmv(hipStream_t s, bool mixed, int T, const ClassDev *d_cls, const int4 *items,
                        int nitems, const double *x, double *y);
 hipError_t launch_kl_basis(hipStream_t s, const ClassDev *d_cls, int n_elem, int M, int kmax,

@@ -207,13 +207,15 @@ def test_graph_equals_eager(ctx):
     assert np.array_equal(out[0][1], out[1][1])
 
 
-def test_mixed_precision_refinement(ctx):
-    print("\n[case] mixed IR 8^3", flush=True)
+@pytest.mark.parametrize("method", [L.MSA_REFINE_RELIABLE, L.MSA_REFINE_RESTART])
+def test_mixed_precision_refinement(ctx, method):
+    print(f"\n[case] mixed 8^3 method={method}", flush=True)
     model = msa.cantilever(8)
     rng = np.random.default_rng(11)
     E = lognormal_E(rng, 3 * model.n_elem, cov=0.3).reshape(3, -1)
     mesh = msa.Mesh(ctx, model.xyz, model.conn, model.dof, model.n_free)
     b = msa.Batch(ctx, [mesh], [3], L.MSA_MIXED, 64)
+    b.set_refinement(method, 8)
     b.set_modulus(0, E)
     b.set_rhs(0, model.f)
     b.assemble()
