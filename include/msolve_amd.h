@@ -62,6 +62,7 @@ void CALCH8K(const int *iInt, const double *E, const double *B,
 
 /* --------------------------------------------------------------- context */
 int msa_ctx_create(int device, msa_ctx **out);
+/* Destroy every mesh and batch of the context first. */
 int msa_ctx_destroy(msa_ctx *ctx);
 /* The HIP stream every kernel of this context runs on (hipStream_t). */
 void *msa_ctx_stream(msa_ctx *ctx);

@@ -2,6 +2,7 @@
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -10,13 +11,18 @@ from ._lib import check, f64, i32, lib, ptr
 
 
 class Context:
-    """One HIP device + stream (msa_ctx)."""
+    """One HIP device + stream (msa_ctx).  Meshes and batches created on it are
+    closed before the context itself (the C ABI requires that order)."""
 
     def __init__(self, device: int = 0):
         h = C.c_void_p()
+        self._children = weakref.WeakSet()
         check(lib().msa_ctx_create(device, C.byref(h)))
         self.h = h
         self.device = device
+
+    def _adopt(self, child):
+        self._children.add(child)
 
     @property
     def stream(self) -> int:
@@ -26,7 +32,9 @@ class Context:
         check(lib().msa_ctx_synchronize(self.h))
 
     def close(self):
-        if self.h:
+        if getattr(self, "h", None):
+            for child in sorted(list(self._children), key=lambda c: isinstance(c, Mesh)):
+                child.close()          # batches first, then meshes
             lib().msa_ctx_destroy(self.h)
             self.h = None
 
@@ -62,6 +70,7 @@ class Mesh:
                                     self.conn.shape[0], ptr(self.conn, C.c_int),
                                     ptr(self.dof, C.c_int), self.n_free, C.byref(h)))
         self.h = h
+        ctx._adopt(self)
         info = np.zeros(8, np.int64)
         check(lib().msa_mesh_info(self.h, ptr(info, C.c_longlong)))
         self.info = dict(blocks=int(info[0]), block_nnz=int(info[1]), nnz=int(info[2]),
@@ -111,6 +120,7 @@ class Batch:
         check(lib().msa_batch_create(ctx.h, len(self.meshes), arr, ptr(ns, C.c_int), precision,
                                      tile, C.byref(h)))
         self.h = h
+        ctx._adopt(self)
 
     @property
     def device_bytes(self) -> int:

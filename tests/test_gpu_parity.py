@@ -26,7 +26,8 @@ def direct_solution(model, E, nu=0.3):
     """Oracle: Ke -> skyline -> LDL^T solve (SolverSkyline semantics)."""
     ke, bad = O.ke_mesh(model.conn, model.xyz, E, nu)
     assert bad == 0
-    ri, data = O.skyline_assemble(model.conn, model.dof, model.n_free, ke)
+    n = model.n_free if hasattr(model, "n_free") else model.n
+    ri, data = O.skyline_assemble(model.conn, model.dof, n, ke)
     fac, rc = O.skyline_factorize(ri, data, 1e-15)
     assert rc == 0
     return O.skyline_solve(ri, fac, model.f)
