@@ -57,6 +57,9 @@ def parse():
     p.add_argument("--max-outer", type=int, default=30)
     p.add_argument("--graph-iters", type=int, default=16)
     p.add_argument("--tile", type=int, default=64)
+    p.add_argument("--period", type=int, default=8, help="FP64 residual replacement period")
+    p.add_argument("--refine", choices=["reliable", "restart"], default="reliable")
+    p.add_argument("--only-mixed", action="store_true", help="skip every comparison run")
     p.add_argument("--seed", type=int, default=20261202)
     p.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison runs")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -75,6 +78,9 @@ def workload(args):
 
 def make_batch(msa, ctx, meshes, models, counts, precision, tile, table, xi, args):
     b = msa.Batch(ctx, meshes, counts, precision, tile)
+    if precision == msa.MSA_MIXED:
+        b.set_refinement(msa.MSA_REFINE_RELIABLE if args.refine == "reliable"
+                         else msa.MSA_REFINE_RESTART, args.period)
     off = 0
     for c, m in enumerate(models):
         b.set_kl(c, table, m.domain, 2.1e5, args.cov)
@@ -104,6 +110,8 @@ def kernel_a_bytes(b, models, counts, rs, launches):
 
 def main():
     args = parse()
+    if args.only_mixed:
+        args.no_fp64 = args.no_cpu = args.no_single = True
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of the "
                                    "Monte Carlo samples is described in DESIGN.md",
@@ -171,7 +179,10 @@ def main():
         ctx.synchronize()
         t64 = time.perf_counter() - t
         extra["fp64_batched_solves_per_s"] = N / t64
-        log(f"[bench] fp64 batched: {t64:.2f} s -> {N / t64:.1f} solves/s")
+        extra["fp64_iters_mean_per_size"] = {
+            str(n): float(np.mean(b64.stats(c)["iters"])) for c, n in enumerate(sizes)}
+        log(f"[bench] fp64 batched: {t64:.2f} s -> {N / t64:.1f} solves/s, iters "
+            f"{extra['fp64_iters_mean_per_size']}")
         kw = dict(run_kw)
         kw["tol"] = args.oracle_tol
         b64.run(**kw)

@@ -180,6 +180,13 @@ int msa_batch_solve(msa_batch *b, double tol, int max_it, double inner_tol,
  * K d = r/|r| in FP32 to inner_tol from zero. */
 int msa_batch_set_refinement(msa_batch *b, int method, int period);
 
+/* Element integration precision of a MIXED batch, whose stiffness values are
+ * stored in FP32: MSA_FP64 (default) integrates in FP64 and rounds each
+ * assembled value; MSA_MIXED integrates in FP32.  FP32 integration leaves
+ * ~4e-7 relative error in K (vs 6e-8 for rounding), which costs about a
+ * quarter more PCG iterations under the FP64 residual replacement. */
+int msa_batch_set_integration(msa_batch *b, int precision);
+
 /* Modulus field -> assembly -> solve in one call (the whole MC job with all
  * inputs resident on the device). */
 int msa_batch_run(msa_batch *b, double nu, double tol, int max_it,

@@ -5,7 +5,8 @@ The compute runs in msolve_amd/_lib/libmsolve_amd.so (HIP, C ABI declared in
 include/msolve_amd.h); this package is the host-side binding.
 """
 from ._lib import (MSA_BREAKDOWN, MSA_FP64, MSA_MIXED, MSA_NAN_INPUT, MSA_NEG_DETJ,
-                   MSA_NOT_CONVERGED, MSA_OK, MsaError, STATUS_NAMES, lib)
+                   MSA_NOT_CONVERGED, MSA_OK, MSA_REFINE_RELIABLE, MSA_REFINE_RESTART,
+                   MsaError, STATUS_NAMES, lib)
 from .analyzers import (ElementStructuralStiffnessProvider,
                         MonteCarloAnalyzerWithStochasticMaterial, SolverPCG)
 from .core import Batch, Context, Mesh, autotune_tile, enumerate_dofs, ke_packed
@@ -13,7 +14,7 @@ from .model import box_mesh, cantilever
 
 __all__ = [
     "MSA_OK", "MSA_NOT_CONVERGED", "MSA_BREAKDOWN", "MSA_NEG_DETJ", "MSA_NAN_INPUT",
-    "MSA_FP64", "MSA_MIXED", "MsaError", "STATUS_NAMES", "lib",
+    "MSA_FP64", "MSA_MIXED", "MSA_REFINE_RELIABLE", "MSA_REFINE_RESTART", "MsaError", "STATUS_NAMES", "lib",
     "Context", "Mesh", "Batch", "ke_packed", "enumerate_dofs", "autotune_tile",
     "box_mesh", "cantilever", "ElementStructuralStiffnessProvider", "SolverPCG",
     "MonteCarloAnalyzerWithStochasticMaterial",

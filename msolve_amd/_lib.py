@@ -31,7 +31,7 @@ EXPORTS = [
     "msa_batch_get_solution", "msa_batch_get_stats", "msa_batch_get_modulus",
     "msa_batch_get_matrix_csr", "msa_batch_spmv", "msa_batch_set_kernel_timing",
     "msa_batch_get_kernel_timing", "msa_batch_spmv_bytes", "msa_autotune_tile",
-    "msa_batch_set_refinement",
+    "msa_batch_set_refinement", "msa_batch_set_integration",
 ]
 
 
@@ -101,6 +101,7 @@ def lib():
         "msa_batch_spmv_bytes": [_vp, _dp],
         "msa_autotune_tile": [_vp, _vp, _i, _i, _ip, _dp],
         "msa_batch_set_refinement": [_vp, _i, _i],
+        "msa_batch_set_integration": [_vp, _i],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -201,6 +201,9 @@ class Batch:
     def set_refinement(self, method=L.MSA_REFINE_RELIABLE, period: int = 8):
         check(lib().msa_batch_set_refinement(self.h, method, period))
 
+    def set_integration(self, precision=L.MSA_FP64):
+        check(lib().msa_batch_set_integration(self.h, precision))
+
     def set_kernel_timing(self, on: bool):
         check(lib().msa_batch_set_kernel_timing(self.h, 1 if on else 0))
 

@@ -61,6 +61,8 @@ struct msa_batch {
     ClassDev *d_cls = nullptr;
     std::vector<int4> items;
     int4 *d_items = nullptr;
+    std::vector<int2> tiles;  // (class, tile) pairs, one reduction workgroup each
+    int2 *d_tiles = nullptr;
     int *d_counts = nullptr;  // [0] PCG active samples, [1] refinement active samples
     std::vector<void *> allocs;
     long long bytes = 0;
@@ -70,6 +72,10 @@ struct msa_batch {
     // residual replacement
     int refine = MSA_REFINE_RELIABLE;
     int period = 8;
+    // element integration precision of MIXED batches (values stored in FP32)
+    int integration = MSA_FP64;
+    double *stage = nullptr;  // FP64 staging tile for MIXED + FP64 integration
+    size_t stage_vals = 0;
     // captured block of PCG iterations (kind 0 plain, 1 with replacement)
     hipGraphExec_t graph = nullptr;
     int graph_iters = 0, graph_kind = -1;
@@ -128,6 +134,20 @@ int timed(msa_batch *b, int kind, F &&launch)
     return MSA_OK;
 }
 
+int reduce(msa_batch *b, int op, double tol = 0.0, int max_it = 0, int max_outer = 0)
+{
+    ReduceArgs a{};
+    a.op = op;
+    a.tol = tol;
+    a.max_it = max_it;
+    a.max_outer = max_outer;
+    a.active_count = b->d_counts;
+    a.ir_count = b->d_counts + 1;
+    return timed(b, KK_OTHER, [&] {
+        return launch_reduce(b->ctx->stream, b->T, b->d_cls, b->d_tiles, (int)b->tiles.size(), a);
+    });
+}
+
 int upload_class_table(msa_batch *b)
 {
     std::vector<ClassDev> h(b->cls.size());
@@ -155,10 +175,12 @@ int launch_iteration_pair(msa_batch *b, int pbuf, double tol, int max_it)
         return launch_pcg_a(s, mx, b->T, b->d_cls, b->d_items, (int)b->items.size(), pbuf,
                             b->d_counts);
     }));
+    MSA_TRY(reduce(b, OP_A));
     MSA_TRY(timed(b, KK_UPDATE, [&] {
         return launch_pcg_b(s, mx, b->T, b->d_cls, b->d_items, (int)b->items.size(), pbuf, tol,
                             max_it, b->d_counts);
     }));
+    MSA_TRY(reduce(b, OP_B, tol, max_it));
     return MSA_OK;
 }
 
@@ -182,7 +204,9 @@ int launch_replacement(msa_batch *b, double tol, int max_it)
         return launch_ru_norm(s, T, b->d_cls, b->d_items, ni, tol, max_it, b->d_counts + 1,
                               b->d_counts);
     }));
+    MSA_TRY(reduce(b, OP_RUNORM, tol, max_it));
     MSA_TRY(timed(b, KK_OTHER, [&] { return launch_ru_restart(s, T, b->d_cls, b->d_items, ni); }));
+    MSA_TRY(reduce(b, OP_RURESTART));
     return MSA_OK;
 }
 
@@ -661,17 +685,17 @@ int msa_batch_create(msa_ctx *ctx, int n_classes, msa_mesh *const *meshes, const
         A((void **)&D.E, (size_t)C.ntiles * h.n_elem * T * sizeof(double));
         A((void **)&D.st, (size_t)C.ntiles * T * sizeof(SampleState));
         A((void **)&D.partial, (size_t)C.ntiles * C.nchunks * 3 * T * sizeof(double));
-        A((void **)&D.counter, (size_t)C.ntiles * 8 * sizeof(unsigned));
         A((void **)&D.negdet, sizeof(int));
         if (rc != MSA_OK) break;
         hipStream_t s = ctx->stream;
         MSA_HIP(hipMemsetAsync(D.st, 0, (size_t)C.ntiles * T * sizeof(SampleState), s));
-        MSA_HIP(hipMemsetAsync(D.counter, 0, (size_t)C.ntiles * 8 * sizeof(unsigned), s));
         MSA_HIP(hipMemsetAsync(D.negdet, 0, sizeof(int), s));
         MSA_HIP(hipMemsetAsync((void *)D.f, 0, (size_t)h.nb * 3 * sizeof(double), s));
         MSA_HIP(hipMemsetAsync(D.E, 0, (size_t)C.ntiles * h.n_elem * T * sizeof(double), s));
-        for (int t = 0; t < C.ntiles; t++)
+        for (int t = 0; t < C.ntiles; t++) {
+            b->tiles.push_back(make_int2(c, t));
             for (int k = 0; k < C.nchunks; k++) b->items.push_back(make_int4(c, t, k, 0));
+        }
     }
     if (rc != MSA_OK) return fail(rc);
     void *p = nullptr;
@@ -681,8 +705,12 @@ int msa_batch_create(msa_ctx *ctx, int n_classes, msa_mesh *const *meshes, const
     b->d_items = (int4 *)p;
     if (rc == MSA_OK) rc = dalloc(b, &p, 4 * sizeof(int));
     b->d_counts = (int *)p;
+    if (rc == MSA_OK) rc = dalloc(b, &p, sizeof(int2) * b->tiles.size());
+    b->d_tiles = (int2 *)p;
     if (rc != MSA_OK) return fail(rc);
     if (hipMemcpy(b->d_items, b->items.data(), sizeof(int4) * b->items.size(),
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->d_tiles, b->tiles.data(), sizeof(int2) * b->tiles.size(),
                   hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(b->d_counts, 0, 4 * sizeof(int)) != hipSuccess) {
         set_error("batch upload failed");
@@ -833,10 +861,46 @@ int msa_batch_assemble(msa_batch *b, double nu)
         MSA_HIP(hipMemsetAsync(C.dev.negdet, 0, sizeof(int), s));
     }
     const int ncol = max_colours(b);
-    for (int k = 0; k < ncol; k++) {
-        int nblk = 0;
-        ColourLaunch cl = colour_launch(b, k, &nblk);
-        MSA_TRY(timed(b, KK_ASSEMBLE, [&] { return launch_assemble(s, mx, b->T, b->d_cls, cl, nblk, nu); }));
+    if (mx && b->integration == MSA_FP64) {
+        // FP64 integration and accumulation of one tile at a time in a staging
+        // buffer, then a single rounding of every value to FP32
+        size_t need = 0;
+        for (auto &C : b->cls) need = std::max(need, (size_t)C.tile_vals);
+        if (b->stage_vals < need) {
+            void *p = nullptr;
+            MSA_TRY(dalloc(b, &p, need * sizeof(double)));
+            b->stage = (double *)p;
+            b->stage_vals = need;
+        }
+        for (size_t c = 0; c < b->cls.size(); c++) {
+            ClassHost &C = b->cls[c];
+            for (int t = 0; t < C.ntiles; t++) {
+                MSA_TRY(timed(b, KK_OTHER, [&] { return launch_fill_f64(s, b->stage, (size_t)C.tile_vals, 0.0); }));
+                for (int k = 0; k < C.mesh->h.ncolours; k++) {
+                    ColourLaunch cl{};
+                    cl.ncls = (int)b->cls.size();
+                    cl.colour = k;
+                    const int cnt = C.mesh->h.colour_ptr[k + 1] - C.mesh->h.colour_ptr[k];
+                    for (int q = 0; q <= cl.ncls; q++) cl.prefix[q] = q <= (int)c ? 0 : cnt;
+                    MSA_TRY(timed(b, KK_ASSEMBLE, [&] {
+                        return launch_assemble(s, 0, b->T, b->d_cls, cl, cnt, nu, t, t + 1, b->stage);
+                    }));
+                }
+                MSA_TRY(timed(b, KK_OTHER, [&] {
+                    return launch_round(s, b->stage, (size_t)C.tile_vals,
+                                        (float *)C.dev.val + (size_t)t * C.tile_vals);
+                }));
+            }
+        }
+    } else {
+        for (int k = 0; k < ncol; k++) {
+            int nblk = 0;
+            ColourLaunch cl = colour_launch(b, k, &nblk);
+            const int kprec = mx ? 2 : 0;
+            MSA_TRY(timed(b, KK_ASSEMBLE, [&] {
+                return launch_assemble(s, kprec, b->T, b->d_cls, cl, nblk, nu, 0, 1 << 30, nullptr);
+            }));
+        }
     }
     MSA_TRY(timed(b, KK_OTHER, [&] {
         return launch_dinv(s, mx, b->T, b->d_cls, b->d_items, (int)b->items.size());
@@ -863,12 +927,14 @@ int msa_batch_solve(msa_batch *b, double tol, int max_it, double inner_tol, int 
     if (!mx) {
         MSA_HIP(hipMemsetAsync(b->d_counts, 0, sizeof(int), s));
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_pcg_init(s, false, T, b->d_cls, b->d_items, ni, 0, b->d_counts); }));
+        MSA_TRY(reduce(b, OP_INIT0));
         return pcg_loop(b, 0, tol, max_it, graph_iters);
     }
     if (b->refine == MSA_REFINE_RELIABLE) {
         MSA_HIP(hipMemsetAsync(b->d_counts, 0, 2 * sizeof(int), s));
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_ir_begin(s, T, b->d_cls, b->d_items, ni, b->d_counts + 1); }));
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_pcg_init(s, true, T, b->d_cls, b->d_items, ni, 1, b->d_counts); }));
+        MSA_TRY(reduce(b, OP_INIT1));
         MSA_TRY(pcg_loop(b, 1, tol, max_it, graph_iters));
         MSA_HIP(hipStreamSynchronize(s));
         return MSA_OK;
@@ -884,6 +950,7 @@ int msa_batch_solve(msa_batch *b, double tol, int max_it, double inner_tol, int 
         if (left <= 0) break;
         MSA_HIP(hipMemsetAsync(b->d_counts, 0, sizeof(int), s));
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_pcg_init(s, true, T, b->d_cls, b->d_items, ni, 1, b->d_counts); }));
+        MSA_TRY(reduce(b, OP_INIT1));
         MSA_TRY(pcg_loop(b, 0, inner_tol, max_it, graph_iters));
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_ir_update(s, T, b->d_cls, b->d_items, ni); }));
         for (int k = 0; k < ncol; k++) {
@@ -894,6 +961,7 @@ int msa_batch_solve(msa_batch *b, double tol, int max_it, double inner_tol, int 
         MSA_TRY(timed(b, KK_OTHER, [&] {
             return launch_ir_norm(s, T, b->d_cls, b->d_items, ni, tol, max_outer, b->d_counts + 1);
         }));
+        MSA_TRY(reduce(b, OP_IRNORM, tol, 0, max_outer));
     }
     MSA_HIP(hipStreamSynchronize(s));
     return MSA_OK;
@@ -1070,6 +1138,16 @@ int msa_batch_set_refinement(msa_batch *b, int method, int period)
     return MSA_OK;
 }
 
+int msa_batch_set_integration(msa_batch *b, int precision)
+{
+    MSA_CHECK(b, MSA_ERR_ARG, "null batch");
+    MSA_CHECK(precision == MSA_FP64 || precision == MSA_MIXED, MSA_ERR_ARG,
+              "integration precision %d", precision);
+    b->integration = precision;
+    b->assembled = false;
+    return MSA_OK;
+}
+
 int msa_batch_set_kernel_timing(msa_batch *b, int enable)
 {
     MSA_CHECK(b, MSA_ERR_ARG, "null batch");
@@ -1133,6 +1211,7 @@ int msa_autotune_tile(msa_ctx *ctx, msa_mesh *mesh, int n_samples, int precision
             if (launch_pcg_init(s, precision == MSA_MIXED, b->T, b->d_cls, b->d_items, ni, 0,
                                 b->d_counts) != hipSuccess)
                 rc = MSA_ERR_HIP;
+            if (rc == MSA_OK) rc = reduce(b, OP_INIT0);
         }
         if (rc == MSA_OK) {
             // warm-up, then 10 timed products (pbuf 0 keeps every sample active)

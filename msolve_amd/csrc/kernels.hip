@@ -60,35 +60,6 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], double *out)
     }
 }
 
-// Ticket: true in exactly one workgroup, the last of `total` to arrive.
-__device__ __forceinline__ bool last_block(unsigned *counter, unsigned total)
-{
-    __shared__ unsigned is_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) is_last = (atomicAdd(counter, 1u) == total - 1u);
-    __syncthreads();
-    if (is_last) {
-        __threadfence();
-        if (threadIdx.x == 0) atomicExch(counter, 0u);
-    }
-    return is_last;
-}
-
-__device__ __forceinline__ double coherent_load(const double *p)
-{
-    return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Sum of partial k of sample lane t over all work items of a tile.
-template <int T>
-__device__ __forceinline__ double sum_partials(const ClassDev &C, int tile, int k, int t)
-{
-    double acc = 0;
-    const double *p = C.partial + ((size_t)tile * C.nchunks * 3 + k) * T + t;
-    for (int c = 0; c < C.nchunks; c++) acc += coherent_load(p + (size_t)c * 3 * T);
-    return acc;
-}
 
 // ------------------------------------------------------------ Hexa8 stiffness
 
@@ -335,9 +306,12 @@ __device__ __forceinline__ int colour_class(const ColourLaunch &cl, int item)
 // Khat staged in LDS, then val += E_s * Khat for every sample of every tile.
 // Elements of one colour share no node, so the read-modify-writes are
 // race-free.  kreal = precision of the element integration.
+// tile_lo/tile_hi restrict the tiles written; a non-null `stage` receives
+// the single tile tile_lo instead of the class's value array (FP64 staging
+// of a mixed batch, rounded once to FP32 afterwards by k_round).
 template <typename kreal, typename real, int T>
 __global__ __launch_bounds__(64) void k_assemble(const ClassDev *__restrict__ cls, ColourLaunch cl,
-                                                 double nu)
+                                                 double nu, int tile_lo, int tile_hi, real *stage)
 {
     constexpr int R = 64 / T;
     __shared__ kreal Kh[576];
@@ -356,9 +330,10 @@ __global__ __launch_bounds__(64) void k_assemble(const ClassDev *__restrict__ cl
     if (__ballot(neg) && lane == 0) atomicAdd(C.negdet, 1);
     const uint8_t *em = C.emap + (size_t)e * 64;
     const int s = lane % T;
-    for (int tile = 0; tile < C.ntiles; tile++) {
+    const int t_end = min(C.ntiles, tile_hi);
+    for (int tile = max(0, tile_lo); tile < t_end; tile++) {
         if (tile * T + s >= C.nsamp) continue;
-        real *val = (real *)C.val + (size_t)tile * C.tile_vals;
+        real *val = stage ? stage : (real *)C.val + (size_t)tile * C.tile_vals;
         const real Es = (real)C.E[((size_t)tile * C.n_elem + e) * T + s];
         for (int pe = lane / T; pe < 576; pe += R) {
             const int pair = pe / 9, ent = pe - 9 * pair;
@@ -455,36 +430,6 @@ __global__ __launch_bounds__(256) void k_pcg_init(const ClassDev *__restrict__ c
         }
     }
     block_partials<T, 2>(acc, C.partial + ((size_t)tile * C.nchunks + chunk) * 3 * T);
-    if (last_block(C.counter + tile * 8 + 0, C.nchunks) && threadIdx.x < T) {
-        const int t = threadIdx.x, smp = tile * T + t;
-        if (smp < C.nsamp) {
-            SampleState &S = C.st[smp];
-            const bool on = (mode == 0) || S.ir_active;
-            S.iters = 0;
-            S.beta = 0;
-            if (!on) {
-                S.active = 0;
-            } else {
-                const double rz = sum_partials<T>(C, tile, 0, t);
-                const double bb = sum_partials<T>(C, tile, 1, t);
-                S.rz = rz;
-                S.bnorm = sqrt(bb);
-                S.relres = 1.0;
-                if (!isfinite(bb) || !isfinite(rz)) {
-                    S.status = MSA_NAN_INPUT;
-                    S.active = 0;
-                } else if (S.bnorm > 1e-25) {
-                    S.status = MSA_NOT_CONVERGED;
-                    S.active = 1;
-                    atomicAdd(active_count, 1);
-                } else {
-                    S.status = MSA_OK;
-                    S.relres = 0;
-                    S.active = 0;
-                }
-            }
-        }
-    }
 }
 
 // Search direction + product + curvature (SolverPCGSimpleSearchVectorCalculator
@@ -554,21 +499,6 @@ __global__ __launch_bounds__(256) void k_pcg_a(const ClassDev *__restrict__ cls,
     }
     double acc[1] = {(double)pq};
     block_partials<T, 1>(acc, C.partial + ((size_t)tile * C.nchunks + chunk) * 3 * T);
-    if (last_block(C.counter + tile * 8 + 1, C.nchunks) && threadIdx.x < T) {
-        const int t = threadIdx.x, smp = tile * T + t;
-        if (smp < C.nsamp && C.st[smp].active) {
-            SampleState &S = C.st[smp];
-            const double v = sum_partials<T>(C, tile, 0, t);
-            S.pq = v;
-            if (!(v > 0) || !isfinite(v)) {
-                S.status = MSA_BREAKDOWN;
-                S.active = 0;
-                atomicSub(active_count, 1);
-            } else {
-                S.alpha = S.rz / v;
-            }
-        }
-    }
 }
 
 // Update + preconditioner + stopping test (SolverPCG.cs:122-136):
@@ -620,34 +550,6 @@ __global__ __launch_bounds__(256) void k_pcg_b(const ClassDev *__restrict__ cls,
     }
     double acc[2] = {(double)rz, (double)rr2};
     block_partials<T, 2>(acc, C.partial + ((size_t)tile * C.nchunks + chunk) * 3 * T);
-    if (last_block(C.counter + tile * 8 + 2, C.nchunks) && threadIdx.x < T) {
-        const int t = threadIdx.x, smp = tile * T + t;
-        if (smp < C.nsamp && C.st[smp].active) {
-            SampleState &S = C.st[smp];
-            const double rzn = sum_partials<T>(C, tile, 0, t);
-            const double rrn = sum_partials<T>(C, tile, 1, t);
-            S.iters += 1;
-            S.rr = rrn;
-            const double err = sqrt(rrn) / S.bnorm;
-            S.relres = err;
-            if (!isfinite(err) || !isfinite(rzn)) {
-                S.status = MSA_BREAKDOWN;
-                S.active = 0;
-                atomicSub(active_count, 1);
-            } else if (err < tol) {
-                S.status = MSA_OK;
-                S.active = 0;
-                atomicSub(active_count, 1);
-            } else if (S.iters >= max_it) {
-                S.status = MSA_NOT_CONVERGED;
-                S.active = 0;
-                atomicSub(active_count, 1);
-            } else {
-                S.beta = rzn / S.rz;
-                S.rz = rzn;
-            }
-        }
-    }
 }
 
 // y = K x for every lane of one tile (tests and the single-product entry).
@@ -868,33 +770,6 @@ __global__ __launch_bounds__(256) void k_ir_norm(const ClassDev *__restrict__ cl
         }
     }
     block_partials<T, 1>(acc, C.partial + ((size_t)tile * C.nchunks + chunk) * 3 * T);
-    if (last_block(C.counter + tile * 8 + 3, C.nchunks) && threadIdx.x < T) {
-        const int t = threadIdx.x, smp = tile * T + t;
-        if (smp < C.nsamp && C.st[smp].ir_active) {
-            SampleState &S = C.st[smp];
-            const double nr = sqrt(sum_partials<T>(C, tile, 0, t));
-            S.total_iters += S.iters;
-            S.outer += 1;
-            S.r64norm = nr;
-            const double err = nr / C.fnorm;
-            S.relres = err;
-            if (!isfinite(err)) {
-                S.status = MSA_BREAKDOWN;
-                S.ir_active = 0;
-                atomicSub(ir_count, 1);
-            } else if (err < tol) {
-                S.status = MSA_OK;
-                S.ir_active = 0;
-                atomicSub(ir_count, 1);
-            } else if (S.outer >= max_outer) {
-                S.status = MSA_NOT_CONVERGED;
-                S.ir_active = 0;
-                atomicSub(ir_count, 1);
-            } else {
-                S.scale = nr;
-            }
-        }
-    }
 }
 
 // ------------------------------------------------------------ reliable updates
@@ -968,40 +843,6 @@ __global__ __launch_bounds__(256) void k_ru_norm(const ClassDev *__restrict__ cl
         }
     }
     block_partials<T, 1>(acc, C.partial + ((size_t)tile * C.nchunks + chunk) * 3 * T);
-    if (last_block(C.counter + tile * 8 + 4, C.nchunks) && threadIdx.x < T) {
-        const int t = threadIdx.x, smp = tile * T + t;
-        if (smp < C.nsamp && C.st[smp].ir_active) {
-            SampleState &S = C.st[smp];
-            const double nr = sqrt(sum_partials<T>(C, tile, 0, t));
-            S.outer += 1;
-            S.total_iters = S.iters;
-            S.r64norm = nr;
-            const double err = nr / C.fnorm;
-            S.relres = err;
-            int done = 0;
-            if (!isfinite(err)) {
-                S.status = MSA_BREAKDOWN;
-                done = 1;
-            } else if (err < tol) {
-                S.status = MSA_OK;
-                done = 1;
-            } else if (S.iters >= max_it || !S.active) {
-                if (S.status != MSA_BREAKDOWN) S.status = MSA_NOT_CONVERGED;
-                done = 1;
-            }
-            if (done) {
-                S.ir_active = 0;
-                atomicSub(ir_count, 1);
-                if (S.active) {
-                    S.active = 0;
-                    atomicSub(active_count, 1);
-                }
-            } else {
-                S.pq = S.scale / nr;  // rescale ratio of the search direction
-                S.scale = nr;
-            }
-        }
-    }
 }
 
 // Restart of the FP32 recurrence from the FP64 residual: r = r64/|r64|,
@@ -1044,10 +885,6 @@ __global__ __launch_bounds__(256) void k_ru_restart(const ClassDev *__restrict__
         }
     }
     block_partials<T, 1>(acc, C.partial + ((size_t)tile * C.nchunks + chunk) * 3 * T);
-    if (last_block(C.counter + tile * 8 + 5, C.nchunks) && threadIdx.x < T) {
-        const int t = threadIdx.x, smp = tile * T + t;
-        if (smp < C.nsamp && C.st[smp].ir_active) C.st[smp].rz = sum_partials<T>(C, tile, 0, t);
-    }
 }
 
 // ------------------------------------------------------------ KL field
@@ -1118,6 +955,186 @@ __global__ void k_kl_eval(const ClassDev *__restrict__ cp, int ntiles, int n_ele
     C.E[id] = exp(res);
 }
 
+
+// ------------------------------------------------------------ reductions
+// Every PCG-type kernel leaves per-sample partial sums, one per work item
+// (chunk).  k_reduce (one 1024-thread workgroup per class tile, launched
+// right after) sums them in a fixed order -- strided per thread, then a
+// binary tree over the thread groups -- and applies the per-sample state
+// update of `op`.  The kernel boundary makes the partials visible; no
+// floating-point atomics, so results are reproducible run to run.
+
+__device__ void apply_op(const ClassDev &C, int smp, const double *v, const ReduceArgs &a)
+{
+    SampleState &S = C.st[smp];
+    switch (a.op) {
+    case OP_INIT0:
+    case OP_INIT1: {  // start of a PCG solve: rz = r.z, |b|
+        const bool on = (a.op == OP_INIT0) || S.ir_active;
+        S.iters = 0;
+        S.beta = 0;
+        if (!on) {
+            S.active = 0;
+            break;
+        }
+        S.rz = v[0];
+        S.bnorm = sqrt(v[1]);
+        S.relres = 1.0;
+        if (!isfinite(v[0]) || !isfinite(v[1])) {
+            S.status = MSA_NAN_INPUT;
+            S.active = 0;
+        } else if (S.bnorm > 1e-25) {
+            S.status = MSA_NOT_CONVERGED;
+            S.active = 1;
+            atomicAdd(a.active_count, 1);
+        } else {
+            S.status = MSA_OK;
+            S.relres = 0;
+            S.active = 0;
+        }
+        break;
+    }
+    case OP_A:  // alpha = rz / p.q
+        if (!S.active) break;
+        S.pq = v[0];
+        if (!(v[0] > 0) || !isfinite(v[0])) {
+            S.status = MSA_BREAKDOWN;
+            S.active = 0;
+            atomicSub(a.active_count, 1);
+        } else {
+            S.alpha = S.rz / v[0];
+        }
+        break;
+    case OP_B: {  // |r|/|b| < tol, else beta = rz_new / rz (SolverPCG.cs:130-136)
+        if (!S.active) break;
+        S.iters += 1;
+        S.rr = v[1];
+        const double err = sqrt(v[1]) / S.bnorm;
+        S.relres = err;
+        if (!isfinite(err) || !isfinite(v[0])) {
+            S.status = MSA_BREAKDOWN;
+            S.active = 0;
+            atomicSub(a.active_count, 1);
+        } else if (err < a.tol) {
+            S.status = MSA_OK;
+            S.active = 0;
+            atomicSub(a.active_count, 1);
+        } else if (S.iters >= a.max_it) {
+            S.status = MSA_NOT_CONVERGED;
+            S.active = 0;
+            atomicSub(a.active_count, 1);
+        } else {
+            S.beta = v[0] / S.rz;
+            S.rz = v[0];
+        }
+        break;
+    }
+    case OP_IRNORM: {  // restarted refinement: |f - K u|/|f| < tol
+        if (!S.ir_active) break;
+        const double nr = sqrt(v[0]);
+        S.total_iters += S.iters;
+        S.outer += 1;
+        S.r64norm = nr;
+        const double err = nr / C.fnorm;
+        S.relres = err;
+        if (!isfinite(err)) {
+            S.status = MSA_BREAKDOWN;
+            S.ir_active = 0;
+            atomicSub(a.ir_count, 1);
+        } else if (err < a.tol) {
+            S.status = MSA_OK;
+            S.ir_active = 0;
+            atomicSub(a.ir_count, 1);
+        } else if (S.outer >= a.max_outer) {
+            S.status = MSA_NOT_CONVERGED;
+            S.ir_active = 0;
+            atomicSub(a.ir_count, 1);
+        } else {
+            S.scale = nr;
+        }
+        break;
+    }
+    case OP_RUNORM: {  // residual replacement: FP64 stopping test, rescale ratio
+        if (!S.ir_active) break;
+        const double nr = sqrt(v[0]);
+        S.outer += 1;
+        S.total_iters = S.iters;
+        S.r64norm = nr;
+        const double err = nr / C.fnorm;
+        S.relres = err;
+        int done = 0;
+        if (!isfinite(err)) {
+            S.status = MSA_BREAKDOWN;
+            done = 1;
+        } else if (err < a.tol) {
+            S.status = MSA_OK;
+            done = 1;
+        } else if (S.iters >= a.max_it || !S.active) {
+            if (S.status != MSA_BREAKDOWN) S.status = MSA_NOT_CONVERGED;
+            done = 1;
+        }
+        if (done) {
+            S.ir_active = 0;
+            atomicSub(a.ir_count, 1);
+            if (S.active) {
+                S.active = 0;
+                atomicSub(a.active_count, 1);
+            }
+        } else {
+            S.pq = S.scale / nr;  // rescale ratio of the search direction
+            S.scale = nr;
+        }
+        break;
+    }
+    case OP_RURESTART:
+        if (S.ir_active) S.rz = v[0];
+        break;
+    }
+}
+
+template <int T>
+__global__ __launch_bounds__(1024) void k_reduce(const ClassDev *__restrict__ cls,
+                                                 const int2 *__restrict__ tiles, ReduceArgs a)
+{
+    constexpr int G = 1024 / T;
+    __shared__ double red[2][1024];
+    const int2 ct = tiles[blockIdx.x];
+    const ClassDev &C = cls[ct.x];
+    const int tile = ct.y;
+    const int t = threadIdx.x % T, g = threadIdx.x / T;
+    const bool two = a.op == OP_INIT0 || a.op == OP_INIT1 || a.op == OP_B;
+    const double *base = C.partial + (size_t)tile * C.nchunks * 3 * T + t;
+    double a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+    int c = g;
+    for (; c + G < C.nchunks; c += 2 * G) {
+        a0 += base[(size_t)c * 3 * T];
+        b0 += base[(size_t)(c + G) * 3 * T];
+        if (two) {
+            a1 += base[(size_t)c * 3 * T + T];
+            b1 += base[(size_t)(c + G) * 3 * T + T];
+        }
+    }
+    if (c < C.nchunks) {
+        a0 += base[(size_t)c * 3 * T];
+        if (two) a1 += base[(size_t)c * 3 * T + T];
+    }
+    red[0][threadIdx.x] = a0 + b0;
+    red[1][threadIdx.x] = a1 + b1;
+    __syncthreads();
+    for (int stride = G / 2; stride >= 1; stride >>= 1) {
+        if (g < stride) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + stride * T];
+            red[1][threadIdx.x] += red[1][threadIdx.x + stride * T];
+        }
+        __syncthreads();
+    }
+    if (g == 0) {
+        const int smp = tile * T + t;
+        const double v[2] = {red[0][t], red[1][t]};
+        if (smp < C.nsamp) apply_op(C, smp, v, a);
+    }
+}
+
 // ------------------------------------------------------------ utilities
 
 template <typename V>
@@ -1126,6 +1143,13 @@ __global__ void k_fill(V *p, size_t n, V v)
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) p[i] = v;
+}
+
+__global__ void k_round(const double *__restrict__ src, size_t n, float *__restrict__ dst)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = (float)src[i];
 }
 
 template <typename real>
@@ -1204,17 +1228,31 @@ hipError_t launch_h8k(hipStream_t s, const double *E, const double *B, const dou
     default: return hipErrorInvalidValue;                                         \
     }
 
-hipError_t launch_assemble(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
-                           const ColourLaunch &cl, int nblocks, double nu)
+hipError_t launch_assemble(hipStream_t s, int kprec, int T, const ClassDev *d_cls,
+                           const ColourLaunch &cl, int nblocks, double nu, int tile_lo,
+                           int tile_hi, double *stage)
 {
     if (nblocks <= 0) return hipSuccess;
-    if (mixed) {
+    if (kprec == 2) {
         MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_assemble<float, float, TT>), dim3(nblocks),
-                                             dim3(64), 0, s, d_cls, cl, nu));
+                                             dim3(64), 0, s, d_cls, cl, nu, tile_lo, tile_hi,
+                                             (float *)nullptr));
+    } else if (kprec == 1) {
+        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_assemble<double, float, TT>), dim3(nblocks),
+                                             dim3(64), 0, s, d_cls, cl, nu, tile_lo, tile_hi,
+                                             (float *)nullptr));
     } else {
         MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_assemble<double, double, TT>), dim3(nblocks),
-                                             dim3(64), 0, s, d_cls, cl, nu));
+                                             dim3(64), 0, s, d_cls, cl, nu, tile_lo, tile_hi,
+                                             stage));
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_round(hipStream_t s, const double *src, size_t n, float *dst)
+{
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_round, dim3(grid1d(n, 256)), dim3(256), 0, s, src, n, dst);
     return hipGetLastError();
 }
 
@@ -1313,6 +1351,15 @@ hipError_t launch_ru_restart(hipStream_t s, int T, const ClassDev *d_cls, const 
                              int nitems)
 {
     MSA_T_LAUNCH(k_ru_restart, d_cls, items, nitems);
+}
+
+hipError_t launch_reduce(hipStream_t s, int T, const ClassDev *d_cls, const int2 *tiles,
+                         int ntiles, const ReduceArgs &a)
+{
+    if (ntiles <= 0) return hipSuccess;
+    MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_reduce<TT>), dim3(ntiles), dim3(1024), 0, s, d_cls,
+                                         tiles, a));
+    return hipGetLastError();
 }
 
 hipError_t launch_kl_basis(hipStream_t s, const ClassDev *d_cls, int n_elem, int M, int kmax,

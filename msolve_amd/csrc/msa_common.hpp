@@ -102,7 +102,6 @@ struct ClassDev {
     double *xi;           // [ntiles*T][M]
     SampleState *st;      // [ntiles*T]
     double *partial;      // [ntiles][nchunks][3][T]
-    unsigned *counter;    // [ntiles][8]
     int *negdet;          // [1]
     int colour_ptr[kMaxColours + 1];
 };
@@ -115,6 +114,19 @@ struct ColourLaunch {
     int prefix[33];
 };
 
+// State updates applied by the reduction kernel after each partial-sum kernel.
+enum ReduceOp { OP_INIT0 = 0, OP_INIT1, OP_A, OP_B, OP_IRNORM, OP_RUNORM, OP_RURESTART };
+
+struct ReduceArgs {
+    int op;
+    int max_it;
+    int max_outer;
+    int pad;
+    double tol;
+    int *active_count;
+    int *ir_count;
+};
+
 enum KernelKind { KK_SPMV = 0, KK_UPDATE = 1, KK_ASSEMBLE = 2, KK_RES64 = 3, KK_OTHER = 4 };
 
 // Launch wrappers (kernels.hip).  T is the lane tile of the batch.
@@ -125,8 +137,12 @@ hipError_t launch_ke_packed(hipStream_t s, int n_elem, const double *xyz8, const
                             double nu, double *ke300, int *status);
 hipError_t launch_gauss_matrices(hipStream_t s, const double *xyz, double *out);
 hipError_t launch_h8k(hipStream_t s, const double *E, const double *B, const double *w, double *K);
-hipError_t launch_assemble(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
-                           const ColourLaunch &cl, int nblocks, double nu);
+// kprec: 0 FP64 integration -> FP64 values, 1 FP64 integration -> FP32
+// values, 2 FP32 integration -> FP32 values
+hipError_t launch_assemble(hipStream_t s, int kprec, int T, const ClassDev *d_cls,
+                           const ColourLaunch &cl, int nblocks, double nu, int tile_lo,
+                           int tile_hi, double *stage);
+hipError_t launch_round(hipStream_t s, const double *src, size_t n, float *dst);
 hipError_t launch_residual64(hipStream_t s, int T, const ClassDev *d_cls, const ColourLaunch &cl,
                              int nblocks, double nu);
 hipError_t launch_dinv(hipStream_t s, bool mixed, int T, const ClassDev *d_cls, const int4 *items,
@@ -151,6 +167,8 @@ hipError_t launch_ru_restart(hipStream_t s, int T, const ClassDev *d_cls, const 
                              int nitems);
 hipError_t launch_spmv(hipStream_t s, bool mixed, int T, const ClassDev *d_cls, const int4 *items,
                        int nitems, const double *x, double *y);
+hipError_t launch_reduce(hipStream_t s, int T, const ClassDev *d_cls, const int2 *tiles,
+                         int ntiles, const ReduceArgs &a);
 hipError_t launch_kl_basis(hipStream_t s, const ClassDev *d_cls, int n_elem, int M, int kmax,
                            const double *w1d, const double *a1d, const int *evpPos,
                            const double *eigs, const double *domain, double sigma);

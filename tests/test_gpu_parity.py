@@ -104,27 +104,36 @@ def partial_bc_model(n=3):
     return m
 
 
-@pytest.mark.parametrize("precision,tile", [(L.MSA_FP64, 1), (L.MSA_FP64, 4), (L.MSA_FP64, 64),
-                                            (L.MSA_MIXED, 64), (L.MSA_MIXED, 16)])
+@pytest.mark.parametrize("precision,tile,integ", [
+    (L.MSA_FP64, 1, None), (L.MSA_FP64, 4, None), (L.MSA_FP64, 64, None),
+    (L.MSA_MIXED, 64, L.MSA_FP64), (L.MSA_MIXED, 16, L.MSA_FP64), (L.MSA_MIXED, 64, L.MSA_MIXED)])
 @pytest.mark.parametrize("bc", ["cantilever", "partial"])
-def test_assembly_matches_oracle(ctx, precision, tile, bc):
-    print(f"\n[case] assembly precision={precision} tile={tile} bc={bc}", flush=True)
+def test_assembly_matches_oracle(ctx, precision, tile, integ, bc):
+    print(f"\n[case] assembly precision={precision} tile={tile} integ={integ} bc={bc}", flush=True)
     model = msa.cantilever(4, 3, 2) if bc == "cantilever" else partial_bc_model(3)
     rng = np.random.default_rng(7)
     ns = 3
     E = lognormal_E(rng, ns * model.n_elem).reshape(ns, -1)
     mesh = msa.Mesh(ctx, model.xyz, model.conn, model.dof, model.n_free)
     b = msa.Batch(ctx, [mesh], [ns], precision, tile)
+    if integ is not None:
+        b.set_integration(integ)
     b.set_modulus(0, E)
     b.set_rhs(0, model.f)
     b.assemble(0.3)
-    bar = 1e-13 if precision == L.MSA_FP64 else 2e-6
+    # FP64: 1e-13; FP32 values rounded once from the FP64 sum: 6e-8 of each
+    # value (plus the FP64 differences); FP32 integration: 2e-6 of the largest
+    bar = 1e-13 if precision == L.MSA_FP64 else (6.1e-8 if integ == L.MSA_FP64 else 2e-6)
     for s in range(ns):
         ke, _ = O.ke_mesh(model.conn, model.xyz, E[s], 0.3)
         rp, col, val = O.csr_assemble(model.conn, model.dof, model.n_free, ke)
         grp, gcol, gval = b.matrix_csr(0, s)
         assert np.array_equal(grp, rp) and np.array_equal(gcol, col)
-        assert np.abs(gval - val).max() <= bar * np.abs(val).max()
+        if precision == L.MSA_MIXED and integ == L.MSA_FP64:
+            nz = val != 0
+            assert (np.abs(gval - val)[nz] <= bar * np.abs(val)[nz]).all()
+        else:
+            assert np.abs(gval - val).max() <= bar * np.abs(val).max()
         x = rng.standard_normal(model.n_free)
         y = b.spmv(0, s, x)
         yref = O.csr_multiply(rp, col, val, x)
