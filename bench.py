@@ -57,9 +57,12 @@ def parse():
     p.add_argument("--max-outer", type=int, default=30)
     p.add_argument("--graph-iters", type=int, default=16)
     p.add_argument("--tile", type=int, default=64)
-    p.add_argument("--period", type=int, default=8, help="FP64 residual replacement period")
+    p.add_argument("--period", type=int, default=4, help="replacement check period (iterations)")
+    p.add_argument("--drop", type=float, default=0.1, help="FP32 residual drop that triggers a replacement")
     p.add_argument("--refine", choices=["reliable", "restart"], default="reliable")
     p.add_argument("--only-mixed", action="store_true", help="skip every comparison run")
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                   help="only run a device copy of this many MB (HBM counter calibration)")
     p.add_argument("--seed", type=int, default=20261202)
     p.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison runs")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -80,7 +83,7 @@ def make_batch(msa, ctx, meshes, models, counts, precision, tile, table, xi, arg
     b = msa.Batch(ctx, meshes, counts, precision, tile)
     if precision == msa.MSA_MIXED:
         b.set_refinement(msa.MSA_REFINE_RELIABLE if args.refine == "reliable"
-                         else msa.MSA_REFINE_RESTART, args.period)
+                         else msa.MSA_REFINE_RESTART, args.period, args.drop)
     off = 0
     for c, m in enumerate(models):
         b.set_kl(c, table, m.domain, 2.1e5, args.cov)
@@ -110,6 +113,16 @@ def kernel_a_bytes(b, models, counts, rs, launches):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, device="cuda", dtype=torch.float32)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": 4 * n, "launches": 3}))
+        return
     if args.only_mixed:
         args.no_fp64 = args.no_cpu = args.no_single = True
     if args.gpus != 1:

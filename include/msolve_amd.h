@@ -172,13 +172,16 @@ int msa_batch_solve(msa_batch *b, double tol, int max_it, double inner_tol,
 
 /* Refinement scheme of MIXED solves (no counterpart in the reference, which
  * is FP64 only).  MSA_REFINE_RELIABLE (default): the FP32 PCG never restarts;
- * every `period` iterations (even) the FP32 correction is folded into the FP64
- * solution, the residual is replaced by the FP64 f - K u, the stopping test
- * |f - K u|/|f| < tol runs on it and the FP32 recurrence continues from it
- * (search direction rescaled); inner_tol and max_outer are unused.
- * MSA_REFINE_RESTART: classic refinement, each outer step solves
- * K d = r/|r| in FP32 to inner_tol from zero. */
-int msa_batch_set_refinement(msa_batch *b, int method, int period);
+ * a sample whose FP32 residual has fallen by `drop` (default 0.1) since its
+ * last replacement -- or whose estimated FP64 residual passes tol -- gets,
+ * at the next check (every `period` iterations, even, default 4), its FP32
+ * correction folded into the FP64 solution, its residual replaced by the
+ * FP64 f - K u, the stopping test |f - K u|/|f| < tol, and the FP32
+ * recurrence continued from that residual (search direction rescaled);
+ * inner_tol and max_outer are unused.  MSA_REFINE_RESTART: classic
+ * refinement, each outer step solves K d = r/|r| in FP32 to inner_tol from
+ * zero (period and drop unused). */
+int msa_batch_set_refinement(msa_batch *b, int method, int period, double drop);
 
 /* Element integration precision of a MIXED batch, whose stiffness values are
  * stored in FP32: MSA_FP64 (default) integrates in FP64 and rounds each

@@ -100,7 +100,7 @@ def lib():
         "msa_batch_get_kernel_timing": [_vp, _dp, _llp],
         "msa_batch_spmv_bytes": [_vp, _dp],
         "msa_autotune_tile": [_vp, _vp, _i, _i, _ip, _dp],
-        "msa_batch_set_refinement": [_vp, _i, _i],
+        "msa_batch_set_refinement": [_vp, _i, _i, _d],
         "msa_batch_set_integration": [_vp, _i],
     }
     for name, args in sig.items():

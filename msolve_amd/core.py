@@ -198,8 +198,8 @@ class Batch:
         check(lib().msa_batch_spmv(self.h, cls, sample, ptr(x, C.c_double), ptr(y, C.c_double)))
         return y
 
-    def set_refinement(self, method=L.MSA_REFINE_RELIABLE, period: int = 8):
-        check(lib().msa_batch_set_refinement(self.h, method, period))
+    def set_refinement(self, method=L.MSA_REFINE_RELIABLE, period: int = 4, drop: float = 0.1):
+        check(lib().msa_batch_set_refinement(self.h, method, period, drop))
 
     def set_integration(self, precision=L.MSA_FP64):
         check(lib().msa_batch_set_integration(self.h, precision))

@@ -71,7 +71,8 @@ struct msa_batch {
     // mixed-precision refinement: restarted inner solves or periodic
     // residual replacement
     int refine = MSA_REFINE_RELIABLE;
-    int period = 8;
+    int period = 4;       // iterations between replacement checks
+    double drop = 0.1;    // FP32 residual reduction that triggers a replacement
     // element integration precision of MIXED batches (values stored in FP32)
     int integration = MSA_FP64;
     double *stage = nullptr;  // FP64 staging tile for MIXED + FP64 integration
@@ -134,13 +135,16 @@ int timed(msa_batch *b, int kind, F &&launch)
     return MSA_OK;
 }
 
-int reduce(msa_batch *b, int op, double tol = 0.0, int max_it = 0, int max_outer = 0)
+int reduce(msa_batch *b, int op, double tol = 0.0, int max_it = 0, int max_outer = 0,
+           double ru_drop = 0.0, double ru_tol = 0.0)
 {
     ReduceArgs a{};
     a.op = op;
     a.tol = tol;
     a.max_it = max_it;
     a.max_outer = max_outer;
+    a.ru_drop = ru_drop;
+    a.ru_tol = ru_tol;
     a.active_count = b->d_counts;
     a.ir_count = b->d_counts + 1;
     return timed(b, KK_OTHER, [&] {
@@ -167,7 +171,8 @@ int read_count(msa_batch *b, int which, int *out)
     return MSA_OK;
 }
 
-int launch_iteration_pair(msa_batch *b, int pbuf, double tol, int max_it)
+// ru_tol > 0: reliable-update mode (inner test off, replacement triggers on).
+int launch_iteration_pair(msa_batch *b, int pbuf, double tol, int max_it, double ru_tol = 0.0)
 {
     const bool mx = is_mixed(b);
     hipStream_t s = b->ctx->stream;
@@ -180,7 +185,10 @@ int launch_iteration_pair(msa_batch *b, int pbuf, double tol, int max_it)
         return launch_pcg_b(s, mx, b->T, b->d_cls, b->d_items, (int)b->items.size(), pbuf, tol,
                             max_it, b->d_counts);
     }));
-    MSA_TRY(reduce(b, OP_B, tol, max_it));
+    if (ru_tol > 0)
+        MSA_TRY(reduce(b, OP_B, 0.0, max_it, 0, b->drop, ru_tol));
+    else
+        MSA_TRY(reduce(b, OP_B, tol, max_it));
     return MSA_OK;
 }
 
@@ -198,7 +206,7 @@ int launch_replacement(msa_batch *b, double tol, int max_it)
     for (int k = 0; k < ncol; k++) {
         int nblk = 0;
         ColourLaunch cl = colour_launch(b, k, &nblk);
-        MSA_TRY(timed(b, KK_RES64, [&] { return launch_residual64(s, T, b->d_cls, cl, nblk, b->nu); }));
+        MSA_TRY(timed(b, KK_RES64, [&] { return launch_residual64(s, T, b->d_cls, cl, nblk, b->nu, 1); }));
     }
     MSA_TRY(timed(b, KK_OTHER, [&] {
         return launch_ru_norm(s, T, b->d_cls, b->d_items, ni, tol, max_it, b->d_counts + 1,
@@ -215,7 +223,7 @@ int launch_replacement(msa_batch *b, double tol, int max_it)
 int launch_block(msa_batch *b, int kind, int iters, double tol, int max_it)
 {
     for (int k = 0; k < iters; k++) {
-        MSA_TRY(launch_iteration_pair(b, k & 1, kind == 1 ? 0.0 : tol, max_it));
+        MSA_TRY(launch_iteration_pair(b, k & 1, kind == 1 ? 0.0 : tol, max_it, kind == 1 ? tol : 0.0));
         if (kind == 1 && (k + 1) % b->period == 0) MSA_TRY(launch_replacement(b, tol, max_it));
     }
     return MSA_OK;
@@ -956,7 +964,7 @@ int msa_batch_solve(msa_batch *b, double tol, int max_it, double inner_tol, int 
         for (int k = 0; k < ncol; k++) {
             int nblk = 0;
             ColourLaunch cl = colour_launch(b, k, &nblk);
-            MSA_TRY(timed(b, KK_RES64, [&] { return launch_residual64(s, T, b->d_cls, cl, nblk, b->nu); }));
+            MSA_TRY(timed(b, KK_RES64, [&] { return launch_residual64(s, T, b->d_cls, cl, nblk, b->nu, 0); }));
         }
         MSA_TRY(timed(b, KK_OTHER, [&] {
             return launch_ir_norm(s, T, b->d_cls, b->d_items, ni, tol, max_outer, b->d_counts + 1);
@@ -1126,15 +1134,17 @@ int msa_batch_spmv(msa_batch *b, int cls, int sample, const double *x, double *y
     return MSA_OK;
 }
 
-int msa_batch_set_refinement(msa_batch *b, int method, int period)
+int msa_batch_set_refinement(msa_batch *b, int method, int period, double drop)
 {
     MSA_CHECK(b, MSA_ERR_ARG, "null batch");
     MSA_CHECK(method == MSA_REFINE_RESTART || method == MSA_REFINE_RELIABLE, MSA_ERR_ARG,
               "refinement method %d", method);
     MSA_CHECK(period >= 2 && period % 2 == 0 && period <= 4096, MSA_ERR_ARG,
-              "replacement period %d must be even, 2..4096", period);
+              "replacement check period %d must be even, 2..4096", period);
+    MSA_CHECK(drop > 0 && drop <= 1, MSA_ERR_ARG, "replacement trigger %g outside (0, 1]", drop);
     b->refine = method;
     b->period = period;
+    b->drop = drop;
     return MSA_OK;
 }
 

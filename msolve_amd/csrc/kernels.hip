@@ -478,10 +478,20 @@ __global__ __launch_bounds__(256) void k_pcg_a(const ClassDev *__restrict__ cls,
                     const real v0 = z[xo] + beta * po[xo];
                     const real v1 = z[xo + T] + beta * po[xo + T];
                     const real v2 = z[xo + 2 * T] + beta * po[xo + 2 * T];
+                    // stiffness values are streamed once per product: load them
+                    // non-temporally so L2 keeps the reused vector gathers
                     const real *m = val + ((size_t)slot * 9 * R + rr) * T + s;
-                    a0 += m[0] * v0 + m[ES] * v1 + m[2 * ES] * v2;
-                    a1 += m[3 * ES] * v0 + m[4 * ES] * v1 + m[5 * ES] * v2;
-                    a2 += m[6 * ES] * v0 + m[7 * ES] * v1 + m[8 * ES] * v2;
+                    const real m0 = __builtin_nontemporal_load(m), m1 = __builtin_nontemporal_load(m + ES),
+                               m2 = __builtin_nontemporal_load(m + 2 * ES);
+                    const real m3 = __builtin_nontemporal_load(m + 3 * ES),
+                               m4 = __builtin_nontemporal_load(m + 4 * ES),
+                               m5 = __builtin_nontemporal_load(m + 5 * ES);
+                    const real m6 = __builtin_nontemporal_load(m + 6 * ES),
+                               m7 = __builtin_nontemporal_load(m + 7 * ES),
+                               m8 = __builtin_nontemporal_load(m + 8 * ES);
+                    a0 += m0 * v0 + m1 * v1 + m2 * v2;
+                    a1 += m3 * v0 + m4 * v1 + m5 * v2;
+                    a2 += m6 * v0 + m7 * v1 + m8 * v2;
                 }
                 const size_t o = (size_t)br * 3 * T + s;
                 const real n0 = z[o] + beta * po[o];
@@ -684,9 +694,10 @@ __global__ __launch_bounds__(256) void k_ir_update(const ClassDev *__restrict__ 
 // FP64 residual r64 -= K u64, matrix-free: one wavefront per element of the
 // current colour recomputes Khat in FP64 (LDS) and applies E_s Khat to the
 // element's displacements of every active sample.
+// need_due = 1: only samples with a pending replacement (reliable updates).
 template <int T>
 __global__ __launch_bounds__(64) void k_residual64(const ClassDev *__restrict__ cls,
-                                                   ColourLaunch cl, double nu)
+                                                   ColourLaunch cl, double nu, int need_due)
 {
     constexpr int R = 64 / T;
     __shared__ double Kh[576];
@@ -699,7 +710,8 @@ __global__ __launch_bounds__(64) void k_residual64(const ClassDev *__restrict__ 
     const int e = C.colour_elems[C.colour_ptr[cl.colour] + (item - cl.prefix[c])];
     const int lane = threadIdx.x & 63;
     bool any = false;
-    for (int smp = lane; smp < C.nsamp; smp += 64) any |= C.st[smp].ir_active != 0;
+    for (int smp = lane; smp < C.nsamp; smp += 64)
+        any |= C.st[smp].ir_active && (!need_due || C.st[smp].ru_due);
     if (!__ballot(any)) return;
     const int *en = C.conn + (size_t)e * 8;
     const int na = en[lane & 7];
@@ -708,7 +720,7 @@ __global__ __launch_bounds__(64) void k_residual64(const ClassDev *__restrict__ 
     const int s = lane % T, sub = lane / T;
     for (int tile = 0; tile < C.ntiles; tile++) {
         const int smp = tile * T + s;
-        if (smp >= C.nsamp || !C.st[smp].ir_active) continue;
+        if (smp >= C.nsamp || !C.st[smp].ir_active || (need_due && !C.st[smp].ru_due)) continue;
         const size_t vb = (size_t)tile * C.tile_vec;
         double u[24];
 #pragma unroll
@@ -793,7 +805,7 @@ __global__ __launch_bounds__(256) void k_ru_fold(const ClassDev *__restrict__ cl
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int rr = lane / T, s = lane % T;
     const int sample = tile * T + s;
-    if (sample >= C.nsamp || !C.st[sample].ir_active) return;
+    if (sample >= C.nsamp || !C.st[sample].ir_active || !C.st[sample].ru_due) return;
     const double sc = C.st[sample].scale;
     const size_t vb = (size_t)tile * C.tile_vec;
     float *x = (float *)C.x + vb;
@@ -827,7 +839,7 @@ __global__ __launch_bounds__(256) void k_ru_norm(const ClassDev *__restrict__ cl
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int rr = lane / T, s = lane % T;
     const int sample = tile * T + s;
-    const bool act = sample < C.nsamp && C.st[sample].ir_active;
+    const bool act = sample < C.nsamp && C.st[sample].ir_active && C.st[sample].ru_due;
     const size_t vb = (size_t)tile * C.tile_vec;
     double acc[1] = {0};
     if (act) {
@@ -860,7 +872,7 @@ __global__ __launch_bounds__(256) void k_ru_restart(const ClassDev *__restrict__
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int rr = lane / T, s = lane % T;
     const int sample = tile * T + s;
-    const bool act = sample < C.nsamp && C.st[sample].ir_active;
+    const bool act = sample < C.nsamp && C.st[sample].ir_active && C.st[sample].ru_due;
     const size_t vb = (size_t)tile * C.tile_vec;
     double acc[1] = {0};
     if (act) {
@@ -973,6 +985,7 @@ __device__ void apply_op(const ClassDev &C, int smp, const double *v, const Redu
         const bool on = (a.op == OP_INIT0) || S.ir_active;
         S.iters = 0;
         S.beta = 0;
+        S.ru_due = 0;
         if (!on) {
             S.active = 0;
             break;
@@ -1001,6 +1014,7 @@ __device__ void apply_op(const ClassDev &C, int smp, const double *v, const Redu
             S.status = MSA_BREAKDOWN;
             S.active = 0;
             atomicSub(a.active_count, 1);
+            if (S.ir_active) S.ru_due = 1;  // reliable updates: final fold + test
         } else {
             S.alpha = S.rz / v[0];
         }
@@ -1015,6 +1029,7 @@ __device__ void apply_op(const ClassDev &C, int smp, const double *v, const Redu
             S.status = MSA_BREAKDOWN;
             S.active = 0;
             atomicSub(a.active_count, 1);
+            if (a.ru_drop > 0) S.ru_due = 1;  // final fold of the correction
         } else if (err < a.tol) {
             S.status = MSA_OK;
             S.active = 0;
@@ -1023,9 +1038,16 @@ __device__ void apply_op(const ClassDev &C, int smp, const double *v, const Redu
             S.status = MSA_NOT_CONVERGED;
             S.active = 0;
             atomicSub(a.active_count, 1);
+            if (a.ru_drop > 0) S.ru_due = 1;
         } else {
             S.beta = v[0] / S.rz;
             S.rz = v[0];
+            // reliable updates: the FP32 residual (normalised to 1 at the last
+            // replacement) fell by ru_drop, or the FP64 stopping test may pass
+            if (a.ru_drop > 0) {
+                const double rn = sqrt(v[1]);
+                if (rn < a.ru_drop || rn * S.scale < a.ru_tol * C.fnorm) S.ru_due = 1;
+            }
         }
         break;
     }
@@ -1055,7 +1077,7 @@ __device__ void apply_op(const ClassDev &C, int smp, const double *v, const Redu
         break;
     }
     case OP_RUNORM: {  // residual replacement: FP64 stopping test, rescale ratio
-        if (!S.ir_active) break;
+        if (!S.ir_active || !S.ru_due) break;
         const double nr = sqrt(v[0]);
         S.outer += 1;
         S.total_iters = S.iters;
@@ -1075,6 +1097,7 @@ __device__ void apply_op(const ClassDev &C, int smp, const double *v, const Redu
         }
         if (done) {
             S.ir_active = 0;
+            S.ru_due = 0;
             atomicSub(a.ir_count, 1);
             if (S.active) {
                 S.active = 0;
@@ -1087,7 +1110,10 @@ __device__ void apply_op(const ClassDev &C, int smp, const double *v, const Redu
         break;
     }
     case OP_RURESTART:
-        if (S.ir_active) S.rz = v[0];
+        if (S.ir_active && S.ru_due) {
+            S.rz = v[0];
+            S.ru_due = 0;
+        }
         break;
     }
 }
@@ -1257,11 +1283,11 @@ hipError_t launch_round(hipStream_t s, const double *src, size_t n, float *dst)
 }
 
 hipError_t launch_residual64(hipStream_t s, int T, const ClassDev *d_cls, const ColourLaunch &cl,
-                             int nblocks, double nu)
+                             int nblocks, double nu, int need_due)
 {
     if (nblocks <= 0) return hipSuccess;
     MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_residual64<TT>), dim3(nblocks), dim3(64), 0, s,
-                                         d_cls, cl, nu));
+                                         d_cls, cl, nu, need_due));
     return hipGetLastError();
 }
 

@@ -54,7 +54,7 @@ struct SampleState {
     double rr, bnorm, relres;    // |r|^2, |b|, |r|/|b|
     double r64norm, scale;       // mixed: |f - K u|, inner right-hand-side scale
     int iters, status, active;   // inner PCG
-    int outer, total_iters, ir_active, negdet;
+    int outer, total_iters, ir_active, ru_due;  // ru_due: FP64 replacement pending
 };
 
 // Host-side mesh analysis (mesh.cpp).
@@ -123,6 +123,8 @@ struct ReduceArgs {
     int max_outer;
     int pad;
     double tol;
+    double ru_drop;  // > 0: reliable-update mode of OP_B (trigger factor)
+    double ru_tol;   // FP64 stopping tolerance seen by OP_B in that mode
     int *active_count;
     int *ir_count;
 };
@@ -144,7 +146,7 @@ hipError_t launch_assemble(hipStream_t s, int kprec, int T, const ClassDev *d_cl
                            int tile_hi, double *stage);
 hipError_t launch_round(hipStream_t s, const double *src, size_t n, float *dst);
 hipError_t launch_residual64(hipStream_t s, int T, const ClassDev *d_cls, const ColourLaunch &cl,
-                             int nblocks, double nu);
+                             int nblocks, double nu, int need_due);
 hipError_t launch_dinv(hipStream_t s, bool mixed, int T, const ClassDev *d_cls, const int4 *items,
                        int nitems);
 hipError_t launch_pcg_init(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,

@@ -225,7 +225,7 @@ def test_mixed_precision_refinement(ctx, method):
     E = lognormal_E(rng, 3 * model.n_elem, cov=0.3).reshape(3, -1)
     mesh = msa.Mesh(ctx, model.xyz, model.conn, model.dof, model.n_free)
     b = msa.Batch(ctx, [mesh], [3], L.MSA_MIXED, 64)
-    b.set_refinement(method, 8)
+    b.set_refinement(method, 4, 0.1)
     b.set_modulus(0, E)
     b.set_rhs(0, model.f)
     b.assemble()
