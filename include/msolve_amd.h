@@ -116,8 +116,10 @@ int msa_ke_packed(msa_ctx *ctx, int n_elem, const double *xyz8,
  *               MSA_MIXED: FP32 integration/assembly/inner PCG, FP64 outer
  *               residual f - K u (matrix-free), update and stopping test.
  *   (mixed refinement scheme: msa_batch_set_refinement)
- *   tile        samples per lane tile: 1, 2, 4, 8, 16, 32 or 64 (0 = pick:
- *               64 when a class has more than 32 samples, else 1). */
+ *   tile        samples per lane tile: 1, 2, 4, 8, 16, 32 or 64 (0 = 1, the
+ *               fastest for whole solves on MI355X: samples converge at
+ *               different iterations and a wider tile keeps iterating
+ *               until its slowest sample is done). */
 int msa_batch_create(msa_ctx *ctx, int n_classes, msa_mesh *const *meshes,
                      const int *n_samples, int precision, int tile,
                      msa_batch **out);

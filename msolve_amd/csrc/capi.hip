@@ -591,8 +591,9 @@ int msa_batch_create(msa_ctx *ctx, int n_classes, msa_mesh *const *meshes, const
         MSA_CHECK(meshes[c] && meshes[c]->ctx == ctx, MSA_ERR_ARG, "mesh %d invalid", c);
         MSA_CHECK(n_samples[c] >= 1, MSA_ERR_ARG, "n_samples[%d]=%d", c, n_samples[c]);
         maxs = std::max(maxs, n_samples[c]);
+        (void)maxs;
     }
-    if (tile == 0) tile = maxs > 32 ? 64 : 1;
+    if (tile == 0) tile = 1;  // one sample per lane tile: no lane waits for a slower sample
     MSA_CHECK(tile == 1 || tile == 2 || tile == 4 || tile == 8 || tile == 16 || tile == 32 ||
                   tile == 64,
               MSA_ERR_ARG, "tile=%d (1,2,4,...,64)", tile);
