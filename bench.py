@@ -176,6 +176,16 @@ def main():
     avg_s = a_ms / 1e3 / max(launches, 1)
     achieved = avg_bytes / avg_s / 1e9 if avg_s > 0 else 0.0
     log(f"[bench] kernel timing: {json.dumps(kt)}")
+    # HBM traffic of k_pcg_a: rocprofv3 FETCH_SIZE/WRITE_SIZE passes (scaled by a
+    # device-copy calibration) measured bytes / algorithmic bytes, applied to
+    # this run's algorithmic bytes per launch (profiles/k_pcg_a_mixed_montecarlo.md)
+    traffic, traffic_ratio = None, None
+    cpath = os.path.join(ROOT, "profiles", "k_pcg_a_counters.json")
+    if os.path.exists(cpath):
+        cj = json.load(open(cpath))
+        traffic_ratio = (cj["fetch_bytes"] + cj["write_bytes"]) / (cj["algorithmic_read"] +
+                                                                   cj["algorithmic_write"])
+        traffic = avg_bytes * traffic_ratio
     u_mixed = [[bm.solution(c, s) for s in range(counts[c])] for c in range(len(models))]
     iters_mixed = [s["iters"].tolist() for s in stats]
     outer_mixed = [s["outer"].tolist() for s in stats]
@@ -278,8 +288,9 @@ def main():
             "max_rel_err_vs_fp64": max_err,
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_pcg_a<float,64> (SpMV + p update + p.q)",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_over_algorithmic": traffic_ratio,
+                     "kernel": f"k_pcg_a<float,{args.tile}> (SpMV + p update + p.q)",
                      "launches": launches, "avg_launch_ms": 1e3 * avg_s,
                      "avg_bytes_per_launch": avg_bytes},
         "cpu_baseline": cpu,
