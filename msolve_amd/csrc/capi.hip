@@ -75,7 +75,7 @@ struct msa_batch {
     double drop = 0.1;    // FP32 residual reduction that triggers a replacement
     // element integration precision of MIXED batches (values stored in FP32)
     int integration = MSA_FP64;
-    double *stage = nullptr;  // FP64 staging tile for MIXED + FP64 integration
+    double *stage = nullptr;  // FP64 staging tiles for MIXED + FP64 integration
     size_t stage_vals = 0;
     // captured block of PCG iterations (kind 0 plain, 1 with replacement)
     hipGraphExec_t graph = nullptr;
@@ -88,6 +88,16 @@ struct msa_batch {
 namespace {
 
 bool is_mixed(const msa_batch *b) { return b->precision == MSA_MIXED; }
+
+// FP64 staging of a mixed batch's assembly: tiles per pass, at most
+// kStageBytes of FP64 values (at least one tile).
+constexpr size_t kStageBytes = (size_t)1 << 30;
+int stage_tiles(const ClassHost &C)
+{
+    const size_t per = (size_t)C.tile_vals * sizeof(double);
+    const size_t g = per ? kStageBytes / per : 1;
+    return (int)std::max<size_t>(1, std::min<size_t>(g, (size_t)C.ntiles));
+}
 size_t real_size(const msa_batch *b) { return is_mixed(b) ? sizeof(float) : sizeof(double); }
 
 int dalloc(msa_batch *b, void **p, size_t bytes)
@@ -695,10 +705,12 @@ int msa_batch_create(msa_ctx *ctx, int n_classes, msa_mesh *const *meshes, const
         A((void **)&D.st, (size_t)C.ntiles * T * sizeof(SampleState));
         A((void **)&D.partial, (size_t)C.ntiles * C.nchunks * 3 * T * sizeof(double));
         A((void **)&D.negdet, sizeof(int));
+        A((void **)&D.due, (1 + (size_t)C.ntiles * T) * sizeof(int));
         if (rc != MSA_OK) break;
         hipStream_t s = ctx->stream;
         MSA_HIP(hipMemsetAsync(D.st, 0, (size_t)C.ntiles * T * sizeof(SampleState), s));
         MSA_HIP(hipMemsetAsync(D.negdet, 0, sizeof(int), s));
+        MSA_HIP(hipMemsetAsync(D.due, 0, sizeof(int), s));
         MSA_HIP(hipMemsetAsync((void *)D.f, 0, (size_t)h.nb * 3 * sizeof(double), s));
         MSA_HIP(hipMemsetAsync(D.E, 0, (size_t)C.ntiles * h.n_elem * T * sizeof(double), s));
         for (int t = 0; t < C.ntiles; t++) {
@@ -871,11 +883,13 @@ int msa_batch_assemble(msa_batch *b, double nu)
     }
     const int ncol = max_colours(b);
     if (mx && b->integration == MSA_FP64) {
-        // FP64 integration and accumulation of one tile at a time in a staging
-        // buffer, then a single rounding of every value to FP32
+        // FP64 integration and accumulation of up to kStageBytes worth of
+        // tiles at a time in a staging buffer, then a single rounding of every
+        // value to FP32
         size_t need = 0;
-        for (auto &C : b->cls) need = std::max(need, (size_t)C.tile_vals);
-        if (b->stage_vals < need) {
+        for (auto &C : b->cls)
+            need = std::max(need, (size_t)C.tile_vals * stage_tiles(C));
+        if (b->stage_vals < need) {  // sized once: the classes never change
             void *p = nullptr;
             MSA_TRY(dalloc(b, &p, need * sizeof(double)));
             b->stage = (double *)p;
@@ -883,8 +897,11 @@ int msa_batch_assemble(msa_batch *b, double nu)
         }
         for (size_t c = 0; c < b->cls.size(); c++) {
             ClassHost &C = b->cls[c];
-            for (int t = 0; t < C.ntiles; t++) {
-                MSA_TRY(timed(b, KK_OTHER, [&] { return launch_fill_f64(s, b->stage, (size_t)C.tile_vals, 0.0); }));
+            const int G = stage_tiles(C);
+            for (int t = 0; t < C.ntiles; t += G) {
+                const int g = std::min(G, C.ntiles - t);
+                const size_t n = (size_t)g * C.tile_vals;
+                MSA_TRY(timed(b, KK_OTHER, [&] { return launch_fill_f64(s, b->stage, n, 0.0); }));
                 for (int k = 0; k < C.mesh->h.ncolours; k++) {
                     ColourLaunch cl{};
                     cl.ncls = (int)b->cls.size();
@@ -892,12 +909,11 @@ int msa_batch_assemble(msa_batch *b, double nu)
                     const int cnt = C.mesh->h.colour_ptr[k + 1] - C.mesh->h.colour_ptr[k];
                     for (int q = 0; q <= cl.ncls; q++) cl.prefix[q] = q <= (int)c ? 0 : cnt;
                     MSA_TRY(timed(b, KK_ASSEMBLE, [&] {
-                        return launch_assemble(s, 0, b->T, b->d_cls, cl, cnt, nu, t, t + 1, b->stage);
+                        return launch_assemble(s, 0, b->T, b->d_cls, cl, cnt, nu, t, t + g, b->stage);
                     }));
                 }
                 MSA_TRY(timed(b, KK_OTHER, [&] {
-                    return launch_round(s, b->stage, (size_t)C.tile_vals,
-                                        (float *)C.dev.val + (size_t)t * C.tile_vals);
+                    return launch_round(s, b->stage, n, (float *)C.dev.val + (size_t)t * C.tile_vals);
                 }));
             }
         }

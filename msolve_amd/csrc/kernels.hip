@@ -7,8 +7,8 @@
 // rows (rr) times T samples (s).  Every value load reads 64 consecutive words;
 // every gathered vector load reads T consecutive words of one block column.
 // Per-sample dot products are reduced wave -> workgroup -> a partial per
-// work item; the last workgroup of a (class, tile) to finish (atomic ticket)
-// sums the partials in a fixed order, so results do not depend on scheduling.
+// work item; k_reduce, launched next, sums the partials of a (class, tile) in
+// a fixed order, so results do not depend on scheduling.
 #include <cmath>
 
 #include "msa_common.hpp"
@@ -16,6 +16,16 @@
 namespace msa {
 
 // ------------------------------------------------------------ helpers
+
+// Pointers kept in ClassDev are generic; the hot kernels cast them to the
+// global address space so loads and stores compile to global_* (counted by
+// vmcnt only) instead of flat_* instructions.
+#define MSA_GLOBAL __attribute__((address_space(1)))
+template <typename X>
+__device__ __forceinline__ MSA_GLOBAL X *gbl(X *p)
+{
+    return (MSA_GLOBAL X *)p;
+}
 
 // Workgroups are dispatched round-robin over the 8 XCDs; give every XCD a
 // contiguous range of work items so neighbouring slices share its L2.
@@ -60,6 +70,29 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], double *out)
     }
 }
 
+
+// Three components of one node in one-sample tiles (T = 1 layout: v[3*block
+// + d]) move as a single 12- or 24-byte access.
+template <typename real>
+struct vec3 {
+    real x, y, z;
+};
+
+template <typename real>
+__device__ __forceinline__ vec3<real> ld3(const MSA_GLOBAL real *p, size_t i)
+{
+    const MSA_GLOBAL real *q = p + 3 * i;
+    return vec3<real>{q[0], q[1], q[2]};
+}
+
+template <typename real>
+__device__ __forceinline__ void st3(MSA_GLOBAL real *p, size_t i, real x, real y, real z)
+{
+    MSA_GLOBAL real *q = p + 3 * i;
+    q[0] = x;
+    q[1] = y;
+    q[2] = z;
+}
 
 // ------------------------------------------------------------ Hexa8 stiffness
 
@@ -307,8 +340,8 @@ __device__ __forceinline__ int colour_class(const ColourLaunch &cl, int item)
 // Elements of one colour share no node, so the read-modify-writes are
 // race-free.  kreal = precision of the element integration.
 // tile_lo/tile_hi restrict the tiles written; a non-null `stage` receives
-// the single tile tile_lo instead of the class's value array (FP64 staging
-// of a mixed batch, rounded once to FP32 afterwards by k_round).
+// tiles tile_lo.. (consecutive, tile_vals apart) instead of the class's value
+// array (FP64 staging of a mixed batch, rounded once to FP32 by k_round).
 template <typename kreal, typename real, int T>
 __global__ __launch_bounds__(64) void k_assemble(const ClassDev *__restrict__ cls, ColourLaunch cl,
                                                  double nu, int tile_lo, int tile_hi, real *stage)
@@ -333,7 +366,8 @@ __global__ __launch_bounds__(64) void k_assemble(const ClassDev *__restrict__ cl
     const int t_end = min(C.ntiles, tile_hi);
     for (int tile = max(0, tile_lo); tile < t_end; tile++) {
         if (tile * T + s >= C.nsamp) continue;
-        real *val = stage ? stage : (real *)C.val + (size_t)tile * C.tile_vals;
+        real *val = stage ? stage + (size_t)(tile - tile_lo) * C.tile_vals
+                          : (real *)C.val + (size_t)tile * C.tile_vals;
         const real Es = (real)C.E[((size_t)tile * C.n_elem + e) * T + s];
         for (int pe = lane / T; pe < 576; pe += R) {
             const int pair = pe / 9, ent = pe - 9 * pair;
@@ -455,54 +489,88 @@ __global__ __launch_bounds__(256) void k_pcg_a(const ClassDev *__restrict__ cls,
     const bool act = sample < C.nsamp && C.st[sample].active;
     const real beta = act ? (real)C.st[sample].beta : (real)0;
     const size_t vb = (size_t)tile * C.tile_vec;
-    const real *__restrict__ z = (const real *)C.z + vb;
-    const real *__restrict__ po = (const real *)(pbuf ? C.p1 : C.p0) + vb;
-    real *__restrict__ pn = (real *)(pbuf ? C.p0 : C.p1) + vb;
-    real *__restrict__ q = (real *)C.q + vb;
-    const real *__restrict__ val = (const real *)C.val + (size_t)tile * C.tile_vals;
+    using V3 = vec3<real>;
+    const MSA_GLOBAL real *__restrict__ z = gbl((const real *)C.z + vb);
+    const MSA_GLOBAL real *__restrict__ po = gbl((const real *)(pbuf ? C.p1 : C.p0) + vb);
+    MSA_GLOBAL real *__restrict__ pn = gbl((real *)(pbuf ? C.p0 : C.p1) + vb);
+    MSA_GLOBAL real *__restrict__ q = gbl((real *)C.q + vb);
+    const MSA_GLOBAL real *__restrict__ val = gbl((const real *)C.val + (size_t)tile * C.tile_vals);
+    const MSA_GLOBAL int *__restrict__ cidx = gbl(C.cidx);
+    const MSA_GLOBAL int *__restrict__ sptr = gbl(C.slice_ptr);
     real pq = 0;
     const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+    // One slot: gathered p = z + beta p_old of block column col, then the 3x3
+    // block times it.  Padding slots (cidx < 0) hold zero values; they read
+    // block 0 instead of branching, so the loads of several slots overlap.
+    auto gather = [&](int col, real &v0, real &v1, real &v2) {
+        if constexpr (T == 1) {
+            const V3 zc = ld3(z, col);
+            const V3 pc = ld3(po, col);
+            v0 = zc.x + beta * pc.x;
+            v1 = zc.y + beta * pc.y;
+            v2 = zc.z + beta * pc.z;
+        } else {
+            const size_t xo = (size_t)col * 3 * T + s;
+            v0 = z[xo] + beta * po[xo];
+            v1 = z[xo + T] + beta * po[xo + T];
+            v2 = z[xo + 2 * T] + beta * po[xo + 2 * T];
+        }
+    };
+    // stiffness values are streamed once per product: load them
+    // non-temporally so L2 keeps the reused vector gathers
+    auto block = [&](int slot, real (&m)[9]) {
+        const MSA_GLOBAL real *mp = val + ((size_t)slot * 9 * R + rr) * T + s;
+#pragma unroll
+        for (int e = 0; e < 9; e++) m[e] = __builtin_nontemporal_load(mp + e * ES);
+    };
     if (__ballot(act)) {
         for (int slice = s0 + wave; slice < s1; slice += kWaves) {
             const int br = slice * R + rr;
-            const int base = C.slice_ptr[slice];
-            const int W = C.slice_ptr[slice + 1] - base;
+            const int base = sptr[slice];
+            const int W = sptr[slice + 1] - base;
             if (act && br < C.nb) {
                 real a0 = 0, a1 = 0, a2 = 0;
-#pragma unroll 3
-                for (int j = 0; j < W; j++) {
-                    const int slot = base + j;
-                    const int col = C.cidx[(size_t)slot * R + rr];
-                    if (col < 0) continue;
-                    const size_t xo = (size_t)col * 3 * T + s;
-                    const real v0 = z[xo] + beta * po[xo];
-                    const real v1 = z[xo + T] + beta * po[xo + T];
-                    const real v2 = z[xo + 2 * T] + beta * po[xo + 2 * T];
-                    // stiffness values are streamed once per product: load them
-                    // non-temporally so L2 keeps the reused vector gathers
-                    const real *m = val + ((size_t)slot * 9 * R + rr) * T + s;
-                    const real m0 = __builtin_nontemporal_load(m), m1 = __builtin_nontemporal_load(m + ES),
-                               m2 = __builtin_nontemporal_load(m + 2 * ES);
-                    const real m3 = __builtin_nontemporal_load(m + 3 * ES),
-                               m4 = __builtin_nontemporal_load(m + 4 * ES),
-                               m5 = __builtin_nontemporal_load(m + 5 * ES);
-                    const real m6 = __builtin_nontemporal_load(m + 6 * ES),
-                               m7 = __builtin_nontemporal_load(m + 7 * ES),
-                               m8 = __builtin_nontemporal_load(m + 8 * ES);
-                    a0 += m0 * v0 + m1 * v1 + m2 * v2;
-                    a1 += m3 * v0 + m4 * v1 + m5 * v2;
-                    a2 += m6 * v0 + m7 * v1 + m8 * v2;
+                int j = 0;
+                for (; j + 3 <= W; j += 3) {
+                    int col[3];
+#pragma unroll
+                    for (int u = 0; u < 3; u++) col[u] = max(cidx[(size_t)(base + j + u) * R + rr], 0);
+                    real v[3][3], m[3][9];
+#pragma unroll
+                    for (int u = 0; u < 3; u++) {
+                        gather(col[u], v[u][0], v[u][1], v[u][2]);
+                        block(base + j + u, m[u]);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 3; u++) {
+                        a0 += m[u][0] * v[u][0] + m[u][1] * v[u][1] + m[u][2] * v[u][2];
+                        a1 += m[u][3] * v[u][0] + m[u][4] * v[u][1] + m[u][5] * v[u][2];
+                        a2 += m[u][6] * v[u][0] + m[u][7] * v[u][1] + m[u][8] * v[u][2];
+                    }
                 }
-                const size_t o = (size_t)br * 3 * T + s;
-                const real n0 = z[o] + beta * po[o];
-                const real n1 = z[o + T] + beta * po[o + T];
-                const real n2 = z[o + 2 * T] + beta * po[o + 2 * T];
-                pn[o] = n0;
-                pn[o + T] = n1;
-                pn[o + 2 * T] = n2;
-                q[o] = a0;
-                q[o + T] = a1;
-                q[o + 2 * T] = a2;
+                for (; j < W; j++) {
+                    const int col = max(cidx[(size_t)(base + j) * R + rr], 0);
+                    real v0, v1, v2, m[9];
+                    gather(col, v0, v1, v2);
+                    block(base + j, m);
+                    a0 += m[0] * v0 + m[1] * v1 + m[2] * v2;
+                    a1 += m[3] * v0 + m[4] * v1 + m[5] * v2;
+                    a2 += m[6] * v0 + m[7] * v1 + m[8] * v2;
+                }
+                real n0, n1, n2;
+                gather(br, n0, n1, n2);
+                if constexpr (T == 1) {
+                    st3(pn, br, n0, n1, n2);
+                    st3(q, br, a0, a1, a2);
+                } else {
+                    const size_t o = (size_t)br * 3 * T + s;
+                    pn[o] = n0;
+                    pn[o + T] = n1;
+                    pn[o + 2 * T] = n2;
+                    q[o] = a0;
+                    q[o + T] = a1;
+                    q[o + 2 * T] = a2;
+                }
                 pq += n0 * a0 + n1 * a1 + n2 * a2;
             }
         }
@@ -532,29 +600,44 @@ __global__ __launch_bounds__(256) void k_pcg_b(const ClassDev *__restrict__ cls,
     const bool act = sample < C.nsamp && C.st[sample].active;
     const real alpha = act ? (real)C.st[sample].alpha : (real)0;
     const size_t vb = (size_t)tile * C.tile_vec;
-    real *__restrict__ x = (real *)C.x + vb;
-    real *__restrict__ r = (real *)C.r + vb;
-    real *__restrict__ z = (real *)C.z + vb;
-    const real *__restrict__ pn = (const real *)(pbuf ? C.p0 : C.p1) + vb;
-    const real *__restrict__ q = (const real *)C.q + vb;
-    const real *__restrict__ dinv = (const real *)C.dinv + vb;
+    MSA_GLOBAL real *__restrict__ x = gbl((real *)C.x + vb);
+    MSA_GLOBAL real *__restrict__ r = gbl((real *)C.r + vb);
+    MSA_GLOBAL real *__restrict__ z = gbl((real *)C.z + vb);
+    const MSA_GLOBAL real *__restrict__ pn = gbl((const real *)(pbuf ? C.p0 : C.p1) + vb);
+    const MSA_GLOBAL real *__restrict__ q = gbl((const real *)C.q + vb);
+    const MSA_GLOBAL real *__restrict__ dinv = gbl((const real *)C.dinv + vb);
     real rz = 0, rr2 = 0;
     const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
     if (act) {
+#pragma unroll 2
         for (int slice = s0 + wave; slice < s1; slice += kWaves) {
             const int br = slice * R + rr;
             if (br >= C.nb) continue;
+            if constexpr (T == 1) {
+                using V = vec3<real>;
+                const V xv = ld3(x, br), pv = ld3(pn, br), rv = ld3(r, br);
+                const V qv = ld3(q, br), dv = ld3(dinv, br);
+                const V xn{xv.x + alpha * pv.x, xv.y + alpha * pv.y, xv.z + alpha * pv.z};
+                const V rn{rv.x - alpha * qv.x, rv.y - alpha * qv.y, rv.z - alpha * qv.z};
+                const V zn{dv.x * rn.x, dv.y * rn.y, dv.z * rn.z};
+                st3(x, br, xn.x, xn.y, xn.z);
+                st3(r, br, rn.x, rn.y, rn.z);
+                st3(z, br, zn.x, zn.y, zn.z);
+                rz += rn.x * zn.x + rn.y * zn.y + rn.z * zn.z;
+                rr2 += rn.x * rn.x + rn.y * rn.y + rn.z * rn.z;
+            } else {
 #pragma unroll
-            for (int d = 0; d < 3; d++) {
-                const size_t o = ((size_t)br * 3 + d) * T + s;
-                const real xv = x[o] + alpha * pn[o];
-                const real rv = r[o] - alpha * q[o];
-                const real zv = dinv[o] * rv;
-                x[o] = xv;
-                r[o] = rv;
-                z[o] = zv;
-                rz += rv * zv;
-                rr2 += rv * rv;
+                for (int d = 0; d < 3; d++) {
+                    const size_t o = ((size_t)br * 3 + d) * T + s;
+                    const real xv = x[o] + alpha * pn[o];
+                    const real rv = r[o] - alpha * q[o];
+                    const real zv = dinv[o] * rv;
+                    x[o] = xv;
+                    r[o] = rv;
+                    z[o] = zv;
+                    rz += rv * zv;
+                    rr2 += rv * rv;
+                }
             }
         }
     }
@@ -691,10 +774,36 @@ __global__ __launch_bounds__(256) void k_ir_update(const ClassDev *__restrict__ 
     }
 }
 
+// r64 -= E_s Khat u64_e for one sample on the element's 24 rows; lanes
+// [0, 24) of `half` compute them.
+__device__ __forceinline__ void residual_rows(const ClassDev &C, const int *en, const double *Kh,
+                                              double Es, size_t vb, int T, int s, int i)
+{
+    const int ia = i / 3, p = i - 3 * (i / 3);
+    const int nd = en[ia];
+    const int b = C.node_block[nd];
+    if (b < 0 || !((C.node_free[nd] >> p) & 1)) return;
+    double y = 0;
+#pragma unroll
+    for (int jn = 0; jn < 8; jn++) {
+        const int nj = en[jn];
+        const int bj = C.node_block[nj];
+        const uint8_t fr = C.node_free[nj];
+        const double *blk = Kh + (ia * 8 + jn) * 9 + p * 3;
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            if (bj >= 0 && ((fr >> d) & 1))
+                y += blk[d] * C.u64[vb + ((size_t)bj * 3 + d) * T + s];
+    }
+    C.r64[vb + ((size_t)b * 3 + p) * T + s] -= Es * y;
+}
+
 // FP64 residual r64 -= K u64, matrix-free: one wavefront per element of the
 // current colour recomputes Khat in FP64 (LDS) and applies E_s Khat to the
 // element's displacements of every active sample.
-// need_due = 1: only samples with a pending replacement (reliable updates).
+// need_due = 1: only samples with a pending replacement (reliable updates);
+// one-sample tiles take them from the class's due list, two per pass (lanes
+// 0-23 and 32-55).
 template <int T>
 __global__ __launch_bounds__(64) void k_residual64(const ClassDev *__restrict__ cls,
                                                    ColourLaunch cl, double nu, int need_due)
@@ -709,45 +818,43 @@ __global__ __launch_bounds__(64) void k_residual64(const ClassDev *__restrict__ 
     const ClassDev &C = cls[c];
     const int e = C.colour_elems[C.colour_ptr[cl.colour] + (item - cl.prefix[c])];
     const int lane = threadIdx.x & 63;
+    const int ndue = need_due ? *C.due : 0;
+    if (need_due && ndue == 0) return;
     bool any = false;
-    for (int smp = lane; smp < C.nsamp; smp += 64)
-        any |= C.st[smp].ir_active && (!need_due || C.st[smp].ru_due);
+    if (T == 1 && need_due) {
+        for (int k = lane; k < ndue; k += 64) {
+            const SampleState &S = C.st[C.due[1 + k]];
+            any |= S.ir_active && S.ru_due;
+        }
+    } else {
+        for (int smp = lane; smp < C.nsamp; smp += 64)
+            any |= C.st[smp].ir_active && (!need_due || C.st[smp].ru_due);
+    }
     if (!__ballot(any)) return;
     const int *en = C.conn + (size_t)e * 8;
     const int na = en[lane & 7];
     const double xa[3] = {C.xyz[3 * na], C.xyz[3 * na + 1], C.xyz[3 * na + 2]};
     (void)wave_khat<double>(xa, nu, Kh, g, w);
+    if constexpr (T == 1) {
+        if (need_due) {
+            const int half = lane >> 5, i = lane & 31;
+            for (int k = half; k < ndue; k += 2) {
+                const int smp = C.due[1 + k];
+                const SampleState &S = C.st[smp];
+                if (i >= 24 || !S.ir_active || !S.ru_due) continue;
+                const double Es = C.E[(size_t)smp * C.n_elem + e];
+                residual_rows(C, en, Kh, Es, (size_t)smp * C.tile_vec, 1, 0, i);
+            }
+            return;
+        }
+    }
     const int s = lane % T, sub = lane / T;
     for (int tile = 0; tile < C.ntiles; tile++) {
         const int smp = tile * T + s;
         if (smp >= C.nsamp || !C.st[smp].ir_active || (need_due && !C.st[smp].ru_due)) continue;
         const size_t vb = (size_t)tile * C.tile_vec;
-        double u[24];
-#pragma unroll
-        for (int jn = 0; jn < 8; jn++) {
-            const int nd = en[jn];
-            const int b = C.node_block[nd];
-            const uint8_t fr = C.node_free[nd];
-#pragma unroll
-            for (int d = 0; d < 3; d++)
-                u[3 * jn + d] = (b >= 0 && ((fr >> d) & 1))
-                                    ? C.u64[vb + ((size_t)b * 3 + d) * T + s]
-                                    : 0.0;
-        }
         const double Es = C.E[((size_t)tile * C.n_elem + e) * T + s];
-        for (int i = sub; i < 24; i += R) {
-            const int ia = i / 3, p = i - 3 * (i / 3);
-            const int nd = en[ia];
-            const int b = C.node_block[nd];
-            if (b < 0 || !((C.node_free[nd] >> p) & 1)) continue;
-            double y = 0;
-#pragma unroll
-            for (int jn = 0; jn < 8; jn++) {
-                const double *blk = Kh + (ia * 8 + jn) * 9 + p * 3;
-                y += blk[0] * u[3 * jn] + blk[1] * u[3 * jn + 1] + blk[2] * u[3 * jn + 2];
-            }
-            C.r64[vb + ((size_t)b * 3 + p) * T + s] -= Es * y;
-        }
+        for (int i = sub; i < 24; i += R) residual_rows(C, en, Kh, Es, vb, T, s, i);
     }
 }
 
@@ -805,7 +912,8 @@ __global__ __launch_bounds__(256) void k_ru_fold(const ClassDev *__restrict__ cl
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int rr = lane / T, s = lane % T;
     const int sample = tile * T + s;
-    if (sample >= C.nsamp || !C.st[sample].ir_active || !C.st[sample].ru_due) return;
+    if (*C.due == 0 || sample >= C.nsamp || !C.st[sample].ir_active || !C.st[sample].ru_due)
+        return;
     const double sc = C.st[sample].scale;
     const size_t vb = (size_t)tile * C.tile_vec;
     float *x = (float *)C.x + vb;
@@ -976,6 +1084,18 @@ __global__ void k_kl_eval(const ClassDev *__restrict__ cp, int ntiles, int n_ele
 // update of `op`.  The kernel boundary makes the partials visible; no
 // floating-point atomics, so results are reproducible run to run.
 
+// A pending FP64 replacement is flagged per sample and the sample appended
+// to the class's due list once (between list resets no flag goes 1 -> 0 ->
+// 1, so the list never exceeds nsamp entries).  The list is reset by the
+// reductions that clear every flag (OP_INIT*, OP_RURESTART); no sample is
+// appended by those.
+__device__ __forceinline__ void set_due(const ClassDev &C, SampleState &S, int smp)
+{
+    if (S.ru_due) return;
+    S.ru_due = 1;
+    C.due[1 + atomicAdd(C.due, 1)] = smp;
+}
+
 __device__ void apply_op(const ClassDev &C, int smp, const double *v, const ReduceArgs &a)
 {
     SampleState &S = C.st[smp];
@@ -1014,7 +1134,7 @@ __device__ void apply_op(const ClassDev &C, int smp, const double *v, const Redu
             S.status = MSA_BREAKDOWN;
             S.active = 0;
             atomicSub(a.active_count, 1);
-            if (S.ir_active) S.ru_due = 1;  // reliable updates: final fold + test
+            if (S.ir_active) set_due(C, S, smp);  // reliable updates: final fold + test
         } else {
             S.alpha = S.rz / v[0];
         }
@@ -1029,7 +1149,7 @@ __device__ void apply_op(const ClassDev &C, int smp, const double *v, const Redu
             S.status = MSA_BREAKDOWN;
             S.active = 0;
             atomicSub(a.active_count, 1);
-            if (a.ru_drop > 0) S.ru_due = 1;  // final fold of the correction
+            if (a.ru_drop > 0) set_due(C, S, smp);  // final fold of the correction
         } else if (err < a.tol) {
             S.status = MSA_OK;
             S.active = 0;
@@ -1038,7 +1158,7 @@ __device__ void apply_op(const ClassDev &C, int smp, const double *v, const Redu
             S.status = MSA_NOT_CONVERGED;
             S.active = 0;
             atomicSub(a.active_count, 1);
-            if (a.ru_drop > 0) S.ru_due = 1;
+            if (a.ru_drop > 0) set_due(C, S, smp);
         } else {
             S.beta = v[0] / S.rz;
             S.rz = v[0];
@@ -1046,7 +1166,7 @@ __device__ void apply_op(const ClassDev &C, int smp, const double *v, const Redu
             // replacement) fell by ru_drop, or the FP64 stopping test may pass
             if (a.ru_drop > 0) {
                 const double rn = sqrt(v[1]);
-                if (rn < a.ru_drop || rn * S.scale < a.ru_tol * C.fnorm) S.ru_due = 1;
+                if (rn < a.ru_drop || rn * S.scale < a.ru_tol * C.fnorm) set_due(C, S, smp);
             }
         }
         break;
@@ -1159,6 +1279,10 @@ __global__ __launch_bounds__(1024) void k_reduce(const ClassDev *__restrict__ cl
         const double v[2] = {red[0][t], red[1][t]};
         if (smp < C.nsamp) apply_op(C, smp, v, a);
     }
+    // these ops clear every replacement flag of the class and append none
+    if ((a.op == OP_INIT0 || a.op == OP_INIT1 || a.op == OP_RURESTART) && tile == 0 &&
+        threadIdx.x == 0)
+        *C.due = 0;
 }
 
 // ------------------------------------------------------------ utilities

@@ -103,6 +103,7 @@ struct ClassDev {
     SampleState *st;      // [ntiles*T]
     double *partial;      // [ntiles][nchunks][3][T]
     int *negdet;          // [1]
+    int *due;             // [1 + nsamp]: count, then samples with a pending replacement
     int colour_ptr[kMaxColours + 1];
 };
 

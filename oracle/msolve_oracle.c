@@ -397,6 +397,54 @@ void orc_ke_element_csharp(const double *xyz, double E, double nu, double *K300)
             }
 }
 
+/* Hexa8.CalculateConsistentMass (Hexa8.cs:403-436): per Gauss point of the
+ * C# rule (xi outer, zeta inner; weight = w_xi w_eta w_zeta detJ, detJ of
+ * Jacobian3D as in orc_ke_element_csharp) and shape values of CalcH8Shape
+ * (Hexa8.cs:137-158), M[3a+d][3b+d] += N_a N_b weight rho for b >= a; the
+ * three direction copies are equal.  Output: packed upper 300. */
+void orc_h8_mass_csharp(const double *xyz, double rho, double *M300)
+{
+    static const double sgn[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                     {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+    double m[8][8];
+    memset(m, 0, sizeof(m));
+    for (int ix = 0; ix < 2; ix++)
+        for (int iy = 0; iy < 2; iy++)
+            for (int iz = 0; iz < 2; iz++) {
+                double xi = GP[1][ix], eta = GP[1][iy], zeta = GP[1][iz];
+                double dxi[8], deta[8], dzeta[8];
+                for (int a = 0; a < 8; a++) {
+                    dxi[a] = sgn[a][0] * 0.125 * (1.0 + sgn[a][1] * eta) * (1.0 + sgn[a][2] * zeta);
+                    deta[a] = sgn[a][1] * 0.125 * (1.0 + sgn[a][0] * xi) * (1.0 + sgn[a][2] * zeta);
+                    dzeta[a] = sgn[a][2] * 0.125 * (1.0 + sgn[a][0] * xi) * (1.0 + sgn[a][1] * eta);
+                }
+                double Jm[3][3] = {{0}};
+                for (int a = 0; a < 8; a++)
+                    for (int j = 0; j < 3; j++) {
+                        Jm[0][j] += xyz[3 * a + j] * dxi[a];
+                        Jm[1][j] += xyz[3 * a + j] * deta[a];
+                        Jm[2][j] += xyz[3 * a + j] * dzeta[a];
+                    }
+                double det = Jm[0][0] * (Jm[1][1] * Jm[2][2] - Jm[2][1] * Jm[1][2]) -
+                             Jm[0][1] * (Jm[1][0] * Jm[2][2] - Jm[2][0] * Jm[1][2]) +
+                             Jm[0][2] * (Jm[1][0] * Jm[2][1] - Jm[2][0] * Jm[1][1]);
+                double wd = GW[1][ix] * GW[1][iy] * GW[1][iz] * det * rho;
+                double N[8];
+                for (int a = 0; a < 8; a++)
+                    N[a] = ((1.0 + sgn[a][0] * xi) * 0.5) * ((1.0 + sgn[a][1] * eta) * 0.5) *
+                           ((1.0 + sgn[a][2] * zeta) * 0.5);
+                for (int a = 0; a < 8; a++)
+                    for (int b = a; b < 8; b++) m[a][b] += N[a] * N[b] * wd;
+            }
+    for (int i = 0; i < 300; i++) M300[i] = 0.0;
+    for (int a = 0; a < 8; a++)
+        for (int b = a; b < 8; b++)
+            for (int d = 0; d < 3; d++) {
+                int r = 3 * a + d, c = 3 * b + d;
+                M300[300 - ((24 - r + 1) * (24 - r) / 2) + c - r] = m[a][b];
+            }
+}
+
 /* SymmetricMatrix2D.IndexOf (SymmetricMatrix2D.cs:44-49) for the 24x24
  * packed upper triangle. */
 static inline int sym_index(int row, int col)
@@ -1071,4 +1119,124 @@ int orc_mc_csr_jacobi(int nex, int ney, int nez, double nu, int M, int kmax,
     free(xyz); free(conn); free(cons); free(dof); free(f); free(rp); free(col); free(val);
     free(Wm); free(Bm); free(E); free(ke); free(x);
     return n;
+}
+
+
+/* ------------------------------------------------------------------------ */
+/* Newmark dynamics                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* Time step i of MassAccelerationSinusoidalLoad (MassAccelerationSinusoidalLoad.cs:17-29):
+ * mag sin(2 pi / period * dt * i) for i < (int)(tlen / dt), else 0. */
+double orc_sinusoidal_acc(int i, double mag, double period, double dt, double tlen)
+{
+    const int steps = (int)(tlen / dt);
+    const double pi = 3.14159265358979323846; /* Math.PI, the double nearest pi */
+    return i < steps ? mag * sin(2 * pi / period * dt * i) : 0.0;
+}
+
+/* Linear implicit Newmark of NewmarkDynamicAnalyzer: coefficients
+ * (NewmarkDynamicAnalyzer.cs:53-77; alpha = beta of the textbook, delta =
+ * gamma), effective matrix K + a0 M + a1 C (ProblemStructural.cs:138-157 with
+ * SkylineMatrix2D.LinearCombination, SkylineMatrix2D.cs:310-340), element
+ * damping C_e = alphaR M_e + betaR K_e (Hexa8.DampingMatrix, Hexa8.cs:444-451),
+ * consistent mass (orc_h8_mass_csharp), per step: forces cleared, static
+ * nodal loads assigned, the mass-acceleration history load M_e a_e added on
+ * free rows (Model.cs:266-280; a_e holds acc(i) on every DOF of direction
+ * acc_dir, constrained nodes included, Hexa8.cs:512-530), RHS = F + M (a0 u +
+ * a2 v + a3 a) + C (a1 u + a4 v + a5 a) (NewmarkDynamicAnalyzer.cs:189-206),
+ * skyline LDL^T solve (the only reference solver that is correct here,
+ * SolverSkyline), then the update of NewmarkDynamicAnalyzer.cs:301-315.
+ * Steps run i = 0 .. nsteps-1 (NewmarkDynamicAnalyzer.cs:284) from rest.
+ * probe_hist[i] = u[probe] after step i.  Returns 0, or < 0 on failure. */
+int orc_newmark_skyline(int n_elem, const int *conn, const double *xyz, const int *dof,
+                        int n, const double *E, double nu, double rho, double alphaR,
+                        double betaR, double nb_alpha, double nb_delta, double dt, int nsteps,
+                        const double *f_static, int acc_dir, double acc_mag,
+                        double acc_period, double acc_tlen, int probe, double *probe_hist,
+                        double *u_out, double *v_out, double *a_out)
+{
+    if (nb_delta < 0.5 || nb_alpha < 0.25 * (0.5 + nb_delta) * (0.5 + nb_delta)) return -1;
+    const double a0 = 1 / (nb_alpha * dt * dt), a1 = nb_delta / (nb_alpha * dt);
+    const double a2 = 1 / (nb_alpha * dt), a3 = 1 / (2 * nb_alpha) - 1;
+    const double a4 = nb_delta / nb_alpha - 1, a5 = dt * 0.5 * (nb_delta / nb_alpha - 2);
+    const double a6 = dt * (1 - nb_delta), a7 = nb_delta * dt;
+    double *ke = (double *)malloc(sizeof(double) * 300 * (size_t)n_elem);
+    double *me = (double *)malloc(sizeof(double) * 300 * (size_t)n_elem);
+    double *ce = (double *)malloc(sizeof(double) * 300 * (size_t)n_elem);
+    int64_t *ri = (int64_t *)malloc(sizeof(int64_t) * ((size_t)n + 1));
+    if (orc_ke_mesh(n_elem, conn, xyz, E, nu, ke) != 0) {
+        free(ke); free(me); free(ce); free(ri);
+        return -2;
+    }
+    for (int e = 0; e < n_elem; e++) {
+        double x8[24];
+        for (int a = 0; a < 8; a++)
+            for (int j = 0; j < 3; j++) x8[3 * a + j] = xyz[3 * conn[8 * e + a] + j];
+        orc_h8_mass_csharp(x8, rho, me + 300 * (size_t)e);
+        for (int i = 0; i < 300; i++)
+            ce[300 * (size_t)e + i] = alphaR * me[300 * (size_t)e + i] + betaR * ke[300 * (size_t)e + i];
+    }
+    const int64_t len = orc_skyline_rowindex(n_elem, conn, dof, n, ri);
+    double *K = (double *)calloc((size_t)len, sizeof(double));
+    double *Mg = (double *)calloc((size_t)len, sizeof(double));
+    double *Cg = (double *)calloc((size_t)len, sizeof(double));
+    orc_skyline_assemble(n_elem, conn, dof, ke, ri, K);
+    orc_skyline_assemble(n_elem, conn, dof, me, ri, Mg);
+    orc_skyline_assemble(n_elem, conn, dof, ce, ri, Cg);
+    for (int64_t j = 0; j < len; j++) K[j] = 1.0 * K[j] + a0 * Mg[j] + a1 * Cg[j];
+    int rc = orc_skyline_factorize(n, ri, K, 1e-15);
+    /* mass-acceleration load for acc = 1: element M_e times the unit vector
+     * of direction acc_dir, free rows only */
+    double *bacc = (double *)calloc((size_t)n, sizeof(double));
+    for (int e = 0; e < n_elem && acc_dir >= 0; e++) {
+        const double *m = me + 300 * (size_t)e;
+        for (int i = 0; i < 24; i++) {
+            const int g = dof[3 * conn[8 * e + i / 3] + i % 3];
+            if (g < 0) continue;
+            double fi = 0;
+            for (int j = 0; j < 24; j++)
+                if (j % 3 == acc_dir) fi += m[sym_index(i, j)] * 1.0;
+            bacc[g] += fi;
+        }
+    }
+    double *u = (double *)calloc((size_t)n, sizeof(double));  /* previous displacement */
+    double *v = (double *)calloc((size_t)n, sizeof(double));  /* current displacement */
+    double *v1 = (double *)calloc((size_t)n, sizeof(double)); /* velocity */
+    double *v2 = (double *)calloc((size_t)n, sizeof(double)); /* acceleration */
+    double *uu = (double *)malloc(sizeof(double) * (size_t)n);
+    double *uc = (double *)malloc(sizeof(double) * (size_t)n);
+    double *uum = (double *)malloc(sizeof(double) * (size_t)n);
+    double *ucc = (double *)malloc(sizeof(double) * (size_t)n);
+    double *rhs = (double *)malloc(sizeof(double) * (size_t)n);
+    double *x = (double *)malloc(sizeof(double) * (size_t)n);
+    for (int step = 0; rc == 0 && step < nsteps; step++) {
+        const double ag = orc_sinusoidal_acc(step, acc_mag, acc_period, dt, acc_tlen);
+        for (int i = 0; i < n; i++) {
+            uu[i] = a0 * v[i] + a2 * v1[i] + a3 * v2[i];
+            uc[i] = a1 * v[i] + a4 * v1[i] + a5 * v2[i];
+        }
+        orc_skyline_multiply(n, ri, Mg, uu, uum);
+        orc_skyline_multiply(n, ri, Cg, uc, ucc);
+        for (int i = 0; i < n; i++) {
+            const double F = (f_static ? f_static[i] : 0.0) + bacc[i] * ag;
+            rhs[i] = F + uum[i] + ucc[i];
+        }
+        orc_skyline_solve(n, ri, K, rhs, x);
+        for (int i = 0; i < n; i++) {
+            u[i] = v[i];
+            v[i] = x[i];
+            const double vv = v2[i];
+            v2[i] = a0 * (v[i] - u[i]) - a2 * v1[i] - a3 * vv;
+            v1[i] += a6 * vv + a7 * v2[i];
+        }
+        if (probe_hist) probe_hist[step] = v[probe];
+    }
+    if (u_out) memcpy(u_out, v, sizeof(double) * (size_t)n);
+    if (v_out) memcpy(v_out, v1, sizeof(double) * (size_t)n);
+    if (a_out) memcpy(a_out, v2, sizeof(double) * (size_t)n);
+    free(ke); free(me); free(ce); free(ri); free(K); free(Mg); free(Cg); free(bacc);
+    free(u); free(v); free(v1); free(v2); free(uu); free(uc); free(uum); free(ucc);
+    free(rhs); free(x);
+    return rc == 0 ? 0 : -3;
 }

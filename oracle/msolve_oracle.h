@@ -45,6 +45,7 @@ void orc_elastic_D(double E, double nu, double *D36);
 int orc_ke_mesh(int n_elem, const int *conn, const double *xyz,
                 const double *E_elem, double nu, double *ke300);
 void orc_ke_element_csharp(const double *xyz, double E, double nu, double *K300);
+void orc_h8_mass_csharp(const double *xyz, double rho, double *M300);
 
 /* Skyline (GlobalMatrixAssemblerSkyline.cs, SkylineMatrix2D.cs) ------------ */
 int64_t orc_skyline_rowindex(int n_elem, const int *conn, const int *dof,
@@ -99,6 +100,15 @@ int orc_mc_csr_jacobi(int nex, int ney, int nez, double nu, int M, int kmax,
                       const double *eigs, double meanE, double cov,
                       int n_samples, const double *xi, double tol, int max_it,
                       double *u_out, int *iters, double *times);
+
+/* Newmark dynamics (NewmarkDynamicAnalyzer.cs, ProblemStructural.cs) ------- */
+double orc_sinusoidal_acc(int i, double mag, double period, double dt, double tlen);
+int orc_newmark_skyline(int n_elem, const int *conn, const double *xyz, const int *dof,
+                        int n, const double *E, double nu, double rho, double alphaR,
+                        double betaR, double nb_alpha, double nb_delta, double dt, int nsteps,
+                        const double *f_static, int acc_dir, double acc_mag,
+                        double acc_period, double acc_tlen, int probe, double *probe_hist,
+                        double *u_out, double *v_out, double *a_out);
 
 #ifdef __cplusplus
 }

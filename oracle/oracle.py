@@ -54,6 +54,10 @@ def lib():
         "orc_elastic_D": (V, [D, D, _f64p]),
         "orc_ke_mesh": (I, [I, _i32p, _f64p, _f64p, D, _f64p]),
         "orc_ke_element_csharp": (V, [_f64p, D, D, _f64p]),
+        "orc_h8_mass_csharp": (V, [_f64p, D, _f64p]),
+        "orc_sinusoidal_acc": (D, [I, D, D, D, D]),
+        "orc_newmark_skyline": (I, [I, _i32p, _f64p, _i32p, I, _f64p, D, D, D, D, D, D, D, I,
+                                    C.c_void_p, I, D, D, D, I, _f64p, _f64p, _f64p, _f64p]),
         "orc_skyline_rowindex": (C.c_int64, [I, _i32p, _i32p, I, _i64p]),
         "orc_skyline_assemble": (V, [I, _i32p, _i32p, _f64p, _i64p, _f64p]),
         "orc_skyline_multiply": (V, [I, _i64p, _f64p, _f64p, _f64p]),
@@ -155,6 +159,39 @@ def ke_element_csharp(xyz8, E, nu):
     K = np.zeros(300)
     lib().orc_ke_element_csharp(np.ascontiguousarray(xyz8, np.float64).ravel(), E, nu, K)
     return K
+
+
+def h8_mass_csharp(xyz8, rho):
+    """Consistent mass of one Hexa8 (Hexa8.cs:403-436), packed upper 300."""
+    M = np.zeros(300)
+    lib().orc_h8_mass_csharp(np.ascontiguousarray(xyz8, np.float64).ravel(), rho, M)
+    return M
+
+
+def sinusoidal_acc(i, mag, period, dt, tlen):
+    return lib().orc_sinusoidal_acc(int(i), mag, period, dt, tlen)
+
+
+def newmark_skyline(model, E, nu=0.3, rho=7.85e-9, alphaR=0.0, betaR=0.0, nb_alpha=0.25,
+                    nb_delta=0.5, dt=1e-4, nsteps=10, f_static=None, acc_dir=2, acc_mag=1.0,
+                    acc_period=0.05, acc_tlen=0.05, probe=None):
+    """Reference Newmark loop with the skyline direct solver; returns
+    (probe history, u, v, a) after nsteps."""
+    n = model.n
+    probe = n - 1 if probe is None else probe
+    hist = np.zeros(max(nsteps, 1))
+    u, v, a = np.zeros(n), np.zeros(n), np.zeros(n)
+    fs = None
+    if f_static is not None:
+        f_static = np.ascontiguousarray(f_static, np.float64)
+        fs = f_static.ctypes.data_as(C.c_void_p)
+    rc = lib().orc_newmark_skyline(model.n_elem, model.conn, model.xyz, model.dof, n,
+                                   np.ascontiguousarray(E, np.float64), nu, rho, alphaR, betaR,
+                                   nb_alpha, nb_delta, dt, nsteps, fs, acc_dir, acc_mag,
+                                   acc_period, acc_tlen, probe, hist, u, v, a)
+    if rc != 0:
+        raise RuntimeError(f"orc_newmark_skyline failed ({rc})")
+    return hist[:nsteps], u, v, a
 
 
 def unpack300(k300):
