@@ -228,6 +228,53 @@ int msa_batch_get_kernel_timing(msa_batch *b, double *ms_total,
  * of the batch (values + shared indices + vectors), and its element count. */
 int msa_batch_spmv_bytes(msa_batch *b, double *bytes_per_launch);
 
+/* ----------------------------------------------------------- dynamics
+ * Linear implicit Newmark time integration of every sample of the batch
+ * (NewmarkDynamicAnalyzer, ISAAR.MSolve.Analyzers/NewmarkDynamicAnalyzer.cs
+ * with ProblemStructural, ISAAR.MSolve.Problems/ProblemStructural.cs).
+ *
+ * msa_batch_set_dynamics switches the batch to the effective stiffness
+ * K + a0 M + a1 C (ProblemStructural.CalculateEffectiveMatrix, :138-157):
+ * consistent mass of density rho (Hexa8.CalculateConsistentMass,
+ * ISAAR.MSolve.PreProcessor/Elements/Hexa8.cs:403-436), Rayleigh damping
+ * C = rayleigh_alpha M + rayleigh_beta K (Hexa8.DampingMatrix, :444-451),
+ * Newmark parameters alpha (>= (0.5 + delta)^2 / 4) and delta (>= 0.5) and
+ * time step dt (InitializeCoefficients, NewmarkDynamicAnalyzer.cs:53-77;
+ * invalid parameters give MSA_ERR_ARG as the reference throws).  Call it
+ * before msa_batch_assemble.  Loads: the class load vector (msa_batch_set_rhs,
+ * optional, constant in time) plus the mass-acceleration history load. */
+int msa_batch_set_dynamics(msa_batch *b, double rho, double rayleigh_alpha,
+                           double rayleigh_beta, double newmark_alpha,
+                           double newmark_delta, double dt);
+/* Reference free DOF of class cls whose displacement is recorded after every
+ * step (msa_batch_get_history); -1 records nothing. */
+int msa_batch_set_probe(msa_batch *b, int cls, int dof);
+/* Runs nsteps steps from rest (NewmarkDynamicAnalyzer.Solve, :280-299): per
+ * step the forces F = f + acc[step] * M_full 1_dir on the free rows
+ * (Model.AssignMassAccelerationHistoryLoads, PreProcessor/Model.cs:266-280;
+ * acc_dir 0/1/2 = X/Y/Z, -1: no history load), the right-hand side F +
+ * M (a0 u + a2 v + a3 a) + C (a1 u + a4 v + a5 a) (CalculateRHSImplicit,
+ * :189-206), a PCG solve to |F_eff - K_eff u|/|F_eff| < tol started from the
+ * previous displacement (the reference's PCG reuses a stale residual here,
+ * so its SolverSkyline is the oracle), and the velocity/acceleration update
+ * (:301-315).  acc[nsteps] is on the host (MassAccelerationSinusoidalLoad
+ * values, PreProcessor/MassAccelerationSinusoidalLoad.cs:17-20).
+ * warm_start selects the first iterate of every step's PCG: 0 zero, 1 the
+ * previous displacement (recommended: ~8-17% fewer iterations than zero on
+ * the 20 Hz base-acceleration workload), 2 the predictor u + dt v + dt^2/2 a
+ * (worse there: the undamped average-acceleration scheme leaves step-to-step
+ * oscillation of the stiff modes in v and a). */
+int msa_batch_newmark(msa_batch *b, int nsteps, const double *acc, int acc_dir,
+                      double tol, int max_it, int graph_iters, int warm_start);
+/* Probe displacement after each step of the last msa_batch_newmark call,
+ * hist[nsteps]. */
+int msa_batch_get_history(msa_batch *b, int cls, int sample, double *hist);
+/* Displacement, velocity and acceleration of one sample after the last step
+ * (reference numbering, [n_free] each; any pointer may be NULL) and the PCG
+ * iterations summed over the steps. */
+int msa_batch_get_dynamic_state(msa_batch *b, int cls, int sample, double *u,
+                                double *v, double *a, long long *iters);
+
 /* ----------------------------------------------------------- autotuner
  * Times the SpMV+dot kernel of mesh `mesh` for every lane tile
  * (1..64 samples per wave) with n_samples synthetic samples and writes the

@@ -335,4 +335,85 @@ class MonteCarloAnalyzerWithStochasticMaterial {
     std::unique_ptr<Batch> batch_;
 };
 
+// MassAccelerationSinusoidalLoad (PreProcessor/MassAccelerationSinusoidalLoad.cs:
+// 11-29): magnitude sin(2 pi / period * timeStep * i) for i < (int)(timeLength
+// / timeStep), else 0.
+class MassAccelerationSinusoidalLoad {
+  public:
+    MassAccelerationSinusoidalLoad(double period, double timeStep, double timeLength,
+                                   double magnitude)
+        : period_(period), timeStep_(timeStep), magnitude_(magnitude),
+          timeSteps_((int)(timeLength / timeStep)) {}
+    double operator[](int currentTimeStep) const
+    {
+        const double pi = 3.14159265358979323846;
+        return currentTimeStep < timeSteps_
+                   ? magnitude_ * std::sin(2 * pi / period_ * timeStep_ * currentTimeStep)
+                   : 0.0;
+    }
+    DOFType DOF = DOFType::Z;
+
+  private:
+    double period_, timeStep_, magnitude_;
+    int timeSteps_;
+};
+
+// NewmarkDynamicAnalyzer (Analyzers/NewmarkDynamicAnalyzer.cs:38-315) over a
+// batch of stochastic-modulus samples: BuildMatrices() forms the effective
+// stiffness (ProblemStructural.CalculateEffectiveMatrix), Solve() runs
+// (int)(totalTime / timeStep) steps from rest with warm-started PCG.
+class NewmarkDynamicAnalyzer {
+  public:
+    NewmarkDynamicAnalyzer(Context &ctx, const Model &model, std::vector<double> E, int samples,
+                           double alpha, double delta, double timeStep, double totalTime,
+                           double density, MassAccelerationSinusoidalLoad load,
+                           int precision = MSA_MIXED, double rayleighAlpha = 0.0,
+                           double rayleighBeta = 0.0, double nu = 0.3)
+        : ctx_(ctx), model_(model), E_(std::move(E)), samples_(samples), alpha_(alpha),
+          delta_(delta), timeStep_(timeStep), totalTime_(totalTime), density_(density),
+          load_(load), precision_(precision), ra_(rayleighAlpha), rb_(rayleighBeta), nu_(nu) {}
+
+    void BuildMatrices()
+    {
+        mesh_.reset(new Mesh(ctx_, model_));
+        batch_.reset(new Batch(ctx_, {mesh_.get()}, {samples_}, precision_, 0));
+        check(msa_batch_set_modulus(batch_->get(), 0, E_.data()));
+        check(msa_batch_set_rhs(batch_->get(), 0, model_.Forces.data()));
+        check(msa_batch_set_dynamics(batch_->get(), density_, ra_, rb_, alpha_, delta_, timeStep_));
+        check(msa_batch_set_probe(batch_->get(), 0, model_.TotalDOFs - 1));
+        check(msa_batch_assemble(batch_->get(), nu_));
+    }
+
+    int Steps() const { return (int)(totalTime_ / timeStep_); }
+
+    void Solve(double tolerance = 1e-8, int maxIterations = 20000, int graphIters = 16)
+    {
+        if (!batch_) BuildMatrices();
+        std::vector<double> acc(Steps());
+        for (int i = 0; i < Steps(); i++) acc[i] = load_[i];
+        check(msa_batch_newmark(batch_->get(), Steps(), acc.data(), (int)load_.DOF, tolerance,
+                                maxIterations, graphIters, 1));
+    }
+
+    // displacement of the last free DOF after every step
+    std::vector<double> History(int sample) const
+    {
+        std::vector<double> h(Steps());
+        check(msa_batch_get_history(batch_->get(), 0, sample, h.data()));
+        return h;
+    }
+
+  private:
+    Context &ctx_;
+    const Model &model_;
+    std::vector<double> E_;
+    int samples_;
+    double alpha_, delta_, timeStep_, totalTime_, density_;
+    MassAccelerationSinusoidalLoad load_;
+    int precision_;
+    double ra_, rb_, nu_;
+    std::unique_ptr<Mesh> mesh_;
+    std::unique_ptr<Batch> batch_;
+};
+
 }  // namespace msolve

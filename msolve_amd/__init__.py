@@ -8,7 +8,8 @@ from ._lib import (MSA_BREAKDOWN, MSA_FP64, MSA_MIXED, MSA_NAN_INPUT, MSA_NEG_DE
                    MSA_NOT_CONVERGED, MSA_OK, MSA_REFINE_RELIABLE, MSA_REFINE_RESTART,
                    MsaError, STATUS_NAMES, lib)
 from .analyzers import (ElementStructuralStiffnessProvider,
-                        MonteCarloAnalyzerWithStochasticMaterial, SolverPCG)
+                        MonteCarloAnalyzerWithStochasticMaterial, NewmarkDynamicAnalyzer,
+                        SolverPCG, sinusoidal_history)
 from .core import Batch, Context, Mesh, autotune_tile, enumerate_dofs, ke_packed
 from .model import box_mesh, cantilever
 
@@ -17,5 +18,5 @@ __all__ = [
     "MSA_FP64", "MSA_MIXED", "MSA_REFINE_RELIABLE", "MSA_REFINE_RESTART", "MsaError", "STATUS_NAMES", "lib",
     "Context", "Mesh", "Batch", "ke_packed", "enumerate_dofs", "autotune_tile",
     "box_mesh", "cantilever", "ElementStructuralStiffnessProvider", "SolverPCG",
-    "MonteCarloAnalyzerWithStochasticMaterial",
+    "MonteCarloAnalyzerWithStochasticMaterial", "NewmarkDynamicAnalyzer", "sinusoidal_history",
 ]

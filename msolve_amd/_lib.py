@@ -32,6 +32,8 @@ EXPORTS = [
     "msa_batch_get_matrix_csr", "msa_batch_spmv", "msa_batch_set_kernel_timing",
     "msa_batch_get_kernel_timing", "msa_batch_spmv_bytes", "msa_autotune_tile",
     "msa_batch_set_refinement", "msa_batch_set_integration",
+    "msa_batch_set_dynamics", "msa_batch_set_probe", "msa_batch_newmark",
+    "msa_batch_get_history", "msa_batch_get_dynamic_state",
 ]
 
 
@@ -102,6 +104,11 @@ def lib():
         "msa_autotune_tile": [_vp, _vp, _i, _i, _ip, _dp],
         "msa_batch_set_refinement": [_vp, _i, _i, _d],
         "msa_batch_set_integration": [_vp, _i],
+        "msa_batch_set_dynamics": [_vp, _d, _d, _d, _d, _d, _d],
+        "msa_batch_set_probe": [_vp, _i, _i],
+        "msa_batch_newmark": [_vp, _i, _dp, _i, _d, _i, _i, _i],
+        "msa_batch_get_history": [_vp, _i, _i, _dp],
+        "msa_batch_get_dynamic_state": [_vp, _i, _i, _dp, _dp, _dp, _llp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

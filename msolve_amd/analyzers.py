@@ -10,6 +10,10 @@ each backed by the HIP library through the C ABI.
   MonteCarloAnalyzerWithStochasticMaterial
       (ISAAR.MSolve.Analyzers/MonteCarloAnalyzerWithStochasticMaterial.cs:
       182-296): the per-sample loop as one batched device job.
+  NewmarkDynamicAnalyzer
+      (ISAAR.MSolve.Analyzers/NewmarkDynamicAnalyzer.cs:38-315 over
+      ProblemStructural, ISAAR.MSolve.Problems/ProblemStructural.cs:104-211):
+      linear implicit Newmark of many stochastic-modulus samples at once.
 Argument meaning and defaults follow the reference (SolverPCG<T>.Solve() uses
 20000 iterations and 1e-7, SolverPCGSkyline.cs:102-108); errors raise
 MsaError instead of the reference's InvalidOperationException.
@@ -133,3 +137,81 @@ class MonteCarloAnalyzerWithStochasticMaterial:
 
     def Stats(self, cls: int):
         return self.batch.stats(cls)
+
+
+def sinusoidal_history(period: float, timeStep: float, timeLength: float, magnitude: float,
+                       nsteps: int):
+    """MassAccelerationSinusoidalLoad[i] for i < nsteps
+    (ISAAR.MSolve.PreProcessor/MassAccelerationSinusoidalLoad.cs:17-29):
+    magnitude sin(2 pi / period * timeStep * i) for i < (int)(timeLength /
+    timeStep), else 0."""
+    steps = int(timeLength / timeStep)
+    i = np.arange(nsteps, dtype=np.float64)
+    out = magnitude * np.sin(2 * np.pi / period * timeStep * i)
+    out[np.arange(nsteps) >= steps] = 0.0
+    return out
+
+
+class NewmarkDynamicAnalyzer:
+    """Linear implicit Newmark over a batch of stochastic-modulus samples.
+
+    `alpha`, `delta`, `timeStep`, `totalTime` as the reference constructor
+    (NewmarkDynamicAnalyzer.cs:38-49; InitializeCoefficients throws for
+    delta < 0.5 or alpha < (0.5 + delta)^2 / 4, here MsaError); `density`,
+    `rayleighAlpha`, `rayleighBeta` are the Hexa8 element properties
+    (Hexa8.cs:87-89); the base acceleration is a MassAccelerationSinusoidalLoad
+    (period, magnitude) on DOF direction `accelerationDOF` (0/1/2 = X/Y/Z).
+    The loop runs (int)(totalTime / timeStep) steps (:284) from rest; every
+    effective-stiffness solve is a Jacobi-PCG started from the previous
+    step's displacement (the reference's SolverPCG would iterate on a stale
+    residual here; SolverSkyline is its correct solver and the oracle)."""
+
+    def __init__(self, ctx: Context, models, E_per_class, alpha=0.25, delta=0.5,
+                 timeStep=1e-4, totalTime=0.05, density=7.85e-9, rayleighAlpha=0.0,
+                 rayleighBeta=0.0, period=0.05, magnitude=1.0, accelerationDOF=2, nu=0.3,
+                 precision=L.MSA_MIXED, tile=0, static_loads=True):
+        self.ctx = ctx
+        self.models = list(models)
+        self.E = [np.atleast_2d(np.asarray(E, np.float64)) for E in E_per_class]
+        self.alpha, self.delta, self.timeStep, self.totalTime = alpha, delta, timeStep, totalTime
+        self.density, self.rayleighAlpha, self.rayleighBeta = density, rayleighAlpha, rayleighBeta
+        self.period, self.magnitude, self.accelerationDOF = period, magnitude, accelerationDOF
+        self.nu, self.precision, self.tile, self.static_loads = nu, precision, tile, static_loads
+        self.batch = None
+        self.times = {}
+
+    @property
+    def steps(self) -> int:
+        return int(self.totalTime / self.timeStep)
+
+    def BuildMatrices(self, probes=None):
+        self.meshes = [Mesh(self.ctx, m.xyz, m.conn, m.dof, m.n_free) for m in self.models]
+        self.batch = Batch(self.ctx, self.meshes, [E.shape[0] for E in self.E], self.precision,
+                           self.tile)
+        for c, m in enumerate(self.models):
+            self.batch.set_modulus(c, self.E[c])
+            if self.static_loads:
+                self.batch.set_rhs(c, m.f)
+        self.batch.set_dynamics(self.density, self.rayleighAlpha, self.rayleighBeta, self.alpha,
+                                self.delta, self.timeStep)
+        for c, m in enumerate(self.models):
+            dof = m.n_free - 1 if probes is None else probes[c]
+            self.batch.set_probe(c, dof)
+        self.batch.assemble(self.nu)
+
+    def Solve(self, tolerance=1e-8, maxIterations=20000, graph_iters=16):
+        if self.batch is None:
+            self.BuildMatrices()
+        acc = sinusoidal_history(self.period, self.timeStep, self.totalTime, self.magnitude,
+                                 self.steps)
+        t0 = time.perf_counter()
+        self.batch.newmark(acc, self.accelerationDOF, tolerance, maxIterations, graph_iters)
+        self.ctx.synchronize()
+        self.times["all"] = time.perf_counter() - t0
+        return self
+
+    def History(self, cls: int, sample: int):
+        return self.batch.history(cls, sample)
+
+    def State(self, cls: int, sample: int):
+        return self.batch.dynamic_state(cls, sample)

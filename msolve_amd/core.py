@@ -204,6 +204,42 @@ class Batch:
     def set_integration(self, precision=L.MSA_FP64):
         check(lib().msa_batch_set_integration(self.h, precision))
 
+    # ------------------------------------------------------------ dynamics
+    def set_dynamics(self, rho: float, rayleigh_alpha: float = 0.0, rayleigh_beta: float = 0.0,
+                     newmark_alpha: float = 0.25, newmark_delta: float = 0.5, dt: float = 1e-4):
+        """Effective-stiffness mode of linear implicit Newmark
+        (NewmarkDynamicAnalyzer.cs:53-77, ProblemStructural.cs:138-157)."""
+        check(lib().msa_batch_set_dynamics(self.h, rho, rayleigh_alpha, rayleigh_beta,
+                                           newmark_alpha, newmark_delta, dt))
+
+    def set_probe(self, cls: int, dof: int):
+        check(lib().msa_batch_set_probe(self.h, cls, dof))
+
+    def newmark(self, acc, acc_dir: int = 2, tol: float = 1e-10, max_it: int = 20000,
+                graph_iters: int = 16, warm_start: int = 1):
+        """Runs len(acc) steps from rest; acc[i] is the base acceleration of
+        step i (MassAccelerationSinusoidalLoad values).  warm_start: first
+        iterate of each step's PCG, 0 zero, 1 previous displacement, 2 the
+        predictor u + dt v + dt^2/2 a."""
+        acc = f64(acc)
+        self.nsteps = acc.size
+        check(lib().msa_batch_newmark(self.h, acc.size, ptr(acc, C.c_double), acc_dir, tol,
+                                      max_it, graph_iters, int(warm_start)))
+
+    def history(self, cls: int, sample: int):
+        h = np.zeros(self.nsteps)
+        check(lib().msa_batch_get_history(self.h, cls, sample, ptr(h, C.c_double)))
+        return h
+
+    def dynamic_state(self, cls: int, sample: int):
+        n = self.meshes[cls].n_free
+        u, v, a = np.zeros(n), np.zeros(n), np.zeros(n)
+        it = C.c_longlong()
+        check(lib().msa_batch_get_dynamic_state(self.h, cls, sample, ptr(u, C.c_double),
+                                                ptr(v, C.c_double), ptr(a, C.c_double),
+                                                C.byref(it)))
+        return u, v, a, it.value
+
     def set_kernel_timing(self, on: bool):
         check(lib().msa_batch_set_kernel_timing(self.h, 1 if on else 0))
 

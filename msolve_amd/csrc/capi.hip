@@ -36,6 +36,7 @@ namespace {
 
 struct ClassHost {
     msa_mesh *mesh = nullptr;
+    long long hist_cap = 0;  // probe history capacity (doubles)
     int nsamp = 0, ntiles = 0, nslices = 0, nchunks = 0, chunk = 0;
     std::vector<int> slice_ptr, cidx;
     long long tile_vals = 0, tile_vec = 0;
@@ -77,6 +78,13 @@ struct msa_batch {
     int integration = MSA_FP64;
     double *stage = nullptr;  // FP64 staging tiles for MIXED + FP64 integration
     size_t stage_vals = 0;
+    // linear Newmark dynamics (msa_batch_set_dynamics)
+    bool dyn = false;
+    DynCoef dc{};
+    double rho = 0;
+    double *d_acc = nullptr;  // base acceleration history of the current run
+    int acc_cap = 0;
+    int *d_step = nullptr;    // device step counter
     // captured block of PCG iterations (kind 0 plain, 1 with replacement)
     hipGraphExec_t graph = nullptr;
     int graph_iters = 0, graph_kind = -1;
@@ -205,6 +213,13 @@ int launch_iteration_pair(msa_batch *b, int pbuf, double tol, int max_it, double
 ColourLaunch colour_launch(const msa_batch *b, int colour, int *nblocks);
 int max_colours(const msa_batch *b);
 
+int max_tiles(const msa_batch *b)
+{
+    int m = 1;
+    for (auto &c : b->cls) m = std::max(m, c.ntiles);
+    return m;
+}
+
 // Fold the FP32 correction into u64, form the FP64 residual, test it and
 // restart the FP32 recurrence from it (reliable-update refinement).
 int launch_replacement(msa_batch *b, double tol, int max_it)
@@ -216,7 +231,7 @@ int launch_replacement(msa_batch *b, double tol, int max_it)
     for (int k = 0; k < ncol; k++) {
         int nblk = 0;
         ColourLaunch cl = colour_launch(b, k, &nblk);
-        MSA_TRY(timed(b, KK_RES64, [&] { return launch_residual64(s, T, b->d_cls, cl, nblk, b->nu, 1); }));
+        MSA_TRY(timed(b, KK_RES64, [&] { return launch_residual64(s, T, b->d_cls, cl, nblk, b->nu, 1, max_tiles(b)); }));
     }
     MSA_TRY(timed(b, KK_OTHER, [&] {
         return launch_ru_norm(s, T, b->d_cls, b->d_items, ni, tol, max_it, b->d_counts + 1,
@@ -909,7 +924,7 @@ int msa_batch_assemble(msa_batch *b, double nu)
                     const int cnt = C.mesh->h.colour_ptr[k + 1] - C.mesh->h.colour_ptr[k];
                     for (int q = 0; q <= cl.ncls; q++) cl.prefix[q] = q <= (int)c ? 0 : cnt;
                     MSA_TRY(timed(b, KK_ASSEMBLE, [&] {
-                        return launch_assemble(s, 0, b->T, b->d_cls, cl, cnt, nu, t, t + g, b->stage);
+                        return launch_assemble(s, 0, b->T, b->d_cls, cl, cnt, nu, t, t + g, b->stage, g);
                     }));
                 }
                 MSA_TRY(timed(b, KK_OTHER, [&] {
@@ -923,7 +938,7 @@ int msa_batch_assemble(msa_batch *b, double nu)
             ColourLaunch cl = colour_launch(b, k, &nblk);
             const int kprec = mx ? 2 : 0;
             MSA_TRY(timed(b, KK_ASSEMBLE, [&] {
-                return launch_assemble(s, kprec, b->T, b->d_cls, cl, nblk, nu, 0, 1 << 30, nullptr);
+                return launch_assemble(s, kprec, b->T, b->d_cls, cl, nblk, nu, 0, 1 << 30, nullptr, max_tiles(b));
             }));
         }
     }
@@ -949,6 +964,12 @@ int msa_batch_solve(msa_batch *b, double tol, int max_it, double inner_tol, int 
     hipStream_t s = b->ctx->stream;
     const bool mx = is_mixed(b);
     const int T = b->T, ni = (int)b->items.size();
+    if (b->dyn) {
+        // static solve with the effective matrix: per-sample right-hand side = f
+        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_fs_static(s, T, b->d_cls, b->d_items, ni); }));
+        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_fs_norm(s, T, b->d_cls, b->d_items, ni); }));
+        MSA_TRY(reduce(b, OP_FNORM));
+    }
     if (!mx) {
         MSA_HIP(hipMemsetAsync(b->d_counts, 0, sizeof(int), s));
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_pcg_init(s, false, T, b->d_cls, b->d_items, ni, 0, b->d_counts); }));
@@ -957,7 +978,7 @@ int msa_batch_solve(msa_batch *b, double tol, int max_it, double inner_tol, int 
     }
     if (b->refine == MSA_REFINE_RELIABLE) {
         MSA_HIP(hipMemsetAsync(b->d_counts, 0, 2 * sizeof(int), s));
-        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_ir_begin(s, T, b->d_cls, b->d_items, ni, b->d_counts + 1); }));
+        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_ir_begin(s, T, b->d_cls, b->d_items, ni, b->d_counts + 1, 0); }));
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_pcg_init(s, true, T, b->d_cls, b->d_items, ni, 1, b->d_counts); }));
         MSA_TRY(reduce(b, OP_INIT1));
         MSA_TRY(pcg_loop(b, 1, tol, max_it, graph_iters));
@@ -967,7 +988,7 @@ int msa_batch_solve(msa_batch *b, double tol, int max_it, double inner_tol, int 
     MSA_CHECK(inner_tol > 0 && inner_tol < 1 && max_outer >= 1, MSA_ERR_ARG,
               "inner_tol=%g max_outer=%d", inner_tol, max_outer);
     MSA_HIP(hipMemsetAsync(b->d_counts + 1, 0, sizeof(int), s));
-    MSA_TRY(timed(b, KK_OTHER, [&] { return launch_ir_begin(s, T, b->d_cls, b->d_items, ni, b->d_counts + 1); }));
+    MSA_TRY(timed(b, KK_OTHER, [&] { return launch_ir_begin(s, T, b->d_cls, b->d_items, ni, b->d_counts + 1, 0); }));
     const int ncol = max_colours(b);
     for (int outer = 0; outer < max_outer; outer++) {
         int left = 0;
@@ -981,7 +1002,7 @@ int msa_batch_solve(msa_batch *b, double tol, int max_it, double inner_tol, int 
         for (int k = 0; k < ncol; k++) {
             int nblk = 0;
             ColourLaunch cl = colour_launch(b, k, &nblk);
-            MSA_TRY(timed(b, KK_RES64, [&] { return launch_residual64(s, T, b->d_cls, cl, nblk, b->nu, 0); }));
+            MSA_TRY(timed(b, KK_RES64, [&] { return launch_residual64(s, T, b->d_cls, cl, nblk, b->nu, 0, max_tiles(b)); }));
         }
         MSA_TRY(timed(b, KK_OTHER, [&] {
             return launch_ir_norm(s, T, b->d_cls, b->d_items, ni, tol, max_outer, b->d_counts + 1);
@@ -1033,6 +1054,232 @@ int msa_batch_get_solution(msa_batch *b, int cls, int sample, double *u)
         MSA_TRY(extract_lane(b, false, (double *)C.dev.x + (size_t)tile * C.tile_vec, n, lane, h));
     const MeshHost &m = C.mesh->h;
     for (int g = 0; g < m.n_free; g++) u[g] = h[m.ref2int[g]];
+    return MSA_OK;
+}
+
+// ---------------------------------------------------------------- dynamics
+
+int msa_batch_set_dynamics(msa_batch *b, double rho, double rayleigh_alpha, double rayleigh_beta,
+                           double newmark_alpha, double newmark_delta, double dt)
+{
+    MSA_CHECK(b, MSA_ERR_ARG, "null batch");
+    // NewmarkDynamicAnalyzer.InitializeCoefficients (NewmarkDynamicAnalyzer.cs:55-57)
+    MSA_CHECK(newmark_delta >= 0.5, MSA_ERR_ARG,
+              "Newmark delta has to be bigger than 0.5 (delta = %g)", newmark_delta);
+    const double alim = 0.25 * (0.5 + newmark_delta) * (0.5 + newmark_delta);
+    MSA_CHECK(newmark_alpha >= alim, MSA_ERR_ARG, "Newmark alpha has to be bigger than %g (alpha = %g)",
+              alim, newmark_alpha);
+    MSA_CHECK(dt > 0 && std::isfinite(dt) && rho >= 0 && std::isfinite(rho) &&
+                  std::isfinite(rayleigh_alpha) && std::isfinite(rayleigh_beta),
+              MSA_ERR_ARG, "dt=%g rho=%g rayleigh=(%g, %g)", dt, rho, rayleigh_alpha, rayleigh_beta);
+    MSA_HIP(hipSetDevice(b->ctx->device));
+    hipStream_t s = b->ctx->stream;
+    const double al = newmark_alpha, de = newmark_delta;
+    DynCoef &c = b->dc;
+    c.a0 = 1 / (al * dt * dt);
+    c.a1 = de / (al * dt);
+    c.a2 = 1 / (al * dt);
+    c.a3 = 1 / (2 * al) - 1;
+    c.a4 = de / al - 1;
+    c.a5 = dt * 0.5 * (de / al - 2);
+    c.a6 = dt * (1 - de);
+    c.a7 = de * dt;
+    c.alphaR = rayleigh_alpha;
+    c.betaR = rayleigh_beta;
+    c.dt = dt;
+    b->rho = rho;
+    if (!b->d_step) {
+        void *p = nullptr;
+        MSA_TRY(dalloc(b, &p, sizeof(int)));
+        b->d_step = (int *)p;
+    }
+    for (auto &C : b->cls) {
+        ClassDev &D = C.dev;
+        const size_t nvec = (size_t)C.ntiles * C.tile_vec;
+        if (!D.dyn) {
+            double **vecs[6] = {&D.fs, &D.uold, &D.vel, &D.acc, &D.wm, &D.wk};
+            for (double **v : vecs) {
+                void *p = nullptr;
+                MSA_TRY(dalloc(b, &p, nvec * sizeof(double)));
+                *v = (double *)p;
+                MSA_HIP(hipMemsetAsync(p, 0, nvec * sizeof(double), s));
+            }
+            void *p = nullptr;
+            MSA_TRY(dalloc(b, &p, (size_t)D.nb * 3 * sizeof(double)));
+            D.bacc = (double *)p;
+            MSA_HIP(hipMemsetAsync(p, 0, (size_t)D.nb * 3 * sizeof(double), s));
+            D.probe = -1;
+            D.hist = nullptr;
+            D.hist_len = 0;
+        }
+        D.dyn = 1;
+        D.cK = 1.0 + c.a1 * rayleigh_beta;   // K + a1 betaR K
+        D.cM = (c.a0 + c.a1 * rayleigh_alpha) * rho;  // a0 M + a1 alphaR M
+        D.rho = rho;
+    }
+    b->dyn = true;
+    b->assembled = false;
+    return upload_class_table(b);
+}
+
+int msa_batch_set_probe(msa_batch *b, int cls, int dof)
+{
+    MSA_CHECK(b && cls >= 0 && cls < (int)b->cls.size(), MSA_ERR_ARG, "bad arguments");
+    MSA_CHECK(b->dyn, MSA_ERR_STATE, "msa_batch_set_dynamics first");
+    ClassHost &C = b->cls[cls];
+    const MeshHost &m = C.mesh->h;
+    MSA_CHECK(dof >= -1 && dof < m.n_free, MSA_ERR_ARG, "dof %d out of range", dof);
+    C.dev.probe = dof < 0 ? -1 : m.ref2int[dof];
+    return upload_class_table(b);
+}
+
+int msa_batch_newmark(msa_batch *b, int nsteps, const double *acc, int acc_dir, double tol,
+                      int max_it, int graph_iters, int warm_start)
+{
+    MSA_CHECK(b, MSA_ERR_ARG, "null batch");
+    MSA_CHECK(b->dyn, MSA_ERR_STATE, "msa_batch_set_dynamics first");
+    MSA_CHECK(b->assembled, MSA_ERR_STATE, "batch not assembled");
+    MSA_CHECK(nsteps >= 1, MSA_ERR_ARG, "nsteps=%d", nsteps);
+    MSA_CHECK(acc_dir >= -1 && acc_dir <= 2, MSA_ERR_ARG, "acc_dir=%d", acc_dir);
+    MSA_CHECK(acc_dir < 0 || acc, MSA_ERR_ARG, "acceleration history is null");
+    MSA_CHECK(tol > 0 && max_it > 0, MSA_ERR_ARG, "tol=%g max_it=%d", tol, max_it);
+    MSA_CHECK(graph_iters >= 0 && graph_iters % 2 == 0, MSA_ERR_ARG,
+              "graph_iters=%d must be even", graph_iters);
+    MSA_CHECK(warm_start >= 0 && warm_start <= 2, MSA_ERR_ARG, "warm_start=%d", warm_start);
+    const bool mx = is_mixed(b);
+    MSA_CHECK(!mx || b->refine == MSA_REFINE_RELIABLE, MSA_ERR_STATE,
+              "dynamics of a MIXED batch uses MSA_REFINE_RELIABLE");
+    MSA_HIP(hipSetDevice(b->ctx->device));
+    hipStream_t s = b->ctx->stream;
+    const int T = b->T, ni = (int)b->items.size();
+    if (acc_dir >= 0) {
+        if (b->acc_cap < nsteps) {
+            void *p = nullptr;
+            MSA_TRY(dalloc(b, &p, (size_t)nsteps * sizeof(double)));
+            b->d_acc = (double *)p;
+            b->acc_cap = nsteps;
+        }
+        MSA_HIP(hipMemcpyAsync(b->d_acc, acc, (size_t)nsteps * sizeof(double),
+                               hipMemcpyHostToDevice, s));
+    }
+    for (auto &C : b->cls) {
+        if (C.dev.probe < 0) {
+            C.dev.hist_len = 0;
+            continue;
+        }
+        const long long need = (long long)C.nsamp * nsteps;
+        if (C.hist_cap < need) {
+            void *p = nullptr;
+            MSA_TRY(dalloc(b, &p, (size_t)need * sizeof(double)));
+            C.dev.hist = (double *)p;
+            C.hist_cap = need;
+        }
+        C.dev.hist_len = nsteps;
+        MSA_HIP(hipMemsetAsync(C.dev.hist, 0, (size_t)need * sizeof(double), s));
+    }
+    MSA_TRY(upload_class_table(b));
+    const int ncol = max_colours(b);
+    // mass-acceleration load of a unit base acceleration
+    for (auto &C : b->cls)
+        MSA_HIP(hipMemsetAsync(C.dev.bacc, 0, (size_t)C.dev.nb * 3 * sizeof(double), s));
+    if (acc_dir >= 0)
+        for (int k = 0; k < ncol; k++) {
+            int nblk = 0;
+            ColourLaunch cl = colour_launch(b, k, &nblk);
+            MSA_TRY(timed(b, KK_OTHER, [&] { return launch_dyn_bacc(s, b->d_cls, cl, nblk, b->nu, acc_dir); }));
+        }
+    MSA_TRY(timed(b, KK_OTHER, [&] { return launch_dyn_reset(s, mx, T, b->d_cls, b->d_items, ni); }));
+    MSA_TRY(timed(b, KK_OTHER, [&] { return launch_set_int(s, b->d_step, 1, 0); }));
+    DynCoef dc = b->dc;
+    dc.acc = acc_dir >= 0 ? b->d_acc : nullptr;
+    dc.step = b->d_step;
+    dc.start = warm_start;
+    for (int step = 0; step < nsteps; step++) {
+        // right-hand side of the step
+        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_dyn_prep(s, mx, T, b->d_cls, b->d_items, ni, dc); }));
+        for (int k = 0; k < ncol; k++) {
+            int nblk = 0;
+            ColourLaunch cl = colour_launch(b, k, &nblk);
+            MSA_TRY(timed(b, KK_RES64, [&] { return launch_dyn_rhs(s, T, b->d_cls, cl, nblk, b->nu, dc.betaR, max_tiles(b)); }));
+        }
+        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_fs_norm(s, T, b->d_cls, b->d_items, ni); }));
+        MSA_TRY(reduce(b, OP_FNORM));
+        // solve from the previous displacement
+        if (!mx && warm_start) {
+            MSA_HIP(hipMemsetAsync(b->d_counts, 0, sizeof(int), s));
+            MSA_TRY(timed(b, KK_OTHER, [&] { return launch_pcg_init_warm(s, T, b->d_cls, b->d_items, ni); }));
+            MSA_TRY(reduce(b, OP_INIT2, tol));
+            MSA_TRY(pcg_loop(b, 0, tol, max_it, graph_iters));
+        } else if (!mx) {
+            MSA_HIP(hipMemsetAsync(b->d_counts, 0, sizeof(int), s));
+            MSA_TRY(timed(b, KK_OTHER, [&] { return launch_pcg_init(s, false, T, b->d_cls, b->d_items, ni, 0, b->d_counts); }));
+            MSA_TRY(reduce(b, OP_INIT0));
+            MSA_TRY(pcg_loop(b, 0, tol, max_it, graph_iters));
+        } else {
+            MSA_HIP(hipMemsetAsync(b->d_counts, 0, 2 * sizeof(int), s));
+            const int warm = warm_start ? 1 : 0;
+            MSA_TRY(timed(b, KK_OTHER, [&] { return launch_ir_begin(s, T, b->d_cls, b->d_items, ni, b->d_counts + 1, warm); }));
+            for (int k = 0; k < ncol; k++) {
+                int nblk = 0;
+                ColourLaunch cl = colour_launch(b, k, &nblk);
+                MSA_TRY(timed(b, KK_RES64, [&] { return launch_residual64(s, T, b->d_cls, cl, nblk, b->nu, 0, max_tiles(b)); }));
+            }
+            MSA_TRY(timed(b, KK_OTHER, [&] {
+                return launch_ir_norm(s, T, b->d_cls, b->d_items, ni, tol, 1 << 30, b->d_counts + 1);
+            }));
+            MSA_TRY(reduce(b, OP_IRNORM, tol, 0, 1 << 30));
+            MSA_TRY(timed(b, KK_OTHER, [&] { return launch_pcg_init(s, true, T, b->d_cls, b->d_items, ni, 1, b->d_counts); }));
+            MSA_TRY(reduce(b, OP_INIT1));
+            MSA_TRY(pcg_loop(b, 1, tol, max_it, graph_iters));
+        }
+        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_dyn_update(s, mx, T, b->d_cls, b->d_items, ni, dc); }));
+        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_step_advance(s, b->d_step); }));
+    }
+    MSA_HIP(hipStreamSynchronize(s));
+    return MSA_OK;
+}
+
+int msa_batch_get_history(msa_batch *b, int cls, int sample, double *hist)
+{
+    MSA_CHECK(b && hist && cls >= 0 && cls < (int)b->cls.size(), MSA_ERR_ARG, "bad arguments");
+    ClassHost &C = b->cls[cls];
+    MSA_CHECK(sample >= 0 && sample < C.nsamp, MSA_ERR_ARG, "sample %d out of range", sample);
+    MSA_CHECK(C.dev.hist && C.dev.hist_len > 0, MSA_ERR_STATE, "no probe history recorded");
+    MSA_HIP(hipSetDevice(b->ctx->device));
+    MSA_HIP(hipMemcpyAsync(hist, C.dev.hist + (size_t)sample * C.dev.hist_len,
+                           (size_t)C.dev.hist_len * sizeof(double), hipMemcpyDeviceToHost,
+                           b->ctx->stream));
+    MSA_HIP(hipStreamSynchronize(b->ctx->stream));
+    return MSA_OK;
+}
+
+int msa_batch_get_dynamic_state(msa_batch *b, int cls, int sample, double *u, double *v,
+                                double *a, long long *iters)
+{
+    MSA_CHECK(b && cls >= 0 && cls < (int)b->cls.size(), MSA_ERR_ARG, "bad arguments");
+    MSA_CHECK(b->dyn, MSA_ERR_STATE, "not a dynamics batch");
+    ClassHost &C = b->cls[cls];
+    MSA_CHECK(sample >= 0 && sample < C.nsamp, MSA_ERR_ARG, "sample %d out of range", sample);
+    MSA_HIP(hipSetDevice(b->ctx->device));
+    const int tile = sample / b->T, lane = sample % b->T;
+    const size_t n = (size_t)C.mesh->h.nb * 3, off = (size_t)tile * C.tile_vec;
+    const MeshHost &m = C.mesh->h;
+    std::vector<double> h;
+    const double *src[3] = {is_mixed(b) ? C.dev.u64 + off : (const double *)C.dev.x + off,
+                            C.dev.vel + off, C.dev.acc + off};
+    double *dst[3] = {u, v, a};
+    for (int k = 0; k < 3; k++) {
+        if (!dst[k]) continue;
+        MSA_TRY(extract_lane(b, false, src[k], n, lane, h));
+        for (int g = 0; g < m.n_free; g++) dst[k][g] = h[m.ref2int[g]];
+    }
+    if (iters) {
+        SampleState st;
+        MSA_HIP(hipMemcpyAsync(&st, C.dev.st + sample, sizeof(SampleState), hipMemcpyDeviceToHost,
+                               b->ctx->stream));
+        MSA_HIP(hipStreamSynchronize(b->ctx->stream));
+        *iters = st.cum_iters;
+    }
     return MSA_OK;
 }
 

@@ -9,6 +9,7 @@
 // Per-sample dot products are reduced wave -> workgroup -> a partial per
 // work item; k_reduce, launched next, sums the partials of a (class, tile) in
 // a fixed order, so results do not depend on scheduling.
+#include <algorithm>
 #include <cmath>
 
 #include "msa_common.hpp"
@@ -94,6 +95,18 @@ __device__ __forceinline__ void st3(MSA_GLOBAL real *p, size_t i, real x, real y
     q[2] = z;
 }
 
+// Right-hand side of row (br, d) for lane s of a tile at vb: the class's
+// shared load vector, or the per-sample one of the dynamics mode.
+__device__ __forceinline__ double fval(const ClassDev &C, size_t vb, int br, int d, int T, int s)
+{
+    return C.fs ? C.fs[vb + ((size_t)br * 3 + d) * T + s] : C.f[br * 3 + d];
+}
+
+__device__ __forceinline__ double fnorm_of(const ClassDev &C, const SampleState &S)
+{
+    return C.fs ? S.fnorm : C.fnorm;
+}
+
 // ------------------------------------------------------------ Hexa8 stiffness
 
 // 13-digit Gauss abscissa of the reference (GaussQuadrature.cs:73-81,
@@ -114,8 +127,13 @@ constexpr double kSq125 = 0.35355339059327376220042218105242;
 // Coordinates are taken relative to node 0 (the Jacobian is translation-
 // invariant; this keeps the FP32 variant accurate on small elements).
 // Returns true in lanes whose Gauss point has detJ < 1e-8.
+// With Nl/Mh (64 entries each) it also stages the unit-density consistent
+// mass Mh[a*8 + b] = sum_gp N_a N_b detJ (Hexa8.cs:403-436, shape values of
+// CalcH8Shape, Hexa8.cs:137-158); the element mass is rho Mh on each of the
+// three displacement directions.
 template <typename kreal>
-__device__ bool wave_khat(const double xa[3], double nu, kreal *Kh, kreal *g, kreal *w)
+__device__ bool wave_khat(const double xa[3], double nu, kreal *Kh, kreal *g, kreal *w,
+                          kreal *Nl = nullptr, kreal *Mh = nullptr)
 {
     const int lane = threadIdx.x & 63;
     const int gp = lane >> 3, a = lane & 7;
@@ -161,6 +179,9 @@ __device__ bool wave_khat(const double xa[3], double nu, kreal *Kh, kreal *g, kr
     for (int j = 0; j < 3; j++)
         g[lane * 3 + j] = Ji[j][0] * dn[0] + Ji[j][1] * dn[1] + Ji[j][2] * dn[2];
     if (a == 0) w[gp] = det;
+    if (Mh)
+        Nl[lane] = ((1 + sx * xi) * (kreal)0.5) * ((1 + sy * eta) * (kreal)0.5) *
+                   ((1 + sz * zeta) * (kreal)0.5);
     __syncthreads();
 
     const int pa = lane >> 3, pb = lane & 7;
@@ -190,6 +211,12 @@ __device__ bool wave_khat(const double xa[3], double nu, kreal *Kh, kreal *g, kr
     }
 #pragma unroll
     for (int k = 0; k < 9; k++) Kh[lane * 9 + k] = K[k];
+    if (Mh) {
+        kreal m = 0;
+#pragma unroll
+        for (int q8 = 0; q8 < 8; q8++) m += (Nl[q8 * 8 + pa] * Nl[q8 * 8 + pb]) * w[q8];
+        Mh[lane] = m;
+    }
     __syncthreads();
     return det < (kreal)0.00000001;
 }
@@ -335,8 +362,9 @@ __device__ __forceinline__ int colour_class(const ColourLaunch &cl, int item)
     return c;
 }
 
-// Assembly: one wavefront per element of the current colour (all classes),
-// Khat staged in LDS, then val += E_s * Khat for every sample of every tile.
+// Assembly: one wavefront per element of the current colour (all classes)
+// and tile group (blockIdx.y strides over the tiles), Khat staged in LDS,
+// then val += E_s * Khat for every sample of its tiles.
 // Elements of one colour share no node, so the read-modify-writes are
 // race-free.  kreal = precision of the element integration.
 // tile_lo/tile_hi restrict the tiles written; a non-null `stage` receives
@@ -355,20 +383,24 @@ __global__ __launch_bounds__(64) void k_assemble(const ClassDev *__restrict__ cl
     if (item >= cl.prefix[cl.ncls]) return;
     const ClassDev &C = cls[c];
     const int e = C.colour_elems[C.colour_ptr[cl.colour] + (item - cl.prefix[c])];
+    __shared__ kreal Nl[64], Mh[64];
     const int lane = threadIdx.x & 63;
     const int *en = C.conn + (size_t)e * 8;
     const int na = en[lane & 7];
     const double xa[3] = {C.xyz[3 * na], C.xyz[3 * na + 1], C.xyz[3 * na + 2]};
-    const bool neg = wave_khat<kreal>(xa, nu, Kh, g, w);
+    const bool neg = wave_khat<kreal>(xa, nu, Kh, g, w, Nl, C.dyn ? Mh : nullptr);
     if (__ballot(neg) && lane == 0) atomicAdd(C.negdet, 1);
+    // effective stiffness of the dynamics mode: cK E Khat + cM Mhat on the
+    // block diagonals (ProblemStructural.cs:138-157)
+    const real cK = C.dyn ? (real)C.cK : (real)1, cM = (real)C.cM;
     const uint8_t *em = C.emap + (size_t)e * 64;
     const int s = lane % T;
     const int t_end = min(C.ntiles, tile_hi);
-    for (int tile = max(0, tile_lo); tile < t_end; tile++) {
+    for (int tile = max(0, tile_lo) + (int)blockIdx.y; tile < t_end; tile += gridDim.y) {
         if (tile * T + s >= C.nsamp) continue;
         real *val = stage ? stage + (size_t)(tile - tile_lo) * C.tile_vals
                           : (real *)C.val + (size_t)tile * C.tile_vals;
-        const real Es = (real)C.E[((size_t)tile * C.n_elem + e) * T + s];
+        const real Es = (real)C.E[((size_t)tile * C.n_elem + e) * T + s] * cK;
         for (int pe = lane / T; pe < 576; pe += R) {
             const int pair = pe / 9, ent = pe - 9 * pair;
             const int j = em[pair];
@@ -380,7 +412,10 @@ __global__ __launch_bounds__(64) void k_assemble(const ClassDev *__restrict__ cl
             const int br = C.node_block[nA];
             const int slot = C.slice_ptr[br / R] + j;
             const size_t idx = (((size_t)slot * 9 + ent) * R + (br % R)) * T + s;
-            val[idx] += Es * (real)Kh[pe];
+            if (C.dyn && p == q)
+                val[idx] += Es * (real)Kh[pe] + cM * (real)Mh[pair];
+            else
+                val[idx] += Es * (real)Kh[pe];
         }
     }
 }
@@ -451,7 +486,7 @@ __global__ __launch_bounds__(256) void k_pcg_init(const ClassDev *__restrict__ c
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 const size_t o = ((size_t)br * 3 + d) * T + s;
-                const real b = mode ? (real)(r64[o] * sc) : (real)C.f[br * 3 + d];
+                const real b = mode ? (real)(r64[o] * sc) : (real)fval(C, vb, br, d, T, s);
                 const real zv = dinv[o] * b;
                 r[o] = b;
                 x[o] = 0;
@@ -688,11 +723,12 @@ __global__ __launch_bounds__(256) void k_spmv(const ClassDev *__restrict__ cls,
 
 // ------------------------------------------------------------ refinement
 
-// u64 = 0, r64 = f, outer state (mixed precision only).
+// u64 = 0 (warm = 0; warm = 1 keeps the previous solution as the start),
+// r64 = f, outer state (mixed precision only).
 template <int T>
 __global__ __launch_bounds__(256) void k_ir_begin(const ClassDev *__restrict__ cls,
                                                   const int4 *__restrict__ items, int nitems,
-                                                  int *ir_count)
+                                                  int *ir_count, int warm)
 {
     constexpr int R = 64 / T;
     const int item = xcd_item(nitems);
@@ -712,8 +748,8 @@ __global__ __launch_bounds__(256) void k_ir_begin(const ClassDev *__restrict__ c
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 const size_t o = vb + ((size_t)br * 3 + d) * T + s;
-                C.u64[o] = 0;
-                C.r64[o] = C.f[br * 3 + d];
+                if (!warm) C.u64[o] = 0;
+                C.r64[o] = fval(C, vb, br, d, T, s);
             }
         }
     }
@@ -721,16 +757,18 @@ __global__ __launch_bounds__(256) void k_ir_begin(const ClassDev *__restrict__ c
         const int smp = tile * T + threadIdx.x;
         if (smp < C.nsamp) {
             SampleState &S = C.st[smp];
-            S.r64norm = C.fnorm;
-            S.scale = C.fnorm;
+            const double fn = fnorm_of(C, S);
+            S.r64norm = fn;
+            S.scale = fn;
             S.outer = 0;
+            S.iters = 0;
             S.total_iters = 0;
             S.relres = 1.0;
             S.active = 0;
-            if (!isfinite(C.fnorm)) {
+            if (!isfinite(fn)) {
                 S.status = MSA_NAN_INPUT;
                 S.ir_active = 0;
-            } else if (C.fnorm > 1e-25) {
+            } else if (fn > 1e-25) {
                 S.status = MSA_NOT_CONVERGED;
                 S.ir_active = 1;
                 atomicAdd(ir_count, 1);
@@ -769,38 +807,46 @@ __global__ __launch_bounds__(256) void k_ir_update(const ClassDev *__restrict__ 
         for (int d = 0; d < 3; d++) {
             const size_t o = ((size_t)br * 3 + d) * T + s;
             C.u64[vb + o] += sc * (double)x[o];
-            C.r64[vb + o] = C.f[br * 3 + d];
+            C.r64[vb + o] = fval(C, vb, br, d, T, s);
         }
     }
 }
 
-// r64 -= E_s Khat u64_e for one sample on the element's 24 rows; lanes
-// [0, 24) of `half` compute them.
-__device__ __forceinline__ void residual_rows(const ClassDev &C, const int *en, const double *Kh,
-                                              double Es, size_t vb, int T, int s, int i)
+// Element operator on one sample, row i = 3a + p of the element:
+// y[row] += cKe * (Khat xk)_i + cMe * sum_b Mhat[a][b] xm[b, p]; rows and
+// columns of constrained DOFs are skipped (vectors hold free rows only).
+__device__ __forceinline__ void apply_rows(const ClassDev &C, const int *en, const double *Kh,
+                                           const double *Mh, double cKe, double cMe,
+                                           const double *xk, const double *xm, double *y,
+                                           size_t vb, int T, int s, int i)
 {
     const int ia = i / 3, p = i - 3 * (i / 3);
     const int nd = en[ia];
     const int b = C.node_block[nd];
     if (b < 0 || !((C.node_free[nd] >> p) & 1)) return;
-    double y = 0;
+    double yk = 0, ym = 0;
 #pragma unroll
     for (int jn = 0; jn < 8; jn++) {
         const int nj = en[jn];
         const int bj = C.node_block[nj];
         const uint8_t fr = C.node_free[nj];
-        const double *blk = Kh + (ia * 8 + jn) * 9 + p * 3;
+        if (cKe != 0) {
+            const double *blk = Kh + (ia * 8 + jn) * 9 + p * 3;
 #pragma unroll
-        for (int d = 0; d < 3; d++)
-            if (bj >= 0 && ((fr >> d) & 1))
-                y += blk[d] * C.u64[vb + ((size_t)bj * 3 + d) * T + s];
+            for (int d = 0; d < 3; d++)
+                if (bj >= 0 && ((fr >> d) & 1))
+                    yk += blk[d] * xk[vb + ((size_t)bj * 3 + d) * T + s];
+        }
+        if (cMe != 0 && bj >= 0 && ((fr >> p) & 1))
+            ym += Mh[ia * 8 + jn] * xm[vb + ((size_t)bj * 3 + p) * T + s];
     }
-    C.r64[vb + ((size_t)b * 3 + p) * T + s] -= Es * y;
+    y[vb + ((size_t)b * 3 + p) * T + s] += cKe * yk + cMe * ym;
 }
 
 // FP64 residual r64 -= K u64, matrix-free: one wavefront per element of the
 // current colour recomputes Khat in FP64 (LDS) and applies E_s Khat to the
-// element's displacements of every active sample.
+// element's displacements of every active sample (dynamics mode: the
+// effective matrix cK E_s Khat + cM Mhat).
 // need_due = 1: only samples with a pending replacement (reliable updates);
 // one-sample tiles take them from the class's due list, two per pass (lanes
 // 0-23 and 32-55).
@@ -812,6 +858,7 @@ __global__ __launch_bounds__(64) void k_residual64(const ClassDev *__restrict__ 
     __shared__ double Kh[576];
     __shared__ double g[192];
     __shared__ double w[8];
+    __shared__ double Nl[64], Mh[64];
     const int item = blockIdx.x;
     const int c = colour_class(cl, item);
     if (item >= cl.prefix[cl.ncls]) return;
@@ -820,41 +867,53 @@ __global__ __launch_bounds__(64) void k_residual64(const ClassDev *__restrict__ 
     const int lane = threadIdx.x & 63;
     const int ndue = need_due ? *C.due : 0;
     if (need_due && ndue == 0) return;
+    // block y handles due-list entries 2y, 2y+1, 2y+2Y, ... (one-sample tiles)
+    // or tiles y, y+Y, ... (Y = gridDim.y)
+    const int Y = gridDim.y, y = blockIdx.y;
     bool any = false;
     if (T == 1 && need_due) {
-        for (int k = lane; k < ndue; k += 64) {
+        for (int j = lane;; j += 64) {
+            const int k = 2 * y + (j & 1) + 2 * Y * (j >> 1);
+            if (k >= ndue) break;
             const SampleState &S = C.st[C.due[1 + k]];
             any |= S.ir_active && S.ru_due;
         }
     } else {
-        for (int smp = lane; smp < C.nsamp; smp += 64)
-            any |= C.st[smp].ir_active && (!need_due || C.st[smp].ru_due);
+        for (int j = lane;; j += 64) {
+            const int tile = y + (j / T) * Y;
+            if (tile >= C.ntiles) break;
+            const int smp = tile * T + j % T;
+            if (smp < C.nsamp) any |= C.st[smp].ir_active && (!need_due || C.st[smp].ru_due);
+        }
     }
     if (!__ballot(any)) return;
     const int *en = C.conn + (size_t)e * 8;
     const int na = en[lane & 7];
     const double xa[3] = {C.xyz[3 * na], C.xyz[3 * na + 1], C.xyz[3 * na + 2]};
-    (void)wave_khat<double>(xa, nu, Kh, g, w);
+    (void)wave_khat<double>(xa, nu, Kh, g, w, Nl, C.dyn ? Mh : nullptr);
+    const double cK = C.dyn ? C.cK : 1.0, cMe = C.dyn ? -C.cM : 0.0;
     if constexpr (T == 1) {
         if (need_due) {
             const int half = lane >> 5, i = lane & 31;
-            for (int k = half; k < ndue; k += 2) {
+            for (int k = 2 * y + half; k < ndue; k += 2 * Y) {
                 const int smp = C.due[1 + k];
                 const SampleState &S = C.st[smp];
                 if (i >= 24 || !S.ir_active || !S.ru_due) continue;
                 const double Es = C.E[(size_t)smp * C.n_elem + e];
-                residual_rows(C, en, Kh, Es, (size_t)smp * C.tile_vec, 1, 0, i);
+                apply_rows(C, en, Kh, Mh, -(Es * cK), cMe, C.u64, C.u64, C.r64,
+                           (size_t)smp * C.tile_vec, 1, 0, i);
             }
             return;
         }
     }
     const int s = lane % T, sub = lane / T;
-    for (int tile = 0; tile < C.ntiles; tile++) {
+    for (int tile = y; tile < C.ntiles; tile += Y) {
         const int smp = tile * T + s;
         if (smp >= C.nsamp || !C.st[smp].ir_active || (need_due && !C.st[smp].ru_due)) continue;
         const size_t vb = (size_t)tile * C.tile_vec;
         const double Es = C.E[((size_t)tile * C.n_elem + e) * T + s];
-        for (int i = sub; i < 24; i += R) residual_rows(C, en, Kh, Es, vb, T, s, i);
+        for (int i = sub; i < 24; i += R)
+            apply_rows(C, en, Kh, Mh, -(Es * cK), cMe, C.u64, C.u64, C.r64, vb, T, s, i);
     }
 }
 
@@ -926,7 +985,7 @@ __global__ __launch_bounds__(256) void k_ru_fold(const ClassDev *__restrict__ cl
             const size_t o = ((size_t)br * 3 + d) * T + s;
             C.u64[vb + o] += sc * (double)x[o];
             x[o] = 0.f;
-            C.r64[vb + o] = C.f[br * 3 + d];
+            C.r64[vb + o] = fval(C, vb, br, d, T, s);
         }
     }
 }
@@ -1005,6 +1064,340 @@ __global__ __launch_bounds__(256) void k_ru_restart(const ClassDev *__restrict__
         }
     }
     block_partials<T, 1>(acc, C.partial + ((size_t)tile * C.nchunks + chunk) * 3 * T);
+}
+
+// ------------------------------------------------------------ Newmark dynamics
+// Linear implicit Newmark (NewmarkDynamicAnalyzer.cs) for every sample of a
+// batch.  The stored matrix is the effective stiffness cK K + cM M; each time
+// step forms its right-hand side on the device (k_dyn_prep, k_dyn_rhs),
+// solves starting from the previous displacement and updates velocity and
+// acceleration (k_dyn_update).
+
+__global__ void k_set_int(int *p, int n, int v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_step_advance(int *step)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *step += 1;
+}
+
+// Mass-acceleration load of a unit base acceleration in direction dir
+// (Hexa8.CalculateAccelerationForces, Hexa8.cs:512-530: M_e times a vector
+// holding 1 on every DOF of that direction, constrained nodes included;
+// added into the free rows, Model.cs:266-280): bacc[row(a, dir)] +=
+// rho sum_b Mhat[a][b].  One wavefront per element of the colour.
+__global__ __launch_bounds__(64) void k_dyn_bacc(const ClassDev *__restrict__ cls,
+                                                 ColourLaunch cl, double nu, int dir)
+{
+    __shared__ double Kh[576];
+    __shared__ double g[192];
+    __shared__ double w[8];
+    __shared__ double Nl[64], Mh[64];
+    const int item = blockIdx.x;
+    const int c = colour_class(cl, item);
+    if (item >= cl.prefix[cl.ncls]) return;
+    const ClassDev &C = cls[c];
+    const int e = C.colour_elems[C.colour_ptr[cl.colour] + (item - cl.prefix[c])];
+    const int lane = threadIdx.x & 63;
+    const int *en = C.conn + (size_t)e * 8;
+    const int na = en[lane & 7];
+    const double xa[3] = {C.xyz[3 * na], C.xyz[3 * na + 1], C.xyz[3 * na + 2]};
+    (void)wave_khat<double>(xa, nu, Kh, g, w, Nl, Mh);
+    if (lane < 8) {
+        const int b = C.node_block[en[lane]];
+        if (b >= 0 && ((C.node_free[en[lane]] >> dir) & 1)) {
+            double f = 0;
+            for (int jn = 0; jn < 8; jn++) f += C.rho * Mh[lane * 8 + jn];
+            C.bacc[(size_t)b * 3 + dir] += f;
+        }
+    }
+}
+
+// Start of a time history: u = v = a = 0 (from rest), summed iterations 0.
+template <int T>
+__global__ __launch_bounds__(256) void k_dyn_reset(const ClassDev *__restrict__ cls,
+                                                   const int4 *__restrict__ items, int nitems,
+                                                   int mixed)
+{
+    constexpr int R = 64 / T;
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const int tile = it.y, chunk = it.z;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rr = lane / T, s = lane % T;
+    const int sample = tile * T + s;
+    const size_t vb = (size_t)tile * C.tile_vec;
+    double *u = mixed ? C.u64 : (double *)C.x;
+    if (sample < C.nsamp) {
+        const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+        for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+            const int br = slice * R + rr;
+            if (br >= C.nb) continue;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const size_t o = vb + ((size_t)br * 3 + d) * T + s;
+                u[o] = 0;
+                C.vel[o] = 0;
+                C.acc[o] = 0;
+            }
+        }
+    }
+    if (chunk == 0 && threadIdx.x < T) {
+        const int smp = tile * T + threadIdx.x;
+        if (smp < C.nsamp) C.st[smp].cum_iters = 0;
+    }
+}
+
+// Right-hand side operands of a step (NewmarkDynamicAnalyzer.cs:189-206):
+// uold = u, wm = (a0 u + a2 v + a3 a) + alphaR (a1 u + a4 v + a5 a),
+// wk = betaR (a1 u + a4 v + a5 a), fs = f_static + acc(step) bacc; k_dyn_rhs
+// then adds rho M wm + K wk (the reference's M uu + C uc with C = alphaR M +
+// betaR K).  start == 2 replaces u by the predictor u + dt v + dt^2/2 a as
+// the first iterate of the step's solve.
+template <int T>
+__global__ __launch_bounds__(256) void k_dyn_prep(const ClassDev *__restrict__ cls,
+                                                  const int4 *__restrict__ items, int nitems,
+                                                  int mixed, DynCoef dc)
+{
+    constexpr int R = 64 / T;
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const int tile = it.y, chunk = it.z;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rr = lane / T, s = lane % T;
+    const int sample = tile * T + s;
+    if (sample >= C.nsamp) return;
+    const size_t vb = (size_t)tile * C.tile_vec;
+    double *u = mixed ? C.u64 : (double *)C.x;
+    const double ag = dc.acc ? dc.acc[*dc.step] : 0.0;
+    const double h2 = 0.5 * dc.dt * dc.dt;
+    const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+    for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+        const int br = slice * R + rr;
+        if (br >= C.nb) continue;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const size_t o = vb + ((size_t)br * 3 + d) * T + s;
+            const double uu = u[o], vv = C.vel[o], aa = C.acc[o];
+            C.uold[o] = uu;
+            const double w1 = dc.a0 * uu + dc.a2 * vv + dc.a3 * aa;
+            const double w2 = dc.a1 * uu + dc.a4 * vv + dc.a5 * aa;
+            C.wm[o] = w1 + dc.alphaR * w2;
+            C.wk[o] = dc.betaR * w2;
+            C.fs[o] = C.f[br * 3 + d] + C.bacc[br * 3 + d] * ag;
+            // start of the step's solve (the solver keeps u as its first iterate)
+            if (dc.start == 2) u[o] = uu + dc.dt * vv + h2 * aa;
+        }
+    }
+}
+
+// fs += rho Mhat wm + E_s Khat wk per element of the colour, every sample
+// (wk is zero, and skipped, without stiffness-proportional damping).
+template <int T>
+__global__ __launch_bounds__(64) void k_dyn_rhs(const ClassDev *__restrict__ cls,
+                                                ColourLaunch cl, double nu, int stiff)
+{
+    constexpr int R = 64 / T;
+    __shared__ double Kh[576];
+    __shared__ double g[192];
+    __shared__ double w[8];
+    __shared__ double Nl[64], Mh[64];
+    const int item = blockIdx.x;
+    const int c = colour_class(cl, item);
+    if (item >= cl.prefix[cl.ncls]) return;
+    const ClassDev &C = cls[c];
+    const int e = C.colour_elems[C.colour_ptr[cl.colour] + (item - cl.prefix[c])];
+    const int lane = threadIdx.x & 63;
+    const int *en = C.conn + (size_t)e * 8;
+    const int na = en[lane & 7];
+    const double xa[3] = {C.xyz[3 * na], C.xyz[3 * na + 1], C.xyz[3 * na + 2]};
+    (void)wave_khat<double>(xa, nu, Kh, g, w, Nl, Mh);
+    const int Y = gridDim.y, y = blockIdx.y;
+    if constexpr (T == 1) {
+        const int half = lane >> 5, i = lane & 31;
+        if (i < 24)
+            for (int smp = 2 * y + half; smp < C.nsamp; smp += 2 * Y) {
+                const double Es = stiff ? C.E[(size_t)smp * C.n_elem + e] : 0.0;
+                apply_rows(C, en, Kh, Mh, Es, C.rho, C.wk, C.wm, C.fs,
+                           (size_t)smp * C.tile_vec, 1, 0, i);
+            }
+    } else {
+        const int s = lane % T, sub = lane / T;
+        for (int tile = y; tile < C.ntiles; tile += Y) {
+            const int smp = tile * T + s;
+            if (smp >= C.nsamp) continue;
+            const size_t vb = (size_t)tile * C.tile_vec;
+            const double Es = stiff ? C.E[((size_t)tile * C.n_elem + e) * T + s] : 0.0;
+            for (int i = sub; i < 24; i += R)
+                apply_rows(C, en, Kh, Mh, Es, C.rho, C.wk, C.wm, C.fs, vb, T, s, i);
+        }
+    }
+}
+
+// fs = f for every sample (a static solve of a dynamics-mode batch).
+template <int T>
+__global__ __launch_bounds__(256) void k_fs_static(const ClassDev *__restrict__ cls,
+                                                   const int4 *__restrict__ items, int nitems)
+{
+    constexpr int R = 64 / T;
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const int tile = it.y, chunk = it.z;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rr = lane / T, s = lane % T;
+    if (tile * T + s >= C.nsamp) return;
+    const size_t vb = (size_t)tile * C.tile_vec;
+    const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+    for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+        const int br = slice * R + rr;
+        if (br >= C.nb) continue;
+#pragma unroll
+        for (int d = 0; d < 3; d++) C.fs[vb + ((size_t)br * 3 + d) * T + s] = C.f[br * 3 + d];
+    }
+}
+
+// |fs|^2 partials per sample (reduced by OP_FNORM).
+template <int T>
+__global__ __launch_bounds__(256) void k_fs_norm(const ClassDev *__restrict__ cls,
+                                                 const int4 *__restrict__ items, int nitems)
+{
+    constexpr int R = 64 / T;
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const int tile = it.y, chunk = it.z;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rr = lane / T, s = lane % T;
+    const int sample = tile * T + s;
+    const size_t vb = (size_t)tile * C.tile_vec;
+    double acc[1] = {0};
+    if (sample < C.nsamp) {
+        const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+        for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+            const int br = slice * R + rr;
+            if (br >= C.nb) continue;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const double v = C.fs[vb + ((size_t)br * 3 + d) * T + s];
+                acc[0] += v * v;
+            }
+        }
+    }
+    block_partials<T, 1>(acc, C.partial + ((size_t)tile * C.nchunks + chunk) * 3 * T);
+}
+
+// Warm-started FP64 PCG start: x kept (the previous step's displacement),
+// r = fs - K x, z = D^-1 r, p = 0; partials r.z and r.r (OP_INIT2).
+template <int T>
+__global__ __launch_bounds__(256) void k_pcg_init_warm(const ClassDev *__restrict__ cls,
+                                                       const int4 *__restrict__ items, int nitems)
+{
+    constexpr int R = 64 / T;
+    constexpr int ES = R * T;
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const int tile = it.y, chunk = it.z;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rr = lane / T, s = lane % T;
+    const int sample = tile * T + s;
+    const size_t vb = (size_t)tile * C.tile_vec;
+    const double *val = (const double *)C.val + (size_t)tile * C.tile_vals;
+    const double *x = (const double *)C.x + vb;
+    double *r = (double *)C.r + vb, *z = (double *)C.z + vb;
+    double *p0 = (double *)C.p0 + vb, *p1 = (double *)C.p1 + vb;
+    const double *dinv = (const double *)C.dinv + vb;
+    double acc[2] = {0, 0};
+    if (sample < C.nsamp) {
+        const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+        for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+            const int br = slice * R + rr;
+            if (br >= C.nb) continue;
+            const int base = C.slice_ptr[slice], W = C.slice_ptr[slice + 1] - base;
+            double kx[3] = {0, 0, 0};
+            for (int j = 0; j < W; j++) {
+                const int slot = base + j;
+                const int col = C.cidx[(size_t)slot * R + rr];
+                if (col < 0) continue;
+                const size_t xo = (size_t)col * 3 * T + s;
+                const double v0 = x[xo], v1 = x[xo + T], v2 = x[xo + 2 * T];
+                const double *m = val + ((size_t)slot * 9 * R + rr) * T + s;
+                kx[0] += m[0] * v0 + m[ES] * v1 + m[2 * ES] * v2;
+                kx[1] += m[3 * ES] * v0 + m[4 * ES] * v1 + m[5 * ES] * v2;
+                kx[2] += m[6 * ES] * v0 + m[7 * ES] * v1 + m[8 * ES] * v2;
+            }
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const size_t o = ((size_t)br * 3 + d) * T + s;
+                const double rv = fval(C, vb, br, d, T, s) - kx[d];
+                const double zv = dinv[o] * rv;
+                r[o] = rv;
+                z[o] = zv;
+                p0[o] = 0;
+                p1[o] = 0;
+                acc[0] += rv * zv;
+                acc[1] += rv * rv;
+            }
+        }
+    }
+    block_partials<T, 2>(acc, C.partial + ((size_t)tile * C.nchunks + chunk) * 3 * T);
+}
+
+// Velocity and acceleration update (NewmarkDynamicAnalyzer.cs:301-315):
+// a_new = a0 (u - u_old) - a2 v - a3 a, v += a6 a + a7 a_new; the probe row
+// of every sample is recorded; iterations of the step are summed.
+template <int T>
+__global__ __launch_bounds__(256) void k_dyn_update(const ClassDev *__restrict__ cls,
+                                                    const int4 *__restrict__ items, int nitems,
+                                                    int mixed, DynCoef dc)
+{
+    constexpr int R = 64 / T;
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const int tile = it.y, chunk = it.z;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rr = lane / T, s = lane % T;
+    const int sample = tile * T + s;
+    const size_t vb = (size_t)tile * C.tile_vec;
+    const double *u = mixed ? C.u64 : (const double *)C.x;
+    const int step = *dc.step;
+    if (sample < C.nsamp) {
+        const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+        for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+            const int br = slice * R + rr;
+            if (br >= C.nb) continue;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const size_t o = vb + ((size_t)br * 3 + d) * T + s;
+                const double un = u[o], ao = C.acc[o];
+                const double an = dc.a0 * (un - C.uold[o]) - dc.a2 * C.vel[o] - dc.a3 * ao;
+                C.vel[o] = C.vel[o] + (dc.a6 * ao + dc.a7 * an);
+                C.acc[o] = an;
+                if (C.hist && C.probe == br * 3 + d && step < C.hist_len)
+                    C.hist[(size_t)sample * C.hist_len + step] = un;
+            }
+        }
+    }
+    if (chunk == 0 && threadIdx.x < T) {
+        const int smp = tile * T + threadIdx.x;
+        if (smp < C.nsamp) {
+            SampleState &S = C.st[smp];
+            S.cum_iters += mixed ? S.total_iters : S.iters;
+        }
+    }
 }
 
 // ------------------------------------------------------------ KL field
@@ -1127,6 +1520,30 @@ __device__ void apply_op(const ClassDev &C, int smp, const double *v, const Redu
         }
         break;
     }
+    case OP_INIT2: {  // warm-started FP64 solve: |r0|/|f| against the step's |f|
+        S.iters = 0;
+        S.beta = 0;
+        S.ru_due = 0;
+        S.rz = v[0];
+        S.bnorm = S.fnorm;
+        const double err = S.fnorm > 1e-25 ? sqrt(v[1]) / S.fnorm : 0.0;
+        S.relres = err;
+        if (!isfinite(v[0]) || !isfinite(v[1]) || !isfinite(S.fnorm)) {
+            S.status = MSA_NAN_INPUT;
+            S.active = 0;
+        } else if (S.fnorm > 1e-25 && !(err < a.tol)) {
+            S.status = MSA_NOT_CONVERGED;
+            S.active = 1;
+            atomicAdd(a.active_count, 1);
+        } else {
+            S.status = MSA_OK;
+            S.active = 0;
+        }
+        break;
+    }
+    case OP_FNORM:
+        S.fnorm = sqrt(v[0]);
+        break;
     case OP_A:  // alpha = rz / p.q
         if (!S.active) break;
         S.pq = v[0];
@@ -1166,7 +1583,7 @@ __device__ void apply_op(const ClassDev &C, int smp, const double *v, const Redu
             // replacement) fell by ru_drop, or the FP64 stopping test may pass
             if (a.ru_drop > 0) {
                 const double rn = sqrt(v[1]);
-                if (rn < a.ru_drop || rn * S.scale < a.ru_tol * C.fnorm) set_due(C, S, smp);
+                if (rn < a.ru_drop || rn * S.scale < a.ru_tol * fnorm_of(C, S)) set_due(C, S, smp);
             }
         }
         break;
@@ -1177,7 +1594,7 @@ __device__ void apply_op(const ClassDev &C, int smp, const double *v, const Redu
         S.total_iters += S.iters;
         S.outer += 1;
         S.r64norm = nr;
-        const double err = nr / C.fnorm;
+        const double err = nr / fnorm_of(C, S);
         S.relres = err;
         if (!isfinite(err)) {
             S.status = MSA_BREAKDOWN;
@@ -1202,7 +1619,7 @@ __device__ void apply_op(const ClassDev &C, int smp, const double *v, const Redu
         S.outer += 1;
         S.total_iters = S.iters;
         S.r64norm = nr;
-        const double err = nr / C.fnorm;
+        const double err = nr / fnorm_of(C, S);
         S.relres = err;
         int done = 0;
         if (!isfinite(err)) {
@@ -1248,7 +1665,7 @@ __global__ __launch_bounds__(1024) void k_reduce(const ClassDev *__restrict__ cl
     const ClassDev &C = cls[ct.x];
     const int tile = ct.y;
     const int t = threadIdx.x % T, g = threadIdx.x / T;
-    const bool two = a.op == OP_INIT0 || a.op == OP_INIT1 || a.op == OP_B;
+    const bool two = a.op == OP_INIT0 || a.op == OP_INIT1 || a.op == OP_INIT2 || a.op == OP_B;
     const double *base = C.partial + (size_t)tile * C.nchunks * 3 * T + t;
     double a0 = 0, a1 = 0, b0 = 0, b1 = 0;
     int c = g;
@@ -1280,8 +1697,8 @@ __global__ __launch_bounds__(1024) void k_reduce(const ClassDev *__restrict__ cl
         if (smp < C.nsamp) apply_op(C, smp, v, a);
     }
     // these ops clear every replacement flag of the class and append none
-    if ((a.op == OP_INIT0 || a.op == OP_INIT1 || a.op == OP_RURESTART) && tile == 0 &&
-        threadIdx.x == 0)
+    if ((a.op == OP_INIT0 || a.op == OP_INIT1 || a.op == OP_INIT2 || a.op == OP_RURESTART) &&
+        tile == 0 && threadIdx.x == 0)
         *C.due = 0;
 }
 
@@ -1378,23 +1795,31 @@ hipError_t launch_h8k(hipStream_t s, const double *E, const double *B, const dou
     default: return hipErrorInvalidValue;                                         \
     }
 
+// Waves per element along blockIdx.y: enough to fill the chip even for the
+// small colours of small meshes, at most one per tile.
+static inline unsigned tile_waves(int nblocks, int ntiles)
+{
+    const int want = (32768 + nblocks - 1) / std::max(nblocks, 1);
+    return (unsigned)std::max(1, std::min(std::min(ntiles, want), 1024));
+}
+
 hipError_t launch_assemble(hipStream_t s, int kprec, int T, const ClassDev *d_cls,
                            const ColourLaunch &cl, int nblocks, double nu, int tile_lo,
-                           int tile_hi, double *stage)
+                           int tile_hi, double *stage, int ntiles)
 {
     if (nblocks <= 0) return hipSuccess;
+    const dim3 grid(nblocks, tile_waves(nblocks, ntiles));
     if (kprec == 2) {
-        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_assemble<float, float, TT>), dim3(nblocks),
-                                             dim3(64), 0, s, d_cls, cl, nu, tile_lo, tile_hi,
+        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_assemble<float, float, TT>), grid, dim3(64), 0,
+                                             s, d_cls, cl, nu, tile_lo, tile_hi,
                                              (float *)nullptr));
     } else if (kprec == 1) {
-        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_assemble<double, float, TT>), dim3(nblocks),
-                                             dim3(64), 0, s, d_cls, cl, nu, tile_lo, tile_hi,
+        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_assemble<double, float, TT>), grid, dim3(64), 0,
+                                             s, d_cls, cl, nu, tile_lo, tile_hi,
                                              (float *)nullptr));
     } else {
-        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_assemble<double, double, TT>), dim3(nblocks),
-                                             dim3(64), 0, s, d_cls, cl, nu, tile_lo, tile_hi,
-                                             stage));
+        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_assemble<double, double, TT>), grid, dim3(64), 0,
+                                             s, d_cls, cl, nu, tile_lo, tile_hi, stage));
     }
     return hipGetLastError();
 }
@@ -1407,11 +1832,14 @@ hipError_t launch_round(hipStream_t s, const double *src, size_t n, float *dst)
 }
 
 hipError_t launch_residual64(hipStream_t s, int T, const ClassDev *d_cls, const ColourLaunch &cl,
-                             int nblocks, double nu, int need_due)
+                             int nblocks, double nu, int need_due, int ntiles)
 {
     if (nblocks <= 0) return hipSuccess;
-    MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_residual64<TT>), dim3(nblocks), dim3(64), 0, s,
-                                         d_cls, cl, nu, need_due));
+    // replacements: a few dozen due samples per check -> 4 waves per element
+    const dim3 grid(nblocks, need_due ? std::min(4u, tile_waves(nblocks, ntiles))
+                                      : tile_waves(nblocks, ntiles));
+    MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_residual64<TT>), grid, dim3(64), 0, s, d_cls, cl, nu,
+                                         need_due));
     return hipGetLastError();
 }
 
@@ -1468,9 +1896,9 @@ hipError_t launch_spmv(hipStream_t s, bool mixed, int T, const ClassDev *d_cls, 
     } while (0)
 
 hipError_t launch_ir_begin(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,
-                           int nitems, int *ir_count)
+                           int nitems, int *ir_count, int warm)
 {
-    MSA_T_LAUNCH(k_ir_begin, d_cls, items, nitems, ir_count);
+    MSA_T_LAUNCH(k_ir_begin, d_cls, items, nitems, ir_count, warm);
 }
 
 hipError_t launch_ir_update(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,
@@ -1510,6 +1938,74 @@ hipError_t launch_reduce(hipStream_t s, int T, const ClassDev *d_cls, const int2
     MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_reduce<TT>), dim3(ntiles), dim3(1024), 0, s, d_cls,
                                          tiles, a));
     return hipGetLastError();
+}
+
+hipError_t launch_set_int(hipStream_t s, int *p, int n, int v)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_set_int, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_advance(hipStream_t s, int *step)
+{
+    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, s, step);
+    return hipGetLastError();
+}
+
+hipError_t launch_dyn_bacc(hipStream_t s, const ClassDev *d_cls, const ColourLaunch &cl,
+                           int nblocks, double nu, int dir)
+{
+    if (nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dyn_bacc, dim3(nblocks), dim3(64), 0, s, d_cls, cl, nu, dir);
+    return hipGetLastError();
+}
+
+hipError_t launch_dyn_reset(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
+                            const int4 *items, int nitems)
+{
+    MSA_T_LAUNCH(k_dyn_reset, d_cls, items, nitems, (int)mixed);
+}
+
+hipError_t launch_dyn_prep(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
+                           const int4 *items, int nitems, const DynCoef &dc)
+{
+    MSA_T_LAUNCH(k_dyn_prep, d_cls, items, nitems, (int)mixed, dc);
+}
+
+hipError_t launch_dyn_rhs(hipStream_t s, int T, const ClassDev *d_cls, const ColourLaunch &cl,
+                          int nblocks, double nu, double betaR, int ntiles)
+{
+    if (nblocks <= 0) return hipSuccess;
+    const int stiff = betaR != 0.0;
+    const dim3 grid(nblocks, tile_waves(nblocks, T == 1 ? (ntiles + 1) / 2 : ntiles));
+    MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_dyn_rhs<TT>), grid, dim3(64), 0, s, d_cls, cl, nu,
+                                         stiff));
+    return hipGetLastError();
+}
+
+hipError_t launch_fs_static(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,
+                            int nitems)
+{
+    MSA_T_LAUNCH(k_fs_static, d_cls, items, nitems);
+}
+
+hipError_t launch_fs_norm(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,
+                          int nitems)
+{
+    MSA_T_LAUNCH(k_fs_norm, d_cls, items, nitems);
+}
+
+hipError_t launch_pcg_init_warm(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,
+                                int nitems)
+{
+    MSA_T_LAUNCH(k_pcg_init_warm, d_cls, items, nitems);
+}
+
+hipError_t launch_dyn_update(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
+                             const int4 *items, int nitems, const DynCoef &dc)
+{
+    MSA_T_LAUNCH(k_dyn_update, d_cls, items, nitems, (int)mixed, dc);
 }
 
 hipError_t launch_kl_basis(hipStream_t s, const ClassDev *d_cls, int n_elem, int M, int kmax,

@@ -53,8 +53,10 @@ struct SampleState {
     double rz, pq, alpha, beta;  // PCG scalars
     double rr, bnorm, relres;    // |r|^2, |b|, |r|/|b|
     double r64norm, scale;       // mixed: |f - K u|, inner right-hand-side scale
+    double fnorm;                // |f| of a per-sample right-hand side (dynamics)
     int iters, status, active;   // inner PCG
     int outer, total_iters, ir_active, ru_due;  // ru_due: FP64 replacement pending
+    int cum_iters;               // dynamics: iterations summed over the time steps
 };
 
 // Host-side mesh analysis (mesh.cpp).
@@ -105,6 +107,26 @@ struct ClassDev {
     int *negdet;          // [1]
     int *due;             // [1 + nsamp]: count, then samples with a pending replacement
     int colour_ptr[kMaxColours + 1];
+    // linear Newmark dynamics (dyn = 1): the stored matrix is the effective
+    // stiffness cK K + cM Mhat (cM includes the density); per-sample state
+    int dyn, probe;       // probe: internal row 3*block + d recorded each step, or -1
+    int hist_len, pad_dyn;
+    double cK, cM, rho;
+    double *fs;           // [ntiles][tile_vec] right-hand side of the current step
+    double *uold, *vel, *acc, *wm, *wk;  // [ntiles][tile_vec]
+    double *bacc;         // [nb*3] mass-acceleration load of a unit base acceleration
+    double *hist;         // [nsamp][hist_len] probe displacement per step
+};
+
+// Newmark coefficients (NewmarkDynamicAnalyzer.cs:53-77) and the load history.
+struct DynCoef {
+    double a0, a1, a2, a3, a4, a5, a6, a7;
+    double alphaR, betaR;  // Rayleigh damping C = alphaR M + betaR K
+    double dt;
+    const double *acc;     // [nsteps] base acceleration per step (device)
+    int *step;             // current step (device counter)
+    int start;             // PCG start of a step: 0 zero, 1 previous displacement,
+                           // 2 predictor u + dt v + dt^2/2 a
 };
 
 // Elementwise launch over one colour of every class: blocks [prefix[c],
@@ -116,7 +138,11 @@ struct ColourLaunch {
 };
 
 // State updates applied by the reduction kernel after each partial-sum kernel.
-enum ReduceOp { OP_INIT0 = 0, OP_INIT1, OP_A, OP_B, OP_IRNORM, OP_RUNORM, OP_RURESTART };
+enum ReduceOp {
+    OP_INIT0 = 0, OP_INIT1, OP_A, OP_B, OP_IRNORM, OP_RUNORM, OP_RURESTART,
+    OP_FNORM,  // |f| of the per-sample right-hand side
+    OP_INIT2   // start of a warm-started FP64 solve (relres against |f|)
+};
 
 struct ReduceArgs {
     int op;
@@ -142,12 +168,13 @@ hipError_t launch_gauss_matrices(hipStream_t s, const double *xyz, double *out);
 hipError_t launch_h8k(hipStream_t s, const double *E, const double *B, const double *w, double *K);
 // kprec: 0 FP64 integration -> FP64 values, 1 FP64 integration -> FP32
 // values, 2 FP32 integration -> FP32 values
+// ntiles: the most tiles any class of the launch writes (sizes blockIdx.y)
 hipError_t launch_assemble(hipStream_t s, int kprec, int T, const ClassDev *d_cls,
                            const ColourLaunch &cl, int nblocks, double nu, int tile_lo,
-                           int tile_hi, double *stage);
+                           int tile_hi, double *stage, int ntiles);
 hipError_t launch_round(hipStream_t s, const double *src, size_t n, float *dst);
 hipError_t launch_residual64(hipStream_t s, int T, const ClassDev *d_cls, const ColourLaunch &cl,
-                             int nblocks, double nu, int need_due);
+                             int nblocks, double nu, int need_due, int ntiles);
 hipError_t launch_dinv(hipStream_t s, bool mixed, int T, const ClassDev *d_cls, const int4 *items,
                        int nitems);
 hipError_t launch_pcg_init(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
@@ -157,7 +184,7 @@ hipError_t launch_pcg_a(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
 hipError_t launch_pcg_b(hipStream_t s, bool mixed, int T, const ClassDev *d_cls, const int4 *items,
                         int nitems, int pbuf, double tol, int max_it, int *active_count);
 hipError_t launch_ir_begin(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,
-                           int nitems, int *ir_count);
+                           int nitems, int *ir_count, int warm);
 hipError_t launch_ir_update(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,
                             int nitems);
 hipError_t launch_ir_norm(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,
@@ -172,6 +199,25 @@ hipError_t launch_spmv(hipStream_t s, bool mixed, int T, const ClassDev *d_cls, 
                        int nitems, const double *x, double *y);
 hipError_t launch_reduce(hipStream_t s, int T, const ClassDev *d_cls, const int2 *tiles,
                          int ntiles, const ReduceArgs &a);
+// dynamics
+hipError_t launch_set_int(hipStream_t s, int *p, int n, int v);
+hipError_t launch_dyn_bacc(hipStream_t s, const ClassDev *d_cls, const ColourLaunch &cl,
+                           int nblocks, double nu, int dir);
+hipError_t launch_dyn_reset(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
+                            const int4 *items, int nitems);
+hipError_t launch_dyn_prep(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
+                           const int4 *items, int nitems, const DynCoef &dc);
+hipError_t launch_dyn_rhs(hipStream_t s, int T, const ClassDev *d_cls, const ColourLaunch &cl,
+                          int nblocks, double nu, double betaR, int ntiles);
+hipError_t launch_fs_static(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,
+                            int nitems);
+hipError_t launch_fs_norm(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,
+                          int nitems);
+hipError_t launch_pcg_init_warm(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,
+                                int nitems);
+hipError_t launch_dyn_update(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
+                             const int4 *items, int nitems, const DynCoef &dc);
+hipError_t launch_step_advance(hipStream_t s, int *step);
 hipError_t launch_kl_basis(hipStream_t s, const ClassDev *d_cls, int n_elem, int M, int kmax,
                            const double *w1d, const double *a1d, const int *evpPos,
                            const double *eigs, const double *domain, double sigma);

@@ -72,6 +72,15 @@ int main(int argc, char **argv)
             std::string k = "u6_" + std::to_string(s);
             dump(k.c_str(), mc.Solution(1, s));
         }
+        // Newmark of the 4^3 block: density 50, dt 2e-3, 30 steps of a 20 Hz
+        // base acceleration in Z plus the static tip load, FP64 PCG to 1e-12
+        std::vector<double> E4n(m4.ElementCount(), 2.1e5);
+        msolve::MassAccelerationSinusoidalLoad acc(0.05, 2e-3, 30 * 2e-3, 1.0);
+        msolve::NewmarkDynamicAnalyzer nm(ctx, m4, E4n, 1, 0.25, 0.5, 2e-3, 30 * 2e-3, 50.0, acc,
+                                          MSA_FP64);
+        nm.BuildMatrices();
+        nm.Solve(1e-12, 20000, 8);
+        dump("newmark_hist", nm.History(0));
         std::vector<double> ke0(ke.begin(), ke.begin() + 300);
         dump("ke0", ke0, true);
         std::printf("}\n");

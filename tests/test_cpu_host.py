@@ -69,3 +69,15 @@ def test_cpp_host_layer_compiles():
     from msolve_amd.build import build_host_check
     exe = build_host_check(verbose=False)
     assert os.path.exists(exe)
+
+
+def test_sinusoidal_history_matches_oracle():
+    h = msa.sinusoidal_history(0.05, 1e-4, 0.05, 1.0, 503)
+    ref = np.array([O.sinusoidal_acc(i, 1.0, 0.05, 1e-4, 0.05) for i in range(503)])
+    assert np.array_equal(h, ref)
+    assert h[500:].tolist() == [0.0, 0.0, 0.0]
+
+
+def test_newmark_analyzer_step_count():
+    an = msa.NewmarkDynamicAnalyzer(None, [], [], timeStep=1e-4, totalTime=0.05)
+    assert an.steps == 500   # (int)(0.05 / 1e-4), NewmarkDynamicAnalyzer.cs:284

@@ -50,5 +50,11 @@ def test_cpp_host_layer(tmp_path):
     for s in range(2):
         E, _ = O.kl_field(c6.conn, c6.xyz, t, xi[3 + s], c6.domain, 2.1e5, 0.3)
         assert rel(out[f"u6_{s}"], direct(c6, E)) < 1e-10
+    nsteps = int(30 * 2e-3 / 2e-3)
+    h_ref, _, _, _ = O.newmark_skyline(c4, np.full(c4.n_elem, 2.1e5), rho=50.0, dt=2e-3,
+                                       nsteps=nsteps, f_static=c4.f, acc_dir=2, acc_mag=1.0,
+                                       acc_period=0.05, acc_tlen=30 * 2e-3, probe=c4.n - 1)
+    assert len(out["newmark_hist"]) == nsteps
+    assert rel(out["newmark_hist"], h_ref) < 1e-9
     ke, _ = O.ke_mesh(c4.conn[:8], c4.xyz, np.ones(1), 0.3)
     assert np.abs(np.array(out["ke0"]) - ke[0]).max() < 1e-13 * np.abs(ke[0]).max()

@@ -217,6 +217,33 @@ def test_graph_equals_eager(ctx):
     assert np.array_equal(out[0][1], out[1][1])
 
 
+@pytest.mark.parametrize("precision", [L.MSA_FP64, L.MSA_MIXED])
+def test_rerun_with_new_tolerance(ctx, precision):
+    """A second run of the same batch with another tolerance re-captures the
+    iteration graph (the old one is destroyed) and re-assembles in place."""
+    print(f"\n[case] rerun new tol precision={precision}", flush=True)
+    models = [msa.cantilever(4), msa.cantilever(6)]
+    meshes = [msa.Mesh(ctx, m.xyz, m.conn, m.dof, m.n_free) for m in models]
+    rng = np.random.default_rng(21)
+    E = [lognormal_E(rng, 3 * m.n_elem).reshape(3, -1) for m in models]
+    b = msa.Batch(ctx, meshes, [3, 3], precision, 0)
+    for c, m in enumerate(models):
+        b.set_modulus(c, E[c])
+        b.set_rhs(c, m.f)
+    tols = (1e-9, 1e-12) if precision == L.MSA_FP64 else (1e-8, 1e-10)
+    for tol in tols:
+        b.assemble(0.3)
+        b.solve(tol=tol, max_it=5000, graph_iters=8)
+        for c, m in enumerate(models):
+            st = b.stats(c)
+            assert (st["status"] == msa.MSA_OK).all(), st
+            for s in range(3):
+                assert rel(b.solution(c, s), direct_solution(m, E[c][s])) < 30 * tol
+    b.close()
+    for m in meshes:
+        m.close()
+
+
 @pytest.mark.parametrize("method", [L.MSA_REFINE_RELIABLE, L.MSA_REFINE_RESTART])
 def test_mixed_precision_refinement(ctx, method):
     print(f"\n[case] mixed 8^3 method={method}", flush=True)
