@@ -37,6 +37,8 @@ namespace {
 struct ClassHost {
     msa_mesh *mesh = nullptr;
     long long hist_cap = 0;  // probe history capacity (doubles)
+    double kc_nu = -2.0;     // Poisson ratio of the element cache (-2: not built)
+    bool kc_mass = false;    // cache holds the consistent mass too
     int nsamp = 0, ntiles = 0, nslices = 0, nchunks = 0, chunk = 0;
     std::vector<int> slice_ptr, cidx;
     long long tile_vals = 0, tile_vec = 0;
@@ -76,8 +78,6 @@ struct msa_batch {
     double drop = 0.1;    // FP32 residual reduction that triggers a replacement
     // element integration precision of MIXED batches (values stored in FP32)
     int integration = MSA_FP64;
-    double *stage = nullptr;  // FP64 staging tiles for MIXED + FP64 integration
-    size_t stage_vals = 0;
     // linear Newmark dynamics (msa_batch_set_dynamics)
     bool dyn = false;
     DynCoef dc{};
@@ -97,15 +97,6 @@ namespace {
 
 bool is_mixed(const msa_batch *b) { return b->precision == MSA_MIXED; }
 
-// FP64 staging of a mixed batch's assembly: tiles per pass, at most
-// kStageBytes of FP64 values (at least one tile).
-constexpr size_t kStageBytes = (size_t)1 << 30;
-int stage_tiles(const ClassHost &C)
-{
-    const size_t per = (size_t)C.tile_vals * sizeof(double);
-    const size_t g = per ? kStageBytes / per : 1;
-    return (int)std::max<size_t>(1, std::min<size_t>(g, (size_t)C.ntiles));
-}
 size_t real_size(const msa_batch *b) { return is_mixed(b) ? sizeof(float) : sizeof(double); }
 
 int dalloc(msa_batch *b, void **p, size_t bytes)
@@ -701,6 +692,35 @@ int msa_batch_create(msa_ctx *ctx, int n_classes, msa_mesh *const *meshes, const
         A(&p, bfree.size());
         D.block_free = (const uint8_t *)p;
         if (rc == MSA_OK) MSA_HIP(hipMemcpy(p, bfree.data(), bfree.size(), hipMemcpyHostToDevice));
+        {
+            // gather lists of the assembly: element contributions per stored
+            // block, in ascending element order
+            std::vector<int> gmap((size_t)nslots * 8 * R, -1);
+            std::vector<uint8_t> cnt((size_t)nslots * R, 0);
+            for (int e = 0; e < h.n_elem && rc == MSA_OK; e++)
+                for (int a = 0; a < 8; a++) {
+                    const int br = h.node_block[h.conn[8 * e + a]];
+                    if (br < 0) continue;
+                    for (int bb = 0; bb < 8; bb++) {
+                        const int j = h.emap[(size_t)e * 64 + a * 8 + bb];
+                        if (j == 255) continue;
+                        const size_t sr = (size_t)(C.slice_ptr[br / R] + j) * R + br % R;
+                        const int k = cnt[sr]++;
+                        if (k >= 8) {
+                            set_error("mesh: more than 8 elements share a node pair");
+                            rc = MSA_ERR_ARG;
+                            break;
+                        }
+                        gmap[((size_t)(C.slice_ptr[br / R] + j) * 8 + k) * R + br % R] =
+                            e * 64 + a * 8 + bb;
+                    }
+                }
+            A(&p, gmap.size() * sizeof(int));
+            D.gmap = (const int *)p;
+            if (rc == MSA_OK)
+                MSA_HIP(hipMemcpy(p, gmap.data(), gmap.size() * sizeof(int), hipMemcpyHostToDevice));
+            A((void **)&D.kc, (size_t)h.n_elem * 576 * sizeof(double));
+        }
         const size_t nvec = (size_t)C.ntiles * C.tile_vec;
         A(&D.val, (size_t)C.ntiles * C.tile_vals * rs);
         A(&D.x, nvec * rs);
@@ -888,57 +908,45 @@ int msa_batch_assemble(msa_batch *b, double nu)
     MSA_HIP(hipSetDevice(b->ctx->device));
     hipStream_t s = b->ctx->stream;
     const bool mx = is_mixed(b);
-    for (auto &C : b->cls) {
-        const size_t n = (size_t)C.ntiles * C.tile_vals;
-        MSA_TRY(timed(b, KK_OTHER, [&] {
-            return mx ? launch_fill_f32(s, (float *)C.dev.val, n, 0.f)
-                      : launch_fill_f64(s, (double *)C.dev.val, n, 0.0);
-        }));
-        MSA_HIP(hipMemsetAsync(C.dev.negdet, 0, sizeof(int), s));
-    }
-    const int ncol = max_colours(b);
-    if (mx && b->integration == MSA_FP64) {
-        // FP64 integration and accumulation of up to kStageBytes worth of
-        // tiles at a time in a staging buffer, then a single rounding of every
-        // value to FP32
-        size_t need = 0;
-        for (auto &C : b->cls)
-            need = std::max(need, (size_t)C.tile_vals * stage_tiles(C));
-        if (b->stage_vals < need) {  // sized once: the classes never change
-            void *p = nullptr;
-            MSA_TRY(dalloc(b, &p, need * sizeof(double)));
-            b->stage = (double *)p;
-            b->stage_vals = need;
-        }
+    // FP64 element cache (rebuilt when nu or the dynamics mode changes): the
+    // gather assembly and the FP64 residual/right-hand-side kernels read it
+    {
+        bool upload = false;
         for (size_t c = 0; c < b->cls.size(); c++) {
             ClassHost &C = b->cls[c];
-            const int G = stage_tiles(C);
-            for (int t = 0; t < C.ntiles; t += G) {
-                const int g = std::min(G, C.ntiles - t);
-                const size_t n = (size_t)g * C.tile_vals;
-                MSA_TRY(timed(b, KK_OTHER, [&] { return launch_fill_f64(s, b->stage, n, 0.0); }));
-                for (int k = 0; k < C.mesh->h.ncolours; k++) {
-                    ColourLaunch cl{};
-                    cl.ncls = (int)b->cls.size();
-                    cl.colour = k;
-                    const int cnt = C.mesh->h.colour_ptr[k + 1] - C.mesh->h.colour_ptr[k];
-                    for (int q = 0; q <= cl.ncls; q++) cl.prefix[q] = q <= (int)c ? 0 : cnt;
-                    MSA_TRY(timed(b, KK_ASSEMBLE, [&] {
-                        return launch_assemble(s, 0, b->T, b->d_cls, cl, cnt, nu, t, t + g, b->stage, g);
-                    }));
-                }
-                MSA_TRY(timed(b, KK_OTHER, [&] {
-                    return launch_round(s, b->stage, n, (float *)C.dev.val + (size_t)t * C.tile_vals);
-                }));
+            if (C.kc_nu == nu && C.kc_mass == (bool)C.dev.dyn) continue;
+            if (C.dev.dyn && !C.dev.mc) {
+                void *p = nullptr;
+                MSA_TRY(dalloc(b, &p, (size_t)C.mesh->h.n_elem * 64 * sizeof(double)));
+                C.dev.mc = (double *)p;
+                upload = true;
             }
+            if (upload) MSA_TRY(upload_class_table(b));
+            MSA_HIP(hipMemsetAsync(C.dev.negdet, 0, sizeof(int), s));
+            MSA_TRY(timed(b, KK_ASSEMBLE, [&] {
+                return launch_khat_cache(s, b->d_cls + c, C.mesh->h.n_elem, nu, C.dev.dyn);
+            }));
+            C.kc_nu = nu;
+            C.kc_mass = C.dev.dyn;
         }
+    }
+    if (!mx || b->integration == MSA_FP64) {
+        // every stored value summed in FP64 and written once
+        MSA_TRY(timed(b, KK_ASSEMBLE, [&] {
+            return launch_assemble_gather(s, mx, b->T, b->d_cls, b->d_items, (int)b->items.size());
+        }));
     } else {
+        // FP32 element integration: coloured FP32 scatter-add
+        for (auto &C : b->cls) {
+            const size_t n = (size_t)C.ntiles * C.tile_vals;
+            MSA_TRY(timed(b, KK_OTHER, [&] { return launch_fill_f32(s, (float *)C.dev.val, n, 0.f); }));
+        }
+        const int ncol = max_colours(b);
         for (int k = 0; k < ncol; k++) {
             int nblk = 0;
             ColourLaunch cl = colour_launch(b, k, &nblk);
-            const int kprec = mx ? 2 : 0;
             MSA_TRY(timed(b, KK_ASSEMBLE, [&] {
-                return launch_assemble(s, kprec, b->T, b->d_cls, cl, nblk, nu, 0, 1 << 30, nullptr, max_tiles(b));
+                return launch_assemble(s, b->T, b->d_cls, cl, nblk, nu, max_tiles(b));
             }));
         }
     }

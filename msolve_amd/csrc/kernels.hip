@@ -355,6 +355,23 @@ __global__ void k_h8k(const double *__restrict__ E, const double *__restrict__ B
     K[ks] = acc;
 }
 
+// Khat (and, with Mh, the unit-density mass) of element e in LDS: from the
+// class's FP64 element cache when it is built, else computed in place.
+__device__ __forceinline__ void stage_khat(const ClassDev &C, int e, const double xa[3], double nu,
+                                           double *Kh, double *g, double *w, double *Nl,
+                                           double *Mh)
+{
+    const int lane = threadIdx.x & 63;
+    if (C.kc && (!Mh || C.mc)) {
+        const double *k = C.kc + (size_t)e * 576;
+        for (int i = lane; i < 576; i += 64) Kh[i] = k[i];
+        if (Mh) Mh[lane] = C.mc[(size_t)e * 64 + lane];
+        __syncthreads();
+    } else {
+        stage_khat(C, e, xa, nu, Kh, g, w, Nl, Mh);
+    }
+}
+
 __device__ __forceinline__ int colour_class(const ColourLaunch &cl, int item)
 {
     int c = 0;
@@ -362,45 +379,40 @@ __device__ __forceinline__ int colour_class(const ColourLaunch &cl, int item)
     return c;
 }
 
-// Assembly: one wavefront per element of the current colour (all classes)
-// and tile group (blockIdx.y strides over the tiles), Khat staged in LDS,
-// then val += E_s * Khat for every sample of its tiles.
-// Elements of one colour share no node, so the read-modify-writes are
-// race-free.  kreal = precision of the element integration.
-// tile_lo/tile_hi restrict the tiles written; a non-null `stage` receives
-// tiles tile_lo.. (consecutive, tile_vals apart) instead of the class's value
-// array (FP64 staging of a mixed batch, rounded once to FP32 by k_round).
-template <typename kreal, typename real, int T>
+// FP32-integration assembly (msa_batch_set_integration(MSA_MIXED)): one
+// wavefront per element of the current colour (all classes) and tile group
+// (blockIdx.y strides over the tiles), Khat integrated in FP32 and staged in
+// LDS, then val += E_s * Khat for every sample of its tiles.  Elements of
+// one colour share no node, so the read-modify-writes are race-free.
+template <int T>
 __global__ __launch_bounds__(64) void k_assemble(const ClassDev *__restrict__ cls, ColourLaunch cl,
-                                                 double nu, int tile_lo, int tile_hi, real *stage)
+                                                 double nu)
 {
     constexpr int R = 64 / T;
-    __shared__ kreal Kh[576];
-    __shared__ kreal g[192];
-    __shared__ kreal w[8];
+    __shared__ float Kh[576];
+    __shared__ float g[192];
+    __shared__ float w[8];
+    __shared__ float Nl[64], Mh[64];
     const int item = blockIdx.x;
     const int c = colour_class(cl, item);
     if (item >= cl.prefix[cl.ncls]) return;
     const ClassDev &C = cls[c];
     const int e = C.colour_elems[C.colour_ptr[cl.colour] + (item - cl.prefix[c])];
-    __shared__ kreal Nl[64], Mh[64];
     const int lane = threadIdx.x & 63;
     const int *en = C.conn + (size_t)e * 8;
     const int na = en[lane & 7];
     const double xa[3] = {C.xyz[3 * na], C.xyz[3 * na + 1], C.xyz[3 * na + 2]};
-    const bool neg = wave_khat<kreal>(xa, nu, Kh, g, w, Nl, C.dyn ? Mh : nullptr);
-    if (__ballot(neg) && lane == 0) atomicAdd(C.negdet, 1);
+    const bool neg = wave_khat<float>(xa, nu, Kh, g, w, Nl, C.dyn ? Mh : nullptr);
+    if (__ballot(neg) && lane == 0 && blockIdx.y == 0) atomicAdd(C.negdet, 1);
     // effective stiffness of the dynamics mode: cK E Khat + cM Mhat on the
     // block diagonals (ProblemStructural.cs:138-157)
-    const real cK = C.dyn ? (real)C.cK : (real)1, cM = (real)C.cM;
+    const float cK = C.dyn ? (float)C.cK : 1.f, cM = (float)C.cM;
     const uint8_t *em = C.emap + (size_t)e * 64;
     const int s = lane % T;
-    const int t_end = min(C.ntiles, tile_hi);
-    for (int tile = max(0, tile_lo) + (int)blockIdx.y; tile < t_end; tile += gridDim.y) {
+    for (int tile = (int)blockIdx.y; tile < C.ntiles; tile += gridDim.y) {
         if (tile * T + s >= C.nsamp) continue;
-        real *val = stage ? stage + (size_t)(tile - tile_lo) * C.tile_vals
-                          : (real *)C.val + (size_t)tile * C.tile_vals;
-        const real Es = (real)C.E[((size_t)tile * C.n_elem + e) * T + s] * cK;
+        float *val = (float *)C.val + (size_t)tile * C.tile_vals;
+        const float Es = (float)C.E[((size_t)tile * C.n_elem + e) * T + s] * cK;
         for (int pe = lane / T; pe < 576; pe += R) {
             const int pair = pe / 9, ent = pe - 9 * pair;
             const int j = em[pair];
@@ -413,9 +425,95 @@ __global__ __launch_bounds__(64) void k_assemble(const ClassDev *__restrict__ cl
             const int slot = C.slice_ptr[br / R] + j;
             const size_t idx = (((size_t)slot * 9 + ent) * R + (br % R)) * T + s;
             if (C.dyn && p == q)
-                val[idx] += Es * (real)Kh[pe] + cM * (real)Mh[pair];
+                val[idx] += Es * Kh[pe] + cM * Mh[pair];
             else
-                val[idx] += Es * (real)Kh[pe];
+                val[idx] += Es * Kh[pe];
+        }
+    }
+}
+
+// FP64 element cache of a class: Khat of every element (unit modulus) and,
+// with_mass, the unit-density consistent mass; one wavefront per element.
+// Counts elements with a Gauss-point detJ < 1e-8 in negdet.
+__global__ __launch_bounds__(64) void k_khat_cache(const ClassDev *__restrict__ cp, double nu,
+                                                   int with_mass)
+{
+    __shared__ double Kh[576];
+    __shared__ double g[192];
+    __shared__ double w[8];
+    __shared__ double Nl[64], Mh[64];
+    const ClassDev &C = *cp;
+    const int e = blockIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int *en = C.conn + (size_t)e * 8;
+    const int na = en[lane & 7];
+    const double xa[3] = {C.xyz[3 * na], C.xyz[3 * na + 1], C.xyz[3 * na + 2]};
+    const bool neg = wave_khat<double>(xa, nu, Kh, g, w, Nl, with_mass ? Mh : nullptr);
+    if (__ballot(neg) && lane == 0) atomicAdd(C.negdet, 1);
+    MSA_GLOBAL double *k = gbl(C.kc + (size_t)e * 576);
+    for (int i = lane; i < 576; i += 64) k[i] = Kh[i];
+    if (with_mass) gbl(C.mc)[(size_t)e * 64 + lane] = Mh[lane];
+}
+
+// Gather assembly (GlobalMatrixAssemblerSkyline.cs:51-103 restated per
+// stored value): every value of block (row, slot) is the FP64 sum over the
+// elements holding both nodes of E_e cK Khat_e (+ cM Mhat_e on the block
+// diagonal in the dynamics mode), in ascending element order, rounded once
+// to `real` and written once.  Entries of constrained components are zero.
+template <typename real, int T>
+__global__ __launch_bounds__(256) void k_assemble_gather(const ClassDev *__restrict__ cls,
+                                                         const int4 *__restrict__ items,
+                                                         int nitems)
+{
+    constexpr int R = 64 / T;
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const int tile = it.y, chunk = it.z;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rr = lane / T, s = lane % T;
+    if (tile * T + s >= C.nsamp) return;
+    const MSA_GLOBAL double *__restrict__ E = gbl((const double *)C.E + (size_t)tile * C.n_elem * T);
+    MSA_GLOBAL real *__restrict__ val = gbl((real *)C.val + (size_t)tile * C.tile_vals);
+    const MSA_GLOBAL int *__restrict__ gmap = gbl(C.gmap);
+    const MSA_GLOBAL int *__restrict__ cidx = gbl(C.cidx);
+    const MSA_GLOBAL double *__restrict__ kc = gbl((const double *)C.kc);
+    const MSA_GLOBAL double *__restrict__ mc = gbl((const double *)C.mc);
+    const double cK = C.dyn ? C.cK : 1.0, cM = C.dyn ? C.cM : 0.0;
+    const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+    for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+        const int br = slice * R + rr;
+        if (br >= C.nb) continue;
+        const int base = C.slice_ptr[slice], W = C.slice_ptr[slice + 1] - base;
+        const uint8_t fr = C.block_free[br];
+        for (int j = 0; j < W; j++) {
+            const int slot = base + j;
+            const int col = cidx[(size_t)slot * R + rr];
+            double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            if (col >= 0) {
+                for (int gi = 0; gi < 8; gi++) {
+                    const int code = gmap[((size_t)slot * 8 + gi) * R + rr];
+                    if (code < 0) break;
+                    const int e = code >> 6, ab = code & 63;
+                    const double Es = E[(size_t)e * T + s] * cK;
+                    const MSA_GLOBAL double *k = kc + (size_t)e * 576 + ab * 9;
+#pragma unroll
+                    for (int q = 0; q < 9; q++) acc[q] += Es * k[q];
+                    if (cM != 0.0) {
+                        const double m = cM * mc[(size_t)e * 64 + ab];
+                        acc[0] += m;
+                        acc[4] += m;
+                        acc[8] += m;
+                    }
+                }
+                const uint8_t fc = C.block_free[col];
+#pragma unroll
+                for (int q = 0; q < 9; q++)
+                    if (!((fr >> (q / 3)) & 1) || !((fc >> (q % 3)) & 1)) acc[q] = 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < 9; q++) val[(((size_t)slot * 9 + q) * R + rr) * T + s] = (real)acc[q];
         }
     }
 }
@@ -890,7 +988,7 @@ __global__ __launch_bounds__(64) void k_residual64(const ClassDev *__restrict__ 
     const int *en = C.conn + (size_t)e * 8;
     const int na = en[lane & 7];
     const double xa[3] = {C.xyz[3 * na], C.xyz[3 * na + 1], C.xyz[3 * na + 2]};
-    (void)wave_khat<double>(xa, nu, Kh, g, w, Nl, C.dyn ? Mh : nullptr);
+    stage_khat(C, e, xa, nu, Kh, g, w, Nl, C.dyn ? Mh : nullptr);
     const double cK = C.dyn ? C.cK : 1.0, cMe = C.dyn ? -C.cM : 0.0;
     if constexpr (T == 1) {
         if (need_due) {
@@ -1105,7 +1203,7 @@ __global__ __launch_bounds__(64) void k_dyn_bacc(const ClassDev *__restrict__ cl
     const int *en = C.conn + (size_t)e * 8;
     const int na = en[lane & 7];
     const double xa[3] = {C.xyz[3 * na], C.xyz[3 * na + 1], C.xyz[3 * na + 2]};
-    (void)wave_khat<double>(xa, nu, Kh, g, w, Nl, Mh);
+    stage_khat(C, e, xa, nu, Kh, g, w, Nl, Mh);
     if (lane < 8) {
         const int b = C.node_block[en[lane]];
         if (b >= 0 && ((C.node_free[en[lane]] >> dir) & 1)) {
@@ -1218,7 +1316,7 @@ __global__ __launch_bounds__(64) void k_dyn_rhs(const ClassDev *__restrict__ cls
     const int *en = C.conn + (size_t)e * 8;
     const int na = en[lane & 7];
     const double xa[3] = {C.xyz[3 * na], C.xyz[3 * na + 1], C.xyz[3 * na + 2]};
-    (void)wave_khat<double>(xa, nu, Kh, g, w, Nl, Mh);
+    stage_khat(C, e, xa, nu, Kh, g, w, Nl, Mh);
     const int Y = gridDim.y, y = blockIdx.y;
     if constexpr (T == 1) {
         const int half = lane >> 5, i = lane & 31;
@@ -1712,13 +1810,6 @@ __global__ void k_fill(V *p, size_t n, V v)
     for (; i < n; i += stride) p[i] = v;
 }
 
-__global__ void k_round(const double *__restrict__ src, size_t n, float *__restrict__ dst)
-{
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) dst[i] = (float)src[i];
-}
-
 template <typename real>
 __global__ void k_extract(const real *__restrict__ src, size_t n, int T, int lane,
                           double *__restrict__ dst)
@@ -1803,33 +1894,23 @@ static inline unsigned tile_waves(int nblocks, int ntiles)
     return (unsigned)std::max(1, std::min(std::min(ntiles, want), 1024));
 }
 
-hipError_t launch_assemble(hipStream_t s, int kprec, int T, const ClassDev *d_cls,
-                           const ColourLaunch &cl, int nblocks, double nu, int tile_lo,
-                           int tile_hi, double *stage, int ntiles)
+hipError_t launch_assemble(hipStream_t s, int T, const ClassDev *d_cls, const ColourLaunch &cl,
+                           int nblocks, double nu, int ntiles)
 {
     if (nblocks <= 0) return hipSuccess;
     const dim3 grid(nblocks, tile_waves(nblocks, ntiles));
-    if (kprec == 2) {
-        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_assemble<float, float, TT>), grid, dim3(64), 0,
-                                             s, d_cls, cl, nu, tile_lo, tile_hi,
-                                             (float *)nullptr));
-    } else if (kprec == 1) {
-        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_assemble<double, float, TT>), grid, dim3(64), 0,
-                                             s, d_cls, cl, nu, tile_lo, tile_hi,
-                                             (float *)nullptr));
-    } else {
-        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_assemble<double, double, TT>), grid, dim3(64), 0,
-                                             s, d_cls, cl, nu, tile_lo, tile_hi, stage));
-    }
+    MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_assemble<TT>), grid, dim3(64), 0, s, d_cls, cl, nu));
     return hipGetLastError();
 }
 
-hipError_t launch_round(hipStream_t s, const double *src, size_t n, float *dst)
+hipError_t launch_khat_cache(hipStream_t s, const ClassDev *d_cls_c, int n_elem, double nu,
+                             int with_mass)
 {
-    if (!n) return hipSuccess;
-    hipLaunchKernelGGL(k_round, dim3(grid1d(n, 256)), dim3(256), 0, s, src, n, dst);
+    if (n_elem <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_khat_cache, dim3(n_elem), dim3(64), 0, s, d_cls_c, nu, with_mass);
     return hipGetLastError();
 }
+
 
 hipError_t launch_residual64(hipStream_t s, int T, const ClassDev *d_cls, const ColourLaunch &cl,
                              int nblocks, double nu, int need_due, int ntiles)
@@ -1885,6 +1966,12 @@ hipError_t launch_spmv(hipStream_t s, bool mixed, int T, const ClassDev *d_cls, 
                        int nitems, const double *x, double *y)
 {
     MSA_PCG_LAUNCH(k_spmv, d_cls, items, nitems, x, y);
+}
+
+hipError_t launch_assemble_gather(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
+                                  const int4 *items, int nitems)
+{
+    MSA_PCG_LAUNCH(k_assemble_gather, d_cls, items, nitems);
 }
 
 #define MSA_T_LAUNCH(KERNEL, ...)                                                          \

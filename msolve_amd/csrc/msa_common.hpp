@@ -106,6 +106,12 @@ struct ClassDev {
     double *partial;      // [ntiles][nchunks][3][T]
     int *negdet;          // [1]
     int *due;             // [1 + nsamp]: count, then samples with a pending replacement
+    // gather assembly: for every stored block (slot, rr) the contributing
+    // elements as e*64 + a*8 + b, at most 8, ascending e, -1 terminated:
+    // gmap[(slot*8 + g)*R + rr]
+    const int *gmap;
+    double *kc;           // [n_elem][576] unit-modulus Khat, block layout (a*8+b)*9 + 3p+q
+    double *mc;           // [n_elem][64] unit-density consistent mass (dynamics)
     int colour_ptr[kMaxColours + 1];
     // linear Newmark dynamics (dyn = 1): the stored matrix is the effective
     // stiffness cK K + cM Mhat (cM includes the density); per-sample state
@@ -166,13 +172,14 @@ hipError_t launch_ke_packed(hipStream_t s, int n_elem, const double *xyz8, const
                             double nu, double *ke300, int *status);
 hipError_t launch_gauss_matrices(hipStream_t s, const double *xyz, double *out);
 hipError_t launch_h8k(hipStream_t s, const double *E, const double *B, const double *w, double *K);
-// kprec: 0 FP64 integration -> FP64 values, 1 FP64 integration -> FP32
-// values, 2 FP32 integration -> FP32 values
-// ntiles: the most tiles any class of the launch writes (sizes blockIdx.y)
-hipError_t launch_assemble(hipStream_t s, int kprec, int T, const ClassDev *d_cls,
-                           const ColourLaunch &cl, int nblocks, double nu, int tile_lo,
-                           int tile_hi, double *stage, int ntiles);
-hipError_t launch_round(hipStream_t s, const double *src, size_t n, float *dst);
+// FP32-integration scatter assembly; ntiles: the most tiles of any class
+// (sizes blockIdx.y)
+hipError_t launch_assemble(hipStream_t s, int T, const ClassDev *d_cls, const ColourLaunch &cl,
+                           int nblocks, double nu, int ntiles);
+hipError_t launch_khat_cache(hipStream_t s, const ClassDev *d_cls_c, int n_elem, double nu,
+                             int with_mass);
+hipError_t launch_assemble_gather(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
+                                  const int4 *items, int nitems);
 hipError_t launch_residual64(hipStream_t s, int T, const ClassDev *d_cls, const ColourLaunch &cl,
                              int nblocks, double nu, int need_due, int ntiles);
 hipError_t launch_dinv(hipStream_t s, bool mixed, int T, const ClassDev *d_cls, const int4 *items,
