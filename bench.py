@@ -57,7 +57,7 @@ def parse():
     p.add_argument("--max-outer", type=int, default=30)
     p.add_argument("--graph-iters", type=int, default=16)
     p.add_argument("--tile", type=int, default=1)
-    p.add_argument("--period", type=int, default=4, help="replacement check period (iterations)")
+    p.add_argument("--period", type=int, default=8, help="replacement check period (iterations)")
     p.add_argument("--drop", type=float, default=0.1, help="FP32 residual drop that triggers a replacement")
     p.add_argument("--refine", choices=["reliable", "restart"], default="reliable")
     p.add_argument("--only-mixed", action="store_true", help="skip every comparison run")

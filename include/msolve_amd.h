@@ -59,6 +59,17 @@ void CALCH8GAUSSMATRICES(const int *iInt, const double *xyz, double *w,
                          double *B);
 void CALCH8K(const int *iInt, const double *E, const double *B,
              const double *w, double *K);
+/* The remaining femelements.dll imports of Hexa8.cs:35-61: CalcH8Strains
+ * (rElHexa8.f90:635-648) strains[8][6] = B_M u for u[24]; CalcH8Forces
+ * (:724-747) forces[24] = sum_M w[M] B_M^T stresses[M] for stresses[8][6];
+ * CalcH8MLumped (:673-699) M[300] packed, sum(w) density / 8 on the 24
+ * diagonal entries. */
+void CALCH8STRAINS(const int *iInt, const double *B, const double *u,
+                   double *strains);
+void CALCH8FORCES(const int *iInt, const double *B, const double *w,
+                  const double *stresses, double *forces);
+void CALCH8MLUMPED(const int *iInt, const double *density, const double *w,
+                   double *M);
 
 /* --------------------------------------------------------------- context */
 int msa_ctx_create(int device, msa_ctx **out);
@@ -176,7 +187,7 @@ int msa_batch_solve(msa_batch *b, double tol, int max_it, double inner_tol,
  * is FP64 only).  MSA_REFINE_RELIABLE (default): the FP32 PCG never restarts;
  * a sample whose FP32 residual has fallen by `drop` (default 0.1) since its
  * last replacement -- or whose estimated FP64 residual passes tol -- gets,
- * at the next check (every `period` iterations, even, default 4), its FP32
+ * at the next check (every `period` iterations, even, default 8), its FP32
  * correction folded into the FP64 solution, its residual replaced by the
  * FP64 f - K u, the stopping test |f - K u|/|f| < tol, and the FP32
  * recurrence continued from that residual (search direction rescaled);
@@ -205,6 +216,16 @@ int msa_batch_get_solution(msa_batch *b, int cls, int sample, double *u);
  * status[n].  Any pointer may be NULL. */
 int msa_batch_get_stats(msa_batch *b, int cls, int *iters, int *outer,
                         double *relres, int *status);
+/* Element results of one solved sample (Hexa8.CalculateStresses and
+ * CalculateForces, ISAAR.MSolve.PreProcessor/Elements/Hexa8.cs:453-510, with
+ * ElasticMaterial3D stresses, Materials/ElasticMaterial3D.cs:20-43): per
+ * element and Gauss point (xi outer, zeta inner) strains[n_elem][8][6] =
+ * B u_e (Voigt xx, yy, zz, xy, yz, zx, engineering shear), stresses
+ * [n_elem][8][6] = D(E_e, nu) strain, internal forces forces[n_elem][24] =
+ * sum w B^T sigma (= Ke u_e).  Any pointer may be NULL.  FP64 from the FP64
+ * solution of the last solve. */
+int msa_batch_element_results(msa_batch *b, int cls, int sample, double *strains,
+                              double *stresses, double *forces);
 /* The modulus field of one sample, E[n_elem]. */
 int msa_batch_get_modulus(msa_batch *b, int cls, int sample, double *E);
 

@@ -23,6 +23,7 @@ STATUS_NAMES = {0: "OK", 1: "NOT_CONVERGED", 2: "BREAKDOWN", 3: "NEG_DETJ", 4: "
 # every symbol include/msolve_amd.h declares
 EXPORTS = [
     "msa_last_error", "msa_version", "CALCH8GAUSSMATRICES", "CALCH8K",
+    "CALCH8STRAINS", "CALCH8FORCES", "CALCH8MLUMPED", "msa_batch_element_results",
     "msa_ctx_create", "msa_ctx_destroy", "msa_ctx_stream", "msa_ctx_synchronize",
     "msa_enumerate_dofs", "msa_mesh_create", "msa_mesh_info", "msa_mesh_destroy",
     "msa_ke_packed", "msa_batch_create", "msa_batch_destroy", "msa_batch_bytes",
@@ -74,6 +75,12 @@ def lib():
     L.CALCH8GAUSSMATRICES.argtypes = [_ip, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
     L.CALCH8K.restype = None
     L.CALCH8K.argtypes = [_ip, _dp, _dp, _dp, _dp]
+    L.CALCH8STRAINS.restype = None
+    L.CALCH8STRAINS.argtypes = [_ip, _dp, _dp, _dp]
+    L.CALCH8FORCES.restype = None
+    L.CALCH8FORCES.argtypes = [_ip, _dp, _dp, _dp, _dp]
+    L.CALCH8MLUMPED.restype = None
+    L.CALCH8MLUMPED.argtypes = [_ip, _dp, _dp, _dp]
     sig = {
         "msa_ctx_create": [_i, C.POINTER(_vp)],
         "msa_ctx_destroy": [_vp],
@@ -96,6 +103,7 @@ def lib():
         "msa_batch_get_solution": [_vp, _i, _i, _dp],
         "msa_batch_get_stats": [_vp, _i, _ip, _ip, _dp, _ip],
         "msa_batch_get_modulus": [_vp, _i, _i, _dp],
+        "msa_batch_element_results": [_vp, _i, _i, _dp, _dp, _dp],
         "msa_batch_get_matrix_csr": [_vp, _i, _i, _llp, _ip, _dp],
         "msa_batch_spmv": [_vp, _i, _i, _dp, _dp],
         "msa_batch_set_kernel_timing": [_vp, _i],

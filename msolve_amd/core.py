@@ -177,6 +177,16 @@ class Batch:
                                         ptr(rel, C.c_double), ptr(st, C.c_int)))
         return dict(iters=it, outer=outer, relres=rel, status=st)
 
+    def element_results(self, cls: int, sample: int):
+        """strains [n_elem, 8, 6], stresses [n_elem, 8, 6], internal forces
+        [n_elem, 24] of one solved sample (Hexa8.CalculateStresses /
+        CalculateForces)."""
+        ne = self.meshes[cls].n_elem
+        eps, sig, f = np.zeros((ne, 8, 6)), np.zeros((ne, 8, 6)), np.zeros((ne, 24))
+        check(lib().msa_batch_element_results(self.h, cls, sample, ptr(eps, C.c_double),
+                                              ptr(sig, C.c_double), ptr(f, C.c_double)))
+        return eps, sig, f
+
     def modulus(self, cls: int, sample: int):
         E = np.zeros(self.meshes[cls].n_elem)
         check(lib().msa_batch_get_modulus(self.h, cls, sample, ptr(E, C.c_double)))
@@ -198,7 +208,7 @@ class Batch:
         check(lib().msa_batch_spmv(self.h, cls, sample, ptr(x, C.c_double), ptr(y, C.c_double)))
         return y
 
-    def set_refinement(self, method=L.MSA_REFINE_RELIABLE, period: int = 4, drop: float = 0.1):
+    def set_refinement(self, method=L.MSA_REFINE_RELIABLE, period: int = 8, drop: float = 0.1):
         check(lib().msa_batch_set_refinement(self.h, method, period, drop))
 
     def set_integration(self, precision=L.MSA_FP64):

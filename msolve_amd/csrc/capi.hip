@@ -74,7 +74,7 @@ struct msa_batch {
     // mixed-precision refinement: restarted inner solves or periodic
     // residual replacement
     int refine = MSA_REFINE_RELIABLE;
-    int period = 4;       // iterations between replacement checks
+    int period = 8;       // iterations between replacement checks
     double drop = 0.1;    // FP32 residual reduction that triggers a replacement
     // element integration precision of MIXED batches (values stored in FP32)
     int integration = MSA_FP64;
@@ -577,6 +577,71 @@ void CALCH8K(const int *iInt, const double *E, const double *B, const double *w,
         set_error("CALCH8K: iInt=%d unsupported (only 2)", *iInt);
     }
     fill_nan(K, 300);
+}
+
+void CALCH8STRAINS(const int *iInt, const double *B, const double *u, double *strains)
+{
+    msa_ctx *ctx = nullptr;
+    bool ok = iInt && *iInt == 2 && B && u && default_ctx(&ctx) == MSA_OK;
+    if (ok) {
+        std::lock_guard<std::mutex> lock(g_default_mutex);
+        double *dB = g_drop_buf, *du = g_drop_buf + 1152, *dS = g_drop_buf + 1176;
+        hipError_t e = hipSetDevice(ctx->device);
+        if (e == hipSuccess) e = hipMemcpyAsync(dB, B, 1152 * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(du, u, 24 * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = launch_h8strains(ctx->stream, dB, du, dS);
+        if (e == hipSuccess) e = hipMemcpyAsync(strains, dS, 48 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess) return;
+        set_error("CALCH8STRAINS: %s", hipGetErrorString(e));
+    } else if (iInt && *iInt != 2) {
+        set_error("CALCH8STRAINS: iInt=%d unsupported (only 2)", *iInt);
+    }
+    fill_nan(strains, 48);
+}
+
+void CALCH8FORCES(const int *iInt, const double *B, const double *w, const double *stresses,
+                  double *forces)
+{
+    msa_ctx *ctx = nullptr;
+    bool ok = iInt && *iInt == 2 && B && w && stresses && default_ctx(&ctx) == MSA_OK;
+    if (ok) {
+        std::lock_guard<std::mutex> lock(g_default_mutex);
+        double *dB = g_drop_buf, *dw = g_drop_buf + 1152, *dS = g_drop_buf + 1160,
+               *dF = g_drop_buf + 1208;
+        hipError_t e = hipSetDevice(ctx->device);
+        if (e == hipSuccess) e = hipMemcpyAsync(dB, B, 1152 * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(dw, w, 8 * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(dS, stresses, 48 * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = launch_h8forces(ctx->stream, dB, dw, dS, dF);
+        if (e == hipSuccess) e = hipMemcpyAsync(forces, dF, 24 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess) return;
+        set_error("CALCH8FORCES: %s", hipGetErrorString(e));
+    } else if (iInt && *iInt != 2) {
+        set_error("CALCH8FORCES: iInt=%d unsupported (only 2)", *iInt);
+    }
+    fill_nan(forces, 24);
+}
+
+void CALCH8MLUMPED(const int *iInt, const double *density, const double *w, double *M)
+{
+    msa_ctx *ctx = nullptr;
+    bool ok = iInt && *iInt == 2 && density && w && default_ctx(&ctx) == MSA_OK;
+    if (ok) {
+        std::lock_guard<std::mutex> lock(g_default_mutex);
+        double *dw = g_drop_buf, *dM = g_drop_buf + 8;
+        hipError_t e = hipSetDevice(ctx->device);
+        if (e == hipSuccess) e = hipMemcpyAsync(dw, w, 8 * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = launch_h8mlumped(ctx->stream, *density, dw, dM);
+        if (e == hipSuccess) e = hipMemcpyAsync(M, dM, 300 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess) return;
+        set_error("CALCH8MLUMPED: %s", hipGetErrorString(e));
+    } else if (iInt && *iInt != 2) {
+        set_error("CALCH8MLUMPED: iInt=%d unsupported (only 2)", *iInt);
+    }
+    fill_nan(M, 300);
 }
 
 // ---------------------------------------------------------------- batch
@@ -1288,6 +1353,37 @@ int msa_batch_get_dynamic_state(msa_batch *b, int cls, int sample, double *u, do
         MSA_HIP(hipStreamSynchronize(b->ctx->stream));
         *iters = st.cum_iters;
     }
+    return MSA_OK;
+}
+
+int msa_batch_element_results(msa_batch *b, int cls, int sample, double *strains,
+                              double *stresses, double *forces)
+{
+    MSA_CHECK(b && cls >= 0 && cls < (int)b->cls.size(), MSA_ERR_ARG, "bad arguments");
+    MSA_CHECK(b->assembled, MSA_ERR_STATE, "batch not assembled");
+    ClassHost &C = b->cls[cls];
+    MSA_CHECK(sample >= 0 && sample < C.nsamp, MSA_ERR_ARG, "sample %d out of range", sample);
+    MSA_HIP(hipSetDevice(b->ctx->device));
+    hipStream_t s = b->ctx->stream;
+    const int ne = C.mesh->h.n_elem, tile = sample / b->T, lane = sample % b->T;
+    const double *u = (is_mixed(b) ? C.dev.u64 : (const double *)C.dev.x) + (size_t)tile * C.tile_vec;
+    double *d = nullptr;
+    const size_t n = (size_t)ne * 120;
+    MSA_HIP(hipMalloc(&d, n * sizeof(double)));
+    hipError_t e = launch_element_results(s, b->T, b->d_cls + cls, ne, u, lane, tile, b->nu,
+                                          strains ? d : nullptr, stresses ? d + (size_t)ne * 48 : nullptr,
+                                          forces ? d + (size_t)ne * 96 : nullptr);
+    if (e == hipSuccess && strains)
+        e = hipMemcpyAsync(strains, d, (size_t)ne * 48 * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && stresses)
+        e = hipMemcpyAsync(stresses, d + (size_t)ne * 48, (size_t)ne * 48 * sizeof(double),
+                           hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && forces)
+        e = hipMemcpyAsync(forces, d + (size_t)ne * 96, (size_t)ne * 24 * sizeof(double),
+                           hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(d);
+    MSA_HIP(e);
     return MSA_OK;
 }
 

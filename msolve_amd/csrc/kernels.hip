@@ -355,6 +355,137 @@ __global__ void k_h8k(const double *__restrict__ E, const double *__restrict__ B
     K[ks] = acc;
 }
 
+// CalcH8Strains (rElHexa8.f90:635-648): strains(r, M) = sum_c B(r, c, M) u(c),
+// lane = M*8 + r (r < 6); B Fortran (6, 24, 8).
+__global__ void k_h8strains(const double *__restrict__ B, const double *__restrict__ u,
+                            double *__restrict__ strains)
+{
+    const int M = threadIdx.x >> 3, r = threadIdx.x & 7;
+    if (M >= 8 || r >= 6) return;
+    double acc = 0.0;
+    for (int c = 0; c < 24; c++) acc += B[(M * 24 + c) * 6 + r] * u[c];
+    strains[M * 6 + r] = acc;
+}
+
+// CalcH8Forces (rElHexa8.f90:724-747): forces(c) = sum_M w(M) sum_r B(r, c, M)
+// stresses(r, M), lane = c.
+__global__ void k_h8forces(const double *__restrict__ B, const double *__restrict__ w,
+                           const double *__restrict__ stresses, double *__restrict__ forces)
+{
+    const int c = threadIdx.x;
+    if (c >= 24) return;
+    double f = 0.0;
+    for (int M = 0; M < 8; M++) {
+        double acc = 0.0;
+        for (int r = 0; r < 6; r++) acc += B[(M * 24 + c) * 6 + r] * stresses[M * 6 + r];
+        f = f + w[M] * acc;
+    }
+    forces[c] = f;
+}
+
+// CalcH8MLumped (rElHexa8.f90:673-699): sum(w) rho / 8 on the 24 diagonal
+// entries of the packed upper 300, zero elsewhere.
+__global__ void k_h8mlumped(double density, const double *__restrict__ w, double *__restrict__ M)
+{
+    double term = 0.0;
+    for (int m = 0; m < 8; m++) term = term + w[m];
+    term = term * density * 0.125;
+    for (int i = threadIdx.x; i < 300; i += blockDim.x) {
+        // diagonal positions: 0, 24, 47, ... (row I starts at sum_{k<I} (24 - k))
+        int I = 0, start = 0;
+        while (start + (24 - I) <= i) {
+            start += 24 - I;
+            I++;
+        }
+        M[i] = (i == start) ? term : 0.0;
+    }
+}
+
+// Element results of one sample (Hexa8.CalculateStresses + CalculateForces,
+// Hexa8.cs:453-510 with CalcH8Strains/CalcH8Forces and ElasticMaterial3D):
+// per element (one wavefront) and Gauss point M the strains B_M u_e, the
+// stresses D(E_e, nu) strain and the internal forces sum_M w_M B_M^T
+// sigma_M (w_M = detJ, the 2-point Gauss weights are 1).  u is the sample's
+// FP64 solution (internal rows, lane s of its tile); constrained DOFs are 0.
+template <int T>
+__global__ __launch_bounds__(64) void k_element_results(const ClassDev *__restrict__ cp,
+                                                        const double *__restrict__ u, int s,
+                                                        int tile, double nu,
+                                                        double *__restrict__ strains,
+                                                        double *__restrict__ stresses,
+                                                        double *__restrict__ forces)
+{
+    __shared__ double Kh[576];
+    __shared__ double g[192];
+    __shared__ double w[8];
+    __shared__ double ue[24];
+    __shared__ double eps[48], sig[48];
+    const ClassDev &C = *cp;
+    const int e = blockIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int *en = C.conn + (size_t)e * 8;
+    const int na = en[lane & 7];
+    const double xa[3] = {C.xyz[3 * na], C.xyz[3 * na + 1], C.xyz[3 * na + 2]};
+    (void)wave_khat<double>(xa, nu, Kh, g, w);
+    if (lane < 24) {
+        const int nd = en[lane / 3], d = lane % 3;
+        const int b = C.node_block[nd];
+        ue[lane] = (b >= 0 && ((C.node_free[nd] >> d) & 1)) ? u[((size_t)b * 3 + d) * T + s] : 0.0;
+    }
+    __syncthreads();
+    const int M = lane >> 3, r = lane & 7;
+    if (r < 6) {
+        // B(r, 3a+j): rows xx, yy, zz, xy, yz, zx of the Voigt order
+        double acc = 0.0;
+        for (int a = 0; a < 8; a++) {
+            const double *ga = g + (M * 8 + a) * 3;
+            const double ux = ue[3 * a], uy = ue[3 * a + 1], uz = ue[3 * a + 2];
+            switch (r) {
+            case 0: acc += ga[0] * ux; break;
+            case 1: acc += ga[1] * uy; break;
+            case 2: acc += ga[2] * uz; break;
+            case 3: acc += ga[1] * ux; acc += ga[0] * uy; break;
+            case 4: acc += ga[2] * uy; acc += ga[1] * uz; break;
+            default: acc += ga[2] * ux; acc += ga[0] * uz; break;
+            }
+        }
+        eps[M * 6 + r] = acc;
+    }
+    __syncthreads();
+    const double Ee = C.E[((size_t)tile * C.n_elem + e) * T + s];
+    if (r < 6) {
+        // ElasticMaterial3D.GetConstitutiveMatrix (ElasticMaterial3D.cs:20-43)
+        const double e1 = Ee / (1 + nu), e2 = e1 * nu / (1 - 2 * nu), e3 = e1 + e2, e4 = e1 * 0.5;
+        const double *ep = eps + M * 6;
+        double v;
+        if (r < 3)
+            v = (r == 0 ? e3 : e2) * ep[0] + (r == 1 ? e3 : e2) * ep[1] + (r == 2 ? e3 : e2) * ep[2];
+        else
+            v = e4 * ep[r];
+        sig[M * 6 + r] = v;
+        if (strains) strains[(size_t)e * 48 + M * 6 + r] = ep[r];
+        if (stresses) stresses[(size_t)e * 48 + M * 6 + r] = v;
+    }
+    __syncthreads();
+    if (forces && lane < 24) {
+        const int a = lane / 3, j = lane % 3;
+        double f = 0.0;
+        for (int m = 0; m < 8; m++) {
+            const double *ga = g + (m * 8 + a) * 3;
+            const double *sg = sig + m * 6;
+            double acc;
+            if (j == 0)
+                acc = ga[0] * sg[0] + ga[1] * sg[3] + ga[2] * sg[5];
+            else if (j == 1)
+                acc = ga[1] * sg[1] + ga[0] * sg[3] + ga[2] * sg[4];
+            else
+                acc = ga[2] * sg[2] + ga[1] * sg[4] + ga[0] * sg[5];
+            f = f + w[m] * acc;
+        }
+        forces[(size_t)e * 24 + lane] = f;
+    }
+}
+
 // Khat (and, with Mh, the unit-density mass) of element e in LDS: from the
 // class's FP64 element cache when it is built, else computed in place.
 __device__ __forceinline__ void stage_khat(const ClassDev &C, int e, const double xa[3], double nu,
@@ -1868,6 +1999,25 @@ hipError_t launch_gauss_matrices(hipStream_t s, const double *xyz, double *out)
     return hipGetLastError();
 }
 
+hipError_t launch_h8strains(hipStream_t s, const double *B, const double *u, double *strains)
+{
+    hipLaunchKernelGGL(k_h8strains, dim3(1), dim3(64), 0, s, B, u, strains);
+    return hipGetLastError();
+}
+
+hipError_t launch_h8forces(hipStream_t s, const double *B, const double *w, const double *stresses,
+                           double *forces)
+{
+    hipLaunchKernelGGL(k_h8forces, dim3(1), dim3(64), 0, s, B, w, stresses, forces);
+    return hipGetLastError();
+}
+
+hipError_t launch_h8mlumped(hipStream_t s, double density, const double *w, double *M)
+{
+    hipLaunchKernelGGL(k_h8mlumped, dim3(1), dim3(64), 0, s, density, w, M);
+    return hipGetLastError();
+}
+
 hipError_t launch_h8k(hipStream_t s, const double *E, const double *B, const double *w, double *K)
 {
     hipLaunchKernelGGL(k_h8k, dim3(5), dim3(64), 0, s, E, B, w, K);
@@ -1903,6 +2053,16 @@ hipError_t launch_assemble(hipStream_t s, int T, const ClassDev *d_cls, const Co
     return hipGetLastError();
 }
 
+hipError_t launch_element_results(hipStream_t s, int T, const ClassDev *d_cls_c, int n_elem,
+                                  const double *u, int lane, int tile, double nu, double *strains,
+                                  double *stresses, double *forces)
+{
+    if (n_elem <= 0) return hipSuccess;
+    MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_element_results<TT>), dim3(n_elem), dim3(64), 0, s,
+                                         d_cls_c, u, lane, tile, nu, strains, stresses, forces));
+    return hipGetLastError();
+}
+
 hipError_t launch_khat_cache(hipStream_t s, const ClassDev *d_cls_c, int n_elem, double nu,
                              int with_mass)
 {
@@ -1916,8 +2076,9 @@ hipError_t launch_residual64(hipStream_t s, int T, const ClassDev *d_cls, const 
                              int nblocks, double nu, int need_due, int ntiles)
 {
     if (nblocks <= 0) return hipSuccess;
-    // replacements: a few dozen due samples per check -> 4 waves per element
-    const dim3 grid(nblocks, need_due ? std::min(4u, tile_waves(nblocks, ntiles))
+    // replacements: a few dozen due samples per check -> 2 waves per element
+    // (each wave stages the element's 4.6 KB Khat)
+    const dim3 grid(nblocks, need_due ? std::min(2u, tile_waves(nblocks, ntiles))
                                       : tile_waves(nblocks, ntiles));
     MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_residual64<TT>), grid, dim3(64), 0, s, d_cls, cl, nu,
                                          need_due));

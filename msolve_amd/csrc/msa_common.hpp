@@ -172,6 +172,13 @@ hipError_t launch_ke_packed(hipStream_t s, int n_elem, const double *xyz8, const
                             double nu, double *ke300, int *status);
 hipError_t launch_gauss_matrices(hipStream_t s, const double *xyz, double *out);
 hipError_t launch_h8k(hipStream_t s, const double *E, const double *B, const double *w, double *K);
+hipError_t launch_h8strains(hipStream_t s, const double *B, const double *u, double *strains);
+hipError_t launch_h8forces(hipStream_t s, const double *B, const double *w, const double *stresses,
+                           double *forces);
+hipError_t launch_h8mlumped(hipStream_t s, double density, const double *w, double *M);
+hipError_t launch_element_results(hipStream_t s, int T, const ClassDev *d_cls_c, int n_elem,
+                                  const double *u, int lane, int tile, double nu, double *strains,
+                                  double *stresses, double *forces);
 // FP32-integration scatter assembly; ntiles: the most tiles of any class
 // (sizes blockIdx.y)
 hipError_t launch_assemble(hipStream_t s, int T, const ClassDev *d_cls, const ColourLaunch &cl,

@@ -1123,6 +1123,98 @@ int orc_mc_csr_jacobi(int nex, int ney, int nez, double nu, int M, int kmax,
 
 
 /* ------------------------------------------------------------------------ */
+/* Element results                                                           */
+/* ------------------------------------------------------------------------ */
+
+/* CalcH8Strains (rElHexa8.f90:635-648): strains(:, M) = B(:, :, M) u per
+ * Gauss point; B is Fortran (6, 24, M) (column-major: B[(M*24 + c)*6 + r]),
+ * strains (6, M). */
+void orc_h8_strains(int iInt, const double *B, const double *u, double *strains)
+{
+    const int M3 = iInt * iInt * iInt;
+    for (int m = 0; m < M3; m++)
+        for (int r = 0; r < 6; r++) {
+            double acc = 0.0;
+            for (int c = 0; c < 24; c++) acc += B[((size_t)m * 24 + c) * 6 + r] * u[c];
+            strains[m * 6 + r] = acc;
+        }
+}
+
+/* CalcH8Forces (rElHexa8.f90:724-747): forces = sum_M w(M) B(:,:,M)^T
+ * stresses(:, M), Gauss points in the order of the loops iX, iY, iZ. */
+void orc_h8_forces(int iInt, const double *B, const double *w, const double *stresses,
+                   double *forces)
+{
+    const int M3 = iInt * iInt * iInt;
+    for (int c = 0; c < 24; c++) forces[c] = 0.0;
+    for (int m = 0; m < M3; m++)
+        for (int c = 0; c < 24; c++) {
+            double acc = 0.0;
+            for (int r = 0; r < 6; r++) acc += B[((size_t)m * 24 + c) * 6 + r] * stresses[m * 6 + r];
+            forces[c] = forces[c] + w[m] * acc;
+        }
+}
+
+/* CalcH8MLumped (rElHexa8.f90:673-699): diagonal mass sum(w) rho / 8 on the
+ * 24 diagonal entries of the packed upper 300 (positions 1, 25, 48, ...). */
+void orc_h8_mlumped(int iInt, double density, const double *w, double *M300)
+{
+    const int M3 = iInt * iInt * iInt;
+    for (int i = 0; i < 300; i++) M300[i] = 0.0;
+    double term = 0.0;
+    for (int m = 0; m < M3; m++) term = term + w[m];
+    term = term * density * 0.125;
+    int pos = 0;
+    for (int s1 = 24; s1 >= 1; s1--) {
+        M300[pos] = term;
+        pos += s1;
+    }
+}
+
+/* Stresses of ElasticMaterial3D (ElasticMaterial3D.cs:20-43): D(E, nu) times
+ * the strains, D as built there (fE1 = E/(1+nu), fE2 = fE1 nu/(1-2nu),
+ * fE3 = fE1 + fE2, fE4 = fE1/2), the product summed over the 6 columns. */
+void orc_elastic_stress(double E, double nu, const double *strain, double *stress)
+{
+    const double e1 = E / (1 + nu), e2 = e1 * nu / (1 - 2 * nu), e3 = e1 + e2, e4 = e1 * 0.5;
+    const double D[6][6] = {{e3, e2, e2, 0, 0, 0}, {e2, e3, e2, 0, 0, 0}, {e2, e2, e3, 0, 0, 0},
+                            {0, 0, 0, e4, 0, 0},   {0, 0, 0, 0, e4, 0},   {0, 0, 0, 0, 0, e4}};
+    for (int i = 0; i < 6; i++) {
+        double acc = 0.0;
+        for (int j = 0; j < 6; j++) acc += D[i][j] * strain[j];
+        stress[i] = acc;
+    }
+}
+
+/* Per-element results of a solved model, as Hexa8.CalculateStresses +
+ * CalculateForces (Hexa8.cs:453-510) evaluate them element by element:
+ * local displacements (constrained DOFs 0), Gauss matrices, strains per
+ * Gauss point, stresses D(E_e) strain, internal forces sum w B^T sigma.
+ * strains/stresses [n_elem][8][6], forces [n_elem][24] (any may be NULL). */
+void orc_element_results(int n_elem, const int *conn, const double *xyz, const int *dof,
+                         const double *u, const double *E, double nu, double *strains,
+                         double *stresses, double *forces)
+{
+    double w[8], S[64], DS[192], J[72], dJ[8], B[1152], x8[24], ue[24], st[48], sg[48], fe[24];
+    for (int e = 0; e < n_elem; e++) {
+        for (int a = 0; a < 8; a++)
+            for (int d = 0; d < 3; d++) {
+                const int n = conn[8 * e + a];
+                x8[3 * a + d] = xyz[3 * n + d];
+                const int g = dof[3 * n + d];
+                ue[3 * a + d] = g >= 0 ? u[g] : 0.0;
+            }
+        orc_h8_gauss_matrices(2, x8, w, S, DS, J, dJ, B);
+        orc_h8_strains(2, B, ue, st);
+        for (int m = 0; m < 8; m++) orc_elastic_stress(E[e], nu, st + 6 * m, sg + 6 * m);
+        orc_h8_forces(2, B, w, sg, fe);
+        if (strains) memcpy(strains + (size_t)e * 48, st, sizeof(st));
+        if (stresses) memcpy(stresses + (size_t)e * 48, sg, sizeof(sg));
+        if (forces) memcpy(forces + (size_t)e * 24, fe, sizeof(fe));
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* Newmark dynamics                                                          */
 /* ------------------------------------------------------------------------ */
 

@@ -101,6 +101,16 @@ int orc_mc_csr_jacobi(int nex, int ney, int nez, double nu, int M, int kmax,
                       int n_samples, const double *xi, double tol, int max_it,
                       double *u_out, int *iters, double *times);
 
+/* Element results (rElHexa8.f90 CalcH8Strains/Forces/MLumped) -------------- */
+void orc_h8_strains(int iInt, const double *B, const double *u, double *strains);
+void orc_h8_forces(int iInt, const double *B, const double *w, const double *stresses,
+                   double *forces);
+void orc_h8_mlumped(int iInt, double density, const double *w, double *M300);
+void orc_elastic_stress(double E, double nu, const double *strain, double *stress);
+void orc_element_results(int n_elem, const int *conn, const double *xyz, const int *dof,
+                         const double *u, const double *E, double nu, double *strains,
+                         double *stresses, double *forces);
+
 /* Newmark dynamics (NewmarkDynamicAnalyzer.cs, ProblemStructural.cs) ------- */
 double orc_sinusoidal_acc(int i, double mag, double period, double dt, double tlen);
 int orc_newmark_skyline(int n_elem, const int *conn, const double *xyz, const int *dof,

@@ -55,6 +55,12 @@ def lib():
         "orc_ke_mesh": (I, [I, _i32p, _f64p, _f64p, D, _f64p]),
         "orc_ke_element_csharp": (V, [_f64p, D, D, _f64p]),
         "orc_h8_mass_csharp": (V, [_f64p, D, _f64p]),
+        "orc_h8_strains": (V, [I, _f64p, _f64p, _f64p]),
+        "orc_h8_forces": (V, [I, _f64p, _f64p, _f64p, _f64p]),
+        "orc_h8_mlumped": (V, [I, D, _f64p, _f64p]),
+        "orc_elastic_stress": (V, [D, D, _f64p, _f64p]),
+        "orc_element_results": (V, [I, _i32p, _f64p, _i32p, _f64p, _f64p, D, _f64p, _f64p,
+                                    _f64p]),
         "orc_sinusoidal_acc": (D, [I, D, D, D, D]),
         "orc_newmark_skyline": (I, [I, _i32p, _f64p, _i32p, I, _f64p, D, D, D, D, D, D, D, I,
                                     C.c_void_p, I, D, D, D, I, _f64p, _f64p, _f64p, _f64p]),
@@ -166,6 +172,37 @@ def h8_mass_csharp(xyz8, rho):
     M = np.zeros(300)
     lib().orc_h8_mass_csharp(np.ascontiguousarray(xyz8, np.float64).ravel(), rho, M)
     return M
+
+
+def h8_strains(B, u):
+    out = np.zeros(48)
+    lib().orc_h8_strains(2, np.ascontiguousarray(B, np.float64).ravel(),
+                         np.ascontiguousarray(u, np.float64), out)
+    return out.reshape(8, 6)
+
+
+def h8_forces(B, w, stresses):
+    out = np.zeros(24)
+    lib().orc_h8_forces(2, np.ascontiguousarray(B, np.float64).ravel(),
+                        np.ascontiguousarray(w, np.float64),
+                        np.ascontiguousarray(stresses, np.float64).ravel(), out)
+    return out
+
+
+def h8_mlumped(density, w):
+    out = np.zeros(300)
+    lib().orc_h8_mlumped(2, density, np.ascontiguousarray(w, np.float64), out)
+    return out
+
+
+def element_results(model, u, E, nu=0.3):
+    """strains/stresses [n_elem, 8, 6] and internal forces [n_elem, 24]."""
+    ne = model.n_elem
+    eps, sig, f = np.zeros(ne * 48), np.zeros(ne * 48), np.zeros(ne * 24)
+    lib().orc_element_results(ne, model.conn, model.xyz, model.dof,
+                              np.ascontiguousarray(u, np.float64),
+                              np.ascontiguousarray(E, np.float64), nu, eps, sig, f)
+    return eps.reshape(ne, 8, 6), sig.reshape(ne, 8, 6), f.reshape(ne, 24)
 
 
 def sinusoidal_acc(i, mag, period, dt, tlen):
