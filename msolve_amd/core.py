@@ -202,6 +202,13 @@ class Batch:
                                              ptr(col, C.c_int), ptr(val, C.c_double)))
         return rp, col, val
 
+    def matrix_skyline(self, cls: int, sample: int):
+        """Assembled K of one sample as SkylineMatrix2D arrays (rowIndex,
+        data), e.g. for matrix_io.write_skyline."""
+        from .matrix_io import csr_to_skyline
+        rp, col, val = self.matrix_csr(cls, sample)
+        return csr_to_skyline(rp, col, val, self.meshes[cls].n_free)
+
     def spmv(self, cls: int, sample: int, x):
         x = f64(x)
         y = np.zeros(self.meshes[cls].n_free)
