@@ -35,6 +35,7 @@ enum {
 
 enum { MSA_FP64 = 0, MSA_MIXED = 1 };
 enum { MSA_REFINE_RESTART = 0, MSA_REFINE_RELIABLE = 1 };
+enum { MSA_SOLVER_PCG = 0, MSA_SOLVER_DIRECT = 1 };
 
 typedef struct msa_ctx msa_ctx;
 typedef struct msa_mesh msa_mesh;
@@ -248,6 +249,30 @@ int msa_batch_get_kernel_timing(msa_batch *b, double *ms_total,
 /* Algorithmic bytes of one launch of the SpMV+dot kernel over every sample
  * of the batch (values + shared indices + vectors), and its element count. */
 int msa_batch_spmv_bytes(msa_batch *b, double *bytes_per_launch);
+
+/* ----------------------------------------------------------- direct solver
+ * Banded Cholesky K = L L^T of every sample's assembled FP64 stiffness on the
+ * GPU: the SkylineMatrix2D.Factorize / Solve pair behind SolverSkyline
+ * (ISAAR.MSolve.Matrices/SkylineMatrix2D.cs:63-212,
+ * ISAAR.MSolve.Solvers/Skyline/SolverSkyline.cs:90-205) restated as a
+ * blocked band Cholesky (64-column blocks: diagonal factor, panel solve,
+ * trailing update of the band window) in the free-DOF order with the
+ * smallest half bandwidth among the block order, reverse Cuthill-McKee and
+ * the lexicographic coordinate orders.  FP64 batches only.  Storage per
+ * sample: n * W doubles (msa_batch_direct_info).  A non-positive pivot gives
+ * the sample MSA_BREAKDOWN. */
+int msa_batch_factorize(msa_batch *b);
+/* u = K^-1 f for every sample from the factors (factorizing first when the
+ * assembled values changed); msa_batch_get_stats then reports 0 iterations,
+ * |f - K u|/|f| and the status. */
+int msa_batch_direct_solve(msa_batch *b);
+/* MSA_SOLVER_PCG (default) or MSA_SOLVER_DIRECT for msa_batch_solve,
+ * msa_batch_run and the steps of msa_batch_newmark (the effective stiffness
+ * is factorized once and every step is two band sweeps). */
+int msa_batch_set_solver(msa_batch *b, int solver);
+/* info[0] band dimension n (free DOFs), [1] half bandwidth, [2] storage
+ * width W, [3] factor bytes per sample, for class cls. */
+int msa_batch_direct_info(msa_batch *b, int cls, long long *info);
 
 /* ----------------------------------------------------------- dynamics
  * Linear implicit Newmark time integration of every sample of the batch

@@ -18,6 +18,7 @@ MSA_OK, MSA_NOT_CONVERGED, MSA_BREAKDOWN, MSA_NEG_DETJ, MSA_NAN_INPUT = 0, 1, 2,
 MSA_ERR_ARG, MSA_ERR_HIP, MSA_ERR_NO_DEVICE, MSA_ERR_OOM, MSA_ERR_STATE = -1, -2, -3, -4, -5
 MSA_FP64, MSA_MIXED = 0, 1
 MSA_REFINE_RESTART, MSA_REFINE_RELIABLE = 0, 1
+MSA_SOLVER_PCG, MSA_SOLVER_DIRECT = 0, 1
 STATUS_NAMES = {0: "OK", 1: "NOT_CONVERGED", 2: "BREAKDOWN", 3: "NEG_DETJ", 4: "NAN_INPUT"}
 
 # every symbol include/msolve_amd.h declares
@@ -35,6 +36,8 @@ EXPORTS = [
     "msa_batch_set_refinement", "msa_batch_set_integration",
     "msa_batch_set_dynamics", "msa_batch_set_probe", "msa_batch_newmark",
     "msa_batch_get_history", "msa_batch_get_dynamic_state",
+    "msa_batch_factorize", "msa_batch_direct_solve", "msa_batch_set_solver",
+    "msa_batch_direct_info",
 ]
 
 
@@ -117,6 +120,10 @@ def lib():
         "msa_batch_newmark": [_vp, _i, _dp, _i, _d, _i, _i, _i],
         "msa_batch_get_history": [_vp, _i, _i, _dp],
         "msa_batch_get_dynamic_state": [_vp, _i, _i, _dp, _dp, _dp, _llp],
+        "msa_batch_factorize": [_vp],
+        "msa_batch_direct_solve": [_vp],
+        "msa_batch_set_solver": [_vp, _i],
+        "msa_batch_direct_info": [_vp, _i, _llp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

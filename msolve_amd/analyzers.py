@@ -7,6 +7,8 @@ each backed by the HIP library through the C ABI.
   SolverPCG                           IIterativeSolver / ISolver over one model
       (ISAAR.MSolve.Solvers/PCGSkyline/SolverPCGSkyline.cs:28-157 with the
       Jacobi calculator of the commented-out :85-91,164)
+  SolverSkyline                       ISolver, direct (ISAAR.MSolve.Solvers/
+      Skyline/SolverSkyline.cs:55-205) on the GPU band Cholesky
   MonteCarloAnalyzerWithStochasticMaterial
       (ISAAR.MSolve.Analyzers/MonteCarloAnalyzerWithStochasticMaterial.cs:
       182-296): the per-sample loop as one batched device job.
@@ -78,6 +80,40 @@ class SolverPCG:
         self.CurrentIteration = n - 1 if self.Status == L.MSA_OK and n > 0 else n
         self.Iterations = n
         self.RelativeResidual = float(st["relres"][0])
+        return self.VectorX
+
+
+class SolverSkyline:
+    """Direct FP64 solve of one model (SolverSkyline, ISAAR.MSolve.Solvers/
+    Skyline/SolverSkyline.cs:55-205): Initialize() assembles and factorizes
+    (a banded Cholesky on the GPU in place of the skyline LDL^T), Solve()
+    forward/back-substitutes; the solution is in VectorX."""
+
+    def __init__(self, ctx: Context, model, E, nu=0.3):
+        self.ctx, self.model, self.nu = ctx, model, nu
+        self.E = np.ascontiguousarray(E, np.float64).ravel()
+        self.mesh = None
+        self.batch = None
+        self.VectorX = None
+
+    def Initialize(self):
+        m = self.model
+        self.mesh = Mesh(self.ctx, m.xyz, m.conn, m.dof, m.n_free)
+        self.batch = Batch(self.ctx, [self.mesh], [1], L.MSA_FP64, 1)
+        self.batch.set_modulus(0, self.E)
+        self.batch.set_rhs(0, m.f)
+        self.batch.set_solver(L.MSA_SOLVER_DIRECT)
+        self.batch.assemble(self.nu)
+        self.batch.factorize()
+
+    def Solve(self):
+        if self.batch is None:
+            self.Initialize()
+        self.batch.direct_solve()
+        st = self.batch.stats(0)
+        self.Status = int(st["status"][0])
+        self.RelativeResidual = float(st["relres"][0])
+        self.VectorX = self.batch.solution(0, 0)
         return self.VectorX
 
 

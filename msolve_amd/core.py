@@ -221,6 +221,23 @@ class Batch:
     def set_integration(self, precision=L.MSA_FP64):
         check(lib().msa_batch_set_integration(self.h, precision))
 
+    # ------------------------------------------------------------ direct solver
+    def factorize(self):
+        """Band Cholesky of every sample's assembled FP64 K (SkylineMatrix2D.Factorize)."""
+        check(lib().msa_batch_factorize(self.h))
+
+    def direct_solve(self):
+        check(lib().msa_batch_direct_solve(self.h))
+
+    def set_solver(self, solver):
+        check(lib().msa_batch_set_solver(self.h, solver))
+
+    def direct_info(self, cls: int):
+        info = np.zeros(4, np.int64)
+        check(lib().msa_batch_direct_info(self.h, cls, ptr(info, C.c_longlong)))
+        return dict(n=int(info[0]), bandwidth=int(info[1]), width=int(info[2]),
+                    bytes_per_sample=int(info[3]))
+
     # ------------------------------------------------------------ dynamics
     def set_dynamics(self, rho: float, rayleigh_alpha: float = 0.0, rayleigh_beta: float = 0.0,
                      newmark_alpha: float = 0.25, newmark_delta: float = 0.5, dt: float = 1e-4):

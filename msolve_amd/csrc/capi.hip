@@ -39,6 +39,10 @@ struct ClassHost {
     long long hist_cap = 0;  // probe history capacity (doubles)
     double kc_nu = -2.0;     // Poisson ratio of the element cache (-2: not built)
     bool kc_mass = false;    // cache holds the consistent mass too
+    // direct solver: band order (host) and storage sizes
+    std::vector<int> bperm, binv;
+    long long bw = -1;       // half bandwidth, -1: order not built
+    size_t band_cap = 0;     // doubles allocated for the factors
     int nsamp = 0, ntiles = 0, nslices = 0, nchunks = 0, chunk = 0;
     std::vector<int> slice_ptr, cidx;
     long long tile_vals = 0, tile_vec = 0;
@@ -70,6 +74,8 @@ struct msa_batch {
     std::vector<void *> allocs;
     long long bytes = 0;
     bool assembled = false;
+    bool factored = false;    // band Cholesky factors match the assembled values
+    int solver = MSA_SOLVER_PCG;
     double nu = 0.3;
     // mixed-precision refinement: restarted inner solves or periodic
     // residual replacement
@@ -331,6 +337,84 @@ double *g_drop_buf = nullptr;  // device scratch of the Fortran drop-ins
 }  // namespace
 
 // ================================================================ C ABI
+
+// Direct solver helpers (band Cholesky).
+namespace {
+
+// Band order and device arrays of every class (built once per batch).
+int prepare_band(msa_batch *b)
+{
+    hipStream_t s = b->ctx->stream;
+    bool upload = false;
+    for (auto &C : b->cls) {
+        if (!C.dev.bperm) {
+            if (C.bw < 0) MSA_TRY(band_order(C.mesh->h, C.bperm, C.binv, C.bw));
+            void *p = nullptr;
+            MSA_TRY(dalloc(b, &p, C.bperm.size() * sizeof(int)));
+            MSA_HIP(hipMemcpyAsync(p, C.bperm.data(), C.bperm.size() * sizeof(int), hipMemcpyHostToDevice, s));
+            C.dev.bperm = (const int *)p;
+            MSA_TRY(dalloc(b, &p, C.binv.size() * sizeof(int)));
+            MSA_HIP(hipMemcpyAsync(p, C.binv.data(), C.binv.size() * sizeof(int), hipMemcpyHostToDevice, s));
+            C.dev.binv = (const int *)p;
+            C.dev.bn = (int)C.bperm.size();
+            // storage width: bandwidth + one block, rounded to 64
+            C.dev.bW = (int)(((C.bw + 1 + kBandNB + 63) / 64) * 64);
+            upload = true;
+        }
+        const size_t need = (size_t)C.nsamp * C.dev.bn * C.dev.bW;
+        if (C.band_cap < need) {
+            void *p = nullptr;
+            MSA_TRY(dalloc(b, &p, need * sizeof(double)));
+            C.dev.band = (double *)p;
+            C.band_cap = need;
+            MSA_TRY(dalloc(b, &p, (size_t)C.nsamp * C.dev.bn * sizeof(double)));
+            C.dev.by = (double *)p;
+            const int nchunk = (C.dev.bW + kBandChunkRows - 1) / kBandChunkRows;
+            MSA_TRY(dalloc(b, &p, (size_t)C.nsamp * nchunk * kBandNB * sizeof(double)));
+            C.dev.bpart = (double *)p;
+            MSA_TRY(dalloc(b, &p, (size_t)C.nsamp * sizeof(int)));
+            C.dev.bfail = (int *)p;
+            upload = true;
+        }
+    }
+    if (upload) MSA_TRY(upload_class_table(b));
+    return MSA_OK;
+}
+
+// Forward and backward band sweeps of class c for every sample, then the
+// solution into the PCG solution vector x.
+int band_solve_class(msa_batch *b, size_t c, bool from_fs)
+{
+    hipStream_t s = b->ctx->stream;
+    ClassHost &C = b->cls[c];
+    const ClassDev *cp = b->d_cls + c;
+    const int n = C.dev.bn, W = C.dev.bW, ns = C.nsamp;
+    MSA_TRY(timed(b, KK_OTHER, [&] { return launch_band_rhs(s, b->T, cp, ns, n, from_fs ? 1 : 0); }));
+    for (int k0 = 0; k0 < n; k0 += kBandNB) {
+        const int nk = std::min(kBandNB, n - k0), r0 = k0 + nk, m = std::min(n - r0, W - kBandNB);
+        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_band_fwd(s, cp, ns, k0, nk, r0, m); }));
+    }
+    const int last = ((n - 1) / kBandNB) * kBandNB;
+    for (int k0 = last; k0 >= 0; k0 -= kBandNB) {
+        const int nk = std::min(kBandNB, n - k0), r0 = k0 + nk, m = std::min(n - r0, W - kBandNB);
+        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_band_bwd(s, cp, ns, k0, nk, r0, m); }));
+    }
+    MSA_TRY(timed(b, KK_OTHER, [&] { return launch_band_out(s, b->T, cp, ns, 3 * C.dev.nb); }));
+    return MSA_OK;
+}
+
+// Residual |f - K x|/|f| and status of a direct solve (OP_DIRECT).
+int direct_stats(msa_batch *b)
+{
+    hipStream_t s = b->ctx->stream;
+    MSA_HIP(hipMemsetAsync(b->d_counts, 0, sizeof(int), s));
+    MSA_TRY(timed(b, KK_OTHER, [&] {
+        return launch_pcg_init_warm(s, b->T, b->d_cls, b->d_items, (int)b->items.size());
+    }));
+    return reduce(b, OP_DIRECT);
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -1020,6 +1104,7 @@ int msa_batch_assemble(msa_batch *b, double nu)
     }));
     b->nu = nu;
     b->assembled = true;
+    b->factored = false;
     return MSA_OK;
 }
 
@@ -1027,6 +1112,7 @@ int msa_batch_solve(msa_batch *b, double tol, int max_it, double inner_tol, int 
                     int graph_iters)
 {
     MSA_CHECK(b, MSA_ERR_ARG, "null batch");
+    if (b->solver == MSA_SOLVER_DIRECT) return msa_batch_direct_solve(b);
     MSA_CHECK(b->assembled, MSA_ERR_STATE, "batch not assembled");
     for (size_t c = 0; c < b->cls.size(); c++)
         MSA_CHECK(b->cls[c].has_rhs, MSA_ERR_STATE, "class %zu: no load vector", c);
@@ -1222,6 +1308,7 @@ int msa_batch_newmark(msa_batch *b, int nsteps, const double *acc, int acc_dir, 
     const bool mx = is_mixed(b);
     MSA_CHECK(!mx || b->refine == MSA_REFINE_RELIABLE, MSA_ERR_STATE,
               "dynamics of a MIXED batch uses MSA_REFINE_RELIABLE");
+    if (b->solver == MSA_SOLVER_DIRECT && !b->factored) MSA_TRY(msa_batch_factorize(b));
     MSA_HIP(hipSetDevice(b->ctx->device));
     hipStream_t s = b->ctx->stream;
     const int T = b->T, ni = (int)b->items.size();
@@ -1278,7 +1365,10 @@ int msa_batch_newmark(msa_batch *b, int nsteps, const double *acc, int acc_dir, 
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_fs_norm(s, T, b->d_cls, b->d_items, ni); }));
         MSA_TRY(reduce(b, OP_FNORM));
         // solve from the previous displacement
-        if (!mx && warm_start) {
+        if (b->solver == MSA_SOLVER_DIRECT) {
+            for (size_t c = 0; c < b->cls.size(); c++) MSA_TRY(band_solve_class(b, c, true));
+            MSA_TRY(direct_stats(b));
+        } else if (!mx && warm_start) {
             MSA_HIP(hipMemsetAsync(b->d_counts, 0, sizeof(int), s));
             MSA_TRY(timed(b, KK_OTHER, [&] { return launch_pcg_init_warm(s, T, b->d_cls, b->d_items, ni); }));
             MSA_TRY(reduce(b, OP_INIT2, tol));
@@ -1309,6 +1399,85 @@ int msa_batch_newmark(msa_batch *b, int nsteps, const double *acc, int acc_dir, 
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_step_advance(s, b->d_step); }));
     }
     MSA_HIP(hipStreamSynchronize(s));
+    return MSA_OK;
+}
+
+// ---------------------------------------------------------------- direct solver
+
+int msa_batch_factorize(msa_batch *b)
+{
+    MSA_CHECK(b, MSA_ERR_ARG, "null batch");
+    MSA_CHECK(!is_mixed(b), MSA_ERR_STATE, "the direct solver needs an FP64 batch");
+    MSA_CHECK(b->assembled, MSA_ERR_STATE, "batch not assembled");
+    MSA_HIP(hipSetDevice(b->ctx->device));
+    hipStream_t s = b->ctx->stream;
+    MSA_TRY(prepare_band(b));
+    for (auto &C : b->cls) {
+        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_fill_f64(s, C.dev.band, C.band_cap, 0.0); }));
+        MSA_HIP(hipMemsetAsync(C.dev.bfail, 0, (size_t)C.nsamp * sizeof(int), s));
+    }
+    MSA_TRY(timed(b, KK_OTHER, [&] {
+        return launch_band_scatter(s, b->T, b->d_cls, b->d_items, (int)b->items.size());
+    }));
+    for (size_t c = 0; c < b->cls.size(); c++) {
+        ClassHost &C = b->cls[c];
+        const ClassDev *cp = b->d_cls + c;
+        const int n = C.dev.bn, W = C.dev.bW, ns = C.nsamp;
+        for (int k0 = 0; k0 < n; k0 += kBandNB) {
+            const int nk = std::min(kBandNB, n - k0), r0 = k0 + nk,
+                      m = std::min(n - r0, W - kBandNB);
+            MSA_TRY(timed(b, KK_ASSEMBLE, [&] { return launch_chol_diag(s, cp, ns, k0, nk); }));
+            MSA_TRY(timed(b, KK_ASSEMBLE, [&] { return launch_chol_panel(s, cp, ns, k0, nk, r0, m); }));
+            MSA_TRY(timed(b, KK_ASSEMBLE, [&] { return launch_chol_update(s, cp, ns, k0, nk, r0, m); }));
+        }
+    }
+    MSA_HIP(hipStreamSynchronize(s));
+    b->factored = true;
+    return MSA_OK;
+}
+
+int msa_batch_direct_solve(msa_batch *b)
+{
+    MSA_CHECK(b, MSA_ERR_ARG, "null batch");
+    for (size_t c = 0; c < b->cls.size(); c++)
+        MSA_CHECK(b->cls[c].has_rhs || b->dyn, MSA_ERR_STATE, "class %zu: no load vector", c);
+    if (!b->factored) MSA_TRY(msa_batch_factorize(b));
+    MSA_HIP(hipSetDevice(b->ctx->device));
+    hipStream_t s = b->ctx->stream;
+    const int T = b->T, ni = (int)b->items.size();
+    if (b->dyn) {
+        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_fs_static(s, T, b->d_cls, b->d_items, ni); }));
+        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_fs_norm(s, T, b->d_cls, b->d_items, ni); }));
+        MSA_TRY(reduce(b, OP_FNORM));
+    }
+    for (size_t c = 0; c < b->cls.size(); c++) MSA_TRY(band_solve_class(b, c, b->dyn));
+    MSA_TRY(direct_stats(b));
+    MSA_HIP(hipStreamSynchronize(s));
+    return MSA_OK;
+}
+
+int msa_batch_set_solver(msa_batch *b, int solver)
+{
+    MSA_CHECK(b, MSA_ERR_ARG, "null batch");
+    MSA_CHECK(solver == MSA_SOLVER_PCG || solver == MSA_SOLVER_DIRECT, MSA_ERR_ARG,
+              "solver=%d", solver);
+    MSA_CHECK(solver == MSA_SOLVER_PCG || !is_mixed(b), MSA_ERR_STATE,
+              "the direct solver needs an FP64 batch");
+    b->solver = solver;
+    return MSA_OK;
+}
+
+int msa_batch_direct_info(msa_batch *b, int cls, long long *info)
+{
+    MSA_CHECK(b && info && cls >= 0 && cls < (int)b->cls.size(), MSA_ERR_ARG, "bad arguments");
+    ClassHost &C = b->cls[cls];
+    if (C.bw < 0) MSA_TRY(band_order(C.mesh->h, C.bperm, C.binv, C.bw));
+    const long long n = (long long)C.bperm.size();
+    const long long W = ((C.bw + 1 + kBandNB + 63) / 64) * 64;
+    info[0] = n;
+    info[1] = C.bw;
+    info[2] = W;
+    info[3] = n * W * (long long)sizeof(double);
     return MSA_OK;
 }
 

@@ -1629,6 +1629,289 @@ __global__ __launch_bounds__(256) void k_dyn_update(const ClassDev *__restrict__
     }
 }
 
+// ------------------------------------------------------------ direct solver
+// Banded Cholesky K = L L^T of each sample's FP64 stiffness (the GPU
+// counterpart of SkylineMatrix2D.Factorize / Solve, SkylineMatrix2D.cs:
+// 63-212, used by SolverSkyline.cs:90-205), blocked by kBandNB = 64 columns:
+// per block the diagonal factor (k_chol_diag), the panel solve below it
+// within the band (k_chol_panel) and the trailing update of the band window
+// (k_chol_update).  Lower band storage band[j*bW + (i - j)], bW >= bandwidth
+// + 64, so every panel is a full rectangle (entries outside the true band
+// stay zero).  grid.y = sample of the class.
+
+// Lower band entries of every sample's assembled K (FP64 batch values).
+template <int T>
+__global__ __launch_bounds__(256) void k_band_scatter(const ClassDev *__restrict__ cls,
+                                                      const int4 *__restrict__ items, int nitems)
+{
+    constexpr int R = 64 / T;
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    if (!C.band) return;
+    const int tile = it.y, chunk = it.z;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rr = lane / T, s = lane % T;
+    const int sample = tile * T + s;
+    if (sample >= C.nsamp) return;
+    const double *val = (const double *)C.val + (size_t)tile * C.tile_vals;
+    double *AB = C.band + (size_t)sample * C.bn * C.bW;
+    const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+    for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+        const int br = slice * R + rr;
+        if (br >= C.nb) continue;
+        const int base = C.slice_ptr[slice], W = C.slice_ptr[slice + 1] - base;
+        for (int j = 0; j < W; j++) {
+            const int slot = base + j;
+            const int col = C.cidx[(size_t)slot * R + rr];
+            if (col < 0) continue;
+            for (int p = 0; p < 3; p++) {
+                const int i = C.binv[3 * br + p];
+                if (i < 0) continue;
+                for (int q = 0; q < 3; q++) {
+                    const int jj = C.binv[3 * col + q];
+                    if (jj < 0 || jj > i) continue;
+                    AB[(size_t)jj * C.bW + (i - jj)] =
+                        val[(((size_t)slot * 9 + 3 * p + q) * R + rr) * T + s];
+                }
+            }
+        }
+    }
+}
+
+// In-place Cholesky of the nk x nk diagonal block at column k0; one
+// workgroup of 64 threads (thread = block row) per sample.
+__global__ __launch_bounds__(64) void k_chol_diag(const ClassDev *__restrict__ cp, int k0, int nk)
+{
+    __shared__ double A[kBandNB][kBandNB + 1];
+    __shared__ int fail;
+    const ClassDev &C = *cp;
+    const int smp = blockIdx.y;
+    const int t = threadIdx.x;
+    double *AB = C.band + (size_t)smp * C.bn * C.bW;
+    if (t == 0) fail = 0;
+    if (t < nk)
+        for (int c = 0; c <= t; c++) A[t][c] = AB[(size_t)(k0 + c) * C.bW + (t - c)];
+    __syncthreads();
+    for (int c = 0; c < nk; c++) {
+        if (t == c) {
+            double d = A[c][c];
+            if (!(d > 0.0) || !isfinite(d)) {
+                fail = 1;
+                d = 1.0;
+            }
+            A[c][c] = sqrt(d);
+        }
+        __syncthreads();
+        if (t > c && t < nk) A[t][c] /= A[c][c];
+        __syncthreads();
+        if (t > c && t < nk)
+            for (int c2 = c + 1; c2 <= t; c2++) A[t][c2] -= A[t][c] * A[c2][c];
+        __syncthreads();
+    }
+    if (t < nk)
+        for (int c = 0; c <= t; c++) AB[(size_t)(k0 + c) * C.bW + (t - c)] = A[t][c];
+    if (t == 0 && fail) C.bfail[smp] = 1;
+}
+
+// Panel rows i in [r0, r0 + m): L21 = A21 L11^-T, 64 rows per workgroup of
+// 256 threads: the 64 x 64 panel tile and the packed L11 in LDS, then a
+// column sweep (scale column c, update the columns right of it).  Columns
+// c >= nk of a last, partial block are left zero.
+__global__ __launch_bounds__(256) void k_chol_panel(const ClassDev *__restrict__ cp, int k0, int nk,
+                                                    int r0, int m)
+{
+    __shared__ double P[kBandNB][kBandNB + 1];
+    __shared__ double Lp[kBandNB * (kBandNB + 1) / 2];  // row c, column c2 <= c at c(c+1)/2 + c2
+    const ClassDev &C = *cp;
+    const int smp = blockIdx.y;
+    const int tid = threadIdx.x;
+    double *AB = C.band + (size_t)smp * C.bn * C.bW;
+    for (int idx = tid; idx < kBandNB * kBandNB; idx += 256) {
+        const int c = idx >> 6, c2 = idx & 63;
+        if (c2 <= c)
+            Lp[c * (c + 1) / 2 + c2] = c < nk ? AB[(size_t)(k0 + c2) * C.bW + (c - c2)]
+                                              : (c == c2 ? 1.0 : 0.0);
+    }
+    const int row0 = r0 + blockIdx.x * kBandNB;
+    for (int idx = tid; idx < kBandNB * kBandNB; idx += 256) {
+        const int c = idx >> 6, r = idx & 63;
+        const int i = row0 + r;
+        P[r][c] = (c < nk && i < r0 + m) ? AB[(size_t)(k0 + c) * C.bW + (i - k0 - c)] : 0.0;
+    }
+    __syncthreads();
+    for (int c = 0; c < nk; c++) {
+        if (tid < kBandNB) P[tid][c] /= Lp[c * (c + 1) / 2 + c];
+        __syncthreads();
+        const int rest = kBandNB * (nk - c - 1);
+        for (int idx = tid; idx < rest; idx += 256) {
+            const int r = idx & 63, c2 = c + 1 + (idx >> 6);
+            P[r][c2] -= P[r][c] * Lp[c2 * (c2 + 1) / 2 + c];
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < kBandNB * kBandNB; idx += 256) {
+        const int c = idx >> 6, r = idx & 63;
+        const int i = row0 + r;
+        if (c < nk && i < r0 + m) AB[(size_t)(k0 + c) * C.bW + (i - k0 - c)] = P[r][c];
+    }
+}
+
+// Trailing update of the band window: A[i][j] -= sum_c L[i][k0+c] L[j][k0+c]
+// for r0 <= j <= i < r0 + m; one 256-thread workgroup per 64 x 64 tile (I, J),
+// J <= I, each thread a 4 x 4 sub-block; the panel columns are staged in LDS
+// in two halves of 32.
+__global__ __launch_bounds__(256) void k_chol_update(const ClassDev *__restrict__ cp, int k0, int nk,
+                                                     int r0, int m)
+{
+    __shared__ double Li[kBandNB][33], Lj[kBandNB][33];
+    const ClassDev &C = *cp;
+    const int smp = blockIdx.y;
+    double *AB = C.band + (size_t)smp * C.bn * C.bW;
+    // tile index -> (I, J) with J <= I
+    const int tix = blockIdx.x;
+    int I = (int)((sqrt(8.0 * tix + 1.0) - 1.0) * 0.5);
+    while ((I + 1) * (I + 2) / 2 <= tix) I++;
+    while (I * (I + 1) / 2 > tix) I--;
+    const int J = tix - I * (I + 1) / 2;
+    const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
+    double acc[4][4] = {};
+    for (int h = 0; h < 2; h++) {
+        for (int e = threadIdx.x; e < kBandNB * 32; e += 256) {
+            const int r = e >> 5, c = (e & 31) + 32 * h;
+            const int ir = r0 + I * kBandNB + r, jr = r0 + J * kBandNB + r;
+            Li[r][c & 31] = (c < nk && ir < r0 + m) ? AB[(size_t)(k0 + c) * C.bW + (ir - k0 - c)] : 0.0;
+            Lj[r][c & 31] = (c < nk && jr < r0 + m) ? AB[(size_t)(k0 + c) * C.bW + (jr - k0 - c)] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int c = 0; c < 32; c++) {
+            double a[4], b[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                a[u] = Li[ti * 4 + u][c];
+                b[u] = Lj[tj * 4 + u][c];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) acc[u][v] += a[u] * b[v];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int i = r0 + I * kBandNB + ti * 4 + u, j = r0 + J * kBandNB + tj * 4 + v;
+            if (i < r0 + m && j < r0 + m && i >= j) AB[(size_t)j * C.bW + (i - j)] -= acc[u][v];
+        }
+}
+
+// by[s][i] = right-hand side of band row i (class load vector, or the
+// per-sample one of the dynamics mode).
+template <int T>
+__global__ void k_band_rhs(const ClassDev *__restrict__ cp, int from_fs)
+{
+    const ClassDev &C = *cp;
+    const int smp = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= C.bn) return;
+    const int row = C.bperm[i];
+    const int tile = smp / T, s = smp % T;
+    C.by[(size_t)smp * C.bn + i] =
+        from_fs ? C.fs[(size_t)tile * C.tile_vec + (size_t)row * T + s] : C.f[row];
+}
+
+// Forward step of L y = b for block k0: y_blk = L11^-1 y_blk (workgroup 0
+// of the x grid, 64 threads), then rows [r0, r0 + m) minus L21 y_blk.  Two
+// launches per block: `phase` 0 is the diagonal solve, 1 the update.
+__global__ __launch_bounds__(64) void k_band_fwd(const ClassDev *__restrict__ cp, int k0, int nk,
+                                                 int r0, int m, int phase)
+{
+    __shared__ double yb[kBandNB];
+    const ClassDev &C = *cp;
+    const int smp = blockIdx.y;
+    const int t = threadIdx.x;
+    const double *AB = C.band + (size_t)smp * C.bn * C.bW;
+    double *y = C.by + (size_t)smp * C.bn;
+    if (phase == 0) {
+        if (t < nk) yb[t] = y[k0 + t];
+        __syncthreads();
+        for (int c = 0; c < nk; c++) {
+            if (t == c) yb[c] /= AB[(size_t)(k0 + c) * C.bW];
+            __syncthreads();
+            if (t > c && t < nk) yb[t] -= AB[(size_t)(k0 + c) * C.bW + (t - c)] * yb[c];
+            __syncthreads();
+        }
+        if (t < nk) y[k0 + t] = yb[t];
+        return;
+    }
+    if (t < nk) yb[t] = y[k0 + t];
+    __syncthreads();
+    const int i = r0 + blockIdx.x * 64 + t;
+    if (i >= r0 + m) return;
+    double acc = 0.0;
+    for (int c = 0; c < nk; c++) acc += AB[(size_t)(k0 + c) * C.bW + (i - k0 - c)] * yb[c];
+    y[i] -= acc;
+}
+
+// Backward step of L^T x = y for block k0: phase 0 writes partial sums
+// bpart[g][c] = sum over rows i of chunk g of L[i][k0+c] x[i]; phase 1
+// (one workgroup) sums them in chunk order and back-substitutes L11^T.
+__global__ __launch_bounds__(64) void k_band_bwd(const ClassDev *__restrict__ cp, int k0, int nk,
+                                                 int r0, int m, int phase)
+{
+    __shared__ double xb[kBandNB];
+    const ClassDev &C = *cp;
+    const int smp = blockIdx.y;
+    const int t = threadIdx.x;
+    const double *AB = C.band + (size_t)smp * C.bn * C.bW;
+    double *x = C.by + (size_t)smp * C.bn;
+    const int nchunk = (C.bW + kBandChunkRows - 1) / kBandChunkRows;
+    double *part = C.bpart + (size_t)smp * nchunk * kBandNB;
+    if (phase == 0) {
+        const int g = blockIdx.x;
+        const int i0 = r0 + g * kBandChunkRows, i1 = min(r0 + m, i0 + kBandChunkRows);
+        if (t < nk) {
+            double acc = 0.0;
+            for (int i = i0; i < i1; i++) acc += AB[(size_t)(k0 + t) * C.bW + (i - k0 - t)] * x[i];
+            part[g * kBandNB + t] = acc;
+        }
+        return;
+    }
+    const int ng = m > 0 ? (m + kBandChunkRows - 1) / kBandChunkRows : 0;
+    if (t < nk) {
+        double v = x[k0 + t];
+        for (int g = 0; g < ng; g++) v -= part[g * kBandNB + t];
+        xb[t] = v;
+    }
+    __syncthreads();
+    for (int c = nk - 1; c >= 0; c--) {
+        if (t == c) xb[c] /= AB[(size_t)(k0 + c) * C.bW];
+        __syncthreads();
+        if (t < c) xb[t] -= AB[(size_t)(k0 + t) * C.bW + (c - t)] * xb[c];
+        __syncthreads();
+    }
+    if (t < nk) x[k0 + t] = xb[t];
+}
+
+// Solution back to the internal rows: x[row] = by[band(row)], constrained
+// components 0 (FP64 batches: the PCG solution vector).
+template <int T>
+__global__ void k_band_out(const ClassDev *__restrict__ cp)
+{
+    const ClassDev &C = *cp;
+    const int smp = blockIdx.y;
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= 3 * C.nb) return;
+    const int tile = smp / T, s = smp % T;
+    const int i = C.binv[row];
+    ((double *)C.x)[(size_t)tile * C.tile_vec + (size_t)row * T + s] =
+        i >= 0 ? C.by[(size_t)smp * C.bn + i] : 0.0;
+}
+
 // ------------------------------------------------------------ KL field
 
 // phi[e][i] = sigma sqrt(eigs_i) prod_d a * (index even ? sin : cos)
@@ -1773,6 +2056,16 @@ __device__ void apply_op(const ClassDev &C, int smp, const double *v, const Redu
     case OP_FNORM:
         S.fnorm = sqrt(v[0]);
         break;
+    case OP_DIRECT: {  // partials of k_pcg_init_warm: v[1] = |f - K x|^2
+        const double fn = fnorm_of(C, S);
+        S.iters = 0;
+        S.total_iters = 0;
+        S.outer = 0;
+        S.active = 0;
+        S.relres = fn > 1e-25 ? sqrt(v[1]) / fn : 0.0;
+        S.status = (C.bfail && C.bfail[smp]) || !isfinite(S.relres) ? MSA_BREAKDOWN : MSA_OK;
+        break;
+    }
     case OP_A:  // alpha = rz / p.q
         if (!S.active) break;
         S.pq = v[0];
@@ -1894,7 +2187,8 @@ __global__ __launch_bounds__(1024) void k_reduce(const ClassDev *__restrict__ cl
     const ClassDev &C = cls[ct.x];
     const int tile = ct.y;
     const int t = threadIdx.x % T, g = threadIdx.x / T;
-    const bool two = a.op == OP_INIT0 || a.op == OP_INIT1 || a.op == OP_INIT2 || a.op == OP_B;
+    const bool two = a.op == OP_INIT0 || a.op == OP_INIT1 || a.op == OP_INIT2 || a.op == OP_B ||
+                     a.op == OP_DIRECT;
     const double *base = C.partial + (size_t)tile * C.nchunks * 3 * T + t;
     double a0 = 0, a1 = 0, b0 = 0, b1 = 0;
     int c = g;
@@ -2185,6 +2479,77 @@ hipError_t launch_reduce(hipStream_t s, int T, const ClassDev *d_cls, const int2
     if (ntiles <= 0) return hipSuccess;
     MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_reduce<TT>), dim3(ntiles), dim3(1024), 0, s, d_cls,
                                          tiles, a));
+    return hipGetLastError();
+}
+
+hipError_t launch_band_scatter(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,
+                               int nitems)
+{
+    MSA_T_LAUNCH(k_band_scatter, d_cls, items, nitems);
+}
+
+hipError_t launch_chol_diag(hipStream_t s, const ClassDev *cp, int nsamp, int k0, int nk)
+{
+    hipLaunchKernelGGL(k_chol_diag, dim3(1, nsamp), dim3(64), 0, s, cp, k0, nk);
+    return hipGetLastError();
+}
+
+hipError_t launch_chol_panel(hipStream_t s, const ClassDev *cp, int nsamp, int k0, int nk, int r0,
+                             int m)
+{
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_chol_panel, dim3((m + kBandNB - 1) / kBandNB, nsamp), dim3(256), 0, s, cp,
+                       k0, nk, r0, m);
+    return hipGetLastError();
+}
+
+hipError_t launch_chol_update(hipStream_t s, const ClassDev *cp, int nsamp, int k0, int nk, int r0,
+                              int m)
+{
+    if (m <= 0) return hipSuccess;
+    const int T = (m + kBandNB - 1) / kBandNB;
+    hipLaunchKernelGGL(k_chol_update, dim3(T * (T + 1) / 2, nsamp), dim3(256), 0, s, cp, k0, nk,
+                       r0, m);
+    return hipGetLastError();
+}
+
+hipError_t launch_band_rhs(hipStream_t s, int T, const ClassDev *cp, int nsamp, int bn,
+                           int from_fs)
+{
+    if (bn <= 0) return hipSuccess;
+    MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_band_rhs<TT>), dim3((bn + 255) / 256, nsamp), dim3(256),
+                                         0, s, cp, from_fs));
+    return hipGetLastError();
+}
+
+hipError_t launch_band_fwd(hipStream_t s, const ClassDev *cp, int nsamp, int k0, int nk, int r0,
+                           int m)
+{
+    hipLaunchKernelGGL(k_band_fwd, dim3(1, nsamp), dim3(64), 0, s, cp, k0, nk, r0, m, 0);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || m <= 0) return e;
+    hipLaunchKernelGGL(k_band_fwd, dim3((m + 63) / 64, nsamp), dim3(64), 0, s, cp, k0, nk, r0, m, 1);
+    return hipGetLastError();
+}
+
+hipError_t launch_band_bwd(hipStream_t s, const ClassDev *cp, int nsamp, int k0, int nk, int r0,
+                           int m)
+{
+    if (m > 0) {
+        hipLaunchKernelGGL(k_band_bwd, dim3((m + kBandChunkRows - 1) / kBandChunkRows, nsamp),
+                           dim3(64), 0, s, cp, k0, nk, r0, m, 0);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_band_bwd, dim3(1, nsamp), dim3(64), 0, s, cp, k0, nk, r0, m, 1);
+    return hipGetLastError();
+}
+
+hipError_t launch_band_out(hipStream_t s, int T, const ClassDev *cp, int nsamp, int nrows)
+{
+    if (nrows <= 0) return hipSuccess;
+    MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_band_out<TT>), dim3((nrows + 255) / 256, nsamp),
+                                         dim3(256), 0, s, cp));
     return hipGetLastError();
 }
 

@@ -149,4 +149,131 @@ int build_mesh_host(MeshHost &m, int n_nodes, const double *xyz, int n_elem,
     return MSA_OK;
 }
 
+// Band ordering of the free DOFs for the direct solver.  Candidates: the
+// block order itself, a reverse Cuthill-McKee order of the block graph
+// (BFS from a pseudo-peripheral block, neighbours by ascending degree,
+// reversed), and the six lexicographic coordinate orders (exact planes of
+// structured meshes); the one with the smallest half bandwidth wins.  Within
+// a block its free components keep the order X, Y, Z.
+static long long dof_bandwidth(const MeshHost &m, const std::vector<int> &pos_block)
+{
+    // first DOF index of every block in the candidate order
+    std::vector<int> order(m.nb);
+    for (int b = 0; b < m.nb; b++) order[pos_block[b]] = b;
+    std::vector<long long> first(m.nb);
+    long long k = 0;
+    for (int q = 0; q < m.nb; q++) {
+        first[order[q]] = k;
+        k += __builtin_popcount(m.node_free[m.block_node[order[q]]]);
+    }
+    long long bw = 0;
+    for (int br = 0; br < m.nb; br++)
+        for (int j = m.brow_ptr[br]; j < m.brow_ptr[br + 1]; j++) {
+            const int bc = m.bcol[j];
+            if (pos_block[bc] > pos_block[br]) continue;
+            // rows of br minus columns of bc: last row of br - first column of bc
+            const long long d = first[br] + __builtin_popcount(m.node_free[m.block_node[br]]) - 1 -
+                                first[bc];
+            bw = std::max(bw, d);
+        }
+    return bw;
+}
+
+int band_order(const MeshHost &m, std::vector<int> &perm, std::vector<int> &inv, long long &bw)
+{
+    std::vector<std::vector<int>> cand;
+    {   // natural
+        std::vector<int> p(m.nb);
+        for (int b = 0; b < m.nb; b++) p[b] = b;
+        cand.push_back(p);
+    }
+    {   // reverse Cuthill-McKee
+        std::vector<int> deg(m.nb);
+        for (int b = 0; b < m.nb; b++) deg[b] = m.brow_ptr[b + 1] - m.brow_ptr[b] - 1;
+        std::vector<int> level(m.nb, -1), order;
+        order.reserve(m.nb);
+        auto bfs = [&](int start, std::vector<int> &out) {
+            std::vector<int> q{start};
+            level[start] = 0;
+            for (size_t h = 0; h < q.size(); h++) {
+                const int b = q[h];
+                std::vector<int> nb;
+                for (int j = m.brow_ptr[b]; j < m.brow_ptr[b + 1]; j++)
+                    if (level[m.bcol[j]] < 0) nb.push_back(m.bcol[j]);
+                std::sort(nb.begin(), nb.end(), [&](int x, int y) {
+                    return deg[x] != deg[y] ? deg[x] < deg[y] : x < y;
+                });
+                for (int c : nb) {
+                    level[c] = level[b] + 1;
+                    q.push_back(c);
+                }
+            }
+            out = q;
+        };
+        for (int s0 = 0; s0 < m.nb; s0++) {
+            if (level[s0] >= 0) continue;
+            // pseudo-peripheral start: minimum degree block of the component,
+            // then the last block of a BFS from it
+            std::vector<int> comp;
+            bfs(s0, comp);
+            int best = comp[0];
+            for (int b : comp)
+                if (deg[b] < deg[best]) best = b;
+            for (int b : comp) level[b] = -1;
+            std::vector<int> tmp;
+            bfs(best, tmp);
+            const int far = tmp.back();
+            for (int b : tmp) level[b] = -1;
+            bfs(far, tmp);
+            order.insert(order.end(), tmp.begin(), tmp.end());
+        }
+        std::reverse(order.begin(), order.end());
+        std::vector<int> p(m.nb);
+        for (int q = 0; q < m.nb; q++) p[order[q]] = q;
+        cand.push_back(p);
+    }
+    {   // lexicographic coordinate orders (slowest axis first)
+        const int axes[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+        for (auto &ax : axes) {
+            std::vector<int> order(m.nb);
+            for (int b = 0; b < m.nb; b++) order[b] = b;
+            std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+                const double *px = &m.xyz[3 * (size_t)m.block_node[x]];
+                const double *py = &m.xyz[3 * (size_t)m.block_node[y]];
+                for (int k = 0; k < 3; k++)
+                    if (px[ax[k]] != py[ax[k]]) return px[ax[k]] < py[ax[k]];
+                return x < y;
+            });
+            std::vector<int> p(m.nb);
+            for (int q = 0; q < m.nb; q++) p[order[q]] = q;
+            cand.push_back(p);
+        }
+    }
+    long long best_bw = -1;
+    size_t best = 0;
+    for (size_t c = 0; c < cand.size(); c++) {
+        const long long b = dof_bandwidth(m, cand[c]);
+        if (best_bw < 0 || b < best_bw) {
+            best_bw = b;
+            best = c;
+        }
+    }
+    const std::vector<int> &pos = cand[best];
+    std::vector<int> order(m.nb);
+    for (int b = 0; b < m.nb; b++) order[pos[b]] = b;
+    perm.clear();
+    inv.assign((size_t)m.nb * 3, -1);
+    for (int q = 0; q < m.nb; q++) {
+        const int b = order[q];
+        const uint8_t fr = m.node_free[m.block_node[b]];
+        for (int d = 0; d < 3; d++)
+            if ((fr >> d) & 1) {
+                inv[3 * (size_t)b + d] = (int)perm.size();
+                perm.push_back(3 * b + d);
+            }
+    }
+    bw = best_bw;
+    return MSA_OK;
+}
+
 }  // namespace msa

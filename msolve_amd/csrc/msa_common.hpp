@@ -82,6 +82,9 @@ struct MeshHost {
 
 int build_mesh_host(MeshHost &m, int n_nodes, const double *xyz, int n_elem,
                     const int *conn, const int *dof, int n_free);
+// Free DOFs in band order for the direct solver: perm[band] = internal row,
+// inv[internal row] = band index or -1; bw = half bandwidth.
+int band_order(const MeshHost &m, std::vector<int> &perm, std::vector<int> &inv, long long &bw);
 
 // Device view of one class of a batch (copied to device memory).
 struct ClassDev {
@@ -122,7 +125,19 @@ struct ClassDev {
     double *uold, *vel, *acc, *wm, *wk;  // [ntiles][tile_vec]
     double *bacc;         // [nb*3] mass-acceleration load of a unit base acceleration
     double *hist;         // [nsamp][hist_len] probe displacement per step
+    // direct solver: lower band of the Cholesky factor per sample,
+    // band[s][j*bW + (i - j)] for 0 <= i - j < bW, in band order
+    int bn, bW;           // band dimension (free DOFs), storage width
+    const int *bperm;     // [bn] band index -> internal row
+    const int *binv;      // [nb*3] internal row -> band index or -1
+    double *band;         // [nsamp][bn*bW]
+    double *by;           // [nsamp][bn] right-hand side / solution in band order
+    double *bpart;        // [nsamp][kBandChunks][64] partial sums of the backward solve
+    int *bfail;           // [nsamp] non-positive pivot met
 };
+
+constexpr int kBandNB = 64;        // block size of the band Cholesky
+constexpr int kBandChunkRows = 512;  // rows per partial of the backward solve
 
 // Newmark coefficients (NewmarkDynamicAnalyzer.cs:53-77) and the load history.
 struct DynCoef {
@@ -147,7 +162,8 @@ struct ColourLaunch {
 enum ReduceOp {
     OP_INIT0 = 0, OP_INIT1, OP_A, OP_B, OP_IRNORM, OP_RUNORM, OP_RURESTART,
     OP_FNORM,  // |f| of the per-sample right-hand side
-    OP_INIT2   // start of a warm-started FP64 solve (relres against |f|)
+    OP_INIT2,  // start of a warm-started FP64 solve (relres against |f|)
+    OP_DIRECT  // after a direct solve: |f - K x|/|f| and the factorization status
 };
 
 struct ReduceArgs {
@@ -213,6 +229,21 @@ hipError_t launch_spmv(hipStream_t s, bool mixed, int T, const ClassDev *d_cls, 
                        int nitems, const double *x, double *y);
 hipError_t launch_reduce(hipStream_t s, int T, const ClassDev *d_cls, const int2 *tiles,
                          int ntiles, const ReduceArgs &a);
+// direct solver (band Cholesky); cp = the class's device ClassDev
+hipError_t launch_band_scatter(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,
+                               int nitems);
+hipError_t launch_chol_diag(hipStream_t s, const ClassDev *cp, int nsamp, int k0, int nk);
+hipError_t launch_chol_panel(hipStream_t s, const ClassDev *cp, int nsamp, int k0, int nk, int r0,
+                             int m);
+hipError_t launch_chol_update(hipStream_t s, const ClassDev *cp, int nsamp, int k0, int nk, int r0,
+                              int m);
+hipError_t launch_band_rhs(hipStream_t s, int T, const ClassDev *cp, int nsamp, int bn,
+                           int from_fs);
+hipError_t launch_band_fwd(hipStream_t s, const ClassDev *cp, int nsamp, int k0, int nk, int r0,
+                           int m);
+hipError_t launch_band_bwd(hipStream_t s, const ClassDev *cp, int nsamp, int k0, int nk, int r0,
+                           int m);
+hipError_t launch_band_out(hipStream_t s, int T, const ClassDev *cp, int nsamp, int nrows);
 // dynamics
 hipError_t launch_set_int(hipStream_t s, int *p, int n, int v);
 hipError_t launch_dyn_bacc(hipStream_t s, const ClassDev *d_cls, const ColourLaunch &cl,
