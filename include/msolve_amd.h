@@ -36,6 +36,7 @@ enum {
 enum { MSA_FP64 = 0, MSA_MIXED = 1 };
 enum { MSA_REFINE_RESTART = 0, MSA_REFINE_RELIABLE = 1 };
 enum { MSA_SOLVER_PCG = 0, MSA_SOLVER_DIRECT = 1 };
+enum { MSA_PRECOND_JACOBI = 0, MSA_PRECOND_MULTIGRID = 1 };
 
 typedef struct msa_ctx msa_ctx;
 typedef struct msa_mesh msa_mesh;
@@ -242,7 +243,8 @@ int msa_batch_spmv(msa_batch *b, int cls, int sample, const double *x,
 
 /* Kernel timing with HIP events on the context stream (eager launches while
  * enabled).  kind: 0 SpMV+dot kernel, 1 update kernel, 2 assembly,
- * 3 FP64 residual, 4 other.  ms_total[5], launches[5]. */
+ * 3 FP64 residual, 4 other, 5 multigrid smoothing products.
+ * ms_total[6], launches[6]. */
 int msa_batch_set_kernel_timing(msa_batch *b, int enable);
 int msa_batch_get_kernel_timing(msa_batch *b, double *ms_total,
                                 long long *launches);
@@ -273,6 +275,42 @@ int msa_batch_set_solver(msa_batch *b, int solver);
 /* info[0] band dimension n (free DOFs), [1] half bandwidth, [2] storage
  * width W, [3] factor bytes per sample, for class cls. */
 int msa_batch_direct_info(msa_batch *b, int cls, long long *info);
+
+/* ----------------------------------------------------------- preconditioner
+ * Preconditioner of the PCG solves (msa_batch_solve, msa_batch_newmark).
+ * Replaces the choice made by SolverPCG's search-vector calculator
+ * (SolverPCG.cs:105-139 with SolverPCGSimpleSearchVectorCalculator.cs:18-41,
+ * Jacobi) and the first-sample LDL^T preconditioner of the Monte Carlo
+ * calculator (MonteCarloSolverPCGDirectMatrixCalculator.cs:38-80, applied in
+ * MCPCGDirect.cs:48-51): both reduce PCG iterations with a matrix reused
+ * across solves; here every sample gets its own multigrid hierarchy.
+ *
+ * MSA_PRECOND_JACOBI (default): z = D^-1 r.
+ * MSA_PRECOND_MULTIGRID: z = one geometric multigrid V-cycle on r, for every
+ * class whose mesh is a structured tensor-product grid of hexahedra with
+ * element counts that halve (each level keeps >= 2 elements per direction,
+ * at most max_levels coarse levels, <= 0: as many as possible) and whose
+ * coarsest level has at most 12000 free DOFs; other classes keep Jacobi.
+ * Coarse levels are rediscretized on the coarsened grid (nodes inherit the
+ * constraints, element modulus = mean of its 8 children) and assembled by
+ * the batch's kernels.  Per level: 2 Chebyshev-Jacobi steps on
+ * [lam/ratio, lam] (lam = Gershgorin bound of D^-1 K per sample; ratio 0
+ * selects 12), restriction P^T, coarse correction, prolongation P
+ * (trilinear), 2 Chebyshev-Jacobi steps; the coarsest level is solved by
+ * band Cholesky per sample.  The cycle is symmetric positive definite, so
+ * PCG keeps its convergence guarantee; the solution is the same, within the
+ * tolerance, as with Jacobi.  The hierarchy is built by the first call with
+ * MSA_PRECOND_MULTIGRID (later calls must pass the same max_levels) and
+ * re-assembled by every msa_batch_assemble. */
+int msa_batch_set_preconditioner(msa_batch *b, int kind, int max_levels, double ratio);
+/* info[0] coarse levels of class cls (0: Jacobi), [1] free DOFs of the
+ * coarsest level, [2] free DOFs summed over the coarse levels, [3] half
+ * bandwidth of the coarsest level. */
+int msa_batch_mg_info(msa_batch *b, int cls, long long *info);
+/* z = B r (one V-cycle) for one sample of class cls, r and z in the
+ * reference numbering.  Uses the batch's PCG work vectors: call it between
+ * solves. */
+int msa_batch_mg_apply(msa_batch *b, int cls, int sample, const double *r, double *z);
 
 /* ----------------------------------------------------------- dynamics
  * Linear implicit Newmark time integration of every sample of the batch

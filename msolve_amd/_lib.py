@@ -19,6 +19,7 @@ MSA_ERR_ARG, MSA_ERR_HIP, MSA_ERR_NO_DEVICE, MSA_ERR_OOM, MSA_ERR_STATE = -1, -2
 MSA_FP64, MSA_MIXED = 0, 1
 MSA_REFINE_RESTART, MSA_REFINE_RELIABLE = 0, 1
 MSA_SOLVER_PCG, MSA_SOLVER_DIRECT = 0, 1
+MSA_PRECOND_JACOBI, MSA_PRECOND_MULTIGRID = 0, 1
 STATUS_NAMES = {0: "OK", 1: "NOT_CONVERGED", 2: "BREAKDOWN", 3: "NEG_DETJ", 4: "NAN_INPUT"}
 
 # every symbol include/msolve_amd.h declares
@@ -37,7 +38,8 @@ EXPORTS = [
     "msa_batch_set_dynamics", "msa_batch_set_probe", "msa_batch_newmark",
     "msa_batch_get_history", "msa_batch_get_dynamic_state",
     "msa_batch_factorize", "msa_batch_direct_solve", "msa_batch_set_solver",
-    "msa_batch_direct_info",
+    "msa_batch_direct_info", "msa_batch_set_preconditioner", "msa_batch_mg_info",
+    "msa_batch_mg_apply",
 ]
 
 
@@ -124,6 +126,9 @@ def lib():
         "msa_batch_direct_solve": [_vp],
         "msa_batch_set_solver": [_vp, _i],
         "msa_batch_direct_info": [_vp, _i, _llp],
+        "msa_batch_set_preconditioner": [_vp, _i, _i, _d],
+        "msa_batch_mg_info": [_vp, _i, _llp],
+        "msa_batch_mg_apply": [_vp, _i, _i, _dp, _dp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -238,6 +238,25 @@ class Batch:
         return dict(n=int(info[0]), bandwidth=int(info[1]), width=int(info[2]),
                     bytes_per_sample=int(info[3]))
 
+    # ------------------------------------------------------------ preconditioner
+    def set_preconditioner(self, kind, max_levels: int = 0, ratio: float = 0.0):
+        """MSA_PRECOND_JACOBI or MSA_PRECOND_MULTIGRID (geometric V-cycle on
+        structured grids; classes whose mesh does not coarsen keep Jacobi)."""
+        check(lib().msa_batch_set_preconditioner(self.h, kind, max_levels, ratio))
+
+    def mg_info(self, cls: int):
+        info = np.zeros(4, np.int64)
+        check(lib().msa_batch_mg_info(self.h, cls, ptr(info, C.c_longlong)))
+        return dict(levels=int(info[0]), coarsest_dofs=int(info[1]), coarse_dofs=int(info[2]),
+                    coarsest_bandwidth=int(info[3]))
+
+    def mg_apply(self, cls: int, sample: int, r):
+        """One V-cycle z = B r for one sample (reference DOF numbering)."""
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.zeros_like(r)
+        check(lib().msa_batch_mg_apply(self.h, cls, sample, ptr(r, C.c_double), ptr(z, C.c_double)))
+        return z
+
     # ------------------------------------------------------------ dynamics
     def set_dynamics(self, rho: float, rayleigh_alpha: float = 0.0, rayleigh_beta: float = 0.0,
                      newmark_alpha: float = 0.25, newmark_delta: float = 0.5, dt: float = 1e-4):
@@ -278,11 +297,11 @@ class Batch:
         check(lib().msa_batch_set_kernel_timing(self.h, 1 if on else 0))
 
     def kernel_timing(self):
-        ms = np.zeros(5)
-        n = np.zeros(5, np.int64)
+        ms = np.zeros(6)
+        n = np.zeros(6, np.int64)
         check(lib().msa_batch_get_kernel_timing(self.h, ptr(ms, C.c_double),
                                                 ptr(n, C.c_longlong)))
-        names = ["spmv_dot", "update", "assemble", "residual64", "other"]
+        names = ["spmv_dot", "update", "assemble", "residual64", "other", "multigrid"]
         return {k: dict(ms=float(ms[i]), launches=int(n[i])) for i, k in enumerate(names)}
 
     def spmv_bytes(self):

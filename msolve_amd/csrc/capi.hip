@@ -54,8 +54,8 @@ struct ClassHost {
 
 struct Timing {
     bool on = false;
-    double ms[5] = {0, 0, 0, 0, 0};
-    long long n[5] = {0, 0, 0, 0, 0};
+    double ms[kKernelKinds] = {0, 0, 0, 0, 0, 0};
+    long long n[kKernelKinds] = {0, 0, 0, 0, 0, 0};
     hipEvent_t e0 = nullptr, e1 = nullptr;
 };
 
@@ -97,6 +97,25 @@ struct msa_batch {
     double graph_tol = -1;
     int graph_maxit = -1;
     Timing timing;
+    // geometric multigrid preconditioner (msa_batch_set_preconditioner)
+    int precond = MSA_PRECOND_JACOBI;
+    double mg_ratio = 12.0;             // smoothing interval [lam/ratio, lam]
+    std::vector<ClassHost> lv;          // coarse levels: class table entry cls.size() + i
+    std::vector<msa_mesh *> mg_meshes;  // their meshes (owned by the batch)
+    std::vector<int> mg_nlev;           // per solved class: coarse levels (0: Jacobi)
+    int mg_depth = 0;                   // deepest level of any hierarchy
+    std::vector<int4 *> mg_smooth;      // per depth: items of the classes that smooth there
+    std::vector<int> mg_nsmooth;
+    std::vector<int4 *> mg_restr;       // per depth >= 1: items of every level there
+    std::vector<int> mg_nrestr;
+    int4 *mg_lv_items = nullptr;        // every coarse level
+    int mg_nlv_items = 0;
+    int4 *mg_co_items = nullptr;        // the coarsest levels
+    int mg_nco_items = 0;
+    std::vector<int> mg_coarsest;       // class table indices of the coarsest levels
+    int mg_max_levels = 0;              // max_levels the hierarchy was built with
+    size_t d_cls_cap = 0;               // ClassDev entries allocated at d_cls
+    bool mg_ready = false;              // level matrices match the assembled classes
 };
 
 namespace {
@@ -126,6 +145,139 @@ int dalloc(msa_batch *b, void **p, size_t bytes)
         int rc_ = (x);                  \
         if (rc_ != MSA_OK) return rc_;  \
     } while (0)
+
+// Device arrays and block-ELL layout of one class (a solved class or a
+// multigrid level) for nsamp samples.
+int setup_class(msa_batch *b, ClassHost &C, msa_mesh *mesh, int nsamp)
+{
+    const int T = b->T, R = b->R;
+    const size_t rs = real_size(b);
+    int rc = MSA_OK;
+    {
+        const MeshHost &h = mesh->h;
+        C.mesh = mesh;
+        C.nsamp = nsamp;
+        C.ntiles = (C.nsamp + T - 1) / T;
+        C.nslices = (h.nb + R - 1) / R;
+        C.chunk = kWaves * std::max(1, 16 / R);
+        C.nchunks = (C.nslices + C.chunk - 1) / C.chunk;
+        C.slice_ptr.assign(C.nslices + 1, 0);
+        for (int sl = 0; sl < C.nslices; sl++) {
+            int wmax = 0;
+            for (int rr = 0; rr < R; rr++) {
+                int br = sl * R + rr;
+                if (br < h.nb) wmax = std::max(wmax, h.brow_ptr[br + 1] - h.brow_ptr[br]);
+            }
+            C.slice_ptr[sl + 1] = C.slice_ptr[sl] + wmax;
+        }
+        const long long nslots = C.slice_ptr[C.nslices];
+        C.cidx.assign((size_t)nslots * R, -1);
+        for (int br = 0; br < h.nb; br++) {
+            int sl = br / R, rr = br % R;
+            for (int j = 0; j < h.brow_ptr[br + 1] - h.brow_ptr[br]; j++)
+                C.cidx[(size_t)(C.slice_ptr[sl] + j) * R + rr] = h.bcol[h.brow_ptr[br] + j];
+        }
+        std::vector<uint8_t> bfree(h.nb);
+        for (int br = 0; br < h.nb; br++) bfree[br] = h.node_free[h.block_node[br]];
+        C.tile_vals = nslots * 9 * R * T;
+        C.tile_vec = (long long)h.nb * 3 * T;
+
+        ClassDev &D = C.dev;
+        memset(&D, 0, sizeof(D));
+        D.nb = h.nb;
+        D.nslices = C.nslices;
+        D.ntiles = C.ntiles;
+        D.nsamp = C.nsamp;
+        D.n_elem = h.n_elem;
+        D.nchunks = C.nchunks;
+        D.chunk = C.chunk;
+        D.tile_vals = C.tile_vals;
+        D.tile_vec = C.tile_vec;
+        D.xyz = mesh->d_xyz;
+        D.conn = mesh->d_conn;
+        D.node_block = mesh->d_node_block;
+        D.node_free = mesh->d_node_free;
+        D.emap = mesh->d_emap;
+        D.colour_elems = mesh->d_colour_elems;
+        MSA_CHECK(h.ncolours <= kMaxColours, MSA_ERR_ARG, "too many colours");
+        for (int k = 0; k <= h.ncolours; k++) D.colour_ptr[k] = h.colour_ptr[k];
+        auto A = [&](void **p, size_t bytes) {
+            if (rc == MSA_OK) rc = dalloc(b, p, bytes);
+        };
+        void *p = nullptr;
+        A(&p, C.slice_ptr.size() * sizeof(int));
+        D.slice_ptr = (const int *)p;
+        if (rc == MSA_OK) MSA_HIP(hipMemcpy(p, C.slice_ptr.data(), C.slice_ptr.size() * sizeof(int), hipMemcpyHostToDevice));
+        A(&p, C.cidx.size() * sizeof(int));
+        D.cidx = (const int *)p;
+        if (rc == MSA_OK) MSA_HIP(hipMemcpy(p, C.cidx.data(), C.cidx.size() * sizeof(int), hipMemcpyHostToDevice));
+        A(&p, h.diag_pos.size() * sizeof(int));
+        D.diag_pos = (const int *)p;
+        if (rc == MSA_OK) MSA_HIP(hipMemcpy(p, h.diag_pos.data(), h.diag_pos.size() * sizeof(int), hipMemcpyHostToDevice));
+        A(&p, bfree.size());
+        D.block_free = (const uint8_t *)p;
+        if (rc == MSA_OK) MSA_HIP(hipMemcpy(p, bfree.data(), bfree.size(), hipMemcpyHostToDevice));
+        {
+            // gather lists of the assembly: element contributions per stored
+            // block, in ascending element order
+            std::vector<int> gmap((size_t)nslots * 8 * R, -1);
+            std::vector<uint8_t> cnt((size_t)nslots * R, 0);
+            for (int e = 0; e < h.n_elem && rc == MSA_OK; e++)
+                for (int a = 0; a < 8; a++) {
+                    const int br = h.node_block[h.conn[8 * e + a]];
+                    if (br < 0) continue;
+                    for (int bb = 0; bb < 8; bb++) {
+                        const int j = h.emap[(size_t)e * 64 + a * 8 + bb];
+                        if (j == 255) continue;
+                        const size_t sr = (size_t)(C.slice_ptr[br / R] + j) * R + br % R;
+                        const int k = cnt[sr]++;
+                        if (k >= 8) {
+                            set_error("mesh: more than 8 elements share a node pair");
+                            rc = MSA_ERR_ARG;
+                            break;
+                        }
+                        gmap[((size_t)(C.slice_ptr[br / R] + j) * 8 + k) * R + br % R] =
+                            e * 64 + a * 8 + bb;
+                    }
+                }
+            A(&p, gmap.size() * sizeof(int));
+            D.gmap = (const int *)p;
+            if (rc == MSA_OK)
+                MSA_HIP(hipMemcpy(p, gmap.data(), gmap.size() * sizeof(int), hipMemcpyHostToDevice));
+            A((void **)&D.kc, (size_t)h.n_elem * 576 * sizeof(double));
+        }
+        const size_t nvec = (size_t)C.ntiles * C.tile_vec;
+        A(&D.val, (size_t)C.ntiles * C.tile_vals * rs);
+        A(&D.x, nvec * rs);
+        A(&D.r, nvec * rs);
+        A(&D.z, nvec * rs);
+        A(&D.p0, nvec * rs);
+        A(&D.p1, nvec * rs);
+        A(&D.q, nvec * rs);
+        A(&D.dinv, nvec * rs);
+        if (is_mixed(b)) {
+            A((void **)&D.u64, nvec * sizeof(double));
+            A((void **)&D.r64, nvec * sizeof(double));
+        }
+        A(&p, (size_t)h.nb * 3 * sizeof(double));
+        D.f = (const double *)p;
+        A((void **)&D.E, (size_t)C.ntiles * h.n_elem * T * sizeof(double));
+        A((void **)&D.st, (size_t)C.ntiles * T * sizeof(SampleState));
+        A((void **)&D.partial, (size_t)C.ntiles * C.nchunks * 3 * T * sizeof(double));
+        A((void **)&D.negdet, sizeof(int));
+        A((void **)&D.due, (1 + (size_t)C.ntiles * T) * sizeof(int));
+        if (rc != MSA_OK) return rc;
+        hipStream_t s = b->ctx->stream;
+        MSA_HIP(hipMemsetAsync(D.st, 0, (size_t)C.ntiles * T * sizeof(SampleState), s));
+        MSA_HIP(hipMemsetAsync(D.negdet, 0, sizeof(int), s));
+        MSA_HIP(hipMemsetAsync(D.due, 0, sizeof(int), s));
+        MSA_HIP(hipMemsetAsync((void *)D.f, 0, (size_t)h.nb * 3 * sizeof(double), s));
+        MSA_HIP(hipMemsetAsync(D.E, 0, (size_t)C.ntiles * h.n_elem * T * sizeof(double), s));
+        D.mg_root = D.mg_coarse = D.mg_fine = -1;
+    }
+    return rc;
+}
+
 
 // Timed launch helper: with timing on, bracket the launch with events on the
 // context stream and wait for it.
@@ -167,10 +319,24 @@ int reduce(msa_batch *b, int op, double tol = 0.0, int max_it = 0, int max_outer
     });
 }
 
+// Class table: the solved classes, then the multigrid levels.  Growing it
+// moves d_cls, which invalidates a captured graph.
 int upload_class_table(msa_batch *b)
 {
-    std::vector<ClassDev> h(b->cls.size());
+    const size_t n = b->cls.size() + b->lv.size();
+    std::vector<ClassDev> h(n);
     for (size_t c = 0; c < b->cls.size(); c++) h[c] = b->cls[c].dev;
+    for (size_t l = 0; l < b->lv.size(); l++) h[b->cls.size() + l] = b->lv[l].dev;
+    if (n > b->d_cls_cap) {
+        void *p = nullptr;
+        MSA_TRY(dalloc(b, &p, sizeof(ClassDev) * n));
+        b->d_cls = (ClassDev *)p;
+        b->d_cls_cap = n;
+        if (b->graph) {
+            (void)hipGraphExecDestroy(b->graph);
+            b->graph = nullptr;
+        }
+    }
     MSA_HIP(hipMemcpyAsync(b->d_cls, h.data(), sizeof(ClassDev) * h.size(),
                            hipMemcpyHostToDevice, b->ctx->stream));
     MSA_HIP(hipStreamSynchronize(b->ctx->stream));
@@ -186,6 +352,9 @@ int read_count(msa_batch *b, int which, int *out)
     return MSA_OK;
 }
 
+int mg_vcycle(msa_batch *b, int pbuf, int mode);
+int mg_values(msa_batch *b);
+
 // ru_tol > 0: reliable-update mode (inner test off, replacement triggers on).
 int launch_iteration_pair(msa_batch *b, int pbuf, double tol, int max_it, double ru_tol = 0.0)
 {
@@ -200,6 +369,7 @@ int launch_iteration_pair(msa_batch *b, int pbuf, double tol, int max_it, double
         return launch_pcg_b(s, mx, b->T, b->d_cls, b->d_items, (int)b->items.size(), pbuf, tol,
                             max_it, b->d_counts);
     }));
+    MSA_TRY(mg_vcycle(b, pbuf, 1));
     if (ru_tol > 0)
         MSA_TRY(reduce(b, OP_B, 0.0, max_it, 0, b->drop, ru_tol));
     else
@@ -236,6 +406,7 @@ int launch_replacement(msa_batch *b, double tol, int max_it)
     }));
     MSA_TRY(reduce(b, OP_RUNORM, tol, max_it));
     MSA_TRY(timed(b, KK_OTHER, [&] { return launch_ru_restart(s, T, b->d_cls, b->d_items, ni); }));
+    MSA_TRY(mg_vcycle(b, 1, 2));  // the restart keeps the search direction in p0
     MSA_TRY(reduce(b, OP_RURESTART));
     return MSA_OK;
 }
@@ -338,15 +509,47 @@ double *g_drop_buf = nullptr;  // device scratch of the Fortran drop-ins
 
 // ================================================================ C ABI
 
+namespace {
+
+// Device copies of a mesh's host analysis.
+int upload_mesh(msa_mesh *m)
+{
+    msa_ctx *ctx = m->ctx;
+    const MeshHost &h = m->h;
+    hipError_t e = hipSetDevice(ctx->device);
+    auto up = [&](void **dst, const void *src, size_t bytes) {
+        if (e != hipSuccess) return;
+        e = hipMalloc(dst, bytes ? bytes : 16);
+        if (e == hipSuccess && bytes)
+            e = hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+    };
+    up((void **)&m->d_xyz, h.xyz.data(), h.xyz.size() * sizeof(double));
+    up((void **)&m->d_conn, h.conn.data(), h.conn.size() * sizeof(int));
+    up((void **)&m->d_node_block, h.node_block.data(), h.node_block.size() * sizeof(int));
+    up((void **)&m->d_node_free, h.node_free.data(), h.node_free.size());
+    up((void **)&m->d_emap, h.emap.data(), h.emap.size());
+    up((void **)&m->d_colour_elems, h.colour_elems.data(), h.colour_elems.size() * sizeof(int));
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        set_error("mesh upload failed: %s", hipGetErrorString(e));
+        return MSA_ERR_HIP;
+    }
+    return MSA_OK;
+}
+
+}  // namespace
+
 // Direct solver helpers (band Cholesky).
 namespace {
 
 // Band order and device arrays of every class (built once per batch).
-int prepare_band(msa_batch *b)
+// Band order, factor storage and solve buffers of one class; *changed is
+// set when its ClassDev changed.
+int band_setup(msa_batch *b, ClassHost &C, bool *changed)
 {
     hipStream_t s = b->ctx->stream;
     bool upload = false;
-    for (auto &C : b->cls) {
+    {
         if (!C.dev.bperm) {
             if (C.bw < 0) MSA_TRY(band_order(C.mesh->h, C.bperm, C.binv, C.bw));
             void *p = nullptr;
@@ -377,6 +580,14 @@ int prepare_band(msa_batch *b)
             upload = true;
         }
     }
+    if (upload) *changed = true;
+    return MSA_OK;
+}
+
+int prepare_band(msa_batch *b)
+{
+    bool upload = false;
+    for (auto &C : b->cls) MSA_TRY(band_setup(b, C, &upload));
     if (upload) MSA_TRY(upload_class_table(b));
     return MSA_OK;
 }
@@ -412,6 +623,284 @@ int direct_stats(msa_batch *b)
         return launch_pcg_init_warm(s, b->T, b->d_cls, b->d_items, (int)b->items.size());
     }));
     return reduce(b, OP_DIRECT);
+}
+
+// ---------------------------------------------------------------- multigrid
+
+constexpr int kMgCoarsestMax = 12000;  // free DOFs of a coarsest level (band Cholesky per sample)
+
+ClassHost &tab(msa_batch *b, int i)
+{
+    return i < (int)b->cls.size() ? b->cls[i] : b->lv[i - b->cls.size()];
+}
+
+template <typename V>
+int upload_vec(msa_batch *b, const std::vector<V> &v, const V **out)
+{
+    void *p = nullptr;
+    MSA_TRY(dalloc(b, &p, std::max<size_t>(v.size(), 1) * sizeof(V)));
+    if (!v.empty()) MSA_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice));
+    *out = (const V *)p;
+    return MSA_OK;
+}
+
+int items_to_device(msa_batch *b, const std::vector<int4> &v, int4 **out, int *n)
+{
+    const int4 *p = nullptr;
+    MSA_TRY(upload_vec(b, v, &p));
+    *out = (int4 *)p;
+    *n = (int)v.size();
+    return MSA_OK;
+}
+
+void class_items(const ClassHost &C, int ci, std::vector<int4> &out)
+{
+    for (int t = 0; t < C.ntiles; t++)
+        for (int k = 0; k < C.nchunks; k++) out.push_back(make_int4(ci, t, k, 0));
+}
+
+// Coarse hierarchies of every class whose mesh is a structured grid that
+// halves at least once and whose coarsest level stays small; the other
+// classes keep the Jacobi preconditioner.
+int mg_build(msa_batch *b, int max_levels)
+{
+    const int ncls = (int)b->cls.size();
+    b->mg_nlev.assign(ncls, 0);
+    std::vector<std::vector<int>> depth_classes(1);  // per depth: table indices
+    for (int c = 0; c < ncls; c++) {
+        // host coarsening first: commit only a usable hierarchy
+        std::vector<MeshHost> meshes;
+        std::vector<MgTransfer> tr;
+        const MeshHost *f = &b->cls[c].mesh->h;
+        while (max_levels <= 0 || (int)meshes.size() < max_levels) {
+            MeshHost ch;
+            MgTransfer t;
+            if (mg_coarsen(*f, ch, t) != MSA_OK) break;
+            meshes.push_back(std::move(ch));
+            tr.push_back(std::move(t));
+            f = &meshes.back();
+        }
+        set_error("");
+        if (meshes.empty() || meshes.back().n_free > kMgCoarsestMax) continue;
+        b->mg_nlev[c] = (int)meshes.size();
+        depth_classes[0].push_back(c);
+        int parent = c;
+        for (size_t l = 0; l < meshes.size(); l++) {
+            msa_mesh *m = new msa_mesh();
+            m->ctx = b->ctx;
+            m->h = std::move(meshes[l]);
+            b->mg_meshes.push_back(m);
+            MSA_TRY(upload_mesh(m));
+            const int ci = (int)(b->cls.size() + b->lv.size());
+            b->lv.emplace_back();
+            ClassHost &L = b->lv.back();
+            MSA_TRY(setup_class(b, L, m, b->cls[c].nsamp));
+            L.has_E = true;
+            ClassDev &D = L.dev;
+            D.mg_root = c;
+            D.mg_fine = parent;
+            D.mg_depth = (int)l + 1;
+            const MgTransfer &t = tr[l];
+            MSA_TRY(upload_vec(b, t.child, &D.mg_child));
+            MSA_TRY(upload_vec(b, t.r_ptr, &D.mg_r_ptr));
+            MSA_TRY(upload_vec(b, t.r_col, &D.mg_r_col));
+            MSA_TRY(upload_vec(b, t.r_w, &D.mg_r_w));
+            ClassDev &P = tab(b, parent).dev;
+            P.mg_coarse = ci;
+            MSA_TRY(upload_vec(b, t.p_ptr, &P.mg_p_ptr));
+            MSA_TRY(upload_vec(b, t.p_col, &P.mg_p_col));
+            MSA_TRY(upload_vec(b, t.p_w, &P.mg_p_w));
+            void *p = nullptr;
+            MSA_TRY(dalloc(b, &p, (size_t)tab(b, parent).ntiles * b->T * sizeof(double)));
+            P.mg_lam = (double *)p;
+            if ((int)depth_classes.size() <= D.mg_depth) depth_classes.resize(D.mg_depth + 1);
+            depth_classes[D.mg_depth].push_back(ci);
+            if (l + 1 == meshes.size()) {
+                bool changed = false;
+                MSA_TRY(band_setup(b, b->lv.back(), &changed));
+                b->mg_coarsest.push_back(ci);
+            }
+            parent = ci;
+        }
+    }
+    b->mg_depth = (int)depth_classes.size() - 1;
+    b->mg_smooth.assign(b->mg_depth, nullptr);
+    b->mg_nsmooth.assign(b->mg_depth, 0);
+    b->mg_restr.assign(b->mg_depth + 1, nullptr);
+    b->mg_nrestr.assign(b->mg_depth + 1, 0);
+    std::vector<int4> all, co;
+    for (int d = 0; d <= b->mg_depth; d++) {
+        std::vector<int4> sm, re;
+        for (int ci : depth_classes[d]) {
+            ClassHost &C = tab(b, ci);
+            if (d > 0) {
+                class_items(C, ci, re);
+                class_items(C, ci, all);
+            }
+            if (C.dev.mg_coarse >= 0)
+                class_items(C, ci, sm);
+            else if (d > 0)
+                class_items(C, ci, co);
+        }
+        if (d < b->mg_depth) MSA_TRY(items_to_device(b, sm, &b->mg_smooth[d], &b->mg_nsmooth[d]));
+        if (d > 0) MSA_TRY(items_to_device(b, re, &b->mg_restr[d], &b->mg_nrestr[d]));
+    }
+    MSA_TRY(items_to_device(b, all, &b->mg_lv_items, &b->mg_nlv_items));
+    MSA_TRY(items_to_device(b, co, &b->mg_co_items, &b->mg_nco_items));
+    MSA_TRY(upload_class_table(b));
+    b->mg_ready = false;
+    return MSA_OK;
+}
+
+// Level matrices from the assembled classes: moduli averaged level by level,
+// the unit-modulus element cache, gather assembly, D^-1, Gershgorin bounds
+// and the band Cholesky factors of the coarsest levels.
+int mg_values(msa_batch *b)
+{
+    b->mg_ready = false;
+    if (b->precond != MSA_PRECOND_MULTIGRID || b->mg_depth == 0) {
+        b->mg_ready = true;
+        return MSA_OK;
+    }
+    hipStream_t s = b->ctx->stream;
+    const bool mx = is_mixed(b);
+    const int T = b->T, nc = (int)b->cls.size();
+    bool upload = false;
+    for (auto &L : b->lv) {
+        const ClassDev &R = b->cls[L.dev.mg_root].dev;
+        if (L.dev.dyn != R.dyn || L.dev.cK != R.cK || L.dev.cM != R.cM) {
+            L.dev.dyn = R.dyn;
+            L.dev.cK = R.cK;
+            L.dev.cM = R.cM;
+            L.dev.rho = R.rho;
+            upload = true;
+        }
+        if (L.dev.dyn && !L.dev.mc) {
+            void *p = nullptr;
+            MSA_TRY(dalloc(b, &p, (size_t)L.mesh->h.n_elem * 64 * sizeof(double)));
+            L.dev.mc = (double *)p;
+            upload = true;
+        }
+    }
+    if (upload) MSA_TRY(upload_class_table(b));
+    for (int d = 1; d <= b->mg_depth; d++)
+        for (size_t l = 0; l < b->lv.size(); l++) {
+            ClassHost &L = b->lv[l];
+            if (L.dev.mg_depth != d) continue;
+            MSA_TRY(timed(b, KK_OTHER, [&] {
+                return launch_mg_coarsen_E(s, T, b->d_cls + nc + l, b->d_cls, L.ntiles, L.dev.n_elem);
+            }));
+        }
+    for (size_t l = 0; l < b->lv.size(); l++) {
+        ClassHost &L = b->lv[l];
+        if (L.kc_nu == b->nu && L.kc_mass == (bool)L.dev.dyn) continue;
+        MSA_HIP(hipMemsetAsync(L.dev.negdet, 0, sizeof(int), s));
+        MSA_TRY(timed(b, KK_ASSEMBLE, [&] {
+            return launch_khat_cache(s, b->d_cls + nc + l, L.dev.n_elem, b->nu, L.dev.dyn);
+        }));
+        L.kc_nu = b->nu;
+        L.kc_mass = L.dev.dyn;
+    }
+    MSA_TRY(timed(b, KK_ASSEMBLE, [&] {
+        return launch_assemble_gather(s, mx, T, b->d_cls, b->mg_lv_items, b->mg_nlv_items);
+    }));
+    MSA_TRY(timed(b, KK_OTHER, [&] {
+        return launch_dinv(s, mx, T, b->d_cls, b->mg_lv_items, b->mg_nlv_items);
+    }));
+    for (int ci = 0; ci < nc + (int)b->lv.size(); ci++) {
+        ClassHost &C = tab(b, ci);
+        if (C.dev.mg_coarse < 0) continue;
+        MSA_TRY(timed(b, KK_OTHER, [&] {
+            return launch_fill_f64(s, C.dev.mg_lam, (size_t)C.ntiles * T, 0.0);
+        }));
+    }
+    for (int d = 0; d < b->mg_depth; d++)
+        MSA_TRY(timed(b, KK_OTHER, [&] {
+            return launch_mg_gersh(s, mx, T, b->d_cls, b->mg_smooth[d], b->mg_nsmooth[d]);
+        }));
+    for (int ci : b->mg_coarsest) {
+        ClassHost &C = tab(b, ci);
+        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_fill_f64(s, C.dev.band, C.band_cap, 0.0); }));
+        MSA_HIP(hipMemsetAsync(C.dev.bfail, 0, (size_t)C.nsamp * sizeof(int), s));
+    }
+    MSA_TRY(timed(b, KK_OTHER, [&] {
+        return launch_band_scatter_any(s, mx, T, b->d_cls, b->mg_co_items, b->mg_nco_items);
+    }));
+    for (int ci : b->mg_coarsest) {
+        ClassHost &C = tab(b, ci);
+        const ClassDev *cp = b->d_cls + ci;
+        const int n = C.dev.bn, W = C.dev.bW, ns = C.nsamp;
+        for (int k0 = 0; k0 < n; k0 += kBandNB) {
+            const int nk = std::min(kBandNB, n - k0), r0 = k0 + nk, m = std::min(n - r0, W - kBandNB);
+            MSA_TRY(timed(b, KK_ASSEMBLE, [&] { return launch_chol_diag(s, cp, ns, k0, nk); }));
+            MSA_TRY(timed(b, KK_ASSEMBLE, [&] { return launch_chol_panel(s, cp, ns, k0, nk, r0, m); }));
+            MSA_TRY(timed(b, KK_ASSEMBLE, [&] { return launch_chol_update(s, cp, ns, k0, nk, r0, m); }));
+        }
+    }
+    // a coarse operator that is not positive definite leaves the cycle undefined
+    for (int ci : b->mg_coarsest) {
+        ClassHost &C = tab(b, ci);
+        std::vector<int> fail(C.nsamp);
+        MSA_HIP(hipMemcpyAsync(fail.data(), C.dev.bfail, fail.size() * sizeof(int),
+                               hipMemcpyDeviceToHost, s));
+        MSA_HIP(hipStreamSynchronize(s));
+        for (int k = 0; k < C.nsamp; k++)
+            MSA_CHECK(!fail[k], MSA_ERR_STATE,
+                      "multigrid: coarsest operator of class %d, sample %d is not positive definite",
+                      C.dev.mg_root, k);
+    }
+    b->mg_ready = true;
+    return MSA_OK;
+}
+
+// z = B r for the classes with a hierarchy (B: one V-cycle); their z held
+// D^-1 r on entry.  pbuf selects the free search-direction buffer, mode the
+// samples served (see k_mg_smooth).
+int mg_vcycle(msa_batch *b, int pbuf, int mode)
+{
+    if (b->precond != MSA_PRECOND_MULTIGRID || b->mg_depth == 0) return MSA_OK;
+    hipStream_t s = b->ctx->stream;
+    const bool mx = is_mixed(b);
+    const int T = b->T;
+    const double ratio = b->mg_ratio;
+    auto smooth = [&](int d, int step) {
+        return timed(b, KK_MG, [&] {
+            return launch_mg_smooth(s, mx, T, b->d_cls, b->mg_smooth[d], b->mg_nsmooth[d], step,
+                                    pbuf, mode, ratio);
+        });
+    };
+    for (int d = 0; d < b->mg_depth; d++) {
+        MSA_TRY(smooth(d, MG_PRE));
+        MSA_TRY(smooth(d, MG_RESID));
+        MSA_TRY(timed(b, KK_OTHER, [&] {
+            return launch_mg_restrict(s, mx, T, b->d_cls, b->mg_restr[d + 1], b->mg_nrestr[d + 1],
+                                      pbuf, mode);
+        }));
+    }
+    for (int ci : b->mg_coarsest) {
+        ClassHost &C = tab(b, ci);
+        const ClassDev *cp = b->d_cls + ci;
+        const int n = C.dev.bn, W = C.dev.bW, ns = C.nsamp;
+        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_mg_band_in(s, mx, T, cp, ns, n); }));
+        for (int k0 = 0; k0 < n; k0 += kBandNB) {
+            const int nk = std::min(kBandNB, n - k0), r0 = k0 + nk, m = std::min(n - r0, W - kBandNB);
+            MSA_TRY(timed(b, KK_OTHER, [&] { return launch_band_fwd(s, cp, ns, k0, nk, r0, m); }));
+        }
+        const int last = ((n - 1) / kBandNB) * kBandNB;
+        for (int k0 = last; k0 >= 0; k0 -= kBandNB) {
+            const int nk = std::min(kBandNB, n - k0), r0 = k0 + nk, m = std::min(n - r0, W - kBandNB);
+            MSA_TRY(timed(b, KK_OTHER, [&] { return launch_band_bwd(s, cp, ns, k0, nk, r0, m); }));
+        }
+        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_mg_band_out(s, mx, T, cp, ns, 3 * C.dev.nb); }));
+    }
+    for (int d = b->mg_depth - 1; d >= 0; d--) {
+        MSA_TRY(timed(b, KK_OTHER, [&] {
+            return launch_mg_prolong(s, mx, T, b->d_cls, b->mg_smooth[d], b->mg_nsmooth[d], mode);
+        }));
+        MSA_TRY(smooth(d, MG_POST1));
+        MSA_TRY(smooth(d, MG_POST2));
+    }
+    return MSA_OK;
 }
 
 }  // namespace
@@ -494,29 +983,15 @@ int msa_mesh_create(msa_ctx *ctx, int n_nodes, const double *xyz, int n_elem, co
         delete m;
         return rc;
     }
-    const MeshHost &h = m->h;
-    hipError_t e = hipSetDevice(ctx->device);
-    auto up = [&](void **dst, const void *src, size_t bytes) {
-        if (e != hipSuccess) return;
-        e = hipMalloc(dst, bytes ? bytes : 16);
-        if (e == hipSuccess && bytes)
-            e = hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
-    };
-    up((void **)&m->d_xyz, h.xyz.data(), h.xyz.size() * sizeof(double));
-    up((void **)&m->d_conn, h.conn.data(), h.conn.size() * sizeof(int));
-    up((void **)&m->d_node_block, h.node_block.data(), h.node_block.size() * sizeof(int));
-    up((void **)&m->d_node_free, h.node_free.data(), h.node_free.size());
-    up((void **)&m->d_emap, h.emap.data(), h.emap.size());
-    up((void **)&m->d_colour_elems, h.colour_elems.data(), h.colour_elems.size() * sizeof(int));
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) {
-        set_error("mesh upload failed: %s", hipGetErrorString(e));
+    rc = upload_mesh(m);
+    if (rc != MSA_OK) {
         msa_mesh_destroy(m);
-        return MSA_ERR_HIP;
+        return rc;
     }
     *out = m;
     return MSA_OK;
 }
+
 
 int msa_mesh_info(const msa_mesh *mesh, long long *info)
 {
@@ -739,6 +1214,7 @@ int msa_batch_destroy(msa_batch *b)
     if (b->timing.e0) (void)hipEventDestroy(b->timing.e0);
     if (b->timing.e1) (void)hipEventDestroy(b->timing.e1);
     for (void *p : b->allocs) (void)hipFree(p);
+    for (msa_mesh *m : b->mg_meshes) msa_mesh_destroy(m);
     delete b;
     return MSA_OK;
 }
@@ -768,8 +1244,6 @@ int msa_batch_create(msa_ctx *ctx, int n_classes, msa_mesh *const *meshes, const
     b->precision = precision;
     b->T = tile;
     b->R = 64 / tile;
-    const int T = b->T, R = b->R;
-    const size_t rs = real_size(b);
     b->cls.resize(n_classes);
     int rc = MSA_OK;
     auto fail = [&](int code) {
@@ -777,126 +1251,10 @@ int msa_batch_create(msa_ctx *ctx, int n_classes, msa_mesh *const *meshes, const
         return code;
     };
     for (int c = 0; c < n_classes && rc == MSA_OK; c++) {
-        ClassHost &C = b->cls[c];
-        const MeshHost &h = meshes[c]->h;
-        C.mesh = meshes[c];
-        C.nsamp = n_samples[c];
-        C.ntiles = (C.nsamp + T - 1) / T;
-        C.nslices = (h.nb + R - 1) / R;
-        C.chunk = kWaves * std::max(1, 16 / R);
-        C.nchunks = (C.nslices + C.chunk - 1) / C.chunk;
-        C.slice_ptr.assign(C.nslices + 1, 0);
-        for (int sl = 0; sl < C.nslices; sl++) {
-            int wmax = 0;
-            for (int rr = 0; rr < R; rr++) {
-                int br = sl * R + rr;
-                if (br < h.nb) wmax = std::max(wmax, h.brow_ptr[br + 1] - h.brow_ptr[br]);
-            }
-            C.slice_ptr[sl + 1] = C.slice_ptr[sl] + wmax;
-        }
-        const long long nslots = C.slice_ptr[C.nslices];
-        C.cidx.assign((size_t)nslots * R, -1);
-        for (int br = 0; br < h.nb; br++) {
-            int sl = br / R, rr = br % R;
-            for (int j = 0; j < h.brow_ptr[br + 1] - h.brow_ptr[br]; j++)
-                C.cidx[(size_t)(C.slice_ptr[sl] + j) * R + rr] = h.bcol[h.brow_ptr[br] + j];
-        }
-        std::vector<uint8_t> bfree(h.nb);
-        for (int br = 0; br < h.nb; br++) bfree[br] = h.node_free[h.block_node[br]];
-        C.tile_vals = nslots * 9 * R * T;
-        C.tile_vec = (long long)h.nb * 3 * T;
-
-        ClassDev &D = C.dev;
-        memset(&D, 0, sizeof(D));
-        D.nb = h.nb;
-        D.nslices = C.nslices;
-        D.ntiles = C.ntiles;
-        D.nsamp = C.nsamp;
-        D.n_elem = h.n_elem;
-        D.nchunks = C.nchunks;
-        D.chunk = C.chunk;
-        D.tile_vals = C.tile_vals;
-        D.tile_vec = C.tile_vec;
-        D.xyz = meshes[c]->d_xyz;
-        D.conn = meshes[c]->d_conn;
-        D.node_block = meshes[c]->d_node_block;
-        D.node_free = meshes[c]->d_node_free;
-        D.emap = meshes[c]->d_emap;
-        D.colour_elems = meshes[c]->d_colour_elems;
-        MSA_CHECK(h.ncolours <= kMaxColours, fail(MSA_ERR_ARG), "too many colours");
-        for (int k = 0; k <= h.ncolours; k++) D.colour_ptr[k] = h.colour_ptr[k];
-        auto A = [&](void **p, size_t bytes) {
-            if (rc == MSA_OK) rc = dalloc(b, p, bytes);
-        };
-        void *p = nullptr;
-        A(&p, C.slice_ptr.size() * sizeof(int));
-        D.slice_ptr = (const int *)p;
-        if (rc == MSA_OK) MSA_HIP(hipMemcpy(p, C.slice_ptr.data(), C.slice_ptr.size() * sizeof(int), hipMemcpyHostToDevice));
-        A(&p, C.cidx.size() * sizeof(int));
-        D.cidx = (const int *)p;
-        if (rc == MSA_OK) MSA_HIP(hipMemcpy(p, C.cidx.data(), C.cidx.size() * sizeof(int), hipMemcpyHostToDevice));
-        A(&p, h.diag_pos.size() * sizeof(int));
-        D.diag_pos = (const int *)p;
-        if (rc == MSA_OK) MSA_HIP(hipMemcpy(p, h.diag_pos.data(), h.diag_pos.size() * sizeof(int), hipMemcpyHostToDevice));
-        A(&p, bfree.size());
-        D.block_free = (const uint8_t *)p;
-        if (rc == MSA_OK) MSA_HIP(hipMemcpy(p, bfree.data(), bfree.size(), hipMemcpyHostToDevice));
-        {
-            // gather lists of the assembly: element contributions per stored
-            // block, in ascending element order
-            std::vector<int> gmap((size_t)nslots * 8 * R, -1);
-            std::vector<uint8_t> cnt((size_t)nslots * R, 0);
-            for (int e = 0; e < h.n_elem && rc == MSA_OK; e++)
-                for (int a = 0; a < 8; a++) {
-                    const int br = h.node_block[h.conn[8 * e + a]];
-                    if (br < 0) continue;
-                    for (int bb = 0; bb < 8; bb++) {
-                        const int j = h.emap[(size_t)e * 64 + a * 8 + bb];
-                        if (j == 255) continue;
-                        const size_t sr = (size_t)(C.slice_ptr[br / R] + j) * R + br % R;
-                        const int k = cnt[sr]++;
-                        if (k >= 8) {
-                            set_error("mesh: more than 8 elements share a node pair");
-                            rc = MSA_ERR_ARG;
-                            break;
-                        }
-                        gmap[((size_t)(C.slice_ptr[br / R] + j) * 8 + k) * R + br % R] =
-                            e * 64 + a * 8 + bb;
-                    }
-                }
-            A(&p, gmap.size() * sizeof(int));
-            D.gmap = (const int *)p;
-            if (rc == MSA_OK)
-                MSA_HIP(hipMemcpy(p, gmap.data(), gmap.size() * sizeof(int), hipMemcpyHostToDevice));
-            A((void **)&D.kc, (size_t)h.n_elem * 576 * sizeof(double));
-        }
-        const size_t nvec = (size_t)C.ntiles * C.tile_vec;
-        A(&D.val, (size_t)C.ntiles * C.tile_vals * rs);
-        A(&D.x, nvec * rs);
-        A(&D.r, nvec * rs);
-        A(&D.z, nvec * rs);
-        A(&D.p0, nvec * rs);
-        A(&D.p1, nvec * rs);
-        A(&D.q, nvec * rs);
-        A(&D.dinv, nvec * rs);
-        if (is_mixed(b)) {
-            A((void **)&D.u64, nvec * sizeof(double));
-            A((void **)&D.r64, nvec * sizeof(double));
-        }
-        A(&p, (size_t)h.nb * 3 * sizeof(double));
-        D.f = (const double *)p;
-        A((void **)&D.E, (size_t)C.ntiles * h.n_elem * T * sizeof(double));
-        A((void **)&D.st, (size_t)C.ntiles * T * sizeof(SampleState));
-        A((void **)&D.partial, (size_t)C.ntiles * C.nchunks * 3 * T * sizeof(double));
-        A((void **)&D.negdet, sizeof(int));
-        A((void **)&D.due, (1 + (size_t)C.ntiles * T) * sizeof(int));
+        rc = setup_class(b, b->cls[c], meshes[c], n_samples[c]);
         if (rc != MSA_OK) break;
-        hipStream_t s = ctx->stream;
-        MSA_HIP(hipMemsetAsync(D.st, 0, (size_t)C.ntiles * T * sizeof(SampleState), s));
-        MSA_HIP(hipMemsetAsync(D.negdet, 0, sizeof(int), s));
-        MSA_HIP(hipMemsetAsync(D.due, 0, sizeof(int), s));
-        MSA_HIP(hipMemsetAsync((void *)D.f, 0, (size_t)h.nb * 3 * sizeof(double), s));
-        MSA_HIP(hipMemsetAsync(D.E, 0, (size_t)C.ntiles * h.n_elem * T * sizeof(double), s));
+        ClassHost &C = b->cls[c];
+        C.dev.mg_root = c;
         for (int t = 0; t < C.ntiles; t++) {
             b->tiles.push_back(make_int2(c, t));
             for (int k = 0; k < C.nchunks; k++) b->items.push_back(make_int4(c, t, k, 0));
@@ -906,6 +1264,7 @@ int msa_batch_create(msa_ctx *ctx, int n_classes, msa_mesh *const *meshes, const
     void *p = nullptr;
     rc = dalloc(b, &p, sizeof(ClassDev) * n_classes);
     b->d_cls = (ClassDev *)p;
+    b->d_cls_cap = n_classes;
     if (rc == MSA_OK) rc = dalloc(b, &p, sizeof(int4) * b->items.size());
     b->d_items = (int4 *)p;
     if (rc == MSA_OK) rc = dalloc(b, &p, 4 * sizeof(int));
@@ -1105,6 +1464,8 @@ int msa_batch_assemble(msa_batch *b, double nu)
     b->nu = nu;
     b->assembled = true;
     b->factored = false;
+    b->mg_ready = false;
+    if (b->precond == MSA_PRECOND_MULTIGRID) MSA_TRY(mg_values(b));
     return MSA_OK;
 }
 
@@ -1123,6 +1484,7 @@ int msa_batch_solve(msa_batch *b, double tol, int max_it, double inner_tol, int 
     hipStream_t s = b->ctx->stream;
     const bool mx = is_mixed(b);
     const int T = b->T, ni = (int)b->items.size();
+    if (!b->mg_ready) MSA_TRY(mg_values(b));
     if (b->dyn) {
         // static solve with the effective matrix: per-sample right-hand side = f
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_fs_static(s, T, b->d_cls, b->d_items, ni); }));
@@ -1132,6 +1494,7 @@ int msa_batch_solve(msa_batch *b, double tol, int max_it, double inner_tol, int 
     if (!mx) {
         MSA_HIP(hipMemsetAsync(b->d_counts, 0, sizeof(int), s));
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_pcg_init(s, false, T, b->d_cls, b->d_items, ni, 0, b->d_counts); }));
+        MSA_TRY(mg_vcycle(b, 1, 0));
         MSA_TRY(reduce(b, OP_INIT0));
         return pcg_loop(b, 0, tol, max_it, graph_iters);
     }
@@ -1139,6 +1502,7 @@ int msa_batch_solve(msa_batch *b, double tol, int max_it, double inner_tol, int 
         MSA_HIP(hipMemsetAsync(b->d_counts, 0, 2 * sizeof(int), s));
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_ir_begin(s, T, b->d_cls, b->d_items, ni, b->d_counts + 1, 0); }));
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_pcg_init(s, true, T, b->d_cls, b->d_items, ni, 1, b->d_counts); }));
+        MSA_TRY(mg_vcycle(b, 1, 3));
         MSA_TRY(reduce(b, OP_INIT1));
         MSA_TRY(pcg_loop(b, 1, tol, max_it, graph_iters));
         MSA_HIP(hipStreamSynchronize(s));
@@ -1155,6 +1519,7 @@ int msa_batch_solve(msa_batch *b, double tol, int max_it, double inner_tol, int 
         if (left <= 0) break;
         MSA_HIP(hipMemsetAsync(b->d_counts, 0, sizeof(int), s));
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_pcg_init(s, true, T, b->d_cls, b->d_items, ni, 1, b->d_counts); }));
+        MSA_TRY(mg_vcycle(b, 1, 3));
         MSA_TRY(reduce(b, OP_INIT1));
         MSA_TRY(pcg_loop(b, 0, inner_tol, max_it, graph_iters));
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_ir_update(s, T, b->d_cls, b->d_items, ni); }));
@@ -1312,6 +1677,7 @@ int msa_batch_newmark(msa_batch *b, int nsteps, const double *acc, int acc_dir, 
     MSA_HIP(hipSetDevice(b->ctx->device));
     hipStream_t s = b->ctx->stream;
     const int T = b->T, ni = (int)b->items.size();
+    if (b->solver != MSA_SOLVER_DIRECT && !b->mg_ready) MSA_TRY(mg_values(b));
     if (acc_dir >= 0) {
         if (b->acc_cap < nsteps) {
             void *p = nullptr;
@@ -1371,11 +1737,13 @@ int msa_batch_newmark(msa_batch *b, int nsteps, const double *acc, int acc_dir, 
         } else if (!mx && warm_start) {
             MSA_HIP(hipMemsetAsync(b->d_counts, 0, sizeof(int), s));
             MSA_TRY(timed(b, KK_OTHER, [&] { return launch_pcg_init_warm(s, T, b->d_cls, b->d_items, ni); }));
+            MSA_TRY(mg_vcycle(b, 1, 0));
             MSA_TRY(reduce(b, OP_INIT2, tol));
             MSA_TRY(pcg_loop(b, 0, tol, max_it, graph_iters));
         } else if (!mx) {
             MSA_HIP(hipMemsetAsync(b->d_counts, 0, sizeof(int), s));
             MSA_TRY(timed(b, KK_OTHER, [&] { return launch_pcg_init(s, false, T, b->d_cls, b->d_items, ni, 0, b->d_counts); }));
+            MSA_TRY(mg_vcycle(b, 1, 0));
             MSA_TRY(reduce(b, OP_INIT0));
             MSA_TRY(pcg_loop(b, 0, tol, max_it, graph_iters));
         } else {
@@ -1392,6 +1760,7 @@ int msa_batch_newmark(msa_batch *b, int nsteps, const double *acc, int acc_dir, 
             }));
             MSA_TRY(reduce(b, OP_IRNORM, tol, 0, 1 << 30));
             MSA_TRY(timed(b, KK_OTHER, [&] { return launch_pcg_init(s, true, T, b->d_cls, b->d_items, ni, 1, b->d_counts); }));
+            MSA_TRY(mg_vcycle(b, 1, 3));
             MSA_TRY(reduce(b, OP_INIT1));
             MSA_TRY(pcg_loop(b, 1, tol, max_it, graph_iters));
         }
@@ -1453,6 +1822,85 @@ int msa_batch_direct_solve(msa_batch *b)
     for (size_t c = 0; c < b->cls.size(); c++) MSA_TRY(band_solve_class(b, c, b->dyn));
     MSA_TRY(direct_stats(b));
     MSA_HIP(hipStreamSynchronize(s));
+    return MSA_OK;
+}
+
+int msa_batch_set_preconditioner(msa_batch *b, int kind, int max_levels, double ratio)
+{
+    MSA_CHECK(b, MSA_ERR_ARG, "null batch");
+    MSA_CHECK(kind == MSA_PRECOND_JACOBI || kind == MSA_PRECOND_MULTIGRID, MSA_ERR_ARG,
+              "preconditioner kind=%d", kind);
+    MSA_CHECK(ratio == 0.0 || ratio > 1.0, MSA_ERR_ARG, "smoothing ratio %g (0 or > 1)", ratio);
+    MSA_HIP(hipSetDevice(b->ctx->device));
+    if (kind == MSA_PRECOND_MULTIGRID) {
+        if (b->mg_nlev.empty()) {
+            MSA_TRY(mg_build(b, max_levels));
+            b->mg_max_levels = max_levels;
+        } else {
+            MSA_CHECK(max_levels == b->mg_max_levels, MSA_ERR_STATE,
+                      "the batch's hierarchy was built with max_levels=%d", b->mg_max_levels);
+        }
+    }
+    b->precond = kind;
+    b->mg_ratio = ratio > 0 ? ratio : 12.0;
+    if (b->graph) {  // the captured iteration block holds the preconditioner
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    b->mg_ready = false;
+    if (b->assembled) MSA_TRY(mg_values(b));
+    return MSA_OK;
+}
+
+int msa_batch_mg_info(msa_batch *b, int cls, long long *info)
+{
+    MSA_CHECK(b && info && cls >= 0 && cls < (int)b->cls.size(), MSA_ERR_ARG, "bad arguments");
+    info[0] = info[1] = info[2] = info[3] = 0;
+    if (b->mg_nlev.empty()) return MSA_OK;
+    info[0] = b->mg_nlev[cls];
+    for (int ci = b->cls[cls].dev.mg_coarse; ci >= 0; ci = tab(b, ci).dev.mg_coarse) {
+        const ClassHost &L = tab(b, ci);
+        info[2] += L.mesh->h.n_free;
+        if (L.dev.mg_coarse < 0) {
+            info[1] = L.mesh->h.n_free;
+            info[3] = L.bw;
+        }
+    }
+    return MSA_OK;
+}
+
+int msa_batch_mg_apply(msa_batch *b, int cls, int sample, const double *r, double *z)
+{
+    MSA_CHECK(b && r && z && cls >= 0 && cls < (int)b->cls.size(), MSA_ERR_ARG, "bad arguments");
+    ClassHost &C = b->cls[cls];
+    MSA_CHECK(sample >= 0 && sample < C.nsamp, MSA_ERR_ARG, "sample %d out of range", sample);
+    MSA_CHECK(b->assembled, MSA_ERR_STATE, "batch not assembled");
+    MSA_CHECK(b->precond == MSA_PRECOND_MULTIGRID && C.dev.mg_coarse >= 0, MSA_ERR_STATE,
+              "class %d has no multigrid hierarchy", cls);
+    MSA_HIP(hipSetDevice(b->ctx->device));
+    if (!b->mg_ready) MSA_TRY(mg_values(b));
+    const MeshHost &m = C.mesh->h;
+    const int tile = sample / b->T, lane = sample % b->T;
+    const size_t n = (size_t)m.nb * 3;
+    const bool mx = is_mixed(b);
+    std::vector<double> xi(n, 0.0);
+    for (int g = 0; g < m.n_free; g++) xi[m.ref2int[g]] = r[g];
+    hipStream_t s = b->ctx->stream;
+    double *din = nullptr;
+    MSA_HIP(hipMalloc(&din, n * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(din, xi.data(), n * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = launch_mg_load(s, mx, b->T, b->d_cls + cls, din, n, tile, lane);
+    int rc = e == hipSuccess ? mg_vcycle(b, 1, 0) : MSA_OK;
+    const size_t rsz = real_size(b);
+    const char *zt = (const char *)C.dev.z + (size_t)tile * C.tile_vec * rsz;
+    if (e == hipSuccess && rc == MSA_OK) e = launch_extract_sample(s, mx, b->T, zt, n, lane, din);
+    if (e == hipSuccess && rc == MSA_OK)
+        e = hipMemcpyAsync(xi.data(), din, n * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(din);
+    MSA_HIP(e);
+    MSA_TRY(rc);
+    for (int g = 0; g < m.n_free; g++) z[g] = xi[m.ref2int[g]];
     return MSA_OK;
 }
 
@@ -1709,7 +2157,7 @@ int msa_batch_set_kernel_timing(msa_batch *b, int enable)
 int msa_batch_get_kernel_timing(msa_batch *b, double *ms_total, long long *launches)
 {
     MSA_CHECK(b && ms_total && launches, MSA_ERR_ARG, "bad arguments");
-    for (int k = 0; k < 5; k++) {
+    for (int k = 0; k < kKernelKinds; k++) {
         ms_total[k] = b->timing.ms[k];
         launches[k] = b->timing.n[k];
     }

@@ -1640,7 +1640,7 @@ __global__ __launch_bounds__(256) void k_dyn_update(const ClassDev *__restrict__
 // stay zero).  grid.y = sample of the class.
 
 // Lower band entries of every sample's assembled K (FP64 batch values).
-template <int T>
+template <typename real, int T>
 __global__ __launch_bounds__(256) void k_band_scatter(const ClassDev *__restrict__ cls,
                                                       const int4 *__restrict__ items, int nitems)
 {
@@ -1655,7 +1655,7 @@ __global__ __launch_bounds__(256) void k_band_scatter(const ClassDev *__restrict
     const int rr = lane / T, s = lane % T;
     const int sample = tile * T + s;
     if (sample >= C.nsamp) return;
-    const double *val = (const double *)C.val + (size_t)tile * C.tile_vals;
+    const real *val = (const real *)C.val + (size_t)tile * C.tile_vals;
     double *AB = C.band + (size_t)sample * C.bn * C.bW;
     const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
     for (int slice = s0 + wave; slice < s1; slice += kWaves) {
@@ -1673,7 +1673,7 @@ __global__ __launch_bounds__(256) void k_band_scatter(const ClassDev *__restrict
                     const int jj = C.binv[3 * col + q];
                     if (jj < 0 || jj > i) continue;
                     AB[(size_t)jj * C.bW + (i - jj)] =
-                        val[(((size_t)slot * 9 + 3 * p + q) * R + rr) * T + s];
+                        (double)val[(((size_t)slot * 9 + 3 * p + q) * R + rr) * T + s];
                 }
             }
         }
@@ -2225,6 +2225,371 @@ __global__ __launch_bounds__(1024) void k_reduce(const ClassDev *__restrict__ cl
         *C.due = 0;
 }
 
+// ------------------------------------------------------------ multigrid
+
+// Geometric multigrid V-cycle used as the PCG preconditioner z = B r.
+// Level 0 is the solved class (b = its PCG residual r), coarser levels are
+// rediscretizations of the mesh coarsened by two (b = restricted residual,
+// stored in the level's r).  Per level: two Chebyshev-Jacobi steps from
+// zero, residual, restriction, coarse correction, prolongation, two
+// Chebyshev-Jacobi steps.  The same polynomial before and after keeps B
+// symmetric; its interval [lam/ratio, lam] uses the Gershgorin bound lam of
+// D^-1 K, so the smoother contracts every mode and B stays positive
+// definite.  Buffers of a level: z holds D^-1 b on entry and the level's
+// result on exit; A = q and B = p_pbuf are free between PCG updates.
+
+__device__ __forceinline__ bool mg_act(const SampleState &S, int mode)
+{
+    switch (mode) {
+    case 0: return true;
+    case 1: return S.active;
+    case 2: return S.ir_active && S.ru_due;
+    default: return S.ir_active;
+    }
+}
+
+// Chebyshev coefficients of the two-step smoother on [lam/ratio, lam]:
+// x1 = x0 + d1, d1 = D^-1 r0 / theta; x2 = x1 + c1 d1 + c2 D^-1 r1.
+struct ChebCoef {
+    double ith, c1, c2;
+};
+
+__device__ __forceinline__ ChebCoef cheb_coef(double lam, double ratio)
+{
+    const double hi = lam, lo = lam / ratio;
+    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo);
+    const double sigma = theta / delta, rho0 = 1.0 / sigma, rho1 = 1.0 / (2.0 * sigma - rho0);
+    return ChebCoef{1.0 / theta, rho1 * rho0, 2.0 * rho1 / delta};
+}
+
+template <typename real, int T>
+__device__ __forceinline__ void gather3(const MSA_GLOBAL real *v, int col, int s, real &v0,
+                                        real &v1, real &v2)
+{
+    if constexpr (T == 1) {
+        const vec3<real> c = ld3(v, col);
+        v0 = c.x;
+        v1 = c.y;
+        v2 = c.z;
+    } else {
+        const size_t o = (size_t)col * 3 * T + s;
+        v0 = v[o];
+        v1 = v[o + T];
+        v2 = v[o + 2 * T];
+    }
+}
+
+// y = K v for block row br (slots [base, base + W) of its slice): groups of
+// three slots, padding columns clamped to block 0 (their values are zero).
+template <typename real, int T>
+__device__ __forceinline__ void row_product(const MSA_GLOBAL real *val, const MSA_GLOBAL int *cidx,
+                                            const MSA_GLOBAL real *v, int base, int W, int rr,
+                                            int s, real &a0, real &a1, real &a2)
+{
+    constexpr int R = 64 / T;
+    constexpr int ES = R * T;
+    a0 = a1 = a2 = 0;
+    int j = 0;
+    for (; j + 3 <= W; j += 3) {
+        int col[3];
+#pragma unroll
+        for (int u = 0; u < 3; u++) col[u] = max(cidx[(size_t)(base + j + u) * R + rr], 0);
+        real x[3][3], m[3][9];
+#pragma unroll
+        for (int u = 0; u < 3; u++) {
+            gather3<real, T>(v, col[u], s, x[u][0], x[u][1], x[u][2]);
+            const MSA_GLOBAL real *mp = val + ((size_t)(base + j + u) * 9 * R + rr) * T + s;
+#pragma unroll
+            for (int e = 0; e < 9; e++) m[u][e] = __builtin_nontemporal_load(mp + e * ES);
+        }
+#pragma unroll
+        for (int u = 0; u < 3; u++) {
+            a0 += m[u][0] * x[u][0] + m[u][1] * x[u][1] + m[u][2] * x[u][2];
+            a1 += m[u][3] * x[u][0] + m[u][4] * x[u][1] + m[u][5] * x[u][2];
+            a2 += m[u][6] * x[u][0] + m[u][7] * x[u][1] + m[u][8] * x[u][2];
+        }
+    }
+    for (; j < W; j++) {
+        const int col = max(cidx[(size_t)(base + j) * R + rr], 0);
+        real x0, x1, x2, m[9];
+        gather3<real, T>(v, col, s, x0, x1, x2);
+        const MSA_GLOBAL real *mp = val + ((size_t)(base + j) * 9 * R + rr) * T + s;
+#pragma unroll
+        for (int e = 0; e < 9; e++) m[e] = __builtin_nontemporal_load(mp + e * ES);
+        a0 += m[0] * x0 + m[1] * x1 + m[2] * x2;
+        a1 += m[3] * x0 + m[4] * x1 + m[5] * x2;
+        a2 += m[6] * x0 + m[7] * x1 + m[8] * x2;
+    }
+}
+
+// One smoothing-level product with its row update (STEP, see MgStep):
+//   MG_PRE   A = (1 + c1) x1 + c2 D^-1 (b - K x1),  x1 = z / theta
+//   MG_RESID B = b - K A
+//   MG_POST1 B = A + D^-1 (b - K A) / theta
+//   MG_POST2 z = B + c1 (B - A) + c2 D^-1 (b - K B); level 0 also writes
+//            the partial sums of r.z (slot 0 of the PCG partials)
+template <typename real, int T, int STEP>
+__global__ __launch_bounds__(256) void k_mg_smooth(const ClassDev *__restrict__ cls,
+                                                   const int4 *__restrict__ items, int nitems,
+                                                   int pbuf, int mode, double ratio)
+{
+    constexpr int R = 64 / T;
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const int tile = it.y, chunk = it.z;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int rr = lane / T, s = lane % T;
+    const int sample = tile * T + s;
+    const bool act = sample < C.nsamp && mg_act(cls[C.mg_root].st[sample], mode);
+    const ChebCoef cc = cheb_coef(act ? C.mg_lam[sample] : 1.0, ratio);
+    const real ith = (real)cc.ith, c1 = (real)cc.c1, c2 = (real)cc.c2;
+    const size_t vb = (size_t)tile * C.tile_vec;
+    const MSA_GLOBAL real *__restrict__ b = gbl((const real *)C.r + vb);
+    const MSA_GLOBAL real *__restrict__ dinv = gbl((const real *)C.dinv + vb);
+    MSA_GLOBAL real *__restrict__ z = gbl((real *)C.z + vb);
+    MSA_GLOBAL real *__restrict__ A = gbl((real *)C.q + vb);
+    MSA_GLOBAL real *__restrict__ B = gbl((real *)(pbuf ? C.p1 : C.p0) + vb);
+    const MSA_GLOBAL real *src = STEP == MG_PRE ? z : (STEP == MG_POST2 ? B : A);
+    MSA_GLOBAL real *dst = STEP == MG_PRE ? A : (STEP == MG_POST2 ? z : B);
+    const MSA_GLOBAL real *__restrict__ val = gbl((const real *)C.val + (size_t)tile * C.tile_vals);
+    const MSA_GLOBAL int *__restrict__ cidx = gbl(C.cidx);
+    const MSA_GLOBAL int *__restrict__ sptr = gbl(C.slice_ptr);
+    real rz = 0;
+    const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+    if (__ballot(act)) {
+        for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+            const int br = slice * R + rr;
+            const int base = sptr[slice];
+            const int W = sptr[slice + 1] - base;
+            if (!act || br >= C.nb) continue;
+            real y[3];
+            row_product<real, T>(val, cidx, src, base, W, rr, s, y[0], y[1], y[2]);
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const size_t o = ((size_t)br * 3 + d) * T + s;
+                const real bv = b[o], dv = dinv[o];
+                real out;
+                if constexpr (STEP == MG_PRE) {
+                    const real x1 = z[o] * ith;
+                    out = x1 + c1 * x1 + c2 * dv * (bv - y[d] * ith);
+                } else if constexpr (STEP == MG_RESID) {
+                    out = bv - y[d];
+                } else if constexpr (STEP == MG_POST1) {
+                    out = A[o] + ith * dv * (bv - y[d]);
+                } else {
+                    const real xb = B[o];
+                    out = xb + c1 * (xb - A[o]) + c2 * dv * (bv - y[d]);
+                    rz += bv * out;
+                }
+                dst[o] = out;
+            }
+        }
+    }
+    if constexpr (STEP == MG_POST2) {
+        if (C.mg_depth == 0) {
+            double acc[1] = {(double)rz};
+            block_partials<T, 1>(acc, C.partial + ((size_t)tile * C.nchunks + chunk) * 3 * T);
+        }
+    }
+}
+
+// Restriction to a coarse level: b_c = P^T (b - K A) of the finer level
+// (its B buffer), free components only; z_c = D_c^-1 b_c starts the coarse
+// smoother.  Items: the coarse level's.
+template <typename real, int T>
+__global__ __launch_bounds__(256) void k_mg_restrict(const ClassDev *__restrict__ cls,
+                                                     const int4 *__restrict__ items, int nitems,
+                                                     int pbuf, int mode)
+{
+    constexpr int R = 64 / T;
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const ClassDev &F = cls[C.mg_fine];
+    const int tile = it.y, chunk = it.z;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rr = lane / T, s = lane % T;
+    const int sample = tile * T + s;
+    if (sample >= C.nsamp || !mg_act(cls[C.mg_root].st[sample], mode)) return;
+    const real *Bf = (const real *)(pbuf ? F.p1 : F.p0) + (size_t)tile * F.tile_vec;
+    const size_t vb = (size_t)tile * C.tile_vec;
+    real *bc = (real *)C.r + vb, *zc = (real *)C.z + vb;
+    const real *dinv = (const real *)C.dinv + vb;
+    const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+    for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+        const int cb = slice * R + rr;
+        if (cb >= C.nb) continue;
+        double acc[3] = {0, 0, 0};
+        for (int k = C.mg_r_ptr[cb]; k < C.mg_r_ptr[cb + 1]; k++) {
+            const int fb = C.mg_r_col[k];
+            const double w = C.mg_r_w[k];
+            const uint8_t ff = F.block_free[fb];
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+                if ((ff >> d) & 1) acc[d] += w * (double)Bf[((size_t)fb * 3 + d) * T + s];
+        }
+        const uint8_t cf = C.block_free[cb];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const size_t o = ((size_t)cb * 3 + d) * T + s;
+            const real v = ((cf >> d) & 1) ? (real)acc[d] : (real)0;
+            bc[o] = v;
+            zc[o] = dinv[o] * v;
+        }
+    }
+}
+
+// Coarse correction: A += P z_c on the free components.  Items: the finer
+// level's.
+template <typename real, int T>
+__global__ __launch_bounds__(256) void k_mg_prolong(const ClassDev *__restrict__ cls,
+                                                    const int4 *__restrict__ items, int nitems,
+                                                    int mode)
+{
+    constexpr int R = 64 / T;
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const ClassDev &K = cls[C.mg_coarse];
+    const int tile = it.y, chunk = it.z;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rr = lane / T, s = lane % T;
+    const int sample = tile * T + s;
+    if (sample >= C.nsamp || !mg_act(cls[C.mg_root].st[sample], mode)) return;
+    const real *zc = (const real *)K.z + (size_t)tile * K.tile_vec;
+    real *A = (real *)C.q + (size_t)tile * C.tile_vec;
+    const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+    for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+        const int fb = slice * R + rr;
+        if (fb >= C.nb) continue;
+        double acc[3] = {0, 0, 0};
+        for (int k = C.mg_p_ptr[fb]; k < C.mg_p_ptr[fb + 1]; k++) {
+            const int cb = C.mg_p_col[k];
+            const double w = C.mg_p_w[k];
+#pragma unroll
+            for (int d = 0; d < 3; d++) acc[d] += w * (double)zc[((size_t)cb * 3 + d) * T + s];
+        }
+        const uint8_t ff = C.block_free[fb];
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            if ((ff >> d) & 1) {
+                const size_t o = ((size_t)fb * 3 + d) * T + s;
+                A[o] = (real)((double)A[o] + acc[d]);
+            }
+    }
+}
+
+// Modulus of a coarse element = mean of its eight children's (fixed order).
+template <int T>
+__global__ void k_mg_coarsen_E(const ClassDev *__restrict__ cp, const ClassDev *__restrict__ cls)
+{
+    const ClassDev &C = *cp;
+    const ClassDev &F = cls[C.mg_fine];
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = (size_t)C.ntiles * C.n_elem * T;
+    if (i >= n) return;
+    const int s = (int)(i % T);
+    const size_t te = i / T;
+    const int e = (int)(te % C.n_elem), tile = (int)(te / C.n_elem);
+    double acc = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        acc += F.E[((size_t)tile * F.n_elem + C.mg_child[8 * (size_t)e + k]) * T + s];
+    C.E[i] = 0.125 * acc;
+}
+
+// lam = max over free rows of sum_j |K_ij| / K_ii (Gershgorin bound of the
+// spectrum of D^-1 K); mg_lam must hold 0 before.  The maximum of
+// non-negative doubles is taken on their bit patterns (order-independent).
+template <typename real, int T>
+__global__ __launch_bounds__(256) void k_mg_gersh(const ClassDev *__restrict__ cls,
+                                                  const int4 *__restrict__ items, int nitems)
+{
+    constexpr int R = 64 / T;
+    constexpr int ES = R * T;
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const int tile = it.y, chunk = it.z;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rr = lane / T, s = lane % T;
+    const int sample = tile * T + s;
+    const real *val = (const real *)C.val + (size_t)tile * C.tile_vals;
+    const real *dinv = (const real *)C.dinv + (size_t)tile * C.tile_vec;
+    double g = 0;
+    const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+    if (sample < C.nsamp) {
+        for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+            const int br = slice * R + rr;
+            if (br >= C.nb) continue;
+            const int base = C.slice_ptr[slice], W = C.slice_ptr[slice + 1] - base;
+            double sum[3] = {0, 0, 0};
+            for (int j = 0; j < W; j++) {
+                const real *m = val + ((size_t)(base + j) * 9 * R + rr) * T + s;
+#pragma unroll
+                for (int p = 0; p < 3; p++)
+                    sum[p] += fabs((double)m[(3 * p) * ES]) + fabs((double)m[(3 * p + 1) * ES]) +
+                              fabs((double)m[(3 * p + 2) * ES]);
+            }
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+                g = fmax(g, sum[p] * fabs((double)dinv[((size_t)br * 3 + p) * T + s]));
+        }
+    }
+#pragma unroll
+    for (int off = T; off < 64; off <<= 1) g = fmax(g, __shfl_xor(g, off, 64));
+    if (rr == 0 && sample < C.nsamp && g > 0)
+        atomicMax((unsigned long long *)&C.mg_lam[sample], (unsigned long long)__double_as_longlong(g));
+}
+
+// One sample's level-0 right-hand side for a single cycle: r = src,
+// z = D^-1 r (the state a PCG update leaves).
+template <typename real, int T>
+__global__ void k_mg_load(const ClassDev *__restrict__ cp, const double *__restrict__ src,
+                          int tile, int lane)
+{
+    const ClassDev &C = *cp;
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= 3 * C.nb) return;
+    const size_t o = (size_t)tile * C.tile_vec + (size_t)row * T + lane;
+    const real v = (real)src[row];
+    ((real *)C.r)[o] = v;
+    ((real *)C.z)[o] = ((const real *)C.dinv)[o] * v;
+}
+
+// Coarsest level: its right-hand side (r) into band order, and the band
+// solution back into z.
+template <typename real, int T>
+__global__ void k_mg_band_in(const ClassDev *__restrict__ cp)
+{
+    const ClassDev &C = *cp;
+    const int smp = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= C.bn) return;
+    const int tile = smp / T, s = smp % T;
+    C.by[(size_t)smp * C.bn + i] =
+        (double)((const real *)C.r)[(size_t)tile * C.tile_vec + (size_t)C.bperm[i] * T + s];
+}
+
+template <typename real, int T>
+__global__ void k_mg_band_out(const ClassDev *__restrict__ cp)
+{
+    const ClassDev &C = *cp;
+    const int smp = blockIdx.y;
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= 3 * C.nb) return;
+    const int tile = smp / T, s = smp % T;
+    const int i = C.binv[row];
+    ((real *)C.z)[(size_t)tile * C.tile_vec + (size_t)row * T + s] =
+        i >= 0 ? (real)C.by[(size_t)smp * C.bn + i] : (real)0;
+}
+
 // ------------------------------------------------------------ utilities
 
 template <typename V>
@@ -2485,7 +2850,14 @@ hipError_t launch_reduce(hipStream_t s, int T, const ClassDev *d_cls, const int2
 hipError_t launch_band_scatter(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,
                                int nitems)
 {
-    MSA_T_LAUNCH(k_band_scatter, d_cls, items, nitems);
+    const bool mixed = false;
+    MSA_PCG_LAUNCH(k_band_scatter, d_cls, items, nitems);
+}
+
+hipError_t launch_band_scatter_any(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
+                                   const int4 *items, int nitems)
+{
+    MSA_PCG_LAUNCH(k_band_scatter, d_cls, items, nitems);
 }
 
 hipError_t launch_chol_diag(hipStream_t s, const ClassDev *cp, int nsamp, int k0, int nk)
@@ -2667,6 +3039,104 @@ hipError_t launch_extract_values(hipStream_t s, bool mixed, int T, const void *v
                                  int lane, double *dst)
 {
     return launch_extract_sample(s, mixed, T, val, n, lane, dst);
+}
+
+template <int STEP>
+static hipError_t mg_smooth_step(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
+                                 const int4 *items, int nitems, int pbuf, int mode, double ratio)
+{
+    if (nitems <= 0) return hipSuccess;
+    const dim3 grid(xcd_grid(nitems)), block(256);
+    if (mixed) {
+        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_mg_smooth<float, TT, STEP>), grid, block, 0, s,
+                                             d_cls, items, nitems, pbuf, mode, ratio));
+    } else {
+        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_mg_smooth<double, TT, STEP>), grid, block, 0, s,
+                                             d_cls, items, nitems, pbuf, mode, ratio));
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_mg_smooth(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
+                            const int4 *items, int nitems, int step, int pbuf, int mode,
+                            double ratio)
+{
+    switch (step) {
+    case MG_PRE: return mg_smooth_step<MG_PRE>(s, mixed, T, d_cls, items, nitems, pbuf, mode, ratio);
+    case MG_RESID: return mg_smooth_step<MG_RESID>(s, mixed, T, d_cls, items, nitems, pbuf, mode, ratio);
+    case MG_POST1: return mg_smooth_step<MG_POST1>(s, mixed, T, d_cls, items, nitems, pbuf, mode, ratio);
+    case MG_POST2: return mg_smooth_step<MG_POST2>(s, mixed, T, d_cls, items, nitems, pbuf, mode, ratio);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_mg_restrict(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
+                              const int4 *items, int nitems, int pbuf, int mode)
+{
+    MSA_PCG_LAUNCH(k_mg_restrict, d_cls, items, nitems, pbuf, mode);
+}
+
+hipError_t launch_mg_prolong(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
+                             const int4 *items, int nitems, int mode)
+{
+    MSA_PCG_LAUNCH(k_mg_prolong, d_cls, items, nitems, mode);
+}
+
+hipError_t launch_mg_coarsen_E(hipStream_t s, int T, const ClassDev *cp, const ClassDev *d_cls,
+                               int ntiles, int n_elem)
+{
+    const size_t n = (size_t)ntiles * n_elem * T;
+    if (!n) return hipSuccess;
+    MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_mg_coarsen_E<TT>), dim3((unsigned)((n + 255) / 256)),
+                                         dim3(256), 0, s, cp, d_cls));
+    return hipGetLastError();
+}
+
+hipError_t launch_mg_gersh(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
+                           const int4 *items, int nitems)
+{
+    MSA_PCG_LAUNCH(k_mg_gersh, d_cls, items, nitems);
+}
+
+hipError_t launch_mg_band_in(hipStream_t s, bool mixed, int T, const ClassDev *cp, int nsamp,
+                             int bn)
+{
+    if (bn <= 0) return hipSuccess;
+    const dim3 grid((bn + 255) / 256, nsamp);
+    if (mixed) {
+        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_mg_band_in<float, TT>), grid, dim3(256), 0, s, cp));
+    } else {
+        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_mg_band_in<double, TT>), grid, dim3(256), 0, s, cp));
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_mg_load(hipStream_t s, bool mixed, int T, const ClassDev *cp, const double *src,
+                          size_t nrows, int tile, int lane)
+{
+    if (!nrows) return hipSuccess;
+    const dim3 grid((unsigned)((nrows + 255) / 256));
+    if (mixed) {
+        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_mg_load<float, TT>), grid, dim3(256), 0, s, cp, src,
+                                             tile, lane));
+    } else {
+        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_mg_load<double, TT>), grid, dim3(256), 0, s, cp, src,
+                                             tile, lane));
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_mg_band_out(hipStream_t s, bool mixed, int T, const ClassDev *cp, int nsamp,
+                              int nrows)
+{
+    if (nrows <= 0) return hipSuccess;
+    const dim3 grid((nrows + 255) / 256, nsamp);
+    if (mixed) {
+        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_mg_band_out<float, TT>), grid, dim3(256), 0, s, cp));
+    } else {
+        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_mg_band_out<double, TT>), grid, dim3(256), 0, s, cp));
+    }
+    return hipGetLastError();
 }
 
 }  // namespace msa

@@ -277,3 +277,163 @@ int band_order(const MeshHost &m, std::vector<int> &perm, std::vector<int> &inv,
 }
 
 }  // namespace msa
+
+// ------------------------------------------------------------ multigrid levels
+
+namespace msa {
+
+// Sorted distinct coordinates of one axis (merged within tol) and the index
+// of every node in them.
+static void axis_grid(const MeshHost &m, int d, std::vector<double> &vals, std::vector<int> &idx)
+{
+    std::vector<double> c(m.n_nodes);
+    for (int n = 0; n < m.n_nodes; n++) c[n] = m.xyz[3 * n + d];
+    std::vector<double> s(c);
+    std::sort(s.begin(), s.end());
+    const double tol = 1e-9 * std::max(1e-300, s.back() - s.front());
+    vals.clear();
+    for (double v : s)
+        if (vals.empty() || v - vals.back() > tol) vals.push_back(v);
+    idx.resize(m.n_nodes);
+    for (int n = 0; n < m.n_nodes; n++) {
+        auto it = std::lower_bound(vals.begin(), vals.end(), c[n] - tol);
+        idx[n] = (int)(it - vals.begin());
+    }
+}
+
+int mg_coarsen(const MeshHost &f, MeshHost &c, MgTransfer &t)
+{
+    std::vector<double> gx[3];
+    std::vector<int> gi[3];
+    for (int d = 0; d < 3; d++) axis_grid(f, d, gx[d], gi[d]);
+    const int nx = (int)gx[0].size(), ny = (int)gx[1].size(), nz = (int)gx[2].size();
+    MSA_CHECK((long long)nx * ny * nz == f.n_nodes, MSA_ERR_ARG,
+              "multigrid: the nodes do not form a tensor-product grid");
+    const int ex = nx - 1, ey = ny - 1, ez = nz - 1;
+    MSA_CHECK(ex >= 4 && ey >= 4 && ez >= 4 && ex % 2 == 0 && ey % 2 == 0 && ez % 2 == 0,
+              MSA_ERR_ARG, "multigrid: %d x %d x %d elements cannot be halved", ex, ey, ez);
+    MSA_CHECK((long long)ex * ey * ez == f.n_elem, MSA_ERR_ARG,
+              "multigrid: element count does not match the grid");
+    std::vector<int> gnode((size_t)nx * ny * nz, -1);
+    for (int n = 0; n < f.n_nodes; n++) {
+        const size_t g = gi[0][n] + (size_t)nx * (gi[1][n] + (size_t)ny * gi[2][n]);
+        MSA_CHECK(gnode[g] < 0, MSA_ERR_ARG, "multigrid: two nodes share grid point");
+        gnode[g] = n;
+    }
+    // every element must be one grid cell
+    std::vector<int> cell_elem((size_t)ex * ey * ez, -1);
+    for (int e = 0; e < f.n_elem; e++) {
+        int lo[3] = {1 << 30, 1 << 30, 1 << 30};
+        for (int a = 0; a < 8; a++)
+            for (int d = 0; d < 3; d++) lo[d] = std::min(lo[d], gi[d][f.conn[8 * e + a]]);
+        int seen = 0;
+        for (int a = 0; a < 8; a++) {
+            const int n = f.conn[8 * e + a];
+            int off = 0;
+            for (int d = 0; d < 3; d++) {
+                const int o = gi[d][n] - lo[d];
+                MSA_CHECK(o == 0 || o == 1, MSA_ERR_ARG, "multigrid: element %d is not a grid cell", e);
+                off |= o << d;
+            }
+            seen |= 1 << off;
+        }
+        MSA_CHECK(seen == 255, MSA_ERR_ARG, "multigrid: element %d is not a grid cell", e);
+        const size_t cell = lo[0] + (size_t)ex * (lo[1] + (size_t)ey * lo[2]);
+        MSA_CHECK(cell_elem[cell] < 0, MSA_ERR_ARG, "multigrid: two elements share a cell");
+        cell_elem[cell] = e;
+    }
+    // coarse grid: every other grid line
+    const int cx = ex / 2 + 1, cy = ey / 2 + 1, cz = ez / 2 + 1;
+    const int cn_nodes = cx * cy * cz, cn_elem = (ex / 2) * (ey / 2) * (ez / 2);
+    std::vector<double> cxyz(3 * (size_t)cn_nodes);
+    std::vector<uint8_t> cfix(3 * (size_t)cn_nodes);
+    for (int K = 0; K < cz; K++)
+        for (int J = 0; J < cy; J++)
+            for (int I = 0; I < cx; I++) {
+                const int cn = I + cx * (J + cy * K);
+                const int fn = gnode[2 * I + (size_t)nx * (2 * J + (size_t)ny * 2 * K)];
+                for (int d = 0; d < 3; d++) {
+                    cxyz[3 * cn + d] = f.xyz[3 * fn + d];
+                    cfix[3 * cn + d] = f.dof[3 * fn + d] < 0;
+                }
+            }
+    std::vector<int> cdof(3 * (size_t)cn_nodes);
+    int cn_free = 0;
+    for (size_t i = 0; i < cdof.size(); i++) cdof[i] = cfix[i] ? -1 : cn_free++;
+    std::vector<int> cconn(8 * (size_t)cn_elem);
+    t.child.assign(8 * (size_t)cn_elem, -1);
+    for (int K = 0; K < ez / 2; K++)
+        for (int J = 0; J < ey / 2; J++)
+            for (int I = 0; I < ex / 2; I++) {
+                const int ce = I + (ex / 2) * (J + (ey / 2) * K);
+                for (int o = 0; o < 8; o++) {
+                    const size_t cell = (2 * I + (o & 1)) +
+                                        (size_t)ex * ((2 * J + ((o >> 1) & 1)) +
+                                                      (size_t)ey * (2 * K + (o >> 2)));
+                    t.child[8 * (size_t)ce + o] = cell_elem[cell];
+                }
+                // local node order of the child at the cell's low corner
+                const int e0 = t.child[8 * (size_t)ce];
+                for (int a = 0; a < 8; a++) {
+                    const int n = f.conn[8 * e0 + a];
+                    const int oi = gi[0][n] - 2 * I, oj = gi[1][n] - 2 * J, ok = gi[2][n] - 2 * K;
+                    cconn[8 * (size_t)ce + a] = (I + oi) + cx * ((J + oj) + cy * (K + ok));
+                }
+            }
+    {
+        const int rc = build_mesh_host(c, cn_nodes, cxyz.data(), cn_elem, cconn.data(), cdof.data(), cn_free);
+        if (rc != MSA_OK) return rc;
+    }
+    // trilinear interpolation: a fine node on an odd grid line takes the two
+    // neighbouring coarse lines, weighted by distance
+    auto axis_w = [&](int d, int i, int *ci, double *w) {
+        if (i % 2 == 0) {
+            ci[0] = i / 2;
+            w[0] = 1.0;
+            return 1;
+        }
+        const double x0 = gx[d][i - 1], x1 = gx[d][i], x2 = gx[d][i + 1];
+        ci[0] = (i - 1) / 2;
+        w[0] = (x2 - x1) / (x2 - x0);
+        ci[1] = (i + 1) / 2;
+        w[1] = (x1 - x0) / (x2 - x0);
+        return 2;
+    };
+    t.p_ptr.assign(f.nb + 1, 0);
+    t.p_col.clear();
+    t.p_w.clear();
+    for (int fb = 0; fb < f.nb; fb++) {
+        const int n = f.block_node[fb];
+        int ci[3][2];
+        double w[3][2];
+        const int k0 = axis_w(0, gi[0][n], ci[0], w[0]);
+        const int k1 = axis_w(1, gi[1][n], ci[1], w[1]);
+        const int k2 = axis_w(2, gi[2][n], ci[2], w[2]);
+        for (int c2 = 0; c2 < k2; c2++)
+            for (int c1 = 0; c1 < k1; c1++)
+                for (int c0 = 0; c0 < k0; c0++) {
+                    const int cn = ci[0][c0] + cx * (ci[1][c1] + cy * ci[2][c2]);
+                    const int cb = c.node_block[cn];
+                    if (cb < 0) continue;
+                    t.p_col.push_back(cb);
+                    t.p_w.push_back(w[0][c0] * w[1][c1] * w[2][c2]);
+                }
+        t.p_ptr[fb + 1] = (int)t.p_col.size();
+    }
+    // restriction = transpose, fine blocks ascending per coarse block
+    t.r_ptr.assign(c.nb + 1, 0);
+    for (int cb : t.p_col) t.r_ptr[cb + 1]++;
+    for (int i = 0; i < c.nb; i++) t.r_ptr[i + 1] += t.r_ptr[i];
+    t.r_col.assign(t.p_col.size(), 0);
+    t.r_w.assign(t.p_col.size(), 0.0);
+    std::vector<int> fill(t.r_ptr.begin(), t.r_ptr.end() - 1);
+    for (int fb = 0; fb < f.nb; fb++)
+        for (int k = t.p_ptr[fb]; k < t.p_ptr[fb + 1]; k++) {
+            const int pos = fill[t.p_col[k]]++;
+            t.r_col[pos] = fb;
+            t.r_w[pos] = t.p_w[k];
+        }
+    return MSA_OK;
+}
+
+}  // namespace msa

@@ -86,6 +86,21 @@ int build_mesh_host(MeshHost &m, int n_nodes, const double *xyz, int n_elem,
 // inv[internal row] = band index or -1; bw = half bandwidth.
 int band_order(const MeshHost &m, std::vector<int> &perm, std::vector<int> &inv, long long &bw);
 
+// Grid transfer between a structured mesh and its coarsening by two in
+// every direction (trilinear interpolation of node values).
+struct MgTransfer {
+    std::vector<int> child;                // [coarse n_elem*8] fine elements of each coarse element
+    std::vector<int> p_ptr, p_col;         // per fine block: coarse blocks it interpolates from
+    std::vector<double> p_w;               //   and their weights
+    std::vector<int> r_ptr, r_col;         // per coarse block: fine blocks (transpose of p)
+    std::vector<double> r_w;
+};
+// Coarse mesh c (every other grid line) of a tensor-product grid mesh f
+// whose element counts are even; MSA_ERR_ARG (with the reason) otherwise.
+// Coarse nodes inherit the constraints of the fine nodes they sit on; each
+// coarse element keeps the local node order of its low-corner child.
+int mg_coarsen(const MeshHost &f, MeshHost &c, MgTransfer &t);
+
 // Device view of one class of a batch (copied to device memory).
 struct ClassDev {
     int nb, nslices, ntiles, nsamp;
@@ -134,6 +149,17 @@ struct ClassDev {
     double *by;           // [nsamp][bn] right-hand side / solution in band order
     double *bpart;        // [nsamp][kBandChunks][64] partial sums of the backward solve
     int *bfail;           // [nsamp] non-positive pivot met
+    // geometric multigrid preconditioner (msa_batch_set_preconditioner).
+    // Every class of a hierarchy is an entry of the class table; mg_* are
+    // table indices (-1: none).  root: the solved class whose sample state
+    // drives the cycle; coarse: the next coarser level; fine: the finer one.
+    int mg_root, mg_coarse, mg_fine, mg_depth;
+    double *mg_lam;       // [ntiles*T] upper bound of the spectrum of D^-1 K (Gershgorin)
+    const int *mg_p_ptr, *mg_p_col;      // prolongation rows of this class's blocks
+    const double *mg_p_w;
+    const int *mg_r_ptr, *mg_r_col;      // restriction rows of this (coarse) class's blocks
+    const double *mg_r_w;
+    const int *mg_child;  // [n_elem*8] elements of the finer level in each element
 };
 
 constexpr int kBandNB = 64;        // block size of the band Cholesky
@@ -178,7 +204,8 @@ struct ReduceArgs {
     int *ir_count;
 };
 
-enum KernelKind { KK_SPMV = 0, KK_UPDATE = 1, KK_ASSEMBLE = 2, KK_RES64 = 3, KK_OTHER = 4 };
+enum KernelKind { KK_SPMV = 0, KK_UPDATE = 1, KK_ASSEMBLE = 2, KK_RES64 = 3, KK_OTHER = 4, KK_MG = 5 };
+constexpr int kKernelKinds = 6;
 
 // Launch wrappers (kernels.hip).  T is the lane tile of the batch.
 hipError_t launch_fill_f64(hipStream_t s, double *p, size_t n, double v);
@@ -244,6 +271,29 @@ hipError_t launch_band_fwd(hipStream_t s, const ClassDev *cp, int nsamp, int k0,
 hipError_t launch_band_bwd(hipStream_t s, const ClassDev *cp, int nsamp, int k0, int nk, int r0,
                            int m);
 hipError_t launch_band_out(hipStream_t s, int T, const ClassDev *cp, int nsamp, int nrows);
+// multigrid preconditioner: items of one level of every hierarchy
+enum MgStep { MG_PRE = 0, MG_RESID = 1, MG_POST1 = 2, MG_POST2 = 3 };
+// which samples a cycle serves: 0 every sample, 1 active, 2 due for a
+// residual replacement (ir_active && ru_due), 3 refining (ir_active)
+hipError_t launch_mg_smooth(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
+                            const int4 *items, int nitems, int step, int pbuf, int mode,
+                            double ratio);
+hipError_t launch_mg_restrict(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
+                              const int4 *items, int nitems, int pbuf, int mode);
+hipError_t launch_mg_prolong(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
+                             const int4 *items, int nitems, int mode);
+hipError_t launch_mg_coarsen_E(hipStream_t s, int T, const ClassDev *cp, const ClassDev *d_cls,
+                               int ntiles, int n_elem);
+hipError_t launch_mg_gersh(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
+                           const int4 *items, int nitems);
+hipError_t launch_mg_band_in(hipStream_t s, bool mixed, int T, const ClassDev *cp, int nsamp,
+                             int bn);
+hipError_t launch_mg_band_out(hipStream_t s, bool mixed, int T, const ClassDev *cp, int nsamp,
+                              int nrows);
+hipError_t launch_mg_load(hipStream_t s, bool mixed, int T, const ClassDev *cp, const double *src,
+                          size_t nrows, int tile, int lane);
+hipError_t launch_band_scatter_any(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
+                                   const int4 *items, int nitems);
 // dynamics
 hipError_t launch_set_int(hipStream_t s, int *p, int n, int v);
 hipError_t launch_dyn_bacc(hipStream_t s, const ClassDev *d_cls, const ColourLaunch &cl,

@@ -1,0 +1,207 @@
+"""GPU parity of the multigrid preconditioner (msa_batch_set_preconditioner)
+against oracle/multigrid.py: one V-cycle on the same vector (FP64 within
+1e-10, FP32 levels within 2e-5), preconditioned solves against the direct
+solution (FP64 PCG to 1e-11: within 1e-9; mixed reliable updates to 1e-9:
+within 1e-8, the Monte Carlo workload's bar), iteration counts against the
+oracle's preconditioned CG, graph replay against eager launches, lane tiles,
+a class that does not coarsen (keeps Jacobi), and Newmark steps."""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spl
+
+import msolve_amd as msa
+from msolve_amd import _lib as L
+from oracle import multigrid as MG
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def lognormal(rng, n, cov=0.3):
+    s = np.sqrt(np.log(1 + cov * cov))
+    return 2.1e5 * np.exp(-0.5 * s * s + s * rng.standard_normal(n))
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def make(ctx, sizes, nsamp, precision, tile=0, seed=0, mg=True):
+    models = [msa.cantilever(n) for n in sizes]
+    meshes = [msa.Mesh(ctx, m.xyz, m.conn, m.dof, m.n_free) for m in models]
+    b = msa.Batch(ctx, meshes, [nsamp] * len(sizes), precision, tile)
+    rng = np.random.default_rng(seed)
+    Es = []
+    for c, m in enumerate(models):
+        E = np.stack([lognormal(rng, m.n_elem) for _ in range(nsamp)])
+        b.set_modulus(c, E)
+        b.set_rhs(c, m.f)
+        Es.append(E)
+    b.assemble(0.3)
+    if mg:
+        b.set_preconditioner(msa.MSA_PRECOND_MULTIGRID)
+    return b, models, meshes, Es
+
+
+def oracle_levels(m, E):
+    return MG.hierarchy(m.xyz.reshape(-1, 3), m.conn, m.dof, m.n_free, E)
+
+
+def close(b, meshes):
+    b.close()
+    for m in meshes:
+        m.close()
+
+
+def test_hierarchy_sizes_match_oracle(ctx):
+    print("\n[case] mg hierarchy sizes 8,10,12,9", flush=True)
+    b, models, meshes, Es = make(ctx, [8, 10, 12, 9], 1, L.MSA_FP64)
+    for c, m in enumerate(models):
+        info = b.mg_info(c)
+        levels, _ = oracle_levels(m, Es[c][0])
+        assert info["levels"] == len(levels) - 1
+        if info["levels"]:
+            assert info["coarsest_dofs"] == levels[-1].n
+            assert info["coarse_dofs"] == sum(L_.n for L_ in levels[1:])
+    assert b.mg_info(3)["levels"] == 0
+    close(b, meshes)
+
+
+@pytest.mark.parametrize("precision,bar", [(L.MSA_FP64, 1e-10), (L.MSA_MIXED, 2e-5)])
+def test_vcycle_matches_oracle(ctx, precision, bar):
+    print(f"\n[case] mg apply precision={precision}", flush=True)
+    b, models, meshes, Es = make(ctx, [8, 10, 12], 2, precision, seed=4)
+    rng = np.random.default_rng(9)
+    for c, m in enumerate(models):
+        levels, Ps = oracle_levels(m, Es[c][1])
+        r = rng.standard_normal(m.n_free)
+        z = b.mg_apply(c, 1, r)
+        zr = MG.vcycle(levels, Ps, r)
+        assert rel(z, zr) < bar, (c, rel(z, zr))
+    close(b, meshes)
+
+
+def test_fp64_solves_match_direct_and_oracle_iterations(ctx):
+    print("\n[case] mg fp64 solves 8,12,9", flush=True)
+    b, models, meshes, Es = make(ctx, [8, 12, 9], 3, L.MSA_FP64, seed=5)
+    b.solve(1e-11, 5000)
+    for c, m in enumerate(models):
+        st = b.stats(c)
+        assert (st["status"] == msa.MSA_OK).all(), st
+        for s in range(3):
+            levels, Ps = oracle_levels(m, Es[c][s])
+            ud = spl.spsolve(levels[0].K.tocsc(), m.f)
+            assert rel(b.solution(c, s), ud) < 1e-9
+            if c < 2:
+                _, it = MG.mg_pcg(levels, Ps, m.f, tol=1e-11)
+                assert abs(int(st["iters"][s]) - it) <= 1, (c, s, st["iters"][s], it)
+                assert it < 25
+        if c == 2:  # 9^3 does not coarsen: Jacobi
+            assert st["iters"].min() > 40
+    close(b, meshes)
+
+
+def test_mixed_reliable_updates_reach_the_workload_bar(ctx):
+    print("\n[case] mg mixed 8,12,10", flush=True)
+    b, models, meshes, Es = make(ctx, [8, 12, 10], 3, L.MSA_MIXED, seed=6)
+    b.solve(1e-9, 5000, graph_iters=8)
+    for c, m in enumerate(models):
+        st = b.stats(c)
+        assert (st["status"] == msa.MSA_OK).all(), st
+        assert st["iters"].max() < 40, st["iters"]
+        for s in range(3):
+            K = oracle_levels(m, Es[c][s])[0][0].K
+            ud = spl.spsolve(K.tocsc(), m.f)
+            assert rel(b.solution(c, s), ud) < 1e-8
+    close(b, meshes)
+
+
+@pytest.mark.parametrize("precision", [L.MSA_FP64, L.MSA_MIXED])
+def test_graph_equals_eager(ctx, precision):
+    print(f"\n[case] mg graph vs eager precision={precision}", flush=True)
+    out = []
+    for g in (0, 8):
+        b, models, meshes, Es = make(ctx, [8, 12], 2, precision, seed=7)
+        b.solve(1e-10, 5000, graph_iters=g)
+        out.append(([b.solution(c, s) for c in range(2) for s in range(2)],
+                    [b.stats(c)["iters"].copy() for c in range(2)]))
+        close(b, meshes)
+    for u0, u1 in zip(out[0][0], out[1][0]):
+        assert np.array_equal(u0, u1)
+    for i0, i1 in zip(out[0][1], out[1][1]):
+        assert np.array_equal(i0, i1)
+
+
+def test_lane_tiles(ctx):
+    print("\n[case] mg tile 4", flush=True)
+    b, models, meshes, Es = make(ctx, [8, 12], 5, L.MSA_FP64, tile=4, seed=8)
+    b.solve(1e-11, 5000)
+    for c, m in enumerate(models):
+        assert (b.stats(c)["status"] == msa.MSA_OK).all()
+        for s in range(5):
+            K = oracle_levels(m, Es[c][s])[0][0].K
+            assert rel(b.solution(c, s), spl.spsolve(K.tocsc(), m.f)) < 1e-9
+    close(b, meshes)
+
+
+def test_restarted_refinement(ctx):
+    print("\n[case] mg restarted refinement", flush=True)
+    b, models, meshes, Es = make(ctx, [8], 2, L.MSA_MIXED, seed=10)
+    b.set_refinement(msa.MSA_REFINE_RESTART)
+    b.solve(1e-9, 5000, inner_tol=1e-4, max_outer=20)
+    m = models[0]
+    for s in range(2):
+        K = oracle_levels(m, Es[0][s])[0][0].K
+        assert rel(b.solution(0, s), spl.spsolve(K.tocsc(), m.f)) < 1e-8
+    close(b, meshes)
+
+
+def test_switch_back_to_jacobi(ctx):
+    print("\n[case] mg then jacobi", flush=True)
+    b, models, meshes, Es = make(ctx, [8], 2, L.MSA_FP64, seed=11)
+    b.solve(1e-11, 5000)
+    it_mg = b.stats(0)["iters"].copy()
+    u_mg = b.solution(0, 0)
+    b.set_preconditioner(msa.MSA_PRECOND_JACOBI)
+    b.solve(1e-11, 5000)
+    it_j = b.stats(0)["iters"]
+    assert (it_j > 3 * it_mg).all()
+    assert rel(b.solution(0, 0), u_mg) < 1e-9
+    with pytest.raises(msa.MsaError):
+        b.set_preconditioner(msa.MSA_PRECOND_MULTIGRID, max_levels=1)
+    with pytest.raises(msa.MsaError):
+        b.set_preconditioner(7)
+    close(b, meshes)
+
+
+@pytest.mark.parametrize("precision,bar", [(L.MSA_FP64, 1e-9), (L.MSA_MIXED, 1e-7)])
+def test_newmark_with_multigrid_matches_skyline_oracle(ctx, precision, bar):
+    print(f"\n[case] mg newmark precision={precision}", flush=True)
+    nex, ney, nez, lz = 8, 8, 4, 0.5
+    model = msa.cantilever(nex, ney, nez, lz=lz)
+    om = O.Cantilever(nex, ney, nez, lz=lz)
+    rng = np.random.default_rng(12)
+    E = np.stack([lognormal(rng, model.n_elem, 0.2) for _ in range(2)])
+    dt, nsteps, period = 2e-3, 20, 0.05
+    acc = np.array([O.sinusoidal_acc(i, 1.0, period, dt, nsteps * dt) for i in range(nsteps)])
+    probe = model.n_free - 1
+    mesh = msa.Mesh(ctx, model.xyz, model.conn, model.dof, model.n_free)
+    b = msa.Batch(ctx, [mesh], [2], precision, 0)
+    b.set_modulus(0, E)
+    b.set_dynamics(50.0, 30.0, 2e-4, 0.25, 0.5, dt)
+    b.set_probe(0, probe)
+    b.assemble(0.3)
+    b.set_preconditioner(msa.MSA_PRECOND_MULTIGRID)
+    assert b.mg_info(0)["levels"] == 1
+    b.newmark(acc, acc_dir=2, tol=1e-12 if precision == L.MSA_FP64 else 1e-10, max_it=5000,
+              graph_iters=8)
+    for s in range(2):
+        h_ref, u_ref, _, _ = O.newmark_skyline(
+            om, E[s], rho=50.0, alphaR=30.0, betaR=2e-4, dt=dt, nsteps=nsteps, acc_dir=2,
+            acc_mag=1.0, acc_period=period, acc_tlen=nsteps * dt, probe=probe)
+        u, _, _, its = b.dynamic_state(0, s)
+        assert rel(b.history(0, s), h_ref) < bar
+        assert rel(u, u_ref) < bar
+        assert its < 30 * nsteps
+    b.close()
+    mesh.close()
