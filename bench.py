@@ -10,7 +10,10 @@ element integration + coloured assembly, FP32 Jacobi-PCG inner solves under
 FP64 iterative refinement (FP64 residual f - K u).  Inputs (meshes, KL
 tables, KL coefficients, loads) are resident on the device before the timed
 region; each step re-evaluates the modulus, re-assembles and re-solves every
-sample.
+sample.  The PCG preconditioner is a geometric multigrid V-cycle per sample
+(--precond mg, default; coarse levels re-assembled every step) or Jacobi
+(--precond jacobi, the reference's SolverPCG choice); the other one is timed
+beside it for comparison.
 
 Reported beside the headline: the same job in plain FP64 (batched), the
 single-sample FP64 GPU PCG, each sample's error against a tightly converged
@@ -57,9 +60,16 @@ def parse():
     p.add_argument("--max-outer", type=int, default=30)
     p.add_argument("--graph-iters", type=int, default=16)
     p.add_argument("--tile", type=int, default=1)
-    p.add_argument("--period", type=int, default=8, help="replacement check period (iterations)")
-    p.add_argument("--drop", type=float, default=0.1, help="FP32 residual drop that triggers a replacement")
+    p.add_argument("--period", type=int, default=None,
+                   help="replacement check period (iterations; default 2 with multigrid, 8 with Jacobi)")
+    p.add_argument("--drop", type=float, default=None,
+                   help="FP32 residual drop that triggers a replacement (default 1e-3 with "
+                        "multigrid, 0.1 with Jacobi)")
     p.add_argument("--refine", choices=["reliable", "restart"], default="reliable")
+    p.add_argument("--precond", choices=["mg", "jacobi"], default="mg",
+                   help="PCG preconditioner: multigrid V-cycle or Jacobi")
+    p.add_argument("--no-compare", action="store_true",
+                   help="skip the timing of the other preconditioner")
     p.add_argument("--only-mixed", action="store_true", help="skip every comparison run")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                    help="only run a device copy of this many MB (HBM counter calibration)")
@@ -86,11 +96,24 @@ def workload(args):
     return [sizes[k] for k in keep], [counts[k] for k in keep]
 
 
-def make_batch(msa, ctx, meshes, models, counts, precision, tile, table, xi, args):
+def refinement(args, precond):
+    """Replacement check period and trigger: a multigrid iteration cuts the
+    residual by about 5x, so checks come every 2 iterations and a
+    replacement waits for a 1e-3 drop; Jacobi-PCG keeps 8 and 0.1."""
+    mg = precond == "mg"
+    return (args.period if args.period is not None else (2 if mg else 8),
+            args.drop if args.drop is not None else (1e-3 if mg else 0.1))
+
+
+def make_batch(msa, ctx, meshes, models, counts, precision, tile, table, xi, args, precond=None):
     b = msa.Batch(ctx, meshes, counts, precision, tile)
+    precond = precond or args.precond
+    if precond == "mg":
+        b.set_preconditioner(msa.MSA_PRECOND_MULTIGRID)
     if precision == msa.MSA_MIXED:
+        period, drop = refinement(args, precond)
         b.set_refinement(msa.MSA_REFINE_RELIABLE if args.refine == "reliable"
-                         else msa.MSA_REFINE_RESTART, args.period, args.drop)
+                         else msa.MSA_REFINE_RESTART, period, drop)
     off = 0
     for c, m in enumerate(models):
         b.set_kl(c, table, m.domain, 2.1e5, args.cov)
@@ -99,6 +122,28 @@ def make_batch(msa, ctx, meshes, models, counts, precision, tile, table, xi, arg
         off += counts[c]
     ctx.synchronize()
     return b
+
+
+def smooth_bytes(b, models, rs):
+    """Algorithmic bytes of all finest-level smoothing launches of one solve.
+    Per cycle and active sample the four steps read the values four times
+    and move 16 vectors (PRE: z gathered, b, D^-1 read, A written; RESID: A,
+    b read, B written; POST1: A, b, D^-1 read, B written; POST2: B, b, D^-1,
+    A read, z written), plus the shared block-column indices and slice
+    pointers once per launch.  Cycles per sample = iterations + replacements
+    (one at the start, one after every replacement that does not finish)."""
+    total = 0.0
+    for c, m in enumerate(models):
+        if b.mg_info(c)["levels"] == 0:
+            continue
+        st = b.stats(c)
+        mesh = b.meshes[c]
+        nnz, n = mesh.info["nnz"], 3 * mesh.info["blocks"]
+        cyc = (st["iters"] + np.maximum(st["outer"], 1)).astype(np.float64)
+        total += cyc.sum() * 4 * (rs * nnz + 4 * rs * n)
+        idx = 4.0 * mesh.info["block_nnz"] + 4.0 * (mesh.info["blocks"] + 1)
+        total += float(cyc.max()) * 4 * idx
+    return total
 
 
 def kernel_a_bytes(b, models, counts, rs, launches):
@@ -136,6 +181,11 @@ def bench_newmark(args):
 
     def make(precision):
         b = msa.Batch(ctx, [mesh], [ns], precision, 1)
+        if args.precond == "mg":
+            b.set_preconditioner(msa.MSA_PRECOND_MULTIGRID)
+        if precision == msa.MSA_MIXED:
+            period, drop = refinement(args, args.precond)
+            b.set_refinement(msa.MSA_REFINE_RELIABLE, period, drop)
         b.set_kl(0, table, model.domain, 2.1e5, 0.2)
         b.set_xi(0, xi)
         b.eval_kl()
@@ -178,7 +228,10 @@ def bench_newmark(args):
         "data": "synthetic",
         "config": {"workload": f"batched Newmark: {ns} lognormal-KL samples (COV 0.2) of a "
                                f"48x48x24 Hexa8 block, rho 7.85e-9, beta 0.25, gamma 0.5, "
-                               f"dt 1e-4, 20 Hz base acceleration, PCG to {args.newmark_tol}",
+                               f"dt 1e-4, 20 Hz base acceleration, "
+                               f"{'multigrid' if args.precond == 'mg' else 'Jacobi'}-preconditioned "
+                               f"PCG to {args.newmark_tol}",
+                   "precond": args.precond,
                    "n_free": model.n_free, "all_ok": bool((st["status"] == msa.MSA_OK).all()),
                    "iters_per_step_mean": float(np.mean(its)) / nsteps,
                    "probe_history_rel_err_vs_fp64": max(errs)},
@@ -249,18 +302,32 @@ def main():
     bm.run(**run_kw)
     kt = bm.kernel_timing()
     bm.set_kernel_timing(False)
-    launches = kt["spmv_dot"]["launches"]
-    a_ms = kt["spmv_dot"]["ms"]
-    bytes_total = kernel_a_bytes(bm, models, counts, 4, launches)
-    avg_bytes = bytes_total / max(launches, 1)
-    avg_s = a_ms / 1e3 / max(launches, 1)
-    achieved = avg_bytes / avg_s / 1e9 if avg_s > 0 else 0.0
     log(f"[bench] kernel timing: {json.dumps(kt)}")
+    a_launches = kt["spmv_dot"]["launches"]
+    a_bytes = kernel_a_bytes(bm, models, counts, 4, a_launches) / max(a_launches, 1)
+    a_s = kt["spmv_dot"]["ms"] / 1e3 / max(a_launches, 1)
+    roof_a = dict(kernel=f"k_pcg_a<float,{args.tile}> (SpMV + p update + p.q)",
+                  launches=a_launches, avg_launch_ms=1e3 * a_s, avg_bytes_per_launch=a_bytes,
+                  achieved=a_bytes / a_s / 1e9 if a_s > 0 else 0.0,
+                  ms_total=kt["spmv_dot"]["ms"])
+    roof_a["frac"] = roof_a["achieved"] / HBM_PEAK_GBS
+    # the dominant kernel: the finest level's smoothing products under multigrid
+    if args.precond == "mg" and kt["multigrid"]["launches"] > 0:
+        launches = kt["multigrid"]["launches"]
+        avg_bytes = smooth_bytes(bm, models, 4) / launches
+        avg_s = kt["multigrid"]["ms"] / 1e3 / launches
+        kname = (f"k_mg_smooth<float,{args.tile},STEP> on the finest level (the cycle's four "
+                 f"products: PRE, RESID, POST1, POST2)")
+    else:
+        launches, avg_bytes, avg_s = a_launches, a_bytes, a_s
+        kname = roof_a["kernel"]
+    achieved = avg_bytes / avg_s / 1e9 if avg_s > 0 else 0.0
     # HBM traffic of k_pcg_a: rocprofv3 FETCH_SIZE/WRITE_SIZE passes (scaled by a
     # device-copy calibration) measured bytes / algorithmic bytes, applied to
     # this run's algorithmic bytes per launch (profiles/k_pcg_a_mixed_montecarlo.md)
     traffic, traffic_ratio = None, None
-    cpath = os.path.join(ROOT, "profiles", "k_pcg_a_counters.json")
+    cpath = os.path.join(ROOT, "profiles", "k_pcg_a_counters.json"
+                         if args.precond == "jacobi" else "k_mg_smooth_counters.json")
     if os.path.exists(cpath):
         cj = json.load(open(cpath))
         traffic_ratio = (cj["fetch_bytes"] + cj["write_bytes"]) / (cj["algorithmic_read"] +
@@ -271,7 +338,23 @@ def main():
     outer_mixed = [s["outer"].tolist() for s in stats]
     bm.close()
 
-    extra = {}
+    extra = {"k_pcg_a": roof_a, "kernel_timing_ms": {k: v["ms"] for k, v in kt.items()}}
+    # ------------------------------------------------ the other preconditioner
+    other = "jacobi" if args.precond == "mg" else "mg"
+    if not args.no_compare and not args.only_mixed:
+        bo = make_batch(msa, ctx, meshes, models, counts, msa.MSA_MIXED, args.tile, table, xi,
+                        args, precond=other)
+        bo.run(**run_kw)
+        ctx.synchronize()
+        t = time.perf_counter()
+        bo.run(**run_kw)
+        ctx.synchronize()
+        to = time.perf_counter() - t
+        extra[f"mixed_{other}_solves_per_s"] = N / to
+        extra[f"mixed_{other}_iters_mean_per_size"] = {
+            str(n): float(np.mean(bo.stats(c)["iters"])) for c, n in enumerate(sizes)}
+        log(f"[bench] mixed with {other}: {to:.2f} s -> {N / to:.1f} solves/s")
+        bo.close()
     # ------------------------------------------------ FP64: batched + oracle
     max_err = None
     if not args.no_fp64:
@@ -309,7 +392,7 @@ def main():
         off = 0
         for c, m in enumerate(models):
             b1 = make_batch(msa, ctx, [meshes[c]], [m], [1], msa.MSA_FP64, 1, table,
-                            xi[off:off + 1], args)
+                            xi[off:off + 1], args, precond="jacobi")
             b1.run(**run_kw)      # warm-up (graph capture)
             ctx.synchronize()
             t = time.perf_counter()
@@ -361,19 +444,23 @@ def main():
             "workload": (f"batched mixed-precision MC: {N} samples, unit-cube Hexa8 cantilevers "
                          f"n in {sizes} (counts {counts}), lognormal KL modulus (mean 2.1e5, "
                          f"COV {args.cov}, {args.terms} terms, corr. length {args.corr}), FP32 "
-                         + (f"Jacobi-PCG with FP64 residual replacement (check every "
-                            f"{args.period} iterations, {args.drop}x drop)"
+                         + ("multigrid-preconditioned " if args.precond == "mg" else "")
+                         + (f"{'PCG' if args.precond == 'mg' else 'Jacobi-PCG'} with FP64 "
+                            f"residual replacement (check every "
+                            f"{refinement(args, args.precond)[0]} iterations, "
+                            f"{refinement(args, args.precond)[1]}x drop)"
                             if args.refine == "reliable" else
                             f"inner Jacobi-PCG (tol {args.inner_tol}) + restarted FP64 refinement")
                          + f" to |f-Ku|/|f| < {args.tol}"),
             "samples": N, "sizes": sizes, "counts": counts, "tile": args.tile,
+            "precond": args.precond,
             "graph_iters": args.graph_iters, "all_converged": bool(all_ok),
             "max_rel_err_vs_fp64": max_err,
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_over_algorithmic": traffic_ratio,
-                     "kernel": f"k_pcg_a<float,{args.tile}> (SpMV + p update + p.q)",
+                     "kernel": kname,
                      "launches": launches, "avg_launch_ms": 1e3 * avg_s,
                      "avg_bytes_per_launch": avg_bytes},
         "cpu_baseline": cpu,

@@ -243,8 +243,8 @@ int msa_batch_spmv(msa_batch *b, int cls, int sample, const double *x,
 
 /* Kernel timing with HIP events on the context stream (eager launches while
  * enabled).  kind: 0 SpMV+dot kernel, 1 update kernel, 2 assembly,
- * 3 FP64 residual, 4 other, 5 multigrid smoothing products.
- * ms_total[6], launches[6]. */
+ * 3 FP64 residual, 4 other, 5 multigrid smoothing products of the finest
+ * level.  ms_total[6], launches[6]. */
 int msa_batch_set_kernel_timing(msa_batch *b, int enable);
 int msa_batch_get_kernel_timing(msa_batch *b, double *ms_total,
                                 long long *launches);
@@ -301,7 +301,9 @@ int msa_batch_direct_info(msa_batch *b, int cls, long long *info);
  * PCG keeps its convergence guarantee; the solution is the same, within the
  * tolerance, as with Jacobi.  The hierarchy is built by the first call with
  * MSA_PRECOND_MULTIGRID (later calls must pass the same max_levels) and
- * re-assembled by every msa_batch_assemble. */
+ * re-assembled by every msa_batch_assemble.  Unless msa_batch_set_refinement
+ * was called, MIXED batches check for residual replacements every 2
+ * iterations with a 1e-3 trigger under multigrid (8 and 0.1 under Jacobi). */
 int msa_batch_set_preconditioner(msa_batch *b, int kind, int max_levels, double ratio);
 /* info[0] coarse levels of class cls (0: Jacobi), [1] free DOFs of the
  * coarsest level, [2] free DOFs summed over the coarse levels, [3] half

@@ -82,6 +82,7 @@ struct msa_batch {
     int refine = MSA_REFINE_RELIABLE;
     int period = 8;       // iterations between replacement checks
     double drop = 0.1;    // FP32 residual reduction that triggers a replacement
+    bool refine_set = false;  // set by msa_batch_set_refinement (else: preconditioner defaults)
     // element integration precision of MIXED batches (values stored in FP32)
     int integration = MSA_FP64;
     // linear Newmark dynamics (msa_batch_set_dynamics)
@@ -653,6 +654,24 @@ int items_to_device(msa_batch *b, const std::vector<int4> &v, int4 **out, int *n
     return MSA_OK;
 }
 
+// CSR rows -> ELL (entry k of row i at k*nrows + i; padding column 0,
+// weight 0); returns the width.
+int to_ell(const std::vector<int> &ptr, const std::vector<int> &col, const std::vector<double> &w,
+           std::vector<int> &ecol, std::vector<float> &ew)
+{
+    const int n = (int)ptr.size() - 1;
+    int width = 0;
+    for (int i = 0; i < n; i++) width = std::max(width, ptr[i + 1] - ptr[i]);
+    ecol.assign((size_t)width * n, 0);
+    ew.assign((size_t)width * n, 0.f);
+    for (int i = 0; i < n; i++)
+        for (int k = ptr[i]; k < ptr[i + 1]; k++) {
+            ecol[(size_t)(k - ptr[i]) * n + i] = col[k];
+            ew[(size_t)(k - ptr[i]) * n + i] = (float)w[k];
+        }
+    return width;
+}
+
 void class_items(const ClassHost &C, int ci, std::vector<int4> &out)
 {
     for (int t = 0; t < C.ntiles; t++)
@@ -698,18 +717,21 @@ int mg_build(msa_batch *b, int max_levels)
             L.has_E = true;
             ClassDev &D = L.dev;
             D.mg_root = c;
+            D.mg_st = b->cls[c].dev.st;
             D.mg_fine = parent;
             D.mg_depth = (int)l + 1;
             const MgTransfer &t = tr[l];
             MSA_TRY(upload_vec(b, t.child, &D.mg_child));
-            MSA_TRY(upload_vec(b, t.r_ptr, &D.mg_r_ptr));
-            MSA_TRY(upload_vec(b, t.r_col, &D.mg_r_col));
-            MSA_TRY(upload_vec(b, t.r_w, &D.mg_r_w));
+            std::vector<int> col;
+            std::vector<float> w;
+            D.mg_r_width = to_ell(t.r_ptr, t.r_col, t.r_w, col, w);
+            MSA_TRY(upload_vec(b, col, &D.mg_r_col));
+            MSA_TRY(upload_vec(b, w, &D.mg_r_w));
             ClassDev &P = tab(b, parent).dev;
             P.mg_coarse = ci;
-            MSA_TRY(upload_vec(b, t.p_ptr, &P.mg_p_ptr));
-            MSA_TRY(upload_vec(b, t.p_col, &P.mg_p_col));
-            MSA_TRY(upload_vec(b, t.p_w, &P.mg_p_w));
+            P.mg_p_width = to_ell(t.p_ptr, t.p_col, t.p_w, col, w);
+            MSA_TRY(upload_vec(b, col, &P.mg_p_col));
+            MSA_TRY(upload_vec(b, w, &P.mg_p_w));
             void *p = nullptr;
             MSA_TRY(dalloc(b, &p, (size_t)tab(b, parent).ntiles * b->T * sizeof(double)));
             P.mg_lam = (double *)p;
@@ -863,8 +885,9 @@ int mg_vcycle(msa_batch *b, int pbuf, int mode)
     const bool mx = is_mixed(b);
     const int T = b->T;
     const double ratio = b->mg_ratio;
+    // timing kind KK_MG: the finest level's smoothing products only
     auto smooth = [&](int d, int step) {
-        return timed(b, KK_MG, [&] {
+        return timed(b, d == 0 ? KK_MG : KK_OTHER, [&] {
             return launch_mg_smooth(s, mx, T, b->d_cls, b->mg_smooth[d], b->mg_nsmooth[d], step,
                                     pbuf, mode, ratio);
         });
@@ -881,17 +904,17 @@ int mg_vcycle(msa_batch *b, int pbuf, int mode)
         ClassHost &C = tab(b, ci);
         const ClassDev *cp = b->d_cls + ci;
         const int n = C.dev.bn, W = C.dev.bW, ns = C.nsamp;
-        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_mg_band_in(s, mx, T, cp, ns, n); }));
+        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_mg_band_in(s, mx, T, cp, ns, n, mode); }));
         for (int k0 = 0; k0 < n; k0 += kBandNB) {
             const int nk = std::min(kBandNB, n - k0), r0 = k0 + nk, m = std::min(n - r0, W - kBandNB);
-            MSA_TRY(timed(b, KK_OTHER, [&] { return launch_band_fwd(s, cp, ns, k0, nk, r0, m); }));
+            MSA_TRY(timed(b, KK_OTHER, [&] { return launch_band_fwd(s, cp, ns, k0, nk, r0, m, mode); }));
         }
         const int last = ((n - 1) / kBandNB) * kBandNB;
         for (int k0 = last; k0 >= 0; k0 -= kBandNB) {
             const int nk = std::min(kBandNB, n - k0), r0 = k0 + nk, m = std::min(n - r0, W - kBandNB);
-            MSA_TRY(timed(b, KK_OTHER, [&] { return launch_band_bwd(s, cp, ns, k0, nk, r0, m); }));
+            MSA_TRY(timed(b, KK_OTHER, [&] { return launch_band_bwd(s, cp, ns, k0, nk, r0, m, mode); }));
         }
-        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_mg_band_out(s, mx, T, cp, ns, 3 * C.dev.nb); }));
+        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_mg_band_out(s, mx, T, cp, ns, 3 * C.dev.nb, mode); }));
     }
     for (int d = b->mg_depth - 1; d >= 0; d--) {
         MSA_TRY(timed(b, KK_OTHER, [&] {
@@ -1843,6 +1866,12 @@ int msa_batch_set_preconditioner(msa_batch *b, int kind, int max_levels, double 
     }
     b->precond = kind;
     b->mg_ratio = ratio > 0 ? ratio : 12.0;
+    if (!b->refine_set) {
+        // a V-cycle iteration cuts the residual about 5x: check for a
+        // replacement every 2 iterations, replace after a 1e-3 drop
+        b->period = kind == MSA_PRECOND_MULTIGRID ? 2 : 8;
+        b->drop = kind == MSA_PRECOND_MULTIGRID ? 1e-3 : 0.1;
+    }
     if (b->graph) {  // the captured iteration block holds the preconditioner
         (void)hipGraphExecDestroy(b->graph);
         b->graph = nullptr;
@@ -2130,6 +2159,7 @@ int msa_batch_set_refinement(msa_batch *b, int method, int period, double drop)
     b->refine = method;
     b->period = period;
     b->drop = drop;
+    b->refine_set = true;
     return MSA_OK;
 }
 

@@ -102,6 +102,18 @@ __device__ __forceinline__ double fval(const ClassDev &C, size_t vb, int br, int
     return C.fs ? C.fs[vb + ((size_t)br * 3 + d) * T + s] : C.f[br * 3 + d];
 }
 
+// Samples a multigrid cycle serves: 0 every sample, 1 active, 2 due for a
+// residual replacement, 3 refining.
+__device__ __forceinline__ bool mg_act(const SampleState &S, int mode)
+{
+    switch (mode) {
+    case 0: return true;
+    case 1: return S.active;
+    case 2: return S.ir_active && S.ru_due;
+    default: return S.ir_active;
+    }
+}
+
 __device__ __forceinline__ double fnorm_of(const ClassDev &C, const SampleState &S)
 {
     return C.fs ? S.fnorm : C.fnorm;
@@ -1123,14 +1135,50 @@ __global__ __launch_bounds__(64) void k_residual64(const ClassDev *__restrict__ 
     const double cK = C.dyn ? C.cK : 1.0, cMe = C.dyn ? -C.cM : 0.0;
     if constexpr (T == 1) {
         if (need_due) {
+            // Two samples per pass (lanes 0-23 and 32-55, lane = element row
+            // i = 3a + p).  Each lane keeps its row of Khat (and Mhat) in
+            // registers for all samples; the element's 24 displacements of a
+            // sample are loaded once and shared by wave shuffles.
+            __shared__ int colb[24];
+            if (lane < 24) {
+                const int nd = en[lane / 3], d = lane % 3;
+                const int bj = C.node_block[nd];
+                colb[lane] = (bj >= 0 && ((C.node_free[nd] >> d) & 1)) ? 3 * bj + d : -1;
+            }
+            __syncthreads();
             const int half = lane >> 5, i = lane & 31;
-            for (int k = 2 * y + half; k < ndue; k += 2 * Y) {
-                const int smp = C.due[1 + k];
-                const SampleState &S = C.st[smp];
-                if (i >= 24 || !S.ir_active || !S.ru_due) continue;
-                const double Es = C.E[(size_t)smp * C.n_elem + e];
-                apply_rows(C, en, Kh, Mh, -(Es * cK), cMe, C.u64, C.u64, C.r64,
-                           (size_t)smp * C.tile_vec, 1, 0, i);
+            const int ii = i < 24 ? i : 0;
+            const int row = i < 24 ? colb[i] : -1;
+            const int ia = ii / 3, p = ii - 3 * (ii / 3);
+            double krow[24], mrow[8];
+#pragma unroll
+            for (int jn = 0; jn < 8; jn++) {
+#pragma unroll
+                for (int d = 0; d < 3; d++) krow[3 * jn + d] = Kh[(ia * 8 + jn) * 9 + p * 3 + d];
+                mrow[jn] = cMe != 0 ? Mh[ia * 8 + jn] : 0.0;
+            }
+            const int base = half << 5;
+            for (int k0 = 2 * y; k0 < ndue; k0 += 2 * Y) {
+                const int k = k0 + half;
+                int smp = 0;
+                bool ok = k < ndue;
+                if (ok) {
+                    smp = C.due[1 + k];
+                    ok = C.st[smp].ir_active && C.st[smp].ru_due;
+                }
+                double uv = 0;
+                if (ok && row >= 0) uv = C.u64[(size_t)smp * C.tile_vec + row];
+                double yk = 0, ym = 0;
+#pragma unroll
+                for (int j = 0; j < 24; j++) yk += krow[j] * __shfl(uv, base + j, 64);
+                if (cMe != 0) {
+#pragma unroll
+                    for (int jn = 0; jn < 8; jn++) ym += mrow[jn] * __shfl(uv, base + 3 * jn + p, 64);
+                }
+                if (ok && row >= 0) {
+                    const double Es = C.E[(size_t)smp * C.n_elem + e];
+                    C.r64[(size_t)smp * C.tile_vec + row] += -(Es * cK) * yk + cMe * ym;
+                }
             }
             return;
         }
@@ -1828,11 +1876,12 @@ __global__ void k_band_rhs(const ClassDev *__restrict__ cp, int from_fs)
 // of the x grid, 64 threads), then rows [r0, r0 + m) minus L21 y_blk.  Two
 // launches per block: `phase` 0 is the diagonal solve, 1 the update.
 __global__ __launch_bounds__(64) void k_band_fwd(const ClassDev *__restrict__ cp, int k0, int nk,
-                                                 int r0, int m, int phase)
+                                                 int r0, int m, int phase, int mode)
 {
     __shared__ double yb[kBandNB];
     const ClassDev &C = *cp;
     const int smp = blockIdx.y;
+    if (mode >= 0 && !mg_act(C.mg_st[smp], mode)) return;  // coarsest multigrid level
     const int t = threadIdx.x;
     const double *AB = C.band + (size_t)smp * C.bn * C.bW;
     double *y = C.by + (size_t)smp * C.bn;
@@ -1861,11 +1910,12 @@ __global__ __launch_bounds__(64) void k_band_fwd(const ClassDev *__restrict__ cp
 // bpart[g][c] = sum over rows i of chunk g of L[i][k0+c] x[i]; phase 1
 // (one workgroup) sums them in chunk order and back-substitutes L11^T.
 __global__ __launch_bounds__(64) void k_band_bwd(const ClassDev *__restrict__ cp, int k0, int nk,
-                                                 int r0, int m, int phase)
+                                                 int r0, int m, int phase, int mode)
 {
     __shared__ double xb[kBandNB];
     const ClassDev &C = *cp;
     const int smp = blockIdx.y;
+    if (mode >= 0 && !mg_act(C.mg_st[smp], mode)) return;  // coarsest multigrid level
     const int t = threadIdx.x;
     const double *AB = C.band + (size_t)smp * C.bn * C.bW;
     double *x = C.by + (size_t)smp * C.bn;
@@ -2238,16 +2288,6 @@ __global__ __launch_bounds__(1024) void k_reduce(const ClassDev *__restrict__ cl
 // definite.  Buffers of a level: z holds D^-1 b on entry and the level's
 // result on exit; A = q and B = p_pbuf are free between PCG updates.
 
-__device__ __forceinline__ bool mg_act(const SampleState &S, int mode)
-{
-    switch (mode) {
-    case 0: return true;
-    case 1: return S.active;
-    case 2: return S.ir_active && S.ru_due;
-    default: return S.ir_active;
-    }
-}
-
 // Chebyshev coefficients of the two-step smoother on [lam/ratio, lam]:
 // x1 = x0 + d1, d1 = D^-1 r0 / theta; x2 = x1 + c1 d1 + c2 D^-1 r1.
 struct ChebCoef {
@@ -2415,22 +2455,29 @@ __global__ __launch_bounds__(256) void k_mg_restrict(const ClassDev *__restrict_
     const int rr = lane / T, s = lane % T;
     const int sample = tile * T + s;
     if (sample >= C.nsamp || !mg_act(cls[C.mg_root].st[sample], mode)) return;
-    const real *Bf = (const real *)(pbuf ? F.p1 : F.p0) + (size_t)tile * F.tile_vec;
+    const MSA_GLOBAL real *Bf = gbl((const real *)(pbuf ? F.p1 : F.p0) + (size_t)tile * F.tile_vec);
     const size_t vb = (size_t)tile * C.tile_vec;
     real *bc = (real *)C.r + vb, *zc = (real *)C.z + vb;
     const real *dinv = (const real *)C.dinv + vb;
+    const MSA_GLOBAL int *rcol = gbl(C.mg_r_col);
+    const MSA_GLOBAL float *rw = gbl(C.mg_r_w);
+    const MSA_GLOBAL uint8_t *ffree = gbl(F.block_free);
+    const int width = C.mg_r_width, nbc = C.nb;
     const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
     for (int slice = s0 + wave; slice < s1; slice += kWaves) {
         const int cb = slice * R + rr;
         if (cb >= C.nb) continue;
         double acc[3] = {0, 0, 0};
-        for (int k = C.mg_r_ptr[cb]; k < C.mg_r_ptr[cb + 1]; k++) {
-            const int fb = C.mg_r_col[k];
-            const double w = C.mg_r_w[k];
-            const uint8_t ff = F.block_free[fb];
-#pragma unroll
-            for (int d = 0; d < 3; d++)
-                if ((ff >> d) & 1) acc[d] += w * (double)Bf[((size_t)fb * 3 + d) * T + s];
+#pragma unroll 3
+        for (int k = 0; k < width; k++) {
+            const int fb = rcol[(size_t)k * nbc + cb];
+            const double w = rw[(size_t)k * nbc + cb];
+            const uint8_t ff = ffree[fb];
+            real v0, v1, v2;
+            gather3<real, T>(Bf, fb, s, v0, v1, v2);
+            acc[0] += (ff & 1) ? w * (double)v0 : 0.0;
+            acc[1] += (ff & 2) ? w * (double)v1 : 0.0;
+            acc[2] += (ff & 4) ? w * (double)v2 : 0.0;
         }
         const uint8_t cf = C.block_free[cb];
 #pragma unroll
@@ -2461,18 +2508,25 @@ __global__ __launch_bounds__(256) void k_mg_prolong(const ClassDev *__restrict__
     const int rr = lane / T, s = lane % T;
     const int sample = tile * T + s;
     if (sample >= C.nsamp || !mg_act(cls[C.mg_root].st[sample], mode)) return;
-    const real *zc = (const real *)K.z + (size_t)tile * K.tile_vec;
+    const MSA_GLOBAL real *zc = gbl((const real *)K.z + (size_t)tile * K.tile_vec);
     real *A = (real *)C.q + (size_t)tile * C.tile_vec;
+    const MSA_GLOBAL int *pcol = gbl(C.mg_p_col);
+    const MSA_GLOBAL float *pw = gbl(C.mg_p_w);
+    const int width = C.mg_p_width, nbf = C.nb;
     const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
     for (int slice = s0 + wave; slice < s1; slice += kWaves) {
         const int fb = slice * R + rr;
         if (fb >= C.nb) continue;
         double acc[3] = {0, 0, 0};
-        for (int k = C.mg_p_ptr[fb]; k < C.mg_p_ptr[fb + 1]; k++) {
-            const int cb = C.mg_p_col[k];
-            const double w = C.mg_p_w[k];
-#pragma unroll
-            for (int d = 0; d < 3; d++) acc[d] += w * (double)zc[((size_t)cb * 3 + d) * T + s];
+#pragma unroll 4
+        for (int k = 0; k < width; k++) {
+            const int cb = pcol[(size_t)k * nbf + fb];
+            const double w = pw[(size_t)k * nbf + fb];
+            real v0, v1, v2;
+            gather3<real, T>(zc, cb, s, v0, v1, v2);
+            acc[0] += w * (double)v0;
+            acc[1] += w * (double)v1;
+            acc[2] += w * (double)v2;
         }
         const uint8_t ff = C.block_free[fb];
 #pragma unroll
@@ -2566,10 +2620,11 @@ __global__ void k_mg_load(const ClassDev *__restrict__ cp, const double *__restr
 // Coarsest level: its right-hand side (r) into band order, and the band
 // solution back into z.
 template <typename real, int T>
-__global__ void k_mg_band_in(const ClassDev *__restrict__ cp)
+__global__ void k_mg_band_in(const ClassDev *__restrict__ cp, int mode)
 {
     const ClassDev &C = *cp;
     const int smp = blockIdx.y;
+    if (!mg_act(C.mg_st[smp], mode)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= C.bn) return;
     const int tile = smp / T, s = smp % T;
@@ -2578,10 +2633,11 @@ __global__ void k_mg_band_in(const ClassDev *__restrict__ cp)
 }
 
 template <typename real, int T>
-__global__ void k_mg_band_out(const ClassDev *__restrict__ cp)
+__global__ void k_mg_band_out(const ClassDev *__restrict__ cp, int mode)
 {
     const ClassDev &C = *cp;
     const int smp = blockIdx.y;
+    if (!mg_act(C.mg_st[smp], mode)) return;
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= 3 * C.nb) return;
     const int tile = smp / T, s = smp % T;
@@ -2735,9 +2791,10 @@ hipError_t launch_residual64(hipStream_t s, int T, const ClassDev *d_cls, const 
                              int nblocks, double nu, int need_due, int ntiles)
 {
     if (nblocks <= 0) return hipSuccess;
-    // replacements: a few dozen due samples per check -> 2 waves per element
-    // (each wave stages the element's 4.6 KB Khat)
-    const dim3 grid(nblocks, need_due ? std::min(2u, tile_waves(nblocks, ntiles))
+    // replacements: up to 16 waves per element share the due list (each
+    // stages the element's 4.6 KB Khat; waves without a due sample exit
+    // before staging)
+    const dim3 grid(nblocks, need_due ? (unsigned)std::max(1, std::min(16, ntiles))
                                       : tile_waves(nblocks, ntiles));
     MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_residual64<TT>), grid, dim3(64), 0, s, d_cls, cl, nu,
                                          need_due));
@@ -2895,25 +2952,26 @@ hipError_t launch_band_rhs(hipStream_t s, int T, const ClassDev *cp, int nsamp, 
 }
 
 hipError_t launch_band_fwd(hipStream_t s, const ClassDev *cp, int nsamp, int k0, int nk, int r0,
-                           int m)
+                           int m, int mode)
 {
-    hipLaunchKernelGGL(k_band_fwd, dim3(1, nsamp), dim3(64), 0, s, cp, k0, nk, r0, m, 0);
+    hipLaunchKernelGGL(k_band_fwd, dim3(1, nsamp), dim3(64), 0, s, cp, k0, nk, r0, m, 0, mode);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || m <= 0) return e;
-    hipLaunchKernelGGL(k_band_fwd, dim3((m + 63) / 64, nsamp), dim3(64), 0, s, cp, k0, nk, r0, m, 1);
+    hipLaunchKernelGGL(k_band_fwd, dim3((m + 63) / 64, nsamp), dim3(64), 0, s, cp, k0, nk, r0, m, 1,
+                       mode);
     return hipGetLastError();
 }
 
 hipError_t launch_band_bwd(hipStream_t s, const ClassDev *cp, int nsamp, int k0, int nk, int r0,
-                           int m)
+                           int m, int mode)
 {
     if (m > 0) {
         hipLaunchKernelGGL(k_band_bwd, dim3((m + kBandChunkRows - 1) / kBandChunkRows, nsamp),
-                           dim3(64), 0, s, cp, k0, nk, r0, m, 0);
+                           dim3(64), 0, s, cp, k0, nk, r0, m, 0, mode);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_band_bwd, dim3(1, nsamp), dim3(64), 0, s, cp, k0, nk, r0, m, 1);
+    hipLaunchKernelGGL(k_band_bwd, dim3(1, nsamp), dim3(64), 0, s, cp, k0, nk, r0, m, 1, mode);
     return hipGetLastError();
 }
 
@@ -3099,14 +3157,16 @@ hipError_t launch_mg_gersh(hipStream_t s, bool mixed, int T, const ClassDev *d_c
 }
 
 hipError_t launch_mg_band_in(hipStream_t s, bool mixed, int T, const ClassDev *cp, int nsamp,
-                             int bn)
+                             int bn, int mode)
 {
     if (bn <= 0) return hipSuccess;
     const dim3 grid((bn + 255) / 256, nsamp);
     if (mixed) {
-        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_mg_band_in<float, TT>), grid, dim3(256), 0, s, cp));
+        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_mg_band_in<float, TT>), grid, dim3(256), 0, s, cp,
+                                             mode));
     } else {
-        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_mg_band_in<double, TT>), grid, dim3(256), 0, s, cp));
+        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_mg_band_in<double, TT>), grid, dim3(256), 0, s, cp,
+                                             mode));
     }
     return hipGetLastError();
 }
@@ -3127,14 +3187,16 @@ hipError_t launch_mg_load(hipStream_t s, bool mixed, int T, const ClassDev *cp, 
 }
 
 hipError_t launch_mg_band_out(hipStream_t s, bool mixed, int T, const ClassDev *cp, int nsamp,
-                              int nrows)
+                              int nrows, int mode)
 {
     if (nrows <= 0) return hipSuccess;
     const dim3 grid((nrows + 255) / 256, nsamp);
     if (mixed) {
-        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_mg_band_out<float, TT>), grid, dim3(256), 0, s, cp));
+        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_mg_band_out<float, TT>), grid, dim3(256), 0, s, cp,
+                                             mode));
     } else {
-        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_mg_band_out<double, TT>), grid, dim3(256), 0, s, cp));
+        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_mg_band_out<double, TT>), grid, dim3(256), 0, s, cp,
+                                             mode));
     }
     return hipGetLastError();
 }

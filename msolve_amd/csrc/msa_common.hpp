@@ -154,11 +154,15 @@ struct ClassDev {
     // table indices (-1: none).  root: the solved class whose sample state
     // drives the cycle; coarse: the next coarser level; fine: the finer one.
     int mg_root, mg_coarse, mg_fine, mg_depth;
+    const SampleState *mg_st;  // sample states of the root class
     double *mg_lam;       // [ntiles*T] upper bound of the spectrum of D^-1 K (Gershgorin)
-    const int *mg_p_ptr, *mg_p_col;      // prolongation rows of this class's blocks
-    const double *mg_p_w;
-    const int *mg_r_ptr, *mg_r_col;      // restriction rows of this (coarse) class's blocks
-    const double *mg_r_w;
+    // transfer rows in ELL layout (entry k of row i at k*nb + i, padding:
+    // column 0 with weight 0): prolongation rows of this class's blocks
+    // (columns: blocks of mg_coarse), restriction rows of this coarse
+    // class's blocks (columns: blocks of mg_fine)
+    int mg_p_width, mg_r_width;
+    const int *mg_p_col, *mg_r_col;
+    const float *mg_p_w, *mg_r_w;
     const int *mg_child;  // [n_elem*8] elements of the finer level in each element
 };
 
@@ -266,10 +270,11 @@ hipError_t launch_chol_update(hipStream_t s, const ClassDev *cp, int nsamp, int 
                               int m);
 hipError_t launch_band_rhs(hipStream_t s, int T, const ClassDev *cp, int nsamp, int bn,
                            int from_fs);
+// mode >= 0: coarsest multigrid level, only the samples mg_act serves
 hipError_t launch_band_fwd(hipStream_t s, const ClassDev *cp, int nsamp, int k0, int nk, int r0,
-                           int m);
+                           int m, int mode = -1);
 hipError_t launch_band_bwd(hipStream_t s, const ClassDev *cp, int nsamp, int k0, int nk, int r0,
-                           int m);
+                           int m, int mode = -1);
 hipError_t launch_band_out(hipStream_t s, int T, const ClassDev *cp, int nsamp, int nrows);
 // multigrid preconditioner: items of one level of every hierarchy
 enum MgStep { MG_PRE = 0, MG_RESID = 1, MG_POST1 = 2, MG_POST2 = 3 };
@@ -287,9 +292,9 @@ hipError_t launch_mg_coarsen_E(hipStream_t s, int T, const ClassDev *cp, const C
 hipError_t launch_mg_gersh(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
                            const int4 *items, int nitems);
 hipError_t launch_mg_band_in(hipStream_t s, bool mixed, int T, const ClassDev *cp, int nsamp,
-                             int bn);
+                             int bn, int mode);
 hipError_t launch_mg_band_out(hipStream_t s, bool mixed, int T, const ClassDev *cp, int nsamp,
-                              int nrows);
+                              int nrows, int mode);
 hipError_t launch_mg_load(hipStream_t s, bool mixed, int T, const ClassDev *cp, const double *src,
                           size_t nrows, int tile, int lane);
 hipError_t launch_band_scatter_any(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
