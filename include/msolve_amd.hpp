@@ -223,10 +223,16 @@ class SolverPCG {
     SolverPCG(Context &ctx, const Model &m, std::vector<double> E, double nu = 0.3)
         : ctx_(ctx), model_(m), E_(std::move(E)), nu_(nu) {}
 
+    // MSA_PRECOND_JACOBI (the reference's choice) or MSA_PRECOND_MULTIGRID;
+    // set before Initialize()
+    int Preconditioner = MSA_PRECOND_JACOBI;
+
     void Initialize()
     {
         mesh_.reset(new Mesh(ctx_, model_));
         batch_.reset(new Batch(ctx_, {mesh_.get()}, {1}, MSA_FP64, 1));
+        if (Preconditioner != MSA_PRECOND_JACOBI)
+            check(msa_batch_set_preconditioner(batch_->get(), Preconditioner, 0, 0.0));
         check(msa_batch_set_modulus(batch_->get(), 0, E_.data()));
         check(msa_batch_set_rhs(batch_->get(), 0, model_.Forces.data()));
         check(msa_batch_assemble(batch_->get(), nu_));
@@ -282,6 +288,8 @@ class MonteCarloAnalyzerWithStochasticMaterial {
           table_(std::move(table)), xi_(std::move(randomNumbers)), meanE_(meanE), cov_(cov),
           nu_(nu), precision_(precision) {}
 
+    int Preconditioner = MSA_PRECOND_JACOBI;  // set before Initialize()
+
     void Initialize()
     {
         std::vector<Mesh *> ptrs;
@@ -290,6 +298,8 @@ class MonteCarloAnalyzerWithStochasticMaterial {
             ptrs.push_back(meshes_.back().get());
         }
         batch_.reset(new Batch(ctx_, ptrs, samples_, precision_, 0));
+        if (Preconditioner != MSA_PRECOND_JACOBI)
+            check(msa_batch_set_preconditioner(batch_->get(), Preconditioner, 0, 0.0));
         size_t off = 0;
         for (size_t c = 0; c < models_.size(); c++) {
             check(msa_batch_set_kl(batch_->get(), (int)c, table_.M, table_.kmax, table_.w1d.data(),
@@ -373,10 +383,14 @@ class NewmarkDynamicAnalyzer {
           delta_(delta), timeStep_(timeStep), totalTime_(totalTime), density_(density),
           load_(load), precision_(precision), ra_(rayleighAlpha), rb_(rayleighBeta), nu_(nu) {}
 
+    int Preconditioner = MSA_PRECOND_JACOBI;  // set before BuildMatrices()
+
     void BuildMatrices()
     {
         mesh_.reset(new Mesh(ctx_, model_));
         batch_.reset(new Batch(ctx_, {mesh_.get()}, {samples_}, precision_, 0));
+        if (Preconditioner != MSA_PRECOND_JACOBI)
+            check(msa_batch_set_preconditioner(batch_->get(), Preconditioner, 0, 0.0));
         check(msa_batch_set_modulus(batch_->get(), 0, E_.data()));
         check(msa_batch_set_rhs(batch_->get(), 0, model_.Forces.data()));
         check(msa_batch_set_dynamics(batch_->get(), density_, ra_, rb_, alpha_, delta_, timeStep_));

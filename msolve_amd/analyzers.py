@@ -47,12 +47,15 @@ class ElementStructuralStiffnessProvider:
 
 
 class SolverPCG:
-    """FP64 Jacobi-PCG on one model (x0 = 0, stop at |r|/|f| < tolerance)."""
+    """FP64 PCG on one model (x0 = 0, stop at |r|/|f| < tolerance); Jacobi
+    preconditioner as the reference's SolverPCG, or preconditioner=
+    MSA_PRECOND_MULTIGRID for the geometric V-cycle."""
 
-    def __init__(self, ctx: Context, model, E, nu=0.3, tile=1):
+    def __init__(self, ctx: Context, model, E, nu=0.3, tile=1, preconditioner=L.MSA_PRECOND_JACOBI):
         self.ctx, self.model, self.nu = ctx, model, nu
         self.E = np.ascontiguousarray(E, np.float64).ravel()
         self.tile = tile
+        self.preconditioner = preconditioner
         self.mesh = None
         self.batch = None
         self.VectorX = None
@@ -63,6 +66,8 @@ class SolverPCG:
         m = self.model
         self.mesh = Mesh(self.ctx, m.xyz, m.conn, m.dof, m.n_free)
         self.batch = Batch(self.ctx, [self.mesh], [1], L.MSA_FP64, self.tile)
+        if self.preconditioner != L.MSA_PRECOND_JACOBI:
+            self.batch.set_preconditioner(self.preconditioner)
         self.batch.set_modulus(0, self.E)
         self.batch.set_rhs(0, m.f)
         self.batch.assemble(self.nu)
@@ -126,8 +131,10 @@ class MonteCarloAnalyzerWithStochasticMaterial:
 
     def __init__(self, ctx: Context, models, samples, table, expansionOrder: int,
                  meanE=2.1e5, cov=0.5, nu=0.3, precision=L.MSA_FP64, tile=0, seed=20261202,
-                 randomsReadFileName: str | None = None, simulationStartFrom: int = 0):
+                 randomsReadFileName: str | None = None, simulationStartFrom: int = 0,
+                 preconditioner=L.MSA_PRECOND_JACOBI):
         self.ctx = ctx
+        self.preconditioner = preconditioner
         self.models = list(models)
         self.samples = [int(s) for s in samples]
         self.table = table
@@ -150,6 +157,8 @@ class MonteCarloAnalyzerWithStochasticMaterial:
             self.randomNumbers = standard_normal(self.seed, total, M)
         self.meshes = [Mesh(self.ctx, m.xyz, m.conn, m.dof, m.n_free) for m in self.models]
         self.batch = Batch(self.ctx, self.meshes, self.samples, self.precision, self.tile)
+        if self.preconditioner != L.MSA_PRECOND_JACOBI:
+            self.batch.set_preconditioner(self.preconditioner)
         off = 0
         for c, m in enumerate(self.models):
             self.batch.set_kl(c, self.table, m.domain, self.meanE, self.cov)
@@ -205,8 +214,10 @@ class NewmarkDynamicAnalyzer:
     def __init__(self, ctx: Context, models, E_per_class, alpha=0.25, delta=0.5,
                  timeStep=1e-4, totalTime=0.05, density=7.85e-9, rayleighAlpha=0.0,
                  rayleighBeta=0.0, period=0.05, magnitude=1.0, accelerationDOF=2, nu=0.3,
-                 precision=L.MSA_MIXED, tile=0, static_loads=True):
+                 precision=L.MSA_MIXED, tile=0, static_loads=True,
+                 preconditioner=L.MSA_PRECOND_JACOBI):
         self.ctx = ctx
+        self.preconditioner = preconditioner
         self.models = list(models)
         self.E = [np.atleast_2d(np.asarray(E, np.float64)) for E in E_per_class]
         self.alpha, self.delta, self.timeStep, self.totalTime = alpha, delta, timeStep, totalTime
@@ -224,6 +235,8 @@ class NewmarkDynamicAnalyzer:
         self.meshes = [Mesh(self.ctx, m.xyz, m.conn, m.dof, m.n_free) for m in self.models]
         self.batch = Batch(self.ctx, self.meshes, [E.shape[0] for E in self.E], self.precision,
                            self.tile)
+        if self.preconditioner != L.MSA_PRECOND_JACOBI:
+            self.batch.set_preconditioner(self.preconditioner)
         for c, m in enumerate(self.models):
             self.batch.set_modulus(c, self.E[c])
             if self.static_loads:

@@ -1,5 +1,5 @@
 // Exercises the C++ host layer (include/msolve_amd.hpp) end to end on the
-// GPU: a plain SolverPCG solve and a two-class Monte Carlo batch, printing
+// GPU: a plain SolverPCG solve (Jacobi and multigrid) and a two-class Monte Carlo batch, printing
 // the results as JSON for tests/test_gpu_host_cpp.py to compare with the
 // oracle.  Usage: host_api_check <kl_table.txt> (M, then rows of w a per
 // axis, evpPos, eigs) ; xi and moduli are derived deterministically.
@@ -81,6 +81,15 @@ int main(int argc, char **argv)
         nm.BuildMatrices();
         nm.Solve(1e-12, 20000, 8);
         dump("newmark_hist", nm.History(0));
+        // the same SolverPCG surface with the multigrid preconditioner (8^3)
+        msolve::Model m8 = msolve::Hexa8MeshGenerator::Cantilever(8);
+        std::vector<double> E8(m8.ElementCount(), 2.1e5);
+        msolve::SolverPCG mg(ctx, m8, E8, 0.3);
+        mg.Preconditioner = MSA_PRECOND_MULTIGRID;
+        mg.Initialize();
+        mg.Solve(20000, 1e-12);
+        std::printf("\"mg_products\": %d, \"mg_status\": %d,\n", mg.Products, mg.Status);
+        dump("u_mg8", mg.VectorX);
         std::vector<double> ke0(ke.begin(), ke.begin() + 300);
         dump("ke0", ke0, true);
         std::printf("}\n");

@@ -56,5 +56,8 @@ def test_cpp_host_layer(tmp_path):
                                        acc_period=0.05, acc_tlen=30 * 2e-3, probe=c4.n - 1)
     assert len(out["newmark_hist"]) == nsteps
     assert rel(out["newmark_hist"], h_ref) < 1e-9
+    c8 = O.Cantilever(8)
+    assert out["mg_status"] == 0 and out["mg_products"] < 25
+    assert rel(out["u_mg8"], direct(c8, np.full(c8.n_elem, 2.1e5))) < 1e-10
     ke, _ = O.ke_mesh(c4.conn[:8], c4.xyz, np.ones(1), 0.3)
     assert np.abs(np.array(out["ke0"]) - ke[0]).max() < 1e-13 * np.abs(ke[0]).max()

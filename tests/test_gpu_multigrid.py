@@ -205,3 +205,18 @@ def test_newmark_with_multigrid_matches_skyline_oracle(ctx, precision, bar):
         assert its < 30 * nsteps
     b.close()
     mesh.close()
+
+
+def test_solver_pcg_surface_with_multigrid(ctx):
+    """analyzers.SolverPCG(preconditioner=MSA_PRECOND_MULTIGRID) on the
+    12^3 cantilever with one modulus: the direct solution in < 25 products."""
+    print("\n[case] SolverPCG multigrid 12^3", flush=True)
+    model = msa.cantilever(12)
+    E = np.full(model.n_elem, 2.1e5)
+    solver = msa.SolverPCG(ctx, model, E, preconditioner=msa.MSA_PRECOND_MULTIGRID)
+    u = solver.Solve(20000, 1e-12)
+    assert solver.Status == msa.MSA_OK and solver.Iterations < 25
+    levels, _ = oracle_levels(model, E)
+    assert rel(u, spl.spsolve(levels[0].K.tocsc(), model.f)) < 1e-10
+    solver.batch.close()
+    solver.mesh.close()
