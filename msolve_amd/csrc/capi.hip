@@ -68,6 +68,8 @@ struct msa_batch {
     ClassDev *d_cls = nullptr;
     std::vector<int4> items;
     int4 *d_items = nullptr;
+    int4 *d_gitems = nullptr;  // one-sample tiles: (class, group of kAsmGroup tiles, chunk)
+    int n_gitems = 0;
     std::vector<int2> tiles;  // (class, tile) pairs, one reduction workgroup each
     int2 *d_tiles = nullptr;
     int *d_counts = nullptr;  // [0] PCG active samples, [1] refinement active samples
@@ -111,6 +113,8 @@ struct msa_batch {
     std::vector<int> mg_nrestr;
     int4 *mg_lv_items = nullptr;        // every coarse level
     int mg_nlv_items = 0;
+    int4 *mg_lv_gitems = nullptr;       // the same in sample groups (one-sample tiles)
+    int mg_nlv_gitems = 0;
     int4 *mg_co_items = nullptr;        // the coarsest levels
     int mg_nco_items = 0;
     std::vector<int> mg_coarsest;       // class table indices of the coarsest levels
@@ -678,6 +682,24 @@ void class_items(const ClassHost &C, int ci, std::vector<int4> &out)
         for (int k = 0; k < C.nchunks; k++) out.push_back(make_int4(ci, t, k, 0));
 }
 
+// Work items of the grouped gather assembly: tiles in groups of kAsmGroup.
+void class_group_items(const ClassHost &C, int ci, std::vector<int4> &out)
+{
+    for (int g = 0; g < (C.ntiles + kAsmGroup - 1) / kAsmGroup; g++)
+        for (int k = 0; k < C.nchunks; k++) out.push_back(make_int4(ci, g, k, 0));
+}
+
+// Gather assembly of the classes of `items` (one-sample tiles: grouped).
+int assemble_items(msa_batch *b, const int4 *items, int nitems, const int4 *gitems, int ngitems)
+{
+    hipStream_t s = b->ctx->stream;
+    const bool mx = is_mixed(b);
+    return timed(b, KK_ASSEMBLE, [&] {
+        return b->T == 1 ? launch_assemble_gather_g(s, mx, b->d_cls, gitems, ngitems)
+                         : launch_assemble_gather(s, mx, b->T, b->d_cls, items, nitems);
+    });
+}
+
 // Coarse hierarchies of every class whose mesh is a structured grid that
 // halves at least once and whose coarsest level stays small; the other
 // classes keep the Jacobi preconditioner.
@@ -768,6 +790,10 @@ int mg_build(msa_batch *b, int max_levels)
         if (d > 0) MSA_TRY(items_to_device(b, re, &b->mg_restr[d], &b->mg_nrestr[d]));
     }
     MSA_TRY(items_to_device(b, all, &b->mg_lv_items, &b->mg_nlv_items));
+    std::vector<int4> gall;
+    for (size_t l = 0; l < b->lv.size(); l++)
+        class_group_items(b->lv[l], (int)(b->cls.size() + l), gall);
+    MSA_TRY(items_to_device(b, gall, &b->mg_lv_gitems, &b->mg_nlv_gitems));
     MSA_TRY(items_to_device(b, co, &b->mg_co_items, &b->mg_nco_items));
     MSA_TRY(upload_class_table(b));
     b->mg_ready = false;
@@ -823,9 +849,7 @@ int mg_values(msa_batch *b)
         L.kc_nu = b->nu;
         L.kc_mass = L.dev.dyn;
     }
-    MSA_TRY(timed(b, KK_ASSEMBLE, [&] {
-        return launch_assemble_gather(s, mx, T, b->d_cls, b->mg_lv_items, b->mg_nlv_items);
-    }));
+    MSA_TRY(assemble_items(b, b->mg_lv_items, b->mg_nlv_items, b->mg_lv_gitems, b->mg_nlv_gitems));
     MSA_TRY(timed(b, KK_OTHER, [&] {
         return launch_dinv(s, mx, T, b->d_cls, b->mg_lv_items, b->mg_nlv_items);
     }));
@@ -1295,6 +1319,15 @@ int msa_batch_create(msa_ctx *ctx, int n_classes, msa_mesh *const *meshes, const
     if (rc == MSA_OK) rc = dalloc(b, &p, sizeof(int2) * b->tiles.size());
     b->d_tiles = (int2 *)p;
     if (rc != MSA_OK) return fail(rc);
+    {
+        std::vector<int4> g;
+        for (int c = 0; c < n_classes; c++) class_group_items(b->cls[c], c, g);
+        const int4 *gp = nullptr;
+        rc = upload_vec(b, g, &gp);
+        if (rc != MSA_OK) return fail(rc);
+        b->d_gitems = (int4 *)gp;
+        b->n_gitems = (int)g.size();
+    }
     if (hipMemcpy(b->d_items, b->items.data(), sizeof(int4) * b->items.size(),
                   hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(b->d_tiles, b->tiles.data(), sizeof(int2) * b->tiles.size(),
@@ -1463,9 +1496,7 @@ int msa_batch_assemble(msa_batch *b, double nu)
     }
     if (!mx || b->integration == MSA_FP64) {
         // every stored value summed in FP64 and written once
-        MSA_TRY(timed(b, KK_ASSEMBLE, [&] {
-            return launch_assemble_gather(s, mx, b->T, b->d_cls, b->d_items, (int)b->items.size());
-        }));
+        MSA_TRY(assemble_items(b, b->d_items, (int)b->items.size(), b->d_gitems, b->n_gitems));
     } else {
         // FP32 element integration: coloured FP32 scatter-add
         for (auto &C : b->cls) {

@@ -663,6 +663,87 @@ __global__ __launch_bounds__(256) void k_assemble_gather(const ClassDev *__restr
 
 // ------------------------------------------------------------ PCG kernels
 
+// One-sample tiles: the gather assembly of G consecutive samples in one
+// pass, so the gather lists and the cached Khat entries of a stored block
+// are read once per G samples instead of once per sample.  Each sample's
+// sum runs in the same (ascending element) order as k_assemble_gather, so
+// the values are bitwise the same.  Items: (class, tile group, chunk).
+template <typename real, int G>
+__global__ __launch_bounds__(256) void k_assemble_gather_g(const ClassDev *__restrict__ cls,
+                                                           const int4 *__restrict__ items,
+                                                           int nitems)
+{
+    constexpr int R = 64;
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const int tile0 = it.y * G, chunk = it.z;
+    const int nt = min(G, C.ntiles - tile0);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rr = lane;
+    const MSA_GLOBAL double *__restrict__ E = gbl((const double *)C.E + (size_t)tile0 * C.n_elem);
+    MSA_GLOBAL real *__restrict__ val = gbl((real *)C.val + (size_t)tile0 * C.tile_vals);
+    const MSA_GLOBAL int *__restrict__ gmap = gbl(C.gmap);
+    const MSA_GLOBAL int *__restrict__ cidx = gbl(C.cidx);
+    const MSA_GLOBAL double *__restrict__ kc = gbl((const double *)C.kc);
+    const MSA_GLOBAL double *__restrict__ mc = gbl((const double *)C.mc);
+    const double cK = C.dyn ? C.cK : 1.0, cM = C.dyn ? C.cM : 0.0;
+    const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+    for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+        const int br = slice * R + rr;
+        if (br >= C.nb) continue;
+        const int base = C.slice_ptr[slice], W = C.slice_ptr[slice + 1] - base;
+        const uint8_t fr = C.block_free[br];
+        for (int j = 0; j < W; j++) {
+            const int slot = base + j;
+            const int col = cidx[(size_t)slot * R + rr];
+            double acc[G][9];
+#pragma unroll
+            for (int g = 0; g < G; g++)
+#pragma unroll
+                for (int q = 0; q < 9; q++) acc[g][q] = 0.0;
+            if (col >= 0) {
+                for (int gi = 0; gi < 8; gi++) {
+                    const int code = gmap[((size_t)slot * 8 + gi) * R + rr];
+                    if (code < 0) break;
+                    const int e = code >> 6, ab = code & 63;
+                    const MSA_GLOBAL double *kp = kc + (size_t)e * 576 + ab * 9;
+                    double k[9];
+#pragma unroll
+                    for (int q = 0; q < 9; q++) k[q] = kp[q];
+                    const double m = cM != 0.0 ? cM * mc[(size_t)e * 64 + ab] : 0.0;
+#pragma unroll
+                    for (int g = 0; g < G; g++) {
+                        if (g >= nt) break;
+                        const double Es = E[(size_t)g * C.n_elem + e] * cK;
+#pragma unroll
+                        for (int q = 0; q < 9; q++) acc[g][q] += Es * k[q];
+                        if (cM != 0.0) {
+                            acc[g][0] += m;
+                            acc[g][4] += m;
+                            acc[g][8] += m;
+                        }
+                    }
+                }
+                const uint8_t fc = C.block_free[col];
+#pragma unroll
+                for (int g = 0; g < G; g++)
+#pragma unroll
+                    for (int q = 0; q < 9; q++)
+                        if (!((fr >> (q / 3)) & 1) || !((fc >> (q % 3)) & 1)) acc[g][q] = 0.0;
+            }
+#pragma unroll
+            for (int g = 0; g < G; g++) {
+                if (g >= nt) break;
+#pragma unroll
+                for (int q = 0; q < 9; q++)
+                    val[(size_t)g * C.tile_vals + ((size_t)slot * 9 + q) * R + rr] = (real)acc[g][q];
+            }
+        }
+    }
+}
+
 template <typename real, int T>
 __global__ __launch_bounds__(256) void k_dinv(const ClassDev *__restrict__ cls,
                                               const int4 *__restrict__ items, int nitems)
@@ -2849,6 +2930,20 @@ hipError_t launch_assemble_gather(hipStream_t s, bool mixed, int T, const ClassD
                                   const int4 *items, int nitems)
 {
     MSA_PCG_LAUNCH(k_assemble_gather, d_cls, items, nitems);
+}
+
+hipError_t launch_assemble_gather_g(hipStream_t s, bool mixed, const ClassDev *d_cls,
+                                    const int4 *gitems, int nitems)
+{
+    if (nitems <= 0) return hipSuccess;
+    const dim3 grid(xcd_grid(nitems)), block(256);
+    if (mixed)
+        hipLaunchKernelGGL((k_assemble_gather_g<float, kAsmGroup>), grid, block, 0, s, d_cls, gitems,
+                           nitems);
+    else
+        hipLaunchKernelGGL((k_assemble_gather_g<double, kAsmGroup>), grid, block, 0, s, d_cls, gitems,
+                           nitems);
+    return hipGetLastError();
 }
 
 #define MSA_T_LAUNCH(KERNEL, ...)                                                          \

@@ -234,6 +234,11 @@ hipError_t launch_khat_cache(hipStream_t s, const ClassDev *d_cls_c, int n_elem,
                              int with_mass);
 hipError_t launch_assemble_gather(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
                                   const int4 *items, int nitems);
+// one-sample tiles: gather assembly of kAsmGroup samples per work item
+// (class, tile group, chunk)
+constexpr int kAsmGroup = 4;
+hipError_t launch_assemble_gather_g(hipStream_t s, bool mixed, const ClassDev *d_cls,
+                                    const int4 *gitems, int nitems);
 hipError_t launch_residual64(hipStream_t s, int T, const ClassDev *d_cls, const ColourLaunch &cl,
                              int nblocks, double nu, int need_due, int ntiles);
 hipError_t launch_dinv(hipStream_t s, bool mixed, int T, const ClassDev *d_cls, const int4 *items,
