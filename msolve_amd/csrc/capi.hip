@@ -109,6 +109,8 @@ struct msa_batch {
     int mg_depth = 0;                   // deepest level of any hierarchy
     std::vector<int4 *> mg_smooth;      // per depth: items of the classes that smooth there
     std::vector<int> mg_nsmooth;
+    std::vector<int4 *> mg_sm_a, mg_sm_f;  // the same split: stored-value / matrix-free products
+    std::vector<int> mg_nsm_a, mg_nsm_f;
     std::vector<int4 *> mg_restr;       // per depth >= 1: items of every level there
     std::vector<int> mg_nrestr;
     int4 *mg_lv_items = nullptr;        // every coarse level
@@ -700,6 +702,28 @@ int assemble_items(msa_batch *b, const int4 *items, int nitems, const int4 *gite
     });
 }
 
+// Matrix-free smoothing products for a class on a uniform structured grid
+// (mixed precision, one-sample tiles); other classes keep the stored values.
+int mf_setup(msa_batch *b, ClassHost &C)
+{
+    if (!is_mixed(b) || b->T != 1 || C.dev.mf) return MSA_OK;
+    GridInfo g;
+    if (grid_info(C.mesh->h, g) != MSA_OK) return MSA_OK;
+    ClassDev &D = C.dev;
+    MSA_TRY(upload_vec(b, g.gblock, &D.mf_gblock));
+    MSA_TRY(upload_vec(b, g.cell_elem, &D.mf_cell));
+    MSA_TRY(upload_vec(b, g.bpos, &D.mf_bpos));
+    void *p = nullptr;
+    MSA_TRY(dalloc(b, &p, 640 * sizeof(float)));
+    D.mf_k = (float *)p;
+    D.mf_nx = g.nx;
+    D.mf_ny = g.ny;
+    D.mf_nz = g.nz;
+    for (int o = 0; o < 8; o++) D.mf_corner[o] = g.corner_node[o];
+    D.mf = 1;
+    return MSA_OK;
+}
+
 // Coarse hierarchies of every class whose mesh is a structured grid that
 // halves at least once and whose coarsest level stays small; the other
 // classes keep the Jacobi preconditioner.
@@ -725,6 +749,7 @@ int mg_build(msa_batch *b, int max_levels)
         if (meshes.empty() || meshes.back().n_free > kMgCoarsestMax) continue;
         b->mg_nlev[c] = (int)meshes.size();
         depth_classes[0].push_back(c);
+        MSA_TRY(mf_setup(b, b->cls[c]));
         int parent = c;
         for (size_t l = 0; l < meshes.size(); l++) {
             msa_mesh *m = new msa_mesh();
@@ -737,6 +762,7 @@ int mg_build(msa_batch *b, int max_levels)
             ClassHost &L = b->lv.back();
             MSA_TRY(setup_class(b, L, m, b->cls[c].nsamp));
             L.has_E = true;
+            MSA_TRY(mf_setup(b, L));
             ClassDev &D = L.dev;
             D.mg_root = c;
             D.mg_st = b->cls[c].dev.st;
@@ -770,23 +796,33 @@ int mg_build(msa_batch *b, int max_levels)
     b->mg_depth = (int)depth_classes.size() - 1;
     b->mg_smooth.assign(b->mg_depth, nullptr);
     b->mg_nsmooth.assign(b->mg_depth, 0);
+    b->mg_sm_a.assign(b->mg_depth, nullptr);
+    b->mg_nsm_a.assign(b->mg_depth, 0);
+    b->mg_sm_f.assign(b->mg_depth, nullptr);
+    b->mg_nsm_f.assign(b->mg_depth, 0);
     b->mg_restr.assign(b->mg_depth + 1, nullptr);
     b->mg_nrestr.assign(b->mg_depth + 1, 0);
     std::vector<int4> all, co;
     for (int d = 0; d <= b->mg_depth; d++) {
-        std::vector<int4> sm, re;
+        std::vector<int4> sm, sma, smf, re;
         for (int ci : depth_classes[d]) {
             ClassHost &C = tab(b, ci);
             if (d > 0) {
                 class_items(C, ci, re);
                 class_items(C, ci, all);
             }
-            if (C.dev.mg_coarse >= 0)
+            if (C.dev.mg_coarse >= 0) {
                 class_items(C, ci, sm);
-            else if (d > 0)
+                class_items(C, ci, C.dev.mf ? smf : sma);
+            } else if (d > 0) {
                 class_items(C, ci, co);
+            }
         }
-        if (d < b->mg_depth) MSA_TRY(items_to_device(b, sm, &b->mg_smooth[d], &b->mg_nsmooth[d]));
+        if (d < b->mg_depth) {
+            MSA_TRY(items_to_device(b, sm, &b->mg_smooth[d], &b->mg_nsmooth[d]));
+            MSA_TRY(items_to_device(b, sma, &b->mg_sm_a[d], &b->mg_nsm_a[d]));
+            MSA_TRY(items_to_device(b, smf, &b->mg_sm_f[d], &b->mg_nsm_f[d]));
+        }
         if (d > 0) MSA_TRY(items_to_device(b, re, &b->mg_restr[d], &b->mg_nrestr[d]));
     }
     MSA_TRY(items_to_device(b, all, &b->mg_lv_items, &b->mg_nlv_items));
@@ -850,6 +886,9 @@ int mg_values(msa_batch *b)
         L.kc_mass = L.dev.dyn;
     }
     MSA_TRY(assemble_items(b, b->mg_lv_items, b->mg_nlv_items, b->mg_lv_gitems, b->mg_nlv_gitems));
+    for (int ci = 0; ci < nc + (int)b->lv.size(); ci++)
+        if (tab(b, ci).dev.mf)
+            MSA_TRY(timed(b, KK_OTHER, [&] { return launch_mf_khat(s, b->d_cls + ci); }));
     MSA_TRY(timed(b, KK_OTHER, [&] {
         return launch_dinv(s, mx, T, b->d_cls, b->mg_lv_items, b->mg_nlv_items);
     }));
@@ -911,10 +950,17 @@ int mg_vcycle(msa_batch *b, int pbuf, int mode)
     const double ratio = b->mg_ratio;
     // timing kind KK_MG: the finest level's smoothing products only
     auto smooth = [&](int d, int step) {
-        return timed(b, d == 0 ? KK_MG : KK_OTHER, [&] {
-            return launch_mg_smooth(s, mx, T, b->d_cls, b->mg_smooth[d], b->mg_nsmooth[d], step,
-                                    pbuf, mode, ratio);
-        });
+        if (b->mg_nsm_a[d] > 0)
+            MSA_TRY(timed(b, d == 0 ? KK_MG : KK_OTHER, [&] {
+                return launch_mg_smooth(s, mx, T, b->d_cls, b->mg_sm_a[d], b->mg_nsm_a[d], step,
+                                        pbuf, mode, ratio, false);
+            }));
+        if (b->mg_nsm_f[d] > 0)
+            MSA_TRY(timed(b, d == 0 ? KK_MG : KK_OTHER, [&] {
+                return launch_mg_smooth(s, mx, T, b->d_cls, b->mg_sm_f[d], b->mg_nsm_f[d], step,
+                                        pbuf, mode, ratio, true);
+            }));
+        return (int)MSA_OK;
     };
     for (int d = 0; d < b->mg_depth; d++) {
         MSA_TRY(smooth(d, MG_PRE));

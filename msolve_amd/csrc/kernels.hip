@@ -2443,13 +2443,106 @@ __device__ __forceinline__ void row_product(const MSA_GLOBAL real *val, const MS
     }
 }
 
+// y = K v for block row br of a uniform structured grid class, from the
+// moduli of the (up to) 8 cells around the node and the shared unit element
+// matrix (FP32): y = sum_cells cK E_c Khat[a][b] v_b (+ cM Mhat[a][b] v_b on
+// the component itself), a = the node's corner in the cell, b over the
+// cell's corners.  The 27 neighbouring blocks are gathered once.
+__device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL float *v,
+                                           const double *Es, int br, float &y0, float &y1,
+                                           float &y2)
+{
+    const int g = gbl(C.mf_bpos)[br];
+    const int i = g & 1023, j = (g >> 10) & 1023, k = g >> 20;
+    const int nx = __builtin_amdgcn_readfirstlane(C.mf_nx);
+    const int ny = __builtin_amdgcn_readfirstlane(C.mf_ny);
+    const int nz = __builtin_amdgcn_readfirstlane(C.mf_nz);
+    const MSA_GLOBAL int *gblock = gbl(C.mf_gblock);
+    const MSA_GLOBAL int *cell = gbl(C.mf_cell);
+    const MSA_GLOBAL double *E = gbl(Es);
+    float xv[27][3];
+#pragma unroll
+    for (int dz = 0; dz < 3; dz++)
+#pragma unroll
+        for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+            for (int dx = 0; dx < 3; dx++) {
+                const int ii = i + dx - 1, jj = j + dy - 1, kk = k + dz - 1;
+                const bool in = ii >= 0 && ii < nx && jj >= 0 && jj < ny && kk >= 0 && kk < nz;
+                const int nb = in ? gblock[ii + nx * (jj + ny * kk)] : -1;
+                float *x = xv[(dz * 3 + dy) * 3 + dx];
+                if (nb >= 0) {
+                    const vec3<float> c = ld3(v, nb);
+                    x[0] = c.x;
+                    x[1] = c.y;
+                    x[2] = c.z;
+                } else {
+                    x[0] = x[1] = x[2] = 0.f;
+                }
+            }
+    const float cK = C.dyn ? (float)C.cK : 1.f, cM = C.dyn ? (float)C.cM : 0.f;
+    // the element matrix is the same for every lane: constant-address loads
+    // with wave-uniform indices (scalar loads, SGPR operands of the FMAs)
+    using cfloat = __attribute__((address_space(4))) const float;
+    cfloat *kh = (cfloat *)C.mf_k, *mh = (cfloat *)C.mf_k + 576;
+    int corner[8];
+#pragma unroll
+    for (int o = 0; o < 8; o++) corner[o] = __builtin_amdgcn_readfirstlane(C.mf_corner[o]);
+    float acc[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int cz = 0; cz < 2; cz++)
+#pragma unroll
+        for (int cy = 0; cy < 2; cy++)
+#pragma unroll
+            for (int cx = 0; cx < 2; cx++) {
+                const int ci = i - 1 + cx, cj = j - 1 + cy, ck = k - 1 + cz;
+                if (ci < 0 || ci >= nx - 1 || cj < 0 || cj >= ny - 1 || ck < 0 || ck >= nz - 1) continue;
+                const float Ec = (float)E[cell[ci + (nx - 1) * (cj + (ny - 1) * ck)]] * cK;
+                const int a = corner[(1 - cx) + 2 * (1 - cy) + 4 * (1 - cz)];
+                float t[3] = {0.f, 0.f, 0.f}, m[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+                for (int oz = 0; oz < 2; oz++)
+#pragma unroll
+                    for (int oy = 0; oy < 2; oy++)
+#pragma unroll
+                        for (int ox = 0; ox < 2; ox++) {
+                            const int b = corner[ox + 2 * oy + 4 * oz];
+                            cfloat *kab = kh + (a * 8 + b) * 9;
+                            const float *xb = xv[((cz + oz) * 3 + (cy + oy)) * 3 + (cx + ox)];
+#pragma unroll
+                            for (int p = 0; p < 3; p++)
+                                t[p] += kab[3 * p] * xb[0] + kab[3 * p + 1] * xb[1] +
+                                        kab[3 * p + 2] * xb[2];
+                            if (cM != 0.f) {
+                                const float mab = mh[a * 8 + b];
+#pragma unroll
+                                for (int p = 0; p < 3; p++) m[p] += mab * xb[p];
+                            }
+                        }
+#pragma unroll
+                for (int p = 0; p < 3; p++) acc[p] += Ec * t[p] + cM * m[p];
+            }
+    y0 = acc[0];
+    y1 = acc[1];
+    y2 = acc[2];
+}
+
+// FP32 copy of element 0's unit Khat (and Mhat) for the matrix-free products.
+__global__ void k_mf_khat(const ClassDev *__restrict__ cp)
+{
+    const ClassDev &C = *cp;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 576) C.mf_k[i] = (float)C.kc[i];
+    else if (i < 640) C.mf_k[i] = C.mc ? (float)C.mc[i - 576] : 0.f;
+}
+
 // One smoothing-level product with its row update (STEP, see MgStep):
 //   MG_PRE   A = (1 + c1) x1 + c2 D^-1 (b - K x1),  x1 = z / theta
 //   MG_RESID B = b - K A
 //   MG_POST1 B = A + D^-1 (b - K A) / theta
 //   MG_POST2 z = B + c1 (B - A) + c2 D^-1 (b - K B); level 0 also writes
 //            the partial sums of r.z (slot 0 of the PCG partials)
-template <typename real, int T, int STEP>
+template <typename real, int T, int STEP, bool MF = false>
 __global__ __launch_bounds__(256) void k_mg_smooth(const ClassDev *__restrict__ cls,
                                                    const int4 *__restrict__ items, int nitems,
                                                    int pbuf, int mode, double ratio)
@@ -2487,7 +2580,11 @@ __global__ __launch_bounds__(256) void k_mg_smooth(const ClassDev *__restrict__ 
             const int W = sptr[slice + 1] - base;
             if (!act || br >= C.nb) continue;
             real y[3];
-            row_product<real, T>(val, cidx, src, base, W, rr, s, y[0], y[1], y[2]);
+            if constexpr (MF)
+                mf_product(C, (const MSA_GLOBAL float *)src, C.E + (size_t)tile * C.n_elem, br,
+                           y[0], y[1], y[2]);
+            else
+                row_product<real, T>(val, cidx, src, base, W, rr, s, y[0], y[1], y[2]);
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 const size_t o = ((size_t)br * 3 + d) * T + s;
@@ -2872,10 +2969,10 @@ hipError_t launch_residual64(hipStream_t s, int T, const ClassDev *d_cls, const 
                              int nblocks, double nu, int need_due, int ntiles)
 {
     if (nblocks <= 0) return hipSuccess;
-    // replacements: up to 16 waves per element share the due list (each
+    // replacements: up to 4 waves per element share the due list (each
     // stages the element's 4.6 KB Khat; waves without a due sample exit
     // before staging)
-    const dim3 grid(nblocks, need_due ? (unsigned)std::max(1, std::min(16, ntiles))
+    const dim3 grid(nblocks, need_due ? (unsigned)std::max(1, std::min(4, ntiles))
                                       : tile_waves(nblocks, ntiles));
     MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_residual64<TT>), grid, dim3(64), 0, s, d_cls, cl, nu,
                                          need_due));
@@ -3196,11 +3293,16 @@ hipError_t launch_extract_values(hipStream_t s, bool mixed, int T, const void *v
 
 template <int STEP>
 static hipError_t mg_smooth_step(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
-                                 const int4 *items, int nitems, int pbuf, int mode, double ratio)
+                                 const int4 *items, int nitems, int pbuf, int mode, double ratio,
+                                 bool mf)
 {
     if (nitems <= 0) return hipSuccess;
     const dim3 grid(xcd_grid(nitems)), block(256);
-    if (mixed) {
+    if (mf) {
+        if (!mixed || T != 1) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_mg_smooth<float, 1, STEP, true>), grid, block, 0, s, d_cls, items,
+                           nitems, pbuf, mode, ratio);
+    } else if (mixed) {
         MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_mg_smooth<float, TT, STEP>), grid, block, 0, s,
                                              d_cls, items, nitems, pbuf, mode, ratio));
     } else {
@@ -3212,13 +3314,13 @@ static hipError_t mg_smooth_step(hipStream_t s, bool mixed, int T, const ClassDe
 
 hipError_t launch_mg_smooth(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
                             const int4 *items, int nitems, int step, int pbuf, int mode,
-                            double ratio)
+                            double ratio, bool mf)
 {
     switch (step) {
-    case MG_PRE: return mg_smooth_step<MG_PRE>(s, mixed, T, d_cls, items, nitems, pbuf, mode, ratio);
-    case MG_RESID: return mg_smooth_step<MG_RESID>(s, mixed, T, d_cls, items, nitems, pbuf, mode, ratio);
-    case MG_POST1: return mg_smooth_step<MG_POST1>(s, mixed, T, d_cls, items, nitems, pbuf, mode, ratio);
-    case MG_POST2: return mg_smooth_step<MG_POST2>(s, mixed, T, d_cls, items, nitems, pbuf, mode, ratio);
+    case MG_PRE: return mg_smooth_step<MG_PRE>(s, mixed, T, d_cls, items, nitems, pbuf, mode, ratio, mf);
+    case MG_RESID: return mg_smooth_step<MG_RESID>(s, mixed, T, d_cls, items, nitems, pbuf, mode, ratio, mf);
+    case MG_POST1: return mg_smooth_step<MG_POST1>(s, mixed, T, d_cls, items, nitems, pbuf, mode, ratio, mf);
+    case MG_POST2: return mg_smooth_step<MG_POST2>(s, mixed, T, d_cls, items, nitems, pbuf, mode, ratio, mf);
     default: return hipErrorInvalidValue;
     }
 }
@@ -3263,6 +3365,12 @@ hipError_t launch_mg_band_in(hipStream_t s, bool mixed, int T, const ClassDev *c
         MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_mg_band_in<double, TT>), grid, dim3(256), 0, s, cp,
                                              mode));
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_mf_khat(hipStream_t s, const ClassDev *cp)
+{
+    hipLaunchKernelGGL(k_mf_khat, dim3(3), dim3(256), 0, s, cp);
     return hipGetLastError();
 }
 

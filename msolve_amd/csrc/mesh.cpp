@@ -437,3 +437,71 @@ int mg_coarsen(const MeshHost &f, MeshHost &c, MgTransfer &t)
 }
 
 }  // namespace msa
+
+namespace msa {
+
+int grid_info(const MeshHost &m, GridInfo &gi_out)
+{
+    std::vector<double> gx[3];
+    std::vector<int> gi[3];
+    for (int d = 0; d < 3; d++) axis_grid(m, d, gx[d], gi[d]);
+    const int nx = (int)gx[0].size(), ny = (int)gx[1].size(), nz = (int)gx[2].size();
+    if ((long long)nx * ny * nz != m.n_nodes || nx < 2 || ny < 2 || nz < 2) return MSA_ERR_ARG;
+    if (nx > 1024 || ny > 1024 || nz > 1024) return MSA_ERR_ARG;
+    if ((long long)(nx - 1) * (ny - 1) * (nz - 1) != m.n_elem) return MSA_ERR_ARG;
+    // equal spacing per direction: every element has the same shape
+    for (int d = 0; d < 3; d++) {
+        const double h = gx[d][1] - gx[d][0];
+        for (size_t i = 1; i + 1 < gx[d].size(); i++)
+            if (std::abs((gx[d][i + 1] - gx[d][i]) - h) > 1e-9 * h) return MSA_ERR_ARG;
+    }
+    GridInfo g;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.gblock.assign((size_t)nx * ny * nz, -2);
+    for (int n = 0; n < m.n_nodes; n++) {
+        const size_t p = gi[0][n] + (size_t)nx * (gi[1][n] + (size_t)ny * gi[2][n]);
+        if (g.gblock[p] != -2) return MSA_ERR_ARG;
+        g.gblock[p] = m.node_block[n];
+    }
+    g.bpos.assign(m.nb, 0);
+    for (int b = 0; b < m.nb; b++) {
+        const int n = m.block_node[b];
+        g.bpos[b] = gi[0][n] | (gi[1][n] << 10) | (gi[2][n] << 20);
+    }
+    // every element one cell, all with element 0's local node order
+    g.cell_elem.assign((size_t)(nx - 1) * (ny - 1) * (nz - 1), -1);
+    int pattern[8] = {0};
+    for (int e = 0; e < m.n_elem; e++) {
+        int lo[3] = {1 << 30, 1 << 30, 1 << 30};
+        for (int a = 0; a < 8; a++)
+            for (int d = 0; d < 3; d++) lo[d] = std::min(lo[d], gi[d][m.conn[8 * e + a]]);
+        for (int a = 0; a < 8; a++) {
+            const int n = m.conn[8 * e + a];
+            int off = 0;
+            for (int d = 0; d < 3; d++) {
+                const int o = gi[d][n] - lo[d];
+                if (o != 0 && o != 1) return MSA_ERR_ARG;
+                off |= o << d;
+            }
+            if (e == 0)
+                pattern[a] = off;
+            else if (pattern[a] != off)
+                return MSA_ERR_ARG;
+        }
+        const size_t cell = lo[0] + (size_t)(nx - 1) * (lo[1] + (size_t)(ny - 1) * lo[2]);
+        if (cell >= g.cell_elem.size() || g.cell_elem[cell] >= 0) return MSA_ERR_ARG;
+        g.cell_elem[cell] = e;
+    }
+    int seen = 0;
+    for (int a = 0; a < 8; a++) {
+        g.corner_node[pattern[a]] = a;
+        seen |= 1 << pattern[a];
+    }
+    if (seen != 255) return MSA_ERR_ARG;
+    gi_out = std::move(g);
+    return MSA_OK;
+}
+
+}  // namespace msa

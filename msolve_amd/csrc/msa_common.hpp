@@ -101,6 +101,18 @@ struct MgTransfer {
 // coarse element keeps the local node order of its low-corner child.
 int mg_coarsen(const MeshHost &f, MeshHost &c, MgTransfer &t);
 
+// A tensor-product grid whose elements are equal cells with one local node
+// order (all elements translates of element 0, so they share one element
+// matrix): grid point -> block (-1 constrained node), cell -> element,
+// block -> packed grid position i | j << 10 | k << 20, corner o = ox + 2 oy
+// + 4 oz -> local node.  MSA_ERR_ARG when the mesh is not such a grid.
+struct GridInfo {
+    int nx = 0, ny = 0, nz = 0;
+    std::vector<int> gblock, cell_elem, bpos;
+    int corner_node[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+int grid_info(const MeshHost &m, GridInfo &g);
+
 // Device view of one class of a batch (copied to device memory).
 struct ClassDev {
     int nb, nslices, ntiles, nsamp;
@@ -164,6 +176,13 @@ struct ClassDev {
     const int *mg_p_col, *mg_r_col;
     const float *mg_p_w, *mg_r_w;
     const int *mg_child;  // [n_elem*8] elements of the finer level in each element
+    // matrix-free smoothing products (mixed precision, one-sample tiles) on
+    // a uniform structured grid: K v from the cell moduli and the shared
+    // unit element matrix mf_k[576] (+ unit mass mf_k[576 + 64])
+    int mf, mf_nx, mf_ny, mf_nz;
+    int mf_corner[8];
+    const int *mf_gblock, *mf_cell, *mf_bpos;
+    float *mf_k;
 };
 
 constexpr int kBandNB = 64;        // block size of the band Cholesky
@@ -285,9 +304,10 @@ hipError_t launch_band_out(hipStream_t s, int T, const ClassDev *cp, int nsamp, 
 enum MgStep { MG_PRE = 0, MG_RESID = 1, MG_POST1 = 2, MG_POST2 = 3 };
 // which samples a cycle serves: 0 every sample, 1 active, 2 due for a
 // residual replacement (ir_active && ru_due), 3 refining (ir_active)
+// mf: the items' classes use the matrix-free product (mixed, T = 1)
 hipError_t launch_mg_smooth(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
                             const int4 *items, int nitems, int step, int pbuf, int mode,
-                            double ratio);
+                            double ratio, bool mf);
 hipError_t launch_mg_restrict(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
                               const int4 *items, int nitems, int pbuf, int mode);
 hipError_t launch_mg_prolong(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
@@ -300,6 +320,7 @@ hipError_t launch_mg_band_in(hipStream_t s, bool mixed, int T, const ClassDev *c
                              int bn, int mode);
 hipError_t launch_mg_band_out(hipStream_t s, bool mixed, int T, const ClassDev *cp, int nsamp,
                               int nrows, int mode);
+hipError_t launch_mf_khat(hipStream_t s, const ClassDev *cp);
 hipError_t launch_mg_load(hipStream_t s, bool mixed, int T, const ClassDev *cp, const double *src,
                           size_t nrows, int tile, int lane);
 hipError_t launch_band_scatter_any(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
