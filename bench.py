@@ -37,6 +37,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "MC solves/sec at 1e-8 rel. err, batched mixed vs plain FP64, 1 MI355X"
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
 
 
 def log(*a):
@@ -144,6 +145,35 @@ def smooth_bytes(b, models, rs):
         idx = 4.0 * mesh.info["block_nnz"] + 4.0 * (mesh.info["blocks"] + 1)
         total += float(cyc.max()) * 4 * idx
     return total
+
+
+def mf_work(b, models):
+    """FP32 operations and algorithmic HBM bytes of all finest-level
+    matrix-free smoothing launches of one solve (None unless every class
+    with a hierarchy uses them).  Per cycle and active sample four products;
+    each does, for every (free node, incident cell) pair, 72 FMA (three
+    rows of the unit element matrix times the cell's 24 values) and 3 FMA
+    (modulus scaling): 150 flop.  Bytes per product and sample: the moduli
+    (8 per element) and 4 vectors of 4*3*nb on average over the four steps
+    (as for the stored-value products, without the values); the grid tables
+    once per launch."""
+    flops = byts = 0.0
+    for c, m in enumerate(models):
+        info = b.mg_info(c)
+        if info["levels"] == 0:
+            continue
+        if not info["matrix_free"]:
+            return None
+        st = b.stats(c)
+        cyc = (st["iters"] + np.maximum(st["outer"], 1)).astype(np.float64)
+        conn = np.asarray(m.conn).reshape(-1, 8)
+        free_node = (np.asarray(m.dof).reshape(-1, 3) >= 0).any(axis=1)
+        inc = float(free_node[conn].sum())
+        nb = float(free_node.sum())
+        flops += cyc.sum() * 4 * inc * 150.0
+        byts += cyc.sum() * 4 * (8.0 * conn.shape[0] + 4 * 4 * 3 * nb)
+        byts += float(cyc.max()) * 4 * 4.0 * (len(free_node) + conn.shape[0] + nb)
+    return flops, byts
 
 
 def kernel_a_bytes(b, models, counts, rs, launches):
@@ -312,22 +342,36 @@ def main():
                   ms_total=kt["spmv_dot"]["ms"])
     roof_a["frac"] = roof_a["achieved"] / HBM_PEAK_GBS
     # the dominant kernel: the finest level's smoothing products under multigrid
+    mf = mf_work(bm, models) if args.precond == "mg" else None
+    bound, unit, peak, flops_per_launch = "hbm", "GB/s", HBM_PEAK_GBS, None
     if args.precond == "mg" and kt["multigrid"]["launches"] > 0:
         launches = kt["multigrid"]["launches"]
-        avg_bytes = smooth_bytes(bm, models, 4) / launches
         avg_s = kt["multigrid"]["ms"] / 1e3 / launches
-        kname = (f"k_mg_smooth<float,{args.tile},STEP> on the finest level (the cycle's four "
-                 f"products: PRE, RESID, POST1, POST2)")
+        if mf is not None:
+            flops_per_launch = mf[0] / launches
+            avg_bytes = mf[1] / launches
+            bound, unit, peak = "mfma", "TFLOP/s", FP32_PEAK_TFLOPS
+            kname = (f"k_mg_smooth<float,1,STEP,true> on the finest level: matrix-free FP32 "
+                     f"products (FP32 VALU, no MFMA; priced against the FP32 peak) of the "
+                     f"cycle's four steps PRE, RESID, POST1, POST2")
+        else:
+            avg_bytes = smooth_bytes(bm, models, 4) / launches
+            kname = (f"k_mg_smooth<float,{args.tile},STEP> on the finest level (the cycle's four "
+                     f"products: PRE, RESID, POST1, POST2)")
     else:
         launches, avg_bytes, avg_s = a_launches, a_bytes, a_s
         kname = roof_a["kernel"]
-    achieved = avg_bytes / avg_s / 1e9 if avg_s > 0 else 0.0
+    if flops_per_launch is not None:
+        achieved = flops_per_launch / avg_s / 1e12 if avg_s > 0 else 0.0
+    else:
+        achieved = avg_bytes / avg_s / 1e9 if avg_s > 0 else 0.0
     # HBM traffic of k_pcg_a: rocprofv3 FETCH_SIZE/WRITE_SIZE passes (scaled by a
     # device-copy calibration) measured bytes / algorithmic bytes, applied to
     # this run's algorithmic bytes per launch (profiles/k_pcg_a_mixed_montecarlo.md)
     traffic, traffic_ratio = None, None
-    cpath = os.path.join(ROOT, "profiles", "k_pcg_a_counters.json"
-                         if args.precond == "jacobi" else "k_mg_smooth_counters.json")
+    cpath = os.path.join(ROOT, "profiles", "k_pcg_a_counters.json" if args.precond == "jacobi"
+                         else ("k_mg_smooth_mf_counters.json" if mf is not None
+                               else "k_mg_smooth_counters.json"))
     if os.path.exists(cpath):
         cj = json.load(open(cpath))
         traffic_ratio = (cj["fetch_bytes"] + cj["write_bytes"]) / (cj["algorithmic_read"] +
@@ -457,12 +501,14 @@ def main():
             "graph_iters": args.graph_iters, "all_converged": bool(all_ok),
             "max_rel_err_vs_fp64": max_err,
         },
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+        "roofline": {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+                     "frac": achieved / peak, "traffic": traffic,
                      "traffic_over_algorithmic": traffic_ratio,
                      "kernel": kname,
                      "launches": launches, "avg_launch_ms": 1e3 * avg_s,
-                     "avg_bytes_per_launch": avg_bytes},
+                     "avg_bytes_per_launch": avg_bytes,
+                     "avg_flop_per_launch": flops_per_launch,
+                     "hbm_gbs": avg_bytes / avg_s / 1e9 if avg_s > 0 else 0.0},
         "cpu_baseline": cpu,
         "extra": extra,
         "iters_mean_per_size": {str(n): float(np.mean(iters_mixed[c])) for c, n in enumerate(sizes)},

@@ -307,7 +307,12 @@ int msa_batch_direct_info(msa_batch *b, int cls, long long *info);
 int msa_batch_set_preconditioner(msa_batch *b, int kind, int max_levels, double ratio);
 /* info[0] coarse levels of class cls (0: Jacobi), [1] free DOFs of the
  * coarsest level, [2] free DOFs summed over the coarse levels, [3] half
- * bandwidth of the coarsest level. */
+ * bandwidth of the coarsest level, [4] 1 if the finest level's smoothing
+ * products are matrix-free, [5] the number of coarse levels that are.
+ * Matrix-free products are used by MIXED batches with one-sample tiles on
+ * grids of equal, equally oriented cells (all elements share one element
+ * matrix): K v from the cell moduli and that matrix in FP32, without
+ * reading the stored values. */
 int msa_batch_mg_info(msa_batch *b, int cls, long long *info);
 /* z = B r (one V-cycle) for one sample of class cls, r and z in the
  * reference numbering.  Uses the batch's PCG work vectors: call it between

@@ -245,10 +245,11 @@ class Batch:
         check(lib().msa_batch_set_preconditioner(self.h, kind, max_levels, ratio))
 
     def mg_info(self, cls: int):
-        info = np.zeros(4, np.int64)
+        info = np.zeros(6, np.int64)
         check(lib().msa_batch_mg_info(self.h, cls, ptr(info, C.c_longlong)))
         return dict(levels=int(info[0]), coarsest_dofs=int(info[1]), coarse_dofs=int(info[2]),
-                    coarsest_bandwidth=int(info[3]))
+                    coarsest_bandwidth=int(info[3]), matrix_free=bool(info[4]),
+                    matrix_free_coarse_levels=int(info[5]))
 
     def mg_apply(self, cls: int, sample: int, r):
         """One V-cycle z = B r for one sample (reference DOF numbering)."""

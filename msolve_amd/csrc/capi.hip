@@ -1961,9 +1961,12 @@ int msa_batch_set_preconditioner(msa_batch *b, int kind, int max_levels, double 
 int msa_batch_mg_info(msa_batch *b, int cls, long long *info)
 {
     MSA_CHECK(b && info && cls >= 0 && cls < (int)b->cls.size(), MSA_ERR_ARG, "bad arguments");
-    info[0] = info[1] = info[2] = info[3] = 0;
+    info[0] = info[1] = info[2] = info[3] = info[4] = info[5] = 0;
     if (b->mg_nlev.empty()) return MSA_OK;
     info[0] = b->mg_nlev[cls];
+    info[4] = b->cls[cls].dev.mf;
+    for (int ci = b->cls[cls].dev.mg_coarse; ci >= 0; ci = tab(b, ci).dev.mg_coarse)
+        info[5] += tab(b, ci).dev.mf;
     for (int ci = b->cls[cls].dev.mg_coarse; ci >= 0; ci = tab(b, ci).dev.mg_coarse) {
         const ClassHost &L = tab(b, ci);
         info[2] += L.mesh->h.n_free;

@@ -2445,9 +2445,10 @@ __device__ __forceinline__ void row_product(const MSA_GLOBAL real *val, const MS
 
 // y = K v for block row br of a uniform structured grid class, from the
 // moduli of the (up to) 8 cells around the node and the shared unit element
-// matrix (FP32): y = sum_cells cK E_c Khat[a][b] v_b (+ cM Mhat[a][b] v_b on
-// the component itself), a = the node's corner in the cell, b over the
-// cell's corners.  The 27 neighbouring blocks are gathered once.
+// matrix (FP32, stored in corner order): y = sum_cells cK E_c Khat[a][b] v_b
+// (+ cM Mhat[a][b] v_b on the component itself), a = the node's corner in
+// the cell, b over the cell's corners.  The 27 neighbouring blocks are
+// gathered once.
 __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL float *v,
                                            const double *Es, int br, float &y0, float &y1,
                                            float &y2)
@@ -2460,80 +2461,112 @@ __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL f
     const MSA_GLOBAL int *gblock = gbl(C.mf_gblock);
     const MSA_GLOBAL int *cell = gbl(C.mf_cell);
     const MSA_GLOBAL double *E = gbl(Es);
+    // 27 neighbours, branch-free so that every load is in flight at once:
+    // clamped grid points, then masked values (outside the grid or a
+    // constrained node: block 0 is read and zeroed)
+    int nbv[27];
+#pragma unroll
+    for (int n = 0; n < 27; n++) {
+        const int dx = n % 3, dy = (n / 3) % 3, dz = n / 9;
+        const int ii = min(max(i + dx - 1, 0), nx - 1), jj = min(max(j + dy - 1, 0), ny - 1),
+                  kk = min(max(k + dz - 1, 0), nz - 1);
+        const bool in = ii == i + dx - 1 && jj == j + dy - 1 && kk == k + dz - 1;
+        const int nb = gblock[ii + nx * (jj + ny * kk)];
+        nbv[n] = in ? nb : -1;
+    }
     float xv[27][3];
 #pragma unroll
-    for (int dz = 0; dz < 3; dz++)
-#pragma unroll
-        for (int dy = 0; dy < 3; dy++)
-#pragma unroll
-            for (int dx = 0; dx < 3; dx++) {
-                const int ii = i + dx - 1, jj = j + dy - 1, kk = k + dz - 1;
-                const bool in = ii >= 0 && ii < nx && jj >= 0 && jj < ny && kk >= 0 && kk < nz;
-                const int nb = in ? gblock[ii + nx * (jj + ny * kk)] : -1;
-                float *x = xv[(dz * 3 + dy) * 3 + dx];
-                if (nb >= 0) {
-                    const vec3<float> c = ld3(v, nb);
-                    x[0] = c.x;
-                    x[1] = c.y;
-                    x[2] = c.z;
-                } else {
-                    x[0] = x[1] = x[2] = 0.f;
-                }
-            }
+    for (int n = 0; n < 27; n++) {
+        const bool ok = nbv[n] >= 0;
+        const vec3<float> c = ld3(v, ok ? nbv[n] : 0);
+        xv[n][0] = ok ? c.x : 0.f;
+        xv[n][1] = ok ? c.y : 0.f;
+        xv[n][2] = ok ? c.z : 0.f;
+    }
     const float cK = C.dyn ? (float)C.cK : 1.f, cM = C.dyn ? (float)C.cM : 0.f;
     // the element matrix is the same for every lane: constant-address loads
     // with wave-uniform indices (scalar loads, SGPR operands of the FMAs)
     using cfloat = __attribute__((address_space(4))) const float;
-    cfloat *kh = (cfloat *)C.mf_k, *mh = (cfloat *)C.mf_k + 576;
-    int corner[8];
+    // an empty asm redefines the pointer on every call, so the 576 loads are
+    // not hoisted out of the caller's row loop (they would not fit the
+    // scalar registers)
+    unsigned long long kp = (unsigned long long)C.mf_k;
+    asm volatile("" : "+s"(kp));
+    cfloat *kh = (cfloat *)kp, *mh = (cfloat *)kp + 576;
+    // the 8 cells around the node (c = cx + 2 cy + 4 cz): every index and
+    // modulus load issued before the arithmetic; cells outside the grid
+    // read cell 0 and get modulus 0
+    int ce[8];
+    bool cin[8];
 #pragma unroll
-    for (int o = 0; o < 8; o++) corner[o] = __builtin_amdgcn_readfirstlane(C.mf_corner[o]);
+    for (int c = 0; c < 8; c++) {
+        const int ci = i - 1 + (c & 1), cj = j - 1 + ((c >> 1) & 1), ck = k - 1 + (c >> 2);
+        cin[c] = ci >= 0 && ci < nx - 1 && cj >= 0 && cj < ny - 1 && ck >= 0 && ck < nz - 1;
+        ce[c] = cell[cin[c] ? ci + (nx - 1) * (cj + (ny - 1) * ck) : 0];
+    }
+    float Ec[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) Ec[c] = cin[c] ? (float)E[ce[c]] * cK : 0.f;
     float acc[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-    for (int cz = 0; cz < 2; cz++)
+    for (int c = 0; c < 8; c++) {
+        const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+        const int a = (1 - cx) + 2 * (1 - cy) + 4 * (1 - cz);  // the node's corner
+        float t[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-        for (int cy = 0; cy < 2; cy++)
+        for (int o = 0; o < 8; o++) {
+            const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
+            cfloat *kab = kh + (a * 8 + o) * 9;
+            const float *xb = xv[((cz + oz) * 3 + (cy + oy)) * 3 + (cx + ox)];
 #pragma unroll
-            for (int cx = 0; cx < 2; cx++) {
-                const int ci = i - 1 + cx, cj = j - 1 + cy, ck = k - 1 + cz;
-                if (ci < 0 || ci >= nx - 1 || cj < 0 || cj >= ny - 1 || ck < 0 || ck >= nz - 1) continue;
-                const float Ec = (float)E[cell[ci + (nx - 1) * (cj + (ny - 1) * ck)]] * cK;
-                const int a = corner[(1 - cx) + 2 * (1 - cy) + 4 * (1 - cz)];
-                float t[3] = {0.f, 0.f, 0.f}, m[3] = {0.f, 0.f, 0.f};
+            for (int p = 0; p < 3; p++)
+                t[p] += kab[3 * p] * xb[0] + kab[3 * p + 1] * xb[1] + kab[3 * p + 2] * xb[2];
+        }
 #pragma unroll
-                for (int oz = 0; oz < 2; oz++)
+        for (int p = 0; p < 3; p++) acc[p] += Ec[c] * t[p];
+        // keep each cell's 72 scalar loads next to their use
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    if (cM != 0.f) {  // consistent mass of the cells inside the grid (dynamics)
 #pragma unroll
-                    for (int oy = 0; oy < 2; oy++)
+        for (int c = 0; c < 8; c++) {
+            const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+            const int a = (1 - cx) + 2 * (1 - cy) + 4 * (1 - cz);
+            float m[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-                        for (int ox = 0; ox < 2; ox++) {
-                            const int b = corner[ox + 2 * oy + 4 * oz];
-                            cfloat *kab = kh + (a * 8 + b) * 9;
-                            const float *xb = xv[((cz + oz) * 3 + (cy + oy)) * 3 + (cx + ox)];
+            for (int o = 0; o < 8; o++) {
+                const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
+                const float mab = mh[a * 8 + o];
+                const float *xb = xv[((cz + oz) * 3 + (cy + oy)) * 3 + (cx + ox)];
 #pragma unroll
-                            for (int p = 0; p < 3; p++)
-                                t[p] += kab[3 * p] * xb[0] + kab[3 * p + 1] * xb[1] +
-                                        kab[3 * p + 2] * xb[2];
-                            if (cM != 0.f) {
-                                const float mab = mh[a * 8 + b];
-#pragma unroll
-                                for (int p = 0; p < 3; p++) m[p] += mab * xb[p];
-                            }
-                        }
-#pragma unroll
-                for (int p = 0; p < 3; p++) acc[p] += Ec * t[p] + cM * m[p];
+                for (int p = 0; p < 3; p++) m[p] += mab * xb[p];
             }
+            const float w = cin[c] ? cM : 0.f;
+#pragma unroll
+            for (int p = 0; p < 3; p++) acc[p] += w * m[p];
+        }
+    }
     y0 = acc[0];
     y1 = acc[1];
     y2 = acc[2];
 }
 
-// FP32 copy of element 0's unit Khat (and Mhat) for the matrix-free products.
+// FP32 copy of element 0's unit Khat (and Mhat) for the matrix-free
+// products, in corner order: block (oa, ob) of corners oa, ob is Khat's
+// block (corner_node[oa], corner_node[ob]).
 __global__ void k_mf_khat(const ClassDev *__restrict__ cp)
 {
     const ClassDev &C = *cp;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < 576) C.mf_k[i] = (float)C.kc[i];
-    else if (i < 640) C.mf_k[i] = C.mc ? (float)C.mc[i - 576] : 0.f;
+    if (i < 576) {
+        const int blk = i / 9, r = i - 9 * (i / 9);
+        const int a = C.mf_corner[blk >> 3], b = C.mf_corner[blk & 7];
+        C.mf_k[i] = (float)C.kc[(a * 8 + b) * 9 + r];
+    } else if (i < 640) {
+        const int blk = i - 576;
+        const int a = C.mf_corner[blk >> 3], b = C.mf_corner[blk & 7];
+        C.mf_k[i] = C.mc ? (float)C.mc[a * 8 + b] : 0.f;
+    }
 }
 
 // One smoothing-level product with its row update (STEP, see MgStep):

@@ -220,3 +220,33 @@ def test_solver_pcg_surface_with_multigrid(ctx):
     assert rel(u, spl.spsolve(levels[0].K.tocsc(), model.f)) < 1e-10
     solver.batch.close()
     solver.mesh.close()
+
+
+@pytest.mark.parametrize("dynamics", [False, True])
+def test_matrix_free_cycle_matches_stored_value_cycle(ctx, dynamics):
+    """MIXED batches on uniform grids smooth with matrix-free products (cell
+    moduli x the shared element matrix, plus the consistent mass in the
+    dynamics mode); FP64 batches use the stored values.  One V-cycle of each
+    on the same vector agree to FP32 accuracy, boundary cells included."""
+    print(f"\n[case] matrix-free vs stored-value cycle dynamics={dynamics}", flush=True)
+    model = msa.cantilever(12, 12, 8, lz=0.5)
+    mesh = msa.Mesh(ctx, model.xyz, model.conn, model.dof, model.n_free)
+    rng = np.random.default_rng(13)
+    E = np.stack([lognormal(rng, model.n_elem) for _ in range(2)])
+    r = rng.standard_normal(model.n_free)
+    z = {}
+    for prec in (L.MSA_FP64, L.MSA_MIXED):
+        b = msa.Batch(ctx, [mesh], [2], prec, 0)
+        b.set_modulus(0, E)
+        b.set_rhs(0, model.f)
+        if dynamics:
+            b.set_dynamics(7.85e-9, 5.0, 1e-5, 0.25, 0.5, 1e-4)
+        b.assemble(0.3)
+        b.set_preconditioner(msa.MSA_PRECOND_MULTIGRID)
+        info = b.mg_info(0)
+        assert info["levels"] == 2
+        assert info["matrix_free"] == (prec == L.MSA_MIXED)
+        z[prec] = b.mg_apply(0, 1, r)
+        b.close()
+    mesh.close()
+    assert rel(z[L.MSA_MIXED], z[L.MSA_FP64]) < 2e-5
