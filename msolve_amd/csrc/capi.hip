@@ -119,6 +119,8 @@ struct msa_batch {
     int mg_nlv_gitems = 0;
     int4 *mg_co_items = nullptr;        // the coarsest levels
     int mg_nco_items = 0;
+    int4 *mf_items = nullptr;           // solved classes with matrix-free products
+    int n_mf_items = 0;
     std::vector<int> mg_coarsest;       // class table indices of the coarsest levels
     int mg_max_levels = 0;              // max_levels the hierarchy was built with
     size_t d_cls_cap = 0;               // ClassDev entries allocated at d_cls
@@ -384,7 +386,7 @@ int launch_iteration_pair(msa_batch *b, int pbuf, double tol, int max_it, double
     return MSA_OK;
 }
 
-ColourLaunch colour_launch(const msa_batch *b, int colour, int *nblocks);
+ColourLaunch colour_launch(const msa_batch *b, int colour, int *nblocks, bool skip_mf = false);
 int max_colours(const msa_batch *b);
 
 int max_tiles(const msa_batch *b)
@@ -401,10 +403,18 @@ int launch_replacement(msa_batch *b, double tol, int max_it)
     hipStream_t s = b->ctx->stream;
     const int T = b->T, ni = (int)b->items.size();
     MSA_TRY(timed(b, KK_OTHER, [&] { return launch_ru_fold(s, T, b->d_cls, b->d_items, ni); }));
+    // matrix-free residual of the uniform-grid classes (multigrid set up),
+    // the coloured element scatter for the others
+    const bool mfres = b->precond == MSA_PRECOND_MULTIGRID && b->mg_ready && b->n_mf_items > 0;
+    if (mfres)
+        MSA_TRY(timed(b, KK_RES64, [&] {
+            return launch_residual64_mf(s, b->d_cls, b->mf_items, b->n_mf_items);
+        }));
     const int ncol = max_colours(b);
     for (int k = 0; k < ncol; k++) {
         int nblk = 0;
-        ColourLaunch cl = colour_launch(b, k, &nblk);
+        ColourLaunch cl = colour_launch(b, k, &nblk, mfres);
+        if (nblk == 0) continue;
         MSA_TRY(timed(b, KK_RES64, [&] { return launch_residual64(s, T, b->d_cls, cl, nblk, b->nu, 1, max_tiles(b)); }));
     }
     MSA_TRY(timed(b, KK_OTHER, [&] {
@@ -485,7 +495,8 @@ int pcg_loop(msa_batch *b, int kind, double tol, int max_it, int graph_iters)
     return MSA_OK;
 }
 
-ColourLaunch colour_launch(const msa_batch *b, int colour, int *nblocks)
+// skip_mf: leave out the classes whose replacement residual is matrix-free
+ColourLaunch colour_launch(const msa_batch *b, int colour, int *nblocks, bool skip_mf)
 {
     ColourLaunch cl{};
     cl.ncls = (int)b->cls.size();
@@ -494,6 +505,7 @@ ColourLaunch colour_launch(const msa_batch *b, int colour, int *nblocks)
     for (size_t c = 0; c < b->cls.size(); c++) {
         const MeshHost &m = b->cls[c].mesh->h;
         int cnt = colour < m.ncolours ? m.colour_ptr[colour + 1] - m.colour_ptr[colour] : 0;
+        if (skip_mf && b->cls[c].dev.mf) cnt = 0;
         cl.prefix[c + 1] = cl.prefix[c] + cnt;
     }
     *nblocks = cl.prefix[cl.ncls];
@@ -716,6 +728,8 @@ int mf_setup(msa_batch *b, ClassHost &C)
     void *p = nullptr;
     MSA_TRY(dalloc(b, &p, 640 * sizeof(float)));
     D.mf_k = (float *)p;
+    MSA_TRY(dalloc(b, &p, 640 * sizeof(double)));
+    D.mf_k64 = (double *)p;
     D.mf_nx = g.nx;
     D.mf_ny = g.ny;
     D.mf_nz = g.nz;
@@ -826,6 +840,12 @@ int mg_build(msa_batch *b, int max_levels)
         if (d > 0) MSA_TRY(items_to_device(b, re, &b->mg_restr[d], &b->mg_nrestr[d]));
     }
     MSA_TRY(items_to_device(b, all, &b->mg_lv_items, &b->mg_nlv_items));
+    {
+        std::vector<int4> mfi;
+        for (int c = 0; c < ncls; c++)
+            if (b->cls[c].dev.mf) class_items(b->cls[c], c, mfi);
+        MSA_TRY(items_to_device(b, mfi, &b->mf_items, &b->n_mf_items));
+    }
     std::vector<int4> gall;
     for (size_t l = 0; l < b->lv.size(); l++)
         class_group_items(b->lv[l], (int)(b->cls.size() + l), gall);

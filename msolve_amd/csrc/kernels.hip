@@ -2551,21 +2551,98 @@ __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL f
     y2 = acc[2];
 }
 
-// FP32 copy of element 0's unit Khat (and Mhat) for the matrix-free
-// products, in corner order: block (oa, ob) of corners oa, ob is Khat's
-// block (corner_node[oa], corner_node[ob]).
+// FP32 and FP64 copies of element 0's unit Khat (and Mhat) for the
+// matrix-free products, in corner order: block (oa, ob) of corners oa, ob
+// is Khat's block (corner_node[oa], corner_node[ob]).
 __global__ void k_mf_khat(const ClassDev *__restrict__ cp)
 {
     const ClassDev &C = *cp;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
     if (i < 576) {
         const int blk = i / 9, r = i - 9 * (i / 9);
         const int a = C.mf_corner[blk >> 3], b = C.mf_corner[blk & 7];
-        C.mf_k[i] = (float)C.kc[(a * 8 + b) * 9 + r];
+        v = C.kc[(a * 8 + b) * 9 + r];
     } else if (i < 640) {
         const int blk = i - 576;
         const int a = C.mf_corner[blk >> 3], b = C.mf_corner[blk & 7];
-        C.mf_k[i] = C.mc ? (float)C.mc[a * 8 + b] : 0.f;
+        v = C.mc ? C.mc[a * 8 + b] : 0.0;
+    }
+    if (i < 640) {
+        C.mf_k[i] = (float)v;
+        C.mf_k64[i] = v;
+    }
+}
+
+// FP64 residual r64 -= (cK E K + cM M) u64 of the samples due for a
+// replacement, matrix-free on a uniform grid class (the stored-value-free
+// counterpart of k_residual64's due path): every free row of a due sample
+// sums its 8 cells, each cell's 8 corners gathered from u64 (FP64), the
+// element matrix in LDS.  One-sample tiles: a work item is one sample, so
+// items of samples that are not due exit before staging anything.
+__global__ __launch_bounds__(256) void k_residual64_mf(const ClassDev *__restrict__ cls,
+                                                       const int4 *__restrict__ items, int nitems)
+{
+    constexpr int R = 64;
+    __shared__ double kh[640];
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const int tile = it.y, chunk = it.z;
+    const int sample = tile;
+    if (*C.due == 0 || sample >= C.nsamp || !C.st[sample].ir_active || !C.st[sample].ru_due) return;
+    for (int t = threadIdx.x; t < 640; t += blockDim.x) kh[t] = C.mf_k64[t];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nx = C.mf_nx, ny = C.mf_ny, nz = C.mf_nz;
+    const MSA_GLOBAL int *gblock = gbl(C.mf_gblock);
+    const MSA_GLOBAL int *cell = gbl(C.mf_cell);
+    const MSA_GLOBAL double *E = gbl((const double *)C.E + (size_t)tile * C.n_elem);
+    const MSA_GLOBAL double *u = gbl((const double *)C.u64 + (size_t)tile * C.tile_vec);
+    MSA_GLOBAL double *r = gbl(C.r64 + (size_t)tile * C.tile_vec);
+    const double cK = C.dyn ? C.cK : 1.0, cM = C.dyn ? C.cM : 0.0;
+    const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+    for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+        const int br = slice * R + lane;
+        if (br >= C.nb) continue;
+        const int g = gbl(C.mf_bpos)[br];
+        const int i = g & 1023, j = (g >> 10) & 1023, k = g >> 20;
+        double y[3] = {0.0, 0.0, 0.0};
+#pragma unroll 1
+        for (int c = 0; c < 8; c++) {
+            const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+            const int ci = i - 1 + cx, cj = j - 1 + cy, ck = k - 1 + cz;
+            if (ci < 0 || ci >= nx - 1 || cj < 0 || cj >= ny - 1 || ck < 0 || ck >= nz - 1) continue;
+            const double Ec = E[cell[ci + (nx - 1) * (cj + (ny - 1) * ck)]] * cK;
+            const int a = (1 - cx) + 2 * (1 - cy) + 4 * (1 - cz);
+            double uc[8][3];
+#pragma unroll
+            for (int o = 0; o < 8; o++) {
+                const int nb = gblock[(ci + (o & 1)) + nx * ((cj + ((o >> 1) & 1)) + ny * (ck + (o >> 2)))];
+                const vec3<double> v = ld3(u, nb >= 0 ? nb : 0);
+                uc[o][0] = nb >= 0 ? v.x : 0.0;
+                uc[o][1] = nb >= 0 ? v.y : 0.0;
+                uc[o][2] = nb >= 0 ? v.z : 0.0;
+            }
+            double t3[3] = {0.0, 0.0, 0.0}, m3[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int o = 0; o < 8; o++) {
+                const double *kab = kh + (a * 8 + o) * 9;
+#pragma unroll
+                for (int p = 0; p < 3; p++)
+                    t3[p] += kab[3 * p] * uc[o][0] + kab[3 * p + 1] * uc[o][1] + kab[3 * p + 2] * uc[o][2];
+                const double mab = kh[576 + a * 8 + o];
+#pragma unroll
+                for (int p = 0; p < 3; p++) m3[p] += mab * uc[o][p];
+            }
+#pragma unroll
+            for (int p = 0; p < 3; p++) y[p] += Ec * t3[p] + cM * m3[p];
+        }
+        const uint8_t fr = C.block_free[br];
+#pragma unroll
+        for (int p = 0; p < 3; p++)
+            if ((fr >> p) & 1) r[(size_t)br * 3 + p] -= y[p];
     }
 }
 
@@ -3398,6 +3475,13 @@ hipError_t launch_mg_band_in(hipStream_t s, bool mixed, int T, const ClassDev *c
         MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_mg_band_in<double, TT>), grid, dim3(256), 0, s, cp,
                                              mode));
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_residual64_mf(hipStream_t s, const ClassDev *d_cls, const int4 *items, int nitems)
+{
+    if (nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_residual64_mf, dim3(xcd_grid(nitems)), dim3(256), 0, s, d_cls, items, nitems);
     return hipGetLastError();
 }
 

@@ -183,6 +183,7 @@ struct ClassDev {
     int mf_corner[8];
     const int *mf_gblock, *mf_cell, *mf_bpos;
     float *mf_k;
+    double *mf_k64;       // [640] the same in FP64 (replacement residuals)
 };
 
 constexpr int kBandNB = 64;        // block size of the band Cholesky
@@ -321,6 +322,8 @@ hipError_t launch_mg_band_in(hipStream_t s, bool mixed, int T, const ClassDev *c
 hipError_t launch_mg_band_out(hipStream_t s, bool mixed, int T, const ClassDev *cp, int nsamp,
                               int nrows, int mode);
 hipError_t launch_mf_khat(hipStream_t s, const ClassDev *cp);
+// replacement residual of the due samples of matrix-free classes (T = 1)
+hipError_t launch_residual64_mf(hipStream_t s, const ClassDev *d_cls, const int4 *items, int nitems);
 hipError_t launch_mg_load(hipStream_t s, bool mixed, int T, const ClassDev *cp, const double *src,
                           size_t nrows, int tile, int lane);
 hipError_t launch_band_scatter_any(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
