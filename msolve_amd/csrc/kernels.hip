@@ -2461,17 +2461,20 @@ __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL f
     const MSA_GLOBAL int *gblock = gbl(C.mf_gblock);
     const MSA_GLOBAL int *cell = gbl(C.mf_cell);
     const MSA_GLOBAL double *E = gbl(Es);
-    // 27 neighbours, branch-free so that every load is in flight at once:
-    // clamped grid points, then masked values (outside the grid or a
-    // constrained node: block 0 is read and zeroed)
+    // 27 neighbours, branch-free so that every load is in flight at once.
+    // Grid point of neighbour (dx, dy, dz) = own point + (dx-1) + (dy-1) nx
+    // + (dz-1) nx ny; outside the grid it reads the own point and is masked
+    // (as is a constrained node: block -1).
+    const bool lo[3] = {i > 0, j > 0, k > 0}, hi[3] = {i < nx - 1, j < ny - 1, k < nz - 1};
+    const int gp0 = i + nx * (j + ny * k), sy = nx, sz = nx * ny;
     int nbv[27];
 #pragma unroll
     for (int n = 0; n < 27; n++) {
         const int dx = n % 3, dy = (n / 3) % 3, dz = n / 9;
-        const int ii = min(max(i + dx - 1, 0), nx - 1), jj = min(max(j + dy - 1, 0), ny - 1),
-                  kk = min(max(k + dz - 1, 0), nz - 1);
-        const bool in = ii == i + dx - 1 && jj == j + dy - 1 && kk == k + dz - 1;
-        const int nb = gblock[ii + nx * (jj + ny * kk)];
+        const bool in = (dx != 0 || lo[0]) && (dx != 2 || hi[0]) && (dy != 0 || lo[1]) &&
+                        (dy != 2 || hi[1]) && (dz != 0 || lo[2]) && (dz != 2 || hi[2]);
+        const int gp = gp0 + (dx - 1) + (dy - 1) * sy + (dz - 1) * sz;
+        const int nb = gblock[in ? gp : gp0];
         nbv[n] = in ? nb : -1;
     }
     float xv[27][3];
@@ -2496,13 +2499,17 @@ __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL f
     // the 8 cells around the node (c = cx + 2 cy + 4 cz): every index and
     // modulus load issued before the arithmetic; cells outside the grid
     // read cell 0 and get modulus 0
+    // cell (cx, cy, cz) around the node: cells i-1+cx, ... exist when cx = 0
+    // and i > 0, or cx = 1 and i < nx - 1 (likewise in y, z)
+    const int cy1 = nx - 1, cz1 = (nx - 1) * (ny - 1);
+    const int c0 = (i - 1) + cy1 * (j - 1) + cz1 * (k - 1);
     int ce[8];
     bool cin[8];
 #pragma unroll
     for (int c = 0; c < 8; c++) {
-        const int ci = i - 1 + (c & 1), cj = j - 1 + ((c >> 1) & 1), ck = k - 1 + (c >> 2);
-        cin[c] = ci >= 0 && ci < nx - 1 && cj >= 0 && cj < ny - 1 && ck >= 0 && ck < nz - 1;
-        ce[c] = cell[cin[c] ? ci + (nx - 1) * (cj + (ny - 1) * ck) : 0];
+        const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+        cin[c] = (cx ? hi[0] : lo[0]) && (cy ? hi[1] : lo[1]) && (cz ? hi[2] : lo[2]);
+        ce[c] = cell[cin[c] ? c0 + cx + cy * cy1 + cz * cz1 : 0];
     }
     float Ec[8];
 #pragma unroll
