@@ -801,7 +801,14 @@ int mg_build(msa_batch *b, int max_levels)
             depth_classes[D.mg_depth].push_back(ci);
             if (l + 1 == meshes.size()) {
                 bool changed = false;
-                MSA_TRY(band_setup(b, b->lv.back(), &changed));
+                ClassHost &Cc = b->lv.back();
+                MSA_TRY(band_setup(b, Cc, &changed));
+                // small coarsest operators: dense inverse, one launch per cycle
+                if (Cc.dev.bn <= kDenseMax) {
+                    void *p = nullptr;
+                    MSA_TRY(dalloc(b, &p, (size_t)Cc.nsamp * Cc.dev.bn * Cc.dev.bn * sizeof(double)));
+                    Cc.dev.binvd = (double *)p;
+                }
                 b->mg_coarsest.push_back(ci);
             }
             parent = ci;
@@ -954,6 +961,18 @@ int mg_values(msa_batch *b)
                       "multigrid: coarsest operator of class %d, sample %d is not positive definite",
                       C.dev.mg_root, k);
     }
+    // dense inverses of the coarsest operators; rows outside the band order
+    // (constrained components) of their solution vectors stay zero
+    for (int ci : b->mg_coarsest) {
+        ClassHost &C = tab(b, ci);
+        if (C.dev.binvd)
+            MSA_TRY(timed(b, KK_ASSEMBLE, [&] { return launch_band_inverse(s, b->d_cls + ci, C.nsamp, C.dev.bn); }));
+        const size_t nz = (size_t)C.ntiles * C.tile_vec;
+        MSA_TRY(timed(b, KK_OTHER, [&] {
+            return mx ? launch_fill_f32(s, (float *)C.dev.z, nz, 0.f)
+                      : launch_fill_f64(s, (double *)C.dev.z, nz, 0.0);
+        }));
+    }
     b->mg_ready = true;
     return MSA_OK;
 }
@@ -994,6 +1013,10 @@ int mg_vcycle(msa_batch *b, int pbuf, int mode)
         ClassHost &C = tab(b, ci);
         const ClassDev *cp = b->d_cls + ci;
         const int n = C.dev.bn, W = C.dev.bW, ns = C.nsamp;
+        if (C.dev.binvd) {
+            MSA_TRY(timed(b, KK_OTHER, [&] { return launch_mg_dense_solve(s, mx, T, cp, ns, n, mode); }));
+            continue;
+        }
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_mg_band_in(s, mx, T, cp, ns, n, mode); }));
         for (int k0 = 0; k0 < n; k0 += kBandNB) {
             const int nk = std::min(kBandNB, n - k0), r0 = k0 + nk, m = std::min(n - r0, W - kBandNB);

@@ -2912,6 +2912,71 @@ __global__ void k_mg_load(const ClassDev *__restrict__ cp, const double *__restr
     ((real *)C.z)[o] = ((const real *)C.dinv)[o] * v;
 }
 
+// Dense inverse X = (L L^T)^-1 of every sample's band Cholesky factor
+// (coarsest multigrid levels, bn <= kDenseMax): one wavefront per column j
+// of one sample; forward L y = e_j and backward L^T x = y with the column
+// in LDS, each row's band sum split over the 64 lanes and reduced by
+// shuffles.  X is symmetric: column j is written as row j (contiguous).
+__global__ __launch_bounds__(64) void k_band_inverse(const ClassDev *__restrict__ cp)
+{
+    __shared__ double y[kDenseMax];
+    const ClassDev &C = *cp;
+    const int smp = blockIdx.y, j = blockIdx.x, lane = threadIdx.x;
+    const int n = C.bn, W = C.bW;
+    const MSA_GLOBAL double *L = gbl(C.band + (size_t)smp * n * W);
+    for (int i = lane; i < n; i += 64) y[i] = 0.0;
+    __syncthreads();
+    // forward: y_i = (delta_ij - sum_{k in [max(j, i-W+1), i)} L_ik y_k) / L_ii
+    for (int i = j; i < n; i++) {
+        double part = 0.0;
+        for (int k2 = max(j, i - W + 1) + lane; k2 < i; k2 += 64)
+            part += L[(size_t)k2 * W + (i - k2)] * y[k2];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+        const double yi = ((i == j ? 1.0 : 0.0) - part) / L[(size_t)i * W];
+        __syncthreads();
+        if (lane == 0) y[i] = yi;
+        __syncthreads();
+    }
+    // backward: x_i = (y_i - sum_{k in (i, min(n, i+W))} L_ki x_k) / L_ii
+    for (int i = n - 1; i >= 0; i--) {
+        double part = 0.0;
+        const int kend = min(n, i + W);
+        for (int k2 = i + 1 + lane; k2 < kend; k2 += 64)
+            part += L[(size_t)i * W + (k2 - i)] * y[k2];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+        const double xi = (y[i] - part) / L[(size_t)i * W];
+        __syncthreads();
+        if (lane == 0) y[i] = xi;
+        __syncthreads();
+    }
+    MSA_GLOBAL double *X = gbl(C.binvd + (size_t)smp * n * n + (size_t)j * n);
+    for (int i = lane; i < n; i += 64) X[i] = y[i];
+}
+
+// Coarsest solve by the dense inverse: z[bperm[i]] = sum_j X[j][i] b[bperm[j]]
+// (X symmetric: lane i reads row j at column i, so loads are coalesced).
+template <typename real, int T>
+__global__ __launch_bounds__(64) void k_mg_dense_solve(const ClassDev *__restrict__ cp, int mode)
+{
+    const ClassDev &C = *cp;
+    const int smp = blockIdx.y;
+    if (!mg_act(C.mg_st[smp], mode)) return;
+    const int n = C.bn;
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int tile = smp / T, s = smp % T;
+    const size_t vb = (size_t)tile * C.tile_vec;
+    const real *b = (const real *)C.r + vb;
+    const MSA_GLOBAL double *X = gbl(C.binvd + (size_t)smp * n * n);
+    double acc = 0.0;
+    if (i < n) {
+        for (int jj = 0; jj < n; jj++)
+            acc += X[(size_t)jj * n + i] * (double)b[(size_t)C.bperm[jj] * T + s];
+        ((real *)C.z)[vb + (size_t)C.bperm[i] * T + s] = (real)acc;
+    }
+}
+
 // Coarsest level: its right-hand side (r) into band order, and the band
 // solution back into z.
 template <typename real, int T>
@@ -3495,6 +3560,29 @@ hipError_t launch_residual64_mf(hipStream_t s, const ClassDev *d_cls, const int4
 hipError_t launch_mf_khat(hipStream_t s, const ClassDev *cp)
 {
     hipLaunchKernelGGL(k_mf_khat, dim3(3), dim3(256), 0, s, cp);
+    return hipGetLastError();
+}
+
+hipError_t launch_band_inverse(hipStream_t s, const ClassDev *cp, int nsamp, int bn)
+{
+    if (bn <= 0) return hipSuccess;
+    if (bn > kDenseMax) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_band_inverse, dim3(bn, nsamp), dim3(64), 0, s, cp);
+    return hipGetLastError();
+}
+
+hipError_t launch_mg_dense_solve(hipStream_t s, bool mixed, int T, const ClassDev *cp, int nsamp,
+                                 int bn, int mode)
+{
+    if (bn <= 0) return hipSuccess;
+    const dim3 grid((bn + 63) / 64, nsamp);
+    if (mixed) {
+        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_mg_dense_solve<float, TT>), grid, dim3(64), 0, s, cp,
+                                             mode));
+    } else {
+        MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_mg_dense_solve<double, TT>), grid, dim3(64), 0, s, cp,
+                                             mode));
+    }
     return hipGetLastError();
 }
 

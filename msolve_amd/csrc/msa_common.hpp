@@ -161,6 +161,7 @@ struct ClassDev {
     double *by;           // [nsamp][bn] right-hand side / solution in band order
     double *bpart;        // [nsamp][kBandChunks][64] partial sums of the backward solve
     int *bfail;           // [nsamp] non-positive pivot met
+    double *binvd;        // coarsest multigrid level: [nsamp][bn][bn] dense inverse, band order
     // geometric multigrid preconditioner (msa_batch_set_preconditioner).
     // Every class of a hierarchy is an entry of the class table; mg_* are
     // table indices (-1: none).  root: the solved class whose sample state
@@ -188,6 +189,7 @@ struct ClassDev {
 
 constexpr int kBandNB = 64;        // block size of the band Cholesky
 constexpr int kBandChunkRows = 512;  // rows per partial of the backward solve
+constexpr int kDenseMax = 256;       // coarsest multigrid levels up to this size use a dense inverse
 
 // Newmark coefficients (NewmarkDynamicAnalyzer.cs:53-77) and the load history.
 struct DynCoef {
@@ -324,6 +326,10 @@ hipError_t launch_mg_band_out(hipStream_t s, bool mixed, int T, const ClassDev *
 hipError_t launch_mf_khat(hipStream_t s, const ClassDev *cp);
 // replacement residual of the due samples of matrix-free classes (T = 1)
 hipError_t launch_residual64_mf(hipStream_t s, const ClassDev *d_cls, const int4 *items, int nitems);
+// coarsest multigrid level: dense inverse from the band factor, and its use
+hipError_t launch_band_inverse(hipStream_t s, const ClassDev *cp, int nsamp, int bn);
+hipError_t launch_mg_dense_solve(hipStream_t s, bool mixed, int T, const ClassDev *cp, int nsamp,
+                                 int bn, int mode);
 hipError_t launch_mg_load(hipStream_t s, bool mixed, int T, const ClassDev *cp, const double *src,
                           size_t nrows, int tile, int lane);
 hipError_t launch_band_scatter_any(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
