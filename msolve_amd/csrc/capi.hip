@@ -734,6 +734,12 @@ int mf_setup(msa_batch *b, ClassHost &C)
     D.mf_ny = g.ny;
     D.mf_nz = g.nz;
     for (int o = 0; o < 8; o++) D.mf_corner[o] = g.corner_node[o];
+    D.mf_cell_id = g.cell_identity;
+    D.mf_box = g.box;
+    for (int d = 0; d < 3; d++) {
+        D.mf_box_lo[d] = g.box_lo[d];
+        D.mf_box_hi[d] = g.box_hi[d];
+    }
     D.mf = 1;
     return MSA_OK;
 }

@@ -2466,16 +2466,36 @@ __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL f
     // + (dz-1) nx ny; outside the grid it reads the own point and is masked
     // (as is a constrained node: block -1).
     const bool lo[3] = {i > 0, j > 0, k > 0}, hi[3] = {i < nx - 1, j < ny - 1, k < nz - 1};
-    const int gp0 = i + nx * (j + ny * k), sy = nx, sz = nx * ny;
     int nbv[27];
+    if (__builtin_amdgcn_readfirstlane(C.mf_box)) {
+        // blocks = a lexicographic sub-box of the grid: neighbour block =
+        // own block + offset, no table lookup
+        const int b0x = __builtin_amdgcn_readfirstlane(C.mf_box_lo[0]);
+        const int b0y = __builtin_amdgcn_readfirstlane(C.mf_box_lo[1]);
+        const int b0z = __builtin_amdgcn_readfirstlane(C.mf_box_lo[2]);
+        const int b1x = __builtin_amdgcn_readfirstlane(C.mf_box_hi[0]);
+        const int b1y = __builtin_amdgcn_readfirstlane(C.mf_box_hi[1]);
+        const int b1z = __builtin_amdgcn_readfirstlane(C.mf_box_hi[2]);
+        const bool blo[3] = {i > b0x, j > b0y, k > b0z}, bhi[3] = {i < b1x, j < b1y, k < b1z};
+        const int sy = b1x - b0x + 1, sz = sy * (b1y - b0y + 1);
 #pragma unroll
-    for (int n = 0; n < 27; n++) {
-        const int dx = n % 3, dy = (n / 3) % 3, dz = n / 9;
-        const bool in = (dx != 0 || lo[0]) && (dx != 2 || hi[0]) && (dy != 0 || lo[1]) &&
-                        (dy != 2 || hi[1]) && (dz != 0 || lo[2]) && (dz != 2 || hi[2]);
-        const int gp = gp0 + (dx - 1) + (dy - 1) * sy + (dz - 1) * sz;
-        const int nb = gblock[in ? gp : gp0];
-        nbv[n] = in ? nb : -1;
+        for (int n = 0; n < 27; n++) {
+            const int dx = n % 3, dy = (n / 3) % 3, dz = n / 9;
+            const bool in = (dx != 0 || blo[0]) && (dx != 2 || bhi[0]) && (dy != 0 || blo[1]) &&
+                            (dy != 2 || bhi[1]) && (dz != 0 || blo[2]) && (dz != 2 || bhi[2]);
+            nbv[n] = in ? br + (dx - 1) + (dy - 1) * sy + (dz - 1) * sz : -1;
+        }
+    } else {
+        const int gp0 = i + nx * (j + ny * k), sy = nx, sz = nx * ny;
+#pragma unroll
+        for (int n = 0; n < 27; n++) {
+            const int dx = n % 3, dy = (n / 3) % 3, dz = n / 9;
+            const bool in = (dx != 0 || lo[0]) && (dx != 2 || hi[0]) && (dy != 0 || lo[1]) &&
+                            (dy != 2 || hi[1]) && (dz != 0 || lo[2]) && (dz != 2 || hi[2]);
+            const int gp = gp0 + (dx - 1) + (dy - 1) * sy + (dz - 1) * sz;
+            const int nb = gblock[in ? gp : gp0];
+            nbv[n] = in ? nb : -1;
+        }
     }
     float xv[27][3];
 #pragma unroll
@@ -2509,7 +2529,11 @@ __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL f
     for (int c = 0; c < 8; c++) {
         const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
         cin[c] = (cx ? hi[0] : lo[0]) && (cy ? hi[1] : lo[1]) && (cz ? hi[2] : lo[2]);
-        ce[c] = cell[cin[c] ? c0 + cx + cy * cy1 + cz * cz1 : 0];
+        ce[c] = cin[c] ? c0 + cx + cy * cy1 + cz * cz1 : 0;
+    }
+    if (!__builtin_amdgcn_readfirstlane(C.mf_cell_id)) {  // element numbers from the table
+#pragma unroll
+        for (int c = 0; c < 8; c++) ce[c] = cell[ce[c]];
     }
     float Ec[8];
 #pragma unroll

@@ -500,6 +500,37 @@ int grid_info(const MeshHost &m, GridInfo &gi_out)
         seen |= 1 << pattern[a];
     }
     if (seen != 255) return MSA_ERR_ARG;
+    // element number = cell number?
+    g.cell_identity = 1;
+    for (size_t c = 0; c < g.cell_elem.size(); c++)
+        if (g.cell_elem[c] != (int)c) {
+            g.cell_identity = 0;
+            break;
+        }
+    // free nodes = a sub-box of the grid, blocks numbered lexicographically in it?
+    int blo[3] = {nx, ny, nz}, bhi[3] = {-1, -1, -1};
+    for (int b = 0; b < m.nb; b++) {
+        const int p[3] = {g.bpos[b] & 1023, (g.bpos[b] >> 10) & 1023, g.bpos[b] >> 20};
+        for (int d = 0; d < 3; d++) {
+            blo[d] = std::min(blo[d], p[d]);
+            bhi[d] = std::max(bhi[d], p[d]);
+        }
+    }
+    g.box = m.nb > 0;
+    const int bx = bhi[0] - blo[0] + 1, by = bhi[1] - blo[1] + 1;
+    for (int k = 0; k < nz && g.box; k++)
+        for (int j = 0; j < ny && g.box; j++)
+            for (int i = 0; i < nx && g.box; i++) {
+                const bool inside = i >= blo[0] && i <= bhi[0] && j >= blo[1] && j <= bhi[1] &&
+                                    k >= blo[2] && k <= bhi[2];
+                const int want =
+                    inside ? (i - blo[0]) + bx * ((j - blo[1]) + by * (k - blo[2])) : -1;
+                if (g.gblock[i + (size_t)nx * (j + (size_t)ny * k)] != want) g.box = 0;
+            }
+    for (int d = 0; d < 3; d++) {
+        g.box_lo[d] = blo[d];
+        g.box_hi[d] = bhi[d];
+    }
     gi_out = std::move(g);
     return MSA_OK;
 }

@@ -110,6 +110,9 @@ struct GridInfo {
     int nx = 0, ny = 0, nz = 0;
     std::vector<int> gblock, cell_elem, bpos;
     int corner_node[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int cell_identity = 0;  // element number == cell number
+    int box = 0;            // the blocks are exactly the grid points of [box_lo, box_hi],
+    int box_lo[3] = {0, 0, 0}, box_hi[3] = {0, 0, 0};  // numbered lexicographically
 };
 int grid_info(const MeshHost &m, GridInfo &g);
 
@@ -182,6 +185,10 @@ struct ClassDev {
     // unit element matrix mf_k[576] (+ unit mass mf_k[576 + 64])
     int mf, mf_nx, mf_ny, mf_nz;
     int mf_corner[8];
+    // index shortcuts: element = cell (mf_cell_id); blocks = a lexicographic
+    // sub-box of the grid (mf_box; neighbour block = own block + offset)
+    int mf_cell_id, mf_box;
+    int mf_box_lo[3], mf_box_hi[3];
     const int *mf_gblock, *mf_cell, *mf_bpos;
     float *mf_k;
     double *mf_k64;       // [640] the same in FP64 (replacement residuals)
