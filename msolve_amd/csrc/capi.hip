@@ -820,6 +820,21 @@ int mg_build(msa_batch *b, int max_levels)
             parent = ci;
         }
     }
+    // structured transfers where both sides are uniform box grids of fully
+    // free blocks
+    auto all_free = [](const MeshHost &m) {
+        for (int bb = 0; bb < m.nb; bb++)
+            if (m.node_free[m.block_node[bb]] != 7) return false;
+        return true;
+    };
+    for (auto &L : b->lv) {
+        ClassHost &P = tab(b, L.dev.mg_fine);
+        if (L.dev.mf && P.dev.mf && L.dev.mf_box && P.dev.mf_box && all_free(L.mesh->h) &&
+            all_free(P.mesh->h)) {
+            L.dev.mg_rstruct = 1;
+            P.dev.mg_pstruct = 1;
+        }
+    }
     b->mg_depth = (int)depth_classes.size() - 1;
     b->mg_smooth.assign(b->mg_depth, nullptr);
     b->mg_nsmooth.assign(b->mg_depth, 0);

@@ -2783,10 +2783,32 @@ __global__ __launch_bounds__(256) void k_mg_restrict(const ClassDev *__restrict_
     const MSA_GLOBAL uint8_t *ffree = gbl(F.block_free);
     const int width = C.mg_r_width, nbc = C.nb;
     const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+    const bool st = __builtin_amdgcn_readfirstlane(C.mg_rstruct);
     for (int slice = s0 + wave; slice < s1; slice += kWaves) {
         const int cb = slice * R + rr;
         if (cb >= C.nb) continue;
         double acc[3] = {0, 0, 0};
+        if (st) {
+            // fine points 2I + d, d in {-1, 0, 1} per direction, weight 1/2
+            // per odd offset; fine blocks = the fine box, lexicographic
+            const int g = C.mf_bpos[cb];
+            const int fi = 2 * (g & 1023), fj = 2 * ((g >> 10) & 1023), fk = 2 * (g >> 20);
+            const int lx = F.mf_box_lo[0], ly = F.mf_box_lo[1], lz = F.mf_box_lo[2];
+            const int hx = F.mf_box_hi[0], hy = F.mf_box_hi[1], hz = F.mf_box_hi[2];
+            const int sy = hx - lx + 1, sz = sy * (hy - ly + 1);
+#pragma unroll
+            for (int n = 0; n < 27; n++) {
+                const int dx = n % 3 - 1, dy = (n / 3) % 3 - 1, dz = n / 9 - 1;
+                const int x = fi + dx, y = fj + dy, z = fk + dz;
+                if (x < lx || x > hx || y < ly || y > hy || z < lz || z > hz) continue;
+                const double w = (dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0);
+                real v0, v1, v2;
+                gather3<real, T>(Bf, (x - lx) + sy * (y - ly) + sz * (z - lz), s, v0, v1, v2);
+                acc[0] += w * (double)v0;
+                acc[1] += w * (double)v1;
+                acc[2] += w * (double)v2;
+            }
+        } else
 #pragma unroll 3
         for (int k = 0; k < width; k++) {
             const int fb = rcol[(size_t)k * nbc + cb];
@@ -2833,10 +2855,33 @@ __global__ __launch_bounds__(256) void k_mg_prolong(const ClassDev *__restrict__
     const MSA_GLOBAL float *pw = gbl(C.mg_p_w);
     const int width = C.mg_p_width, nbf = C.nb;
     const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+    const bool st = __builtin_amdgcn_readfirstlane(C.mg_pstruct);
     for (int slice = s0 + wave; slice < s1; slice += kWaves) {
         const int fb = slice * R + rr;
         if (fb >= C.nb) continue;
         double acc[3] = {0, 0, 0};
+        if (st) {
+            // coarse points: i/2 for even i, (i-1)/2 and (i+1)/2 with weight
+            // 1/2 for odd i (per direction); coarse blocks = the coarse box
+            const int g = C.mf_bpos[fb];
+            const int i = g & 1023, j = (g >> 10) & 1023, k2 = g >> 20;
+            const int lx = K.mf_box_lo[0], ly = K.mf_box_lo[1], lz = K.mf_box_lo[2];
+            const int hx = K.mf_box_hi[0], hy = K.mf_box_hi[1], hz = K.mf_box_hi[2];
+            const int sy = hx - lx + 1, sz = sy * (hy - ly + 1);
+#pragma unroll
+            for (int n = 0; n < 8; n++) {
+                const int ox = n & 1, oy = (n >> 1) & 1, oz = n >> 2;
+                if ((ox && !(i & 1)) || (oy && !(j & 1)) || (oz && !(k2 & 1))) continue;
+                const int x = (i >> 1) + ox, y = (j >> 1) + oy, z = (k2 >> 1) + oz;
+                if (x < lx || x > hx || y < ly || y > hy || z < lz || z > hz) continue;
+                const double w = ((i & 1) ? 0.5 : 1.0) * ((j & 1) ? 0.5 : 1.0) * ((k2 & 1) ? 0.5 : 1.0);
+                real v0, v1, v2;
+                gather3<real, T>(zc, (x - lx) + sy * (y - ly) + sz * (z - lz), s, v0, v1, v2);
+                acc[0] += w * v0;
+                acc[1] += w * v1;
+                acc[2] += w * v2;
+            }
+        } else
 #pragma unroll 4
         for (int k = 0; k < width; k++) {
             const int cb = pcol[(size_t)k * nbf + fb];

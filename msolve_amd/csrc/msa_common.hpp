@@ -180,6 +180,10 @@ struct ClassDev {
     const int *mg_p_col, *mg_r_col;
     const float *mg_p_w, *mg_r_w;
     const int *mg_child;  // [n_elem*8] elements of the finer level in each element
+    // transfers from the grid structure instead of the ELL lists (uniform
+    // box grids on both sides, every block fully free): prolongation of
+    // this class (mg_pstruct) / restriction into it (mg_rstruct)
+    int mg_pstruct, mg_rstruct;
     // matrix-free smoothing products (mixed precision, one-sample tiles) on
     // a uniform structured grid: K v from the cell moduli and the shared
     // unit element matrix mf_k[576] (+ unit mass mf_k[576 + 64])
