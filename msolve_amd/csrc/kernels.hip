@@ -2443,12 +2443,36 @@ __device__ __forceinline__ void row_product(const MSA_GLOBAL real *val, const MS
     }
 }
 
+// Range-checked loads: a raw buffer descriptor (stride 0) over `bytes`
+// bytes at p; a load at a byte offset >= bytes returns zeros (the buffer
+// range check of the CDNA memory pipeline), so a missing neighbour is read
+// as zero without a select.  p and bytes must be wave-uniform.
+// (the typed LLVM intrinsics are bound directly: the clang builtins for
+// 64/96/128-bit buffer loads of this toolchain return a single dword)
+typedef float f32x3 __attribute__((ext_vector_type(3)));
+__device__ f32x3 buffer_load_f32x3(__amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset,
+                                   int aux) __asm("llvm.amdgcn.raw.ptr.buffer.load.v3f32");
+__device__ double buffer_load_f64(__amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset,
+                                  int aux) __asm("llvm.amdgcn.raw.ptr.buffer.load.f64");
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t wave_buffer(const void *p, unsigned bytes)
+{
+    const unsigned long long a = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+    void *q = (void *)(((unsigned long long)hi << 32) | lo);
+    // dword 3 of the descriptor: 32-bit data format (gfx9 family)
+    return __builtin_amdgcn_make_buffer_rsrc(q, (short)0, (int)__builtin_amdgcn_readfirstlane(bytes),
+                                             0x00020000);
+}
+
 // y = K v for block row br of a uniform structured grid class, from the
 // moduli of the (up to) 8 cells around the node and the shared unit element
 // matrix (FP32, stored in corner order): y = sum_cells cK E_c Khat[a][b] v_b
 // (+ cM Mhat[a][b] v_b on the component itself), a = the node's corner in
 // the cell, b over the cell's corners.  The 27 neighbouring blocks are
-// gathered once.
+// gathered once; neighbours outside the grid or constrained (block -1) and
+// cells outside the grid are read at an offset past the end of their buffer
+// and come back as zeros.
 __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL float *v,
                                            const double *Es, int br, float &y0, float &y1,
                                            float &y2)
@@ -2458,15 +2482,17 @@ __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL f
     const int nx = __builtin_amdgcn_readfirstlane(C.mf_nx);
     const int ny = __builtin_amdgcn_readfirstlane(C.mf_ny);
     const int nz = __builtin_amdgcn_readfirstlane(C.mf_nz);
+    const int nb = __builtin_amdgcn_readfirstlane(C.nb);
+    const int ne = __builtin_amdgcn_readfirstlane(C.n_elem);
     const MSA_GLOBAL int *gblock = gbl(C.mf_gblock);
     const MSA_GLOBAL int *cell = gbl(C.mf_cell);
-    const MSA_GLOBAL double *E = gbl(Es);
-    // 27 neighbours, branch-free so that every load is in flight at once.
-    // Grid point of neighbour (dx, dy, dz) = own point + (dx-1) + (dy-1) nx
-    // + (dz-1) nx ny; outside the grid it reads the own point and is masked
-    // (as is a constrained node: block -1).
+    const __amdgpu_buffer_rsrc_t vb = wave_buffer((const void *)v, (unsigned)nb * 12u);
+    const __amdgpu_buffer_rsrc_t eb = wave_buffer((const void *)Es, (unsigned)ne * 8u);
+    const unsigned vout = (unsigned)nb * 12u, eout = (unsigned)ne * 8u;
+    // 27 neighbours: byte offsets, every load in flight at once.  Grid point
+    // of neighbour (dx, dy, dz) = own point + (dx-1) + (dy-1) nx + (dz-1) nx ny
     const bool lo[3] = {i > 0, j > 0, k > 0}, hi[3] = {i < nx - 1, j < ny - 1, k < nz - 1};
-    int nbv[27];
+    unsigned off[27];
     if (__builtin_amdgcn_readfirstlane(C.mf_box)) {
         // blocks = a lexicographic sub-box of the grid: neighbour block =
         // own block + offset, no table lookup
@@ -2478,12 +2504,14 @@ __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL f
         const int b1z = __builtin_amdgcn_readfirstlane(C.mf_box_hi[2]);
         const bool blo[3] = {i > b0x, j > b0y, k > b0z}, bhi[3] = {i < b1x, j < b1y, k < b1z};
         const int sy = b1x - b0x + 1, sz = sy * (b1y - b0y + 1);
+        const unsigned own = (unsigned)br * 12u;
 #pragma unroll
         for (int n = 0; n < 27; n++) {
             const int dx = n % 3, dy = (n / 3) % 3, dz = n / 9;
             const bool in = (dx != 0 || blo[0]) && (dx != 2 || bhi[0]) && (dy != 0 || blo[1]) &&
                             (dy != 2 || bhi[1]) && (dz != 0 || blo[2]) && (dz != 2 || bhi[2]);
-            nbv[n] = in ? br + (dx - 1) + (dy - 1) * sy + (dz - 1) * sz : -1;
+            const int rel = 12 * ((dx - 1) + (dy - 1) * sy + (dz - 1) * sz);  // wave-uniform
+            off[n] = in ? own + (unsigned)rel : vout;
         }
     } else {
         const int gp0 = i + nx * (j + ny * k), sy = nx, sz = nx * ny;
@@ -2493,18 +2521,17 @@ __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL f
             const bool in = (dx != 0 || lo[0]) && (dx != 2 || hi[0]) && (dy != 0 || lo[1]) &&
                             (dy != 2 || hi[1]) && (dz != 0 || lo[2]) && (dz != 2 || hi[2]);
             const int gp = gp0 + (dx - 1) + (dy - 1) * sy + (dz - 1) * sz;
-            const int nb = gblock[in ? gp : gp0];
-            nbv[n] = in ? nb : -1;
+            const int b = gblock[in ? gp : gp0];
+            off[n] = (in && b >= 0) ? (unsigned)b * 12u : vout;
         }
     }
     float xv[27][3];
 #pragma unroll
     for (int n = 0; n < 27; n++) {
-        const bool ok = nbv[n] >= 0;
-        const vec3<float> c = ld3(v, ok ? nbv[n] : 0);
-        xv[n][0] = ok ? c.x : 0.f;
-        xv[n][1] = ok ? c.y : 0.f;
-        xv[n][2] = ok ? c.z : 0.f;
+        const f32x3 w = buffer_load_f32x3(vb, (int)off[n], 0, 0);
+        xv[n][0] = w.x;
+        xv[n][1] = w.y;
+        xv[n][2] = w.z;
     }
     const float cK = C.dyn ? (float)C.cK : 1.f, cM = C.dyn ? (float)C.cM : 0.f;
     // the element matrix is the same for every lane: constant-address loads
@@ -2516,11 +2543,9 @@ __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL f
     unsigned long long kp = (unsigned long long)C.mf_k;
     asm volatile("" : "+s"(kp));
     cfloat *kh = (cfloat *)kp, *mh = (cfloat *)kp + 576;
-    // the 8 cells around the node (c = cx + 2 cy + 4 cz): every index and
-    // modulus load issued before the arithmetic; cells outside the grid
-    // read cell 0 and get modulus 0
-    // cell (cx, cy, cz) around the node: cells i-1+cx, ... exist when cx = 0
-    // and i > 0, or cx = 1 and i < nx - 1 (likewise in y, z)
+    // the 8 cells around the node (c = cx + 2 cy + 4 cz): cell (cx, cy, cz)
+    // exists when cx = 0 and i > 0, or cx = 1 and i < nx - 1 (likewise in
+    // y, z); missing cells read modulus zero
     const int cy1 = nx - 1, cz1 = (nx - 1) * (ny - 1);
     const int c0 = (i - 1) + cy1 * (j - 1) + cz1 * (k - 1);
     int ce[8];
@@ -2537,7 +2562,10 @@ __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL f
     }
     float Ec[8];
 #pragma unroll
-    for (int c = 0; c < 8; c++) Ec[c] = cin[c] ? (float)E[ce[c]] * cK : 0.f;
+    for (int c = 0; c < 8; c++) {
+        const double e = buffer_load_f64(eb, (int)(cin[c] ? (unsigned)ce[c] * 8u : eout), 0, 0);
+        Ec[c] = (float)e * cK;
+    }
     float acc[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int c = 0; c < 8; c++) {
@@ -2549,12 +2577,16 @@ __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL f
             const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
             cfloat *kab = kh + (a * 8 + o) * 9;
             const float *xb = xv[((cz + oz) * 3 + (cy + oy)) * 3 + (cx + ox)];
+            // one FMA per term (no product-sum trees)
 #pragma unroll
-            for (int p = 0; p < 3; p++)
-                t[p] += kab[3 * p] * xb[0] + kab[3 * p + 1] * xb[1] + kab[3 * p + 2] * xb[2];
+            for (int p = 0; p < 3; p++) {
+                t[p] = __builtin_fmaf(kab[3 * p], xb[0], t[p]);
+                t[p] = __builtin_fmaf(kab[3 * p + 1], xb[1], t[p]);
+                t[p] = __builtin_fmaf(kab[3 * p + 2], xb[2], t[p]);
+            }
         }
 #pragma unroll
-        for (int p = 0; p < 3; p++) acc[p] += Ec[c] * t[p];
+        for (int p = 0; p < 3; p++) acc[p] = __builtin_fmaf(Ec[c], t[p], acc[p]);
         // keep each cell's 72 scalar loads next to their use
         __builtin_amdgcn_sched_barrier(0);
     }
@@ -2570,11 +2602,11 @@ __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL f
                 const float mab = mh[a * 8 + o];
                 const float *xb = xv[((cz + oz) * 3 + (cy + oy)) * 3 + (cx + ox)];
 #pragma unroll
-                for (int p = 0; p < 3; p++) m[p] += mab * xb[p];
+                for (int p = 0; p < 3; p++) m[p] = __builtin_fmaf(mab, xb[p], m[p]);
             }
             const float w = cin[c] ? cM : 0.f;
 #pragma unroll
-            for (int p = 0; p < 3; p++) acc[p] += w * m[p];
+            for (int p = 0; p < 3; p++) acc[p] = __builtin_fmaf(w, m[p], acc[p]);
         }
     }
     y0 = acc[0];
