@@ -611,6 +611,27 @@ int prepare_band(msa_batch *b)
     return MSA_OK;
 }
 
+// Forward and backward sweeps with the band factor of every sample (in
+// place on by): one launch for bands of at most kBandOneMax rows, else two
+// launch pairs per 64-column block.  mode >= 0: coarsest multigrid level,
+// samples selected by the cycle's mode.
+int band_sweeps(msa_batch *b, const ClassDev *cp, int n, int W, int ns, int mode)
+{
+    hipStream_t s = b->ctx->stream;
+    if (n <= kBandOneMax)
+        return timed(b, KK_OTHER, [&] { return launch_band_solve_one(s, cp, ns, n, mode); });
+    for (int k0 = 0; k0 < n; k0 += kBandNB) {
+        const int nk = std::min(kBandNB, n - k0), r0 = k0 + nk, m = std::min(n - r0, W - kBandNB);
+        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_band_fwd(s, cp, ns, k0, nk, r0, m, mode); }));
+    }
+    const int last = ((n - 1) / kBandNB) * kBandNB;
+    for (int k0 = last; k0 >= 0; k0 -= kBandNB) {
+        const int nk = std::min(kBandNB, n - k0), r0 = k0 + nk, m = std::min(n - r0, W - kBandNB);
+        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_band_bwd(s, cp, ns, k0, nk, r0, m, mode); }));
+    }
+    return MSA_OK;
+}
+
 // Forward and backward band sweeps of class c for every sample, then the
 // solution into the PCG solution vector x.
 int band_solve_class(msa_batch *b, size_t c, bool from_fs)
@@ -620,15 +641,7 @@ int band_solve_class(msa_batch *b, size_t c, bool from_fs)
     const ClassDev *cp = b->d_cls + c;
     const int n = C.dev.bn, W = C.dev.bW, ns = C.nsamp;
     MSA_TRY(timed(b, KK_OTHER, [&] { return launch_band_rhs(s, b->T, cp, ns, n, from_fs ? 1 : 0); }));
-    for (int k0 = 0; k0 < n; k0 += kBandNB) {
-        const int nk = std::min(kBandNB, n - k0), r0 = k0 + nk, m = std::min(n - r0, W - kBandNB);
-        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_band_fwd(s, cp, ns, k0, nk, r0, m); }));
-    }
-    const int last = ((n - 1) / kBandNB) * kBandNB;
-    for (int k0 = last; k0 >= 0; k0 -= kBandNB) {
-        const int nk = std::min(kBandNB, n - k0), r0 = k0 + nk, m = std::min(n - r0, W - kBandNB);
-        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_band_bwd(s, cp, ns, k0, nk, r0, m); }));
-    }
+    MSA_TRY(band_sweeps(b, cp, n, W, ns, -1));
     MSA_TRY(timed(b, KK_OTHER, [&] { return launch_band_out(s, b->T, cp, ns, 3 * C.dev.nb); }));
     return MSA_OK;
 }
@@ -1039,15 +1052,7 @@ int mg_vcycle(msa_batch *b, int pbuf, int mode)
             continue;
         }
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_mg_band_in(s, mx, T, cp, ns, n, mode); }));
-        for (int k0 = 0; k0 < n; k0 += kBandNB) {
-            const int nk = std::min(kBandNB, n - k0), r0 = k0 + nk, m = std::min(n - r0, W - kBandNB);
-            MSA_TRY(timed(b, KK_OTHER, [&] { return launch_band_fwd(s, cp, ns, k0, nk, r0, m, mode); }));
-        }
-        const int last = ((n - 1) / kBandNB) * kBandNB;
-        for (int k0 = last; k0 >= 0; k0 -= kBandNB) {
-            const int nk = std::min(kBandNB, n - k0), r0 = k0 + nk, m = std::min(n - r0, W - kBandNB);
-            MSA_TRY(timed(b, KK_OTHER, [&] { return launch_band_bwd(s, cp, ns, k0, nk, r0, m, mode); }));
-        }
+        MSA_TRY(band_sweeps(b, cp, n, W, ns, mode));
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_mg_band_out(s, mx, T, cp, ns, 3 * C.dev.nb, mode); }));
     }
     for (int d = b->mg_depth - 1; d >= 0; d--) {

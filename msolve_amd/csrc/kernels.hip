@@ -2028,6 +2028,94 @@ __global__ __launch_bounds__(64) void k_band_bwd(const ClassDev *__restrict__ cp
     if (t < nk) x[k0 + t] = xb[t];
 }
 
+// Both band sweeps (L y = b, then L^T x = y) of one sample in one
+// workgroup, for bands of at most kBandOneMax rows: the same 64-column
+// blocks and the same order of operations as k_band_fwd / k_band_bwd (so the
+// same results), without a launch per block.  y lives in LDS; each block's
+// diagonal triangle is staged in LDS (rows padded to 65 against bank
+// conflicts) and solved by wave 0 with the right-hand side in registers.
+__device__ __forceinline__ double readlane_f64(double v, int l)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, l);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), l);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+__global__ __launch_bounds__(256) void k_band_solve_one(const ClassDev *__restrict__ cp, int mode)
+{
+    __shared__ double yl[kBandOneMax];
+    __shared__ double Ld[kBandNB][kBandNB + 1];
+    __shared__ double part[kBandNB];
+    const ClassDev &C = *cp;
+    const int smp = blockIdx.x;
+    if (mode >= 0 && !mg_act(C.mg_st[smp], mode)) return;  // coarsest multigrid level
+    const int n = C.bn, W = C.bW;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const double *AB = C.band + (size_t)smp * n * W;
+    double *y = C.by + (size_t)smp * n;
+    for (int i = t; i < n; i += 256) yl[i] = y[i];
+    // Ld[c][j] = AB[(k0 + c) W + j] = L[k0 + c + j][k0 + c], j < nk - c
+    auto stage = [&](int k0, int nk) {
+        for (int e = t; e < kBandNB * kBandNB; e += 256) {
+            const int c = e >> 6, j = e & 63;
+            Ld[c][j] = (c < nk && j < nk - c) ? AB[(size_t)(k0 + c) * W + j] : 0.0;
+        }
+    };
+    __syncthreads();
+    for (int k0 = 0; k0 < n; k0 += kBandNB) {
+        const int nk = min(kBandNB, n - k0), r0 = k0 + nk, m = min(n - r0, W - kBandNB);
+        stage(k0, nk);
+        __syncthreads();
+        if (wave == 0) {
+            double v = lane < nk ? yl[k0 + lane] : 0.0;
+            for (int c = 0; c < nk; c++) {
+                const double yc = readlane_f64(v, c) / Ld[c][0];
+                if (lane == c) v = yc;
+                if (lane > c && lane < nk) v -= Ld[c][lane - c] * yc;
+            }
+            if (lane < nk) yl[k0 + lane] = v;
+        }
+        __syncthreads();
+        for (int i = r0 + t; i < r0 + m; i += 256) {
+            double acc = 0.0;
+            for (int c = 0; c < nk; c++) acc += AB[(size_t)(k0 + c) * W + (i - k0 - c)] * yl[k0 + c];
+            yl[i] -= acc;
+        }
+        __syncthreads();
+    }
+    const int last = ((n - 1) / kBandNB) * kBandNB;
+    for (int k0 = last; k0 >= 0; k0 -= kBandNB) {
+        const int nk = min(kBandNB, n - k0), r0 = k0 + nk, m = min(n - r0, W - kBandNB);
+        stage(k0, nk);
+        // column partials of L21^T x in chunks of kBandChunkRows rows,
+        // subtracted chunk by chunk (the order of k_band_bwd)
+        if (wave == 1 && lane < nk) {
+            double v = yl[k0 + lane];
+            for (int i0 = r0; i0 < r0 + m; i0 += kBandChunkRows) {
+                const int i1 = min(r0 + m, i0 + kBandChunkRows);
+                double acc = 0.0;
+                for (int i = i0; i < i1; i++)
+                    acc += AB[(size_t)(k0 + lane) * W + (i - k0 - lane)] * yl[i];
+                v -= acc;
+            }
+            part[lane] = v;
+        }
+        __syncthreads();
+        if (wave == 0) {
+            double v = lane < nk ? part[lane] : 0.0;
+            for (int c = nk - 1; c >= 0; c--) {
+                const double xc = readlane_f64(v, c) / Ld[c][0];
+                if (lane == c) v = xc;
+                if (lane < c) v -= Ld[lane][c - lane] * xc;
+            }
+            if (lane < nk) yl[k0 + lane] = v;
+        }
+        __syncthreads();
+    }
+    for (int i = t; i < n; i += 256) y[i] = yl[i];
+}
+
 // Solution back to the internal rows: x[row] = by[band(row)], constrained
 // components 0 (FP64 batches: the PCG solution vector).
 template <int T>
@@ -3447,6 +3535,14 @@ hipError_t launch_band_bwd(hipStream_t s, const ClassDev *cp, int nsamp, int k0,
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(k_band_bwd, dim3(1, nsamp), dim3(64), 0, s, cp, k0, nk, r0, m, 1, mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_band_solve_one(hipStream_t s, const ClassDev *cp, int nsamp, int bn, int mode)
+{
+    if (bn <= 0) return hipSuccess;
+    if (bn > kBandOneMax) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_band_solve_one, dim3(nsamp), dim3(256), 0, s, cp, mode);
     return hipGetLastError();
 }
 

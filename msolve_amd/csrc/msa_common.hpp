@@ -201,6 +201,7 @@ struct ClassDev {
 constexpr int kBandNB = 64;        // block size of the band Cholesky
 constexpr int kBandChunkRows = 512;  // rows per partial of the backward solve
 constexpr int kDenseMax = 256;       // coarsest multigrid levels up to this size use a dense inverse
+constexpr int kBandOneMax = 2048;    // bands up to this many rows: both sweeps in one launch
 
 // Newmark coefficients (NewmarkDynamicAnalyzer.cs:53-77) and the load history.
 struct DynCoef {
@@ -311,6 +312,7 @@ hipError_t launch_band_rhs(hipStream_t s, int T, const ClassDev *cp, int nsamp, 
 // mode >= 0: coarsest multigrid level, only the samples mg_act serves
 hipError_t launch_band_fwd(hipStream_t s, const ClassDev *cp, int nsamp, int k0, int nk, int r0,
                            int m, int mode = -1);
+hipError_t launch_band_solve_one(hipStream_t s, const ClassDev *cp, int nsamp, int bn, int mode);
 hipError_t launch_band_bwd(hipStream_t s, const ClassDev *cp, int nsamp, int k0, int nk, int r0,
                            int m, int mode = -1);
 hipError_t launch_band_out(hipStream_t s, int T, const ClassDev *cp, int nsamp, int nrows);
