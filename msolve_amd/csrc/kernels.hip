@@ -624,6 +624,17 @@ __global__ __launch_bounds__(256) void k_assemble_gather(const ClassDev *__restr
     const MSA_GLOBAL double *__restrict__ kc = gbl((const double *)C.kc);
     const MSA_GLOBAL double *__restrict__ mc = gbl((const double *)C.mc);
     const double cK = C.dyn ? C.cK : 1.0, cM = C.dyn ? C.cM : 0.0;
+    // uniform structured grids (one element matrix for every element, up
+    // to rounding): element 0's Khat (and Mhat) from LDS instead of the
+    // per-element cache
+    __shared__ double kl[640];
+    const bool uni = C.mf;
+    if (uni) {
+        for (int q = threadIdx.x; q < 576; q += blockDim.x) kl[q] = kc[q];
+        if (cM != 0.0)
+            for (int q = threadIdx.x; q < 64; q += blockDim.x) kl[576 + q] = mc[q];
+        __syncthreads();
+    }
     const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
     for (int slice = s0 + wave; slice < s1; slice += kWaves) {
         const int br = slice * R + rr;
@@ -689,6 +700,17 @@ __global__ __launch_bounds__(256) void k_assemble_gather_g(const ClassDev *__res
     const MSA_GLOBAL double *__restrict__ kc = gbl((const double *)C.kc);
     const MSA_GLOBAL double *__restrict__ mc = gbl((const double *)C.mc);
     const double cK = C.dyn ? C.cK : 1.0, cM = C.dyn ? C.cM : 0.0;
+    // uniform structured grids (one element matrix for every element, up
+    // to rounding): element 0's Khat (and Mhat) from LDS instead of the
+    // per-element cache
+    __shared__ double kl[640];
+    const bool uni = C.mf;
+    if (uni) {
+        for (int q = threadIdx.x; q < 576; q += blockDim.x) kl[q] = kc[q];
+        if (cM != 0.0)
+            for (int q = threadIdx.x; q < 64; q += blockDim.x) kl[576 + q] = mc[q];
+        __syncthreads();
+    }
     const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
     for (int slice = s0 + wave; slice < s1; slice += kWaves) {
         const int br = slice * R + rr;
@@ -708,11 +730,17 @@ __global__ __launch_bounds__(256) void k_assemble_gather_g(const ClassDev *__res
                     const int code = gmap[((size_t)slot * 8 + gi) * R + rr];
                     if (code < 0) break;
                     const int e = code >> 6, ab = code & 63;
-                    const MSA_GLOBAL double *kp = kc + (size_t)e * 576 + ab * 9;
-                    double k[9];
+                    double k[9], m = 0.0;
+                    if (uni) {
 #pragma unroll
-                    for (int q = 0; q < 9; q++) k[q] = kp[q];
-                    const double m = cM != 0.0 ? cM * mc[(size_t)e * 64 + ab] : 0.0;
+                        for (int q = 0; q < 9; q++) k[q] = kl[ab * 9 + q];
+                        if (cM != 0.0) m = cM * kl[576 + ab];
+                    } else {
+                        const MSA_GLOBAL double *kp = kc + (size_t)e * 576 + ab * 9;
+#pragma unroll
+                        for (int q = 0; q < 9; q++) k[q] = kp[q];
+                        if (cM != 0.0) m = cM * mc[(size_t)e * 64 + ab];
+                    }
 #pragma unroll
                     for (int g = 0; g < G; g++) {
                         if (g >= nt) break;
@@ -2179,24 +2207,38 @@ __global__ void k_kl_basis(const ClassDev *__restrict__ cp, int n_elem, int M, i
 template <int T>
 __global__ void k_kl_eval(const ClassDev *__restrict__ cp, int ntiles, int n_elem, int M)
 {
+    // one thread: element e, lane s of kKlTiles consecutive tiles; the
+    // element's row of phi is read once for all of them
     const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t total = (size_t)ntiles * n_elem * T;
+    const int ngroups = (ntiles + kKlTiles - 1) / kKlTiles;
+    const size_t total = (size_t)ngroups * n_elem * T;
     if (id >= total) return;
     const ClassDev &C = *cp;
     const int s = (int)(id % T);
     const size_t te = id / T;
     const int e = (int)(te % n_elem);
-    const int tile = (int)(te / n_elem);
-    const int smp = tile * T + s;
-    if (smp >= C.nsamp) {
-        C.E[id] = 0.0;
-        return;
+    const int t0 = (int)(te / n_elem) * kKlTiles;
+    const int nt = min(kKlTiles, ntiles - t0);
+    double res[kKlTiles];
+    const double *xs[kKlTiles];
+#pragma unroll
+    for (int g = 0; g < kKlTiles; g++) {
+        res[g] = C.c0;
+        const int smp = min((t0 + g) * T + s, C.nsamp - 1);
+        xs[g] = C.xi + (size_t)smp * M;
     }
-    double res = C.c0;
-    const double *xs = C.xi + (size_t)smp * M;
     const double *ph = C.phi + (size_t)e * M;
-    for (int i = 0; i < M; i++) res += xs[i] * ph[i];
-    C.E[id] = exp(res);
+    for (int i = 0; i < M; i++) {
+        const double p = ph[i];
+#pragma unroll
+        for (int g = 0; g < kKlTiles; g++) res[g] += xs[g][i] * p;
+    }
+#pragma unroll
+    for (int g = 0; g < kKlTiles; g++) {
+        if (g >= nt) break;
+        const int smp = (t0 + g) * T + s;
+        C.E[((size_t)(t0 + g) * n_elem + e) * T + s] = smp < C.nsamp ? exp(res[g]) : 0.0;
+    }
 }
 
 
@@ -3634,7 +3676,7 @@ hipError_t launch_kl_basis(hipStream_t s, const ClassDev *d_cls, int n_elem, int
 hipError_t launch_kl_eval(hipStream_t s, int T, const ClassDev *d_cls, int ntiles, int n_elem,
                           int M)
 {
-    const size_t total = (size_t)ntiles * n_elem * T;
+    const size_t total = (size_t)((ntiles + kKlTiles - 1) / kKlTiles) * n_elem * T;
     const unsigned g = (unsigned)((total + 255) / 256);
     MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_kl_eval<TT>), dim3(g), dim3(256), 0, s, d_cls, ntiles,
                                          n_elem, M));

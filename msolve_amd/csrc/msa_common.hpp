@@ -202,6 +202,7 @@ constexpr int kBandNB = 64;        // block size of the band Cholesky
 constexpr int kBandChunkRows = 512;  // rows per partial of the backward solve
 constexpr int kDenseMax = 256;       // coarsest multigrid levels up to this size use a dense inverse
 constexpr int kBandOneMax = 2048;    // bands up to this many rows: both sweeps in one launch
+constexpr int kKlTiles = 8;          // tiles per thread of the KL field evaluation
 
 // Newmark coefficients (NewmarkDynamicAnalyzer.cs:53-77) and the load history.
 struct DynCoef {
