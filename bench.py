@@ -351,9 +351,9 @@ def main():
             flops_per_launch = mf[0] / launches
             avg_bytes = mf[1] / launches
             bound, unit, peak = "mfma", "TFLOP/s", FP32_PEAK_TFLOPS
-            kname = (f"k_mg_smooth<float,1,STEP,true> on the finest level: matrix-free FP32 "
-                     f"products (FP32 VALU, no MFMA; priced against the FP32 peak) of the "
-                     f"cycle's four steps PRE, RESID, POST1, POST2")
+            kname = (f"k_mg_smooth<float,1,STEP,2> on the finest level: matrix-free FP32 "
+                     f"products on box grids (FP32 VALU, no MFMA; priced against the FP32 peak) "
+                     f"of the cycle's four steps PRE, RESID, POST1, POST2")
         else:
             avg_bytes = smooth_bytes(bm, models, 4) / launches
             kname = (f"k_mg_smooth<float,{args.tile},STEP> on the finest level (the cycle's four "

@@ -109,8 +109,9 @@ struct msa_batch {
     int mg_depth = 0;                   // deepest level of any hierarchy
     std::vector<int4 *> mg_smooth;      // per depth: items of the classes that smooth there
     std::vector<int> mg_nsmooth;
-    std::vector<int4 *> mg_sm_a, mg_sm_f;  // the same split: stored-value / matrix-free products
-    std::vector<int> mg_nsm_a, mg_nsm_f;
+    std::vector<int4 *> mg_sm_a, mg_sm_f, mg_sm_b;  // the same split: stored-value / matrix-free
+                                                   // products on general / box grids
+    std::vector<int> mg_nsm_a, mg_nsm_f, mg_nsm_b;
     std::vector<int4 *> mg_restr;       // per depth >= 1: items of every level there
     std::vector<int> mg_nrestr;
     int4 *mg_lv_items = nullptr;        // every coarse level
@@ -854,12 +855,14 @@ int mg_build(msa_batch *b, int max_levels)
     b->mg_sm_a.assign(b->mg_depth, nullptr);
     b->mg_nsm_a.assign(b->mg_depth, 0);
     b->mg_sm_f.assign(b->mg_depth, nullptr);
+    b->mg_sm_b.assign(b->mg_depth, nullptr);
+    b->mg_nsm_b.assign(b->mg_depth, 0);
     b->mg_nsm_f.assign(b->mg_depth, 0);
     b->mg_restr.assign(b->mg_depth + 1, nullptr);
     b->mg_nrestr.assign(b->mg_depth + 1, 0);
     std::vector<int4> all, co;
     for (int d = 0; d <= b->mg_depth; d++) {
-        std::vector<int4> sm, sma, smf, re;
+        std::vector<int4> sm, sma, smf, smb, re;
         for (int ci : depth_classes[d]) {
             ClassHost &C = tab(b, ci);
             if (d > 0) {
@@ -868,7 +871,7 @@ int mg_build(msa_batch *b, int max_levels)
             }
             if (C.dev.mg_coarse >= 0) {
                 class_items(C, ci, sm);
-                class_items(C, ci, C.dev.mf ? smf : sma);
+                class_items(C, ci, !C.dev.mf ? sma : (C.dev.mf_box ? smb : smf));
             } else if (d > 0) {
                 class_items(C, ci, co);
             }
@@ -877,6 +880,7 @@ int mg_build(msa_batch *b, int max_levels)
             MSA_TRY(items_to_device(b, sm, &b->mg_smooth[d], &b->mg_nsmooth[d]));
             MSA_TRY(items_to_device(b, sma, &b->mg_sm_a[d], &b->mg_nsm_a[d]));
             MSA_TRY(items_to_device(b, smf, &b->mg_sm_f[d], &b->mg_nsm_f[d]));
+            MSA_TRY(items_to_device(b, smb, &b->mg_sm_b[d], &b->mg_nsm_b[d]));
         }
         if (d > 0) MSA_TRY(items_to_device(b, re, &b->mg_restr[d], &b->mg_nrestr[d]));
     }
@@ -1026,12 +1030,17 @@ int mg_vcycle(msa_batch *b, int pbuf, int mode)
         if (b->mg_nsm_a[d] > 0)
             MSA_TRY(timed(b, d == 0 ? KK_MG : KK_OTHER, [&] {
                 return launch_mg_smooth(s, mx, T, b->d_cls, b->mg_sm_a[d], b->mg_nsm_a[d], step,
-                                        pbuf, mode, ratio, false);
+                                        pbuf, mode, ratio, 0);
             }));
         if (b->mg_nsm_f[d] > 0)
             MSA_TRY(timed(b, d == 0 ? KK_MG : KK_OTHER, [&] {
                 return launch_mg_smooth(s, mx, T, b->d_cls, b->mg_sm_f[d], b->mg_nsm_f[d], step,
-                                        pbuf, mode, ratio, true);
+                                        pbuf, mode, ratio, 1);
+            }));
+        if (b->mg_nsm_b[d] > 0)
+            MSA_TRY(timed(b, d == 0 ? KK_MG : KK_OTHER, [&] {
+                return launch_mg_smooth(s, mx, T, b->d_cls, b->mg_sm_b[d], b->mg_nsm_b[d], step,
+                                        pbuf, mode, ratio, 2);
             }));
         return (int)MSA_OK;
     };

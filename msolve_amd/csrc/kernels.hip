@@ -2105,8 +2105,11 @@ __global__ __launch_bounds__(256) void k_band_solve_one(const ClassDev *__restri
             if (lane < nk) yl[k0 + lane] = v;
         }
         __syncthreads();
+        // (unrolled so that the loads of a group issue together; the sum
+        // keeps its order)
         for (int i = r0 + t; i < r0 + m; i += 256) {
             double acc = 0.0;
+#pragma unroll 16
             for (int c = 0; c < nk; c++) acc += AB[(size_t)(k0 + c) * W + (i - k0 - c)] * yl[k0 + c];
             yl[i] -= acc;
         }
@@ -2123,6 +2126,7 @@ __global__ __launch_bounds__(256) void k_band_solve_one(const ClassDev *__restri
             for (int i0 = r0; i0 < r0 + m; i0 += kBandChunkRows) {
                 const int i1 = min(r0 + m, i0 + kBandChunkRows);
                 double acc = 0.0;
+#pragma unroll 16
                 for (int i = i0; i < i1; i++)
                     acc += AB[(size_t)(k0 + lane) * W + (i - k0 - lane)] * yl[i];
                 v -= acc;
@@ -2595,14 +2599,100 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t wave_buffer(const void *p, uns
                                              0x00020000);
 }
 
+// Moduli of the 8 cells around grid node (i, j, k) (c = cx + 2 cy + 4 cz;
+// cell (cx, cy, cz) exists when cx = 0 and i > 0, or cx = 1 and i < nx - 1,
+// likewise in y, z), times cK; missing cells read modulus zero (offset past
+// the end of the modulus buffer).  cw: 1 where the cell exists, else 0 (a
+// value per lane rather than a lane mask, which would hold scalar registers
+// through the products).
+__device__ __forceinline__ void mf_moduli(const ClassDev &C, const double *Es, int i, int j, int k,
+                                          int nx, int ny, const bool (&lo)[3],
+                                          const bool (&hi)[3], float cK, float (&Ec)[8],
+                                          float (&cw)[8])
+{
+    bool cin[8];
+    const int ne = __builtin_amdgcn_readfirstlane(C.n_elem);
+    const __amdgpu_buffer_rsrc_t eb = wave_buffer((const void *)Es, (unsigned)ne * 8u);
+    const unsigned eout = (unsigned)ne * 8u;
+    const int cy1 = nx - 1, cz1 = (nx - 1) * (ny - 1);
+    const int c0 = (i - 1) + cy1 * (j - 1) + cz1 * (k - 1);
+    int ce[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+        cin[c] = (cx ? hi[0] : lo[0]) && (cy ? hi[1] : lo[1]) && (cz ? hi[2] : lo[2]);
+        ce[c] = cin[c] ? c0 + cx + cy * cy1 + cz * cz1 : 0;
+    }
+    if (!__builtin_amdgcn_readfirstlane(C.mf_cell_id)) {  // element numbers from the table
+        const MSA_GLOBAL int *cell = gbl(C.mf_cell);
+#pragma unroll
+        for (int c = 0; c < 8; c++) ce[c] = cell[ce[c]];
+    }
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const double e = buffer_load_f64(eb, (int)(cin[c] ? (unsigned)ce[c] * 8u : eout), 0, 0);
+        Ec[c] = (float)e * cK;
+        cw[c] = cin[c] ? 1.f : 0.f;
+    }
+}
+
+// The element matrix is the same for every lane: constant-address loads
+// with wave-uniform indices (scalar loads, SGPR operands of the FMAs).  An
+// empty asm redefines the pointer on every call, so the 576 loads are not
+// hoisted out of the caller's row loop (they would not fit the scalar
+// registers).
+using cfloat = __attribute__((address_space(4))) const float;
+__device__ __forceinline__ cfloat *mf_khat(const ClassDev &C)
+{
+    unsigned long long kp = (unsigned long long)C.mf_k;
+    asm volatile("" : "+s"(kp));
+    return (cfloat *)kp;
+}
+
+// Consistent mass of the cells inside the grid (dynamics): acc += cM sum
+// over existing cells of Mhat[a][b] v_b (component by component).
+__device__ __forceinline__ void mf_mass(cfloat *mh, const float (&xv)[27][3], const float (&cw)[8],
+                                        float cM, float (&acc)[3])
+{
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+        const int a = (1 - cx) + 2 * (1 - cy) + 4 * (1 - cz);
+        float m[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int o = 0; o < 8; o++) {
+            const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
+            const float mab = mh[a * 8 + o];
+            const float *xb = xv[((cz + oz) * 3 + (cy + oy)) * 3 + (cx + ox)];
+#pragma unroll
+            for (int p = 0; p < 3; p++) m[p] = __builtin_fmaf(mab, xb[p], m[p]);
+        }
+        const float w = cw[c] * cM;
+#pragma unroll
+        for (int p = 0; p < 3; p++) acc[p] = __builtin_fmaf(w, m[p], acc[p]);
+    }
+}
+
 // y = K v for block row br of a uniform structured grid class, from the
 // moduli of the (up to) 8 cells around the node and the shared unit element
 // matrix (FP32, stored in corner order): y = sum_cells cK E_c Khat[a][b] v_b
 // (+ cM Mhat[a][b] v_b on the component itself), a = the node's corner in
-// the cell, b over the cell's corners.  The 27 neighbouring blocks are
-// gathered once; neighbours outside the grid or constrained (block -1) and
-// cells outside the grid are read at an offset past the end of their buffer
+// the cell, b over the cell's corners.  Neighbours outside the grid or
+// constrained (block -1) are read at an offset past the end of the vector
 // and come back as zeros.
+//
+// Blocks forming a lexicographic sub-box of the grid (x fastest): lane l of
+// the wave holds block br = (first block of the wave) + l, so its x - 1 and
+// x + 1 neighbours are held by lanes l - 1 and l + 1.  Each lane loads the 9
+// neighbours of its own x column; the columns at x -/+ 1 come from the
+// neighbouring lanes by whole-wave DPP shifts (lanes 0 and 63 load their
+// missing column themselves), and the x -/+ 1 part of each cell is switched
+// off at the ends of a row (where the neighbouring lane holds another row).
+//
+// BOX selects the box-grid form at compile time: kernels holding both
+// forms would share one scalar-register budget for their element-matrix
+// loads.
+template <bool BOX>
 __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL float *v,
                                            const double *Es, int br, float &y0, float &y1,
                                            float &y2)
@@ -2613,19 +2703,13 @@ __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL f
     const int ny = __builtin_amdgcn_readfirstlane(C.mf_ny);
     const int nz = __builtin_amdgcn_readfirstlane(C.mf_nz);
     const int nb = __builtin_amdgcn_readfirstlane(C.nb);
-    const int ne = __builtin_amdgcn_readfirstlane(C.n_elem);
-    const MSA_GLOBAL int *gblock = gbl(C.mf_gblock);
-    const MSA_GLOBAL int *cell = gbl(C.mf_cell);
     const __amdgpu_buffer_rsrc_t vb = wave_buffer((const void *)v, (unsigned)nb * 12u);
-    const __amdgpu_buffer_rsrc_t eb = wave_buffer((const void *)Es, (unsigned)ne * 8u);
-    const unsigned vout = (unsigned)nb * 12u, eout = (unsigned)ne * 8u;
-    // 27 neighbours: byte offsets, every load in flight at once.  Grid point
-    // of neighbour (dx, dy, dz) = own point + (dx-1) + (dy-1) nx + (dz-1) nx ny
+    const unsigned vout = (unsigned)nb * 12u;
     const bool lo[3] = {i > 0, j > 0, k > 0}, hi[3] = {i < nx - 1, j < ny - 1, k < nz - 1};
-    unsigned off[27];
-    if (__builtin_amdgcn_readfirstlane(C.mf_box)) {
-        // blocks = a lexicographic sub-box of the grid: neighbour block =
-        // own block + offset, no table lookup
+    const float cK = C.dyn ? (float)C.cK : 1.f, cM = C.dyn ? (float)C.cM : 0.f;
+    cfloat *kh = mf_khat(C);
+    float acc[3] = {0.f, 0.f, 0.f};
+    if constexpr (BOX) {
         const int b0x = __builtin_amdgcn_readfirstlane(C.mf_box_lo[0]);
         const int b0y = __builtin_amdgcn_readfirstlane(C.mf_box_lo[1]);
         const int b0z = __builtin_amdgcn_readfirstlane(C.mf_box_lo[2]);
@@ -2635,15 +2719,100 @@ __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL f
         const bool blo[3] = {i > b0x, j > b0y, k > b0z}, bhi[3] = {i < b1x, j < b1y, k < b1z};
         const int sy = b1x - b0x + 1, sz = sy * (b1y - b0y + 1);
         const unsigned own = (unsigned)br * 12u;
+        const int lane = threadIdx.x & 63;
+        // own column (dx = 0): m = dy + 3 dz
+        unsigned off[9];
+        float x0[9][3], xf[9][3];
 #pragma unroll
-        for (int n = 0; n < 27; n++) {
-            const int dx = n % 3, dy = (n / 3) % 3, dz = n / 9;
-            const bool in = (dx != 0 || blo[0]) && (dx != 2 || bhi[0]) && (dy != 0 || blo[1]) &&
-                            (dy != 2 || bhi[1]) && (dz != 0 || blo[2]) && (dz != 2 || bhi[2]);
-            const int rel = 12 * ((dx - 1) + (dy - 1) * sy + (dz - 1) * sz);  // wave-uniform
-            off[n] = in ? own + (unsigned)rel : vout;
+        for (int m = 0; m < 9; m++) {
+            const int dy = m % 3, dz = m / 3;
+            const bool in = (dy != 0 || blo[1]) && (dy != 2 || bhi[1]) && (dz != 0 || blo[2]) &&
+                            (dz != 2 || bhi[2]);
+            const int rel = 12 * ((dy - 1) * sy + (dz - 1) * sz);  // wave-uniform
+            off[m] = in ? own + (unsigned)rel : vout;
         }
+#pragma unroll
+        for (int m = 0; m < 9; m++) {
+            const f32x3 w = buffer_load_f32x3(vb, (int)off[m], 0, 0);
+            x0[m][0] = w.x;
+            x0[m][1] = w.y;
+            x0[m][2] = w.z;
+        }
+        // lane 0: the column at x - 1, lane 63: at x + 1 (zeros past the
+        // box ends; other lanes take them from their neighbours)
+#pragma unroll
+        for (int m = 0; m < 9; m++) xf[m][0] = xf[m][1] = xf[m][2] = 0.f;
+        if (lane == 0 || lane == 63) {
+            const bool okx = lane == 0 ? blo[0] : bhi[0];
+            const unsigned sh = lane == 0 ? (unsigned)-12 : 12u;
+#pragma unroll
+            for (int m = 0; m < 9; m++) {
+                const f32x3 w =
+                    buffer_load_f32x3(vb, (int)((okx && off[m] != vout) ? off[m] + sh : vout), 0, 0);
+                xf[m][0] = w.x;
+                xf[m][1] = w.y;
+                xf[m][2] = w.z;
+            }
+        }
+        float Ec[8], cw[8];
+        mf_moduli(C, Es, i, j, k, nx, ny, lo, hi, cK, Ec, cw);
+        // x - 1 and x + 1 parts count where the neighbour is in the box
+        const float fm = blo[0] ? 1.f : 0.f, fp = bhi[0] ? 1.f : 0.f;
+        // column values: dx = 0 own, dx = -1 / +1 from lane l - 1 / l + 1
+        // (lanes 0 / 63: their own loads), times fm / fp
+        auto col = [&](int m, int dx, int p) -> float {
+            if (dx == 1) return x0[m][p];
+            const int o = __builtin_bit_cast(int, xf[m][p]), s = __builtin_bit_cast(int, x0[m][p]);
+            const int v = dx == 0 ? __builtin_amdgcn_update_dpp(o, s, 0x138, 0xf, 0xf, false)
+                                  : __builtin_amdgcn_update_dpp(o, s, 0x130, 0xf, 0xf, false);
+            return __builtin_bit_cast(float, v);
+        };
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+            const int a = (1 - cx) + 2 * (1 - cy) + 4 * (1 - cz);  // the node's corner
+            float t0[3] = {0.f, 0.f, 0.f}, ts[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+            for (int o = 0; o < 8; o++) {
+                const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
+                const int m = (cy + oy) + 3 * (cz + oz);
+                cfloat *kab = kh + (a * 8 + o) * 9;
+                const int dx = cx + ox;  // 0: x - 1, 1: own column, 2: x + 1
+                const float xb[3] = {col(m, dx, 0), col(m, dx, 1), col(m, dx, 2)};
+                float(&tt)[3] = dx == 1 ? t0 : ts;
+#pragma unroll
+                for (int p = 0; p < 3; p++) {
+                    tt[p] = __builtin_fmaf(kab[3 * p], xb[0], tt[p]);
+                    tt[p] = __builtin_fmaf(kab[3 * p + 1], xb[1], tt[p]);
+                    tt[p] = __builtin_fmaf(kab[3 * p + 2], xb[2], tt[p]);
+                }
+            }
+            const float f = cx ? fp : fm;
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+                acc[p] = __builtin_fmaf(Ec[c], __builtin_fmaf(f, ts[p], t0[p]), acc[p]);
+            // keep each cell's 72 scalar loads next to their use
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (cM != 0.f) {
+            float xs[27][3];  // (dx, dy, dz), x fastest
+#pragma unroll
+            for (int m = 0; m < 9; m++)
+#pragma unroll
+                for (int dx = 0; dx < 3; dx++)
+#pragma unroll
+                    for (int p = 0; p < 3; p++)
+                        xs[m * 3 + dx][p] = col(m, dx, p) * (dx == 0 ? fm : (dx == 2 ? fp : 1.f));
+            mf_mass(kh + 576, xs, cw, cM, acc);
+        }
+        y0 = acc[0];
+        y1 = acc[1];
+        y2 = acc[2];
     } else {
+    // general grids: the 27 neighbours from the grid-point table
+    const MSA_GLOBAL int *gblock = gbl(C.mf_gblock);
+    unsigned off[27];
+    {
         const int gp0 = i + nx * (j + ny * k), sy = nx, sz = nx * ny;
 #pragma unroll
         for (int n = 0; n < 27; n++) {
@@ -2663,40 +2832,8 @@ __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL f
         xv[n][1] = w.y;
         xv[n][2] = w.z;
     }
-    const float cK = C.dyn ? (float)C.cK : 1.f, cM = C.dyn ? (float)C.cM : 0.f;
-    // the element matrix is the same for every lane: constant-address loads
-    // with wave-uniform indices (scalar loads, SGPR operands of the FMAs)
-    using cfloat = __attribute__((address_space(4))) const float;
-    // an empty asm redefines the pointer on every call, so the 576 loads are
-    // not hoisted out of the caller's row loop (they would not fit the
-    // scalar registers)
-    unsigned long long kp = (unsigned long long)C.mf_k;
-    asm volatile("" : "+s"(kp));
-    cfloat *kh = (cfloat *)kp, *mh = (cfloat *)kp + 576;
-    // the 8 cells around the node (c = cx + 2 cy + 4 cz): cell (cx, cy, cz)
-    // exists when cx = 0 and i > 0, or cx = 1 and i < nx - 1 (likewise in
-    // y, z); missing cells read modulus zero
-    const int cy1 = nx - 1, cz1 = (nx - 1) * (ny - 1);
-    const int c0 = (i - 1) + cy1 * (j - 1) + cz1 * (k - 1);
-    int ce[8];
-    bool cin[8];
-#pragma unroll
-    for (int c = 0; c < 8; c++) {
-        const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
-        cin[c] = (cx ? hi[0] : lo[0]) && (cy ? hi[1] : lo[1]) && (cz ? hi[2] : lo[2]);
-        ce[c] = cin[c] ? c0 + cx + cy * cy1 + cz * cz1 : 0;
-    }
-    if (!__builtin_amdgcn_readfirstlane(C.mf_cell_id)) {  // element numbers from the table
-#pragma unroll
-        for (int c = 0; c < 8; c++) ce[c] = cell[ce[c]];
-    }
-    float Ec[8];
-#pragma unroll
-    for (int c = 0; c < 8; c++) {
-        const double e = buffer_load_f64(eb, (int)(cin[c] ? (unsigned)ce[c] * 8u : eout), 0, 0);
-        Ec[c] = (float)e * cK;
-    }
-    float acc[3] = {0.f, 0.f, 0.f};
+    float Ec[8], cw[8];
+    mf_moduli(C, Es, i, j, k, nx, ny, lo, hi, cK, Ec, cw);
 #pragma unroll
     for (int c = 0; c < 8; c++) {
         const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
@@ -2717,31 +2854,13 @@ __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL f
         }
 #pragma unroll
         for (int p = 0; p < 3; p++) acc[p] = __builtin_fmaf(Ec[c], t[p], acc[p]);
-        // keep each cell's 72 scalar loads next to their use
         __builtin_amdgcn_sched_barrier(0);
     }
-    if (cM != 0.f) {  // consistent mass of the cells inside the grid (dynamics)
-#pragma unroll
-        for (int c = 0; c < 8; c++) {
-            const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
-            const int a = (1 - cx) + 2 * (1 - cy) + 4 * (1 - cz);
-            float m[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-            for (int o = 0; o < 8; o++) {
-                const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
-                const float mab = mh[a * 8 + o];
-                const float *xb = xv[((cz + oz) * 3 + (cy + oy)) * 3 + (cx + ox)];
-#pragma unroll
-                for (int p = 0; p < 3; p++) m[p] = __builtin_fmaf(mab, xb[p], m[p]);
-            }
-            const float w = cin[c] ? cM : 0.f;
-#pragma unroll
-            for (int p = 0; p < 3; p++) acc[p] = __builtin_fmaf(w, m[p], acc[p]);
-        }
-    }
+    if (cM != 0.f) mf_mass(kh + 576, xv, cw, cM, acc);
     y0 = acc[0];
     y1 = acc[1];
     y2 = acc[2];
+    }
 }
 
 // FP32 and FP64 copies of element 0's unit Khat (and Mhat) for the
@@ -2845,7 +2964,7 @@ __global__ __launch_bounds__(256) void k_residual64_mf(const ClassDev *__restric
 //   MG_POST1 B = A + D^-1 (b - K A) / theta
 //   MG_POST2 z = B + c1 (B - A) + c2 D^-1 (b - K B); level 0 also writes
 //            the partial sums of r.z (slot 0 of the PCG partials)
-template <typename real, int T, int STEP, bool MF = false>
+template <typename real, int T, int STEP, int MF = 0>
 __global__ __launch_bounds__(256) void k_mg_smooth(const ClassDev *__restrict__ cls,
                                                    const int4 *__restrict__ items, int nitems,
                                                    int pbuf, int mode, double ratio)
@@ -2883,9 +3002,9 @@ __global__ __launch_bounds__(256) void k_mg_smooth(const ClassDev *__restrict__ 
             const int W = sptr[slice + 1] - base;
             if (!act || br >= C.nb) continue;
             real y[3];
-            if constexpr (MF)
-                mf_product(C, (const MSA_GLOBAL float *)src, C.E + (size_t)tile * C.n_elem, br,
-                           y[0], y[1], y[2]);
+            if constexpr (MF != 0)
+                mf_product<MF == 2>(C, (const MSA_GLOBAL float *)src, C.E + (size_t)tile * C.n_elem,
+                                    br, y[0], y[1], y[2]);
             else
                 row_product<real, T>(val, cidx, src, base, W, rr, s, y[0], y[1], y[2]);
 #pragma unroll
@@ -3715,14 +3834,18 @@ hipError_t launch_extract_values(hipStream_t s, bool mixed, int T, const void *v
 template <int STEP>
 static hipError_t mg_smooth_step(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
                                  const int4 *items, int nitems, int pbuf, int mode, double ratio,
-                                 bool mf)
+                                 int mf)
 {
     if (nitems <= 0) return hipSuccess;
     const dim3 grid(xcd_grid(nitems)), block(256);
-    if (mf) {
+    if (mf != 0) {
         if (!mixed || T != 1) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((k_mg_smooth<float, 1, STEP, true>), grid, block, 0, s, d_cls, items,
-                           nitems, pbuf, mode, ratio);
+        if (mf == 2)
+            hipLaunchKernelGGL((k_mg_smooth<float, 1, STEP, 2>), grid, block, 0, s, d_cls, items,
+                               nitems, pbuf, mode, ratio);
+        else
+            hipLaunchKernelGGL((k_mg_smooth<float, 1, STEP, 1>), grid, block, 0, s, d_cls, items,
+                               nitems, pbuf, mode, ratio);
     } else if (mixed) {
         MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_mg_smooth<float, TT, STEP>), grid, block, 0, s,
                                              d_cls, items, nitems, pbuf, mode, ratio));
@@ -3735,7 +3858,7 @@ static hipError_t mg_smooth_step(hipStream_t s, bool mixed, int T, const ClassDe
 
 hipError_t launch_mg_smooth(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
                             const int4 *items, int nitems, int step, int pbuf, int mode,
-                            double ratio, bool mf)
+                            double ratio, int mf)
 {
     switch (step) {
     case MG_PRE: return mg_smooth_step<MG_PRE>(s, mixed, T, d_cls, items, nitems, pbuf, mode, ratio, mf);

@@ -321,10 +321,11 @@ hipError_t launch_band_out(hipStream_t s, int T, const ClassDev *cp, int nsamp, 
 enum MgStep { MG_PRE = 0, MG_RESID = 1, MG_POST1 = 2, MG_POST2 = 3 };
 // which samples a cycle serves: 0 every sample, 1 active, 2 due for a
 // residual replacement (ir_active && ru_due), 3 refining (ir_active)
-// mf: the items' classes use the matrix-free product (mixed, T = 1)
+// mf: 0 stored values; the items' classes use the matrix-free product
+// (mixed, T = 1) on general uniform grids (1) or on box grids (2)
 hipError_t launch_mg_smooth(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
                             const int4 *items, int nitems, int step, int pbuf, int mode,
-                            double ratio, bool mf);
+                            double ratio, int mf);
 hipError_t launch_mg_restrict(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
                               const int4 *items, int nitems, int pbuf, int mode);
 hipError_t launch_mg_prolong(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
