@@ -2259,7 +2259,12 @@ int msa_batch_get_matrix_csr(msa_batch *b, int cls, int sample, long long *rowpt
                 const int gc = m.dof[3 * mc + q];
                 if (gc < 0) continue;
                 col[k] = gc;
-                val[k] = hv[((size_t)(C.slice_ptr[sl] + j) * 9 + 3 * p + q) * R + rr];
+                const size_t slot = (size_t)C.slice_ptr[sl] + j;
+                // the extracted lane keeps the one-sample-tile layout at
+                // T = 1, and is entry-major per row lane otherwise
+                val[k] = hv[b->T > 1 ? (slot * 9 + 3 * p + q) * R + rr
+                            : (rs == 4 ? val_at<float, 1>(slot, 3 * p + q, rr, 0)
+                                       : val_at<double, 1>(slot, 3 * p + q, rr, 0))];
                 k++;
             }
         }

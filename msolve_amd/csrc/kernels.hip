@@ -86,6 +86,58 @@ __device__ __forceinline__ vec3<real> ld3(const MSA_GLOBAL real *p, size_t i)
     return vec3<real>{q[0], q[1], q[2]};
 }
 
+// The 9 entries of one stored block (slot, row lane rr, sample lane s);
+// NT: non-temporal loads (values streamed once per product).  One-sample
+// tiles read 16-byte groups (val_at).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+template <typename real, int T, bool NT = false>
+__device__ __forceinline__ void load_block(const MSA_GLOBAL real *val, size_t slot, int rr, int s,
+                                           real (&m)[9])
+{
+    if constexpr (T == 1) {
+        const MSA_GLOBAL real *b = val + slot * 576;
+        if constexpr (sizeof(real) == 4) {
+            const MSA_GLOBAL f32x4 *p = (const MSA_GLOBAL f32x4 *)(b + rr * 4);
+            f32x4 g0, g1;
+            if constexpr (NT) {
+                g0 = __builtin_nontemporal_load(p);
+                g1 = __builtin_nontemporal_load(p + 64);
+            } else {
+                g0 = p[0];
+                g1 = p[64];
+            }
+            m[0] = g0.x; m[1] = g0.y; m[2] = g0.z; m[3] = g0.w;
+            m[4] = g1.x; m[5] = g1.y; m[6] = g1.z; m[7] = g1.w;
+        } else {
+            const MSA_GLOBAL f64x2 *p = (const MSA_GLOBAL f64x2 *)(b + rr * 2);
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                f64x2 d;
+                if constexpr (NT)
+                    d = __builtin_nontemporal_load(p + 64 * g);
+                else
+                    d = p[64 * g];
+                m[2 * g] = d.x;
+                m[2 * g + 1] = d.y;
+            }
+        }
+        if constexpr (NT)
+            m[8] = __builtin_nontemporal_load(b + 512 + rr);
+        else
+            m[8] = b[512 + rr];
+    } else {
+#pragma unroll
+        for (int e = 0; e < 9; e++) {
+            const MSA_GLOBAL real *p = val + val_at<real, T>(slot, e, rr, s);
+            if constexpr (NT)
+                m[e] = __builtin_nontemporal_load(p);
+            else
+                m[e] = *p;
+        }
+    }
+}
+
 template <typename real>
 __device__ __forceinline__ void st3(MSA_GLOBAL real *p, size_t i, real x, real y, real z)
 {
@@ -566,7 +618,7 @@ __global__ __launch_bounds__(64) void k_assemble(const ClassDev *__restrict__ cl
             if (!((C.node_free[nA] >> p) & 1) || !((C.node_free[nB] >> q) & 1)) continue;
             const int br = C.node_block[nA];
             const int slot = C.slice_ptr[br / R] + j;
-            const size_t idx = (((size_t)slot * 9 + ent) * R + (br % R)) * T + s;
+            const size_t idx = val_at<float, T>(slot, ent, br % R, s);
             if (C.dyn && p == q)
                 val[idx] += Es * Kh[pe] + cM * Mh[pair];
             else
@@ -667,7 +719,7 @@ __global__ __launch_bounds__(256) void k_assemble_gather(const ClassDev *__restr
                     if (!((fr >> (q / 3)) & 1) || !((fc >> (q % 3)) & 1)) acc[q] = 0.0;
             }
 #pragma unroll
-            for (int q = 0; q < 9; q++) val[(((size_t)slot * 9 + q) * R + rr) * T + s] = (real)acc[q];
+            for (int q = 0; q < 9; q++) val[val_at<real, T>(slot, q, rr, s)] = (real)acc[q];
         }
     }
 }
@@ -766,7 +818,7 @@ __global__ __launch_bounds__(256) void k_assemble_gather_g(const ClassDev *__res
                 if (g >= nt) break;
 #pragma unroll
                 for (int q = 0; q < 9; q++)
-                    val[(size_t)g * C.tile_vals + ((size_t)slot * 9 + q) * R + rr] = (real)acc[g][q];
+                    val[(size_t)g * C.tile_vals + val_at<real, 1>(slot, q, rr, 0)] = (real)acc[g][q];
             }
         }
     }
@@ -795,7 +847,7 @@ __global__ __launch_bounds__(256) void k_dinv(const ClassDev *__restrict__ cls,
         const uint8_t fr = C.block_free[br];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            const real v = val[(((size_t)slot * 9 + 4 * d) * R + rr) * T + s];
+            const real v = val[val_at<real, T>(slot, 4 * d, rr, s)];
             dinv[((size_t)br * 3 + d) * T + s] =
                 (((fr >> d) & 1) && v != (real)0) ? (real)1 / v : (real)0;
         }
@@ -861,7 +913,6 @@ __global__ __launch_bounds__(256) void k_pcg_a(const ClassDev *__restrict__ cls,
                                                int pbuf, int *active_count)
 {
     constexpr int R = 64 / T;
-    constexpr int ES = R * T;  // stride between the 9 entries of a block = 64
     const int item = xcd_item(nitems);
     if (item < 0) return;
     const int4 it = items[item];
@@ -903,11 +954,7 @@ __global__ __launch_bounds__(256) void k_pcg_a(const ClassDev *__restrict__ cls,
     };
     // stiffness values are streamed once per product: load them
     // non-temporally so L2 keeps the reused vector gathers
-    auto block = [&](int slot, real (&m)[9]) {
-        const MSA_GLOBAL real *mp = val + ((size_t)slot * 9 * R + rr) * T + s;
-#pragma unroll
-        for (int e = 0; e < 9; e++) m[e] = __builtin_nontemporal_load(mp + e * ES);
-    };
+    auto block = [&](int slot, real (&m)[9]) { load_block<real, T, true>(val, slot, rr, s, m); };
     if (__ballot(act)) {
         for (int slice = s0 + wave; slice < s1; slice += kWaves) {
             const int br = slice * R + rr;
@@ -916,16 +963,27 @@ __global__ __launch_bounds__(256) void k_pcg_a(const ClassDev *__restrict__ cls,
             if (act && br < C.nb) {
                 real a0 = 0, a1 = 0, a2 = 0;
                 int j = 0;
+                // the next group's column indices load with this group's
+                // values and gathers (one memory round trip per group)
+                // (indices past the row's last slot re-read that slot: no
+                // branch, so nothing waits on them early)
+                const int last = max(W - 1, 0);
+                int nxt[3];
+#pragma unroll
+                for (int u = 0; u < 3; u++) nxt[u] = max(cidx[(size_t)(base + min(u, last)) * R + rr], 0);
                 for (; j + 3 <= W; j += 3) {
                     int col[3];
 #pragma unroll
-                    for (int u = 0; u < 3; u++) col[u] = max(cidx[(size_t)(base + j + u) * R + rr], 0);
+                    for (int u = 0; u < 3; u++) col[u] = nxt[u];
                     real v[3][3], m[3][9];
 #pragma unroll
                     for (int u = 0; u < 3; u++) {
                         gather(col[u], v[u][0], v[u][1], v[u][2]);
                         block(base + j + u, m[u]);
                     }
+#pragma unroll
+                    for (int u = 0; u < 3; u++)
+                        nxt[u] = max(cidx[(size_t)(base + min(j + 3 + u, last)) * R + rr], 0);
 #pragma unroll
                     for (int u = 0; u < 3; u++) {
                         a0 += m[u][0] * v[u][0] + m[u][1] * v[u][1] + m[u][2] * v[u][2];
@@ -1038,7 +1096,6 @@ __global__ __launch_bounds__(256) void k_spmv(const ClassDev *__restrict__ cls,
                                               double *__restrict__ yout)
 {
     constexpr int R = 64 / T;
-    constexpr int ES = R * T;
     const int item = xcd_item(nitems);
     if (item < 0) return;
     const int4 it = items[item];
@@ -1059,10 +1116,11 @@ __global__ __launch_bounds__(256) void k_spmv(const ClassDev *__restrict__ cls,
             if (col < 0) continue;
             const size_t xo = (size_t)col * 3 * T + s;
             const double v0 = xin[xo], v1 = xin[xo + T], v2 = xin[xo + 2 * T];
-            const real *m = val + ((size_t)slot * 9 * R + rr) * T + s;
-            a0 += (double)m[0] * v0 + (double)m[ES] * v1 + (double)m[2 * ES] * v2;
-            a1 += (double)m[3 * ES] * v0 + (double)m[4 * ES] * v1 + (double)m[5 * ES] * v2;
-            a2 += (double)m[6 * ES] * v0 + (double)m[7 * ES] * v1 + (double)m[8 * ES] * v2;
+            real m[9];
+            load_block<real, T>(gbl(val), slot, rr, s, m);
+            a0 += (double)m[0] * v0 + (double)m[1] * v1 + (double)m[2] * v2;
+            a1 += (double)m[3] * v0 + (double)m[4] * v1 + (double)m[5] * v2;
+            a2 += (double)m[6] * v0 + (double)m[7] * v1 + (double)m[8] * v2;
         }
         const size_t o = (size_t)br * 3 * T + s;
         yout[o] = a0;
@@ -1689,7 +1747,6 @@ __global__ __launch_bounds__(256) void k_pcg_init_warm(const ClassDev *__restric
                                                        const int4 *__restrict__ items, int nitems)
 {
     constexpr int R = 64 / T;
-    constexpr int ES = R * T;
     const int item = xcd_item(nitems);
     if (item < 0) return;
     const int4 it = items[item];
@@ -1718,10 +1775,11 @@ __global__ __launch_bounds__(256) void k_pcg_init_warm(const ClassDev *__restric
                 if (col < 0) continue;
                 const size_t xo = (size_t)col * 3 * T + s;
                 const double v0 = x[xo], v1 = x[xo + T], v2 = x[xo + 2 * T];
-                const double *m = val + ((size_t)slot * 9 * R + rr) * T + s;
-                kx[0] += m[0] * v0 + m[ES] * v1 + m[2 * ES] * v2;
-                kx[1] += m[3 * ES] * v0 + m[4 * ES] * v1 + m[5 * ES] * v2;
-                kx[2] += m[6 * ES] * v0 + m[7 * ES] * v1 + m[8 * ES] * v2;
+                double m[9];
+                load_block<double, T>(gbl(val), slot, rr, s, m);
+                kx[0] += m[0] * v0 + m[1] * v1 + m[2] * v2;
+                kx[1] += m[3] * v0 + m[4] * v1 + m[5] * v2;
+                kx[2] += m[6] * v0 + m[7] * v1 + m[8] * v2;
             }
 #pragma unroll
             for (int d = 0; d < 3; d++) {
@@ -1830,7 +1888,7 @@ __global__ __launch_bounds__(256) void k_band_scatter(const ClassDev *__restrict
                     const int jj = C.binv[3 * col + q];
                     if (jj < 0 || jj > i) continue;
                     AB[(size_t)jj * C.bW + (i - jj)] =
-                        (double)val[(((size_t)slot * 9 + 3 * p + q) * R + rr) * T + s];
+                        (double)val[val_at<real, T>(slot, 3 * p + q, rr, s)];
                 }
             }
         }
@@ -2542,7 +2600,6 @@ __device__ __forceinline__ void row_product(const MSA_GLOBAL real *val, const MS
                                             int s, real &a0, real &a1, real &a2)
 {
     constexpr int R = 64 / T;
-    constexpr int ES = R * T;
     a0 = a1 = a2 = 0;
     int j = 0;
     for (; j + 3 <= W; j += 3) {
@@ -2553,9 +2610,7 @@ __device__ __forceinline__ void row_product(const MSA_GLOBAL real *val, const MS
 #pragma unroll
         for (int u = 0; u < 3; u++) {
             gather3<real, T>(v, col[u], s, x[u][0], x[u][1], x[u][2]);
-            const MSA_GLOBAL real *mp = val + ((size_t)(base + j + u) * 9 * R + rr) * T + s;
-#pragma unroll
-            for (int e = 0; e < 9; e++) m[u][e] = __builtin_nontemporal_load(mp + e * ES);
+            load_block<real, T, true>(val, base + j + u, rr, s, m[u]);
         }
 #pragma unroll
         for (int u = 0; u < 3; u++) {
@@ -2568,9 +2623,7 @@ __device__ __forceinline__ void row_product(const MSA_GLOBAL real *val, const MS
         const int col = max(cidx[(size_t)(base + j) * R + rr], 0);
         real x0, x1, x2, m[9];
         gather3<real, T>(v, col, s, x0, x1, x2);
-        const MSA_GLOBAL real *mp = val + ((size_t)(base + j) * 9 * R + rr) * T + s;
-#pragma unroll
-        for (int e = 0; e < 9; e++) m[e] = __builtin_nontemporal_load(mp + e * ES);
+        load_block<real, T, true>(val, base + j, rr, s, m);
         a0 += m[0] * x0 + m[1] * x1 + m[2] * x2;
         a1 += m[3] * x0 + m[4] * x1 + m[5] * x2;
         a2 += m[6] * x0 + m[7] * x1 + m[8] * x2;
@@ -3210,7 +3263,6 @@ __global__ __launch_bounds__(256) void k_mg_gersh(const ClassDev *__restrict__ c
                                                   const int4 *__restrict__ items, int nitems)
 {
     constexpr int R = 64 / T;
-    constexpr int ES = R * T;
     const int item = xcd_item(nitems);
     if (item < 0) return;
     const int4 it = items[item];
@@ -3230,11 +3282,12 @@ __global__ __launch_bounds__(256) void k_mg_gersh(const ClassDev *__restrict__ c
             const int base = C.slice_ptr[slice], W = C.slice_ptr[slice + 1] - base;
             double sum[3] = {0, 0, 0};
             for (int j = 0; j < W; j++) {
-                const real *m = val + ((size_t)(base + j) * 9 * R + rr) * T + s;
+                real m[9];
+                load_block<real, T, true>(gbl(val), base + j, rr, s, m);
 #pragma unroll
                 for (int p = 0; p < 3; p++)
-                    sum[p] += fabs((double)m[(3 * p) * ES]) + fabs((double)m[(3 * p + 1) * ES]) +
-                              fabs((double)m[(3 * p + 2) * ES]);
+                    sum[p] += fabs((double)m[3 * p]) + fabs((double)m[3 * p + 1]) +
+                              fabs((double)m[3 * p + 2]);
             }
 #pragma unroll
             for (int p = 0; p < 3; p++)

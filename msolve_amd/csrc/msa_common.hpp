@@ -28,6 +28,25 @@ namespace msa {
 constexpr int kMaxColours = 64;
 constexpr int kWaves = 4;  // waves per workgroup of the PCG kernels
 
+// Position of entry e (row-major in the 3x3 block) of stored slot `slot`,
+// block row lane rr and sample lane s within one tile's values.
+// One-sample tiles (T = 1): per slot, entries 0-7 in 16-byte groups per
+// row lane (4 floats or 2 doubles), then entry 8 per row lane, so that a
+// lane reads its block with 16-byte loads; T > 1: entry-major, then row
+// lane, then sample lane.  Either way a slot holds 9 x 64 values.
+template <typename real, int T>
+__host__ __device__ constexpr size_t val_at(size_t slot, int e, int rr, int s)
+{
+    if constexpr (T == 1) {
+        constexpr int W = 16 / (int)sizeof(real);
+        return slot * 576 + (e < 8 ? (size_t)(e / W) * 64 * W + (size_t)rr * W + e % W
+                                   : (size_t)512 + rr);
+    } else {
+        constexpr int R = 64 / T;
+        return ((slot * 9 + e) * R + rr) * T + s;
+    }
+}
+
 void set_error(const char *fmt, ...);
 
 #define MSA_HIP(call)                                                         \
