@@ -138,6 +138,24 @@ __device__ __forceinline__ void load_block(const MSA_GLOBAL real *val, size_t sl
     }
 }
 
+// Store of one block in the one-sample-tile layout (16-byte groups).
+template <typename real>
+__device__ __forceinline__ void store_block(MSA_GLOBAL real *val, size_t slot, int rr,
+                                            const real (&m)[9])
+{
+    MSA_GLOBAL real *b = val + slot * 576;
+    if constexpr (sizeof(real) == 4) {
+        MSA_GLOBAL f32x4 *p = (MSA_GLOBAL f32x4 *)(b + rr * 4);
+        p[0] = f32x4{m[0], m[1], m[2], m[3]};
+        p[64] = f32x4{m[4], m[5], m[6], m[7]};
+    } else {
+        MSA_GLOBAL f64x2 *p = (MSA_GLOBAL f64x2 *)(b + rr * 2);
+#pragma unroll
+        for (int g = 0; g < 4; g++) p[64 * g] = f64x2{m[2 * g], m[2 * g + 1]};
+    }
+    b[512 + rr] = m[8];
+}
+
 template <typename real>
 __device__ __forceinline__ void st3(MSA_GLOBAL real *p, size_t i, real x, real y, real z)
 {
@@ -816,9 +834,10 @@ __global__ __launch_bounds__(256) void k_assemble_gather_g(const ClassDev *__res
 #pragma unroll
             for (int g = 0; g < G; g++) {
                 if (g >= nt) break;
+                real m[9];
 #pragma unroll
-                for (int q = 0; q < 9; q++)
-                    val[(size_t)g * C.tile_vals + val_at<real, 1>(slot, q, rr, 0)] = (real)acc[g][q];
+                for (int q = 0; q < 9; q++) m[q] = (real)acc[g][q];
+                store_block<real>(val + (size_t)g * C.tile_vals, slot, rr, m);
             }
         }
     }
@@ -3130,14 +3149,31 @@ __global__ __launch_bounds__(256) void k_mg_restrict(const ClassDev *__restrict_
             const int lx = F.mf_box_lo[0], ly = F.mf_box_lo[1], lz = F.mf_box_lo[2];
             const int hx = F.mf_box_hi[0], hy = F.mf_box_hi[1], hz = F.mf_box_hi[2];
             const int sy = hx - lx + 1, sz = sy * (hy - ly + 1);
+            // branch-free: points outside the fine box are read past the
+            // end of the vector (zeros)
+            const int nbf = F.nb;
+            const __amdgpu_buffer_rsrc_t fbuf = wave_buffer((const void *)Bf, (unsigned)nbf * 3u * sizeof(real));
+            const unsigned fout = (unsigned)nbf * 3u * sizeof(real);
 #pragma unroll
             for (int n = 0; n < 27; n++) {
                 const int dx = n % 3 - 1, dy = (n / 3) % 3 - 1, dz = n / 9 - 1;
                 const int x = fi + dx, y = fj + dy, z = fk + dz;
-                if (x < lx || x > hx || y < ly || y > hy || z < lz || z > hz) continue;
+                const bool in = x >= lx && x <= hx && y >= ly && y <= hy && z >= lz && z <= hz;
+                const unsigned off =
+                    in ? (unsigned)((x - lx) + sy * (y - ly) + sz * (z - lz)) * 3u * (unsigned)sizeof(real)
+                       : fout;
                 const double w = (dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0);
                 real v0, v1, v2;
-                gather3<real, T>(Bf, (x - lx) + sy * (y - ly) + sz * (z - lz), s, v0, v1, v2);
+                if constexpr (sizeof(real) == 4) {
+                    const f32x3 v = buffer_load_f32x3(fbuf, (int)off, 0, 0);
+                    v0 = v.x;
+                    v1 = v.y;
+                    v2 = v.z;
+                } else {
+                    v0 = buffer_load_f64(fbuf, (int)off, 0, 0);
+                    v1 = buffer_load_f64(fbuf, (int)(off == fout ? fout : off + 8u), 0, 0);
+                    v2 = buffer_load_f64(fbuf, (int)(off == fout ? fout : off + 16u), 0, 0);
+                }
                 acc[0] += w * (double)v0;
                 acc[1] += w * (double)v1;
                 acc[2] += w * (double)v2;
@@ -3196,24 +3232,43 @@ __global__ __launch_bounds__(256) void k_mg_prolong(const ClassDev *__restrict__
         double acc[3] = {0, 0, 0};
         if (st) {
             // coarse points: i/2 for even i, (i-1)/2 and (i+1)/2 with weight
-            // 1/2 for odd i (per direction); coarse blocks = the coarse box
+            // 1/2 for odd i (per direction); coarse blocks = the coarse box.
+            // Branch-free: the second point of an even direction has weight
+            // 0, and points outside the coarse box are read past the end of
+            // the vector (zeros).
             const int g = C.mf_bpos[fb];
             const int i = g & 1023, j = (g >> 10) & 1023, k2 = g >> 20;
             const int lx = K.mf_box_lo[0], ly = K.mf_box_lo[1], lz = K.mf_box_lo[2];
             const int hx = K.mf_box_hi[0], hy = K.mf_box_hi[1], hz = K.mf_box_hi[2];
             const int sy = hx - lx + 1, sz = sy * (hy - ly + 1);
+            const int nbc = K.nb;
+            const __amdgpu_buffer_rsrc_t cb = wave_buffer((const void *)zc, (unsigned)nbc * 3u * sizeof(real));
+            const unsigned cout = (unsigned)nbc * 3u * sizeof(real);
+            const double wx = (i & 1) ? 0.5 : 1.0, wy = (j & 1) ? 0.5 : 1.0, wz = (k2 & 1) ? 0.5 : 1.0;
 #pragma unroll
             for (int n = 0; n < 8; n++) {
                 const int ox = n & 1, oy = (n >> 1) & 1, oz = n >> 2;
-                if ((ox && !(i & 1)) || (oy && !(j & 1)) || (oz && !(k2 & 1))) continue;
                 const int x = (i >> 1) + ox, y = (j >> 1) + oy, z = (k2 >> 1) + oz;
-                if (x < lx || x > hx || y < ly || y > hy || z < lz || z > hz) continue;
-                const double w = ((i & 1) ? 0.5 : 1.0) * ((j & 1) ? 0.5 : 1.0) * ((k2 & 1) ? 0.5 : 1.0);
+                const bool in = x >= lx && x <= hx && y >= ly && y <= hy && z >= lz && z <= hz;
+                const unsigned off =
+                    in ? (unsigned)((x - lx) + sy * (y - ly) + sz * (z - lz)) * 3u * (unsigned)sizeof(real)
+                       : cout;
+                const double w = (ox ? ((i & 1) ? wx : 0.0) : wx) * (oy ? ((j & 1) ? wy : 0.0) : wy) *
+                                 (oz ? ((k2 & 1) ? wz : 0.0) : wz);
                 real v0, v1, v2;
-                gather3<real, T>(zc, (x - lx) + sy * (y - ly) + sz * (z - lz), s, v0, v1, v2);
-                acc[0] += w * v0;
-                acc[1] += w * v1;
-                acc[2] += w * v2;
+                if constexpr (sizeof(real) == 4) {
+                    const f32x3 v = buffer_load_f32x3(cb, (int)off, 0, 0);
+                    v0 = v.x;
+                    v1 = v.y;
+                    v2 = v.z;
+                } else {
+                    v0 = buffer_load_f64(cb, (int)off, 0, 0);
+                    v1 = buffer_load_f64(cb, (int)(off == cout ? cout : off + 8u), 0, 0);
+                    v2 = buffer_load_f64(cb, (int)(off == cout ? cout : off + 16u), 0, 0);
+                }
+                acc[0] += w * (double)v0;
+                acc[1] += w * (double)v1;
+                acc[2] += w * (double)v2;
             }
         } else
 #pragma unroll 4
