@@ -250,3 +250,75 @@ def test_matrix_free_cycle_matches_stored_value_cycle(ctx, dynamics):
         b.close()
     mesh.close()
     assert rel(z[L.MSA_MIXED], z[L.MSA_FP64]) < 2e-5
+
+
+def _grid_variant(kind):
+    """Uniform grids for the matrix-free forms: "long" (80 x 4 x 4: rows of
+    80 free nodes, so wavefronts start and end inside rows and the lanes at
+    both wave ends load their x - 1 / x + 1 columns themselves), "hole"
+    (12 x 12 x 8 with one interior node fixed: the free nodes are not a
+    box, so the general form with the grid-point table runs) and "roller"
+    (z fixed on the z = 0 face as well: partially free blocks, box form,
+    the ELL transfers)."""
+    from types import SimpleNamespace
+    from msolve_amd.core import enumerate_dofs
+    if kind == "long":
+        m = msa.cantilever(80, 4, 4, lx=4.0, ly=0.2, lz=0.2)
+        return SimpleNamespace(xyz=m.xyz, conn=m.conn, dof=m.dof, n_free=m.n_free,
+                               n_elem=m.n_elem, f=m.f)
+    m = msa.cantilever(12, 12, 8, lz=0.5)
+    nX, nY = 13, 13
+    cons = np.asarray(m.constrained, np.uint8).reshape(-1, 3).copy()
+    if kind == "hole":
+        cons[6 + nX * (6 + nY * 4), :] = 1
+    else:
+        cons[np.arange(nX * nY), 2] = 1
+    dof, n_free = enumerate_dofs(cons)
+    xyz = np.asarray(m.xyz).reshape(-1, 3)
+    f = np.zeros(n_free)
+    tip = np.nonzero(np.isclose(xyz[:, 0], xyz[:, 0].max()))[0]
+    for nd in tip:
+        if dof[3 * nd + 2] >= 0:
+            f[dof[3 * nd + 2]] = -100.0 / tip.size
+    return SimpleNamespace(xyz=m.xyz, conn=m.conn, dof=dof, n_free=n_free, n_elem=m.n_elem, f=f)
+
+
+@pytest.mark.parametrize("kind", ["long", "hole", "roller"])
+def test_matrix_free_forms_match_oracle_cycle(ctx, kind):
+    """One V-cycle of a MIXED batch (matrix-free products: box-grid form
+    with the x columns shared between lanes, or the general form) and of an
+    FP64 batch (stored values) against the oracle's cycle, on grids that
+    exercise rows longer than a wavefront, a non-box set of free nodes and
+    partially free blocks.  MIXED batches store the coarsest operator in
+    FP32 and factor that: on the thin 80 x 4 x 4 beam its condition number
+    turns the rounding into a 1e-4 change of the cycle, so the MIXED cycle
+    is held to the oracle's cycle with the coarsest matrix rounded to FP32
+    (the same operator), the FP64 cycle to the plain one."""
+    import scipy.sparse as sp
+    print(f"\n[case] matrix-free forms vs oracle cycle grid={kind}", flush=True)
+    model = _grid_variant(kind)
+    mesh = msa.Mesh(ctx, model.xyz, model.conn, model.dof, model.n_free)
+    rng = np.random.default_rng(17)
+    E = lognormal(rng, model.n_elem)
+    r = rng.standard_normal(model.n_free)
+    levels, Ps = MG.hierarchy(np.asarray(model.xyz).reshape(-1, 3), model.conn, model.dof,
+                              model.n_free, E)
+    ref = {L.MSA_FP64: MG.vcycle(levels, Ps, r)}
+    Kc = levels[-1].K
+    levels[-1].K = sp.csr_matrix((Kc.data.astype(np.float32).astype(np.float64), Kc.indices,
+                                  Kc.indptr), shape=Kc.shape)
+    ref[L.MSA_MIXED] = MG.vcycle(levels, Ps, r)
+    bar = {L.MSA_FP64: 1e-9, L.MSA_MIXED: 1e-6}
+    for prec in (L.MSA_FP64, L.MSA_MIXED):
+        b = msa.Batch(ctx, [mesh], [1], prec, 0)
+        b.set_modulus(0, E[None, :])
+        b.set_rhs(0, model.f)
+        b.assemble(0.3)
+        b.set_preconditioner(msa.MSA_PRECOND_MULTIGRID)
+        info = b.mg_info(0)
+        assert info["levels"] == len(Ps)  # coarse levels
+        assert info["matrix_free"] == (prec == L.MSA_MIXED)
+        z = b.mg_apply(0, 0, r)
+        b.close()
+        assert rel(z, ref[prec]) < bar[prec], (prec, rel(z, ref[prec]))
+    mesh.close()
