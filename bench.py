@@ -77,6 +77,9 @@ def parse():
     p.add_argument("--seed", type=int, default=20261202)
     p.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison runs")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--cpu-threads", type=int, default=-1,
+                   help="threads of the multi-core CPU baseline (-1: host cores up to 16, "
+                        "0: skip it)")
     p.add_argument("--no-single", action="store_true", help="skip the single-sample FP64 runs")
     p.add_argument("--newmark", type=int, default=0, metavar="STEPS",
                    help="instead of the Monte Carlo job, time STEPS Newmark steps of the "
@@ -470,6 +473,38 @@ def main():
                           f"CSR/Jacobi-PCG FP64 restatement of the reference per-sample loop "
                           f"(KL modulus, Ke, assembly, PCG to {args.tol}), serial; rate = "
                           f"{N} / sum(class count x measured time), {t_meas:.1f} s measured")
+        # the same restatement on several host cores at once (samples are
+        # independent; the oracle's C calls release the interpreter lock):
+        # every thread solves one other sample of each size, the per-size
+        # times under that load weight the class counts as above
+        nthr = args.cpu_threads
+        if nthr < 0:
+            nthr = min(16, len(os.sched_getaffinity(0)))
+        if nthr > 1:
+            from concurrent.futures import ThreadPoolExecutor
+
+            def cpu_job(tid):
+                res, o = [], 0
+                for c, n in enumerate(sizes):
+                    row = o + (1 + tid) % counts[c]
+                    r = O.mc_reference(n, otab, xi[row:row + 1], cov=args.cov, tol=args.tol,
+                                       keep_u=False, storage="csr")
+                    res.append(r["times"]["all"])
+                    o += counts[c]
+                return res
+
+            t0 = time.perf_counter()
+            with ThreadPoolExecutor(nthr) as ex:
+                res = list(ex.map(cpu_job, range(nthr)))
+            wall = time.perf_counter() - t0
+            per_par = [sum(r[c] for r in res) / nthr for c in range(len(sizes))]
+            rate = nthr * N / sum(counts[c] * per_par[c] for c in range(len(sizes)))
+            extra["cpu_baseline_parallel"] = dict(
+                value=rate, unit="solves/s", cores=nthr, kind="port",
+                sample=f"{nthr} threads, each one further sample per mesh size {sizes}, the same "
+                       f"oracle restatement; rate = {nthr} x {N} / sum(class count x mean "
+                       f"per-size time under load), {wall:.1f} s wall")
+            log(f"[bench] cpu x{nthr}: {rate:.2f} solves/s ({wall:.1f} s wall)")
 
     out = {
         "metric": METRIC,
