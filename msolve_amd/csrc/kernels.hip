@@ -2660,6 +2660,9 @@ __device__ f32x3 buffer_load_f32x3(__amdgpu_buffer_rsrc_t rsrc, int voffset, int
                                    int aux) __asm("llvm.amdgcn.raw.ptr.buffer.load.v3f32");
 __device__ double buffer_load_f64(__amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset,
                                   int aux) __asm("llvm.amdgcn.raw.ptr.buffer.load.f64");
+typedef double f64x2v __attribute__((ext_vector_type(2)));
+__device__ f64x2v buffer_load_f64x2(__amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset,
+                                    int aux) __asm("llvm.amdgcn.raw.ptr.buffer.load.v2f64");
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t wave_buffer(const void *p, unsigned bytes)
 {
     const unsigned long long a = (unsigned long long)p;
@@ -2993,7 +2996,16 @@ __global__ __launch_bounds__(256) void k_residual64_mf(const ClassDev *__restric
         const int g = gbl(C.mf_bpos)[br];
         const int i = g & 1023, j = (g >> 10) & 1023, k = g >> 20;
         double y[3] = {0.0, 0.0, 0.0};
-#pragma unroll 1
+        // box grids: corner blocks by arithmetic and range-checked loads (a
+        // corner outside the box reads zeros), so no load waits on a table
+        // lookup; two cells per step keep 16 corner loads in flight
+        const bool box = C.mf_box;
+        const int b0x = C.mf_box_lo[0], b0y = C.mf_box_lo[1], b0z = C.mf_box_lo[2];
+        const int sxb = C.mf_box_hi[0] - b0x + 1, syb = C.mf_box_hi[1] - b0y + 1;
+        const int bzn = C.mf_box_hi[2] - b0z + 1;
+        const __amdgpu_buffer_rsrc_t ub = wave_buffer((const void *)u, (unsigned)C.nb * 24u);
+        const unsigned uout = (unsigned)C.nb * 24u;
+#pragma unroll 2
         for (int c = 0; c < 8; c++) {
             const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
             const int ci = i - 1 + cx, cj = j - 1 + cy, ck = k - 1 + cz;
@@ -3003,11 +3015,23 @@ __global__ __launch_bounds__(256) void k_residual64_mf(const ClassDev *__restric
             double uc[8][3];
 #pragma unroll
             for (int o = 0; o < 8; o++) {
-                const int nb = gblock[(ci + (o & 1)) + nx * ((cj + ((o >> 1) & 1)) + ny * (ck + (o >> 2)))];
-                const vec3<double> v = ld3(u, nb >= 0 ? nb : 0);
-                uc[o][0] = nb >= 0 ? v.x : 0.0;
-                uc[o][1] = nb >= 0 ? v.y : 0.0;
-                uc[o][2] = nb >= 0 ? v.z : 0.0;
+                const int gx = ci + (o & 1), gy = cj + ((o >> 1) & 1), gz = ck + (o >> 2);
+                if (box) {
+                    const int x = gx - b0x, yy = gy - b0y, z = gz - b0z;
+                    const bool in = (unsigned)x < (unsigned)sxb && (unsigned)yy < (unsigned)syb &&
+                                    (unsigned)z < (unsigned)bzn;
+                    const unsigned off = in ? (unsigned)(x + sxb * (yy + syb * z)) * 24u : uout;
+                    const f64x2v v = buffer_load_f64x2(ub, (int)off, 0, 0);
+                    uc[o][0] = v.x;
+                    uc[o][1] = v.y;
+                    uc[o][2] = buffer_load_f64(ub, (int)(in ? off + 16u : uout), 0, 0);
+                } else {
+                    const int nb = gblock[gx + nx * (gy + ny * gz)];
+                    const vec3<double> v = ld3(u, nb >= 0 ? nb : 0);
+                    uc[o][0] = nb >= 0 ? v.x : 0.0;
+                    uc[o][1] = nb >= 0 ? v.y : 0.0;
+                    uc[o][2] = nb >= 0 ? v.z : 0.0;
+                }
             }
             double t3[3] = {0.0, 0.0, 0.0}, m3[3] = {0.0, 0.0, 0.0};
 #pragma unroll
