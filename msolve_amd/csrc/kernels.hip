@@ -716,9 +716,15 @@ __global__ __launch_bounds__(256) void k_assemble_gather(const ClassDev *__restr
             const int col = cidx[(size_t)slot * R + rr];
             double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
             if (col >= 0) {
+                // the slot's contribution codes load together (a code < 0
+                // ends the list: the rest are skipped, not waited on)
+                int codes[8];
+#pragma unroll
+                for (int gi = 0; gi < 8; gi++) codes[gi] = gmap[((size_t)slot * 8 + gi) * R + rr];
+#pragma unroll 2
                 for (int gi = 0; gi < 8; gi++) {
-                    const int code = gmap[((size_t)slot * 8 + gi) * R + rr];
-                    if (code < 0) break;
+                    const int code = codes[gi];
+                    if (code < 0) continue;
                     const int e = code >> 6, ab = code & 63;
                     const double Es = E[(size_t)e * T + s] * cK;
                     const MSA_GLOBAL double *k = kc + (size_t)e * 576 + ab * 9;
@@ -796,9 +802,15 @@ __global__ __launch_bounds__(256) void k_assemble_gather_g(const ClassDev *__res
 #pragma unroll
                 for (int q = 0; q < 9; q++) acc[g][q] = 0.0;
             if (col >= 0) {
+                // the slot's contribution codes load together (a code < 0
+                // ends the list: the rest are skipped, not waited on)
+                int codes[8];
+#pragma unroll
+                for (int gi = 0; gi < 8; gi++) codes[gi] = gmap[((size_t)slot * 8 + gi) * R + rr];
+#pragma unroll 2
                 for (int gi = 0; gi < 8; gi++) {
-                    const int code = gmap[((size_t)slot * 8 + gi) * R + rr];
-                    if (code < 0) break;
+                    const int code = codes[gi];
+                    if (code < 0) continue;
                     const int e = code >> 6, ab = code & 63;
                     double k[9], m = 0.0;
                     if (uni) {
