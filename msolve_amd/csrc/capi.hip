@@ -841,6 +841,10 @@ int mg_build(msa_batch *b, int max_levels)
             if (m.node_free[m.block_node[bb]] != 7) return false;
         return true;
     };
+    for (int ci = 0; ci < ncls + (int)b->lv.size(); ci++) {
+        ClassDev &D = tab(b, ci).dev;
+        D.mf_allfree = D.mf && D.mf_box && all_free(tab(b, ci).mesh->h) ? 1 : 0;
+    }
     for (auto &L : b->lv) {
         ClassHost &P = tab(b, L.dev.mg_fine);
         if (L.dev.mf && P.dev.mf && L.dev.mf_box && P.dev.mf_box && all_free(L.mesh->h) &&
@@ -964,10 +968,16 @@ int mg_values(msa_batch *b)
             return launch_fill_f64(s, C.dev.mg_lam, (size_t)C.ntiles * T, 0.0);
         }));
     }
-    for (int d = 0; d < b->mg_depth; d++)
+    // Gershgorin bounds: matrix-free on box grids of fully free blocks (the
+    // stored-value kernel skips those classes)
+    for (int d = 0; d < b->mg_depth; d++) {
         MSA_TRY(timed(b, KK_OTHER, [&] {
             return launch_mg_gersh(s, mx, T, b->d_cls, b->mg_smooth[d], b->mg_nsmooth[d]);
         }));
+        MSA_TRY(timed(b, KK_OTHER, [&] {
+            return launch_mg_gersh_mf(s, b->d_cls, b->mg_sm_b[d], b->mg_nsm_b[d]);
+        }));
+    }
     for (int ci : b->mg_coarsest) {
         ClassHost &C = tab(b, ci);
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_fill_f64(s, C.dev.band, C.band_cap, 0.0); }));

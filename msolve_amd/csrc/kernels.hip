@@ -3362,6 +3362,7 @@ __global__ __launch_bounds__(256) void k_mg_gersh(const ClassDev *__restrict__ c
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int rr = lane / T, s = lane % T;
     const int sample = tile * T + s;
+    if (C.mf_allfree) return;  // k_mg_gersh_mf
     const real *val = (const real *)C.val + (size_t)tile * C.tile_vals;
     const real *dinv = (const real *)C.dinv + (size_t)tile * C.tile_vec;
     double g = 0;
@@ -3389,6 +3390,93 @@ __global__ __launch_bounds__(256) void k_mg_gersh(const ClassDev *__restrict__ c
     for (int off = T; off < 64; off <<= 1) g = fmax(g, __shfl_xor(g, off, 64));
     if (rr == 0 && sample < C.nsamp && g > 0)
         atomicMax((unsigned long long *)&C.mg_lam[sample], (unsigned long long)__double_as_longlong(g));
+}
+
+// The same bound matrix-free, on box grids of fully free blocks (one-sample
+// tiles, mixed precision): per node, each of the 27 neighbour blocks is
+// formed from the moduli of the cells the two nodes share and the FP32 unit
+// element matrix (+ the consistent mass on the diagonal in the dynamics
+// mode), its absolute row sums accumulated; lambda = max over rows of
+// sum_j |K_ij| / K_ii.  Neighbours outside the box are constrained nodes.
+__global__ __launch_bounds__(256) void k_mg_gersh_mf(const ClassDev *__restrict__ cls,
+                                                     const int4 *__restrict__ items, int nitems)
+{
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    if (!C.mf_allfree) return;
+    const int tile = it.y, chunk = it.z;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int sample = tile;
+    float g = 0.f;
+    if (sample < C.nsamp) {
+        const int nx = __builtin_amdgcn_readfirstlane(C.mf_nx);
+        const int ny = __builtin_amdgcn_readfirstlane(C.mf_ny);
+        const int nz = __builtin_amdgcn_readfirstlane(C.mf_nz);
+        const int b0x = C.mf_box_lo[0], b0y = C.mf_box_lo[1], b0z = C.mf_box_lo[2];
+        const int b1x = C.mf_box_hi[0], b1y = C.mf_box_hi[1], b1z = C.mf_box_hi[2];
+        const float cK = C.dyn ? (float)C.cK : 1.f, cM = C.dyn ? (float)C.cM : 0.f;
+        const double *Es = C.E + (size_t)tile * C.n_elem;
+        const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+        for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+            const int br = slice * 64 + lane;
+            if (br >= C.nb) continue;
+            const int gp = gbl(C.mf_bpos)[br];
+            const int i = gp & 1023, j = (gp >> 10) & 1023, k = gp >> 20;
+            const bool lo[3] = {i > 0, j > 0, k > 0}, hi[3] = {i < nx - 1, j < ny - 1, k < nz - 1};
+            const bool blo[3] = {i > b0x, j > b0y, k > b0z}, bhi[3] = {i < b1x, j < b1y, k < b1z};
+            float Ec[8], cw[8];
+            mf_moduli(C, Es, i, j, k, nx, ny, lo, hi, cK, Ec, cw);
+            float cm[8];  // mass weight per cell (zero without dynamics)
+#pragma unroll
+            for (int c = 0; c < 8; c++) cm[c] = cw[c] * cM;
+            float rs[3] = {0.f, 0.f, 0.f}, dg[3] = {1.f, 1.f, 1.f};
+#pragma unroll
+            for (int n = 0; n < 27; n++) {
+                // the element-matrix pointer redefined per neighbour: its
+                // scalar loads stay with their neighbour
+                cfloat *kh = mf_khat(C), *mh = kh + 576;
+                const int dx = n % 3 - 1, dy = (n / 3) % 3 - 1, dz = n / 9 - 1;
+                const bool in = (dx >= 0 || blo[0]) && (dx <= 0 || bhi[0]) && (dy >= 0 || blo[1]) &&
+                                (dy <= 0 || bhi[1]) && (dz >= 0 || blo[2]) && (dz <= 0 || bhi[2]);
+                float B[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int c = 0; c < 8; c++) {
+                    const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+                    // cell c spans node offsets {c* - 1, c*} in each direction
+                    if (dx < cx - 1 || dx > cx || dy < cy - 1 || dy > cy || dz < cz - 1 || dz > cz) continue;
+                    const int a = (1 - cx) + 2 * (1 - cy) + 4 * (1 - cz);
+                    const int bb = (dx - cx + 1) + 2 * (dy - cy + 1) + 4 * (dz - cz + 1);
+                    cfloat *kab = kh + (a * 8 + bb) * 9;
+#pragma unroll
+                    for (int q = 0; q < 9; q++) B[q] = __builtin_fmaf(Ec[c], kab[q], B[q]);
+                    const float mab = mh[a * 8 + bb];
+                    B[0] = __builtin_fmaf(cm[c], mab, B[0]);
+                    B[4] = __builtin_fmaf(cm[c], mab, B[4]);
+                    B[8] = __builtin_fmaf(cm[c], mab, B[8]);
+                }
+                const float f = in ? 1.f : 0.f;
+#pragma unroll
+                for (int d = 0; d < 3; d++)
+                    rs[d] = __builtin_fmaf(f, fabsf(B[3 * d]) + fabsf(B[3 * d + 1]) + fabsf(B[3 * d + 2]),
+                                           rs[d]);
+                if (n == 13) {
+                    dg[0] = B[0];
+                    dg[1] = B[4];
+                    dg[2] = B[8];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+#pragma unroll
+            for (int d = 0; d < 3; d++) g = fmaxf(g, rs[d] / fabsf(dg[d]));
+        }
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) g = fmaxf(g, __shfl_xor(g, off, 64));
+    if (lane == 0 && sample < C.nsamp && g > 0.f)
+        atomicMax((unsigned long long *)&C.mg_lam[sample],
+                  (unsigned long long)__double_as_longlong((double)g));
 }
 
 // One sample's level-0 right-hand side for a single cycle: r = src,
@@ -4039,6 +4127,13 @@ hipError_t launch_mg_gersh(hipStream_t s, bool mixed, int T, const ClassDev *d_c
                            const int4 *items, int nitems)
 {
     MSA_PCG_LAUNCH(k_mg_gersh, d_cls, items, nitems);
+}
+
+hipError_t launch_mg_gersh_mf(hipStream_t s, const ClassDev *d_cls, const int4 *items, int nitems)
+{
+    if (nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mg_gersh_mf, dim3(xcd_grid(nitems)), dim3(256), 0, s, d_cls, items, nitems);
+    return hipGetLastError();
 }
 
 hipError_t launch_mg_band_in(hipStream_t s, bool mixed, int T, const ClassDev *cp, int nsamp,

@@ -212,6 +212,10 @@ struct ClassDev {
     // sub-box of the grid (mf_box; neighbour block = own block + offset)
     int mf_cell_id, mf_box;
     int mf_box_lo[3], mf_box_hi[3];
+    // box grid whose blocks are all fully free (every in-box neighbour
+    // couples all three components): the Gershgorin bound is formed
+    // matrix-free (k_mg_gersh_mf)
+    int mf_allfree;
     const int *mf_gblock, *mf_cell, *mf_bpos;
     float *mf_k;
     double *mf_k64;       // [640] the same in FP64 (replacement residuals)
@@ -351,6 +355,7 @@ hipError_t launch_mg_prolong(hipStream_t s, bool mixed, int T, const ClassDev *d
                              const int4 *items, int nitems, int mode);
 hipError_t launch_mg_coarsen_E(hipStream_t s, int T, const ClassDev *cp, const ClassDev *d_cls,
                                int ntiles, int n_elem);
+hipError_t launch_mg_gersh_mf(hipStream_t s, const ClassDev *d_cls, const int4 *items, int nitems);
 hipError_t launch_mg_gersh(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
                            const int4 *items, int nitems);
 hipError_t launch_mg_band_in(hipStream_t s, bool mixed, int T, const ClassDev *cp, int nsamp,
