@@ -3285,12 +3285,14 @@ __global__ __launch_bounds__(256) void k_mg_prolong(const ClassDev *__restrict__
             for (int n = 0; n < 8; n++) {
                 const int ox = n & 1, oy = (n >> 1) & 1, oz = n >> 2;
                 const int x = (i >> 1) + ox, y = (j >> 1) + oy, z = (k2 >> 1) + oz;
-                const bool in = x >= lx && x <= hx && y >= ly && y <= hy && z >= lz && z <= hz;
+                // points of weight 0 (the second point of an even direction)
+                // are read past the end too, so they fetch nothing
+                const bool used = (!ox || (i & 1)) && (!oy || (j & 1)) && (!oz || (k2 & 1));
+                const bool in = used && x >= lx && x <= hx && y >= ly && y <= hy && z >= lz && z <= hz;
                 const unsigned off =
                     in ? (unsigned)((x - lx) + sy * (y - ly) + sz * (z - lz)) * 3u * (unsigned)sizeof(real)
                        : cout;
-                const double w = (ox ? ((i & 1) ? wx : 0.0) : wx) * (oy ? ((j & 1) ? wy : 0.0) : wy) *
-                                 (oz ? ((k2 & 1) ? wz : 0.0) : wz);
+                const double w = wx * wy * wz;
                 real v0, v1, v2;
                 if constexpr (sizeof(real) == 4) {
                     const f32x3 v = buffer_load_f32x3(cb, (int)off, 0, 0);
