@@ -2896,57 +2896,57 @@ __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL f
         y1 = acc[1];
         y2 = acc[2];
     } else {
-    // general grids: the 27 neighbours from the grid-point table
-    const MSA_GLOBAL int *gblock = gbl(C.mf_gblock);
-    unsigned off[27];
-    {
-        const int gp0 = i + nx * (j + ny * k), sy = nx, sz = nx * ny;
+        // general grids: the 27 neighbours from the grid-point table
+        const MSA_GLOBAL int *gblock = gbl(C.mf_gblock);
+        unsigned off[27];
+        {
+            const int gp0 = i + nx * (j + ny * k), sy = nx, sz = nx * ny;
 #pragma unroll
-        for (int n = 0; n < 27; n++) {
-            const int dx = n % 3, dy = (n / 3) % 3, dz = n / 9;
-            const bool in = (dx != 0 || lo[0]) && (dx != 2 || hi[0]) && (dy != 0 || lo[1]) &&
-                            (dy != 2 || hi[1]) && (dz != 0 || lo[2]) && (dz != 2 || hi[2]);
-            const int gp = gp0 + (dx - 1) + (dy - 1) * sy + (dz - 1) * sz;
-            const int b = gblock[in ? gp : gp0];
-            off[n] = (in && b >= 0) ? (unsigned)b * 12u : vout;
-        }
-    }
-    float xv[27][3];
-#pragma unroll
-    for (int n = 0; n < 27; n++) {
-        const f32x3 w = buffer_load_f32x3(vb, (int)off[n], 0, 0);
-        xv[n][0] = w.x;
-        xv[n][1] = w.y;
-        xv[n][2] = w.z;
-    }
-    float Ec[8], cw[8];
-    mf_moduli(C, Es, i, j, k, nx, ny, lo, hi, cK, Ec, cw);
-#pragma unroll
-    for (int c = 0; c < 8; c++) {
-        const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
-        const int a = (1 - cx) + 2 * (1 - cy) + 4 * (1 - cz);  // the node's corner
-        float t[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-        for (int o = 0; o < 8; o++) {
-            const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
-            cfloat *kab = kh + (a * 8 + o) * 9;
-            const float *xb = xv[((cz + oz) * 3 + (cy + oy)) * 3 + (cx + ox)];
-            // one FMA per term (no product-sum trees)
-#pragma unroll
-            for (int p = 0; p < 3; p++) {
-                t[p] = __builtin_fmaf(kab[3 * p], xb[0], t[p]);
-                t[p] = __builtin_fmaf(kab[3 * p + 1], xb[1], t[p]);
-                t[p] = __builtin_fmaf(kab[3 * p + 2], xb[2], t[p]);
+            for (int n = 0; n < 27; n++) {
+                const int dx = n % 3, dy = (n / 3) % 3, dz = n / 9;
+                const bool in = (dx != 0 || lo[0]) && (dx != 2 || hi[0]) && (dy != 0 || lo[1]) &&
+                                (dy != 2 || hi[1]) && (dz != 0 || lo[2]) && (dz != 2 || hi[2]);
+                const int gp = gp0 + (dx - 1) + (dy - 1) * sy + (dz - 1) * sz;
+                const int b = gblock[in ? gp : gp0];
+                off[n] = (in && b >= 0) ? (unsigned)b * 12u : vout;
             }
         }
+        float xv[27][3];
 #pragma unroll
-        for (int p = 0; p < 3; p++) acc[p] = __builtin_fmaf(Ec[c], t[p], acc[p]);
-        __builtin_amdgcn_sched_barrier(0);
-    }
-    if (cM != 0.f) mf_mass(kh + 576, xv, cw, cM, acc);
-    y0 = acc[0];
-    y1 = acc[1];
-    y2 = acc[2];
+        for (int n = 0; n < 27; n++) {
+            const f32x3 w = buffer_load_f32x3(vb, (int)off[n], 0, 0);
+            xv[n][0] = w.x;
+            xv[n][1] = w.y;
+            xv[n][2] = w.z;
+        }
+        float Ec[8], cw[8];
+        mf_moduli(C, Es, i, j, k, nx, ny, lo, hi, cK, Ec, cw);
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+            const int a = (1 - cx) + 2 * (1 - cy) + 4 * (1 - cz);  // the node's corner
+            float t[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+            for (int o = 0; o < 8; o++) {
+                const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
+                cfloat *kab = kh + (a * 8 + o) * 9;
+                const float *xb = xv[((cz + oz) * 3 + (cy + oy)) * 3 + (cx + ox)];
+                // one FMA per term (no product-sum trees)
+#pragma unroll
+                for (int p = 0; p < 3; p++) {
+                    t[p] = __builtin_fmaf(kab[3 * p], xb[0], t[p]);
+                    t[p] = __builtin_fmaf(kab[3 * p + 1], xb[1], t[p]);
+                    t[p] = __builtin_fmaf(kab[3 * p + 2], xb[2], t[p]);
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < 3; p++) acc[p] = __builtin_fmaf(Ec[c], t[p], acc[p]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (cM != 0.f) mf_mass(kh + 576, xv, cw, cM, acc);
+        y0 = acc[0];
+        y1 = acc[1];
+        y2 = acc[2];
     }
 }
 
