@@ -823,6 +823,15 @@ int mg_build(msa_batch *b, int max_levels)
                 bool changed = false;
                 ClassHost &Cc = b->lv.back();
                 MSA_TRY(band_setup(b, Cc, &changed));
+                // MIXED batches factor the coarsest level from an FP64 copy
+                // of its values (the FP32 rounding, amplified by the
+                // condition number of a thin structure's coarse operator,
+                // would weaken the cycle)
+                if (is_mixed(b) && b->T == 1) {
+                    void *p = nullptr;
+                    MSA_TRY(dalloc(b, &p, (size_t)Cc.ntiles * Cc.tile_vals * sizeof(double)));
+                    Cc.dev.val64 = (double *)p;
+                }
                 // small coarsest operators: dense inverse, one launch per cycle
                 if (Cc.dev.bn <= kDenseMax) {
                     void *p = nullptr;

@@ -850,6 +850,11 @@ __global__ __launch_bounds__(256) void k_assemble_gather_g(const ClassDev *__res
 #pragma unroll
                 for (int q = 0; q < 9; q++) m[q] = (real)acc[g][q];
                 store_block<real>(val + (size_t)g * C.tile_vals, slot, rr, m);
+                if constexpr (sizeof(real) == 4) {
+                    if (C.val64)  // coarsest level of a MIXED batch: the FP64 sums too
+                        store_block<double>(gbl(C.val64) + (size_t)(tile0 + g) * C.tile_vals, slot, rr,
+                                            acc[g]);
+                }
             }
         }
     }
@@ -1919,7 +1924,8 @@ __global__ __launch_bounds__(256) void k_band_scatter(const ClassDev *__restrict
                     const int jj = C.binv[3 * col + q];
                     if (jj < 0 || jj > i) continue;
                     AB[(size_t)jj * C.bW + (i - jj)] =
-                        (double)val[val_at<real, T>(slot, 3 * p + q, rr, s)];
+                        C.val64 ? C.val64[(size_t)tile * C.tile_vals + val_at<double, T>(slot, 3 * p + q, rr, s)]
+                                : (double)val[val_at<real, T>(slot, 3 * p + q, rr, s)];
                 }
             }
         }

@@ -212,6 +212,10 @@ struct ClassDev {
     // sub-box of the grid (mf_box; neighbour block = own block + offset)
     int mf_cell_id, mf_box;
     int mf_box_lo[3], mf_box_hi[3];
+    // FP64 copy of the stored values (same layout as a one-sample-tile FP64
+    // batch) for the coarsest multigrid level of a MIXED batch: the band
+    // factor is computed from these, not from the FP32 values (null: none)
+    double *val64;
     // box grid whose blocks are all fully free (every in-box neighbour
     // couples all three components): the Gershgorin bound is formed
     // matrix-free (k_mg_gersh_mf)
