@@ -2678,9 +2678,8 @@ __device__ f32x3 buffer_load_f32x3(__amdgpu_buffer_rsrc_t rsrc, int voffset, int
                                    int aux) __asm("llvm.amdgcn.raw.ptr.buffer.load.v3f32");
 __device__ double buffer_load_f64(__amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset,
                                   int aux) __asm("llvm.amdgcn.raw.ptr.buffer.load.f64");
-typedef double f64x2v __attribute__((ext_vector_type(2)));
-__device__ f64x2v buffer_load_f64x2(__amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset,
-                                    int aux) __asm("llvm.amdgcn.raw.ptr.buffer.load.v2f64");
+__device__ f64x2 buffer_load_f64x2(__amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset,
+                                   int aux) __asm("llvm.amdgcn.raw.ptr.buffer.load.v2f64");
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t wave_buffer(const void *p, unsigned bytes)
 {
     const unsigned long long a = (unsigned long long)p;
@@ -3039,7 +3038,7 @@ __global__ __launch_bounds__(256) void k_residual64_mf(const ClassDev *__restric
                     const bool in = (unsigned)x < (unsigned)sxb && (unsigned)yy < (unsigned)syb &&
                                     (unsigned)z < (unsigned)bzn;
                     const unsigned off = in ? (unsigned)(x + sxb * (yy + syb * z)) * 24u : uout;
-                    const f64x2v v = buffer_load_f64x2(ub, (int)off, 0, 0);
+                    const f64x2 v = buffer_load_f64x2(ub, (int)off, 0, 0);
                     uc[o][0] = v.x;
                     uc[o][1] = v.y;
                     uc[o][2] = buffer_load_f64(ub, (int)(in ? off + 16u : uout), 0, 0);
