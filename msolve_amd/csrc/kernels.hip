@@ -3558,10 +3558,15 @@ __global__ __launch_bounds__(64) void k_mg_dense_solve(const ClassDev *__restric
     const size_t vb = (size_t)tile * C.tile_vec;
     const real *b = (const real *)C.r + vb;
     const MSA_GLOBAL double *X = gbl(C.binvd + (size_t)smp * n * n);
+    // the right-hand side in band order, staged once per workgroup, so the
+    // loads of the inverse's columns do not wait on index loads
+    __shared__ double bl[kDenseMax];
+    for (int jj = threadIdx.x; jj < n; jj += 64) bl[jj] = (double)b[(size_t)C.bperm[jj] * T + s];
+    __syncthreads();
     double acc = 0.0;
     if (i < n) {
-        for (int jj = 0; jj < n; jj++)
-            acc += X[(size_t)jj * n + i] * (double)b[(size_t)C.bperm[jj] * T + s];
+#pragma unroll 8
+        for (int jj = 0; jj < n; jj++) acc += X[(size_t)jj * n + i] * bl[jj];
         ((real *)C.z)[vb + (size_t)C.bperm[i] * T + s] = (real)acc;
     }
 }
