@@ -3232,12 +3232,21 @@ __global__ __launch_bounds__(256) void k_mg_restrict(const ClassDev *__restrict_
             acc[2] += (ff & 4) ? w * (double)v2 : 0.0;
         }
         const uint8_t cf = C.block_free[cb];
+        if constexpr (T == 1) {
+            const real v0 = (cf & 1) ? (real)acc[0] : (real)0;
+            const real v1 = (cf & 2) ? (real)acc[1] : (real)0;
+            const real v2 = (cf & 4) ? (real)acc[2] : (real)0;
+            const vec3<real> dv = ld3(gbl(dinv), cb);
+            st3(gbl(bc), cb, v0, v1, v2);
+            st3(gbl(zc), cb, dv.x * v0, dv.y * v1, dv.z * v2);
+        } else {
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-            const size_t o = ((size_t)cb * 3 + d) * T + s;
-            const real v = ((cf >> d) & 1) ? (real)acc[d] : (real)0;
-            bc[o] = v;
-            zc[o] = dinv[o] * v;
+            for (int d = 0; d < 3; d++) {
+                const size_t o = ((size_t)cb * 3 + d) * T + s;
+                const real v = ((cf >> d) & 1) ? (real)acc[d] : (real)0;
+                bc[o] = v;
+                zc[o] = dinv[o] * v;
+            }
         }
     }
 }
@@ -3325,12 +3334,22 @@ __global__ __launch_bounds__(256) void k_mg_prolong(const ClassDev *__restrict__
             acc[2] += w * (double)v2;
         }
         const uint8_t ff = C.block_free[fb];
+        if constexpr (T == 1) {
+            // the node's three components move as one 12-byte access; fixed
+            // components are written back unchanged
+            MSA_GLOBAL real *Ag = gbl(A);
+            const vec3<real> a = ld3(Ag, fb);
+            st3(Ag, fb, (ff & 1) ? (real)((double)a.x + acc[0]) : a.x,
+                (ff & 2) ? (real)((double)a.y + acc[1]) : a.y,
+                (ff & 4) ? (real)((double)a.z + acc[2]) : a.z);
+        } else {
 #pragma unroll
-        for (int d = 0; d < 3; d++)
-            if ((ff >> d) & 1) {
-                const size_t o = ((size_t)fb * 3 + d) * T + s;
-                A[o] = (real)((double)A[o] + acc[d]);
-            }
+            for (int d = 0; d < 3; d++)
+                if ((ff >> d) & 1) {
+                    const size_t o = ((size_t)fb * 3 + d) * T + s;
+                    A[o] = (real)((double)A[o] + acc[d]);
+                }
+        }
     }
 }
 
