@@ -1460,12 +1460,23 @@ __global__ __launch_bounds__(256) void k_ru_fold(const ClassDev *__restrict__ cl
     for (int slice = s0 + wave; slice < s1; slice += kWaves) {
         const int br = slice * R + rr;
         if (br >= C.nb) continue;
+        if constexpr (T == 1) {
+            // each vector's three components move as one access
+            MSA_GLOBAL double *u = gbl(C.u64 + vb), *r = gbl(C.r64 + vb);
+            MSA_GLOBAL float *xg = gbl(x);
+            const vec3<double> uv = ld3(u, br);
+            const vec3<float> xv = ld3(xg, br);
+            st3(u, br, uv.x + sc * (double)xv.x, uv.y + sc * (double)xv.y, uv.z + sc * (double)xv.z);
+            st3(xg, br, 0.f, 0.f, 0.f);
+            st3(r, br, fval(C, vb, br, 0, T, s), fval(C, vb, br, 1, T, s), fval(C, vb, br, 2, T, s));
+        } else {
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-            const size_t o = ((size_t)br * 3 + d) * T + s;
-            C.u64[vb + o] += sc * (double)x[o];
-            x[o] = 0.f;
-            C.r64[vb + o] = fval(C, vb, br, d, T, s);
+            for (int d = 0; d < 3; d++) {
+                const size_t o = ((size_t)br * 3 + d) * T + s;
+                C.u64[vb + o] += sc * (double)x[o];
+                x[o] = 0.f;
+                C.r64[vb + o] = fval(C, vb, br, d, T, s);
+            }
         }
     }
 }
@@ -1531,15 +1542,30 @@ __global__ __launch_bounds__(256) void k_ru_restart(const ClassDev *__restrict__
         for (int slice = s0 + wave; slice < s1; slice += kWaves) {
             const int br = slice * R + rr;
             if (br >= C.nb) continue;
+            if constexpr (T == 1) {
+                // each vector's three components move as one access; the
+                // same arithmetic and summation order as the general form
+                const vec3<double> rd = ld3(gbl((const double *)C.r64 + vb), br);
+                const vec3<float> dv = ld3(gbl(dinv), br), pv = ld3(gbl((const float *)p0), br);
+                const float r0 = (float)(rd.x * inv), r1 = (float)(rd.y * inv), r2 = (float)(rd.z * inv);
+                const float z0 = dv.x * r0, z1 = dv.y * r1, z2 = dv.z * r2;
+                st3(gbl(r), br, r0, r1, r2);
+                st3(gbl(z), br, z0, z1, z2);
+                st3(gbl(p0), br, pv.x * ratio, pv.y * ratio, pv.z * ratio);
+                acc[0] += (double)(r0 * z0);
+                acc[0] += (double)(r1 * z1);
+                acc[0] += (double)(r2 * z2);
+            } else {
 #pragma unroll
-            for (int d = 0; d < 3; d++) {
-                const size_t o = ((size_t)br * 3 + d) * T + s;
-                const float rv = (float)(C.r64[vb + o] * inv);
-                const float zv = dinv[o] * rv;
-                r[o] = rv;
-                z[o] = zv;
-                p0[o] *= ratio;
-                acc[0] += (double)(rv * zv);
+                for (int d = 0; d < 3; d++) {
+                    const size_t o = ((size_t)br * 3 + d) * T + s;
+                    const float rv = (float)(C.r64[vb + o] * inv);
+                    const float zv = dinv[o] * rv;
+                    r[o] = rv;
+                    z[o] = zv;
+                    p0[o] *= ratio;
+                    acc[0] += (double)(rv * zv);
+                }
             }
         }
     }
