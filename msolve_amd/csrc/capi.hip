@@ -1417,12 +1417,9 @@ int msa_batch_create(msa_ctx *ctx, int n_classes, msa_mesh *const *meshes, const
     MSA_CHECK(n_classes >= 1 && n_classes <= 32, MSA_ERR_ARG, "n_classes=%d (1..32)", n_classes);
     MSA_CHECK(precision == MSA_FP64 || precision == MSA_MIXED, MSA_ERR_ARG, "precision=%d",
               precision);
-    int maxs = 0;
     for (int c = 0; c < n_classes; c++) {
         MSA_CHECK(meshes[c] && meshes[c]->ctx == ctx, MSA_ERR_ARG, "mesh %d invalid", c);
         MSA_CHECK(n_samples[c] >= 1, MSA_ERR_ARG, "n_samples[%d]=%d", c, n_samples[c]);
-        maxs = std::max(maxs, n_samples[c]);
-        (void)maxs;
     }
     if (tile == 0) tile = 1;  // one sample per lane tile: no lane waits for a slower sample
     MSA_CHECK(tile == 1 || tile == 2 || tile == 4 || tile == 8 || tile == 16 || tile == 32 ||
