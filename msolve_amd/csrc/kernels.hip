@@ -3146,6 +3146,38 @@ __global__ __launch_bounds__(256) void k_mg_smooth(const ClassDev *__restrict__ 
                                     br, y[0], y[1], y[2]);
             else
                 row_product<real, T>(val, cidx, src, base, W, rr, s, y[0], y[1], y[2]);
+            if constexpr (T == 1) {
+                // the row's three components of each vector move as one
+                // 12- or 24-byte access; the same arithmetic as below
+                real bv[3], dv[3], xa[3], xb[3], out[3];
+                auto get = [&](const MSA_GLOBAL real *p, real (&v)[3]) {
+                    const vec3<real> t = ld3(p, br);
+                    v[0] = t.x;
+                    v[1] = t.y;
+                    v[2] = t.z;
+                };
+                get(b, bv);
+                if constexpr (STEP != MG_RESID) get(dinv, dv);
+                if constexpr (STEP == MG_PRE) get(z, xa);
+                if constexpr (STEP == MG_POST1 || STEP == MG_POST2) get(A, xa);
+                if constexpr (STEP == MG_POST2) get(B, xb);
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    if constexpr (STEP == MG_PRE) {
+                        const real x1 = xa[d] * ith;
+                        out[d] = x1 + c1 * x1 + c2 * dv[d] * (bv[d] - y[d] * ith);
+                    } else if constexpr (STEP == MG_RESID) {
+                        out[d] = bv[d] - y[d];
+                    } else if constexpr (STEP == MG_POST1) {
+                        out[d] = xa[d] + ith * dv[d] * (bv[d] - y[d]);
+                    } else {
+                        out[d] = xb[d] + c1 * (xb[d] - xa[d]) + c2 * dv[d] * (bv[d] - y[d]);
+                        rz += bv[d] * out[d];
+                    }
+                }
+                st3(dst, br, out[0], out[1], out[2]);
+                continue;
+            }
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 const size_t o = ((size_t)br * 3 + d) * T + s;
