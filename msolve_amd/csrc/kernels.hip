@@ -921,6 +921,34 @@ __global__ __launch_bounds__(256) void k_pcg_init(const ClassDev *__restrict__ c
         for (int slice = s0 + wave; slice < s1; slice += kWaves) {
             const int br = slice * R + rr;
             if (br >= C.nb) continue;
+            if constexpr (T == 1) {
+                // each vector's three components move as one access; the
+                // same arithmetic and summation order as below
+                real bq[3];
+                if (mode) {
+                    const vec3<double> t = ld3(gbl(r64), br);
+                    bq[0] = (real)(t.x * sc);
+                    bq[1] = (real)(t.y * sc);
+                    bq[2] = (real)(t.z * sc);
+                } else {
+#pragma unroll
+                    for (int d = 0; d < 3; d++) bq[d] = (real)fval(C, vb, br, d, T, s);
+                }
+                const vec3<real> dv = ld3(gbl(dinv), br);
+                const real z0 = dv.x * bq[0], z1 = dv.y * bq[1], z2 = dv.z * bq[2];
+                st3(gbl(r), br, bq[0], bq[1], bq[2]);
+                st3(gbl(x), br, (real)0, (real)0, (real)0);
+                st3(gbl(z), br, z0, z1, z2);
+                st3(gbl(p0), br, (real)0, (real)0, (real)0);
+                st3(gbl(p1), br, (real)0, (real)0, (real)0);
+                acc[0] += (double)(bq[0] * z0);
+                acc[1] += (double)(bq[0] * bq[0]);
+                acc[0] += (double)(bq[1] * z1);
+                acc[1] += (double)(bq[1] * bq[1]);
+                acc[0] += (double)(bq[2] * z2);
+                acc[1] += (double)(bq[2] * bq[2]);
+                continue;
+            }
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 const size_t o = ((size_t)br * 3 + d) * T + s;
@@ -3091,9 +3119,11 @@ __global__ __launch_bounds__(256) void k_residual64_mf(const ClassDev *__restric
             for (int p = 0; p < 3; p++) y[p] += Ec * t3[p] + cM * m3[p];
         }
         const uint8_t fr = C.block_free[br];
-#pragma unroll
-        for (int p = 0; p < 3; p++)
-            if ((fr >> p) & 1) r[(size_t)br * 3 + p] -= y[p];
+        // the row's three components move as one 24-byte access; fixed
+        // components are written back unchanged
+        const vec3<double> rv = ld3((const MSA_GLOBAL double *)r, br);
+        st3(r, br, (fr & 1) ? rv.x - y[0] : rv.x, (fr & 2) ? rv.y - y[1] : rv.y,
+            (fr & 4) ? rv.z - y[2] : rv.z);
     }
 }
 
