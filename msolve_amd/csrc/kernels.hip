@@ -881,11 +881,17 @@ __global__ __launch_bounds__(256) void k_dinv(const ClassDev *__restrict__ cls,
         if (br >= C.nb) continue;
         const int slot = C.slice_ptr[slice] + C.diag_pos[br];
         const uint8_t fr = C.block_free[br];
+        real w[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             const real v = val[val_at<real, T>(slot, 4 * d, rr, s)];
-            dinv[((size_t)br * 3 + d) * T + s] =
-                (((fr >> d) & 1) && v != (real)0) ? (real)1 / v : (real)0;
+            w[d] = (((fr >> d) & 1) && v != (real)0) ? (real)1 / v : (real)0;
+        }
+        if constexpr (T == 1) {
+            st3(gbl(dinv), br, w[0], w[1], w[2]);  // one 12- or 24-byte store
+        } else {
+#pragma unroll
+            for (int d = 0; d < 3; d++) dinv[((size_t)br * 3 + d) * T + s] = w[d];
         }
     }
 }
@@ -1217,6 +1223,14 @@ __global__ __launch_bounds__(256) void k_ir_begin(const ClassDev *__restrict__ c
         for (int slice = s0 + wave; slice < s1; slice += kWaves) {
             const int br = slice * R + rr;
             if (br >= C.nb) continue;
+            if constexpr (T == 1) {
+                // the loads first, then one 24-byte store per vector
+                const double f0 = fval(C, vb, br, 0, T, s), f1 = fval(C, vb, br, 1, T, s),
+                             f2 = fval(C, vb, br, 2, T, s);
+                if (!warm) st3(gbl(C.u64 + vb), br, 0.0, 0.0, 0.0);
+                st3(gbl(C.r64 + vb), br, f0, f1, f2);
+                continue;
+            }
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 const size_t o = vb + ((size_t)br * 3 + d) * T + s;
