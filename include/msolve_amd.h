@@ -116,6 +116,19 @@ int msa_mesh_destroy(msa_mesh *mesh);
  * MSA_OK or MSA_NEG_DETJ.  FP64 on the GPU. */
 int msa_ke_packed(msa_ctx *ctx, int n_elem, const double *xyz8,
                   const double *E, double nu, double *ke300, int *elem_status);
+/* No counterpart in the reference.  msa_ke_packed without any device
+ * allocation, for callers that cannot allocate (code under HIP graph
+ * capture elsewhere in the process, tight serving loops, harnesses that
+ * forbid hipMalloc): same arguments, same kernel and bit-identical output.
+ * The device staging lives in one fixed array per GPU (about 10.7 MB, held
+ * until the process ends), so n_elem is limited to 4096 per call; larger
+ * n_elem returns MSA_ERR_ARG.  Calls that overlap on one GPU (several
+ * threads or contexts on the same device) share that array and race: the
+ * caller serialises them.  The context must come from msa_ctx_create, which
+ * looks up the array's device address. */
+int msa_ke_packed_noalloc(msa_ctx *ctx, int n_elem, const double *xyz8,
+                          const double *E, double nu, double *ke300,
+                          int *elem_status);
 
 /* ----------------------------------------------------------- MC batch
  * A batch solves many stochastic-modulus samples at once: n_classes meshes

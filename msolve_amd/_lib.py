@@ -28,7 +28,8 @@ EXPORTS = [
     "CALCH8STRAINS", "CALCH8FORCES", "CALCH8MLUMPED", "msa_batch_element_results",
     "msa_ctx_create", "msa_ctx_destroy", "msa_ctx_stream", "msa_ctx_synchronize",
     "msa_enumerate_dofs", "msa_mesh_create", "msa_mesh_info", "msa_mesh_destroy",
-    "msa_ke_packed", "msa_batch_create", "msa_batch_destroy", "msa_batch_bytes",
+    "msa_ke_packed", "msa_ke_packed_noalloc", "msa_batch_create", "msa_batch_destroy",
+    "msa_batch_bytes",
     "msa_batch_set_modulus", "msa_batch_set_kl", "msa_batch_set_xi", "msa_batch_eval_kl",
     "msa_batch_set_rhs", "msa_batch_assemble", "msa_batch_solve", "msa_batch_run",
     "msa_batch_get_solution", "msa_batch_get_stats", "msa_batch_get_modulus",
@@ -95,6 +96,7 @@ def lib():
         "msa_mesh_info": [_vp, _llp],
         "msa_mesh_destroy": [_vp],
         "msa_ke_packed": [_vp, _i, _dp, _dp, _d, _dp, _ip],
+        "msa_ke_packed_noalloc": [_vp, _i, _dp, _dp, _d, _dp, _ip],
         "msa_batch_create": [_vp, _i, C.POINTER(_vp), _ip, _i, _i, C.POINTER(_vp)],
         "msa_batch_destroy": [_vp],
         "msa_batch_set_modulus": [_vp, _i, _dp],

@@ -16,7 +16,7 @@ SRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIB_DIR, "libmsolve_amd.so")
-SOURCES = ["kernels.hip", "capi.hip", "mesh.cpp"]
+SOURCES = ["kernels.hip", "capi.hip", "mesh.cpp", "noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = [

@@ -1114,6 +1114,8 @@ int msa_ctx_create(int device, msa_ctx **out)
     MSA_CHECK(device >= 0 && device < n, MSA_ERR_ARG, "device %d out of range (%d devices)",
               device, n);
     MSA_HIP(hipSetDevice(device));
+    // this device's address of the no-allocation entry point's array
+    MSA_TRY(noalloc_setup(device));
     msa_ctx *c = new msa_ctx();
     c->device = device;
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);

@@ -230,6 +230,12 @@ constexpr int kBandChunkRows = 512;  // rows per partial of the backward solve
 constexpr int kDenseMax = 256;       // coarsest multigrid levels up to this size use a dense inverse
 constexpr int kBandOneMax = 2048;    // bands up to this many rows: both sweeps in one launch
 constexpr int kKlTiles = 8;          // tiles per thread of the KL field evaluation
+constexpr int kNoallocMaxElem = 4096; // elements per call of msa_ke_packed_noalloc
+
+// The no-allocation entry point's device array (noalloc.hip): looks up the
+// current device's address of the array once; msa_ctx_create calls it.
+int noalloc_setup(int device);
+size_t noalloc_scratch_bytes();
 
 // Newmark coefficients (NewmarkDynamicAnalyzer.cs:53-77) and the load history.
 struct DynCoef {
