@@ -36,7 +36,7 @@ enum {
 enum { MSA_FP64 = 0, MSA_MIXED = 1 };
 enum { MSA_REFINE_RESTART = 0, MSA_REFINE_RELIABLE = 1 };
 enum { MSA_SOLVER_PCG = 0, MSA_SOLVER_DIRECT = 1 };
-enum { MSA_PRECOND_JACOBI = 0, MSA_PRECOND_MULTIGRID = 1 };
+enum { MSA_PRECOND_JACOBI = 0, MSA_PRECOND_MULTIGRID = 1, MSA_PRECOND_FIRST_SAMPLE = 2 };
 
 typedef struct msa_ctx msa_ctx;
 typedef struct msa_mesh msa_mesh;
@@ -211,6 +211,16 @@ int msa_batch_solve(msa_batch *b, double tol, int max_it, double inner_tol,
  * zero (period and drop unused). */
 int msa_batch_set_refinement(msa_batch *b, int method, int period, double drop);
 
+/* Captured PCG iteration blocks (no counterpart in the reference): info[0]
+ * graphs captured so far, [1] kernel nodes of the current graph (0: none),
+ * [2] PCG iterations per replay, [3] graph launches so far.  Every captured
+ * graph is checked on the host before it is instantiated: kernel nodes only,
+ * non-empty grids, workgroups within each kernel's launch bounds, LDS within
+ * a CU; a failed check fails the solve with MSA_ERR_STATE.  Changing the
+ * tolerance, the iteration limit, the refinement scheme or the
+ * preconditioner re-captures at the next solve. */
+int msa_batch_graph_info(msa_batch *b, long long *info);
+
 /* Element integration precision of a MIXED batch, whose stiffness values are
  * stored in FP32: MSA_FP64 (default) integrates in FP64 and rounds each
  * assembled value; MSA_MIXED integrates in FP32.  FP32 integration leaves
@@ -314,9 +324,16 @@ int msa_batch_direct_info(msa_batch *b, int cls, long long *info);
  * PCG keeps its convergence guarantee; the solution is the same, within the
  * tolerance, as with Jacobi.  The hierarchy is built by the first call with
  * MSA_PRECOND_MULTIGRID (later calls must pass the same max_levels) and
- * re-assembled by every msa_batch_assemble.  Unless msa_batch_set_refinement
- * was called, MIXED batches check for residual replacements every 2
- * iterations with a 1e-3 trigger under multigrid (8 and 0.1 under Jacobi). */
+ * re-assembled by every msa_batch_assemble.
+ * MSA_PRECOND_FIRST_SAMPLE: the reference's Monte Carlo preconditioner
+ * itself (MonteCarloSolverPCGDirectMatrixCalculator.cs:38-80): the band
+ * Cholesky factor (= LDL^T) of the first sample's assembled K of each class,
+ * computed at assembly and shared by every sample of the class; z = L^-T
+ * L^-1 r by forward and backward band sweeps.  One factor of n x (band + 64)
+ * doubles per class (msa_batch_direct_info); max_levels and ratio unused.
+ * Unless msa_batch_set_refinement was called, MIXED batches check for
+ * residual replacements every 2 iterations with a 1e-3 trigger under
+ * multigrid and the first-sample factor (8 and 0.1 under Jacobi). */
 int msa_batch_set_preconditioner(msa_batch *b, int kind, int max_levels, double ratio);
 /* info[0] coarse levels of class cls (0: Jacobi), [1] free DOFs of the
  * coarsest level, [2] free DOFs summed over the coarse levels, [3] half
@@ -386,6 +403,29 @@ int msa_batch_get_dynamic_state(msa_batch *b, int cls, int sample, double *u,
  * 1, 2, 4, 8, 16, 32, 64; negative = skipped). */
 int msa_autotune_tile(msa_ctx *ctx, msa_mesh *mesh, int n_samples,
                       int precision, int *best_tile, double *ms_per_tile);
+
+/* Whole-solve schedule autotuner (no counterpart in the reference; the batch
+ * schedule of the product replacing SkylineMatrix2D.Multiply inside
+ * SolverPCG.Solve, SolverPCG.cs:105-149).  For each candidate (lane tile T,
+ * slot unroll U) of msa_autotune_candidates it builds a trial batch of the
+ * given meshes with samples_per_class samples each (fixed lognormal moduli,
+ * COV 0.3; unit load on every free DOF), the given precision and
+ * preconditioner, and times one assembly + PCG solve (tolerance 1e-9 MIXED,
+ * 1e-10 FP64) after a warm-up; ms_per_sample[MSA_TUNE_CANDIDATES] (may be
+ * NULL) receives the time per sample (negative: the candidate could not be
+ * built or did not converge).  The fastest is written to *best_tile and
+ * *best_unroll: create the batch with that tile and call
+ * msa_batch_set_schedule with that unroll. */
+#define MSA_TUNE_CANDIDATES 8
+int msa_autotune_schedule(msa_ctx *ctx, int n_classes, msa_mesh *const *meshes,
+                          int samples_per_class, int precision, int precond,
+                          int *best_tile, int *best_unroll, double *ms_per_sample);
+/* The candidates, tiles[MSA_TUNE_CANDIDATES] and unrolls[...]. */
+int msa_autotune_candidates(int *tiles, int *unrolls);
+/* Slots per group of the SpMV's rows (1..4; lane tiles T > 1 only 3, the
+ * default): the loads of a group's slots issue together.  Results do not
+ * depend on it (the slots are summed in the same order). */
+int msa_batch_set_schedule(msa_batch *b, int unroll);
 
 #ifdef __cplusplus
 }

@@ -19,7 +19,8 @@ MSA_ERR_ARG, MSA_ERR_HIP, MSA_ERR_NO_DEVICE, MSA_ERR_OOM, MSA_ERR_STATE = -1, -2
 MSA_FP64, MSA_MIXED = 0, 1
 MSA_REFINE_RESTART, MSA_REFINE_RELIABLE = 0, 1
 MSA_SOLVER_PCG, MSA_SOLVER_DIRECT = 0, 1
-MSA_PRECOND_JACOBI, MSA_PRECOND_MULTIGRID = 0, 1
+MSA_PRECOND_JACOBI, MSA_PRECOND_MULTIGRID, MSA_PRECOND_FIRST_SAMPLE = 0, 1, 2
+TUNE_CANDIDATES = 8
 STATUS_NAMES = {0: "OK", 1: "NOT_CONVERGED", 2: "BREAKDOWN", 3: "NEG_DETJ", 4: "NAN_INPUT"}
 
 # every symbol include/msolve_amd.h declares
@@ -40,7 +41,8 @@ EXPORTS = [
     "msa_batch_get_history", "msa_batch_get_dynamic_state",
     "msa_batch_factorize", "msa_batch_direct_solve", "msa_batch_set_solver",
     "msa_batch_direct_info", "msa_batch_set_preconditioner", "msa_batch_mg_info",
-    "msa_batch_mg_apply",
+    "msa_batch_mg_apply", "msa_batch_graph_info", "msa_autotune_schedule",
+    "msa_autotune_candidates", "msa_batch_set_schedule",
 ]
 
 
@@ -131,6 +133,10 @@ def lib():
         "msa_batch_set_preconditioner": [_vp, _i, _i, _d],
         "msa_batch_mg_info": [_vp, _i, _llp],
         "msa_batch_mg_apply": [_vp, _i, _i, _dp, _dp],
+        "msa_batch_graph_info": [_vp, _llp],
+        "msa_autotune_schedule": [_vp, _i, C.POINTER(_vp), _i, _i, _i, _ip, _ip, _dp],
+        "msa_autotune_candidates": [_ip, _ip],
+        "msa_batch_set_schedule": [_vp, _i],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -305,6 +305,19 @@ class Batch:
         names = ["spmv_dot", "update", "assemble", "residual64", "other", "multigrid"]
         return {k: dict(ms=float(ms[i]), launches=int(n[i])) for i, k in enumerate(names)}
 
+    def set_schedule(self, unroll: int):
+        """Slots per group of the SpMV's rows (1..4 at lane tile 1; 3 otherwise)."""
+        check(lib().msa_batch_set_schedule(self.h, int(unroll)))
+
+    def graph_info(self):
+        """Captured PCG iteration blocks: graphs captured (each validated on
+        the host), kernel nodes of the current one, iterations per replay,
+        replays so far."""
+        info = np.zeros(4, np.int64)
+        check(lib().msa_batch_graph_info(self.h, ptr(info, C.c_longlong)))
+        return dict(captures=int(info[0]), kernel_nodes=int(info[1]),
+                    iters_per_replay=int(info[2]), replays=int(info[3]))
+
     def spmv_bytes(self):
         b = np.zeros(2)
         check(lib().msa_batch_spmv_bytes(self.h, ptr(b, C.c_double)))
@@ -328,3 +341,20 @@ def autotune_tile(ctx: Context, mesh: Mesh, n_samples: int, precision=L.MSA_MIXE
     check(lib().msa_autotune_tile(ctx.h, mesh.h, n_samples, precision, C.byref(best),
                                   ptr(ms, C.c_double)))
     return best.value, dict(zip([1, 2, 4, 8, 16, 32, 64], ms.tolist()))
+
+
+def autotune_schedule(ctx: Context, meshes, samples_per_class: int, precision=L.MSA_MIXED,
+                      precond=L.MSA_PRECOND_MULTIGRID):
+    """Times one assembly + solve per sample for every candidate (lane tile,
+    slot unroll) on trial batches of `meshes`; returns dict(tile, unroll,
+    table) with table[(tile, unroll)] = ms per sample (negative: skipped)."""
+    arr = (C.c_void_p * len(meshes))(*[m.h for m in meshes])
+    bt, bu = C.c_int(), C.c_int()
+    ms = np.zeros(L.TUNE_CANDIDATES)
+    check(lib().msa_autotune_schedule(ctx.h, len(meshes), arr, int(samples_per_class), precision,
+                                      precond, C.byref(bt), C.byref(bu), ptr(ms, C.c_double)))
+    tiles = np.zeros(L.TUNE_CANDIDATES, np.int32)
+    unrolls = np.zeros(L.TUNE_CANDIDATES, np.int32)
+    check(lib().msa_autotune_candidates(ptr(tiles, C.c_int), ptr(unrolls, C.c_int)))
+    table = {(int(t), int(u)): float(v) for t, u, v in zip(tiles, unrolls, ms)}
+    return dict(tile=bt.value, unroll=bu.value, table=table)

@@ -3,7 +3,9 @@
 // stream; host syncs happen only where a result must reach the host (the
 // convergence poll between captured iteration blocks, downloads).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <random>
 #include <cstdarg>
 #include <cstring>
 #include <mutex>
@@ -64,6 +66,7 @@ struct Timing {
 struct msa_batch {
     msa_ctx *ctx = nullptr;
     int precision = MSA_FP64, T = 1, R = 64;
+    int unroll = 3;  // slots per group of k_pcg_a's rows (msa_batch_set_schedule)
     std::vector<ClassHost> cls;
     ClassDev *d_cls = nullptr;
     std::vector<int4> items;
@@ -99,6 +102,9 @@ struct msa_batch {
     int graph_iters = 0, graph_kind = -1;
     double graph_tol = -1;
     int graph_maxit = -1;
+    long long graph_kernels = 0;   // kernel nodes of the current graph
+    long long graph_captures = 0;  // graphs captured (and validated) so far
+    long long graph_replays = 0;   // launches of captured graphs so far
     Timing timing;
     // geometric multigrid preconditioner (msa_batch_set_preconditioner)
     int precond = MSA_PRECOND_JACOBI;
@@ -364,6 +370,7 @@ int read_count(msa_batch *b, int which, int *out)
 
 int mg_vcycle(msa_batch *b, int pbuf, int mode);
 int mg_values(msa_batch *b);
+int precond_values(msa_batch *b);
 
 // ru_tol > 0: reliable-update mode (inner test off, replacement triggers on).
 int launch_iteration_pair(msa_batch *b, int pbuf, double tol, int max_it, double ru_tol = 0.0)
@@ -371,8 +378,8 @@ int launch_iteration_pair(msa_batch *b, int pbuf, double tol, int max_it, double
     const bool mx = is_mixed(b);
     hipStream_t s = b->ctx->stream;
     MSA_TRY(timed(b, KK_SPMV, [&] {
-        return launch_pcg_a(s, mx, b->T, b->d_cls, b->d_items, (int)b->items.size(), pbuf,
-                            b->d_counts);
+        return launch_pcg_a(s, mx, b->T, b->unroll, b->d_cls, b->d_items, (int)b->items.size(),
+                            pbuf, b->d_counts);
     }));
     MSA_TRY(reduce(b, OP_A));
     MSA_TRY(timed(b, KK_UPDATE, [&] {
@@ -440,6 +447,52 @@ int launch_block(msa_batch *b, int kind, int iters, double tol, int max_it)
     return MSA_OK;
 }
 
+// Host-side check of a captured graph before it is instantiated: only kernel
+// nodes (no memset or copy node: those have given wrong results on replays
+// here), each with a non-empty grid and workgroup, at most 2^32 - 1
+// work-items per dimension (the dispatch packet's 32-bit grid size), a
+// workgroup within the kernel's own limit (its __launch_bounds__), and LDS
+// within the 160 KB of a CU.
+int validate_graph(hipGraph_t g, long long *n_kernels)
+{
+    size_t nn = 0;
+    MSA_HIP(hipGraphGetNodes(g, nullptr, &nn));
+    std::vector<hipGraphNode_t> nodes(nn);
+    if (nn) MSA_HIP(hipGraphGetNodes(g, nodes.data(), &nn));
+    long long k = 0;
+    for (size_t i = 0; i < nn; i++) {
+        hipGraphNodeType t;
+        MSA_HIP(hipGraphNodeGetType(nodes[i], &t));
+        if (t == hipGraphNodeTypeEmpty) continue;
+        MSA_CHECK(t == hipGraphNodeTypeKernel, MSA_ERR_STATE,
+                  "captured graph: node %zu has type %d; only kernel nodes may be recorded", i,
+                  (int)t);
+        hipKernelNodeParams p{};
+        MSA_HIP(hipGraphKernelNodeGetParams(nodes[i], &p));
+        const dim3 G = p.gridDim, B = p.blockDim;
+        MSA_CHECK(G.x >= 1 && G.y >= 1 && G.z >= 1 && B.x >= 1 && B.y >= 1 && B.z >= 1,
+                  MSA_ERR_STATE, "captured graph: kernel node %zu has grid (%u,%u,%u) block (%u,%u,%u)",
+                  i, G.x, G.y, G.z, B.x, B.y, B.z);
+        const unsigned long long lim = 0xffffffffull;
+        MSA_CHECK((unsigned long long)G.x * B.x <= lim && (unsigned long long)G.y * B.y <= lim &&
+                      (unsigned long long)G.z * B.z <= lim,
+                  MSA_ERR_STATE, "captured graph: kernel node %zu exceeds 2^32-1 work-items in a "
+                  "dimension", i);
+        hipFuncAttributes fa{};
+        MSA_HIP(hipFuncGetAttributes(&fa, p.func));
+        const long long threads = (long long)B.x * B.y * B.z;
+        MSA_CHECK(threads <= fa.maxThreadsPerBlock, MSA_ERR_STATE,
+                  "captured graph: kernel node %zu has %lld threads per workgroup, its kernel "
+                  "allows %d", i, threads, fa.maxThreadsPerBlock);
+        MSA_CHECK(fa.sharedSizeBytes + p.sharedMemBytes <= 160u * 1024u, MSA_ERR_STATE,
+                  "captured graph: kernel node %zu needs %zu bytes of LDS", i,
+                  (size_t)(fa.sharedSizeBytes + p.sharedMemBytes));
+        k++;
+    }
+    *n_kernels = k;
+    return MSA_OK;
+}
+
 int build_graph(msa_batch *b, int kind, int iters, double tol, int max_it)
 {
     if (b->graph) {
@@ -451,6 +504,7 @@ int build_graph(msa_batch *b, int kind, int iters, double tol, int max_it)
     int rc = launch_block(b, kind, iters, tol, max_it);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc == MSA_OK && e == hipSuccess) rc = validate_graph(g, &b->graph_kernels);
     if (rc != MSA_OK) {
         if (g) (void)hipGraphDestroy(g);
         return rc;
@@ -463,6 +517,7 @@ int build_graph(msa_batch *b, int kind, int iters, double tol, int max_it)
     b->graph_kind = kind;
     b->graph_tol = tol;
     b->graph_maxit = max_it;
+    b->graph_captures++;
     return MSA_OK;
 }
 
@@ -488,6 +543,7 @@ int pcg_loop(msa_batch *b, int kind, double tol, int max_it, int graph_iters)
         }
         if (use_graph) {
             MSA_HIP(hipGraphLaunch(b->graph, b->ctx->stream));
+            b->graph_replays++;
         } else {
             MSA_TRY(launch_block(b, kind, block, tol, max_it));
         }
@@ -562,43 +618,51 @@ int upload_mesh(msa_mesh *m)
 // Direct solver helpers (band Cholesky).
 namespace {
 
-// Band order and device arrays of every class (built once per batch).
 // Band order, factor storage and solve buffers of one class; *changed is
-// set when its ClassDev changed.
-int band_setup(msa_batch *b, ClassHost &C, bool *changed)
+// set when its ClassDev changed.  shared: one factor for all samples (the
+// first-sample preconditioner), else one per sample (the direct solver).
+int band_setup(msa_batch *b, ClassHost &C, bool *changed, bool shared = false)
 {
     hipStream_t s = b->ctx->stream;
     bool upload = false;
-    {
-        if (!C.dev.bperm) {
-            if (C.bw < 0) MSA_TRY(band_order(C.mesh->h, C.bperm, C.binv, C.bw));
-            void *p = nullptr;
-            MSA_TRY(dalloc(b, &p, C.bperm.size() * sizeof(int)));
-            MSA_HIP(hipMemcpyAsync(p, C.bperm.data(), C.bperm.size() * sizeof(int), hipMemcpyHostToDevice, s));
-            C.dev.bperm = (const int *)p;
-            MSA_TRY(dalloc(b, &p, C.binv.size() * sizeof(int)));
-            MSA_HIP(hipMemcpyAsync(p, C.binv.data(), C.binv.size() * sizeof(int), hipMemcpyHostToDevice, s));
-            C.dev.binv = (const int *)p;
-            C.dev.bn = (int)C.bperm.size();
-            // storage width: bandwidth + one block, rounded to 64
-            C.dev.bW = (int)(((C.bw + 1 + kBandNB + 63) / 64) * 64);
-            upload = true;
-        }
-        const size_t need = (size_t)C.nsamp * C.dev.bn * C.dev.bW;
-        if (C.band_cap < need) {
-            void *p = nullptr;
-            MSA_TRY(dalloc(b, &p, need * sizeof(double)));
-            C.dev.band = (double *)p;
-            C.band_cap = need;
-            MSA_TRY(dalloc(b, &p, (size_t)C.nsamp * C.dev.bn * sizeof(double)));
-            C.dev.by = (double *)p;
-            const int nchunk = (C.dev.bW + kBandChunkRows - 1) / kBandChunkRows;
-            MSA_TRY(dalloc(b, &p, (size_t)C.nsamp * nchunk * kBandNB * sizeof(double)));
-            C.dev.bpart = (double *)p;
-            MSA_TRY(dalloc(b, &p, (size_t)C.nsamp * sizeof(int)));
-            C.dev.bfail = (int *)p;
-            upload = true;
-        }
+    if (!C.dev.bperm) {
+        if (C.bw < 0) MSA_TRY(band_order(C.mesh->h, C.bperm, C.binv, C.bw));
+        void *p = nullptr;
+        MSA_TRY(dalloc(b, &p, C.bperm.size() * sizeof(int)));
+        MSA_HIP(hipMemcpyAsync(p, C.bperm.data(), C.bperm.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        C.dev.bperm = (const int *)p;
+        MSA_TRY(dalloc(b, &p, C.binv.size() * sizeof(int)));
+        MSA_HIP(hipMemcpyAsync(p, C.binv.data(), C.binv.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        C.dev.binv = (const int *)p;
+        C.dev.bn = (int)C.bperm.size();
+        // storage width: bandwidth + one block, rounded to 64
+        C.dev.bW = (int)(((C.bw + 1 + kBandNB + 63) / 64) * 64);
+        upload = true;
+    }
+    const size_t per = (size_t)C.dev.bn * C.dev.bW;
+    const size_t need = shared ? per : (size_t)C.nsamp * per;
+    if (C.band_cap < need) {
+        void *p = nullptr;
+        MSA_TRY(dalloc(b, &p, need * sizeof(double)));
+        C.dev.band = (double *)p;
+        C.band_cap = need;
+        upload = true;
+    }
+    if (!C.dev.by) {
+        void *p = nullptr;
+        MSA_TRY(dalloc(b, &p, (size_t)C.nsamp * C.dev.bn * sizeof(double)));
+        C.dev.by = (double *)p;
+        const int nchunk = (C.dev.bW + kBandChunkRows - 1) / kBandChunkRows;
+        MSA_TRY(dalloc(b, &p, (size_t)C.nsamp * nchunk * kBandNB * sizeof(double)));
+        C.dev.bpart = (double *)p;
+        MSA_TRY(dalloc(b, &p, (size_t)C.nsamp * sizeof(int)));
+        C.dev.bfail = (int *)p;
+        upload = true;
+    }
+    const long long ss = shared ? 0 : (long long)per;
+    if (C.dev.band_ss != ss) {
+        C.dev.band_ss = ss;
+        upload = true;
     }
     if (upload) *changed = true;
     return MSA_OK;
@@ -651,7 +715,8 @@ int band_solve_class(msa_batch *b, size_t c, bool from_fs)
 int direct_stats(msa_batch *b)
 {
     hipStream_t s = b->ctx->stream;
-    MSA_HIP(hipMemsetAsync(b->d_counts, 0, sizeof(int), s));
+    // a fill kernel, not a memset: the Newmark step graph records this
+    MSA_TRY(timed(b, KK_OTHER, [&] { return launch_set_int(s, b->d_counts, 1, 0); }));
     MSA_TRY(timed(b, KK_OTHER, [&] {
         return launch_pcg_init_warm(s, b->T, b->d_cls, b->d_items, (int)b->items.size());
     }));
@@ -1034,11 +1099,90 @@ int mg_values(msa_batch *b)
     return MSA_OK;
 }
 
+// First-sample direct preconditioner (the Monte Carlo calculator of
+// MonteCarloSolverPCGDirectMatrixCalculator.cs:38-80: the LDL^T of the first
+// sample's K, reused for every sample): the band Cholesky of sample 0's
+// assembled values of every class, one factor shared by the class's samples.
+int first_values(msa_batch *b)
+{
+    hipStream_t s = b->ctx->stream;
+    const bool mx = is_mixed(b);
+    bool upload = false;
+    for (auto &C : b->cls) {
+        MSA_TRY(band_setup(b, C, &upload, true));
+        if (C.dev.mg_st != C.dev.st) {
+            C.dev.mg_st = C.dev.st;  // the sweeps select samples by the cycle's mode
+            upload = true;
+        }
+    }
+    if (upload) MSA_TRY(upload_class_table(b));
+    for (auto &C : b->cls) {
+        MSA_TRY(timed(b, KK_OTHER, [&] {
+            return launch_fill_f64(s, C.dev.band, (size_t)C.dev.bn * C.dev.bW, 0.0);
+        }));
+        MSA_HIP(hipMemsetAsync(C.dev.bfail, 0, sizeof(int), s));
+    }
+    MSA_TRY(timed(b, KK_OTHER, [&] {
+        return launch_band_scatter_any(s, mx, b->T, b->d_cls, b->d_items, (int)b->items.size());
+    }));
+    for (size_t c = 0; c < b->cls.size(); c++) {
+        ClassHost &C = b->cls[c];
+        const ClassDev *cp = b->d_cls + c;
+        const int n = C.dev.bn, W = C.dev.bW;
+        for (int k0 = 0; k0 < n; k0 += kBandNB) {
+            const int nk = std::min(kBandNB, n - k0), r0 = k0 + nk, m = std::min(n - r0, W - kBandNB);
+            MSA_TRY(timed(b, KK_ASSEMBLE, [&] { return launch_chol_diag(s, cp, 1, k0, nk); }));
+            MSA_TRY(timed(b, KK_ASSEMBLE, [&] { return launch_chol_panel(s, cp, 1, k0, nk, r0, m); }));
+            MSA_TRY(timed(b, KK_ASSEMBLE, [&] { return launch_chol_update(s, cp, 1, k0, nk, r0, m); }));
+        }
+    }
+    for (size_t c = 0; c < b->cls.size(); c++) {
+        MSA_HIP(hipMemcpyAsync(b->ctx->h_pinned, b->cls[c].dev.bfail, sizeof(int),
+                               hipMemcpyDeviceToHost, s));
+        MSA_HIP(hipStreamSynchronize(s));
+        MSA_CHECK(*b->ctx->h_pinned == 0, MSA_ERR_STATE,
+                  "first-sample preconditioner: the first sample's K of class %zu is not "
+                  "positive definite", c);
+    }
+    b->factored = false;  // a direct solver's factor of sample 0 was overwritten
+    return MSA_OK;
+}
+
+// Values of the chosen preconditioner from the assembled classes.
+int precond_values(msa_batch *b)
+{
+    if (b->precond != MSA_PRECOND_FIRST_SAMPLE) return mg_values(b);
+    b->mg_ready = false;
+    MSA_TRY(first_values(b));
+    b->mg_ready = true;
+    return MSA_OK;
+}
+
+// z = L^-T L^-1 r with the shared first-sample factor, for the samples the
+// mode selects (see k_mg_smooth), and the r.z partials of the PCG.
+int first_apply(msa_batch *b, int mode)
+{
+    hipStream_t s = b->ctx->stream;
+    const bool mx = is_mixed(b);
+    const int T = b->T;
+    for (size_t c = 0; c < b->cls.size(); c++) {
+        ClassHost &C = b->cls[c];
+        const ClassDev *cp = b->d_cls + c;
+        const int n = C.dev.bn, W = C.dev.bW, ns = C.nsamp;
+        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_mg_band_in(s, mx, T, cp, ns, n, mode); }));
+        MSA_TRY(band_sweeps(b, cp, n, W, ns, mode));
+    }
+    return timed(b, KK_OTHER, [&] {
+        return launch_first_out(s, mx, T, b->d_cls, b->d_items, (int)b->items.size(), mode);
+    });
+}
+
 // z = B r for the classes with a hierarchy (B: one V-cycle); their z held
 // D^-1 r on entry.  pbuf selects the free search-direction buffer, mode the
 // samples served (see k_mg_smooth).
 int mg_vcycle(msa_batch *b, int pbuf, int mode)
 {
+    if (b->precond == MSA_PRECOND_FIRST_SAMPLE) return first_apply(b, mode);
     if (b->precond != MSA_PRECOND_MULTIGRID || b->mg_depth == 0) return MSA_OK;
     hipStream_t s = b->ctx->stream;
     const bool mx = is_mixed(b);
@@ -1661,7 +1805,7 @@ int msa_batch_assemble(msa_batch *b, double nu)
     b->assembled = true;
     b->factored = false;
     b->mg_ready = false;
-    if (b->precond == MSA_PRECOND_MULTIGRID) MSA_TRY(mg_values(b));
+    if (b->precond != MSA_PRECOND_JACOBI) MSA_TRY(precond_values(b));
     return MSA_OK;
 }
 
@@ -1680,7 +1824,7 @@ int msa_batch_solve(msa_batch *b, double tol, int max_it, double inner_tol, int 
     hipStream_t s = b->ctx->stream;
     const bool mx = is_mixed(b);
     const int T = b->T, ni = (int)b->items.size();
-    if (!b->mg_ready) MSA_TRY(mg_values(b));
+    if (!b->mg_ready) MSA_TRY(precond_values(b));
     if (b->dyn) {
         // static solve with the effective matrix: per-sample right-hand side = f
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_fs_static(s, T, b->d_cls, b->d_items, ni); }));
@@ -1853,6 +1997,95 @@ int msa_batch_set_probe(msa_batch *b, int cls, int dof)
     return upload_class_table(b);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Records the launches of fn() on the context stream into an instantiated,
+// host-validated graph.
+template <typename F>
+int capture(msa_batch *b, hipGraphExec_t *out, F &&fn)
+{
+    hipStream_t s = b->ctx->stream;
+    MSA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = fn();
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    long long nk = 0;
+    if (rc == MSA_OK && e == hipSuccess) rc = validate_graph(g, &nk);
+    if (rc == MSA_OK && e == hipSuccess) e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    if (rc != MSA_OK) return rc;
+    MSA_HIP(e);
+    b->graph_captures++;
+    return MSA_OK;
+}
+
+// One Newmark step up to its PCG iterations (NewmarkDynamicAnalyzer.Solve,
+// NewmarkDynamicAnalyzer.cs:280-299): the effective right-hand side
+// (:189-206, the mass-acceleration load of the step), its norm, and the
+// start of the step's solve from the previous displacement (or zero), or
+// the direct solve.  Kernels only (no memset), so it can be captured.
+int newmark_head(msa_batch *b, const DynCoef &dc, double tol, int max_it)
+{
+    hipStream_t s = b->ctx->stream;
+    const bool mx = is_mixed(b);
+    const int T = b->T, ni = (int)b->items.size(), ncol = max_colours(b);
+    MSA_TRY(timed(b, KK_OTHER, [&] { return launch_dyn_prep(s, mx, T, b->d_cls, b->d_items, ni, dc); }));
+    for (int k = 0; k < ncol; k++) {
+        int nblk = 0;
+        ColourLaunch cl = colour_launch(b, k, &nblk);
+        MSA_TRY(timed(b, KK_RES64, [&] { return launch_dyn_rhs(s, T, b->d_cls, cl, nblk, b->nu, dc.betaR, max_tiles(b)); }));
+    }
+    MSA_TRY(timed(b, KK_OTHER, [&] { return launch_fs_norm(s, T, b->d_cls, b->d_items, ni); }));
+    MSA_TRY(reduce(b, OP_FNORM));
+    if (b->solver == MSA_SOLVER_DIRECT) {
+        for (size_t c = 0; c < b->cls.size(); c++) MSA_TRY(band_solve_class(b, c, true));
+        return direct_stats(b);
+    }
+    if (!mx) {
+        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_set_int(s, b->d_counts, 1, 0); }));
+        if (dc.start) {
+            MSA_TRY(timed(b, KK_OTHER, [&] { return launch_pcg_init_warm(s, T, b->d_cls, b->d_items, ni); }));
+            MSA_TRY(mg_vcycle(b, 1, 0));
+            return reduce(b, OP_INIT2, tol);
+        }
+        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_pcg_init(s, false, T, b->d_cls, b->d_items, ni, 0, b->d_counts); }));
+        MSA_TRY(mg_vcycle(b, 1, 0));
+        return reduce(b, OP_INIT0);
+    }
+    MSA_TRY(timed(b, KK_OTHER, [&] { return launch_set_int(s, b->d_counts, 2, 0); }));
+    const int warm = dc.start ? 1 : 0;
+    MSA_TRY(timed(b, KK_OTHER, [&] { return launch_ir_begin(s, T, b->d_cls, b->d_items, ni, b->d_counts + 1, warm); }));
+    for (int k = 0; k < ncol; k++) {
+        int nblk = 0;
+        ColourLaunch cl = colour_launch(b, k, &nblk);
+        MSA_TRY(timed(b, KK_RES64, [&] { return launch_residual64(s, T, b->d_cls, cl, nblk, b->nu, 0, max_tiles(b)); }));
+    }
+    MSA_TRY(timed(b, KK_OTHER, [&] {
+        return launch_ir_norm(s, T, b->d_cls, b->d_items, ni, tol, 1 << 30, b->d_counts + 1);
+    }));
+    MSA_TRY(reduce(b, OP_IRNORM, tol, 0, 1 << 30));
+    MSA_TRY(timed(b, KK_OTHER, [&] { return launch_pcg_init(s, true, T, b->d_cls, b->d_items, ni, 1, b->d_counts); }));
+    MSA_TRY(mg_vcycle(b, 1, 3));
+    return reduce(b, OP_INIT1);
+}
+
+// The end of a Newmark step: velocity and acceleration update
+// (NewmarkDynamicAnalyzer.cs:301-315), probe record, step counter.
+int newmark_tail(msa_batch *b, const DynCoef &dc)
+{
+    hipStream_t s = b->ctx->stream;
+    MSA_TRY(timed(b, KK_OTHER, [&] {
+        return launch_dyn_update(s, is_mixed(b), b->T, b->d_cls, b->d_items, (int)b->items.size(), dc);
+    }));
+    return timed(b, KK_OTHER, [&] { return launch_step_advance(s, b->d_step); });
+}
+
+}  // namespace
+
+extern "C" {
+
 int msa_batch_newmark(msa_batch *b, int nsteps, const double *acc, int acc_dir, double tol,
                       int max_it, int graph_iters, int warm_start)
 {
@@ -1873,7 +2106,7 @@ int msa_batch_newmark(msa_batch *b, int nsteps, const double *acc, int acc_dir, 
     MSA_HIP(hipSetDevice(b->ctx->device));
     hipStream_t s = b->ctx->stream;
     const int T = b->T, ni = (int)b->items.size();
-    if (b->solver != MSA_SOLVER_DIRECT && !b->mg_ready) MSA_TRY(mg_values(b));
+    if (b->solver != MSA_SOLVER_DIRECT && !b->mg_ready) MSA_TRY(precond_values(b));
     if (acc_dir >= 0) {
         if (b->acc_cap < nsteps) {
             void *p = nullptr;
@@ -1916,55 +2149,56 @@ int msa_batch_newmark(msa_batch *b, int nsteps, const double *acc, int acc_dir, 
     dc.acc = acc_dir >= 0 ? b->d_acc : nullptr;
     dc.step = b->d_step;
     dc.start = warm_start;
-    for (int step = 0; step < nsteps; step++) {
-        // right-hand side of the step
-        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_dyn_prep(s, mx, T, b->d_cls, b->d_items, ni, dc); }));
-        for (int k = 0; k < ncol; k++) {
-            int nblk = 0;
-            ColourLaunch cl = colour_launch(b, k, &nblk);
-            MSA_TRY(timed(b, KK_RES64, [&] { return launch_dyn_rhs(s, T, b->d_cls, cl, nblk, b->nu, dc.betaR, max_tiles(b)); }));
+    const bool direct = b->solver == MSA_SOLVER_DIRECT;
+    const int kind = mx ? 1 : 0;
+    if (graph_iters <= 0 || b->timing.on) {
+        for (int step = 0; step < nsteps; step++) {
+            MSA_TRY(newmark_head(b, dc, tol, max_it));
+            if (!direct) MSA_TRY(pcg_loop(b, kind, tol, max_it, graph_iters));
+            MSA_TRY(newmark_tail(b, dc));
         }
-        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_fs_norm(s, T, b->d_cls, b->d_items, ni); }));
-        MSA_TRY(reduce(b, OP_FNORM));
-        // solve from the previous displacement
-        if (b->solver == MSA_SOLVER_DIRECT) {
-            for (size_t c = 0; c < b->cls.size(); c++) MSA_TRY(band_solve_class(b, c, true));
-            MSA_TRY(direct_stats(b));
-        } else if (!mx && warm_start) {
-            MSA_HIP(hipMemsetAsync(b->d_counts, 0, sizeof(int), s));
-            MSA_TRY(timed(b, KK_OTHER, [&] { return launch_pcg_init_warm(s, T, b->d_cls, b->d_items, ni); }));
-            MSA_TRY(mg_vcycle(b, 1, 0));
-            MSA_TRY(reduce(b, OP_INIT2, tol));
-            MSA_TRY(pcg_loop(b, 0, tol, max_it, graph_iters));
-        } else if (!mx) {
-            MSA_HIP(hipMemsetAsync(b->d_counts, 0, sizeof(int), s));
-            MSA_TRY(timed(b, KK_OTHER, [&] { return launch_pcg_init(s, false, T, b->d_cls, b->d_items, ni, 0, b->d_counts); }));
-            MSA_TRY(mg_vcycle(b, 1, 0));
-            MSA_TRY(reduce(b, OP_INIT0));
-            MSA_TRY(pcg_loop(b, 0, tol, max_it, graph_iters));
-        } else {
-            MSA_HIP(hipMemsetAsync(b->d_counts, 0, 2 * sizeof(int), s));
-            const int warm = warm_start ? 1 : 0;
-            MSA_TRY(timed(b, KK_OTHER, [&] { return launch_ir_begin(s, T, b->d_cls, b->d_items, ni, b->d_counts + 1, warm); }));
-            for (int k = 0; k < ncol; k++) {
-                int nblk = 0;
-                ColourLaunch cl = colour_launch(b, k, &nblk);
-                MSA_TRY(timed(b, KK_RES64, [&] { return launch_residual64(s, T, b->d_cls, cl, nblk, b->nu, 0, max_tiles(b)); }));
-            }
-            MSA_TRY(timed(b, KK_OTHER, [&] {
-                return launch_ir_norm(s, T, b->d_cls, b->d_items, ni, tol, 1 << 30, b->d_counts + 1);
-            }));
-            MSA_TRY(reduce(b, OP_IRNORM, tol, 0, 1 << 30));
-            MSA_TRY(timed(b, KK_OTHER, [&] { return launch_pcg_init(s, true, T, b->d_cls, b->d_items, ni, 1, b->d_counts); }));
-            MSA_TRY(mg_vcycle(b, 1, 3));
-            MSA_TRY(reduce(b, OP_INIT1));
-            MSA_TRY(pcg_loop(b, 1, tol, max_it, graph_iters));
-        }
-        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_dyn_update(s, mx, T, b->d_cls, b->d_items, ni, dc); }));
-        MSA_TRY(timed(b, KK_OTHER, [&] { return launch_step_advance(s, b->d_step); }));
+        MSA_HIP(hipStreamSynchronize(s));
+        return MSA_OK;
     }
-    MSA_HIP(hipStreamSynchronize(s));
-    return MSA_OK;
+    // Captured steps: G0 = head + first iteration block (step 0), G1 = the
+    // previous step's update + head + first block (steps 1..n-1), G2 = the
+    // last update.  One graph launch and one poll per step; further blocks
+    // (the PCG block graph) only while samples are still iterating.
+    int block = graph_iters;
+    if (kind == 1) block = ((block + b->period - 1) / b->period) * b->period;
+    if (!direct && (b->graph == nullptr || b->graph_iters != block || b->graph_kind != kind ||
+                    b->graph_tol != tol || b->graph_maxit != max_it))
+        MSA_TRY(build_graph(b, kind, block, tol, max_it));
+    hipGraphExec_t G[3] = {nullptr, nullptr, nullptr};
+    auto destroy = [&] {
+        for (auto &g : G)
+            if (g) (void)hipGraphExecDestroy(g);
+    };
+    int rc = MSA_OK;
+    for (int k = 0; k < 3 && rc == MSA_OK; k++)
+        rc = capture(b, &G[k], [&]() -> int {
+            if (k >= 1) MSA_TRY(newmark_tail(b, dc));
+            if (k == 2) return MSA_OK;
+            MSA_TRY(newmark_head(b, dc, tol, max_it));
+            return direct ? MSA_OK : launch_block(b, kind, block, tol, max_it);
+        });
+    for (int step = 0; step < nsteps && rc == MSA_OK; step++) {
+        hipError_t e = hipGraphLaunch(G[step == 0 ? 0 : 1], s);
+        if (e != hipSuccess) {
+            set_error("Newmark step graph launch: %s", hipGetErrorString(e));
+            rc = MSA_ERR_HIP;
+            break;
+        }
+        b->graph_replays++;
+        if (!direct) rc = pcg_loop(b, kind, tol, max_it, graph_iters);
+    }
+    if (rc == MSA_OK && hipGraphLaunch(G[2], s) != hipSuccess) {
+        set_error("Newmark update graph launch failed");
+        rc = MSA_ERR_HIP;
+    }
+    if (rc == MSA_OK && hipStreamSynchronize(s) != hipSuccess) rc = MSA_ERR_HIP;
+    destroy();
+    return rc;
 }
 
 // ---------------------------------------------------------------- direct solver
@@ -1977,6 +2211,7 @@ int msa_batch_factorize(msa_batch *b)
     MSA_HIP(hipSetDevice(b->ctx->device));
     hipStream_t s = b->ctx->stream;
     MSA_TRY(prepare_band(b));
+    if (b->precond == MSA_PRECOND_FIRST_SAMPLE) b->mg_ready = false;  // its shared factor is overwritten
     for (auto &C : b->cls) {
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_fill_f64(s, C.dev.band, C.band_cap, 0.0); }));
         MSA_HIP(hipMemsetAsync(C.dev.bfail, 0, (size_t)C.nsamp * sizeof(int), s));
@@ -2024,8 +2259,9 @@ int msa_batch_direct_solve(msa_batch *b)
 int msa_batch_set_preconditioner(msa_batch *b, int kind, int max_levels, double ratio)
 {
     MSA_CHECK(b, MSA_ERR_ARG, "null batch");
-    MSA_CHECK(kind == MSA_PRECOND_JACOBI || kind == MSA_PRECOND_MULTIGRID, MSA_ERR_ARG,
-              "preconditioner kind=%d", kind);
+    MSA_CHECK(kind == MSA_PRECOND_JACOBI || kind == MSA_PRECOND_MULTIGRID ||
+                  kind == MSA_PRECOND_FIRST_SAMPLE,
+              MSA_ERR_ARG, "preconditioner kind=%d", kind);
     MSA_CHECK(ratio == 0.0 || ratio > 1.0, MSA_ERR_ARG, "smoothing ratio %g (0 or > 1)", ratio);
     MSA_HIP(hipSetDevice(b->ctx->device));
     if (kind == MSA_PRECOND_MULTIGRID) {
@@ -2042,15 +2278,16 @@ int msa_batch_set_preconditioner(msa_batch *b, int kind, int max_levels, double 
     if (!b->refine_set) {
         // a V-cycle iteration cuts the residual about 5x: check for a
         // replacement every 2 iterations, replace after a 1e-3 drop
-        b->period = kind == MSA_PRECOND_MULTIGRID ? 2 : 8;
-        b->drop = kind == MSA_PRECOND_MULTIGRID ? 1e-3 : 0.1;
+        // (the same for the first-sample factor: a strong preconditioner)
+        b->period = kind != MSA_PRECOND_JACOBI ? 2 : 8;
+        b->drop = kind != MSA_PRECOND_JACOBI ? 1e-3 : 0.1;
     }
     if (b->graph) {  // the captured iteration block holds the preconditioner
         (void)hipGraphExecDestroy(b->graph);
         b->graph = nullptr;
     }
     b->mg_ready = false;
-    if (b->assembled) MSA_TRY(mg_values(b));
+    if (b->assembled) MSA_TRY(precond_values(b));
     return MSA_OK;
 }
 
@@ -2341,6 +2578,20 @@ int msa_batch_set_refinement(msa_batch *b, int method, int period, double drop)
     b->period = period;
     b->drop = drop;
     b->refine_set = true;
+    if (b->graph) {  // the captured iteration block holds the period and the trigger
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    return MSA_OK;
+}
+
+int msa_batch_graph_info(msa_batch *b, long long *info)
+{
+    MSA_CHECK(b && info, MSA_ERR_ARG, "bad arguments");
+    info[0] = b->graph_captures;
+    info[1] = b->graph ? b->graph_kernels : 0;
+    info[2] = b->graph ? b->graph_iters : 0;
+    info[3] = b->graph_replays;
     return MSA_OK;
 }
 
@@ -2422,11 +2673,11 @@ int msa_autotune_tile(msa_ctx *ctx, msa_mesh *mesh, int n_samples, int precision
         if (rc == MSA_OK) {
             // warm-up, then 10 timed products (pbuf 0 keeps every sample active)
             const int reps = 10;
-            hipError_t e = launch_pcg_a(s, precision == MSA_MIXED, b->T, b->d_cls, b->d_items, ni, 0,
+            hipError_t e = launch_pcg_a(s, precision == MSA_MIXED, b->T, 3, b->d_cls, b->d_items, ni, 0,
                                         b->d_counts);
             if (e == hipSuccess) e = hipEventRecord(b->timing.e0, s);
             for (int r = 0; r < reps && e == hipSuccess; r++)
-                e = launch_pcg_a(s, precision == MSA_MIXED, b->T, b->d_cls, b->d_items, ni, 0,
+                e = launch_pcg_a(s, precision == MSA_MIXED, b->T, 3, b->d_cls, b->d_items, ni, 0,
                                  b->d_counts);
             if (e == hipSuccess) e = hipEventRecord(b->timing.e1, s);
             if (e == hipSuccess) e = hipEventSynchronize(b->timing.e1);
@@ -2445,6 +2696,106 @@ int msa_autotune_tile(msa_ctx *ctx, msa_mesh *mesh, int n_samples, int precision
     }
     MSA_CHECK(*best_tile > 0, MSA_ERR_STATE, "autotune: no tile could be timed (%s)",
               g_last_error.c_str());
+    return MSA_OK;
+}
+
+int msa_batch_set_schedule(msa_batch *b, int unroll)
+{
+    MSA_CHECK(b, MSA_ERR_ARG, "null batch");
+    MSA_CHECK(unroll >= 1 && unroll <= 4, MSA_ERR_ARG, "unroll=%d (1..4)", unroll);
+    MSA_CHECK(b->T == 1 || unroll == 3, MSA_ERR_ARG,
+              "lane tiles T > 1 use groups of 3 slots (unroll=%d, T=%d)", unroll, b->T);
+    if (b->unroll != unroll && b->graph) {  // the captured iteration block holds the kernel
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    b->unroll = unroll;
+    return MSA_OK;
+}
+
+// Candidate schedules of msa_autotune_schedule: (lane tile T, slot unroll U).
+static const int kTuneCand[MSA_TUNE_CANDIDATES][2] = {{1, 1}, {1, 2}, {1, 3}, {1, 4},
+                                                      {2, 3}, {4, 3}, {16, 3}, {64, 3}};
+
+int msa_autotune_schedule(msa_ctx *ctx, int n_classes, msa_mesh *const *meshes,
+                          int samples_per_class, int precision, int precond, int *best_tile,
+                          int *best_unroll, double *ms_per_sample)
+{
+    MSA_CHECK(ctx && meshes && best_tile && best_unroll && n_classes >= 1 && n_classes <= 32 &&
+                  samples_per_class >= 1,
+              MSA_ERR_ARG, "msa_autotune_schedule: bad arguments");
+    MSA_CHECK(precond == MSA_PRECOND_JACOBI || precond == MSA_PRECOND_MULTIGRID ||
+                  precond == MSA_PRECOND_FIRST_SAMPLE,
+              MSA_ERR_ARG, "preconditioner kind=%d", precond);
+    *best_tile = *best_unroll = 0;
+    std::vector<int> ns(n_classes, samples_per_class);
+    // a fixed lognormal modulus field (COV 0.3) per sample and a unit load on
+    // every free DOF: samples converge in different iteration counts, as in
+    // a Monte Carlo batch
+    std::vector<std::vector<double>> E(n_classes), f(n_classes);
+    {
+        std::mt19937_64 rng(20261202);
+        std::normal_distribution<double> nd(0.0, 1.0);
+        const double sg = sqrt(log(1.0 + 0.09));
+        for (int c = 0; c < n_classes; c++) {
+            const MeshHost &h = meshes[c]->h;
+            E[c].resize((size_t)samples_per_class * h.n_elem);
+            for (double &e : E[c]) e = 2.1e5 * exp(-0.5 * sg * sg + sg * nd(rng));
+            f[c].assign(h.n_free, 1.0);
+        }
+    }
+    const double tol = precision == MSA_MIXED ? 1e-9 : 1e-10;
+    double best = 1e300;
+    for (int k = 0; k < MSA_TUNE_CANDIDATES; k++) {
+        if (ms_per_sample) ms_per_sample[k] = -1;
+        const int T = kTuneCand[k][0], U = kTuneCand[k][1];
+        msa_batch *b = nullptr;
+        int rc = msa_batch_create(ctx, n_classes, meshes, ns.data(), precision, T, &b);
+        if (rc != MSA_OK) continue;
+        if (precond != MSA_PRECOND_JACOBI) rc = msa_batch_set_preconditioner(b, precond, 0, 0.0);
+        if (rc == MSA_OK) rc = msa_batch_set_schedule(b, U);
+        for (int c = 0; c < n_classes && rc == MSA_OK; c++) {
+            rc = msa_batch_set_modulus(b, c, E[c].data());
+            if (rc == MSA_OK) rc = msa_batch_set_rhs(b, c, f[c].data());
+        }
+        // warm-up (graph capture), then one timed assembly + solve
+        double t = 0;
+        for (int rep = 0; rep < 2 && rc == MSA_OK; rep++) {
+            const auto t0 = std::chrono::steady_clock::now();
+            rc = msa_batch_assemble(b, 0.3);
+            if (rc == MSA_OK) rc = msa_batch_solve(b, tol, 20000, 1e-4, 30, 16);
+            if (rc == MSA_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = MSA_ERR_HIP;
+            t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+        bool ok = rc == MSA_OK;
+        for (int c = 0; c < n_classes && ok; c++) {
+            std::vector<int> st(samples_per_class);
+            ok = msa_batch_get_stats(b, c, nullptr, nullptr, nullptr, st.data()) == MSA_OK;
+            for (int v : st) ok = ok && v == MSA_OK;
+        }
+        msa_batch_destroy(b);
+        if (!ok) continue;
+        const double per = 1e3 * t / ((double)n_classes * samples_per_class);
+        if (ms_per_sample) ms_per_sample[k] = per;
+        if (per < best) {
+            best = per;
+            *best_tile = T;
+            *best_unroll = U;
+        }
+    }
+    MSA_CHECK(*best_tile > 0, MSA_ERR_STATE, "autotune: no schedule could be timed (%s)",
+              g_last_error.c_str());
+    set_error("");
+    return MSA_OK;
+}
+
+int msa_autotune_candidates(int *tiles, int *unrolls)
+{
+    MSA_CHECK(tiles && unrolls, MSA_ERR_ARG, "bad arguments");
+    for (int k = 0; k < MSA_TUNE_CANDIDATES; k++) {
+        tiles[k] = kTuneCand[k][0];
+        unrolls[k] = kTuneCand[k][1];
+    }
     return MSA_OK;
 }
 

@@ -977,7 +977,9 @@ __global__ __launch_bounds__(256) void k_pcg_init(const ClassDev *__restrict__ c
 // .cs:18-41, MCPCGDirect.cs:48-51): p = z + beta p_old on the fly for every
 // gathered column, q = K p, own rows of p written to the other buffer,
 // pq = p.q per sample; the last workgroup of each tile sets alpha = rz/pq.
-template <typename real, int T>
+// U: slots per group of a row (the loads of a group issue together; the
+// schedule the autotuner picks); every U sums the slots in the same order.
+template <typename real, int T, int U = 3>
 __global__ __launch_bounds__(256) void k_pcg_a(const ClassDev *__restrict__ cls,
                                                const int4 *__restrict__ items, int nitems,
                                                int pbuf, int *active_count)
@@ -1038,24 +1040,24 @@ __global__ __launch_bounds__(256) void k_pcg_a(const ClassDev *__restrict__ cls,
                 // (indices past the row's last slot re-read that slot: no
                 // branch, so nothing waits on them early)
                 const int last = max(W - 1, 0);
-                int nxt[3];
+                int nxt[U];
 #pragma unroll
-                for (int u = 0; u < 3; u++) nxt[u] = max(cidx[(size_t)(base + min(u, last)) * R + rr], 0);
-                for (; j + 3 <= W; j += 3) {
-                    int col[3];
+                for (int u = 0; u < U; u++) nxt[u] = max(cidx[(size_t)(base + min(u, last)) * R + rr], 0);
+                for (; j + U <= W; j += U) {
+                    int col[U];
 #pragma unroll
-                    for (int u = 0; u < 3; u++) col[u] = nxt[u];
-                    real v[3][3], m[3][9];
+                    for (int u = 0; u < U; u++) col[u] = nxt[u];
+                    real v[U][3], m[U][9];
 #pragma unroll
-                    for (int u = 0; u < 3; u++) {
+                    for (int u = 0; u < U; u++) {
                         gather(col[u], v[u][0], v[u][1], v[u][2]);
                         block(base + j + u, m[u]);
                     }
 #pragma unroll
-                    for (int u = 0; u < 3; u++)
-                        nxt[u] = max(cidx[(size_t)(base + min(j + 3 + u, last)) * R + rr], 0);
+                    for (int u = 0; u < U; u++)
+                        nxt[u] = max(cidx[(size_t)(base + min(j + U + u, last)) * R + rr], 0);
 #pragma unroll
-                    for (int u = 0; u < 3; u++) {
+                    for (int u = 0; u < U; u++) {
                         a0 += m[u][0] * v[u][0] + m[u][1] * v[u][1] + m[u][2] * v[u][2];
                         a1 += m[u][3] * v[u][0] + m[u][4] * v[u][1] + m[u][5] * v[u][2];
                         a2 += m[u][6] * v[u][0] + m[u][7] * v[u][1] + m[u][8] * v[u][2];
@@ -1974,8 +1976,9 @@ __global__ __launch_bounds__(256) void k_band_scatter(const ClassDev *__restrict
     const int rr = lane / T, s = lane % T;
     const int sample = tile * T + s;
     if (sample >= C.nsamp) return;
+    if (C.band_ss == 0 && sample != 0) return;  // one factor shared by all samples: sample 0's
     const real *val = (const real *)C.val + (size_t)tile * C.tile_vals;
-    double *AB = C.band + (size_t)sample * C.bn * C.bW;
+    double *AB = C.band + (size_t)sample * C.band_ss;
     const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
     for (int slice = s0 + wave; slice < s1; slice += kWaves) {
         const int br = slice * R + rr;
@@ -2009,7 +2012,7 @@ __global__ __launch_bounds__(64) void k_chol_diag(const ClassDev *__restrict__ c
     const ClassDev &C = *cp;
     const int smp = blockIdx.y;
     const int t = threadIdx.x;
-    double *AB = C.band + (size_t)smp * C.bn * C.bW;
+    double *AB = C.band + (size_t)smp * C.band_ss;
     if (t == 0) fail = 0;
     if (t < nk)
         for (int c = 0; c <= t; c++) A[t][c] = AB[(size_t)(k0 + c) * C.bW + (t - c)];
@@ -2047,7 +2050,7 @@ __global__ __launch_bounds__(256) void k_chol_panel(const ClassDev *__restrict__
     const ClassDev &C = *cp;
     const int smp = blockIdx.y;
     const int tid = threadIdx.x;
-    double *AB = C.band + (size_t)smp * C.bn * C.bW;
+    double *AB = C.band + (size_t)smp * C.band_ss;
     for (int idx = tid; idx < kBandNB * kBandNB; idx += 256) {
         const int c = idx >> 6, c2 = idx & 63;
         if (c2 <= c)
@@ -2088,7 +2091,7 @@ __global__ __launch_bounds__(256) void k_chol_update(const ClassDev *__restrict_
     __shared__ double Li[kBandNB][33], Lj[kBandNB][33];
     const ClassDev &C = *cp;
     const int smp = blockIdx.y;
-    double *AB = C.band + (size_t)smp * C.bn * C.bW;
+    double *AB = C.band + (size_t)smp * C.band_ss;
     // tile index -> (I, J) with J <= I
     const int tix = blockIdx.x;
     int I = (int)((sqrt(8.0 * tix + 1.0) - 1.0) * 0.5);
@@ -2155,7 +2158,7 @@ __global__ __launch_bounds__(64) void k_band_fwd(const ClassDev *__restrict__ cp
     const int smp = blockIdx.y;
     if (mode >= 0 && !mg_act(C.mg_st[smp], mode)) return;  // coarsest multigrid level
     const int t = threadIdx.x;
-    const double *AB = C.band + (size_t)smp * C.bn * C.bW;
+    const double *AB = C.band + (size_t)smp * C.band_ss;
     double *y = C.by + (size_t)smp * C.bn;
     if (phase == 0) {
         if (t < nk) yb[t] = y[k0 + t];
@@ -2189,7 +2192,7 @@ __global__ __launch_bounds__(64) void k_band_bwd(const ClassDev *__restrict__ cp
     const int smp = blockIdx.y;
     if (mode >= 0 && !mg_act(C.mg_st[smp], mode)) return;  // coarsest multigrid level
     const int t = threadIdx.x;
-    const double *AB = C.band + (size_t)smp * C.bn * C.bW;
+    const double *AB = C.band + (size_t)smp * C.band_ss;
     double *x = C.by + (size_t)smp * C.bn;
     const int nchunk = (C.bW + kBandChunkRows - 1) / kBandChunkRows;
     double *part = C.bpart + (size_t)smp * nchunk * kBandNB;
@@ -2243,7 +2246,7 @@ __global__ __launch_bounds__(256) void k_band_solve_one(const ClassDev *__restri
     if (mode >= 0 && !mg_act(C.mg_st[smp], mode)) return;  // coarsest multigrid level
     const int n = C.bn, W = C.bW;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const double *AB = C.band + (size_t)smp * n * W;
+    const double *AB = C.band + (size_t)smp * C.band_ss;
     double *y = C.by + (size_t)smp * n;
     for (int i = t; i < n; i += 256) yl[i] = y[i];
     // Ld[c][j] = AB[(k0 + c) W + j] = L[k0 + c + j][k0 + c], j < nk - c
@@ -3633,7 +3636,7 @@ __global__ __launch_bounds__(64) void k_band_inverse(const ClassDev *__restrict_
     const ClassDev &C = *cp;
     const int smp = blockIdx.y, j = blockIdx.x, lane = threadIdx.x;
     const int n = C.bn, W = C.bW;
-    const MSA_GLOBAL double *L = gbl(C.band + (size_t)smp * n * W);
+    const MSA_GLOBAL double *L = gbl(C.band + (size_t)smp * C.band_ss);
     for (int i = lane; i < n; i += 64) y[i] = 0.0;
     __syncthreads();
     // forward: y_i = (delta_ij - sum_{k in [max(j, i-W+1), i)} L_ik y_k) / L_ii
@@ -3719,6 +3722,49 @@ __global__ void k_mg_band_out(const ClassDev *__restrict__ cp, int mode)
     const int i = C.binv[row];
     ((real *)C.z)[(size_t)tile * C.tile_vec + (size_t)row * T + s] =
         i >= 0 ? (real)C.by[(size_t)smp * C.bn + i] : (real)0;
+}
+
+// First-sample direct preconditioner (the Monte Carlo calculator's
+// LDL^T of the first sample's K, applied to every sample): z = the band
+// solution of the solved class's rows (constrained components 0), and the
+// r.z partials of the PCG (slot 0, as the multigrid cycle's last step
+// writes them) for the samples the mode selects.
+template <typename real, int T>
+__global__ __launch_bounds__(256) void k_first_out(const ClassDev *__restrict__ cls,
+                                                   const int4 *__restrict__ items, int nitems,
+                                                   int mode)
+{
+    constexpr int R = 64 / T;
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const int tile = it.y, chunk = it.z;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rr = lane / T, s = lane % T;
+    const int sample = tile * T + s;
+    const bool act = sample < C.nsamp && mg_act(C.st[sample], mode);
+    const size_t vb = (size_t)tile * C.tile_vec;
+    const real *r = (const real *)C.r + vb;
+    real *z = (real *)C.z + vb;
+    const double *y = C.by + (size_t)(act ? sample : 0) * C.bn;
+    double rz = 0.0;
+    const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+    for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+        const int br = slice * R + rr;
+        if (!act || br >= C.nb) continue;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const int row = 3 * br + d;
+            const int i = C.binv[row];
+            const size_t o = (size_t)row * T + s;
+            const real zv = i >= 0 ? (real)y[i] : (real)0;
+            z[o] = zv;
+            rz += (double)(r[o] * zv);
+        }
+    }
+    double acc[1] = {rz};
+    block_partials<T, 1>(acc, C.partial + ((size_t)tile * C.nchunks + chunk) * 3 * T);
 }
 
 // ------------------------------------------------------------ utilities
@@ -3902,9 +3948,30 @@ hipError_t launch_pcg_init(hipStream_t s, bool mixed, int T, const ClassDev *d_c
     MSA_PCG_LAUNCH(k_pcg_init, d_cls, items, nitems, mode, active_count);
 }
 
-hipError_t launch_pcg_a(hipStream_t s, bool mixed, int T, const ClassDev *d_cls, const int4 *items,
-                        int nitems, int pbuf, int *active_count)
+template <typename real>
+static hipError_t pcg_a_unrolled(hipStream_t s, int U, dim3 grid, const ClassDev *d_cls,
+                                 const int4 *items, int nitems, int pbuf, int *active_count)
 {
+    switch (U) {
+    case 1: hipLaunchKernelGGL((k_pcg_a<real, 1, 1>), grid, dim3(256), 0, s, d_cls, items, nitems, pbuf, active_count); break;
+    case 2: hipLaunchKernelGGL((k_pcg_a<real, 1, 2>), grid, dim3(256), 0, s, d_cls, items, nitems, pbuf, active_count); break;
+    case 3: hipLaunchKernelGGL((k_pcg_a<real, 1, 3>), grid, dim3(256), 0, s, d_cls, items, nitems, pbuf, active_count); break;
+    case 4: hipLaunchKernelGGL((k_pcg_a<real, 1, 4>), grid, dim3(256), 0, s, d_cls, items, nitems, pbuf, active_count); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_pcg_a(hipStream_t s, bool mixed, int T, int U, const ClassDev *d_cls,
+                        const int4 *items, int nitems, int pbuf, int *active_count)
+{
+    if (T == 1 && U != 3) {
+        if (nitems <= 0) return hipSuccess;
+        const dim3 grid(xcd_grid(nitems));
+        return mixed ? pcg_a_unrolled<float>(s, U, grid, d_cls, items, nitems, pbuf, active_count)
+                     : pcg_a_unrolled<double>(s, U, grid, d_cls, items, nitems, pbuf, active_count);
+    }
+    if (U != 3) return hipErrorInvalidValue;  // lane tiles T > 1 use groups of 3 slots
     MSA_PCG_LAUNCH(k_pcg_a, d_cls, items, nitems, pbuf, active_count);
 }
 
@@ -4333,6 +4400,12 @@ hipError_t launch_mg_load(hipStream_t s, bool mixed, int T, const ClassDev *cp, 
                                              tile, lane));
     }
     return hipGetLastError();
+}
+
+hipError_t launch_first_out(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
+                           const int4 *items, int nitems, int mode)
+{
+    MSA_PCG_LAUNCH(k_first_out, d_cls, items, nitems, mode);
 }
 
 hipError_t launch_mg_band_out(hipStream_t s, bool mixed, int T, const ClassDev *cp, int nsamp,

@@ -180,6 +180,8 @@ struct ClassDev {
     const int *bperm;     // [bn] band index -> internal row
     const int *binv;      // [nb*3] internal row -> band index or -1
     double *band;         // [nsamp][bn*bW]
+    long long band_ss;    // doubles between two samples' factors: bn*bW, or 0 when every
+                          // sample uses the one factor at band (first-sample preconditioner)
     double *by;           // [nsamp][bn] right-hand side / solution in band order
     double *bpart;        // [nsamp][kBandChunks][64] partial sums of the backward solve
     int *bfail;           // [nsamp] non-positive pivot met
@@ -313,8 +315,9 @@ hipError_t launch_dinv(hipStream_t s, bool mixed, int T, const ClassDev *d_cls, 
                        int nitems);
 hipError_t launch_pcg_init(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
                            const int4 *items, int nitems, int mode, int *active_count);
-hipError_t launch_pcg_a(hipStream_t s, bool mixed, int T, const ClassDev *d_cls, const int4 *items,
-                        int nitems, int pbuf, int *active_count);
+// U: slots per group of a row (1..4 at T = 1, 3 for T > 1)
+hipError_t launch_pcg_a(hipStream_t s, bool mixed, int T, int U, const ClassDev *d_cls,
+                        const int4 *items, int nitems, int pbuf, int *active_count);
 hipError_t launch_pcg_b(hipStream_t s, bool mixed, int T, const ClassDev *d_cls, const int4 *items,
                         int nitems, int pbuf, double tol, int max_it, int *active_count);
 hipError_t launch_ir_begin(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,
@@ -370,6 +373,9 @@ hipError_t launch_mg_gersh(hipStream_t s, bool mixed, int T, const ClassDev *d_c
                            const int4 *items, int nitems);
 hipError_t launch_mg_band_in(hipStream_t s, bool mixed, int T, const ClassDev *cp, int nsamp,
                              int bn, int mode);
+// first-sample direct preconditioner: z from the band solution + r.z partials
+hipError_t launch_first_out(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
+                           const int4 *items, int nitems, int mode);
 hipError_t launch_mg_band_out(hipStream_t s, bool mixed, int T, const ClassDev *cp, int nsamp,
                               int nrows, int mode);
 hipError_t launch_mf_khat(hipStream_t s, const ClassDev *cp);

@@ -1,0 +1,76 @@
+"""The schedule autotuner (msa_autotune_schedule) on the GPU: it times every
+candidate (lane tile, slot unroll), returns the fastest, and a batch built
+with that schedule solves to the oracle's skyline direct solution; every
+slot unroll of the one-sample-tile product gives the same solution (the
+slots are summed in the same order)."""
+import numpy as np
+import pytest
+
+import msolve_amd as msa
+from msolve_amd import _lib as L
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+
+
+def direct(model, E):
+    ke, bad = O.ke_mesh(model.conn, model.xyz, E, 0.3)
+    assert bad == 0
+    ri, data = O.skyline_assemble(model.conn, model.dof, model.n_free, ke)
+    fac, rc = O.skyline_factorize(ri, data, 1e-15)
+    assert rc == 0
+    return O.skyline_solve(ri, fac, model.f)
+
+
+def lognormal(rng, shape, cov=0.3):
+    s = np.sqrt(np.log(1 + cov * cov))
+    return 2.1e5 * np.exp(-0.5 * s * s + s * rng.standard_normal(shape))
+
+
+@pytest.mark.parametrize("precond", [L.MSA_PRECOND_MULTIGRID, L.MSA_PRECOND_JACOBI])
+def test_autotuned_schedule_solves_to_oracle(ctx, precond):
+    print(f"\n[case] autotune precond={precond}", flush=True)
+    models = [msa.cantilever(8), msa.cantilever(12)]
+    meshes = [msa.Mesh(ctx, m.xyz, m.conn, m.dof, m.n_free) for m in models]
+    r = msa.autotune_schedule(ctx, meshes, 16, L.MSA_MIXED, precond)
+    print(f"[autotune] chosen tile {r['tile']} unroll {r['unroll']}; ms per sample {r['table']}",
+          flush=True)
+    timed = {k: v for k, v in r["table"].items() if v > 0}
+    assert (r["tile"], r["unroll"]) in timed
+    assert timed[(r["tile"], r["unroll"])] == min(timed.values())
+    assert all(r["table"][k] > 0 for k in r["table"] if k[0] <= 16)  # 16 samples fill these tiles
+    rng = np.random.default_rng(8)
+    E = [lognormal(rng, (3, m.n_elem)) for m in models]
+    sols = {}
+    for tile, unroll in [(r["tile"], r["unroll"])] + [(1, u) for u in (1, 2, 3, 4)]:
+        b = msa.Batch(ctx, meshes, [3, 3], L.MSA_MIXED, tile)
+        if precond != L.MSA_PRECOND_JACOBI:
+            b.set_preconditioner(precond)
+        b.set_schedule(unroll)
+        for c, m in enumerate(models):
+            b.set_modulus(c, E[c])
+            b.set_rhs(c, m.f)
+        b.assemble(0.3)
+        b.solve(tol=1e-10, max_it=5000, graph_iters=8)
+        sols[(tile, unroll)] = [[b.solution(c, s) for s in range(3)] for c in range(2)]
+        for c in range(2):
+            assert (b.stats(c)["status"] == msa.MSA_OK).all()
+        b.close()
+    for c, m in enumerate(models):
+        for s in range(3):
+            ud = direct(m, E[c][s])
+            assert rel(sols[(r["tile"], r["unroll"])][c][s], ud) < 1e-8
+            for u in (1, 2, 4):
+                assert rel(sols[(1, u)][c][s], sols[(1, 3)][c][s]) < 1e-13
+    with pytest.raises(msa.MsaError):
+        bb = msa.Batch(ctx, meshes, [3, 3], L.MSA_MIXED, 4)
+        try:
+            bb.set_schedule(2)  # lane tiles > 1 only group 3 slots
+        finally:
+            bb.close()
+    for m in meshes:
+        m.close()

@@ -122,7 +122,54 @@ def test_newmark_warm_start_saves_iterations(ctx):
             for mode in (1, 2):
                 assert rel(res[prec, mode][s][0], hc) < 1e-8
             iw = res[prec, 1][s][1]
-            assert iw < ic, (prec, iw, ic)
+            print(f"[warm start] precision {prec} sample {s}: {iw} / {ic} iterations "
+                  f"({100 * (1 - iw / ic):.1f}% saved)", flush=True)
+            # the response is quasi-static (a0 M << K), so the previous
+            # displacement is nearly parallel to the new one: a warm start
+            # removes only the amplitude change of the step from the initial
+            # residual (about 1/i at step i of a load growing like i), a few
+            # decades of the ten the tolerance asks for
+            assert iw <= 0.97 * ic, (prec, iw, ic)
+
+
+@pytest.mark.parametrize("precision,solver", [(L.MSA_FP64, L.MSA_SOLVER_PCG),
+                                              (L.MSA_MIXED, L.MSA_SOLVER_PCG),
+                                              (L.MSA_FP64, L.MSA_SOLVER_DIRECT)])
+def test_newmark_step_graph_equals_eager(ctx, precision, solver):
+    """Captured time steps (one graph per step: the previous update, the
+    step's right-hand side and solve start, the first iteration block)
+    give the same histories as eager launches, bit for bit, with the
+    multigrid preconditioner; one capture set per call."""
+    print(f"\n[case] newmark graph vs eager precision={precision} solver={solver}", flush=True)
+    model = msa.cantilever(8, 8, 4, lz=0.5)
+    mesh = msa.Mesh(ctx, model.xyz, model.conn, model.dof, model.n_free)
+    rng = np.random.default_rng(9)
+    E = np.stack([lognormal(rng, model.n_elem) for _ in range(3)])
+    acc = np.array([O.sinusoidal_acc(i, 1.0, 0.05, 1e-4, 0.05) for i in range(25)])
+    out = []
+    for g in (0, 8):
+        b = msa.Batch(ctx, [mesh], [3], precision, 1)
+        if solver == L.MSA_SOLVER_DIRECT:
+            b.set_solver(solver)
+        else:
+            b.set_preconditioner(L.MSA_PRECOND_MULTIGRID)
+        b.set_modulus(0, E)
+        b.set_rhs(0, model.f)
+        b.set_dynamics(7.85e-9, 0, 0, 0.25, 0.5, 1e-4)
+        b.set_probe(0, model.n_free - 1)
+        b.assemble(0.3)
+        b.newmark(acc, acc_dir=2, tol=1e-10, max_it=20000, graph_iters=g)
+        gi = b.graph_info()
+        out.append([(b.history(0, s), b.dynamic_state(0, s)) for s in range(3)])
+        assert (b.stats(0)["status"] == msa.MSA_OK).all()
+        if g:
+            assert gi["captures"] >= 3 and gi["replays"] >= 25, gi
+        b.close()
+    mesh.close()
+    for s in range(3):
+        assert np.array_equal(out[0][s][0], out[1][s][0])
+        for k in range(3):
+            assert np.array_equal(out[0][s][1][k], out[1][s][1][k])
 
 
 def test_newmark_argument_checks(ctx):
