@@ -5,27 +5,34 @@ Workload (one "step"): the batched mixed-precision Monte Carlo job of
 1024 samples whose cube-mesh sizes are drawn uniformly from
 n in {16, 24, 32, 40, 48} elements per side; per sample a lognormal
 Karhunen-Loeve modulus field (40 terms, exponential covariance, correlation
-length 0.25, mean 2.1e5, COV 0.3) evaluated at element centroids, FP32
-element integration + coloured assembly, FP32 Jacobi-PCG inner solves under
-FP64 iterative refinement (FP64 residual f - K u).  Inputs (meshes, KL
-tables, KL coefficients, loads) are resident on the device before the timed
-region; each step re-evaluates the modulus, re-assembles and re-solves every
-sample.  The PCG preconditioner is a geometric multigrid V-cycle per sample
-(--precond mg, default; coarse levels re-assembled every step) or Jacobi
-(--precond jacobi, the reference's SolverPCG choice); the other one is timed
-beside it for comparison.
+length 0.25, mean 2.1e5, COV 0.3) evaluated at element centroids, FP64
+element integration rounded once to FP32 stored values, FP32 PCG under FP64
+residual replacement to |f - K u|/|f| < 1e-9.  Inputs (meshes, KL tables,
+KL coefficients, loads) are resident on the device before the timed region;
+each step re-evaluates the modulus, re-assembles and re-solves every sample.
+The PCG preconditioner is a geometric multigrid V-cycle per sample
+(--precond mg, default) or Jacobi (--precond jacobi, the reference's
+SolverPCG choice); the other one is timed beside it for comparison.
 
-Reported beside the headline: the same job in plain FP64 (batched), the
-single-sample FP64 GPU PCG, each sample's error against a tightly converged
-FP64 solve, the SpMV kernel's roofline (HIP events) and a CPU baseline (the
-oracle's CSR/Jacobi restatement of the reference loop, one sample per mesh
-size, one core).
+Reported beside the headline (all in "extra"):
+  - each class's first sample against the CPU oracle's own solve (CSR
+    Jacobi-PCG to 1e-12): the independent error of the headline;
+  - the same job in plain FP64 (batched) and its error against a tightly
+    converged FP64 batch of all 1024 samples;
+  - the single-sample FP64 GPU Jacobi-PCG per mesh size;
+  - 256 samples of 32^3 in FP64 (Jacobi-PCG to 1e-10);
+  - one 64^3 FP64 Jacobi-PCG solve to 1e-10;
+  - the batched Newmark dynamics of a 48x48x24 block (64 samples, 500 steps);
+  - the roofline of the kernels that take the most time, from HIP events on
+    the context stream;
+  - the CPU baseline: the oracle's restatement of the reference per-sample
+    loop, one sample per mesh size, pinned to one core, and the same on
+    every core of the host share.
 """
 from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -37,7 +44,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "MC solves/sec at 1e-8 rel. err, batched mixed vs plain FP64, 1 MI355X"
 HBM_PEAK_GBS = 8000.0
-FP32_PEAK_TFLOPS = 157.3
+FP32_PEAK_TFLOPS = 157.3      # MI355X published FP32 rate (packed / MFMA)
+FP32_VALU_NOPACK_TFLOPS = 78.65  # the same without packed FP32 (disabled in this build)
+HOST_CORE_SHARE = 16          # host cores one GPU gets on the benchmark box
 
 
 def log(*a):
@@ -56,11 +65,18 @@ def parse():
     p.add_argument("--corr", type=float, default=0.25)
     p.add_argument("--tol", type=float, default=1e-9, help="FP64 residual tolerance |f-Ku|/|f|")
     p.add_argument("--inner-tol", type=float, default=1e-4)
-    p.add_argument("--oracle-tol", type=float, default=1e-13)
+    p.add_argument("--oracle-tol", type=float, default=1e-12,
+                   help="tolerance of the CPU oracle solves the headline is checked against")
+    p.add_argument("--fp64-tol", type=float, default=1e-13,
+                   help="tolerance of the tightly converged FP64 GPU batch")
     p.add_argument("--max-it", type=int, default=20000)
     p.add_argument("--max-outer", type=int, default=30)
     p.add_argument("--graph-iters", type=int, default=16)
-    p.add_argument("--tile", type=int, default=1)
+    p.add_argument("--tile", type=int, default=0,
+                   help="lane tile of the batches (0: tile and slot unroll from the autotuner)")
+    p.add_argument("--unroll", type=int, default=3, help="slot unroll when --tile is given")
+    p.add_argument("--tune-samples", type=int, default=64,
+                   help="samples per class of the autotuner's trial batches")
     p.add_argument("--period", type=int, default=None,
                    help="replacement check period (iterations; default 2 with multigrid, 8 with Jacobi)")
     p.add_argument("--drop", type=float, default=None,
@@ -78,12 +94,19 @@ def parse():
     p.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison runs")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-threads", type=int, default=-1,
-                   help="threads of the multi-core CPU baseline (-1: host cores up to 16, "
-                        "0: skip it)")
+                   help="threads of the multi-core CPU baseline (-1: every core of the host "
+                        "share, 0: skip it)")
     p.add_argument("--no-single", action="store_true", help="skip the single-sample FP64 runs")
+    p.add_argument("--no-c2", action="store_true", help="skip the 64^3 FP64 single solve")
+    p.add_argument("--no-c3", action="store_true", help="skip the 256 x 32^3 FP64 batch")
+    p.add_argument("--no-c5", action="store_true", help="skip the batched Newmark dynamics")
+    p.add_argument("--only", choices=["c2", "c3", "c5"], default=None,
+                   help="run only this comparison workload (profiling)")
     p.add_argument("--newmark", type=int, default=0, metavar="STEPS",
                    help="instead of the Monte Carlo job, time STEPS Newmark steps of the "
                         "batched dynamics workload (48x48x24 block, 64 samples)")
+    p.add_argument("--newmark-steps", type=int, default=500,
+                   help="time steps of the dynamics workload in the default run")
     p.add_argument("--newmark-samples", type=int, default=64)
     p.add_argument("--newmark-tol", type=float, default=1e-8)
     p.add_argument("--newmark-cold", action="store_true",
@@ -109,7 +132,8 @@ def refinement(args, precond):
             args.drop if args.drop is not None else (1e-3 if mg else 0.1))
 
 
-def make_batch(msa, ctx, meshes, models, counts, precision, tile, table, xi, args, precond=None):
+def make_batch(msa, ctx, meshes, models, counts, precision, tile, table, xi, args, precond=None,
+               cov=None):
     b = msa.Batch(ctx, meshes, counts, precision, tile)
     precond = precond or args.precond
     if precond == "mg":
@@ -120,7 +144,7 @@ def make_batch(msa, ctx, meshes, models, counts, precision, tile, table, xi, arg
                          else msa.MSA_REFINE_RESTART, period, drop)
     off = 0
     for c, m in enumerate(models):
-        b.set_kl(c, table, m.domain, 2.1e5, args.cov)
+        b.set_kl(c, table, m.domain, 2.1e5, args.cov if cov is None else cov)
         b.set_xi(c, xi[off:off + counts[c]])
         b.set_rhs(c, m.f)
         off += counts[c]
@@ -128,28 +152,31 @@ def make_batch(msa, ctx, meshes, models, counts, precision, tile, table, xi, arg
     return b
 
 
-def smooth_bytes(b, models, rs):
-    """Algorithmic bytes of all finest-level smoothing launches of one solve.
-    Per cycle and active sample the four steps read the values four times
-    and move 16 vectors (PRE: z gathered, b, D^-1 read, A written; RESID: A,
-    b read, B written; POST1: A, b, D^-1 read, B written; POST2: B, b, D^-1,
-    A read, z written), plus the shared block-column indices and slice
-    pointers once per launch.  Cycles per sample = iterations + replacements
-    (one at the start, one after every replacement that does not finish)."""
-    total = 0.0
-    for c, m in enumerate(models):
-        if b.mg_info(c)["levels"] == 0:
-            continue
-        st = b.stats(c)
-        mesh = b.meshes[c]
-        nnz, n = mesh.info["nnz"], 3 * mesh.info["blocks"]
-        cyc = (st["iters"] + np.maximum(st["outer"], 1)).astype(np.float64)
-        total += cyc.sum() * 4 * (rs * nnz + 4 * rs * n)
-        idx = 4.0 * mesh.info["block_nnz"] + 4.0 * (mesh.info["blocks"] + 1)
-        total += float(cyc.max()) * 4 * idx
-    return total
+def host_info():
+    """Host of this run: core counts, CPU model and clocks (from /proc and
+    /sys, no external tools)."""
+    aff = sorted(os.sched_getaffinity(0))
+    model, mhz = None, []
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name") and model is None:
+                    model = line.split(":", 1)[1].strip()
+                elif line.startswith("cpu MHz"):
+                    mhz.append(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    fmax = None
+    try:
+        with open(f"/sys/devices/system/cpu/cpu{aff[0]}/cpufreq/cpuinfo_max_freq") as fh:
+            fmax = int(fh.read().strip()) / 1000.0
+    except (OSError, ValueError, IndexError):
+        pass
+    return dict(nproc=os.cpu_count(), affinity_cpus=len(aff), model=model,
+                mhz_mean=float(np.mean(mhz)) if mhz else None, mhz_max=fmax)
 
 
+# ------------------------------------------------------------------ work models
 def mf_work(b, models):
     """FP32 operations and algorithmic HBM bytes of all finest-level
     matrix-free smoothing launches of one solve (None unless every class
@@ -157,9 +184,8 @@ def mf_work(b, models):
     each does, for every (free node, incident cell) pair, 72 FMA (three
     rows of the unit element matrix times the cell's 24 values) and 3 FMA
     (modulus scaling): 150 flop.  Bytes per product and sample: the moduli
-    (8 per element) and 4 vectors of 4*3*nb on average over the four steps
-    (as for the stored-value products, without the values); the grid tables
-    once per launch."""
+    (8 per element) and 4 vectors of 4*3*nb on average over the four steps;
+    the grid tables once per launch."""
     flops = byts = 0.0
     for c, m in enumerate(models):
         info = b.mg_info(c)
@@ -179,15 +205,34 @@ def mf_work(b, models):
     return flops, byts
 
 
-def kernel_a_bytes(b, models, counts, rs, launches):
-    """Algorithmic bytes moved by all SpMV+dot launches of one solve:
-    per active sample and iteration the matrix values and the four vectors
-    (z, p_old read; p_new, q written); the shared block-column indices and
-    slice pointers once per launch per class with active samples."""
+def smooth_bytes(b, models, rs):
+    """Algorithmic bytes of all finest-level stored-value smoothing launches
+    of one solve: per cycle and active sample the four steps read the values
+    four times and move 16 vectors, plus the shared block-column indices and
+    slice pointers once per launch."""
     total = 0.0
     for c, m in enumerate(models):
+        if b.mg_info(c)["levels"] == 0:
+            continue
         st = b.stats(c)
         mesh = b.meshes[c]
+        nnz, n = mesh.info["nnz"], 3 * mesh.info["blocks"]
+        cyc = (st["iters"] + np.maximum(st["outer"], 1)).astype(np.float64)
+        total += cyc.sum() * 4 * (rs * nnz + 4 * rs * n)
+        idx = 4.0 * mesh.info["block_nnz"] + 4.0 * (mesh.info["blocks"] + 1)
+        total += float(cyc.max()) * 4 * idx
+    return total
+
+
+def kernel_a_bytes(b, rs):
+    """Algorithmic bytes moved by all SpMV+dot (k_pcg_a) launches of one
+    solve: per active sample and iteration the matrix values and the four
+    vectors (z, p_old read; p_new, q written); the shared block-column
+    indices and slice pointers once per launch of a class with active
+    samples."""
+    total = 0.0
+    for c, mesh in enumerate(b.meshes):
+        st = b.stats(c)
         nnz, n = mesh.info["nnz"], 3 * mesh.info["blocks"]
         its = st["iters"].astype(np.float64)
         total += its.sum() * (rs * nnz + 4 * rs * n)
@@ -196,31 +241,183 @@ def kernel_a_bytes(b, models, counts, rs, launches):
     return total
 
 
-def bench_newmark(args):
+def counters(name):
+    """Measured/algorithmic HBM bytes of a kernel from its rocprofv3
+    FETCH_SIZE/WRITE_SIZE passes (profiles/<name>), or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    cj = json.load(open(path))
+    return (cj["fetch_bytes"] + cj["write_bytes"]) / (cj["algorithmic_read"] +
+                                                     cj["algorithmic_write"])
+
+
+def spmv_roofline(b, kt, rs, name, counter_file):
+    """Roofline entry of k_pcg_a from one kernel-timing solve."""
+    launches = kt["spmv_dot"]["launches"]
+    a_s = kt["spmv_dot"]["ms"] / 1e3 / max(launches, 1)
+    a_bytes = kernel_a_bytes(b, rs) / max(launches, 1)
+    ratio = counters(counter_file)
+    ach = a_bytes / a_s / 1e9 if a_s > 0 else 0.0
+    return dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=ach / HBM_PEAK_GBS,
+                traffic=a_bytes * ratio if ratio is not None else None,
+                traffic_over_algorithmic=ratio, kernel=name, launches=launches,
+                avg_launch_ms=1e3 * a_s, avg_bytes_per_launch=a_bytes,
+                ms_total=kt["spmv_dot"]["ms"])
+
+
+def smoother_roofline(bm, models, kt, args):
+    """Roofline entry of the finest level's smoothing products."""
+    launches = kt["multigrid"]["launches"]
+    if launches == 0:
+        return None
+    avg_s = kt["multigrid"]["ms"] / 1e3 / launches
+    mf = mf_work(bm, models)
+    if mf is not None:
+        flop = mf[0] / launches
+        ach = flop / avg_s / 1e12 if avg_s > 0 else 0.0
+        ratio = counters("k_mg_smooth_mf_counters.json")
+        byts = mf[1] / launches
+        return dict(bound="valu", achieved=ach, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+                    frac=ach / FP32_PEAK_TFLOPS,
+                    frac_of_valu_without_packed_fp32=ach / FP32_VALU_NOPACK_TFLOPS,
+                    traffic=byts * ratio if ratio is not None else None,
+                    traffic_over_algorithmic=ratio,
+                    kernel="k_mg_smooth<float,1,STEP,2> on the finest level: matrix-free FP32 "
+                           "products on box grids (PRE, RESID, POST1, POST2), FP32 VALU",
+                    launches=launches, avg_launch_ms=1e3 * avg_s, avg_flop_per_launch=flop,
+                    avg_bytes_per_launch=byts, hbm_gbs=byts / avg_s / 1e9,
+                    ms_total=kt["multigrid"]["ms"])
+    byts = smooth_bytes(bm, models, 4) / launches
+    ratio = counters("k_mg_smooth_counters.json")
+    ach = byts / avg_s / 1e9
+    return dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=ach / HBM_PEAK_GBS, traffic=byts * ratio if ratio is not None else None,
+                traffic_over_algorithmic=ratio,
+                kernel=f"k_mg_smooth<float,{args.tile},STEP> on the finest level (stored values)",
+                launches=launches, avg_launch_ms=1e3 * avg_s, avg_bytes_per_launch=byts,
+                ms_total=kt["multigrid"]["ms"])
+
+
+# ------------------------------------------------------------------ oracle checks
+def oracle_true_residual(O, model, E, u, nu=0.3):
+    """|f - K u|/|f| with the oracle's own element matrices and CSR assembly
+    (independent of the GPU's operator)."""
+    ke, _ = O.ke_mesh(model.conn, model.xyz, E, nu)
+    rp, col, val = O.csr_assemble(model.conn, model.dof, model.n_free, ke)
+    r = model.f - O.csr_multiply(rp, col, val, u)
+    return float(np.linalg.norm(r) / np.linalg.norm(model.f))
+
+
+# ------------------------------------------------------------------ legs
+def leg_c3(msa, ctx, args):
+    """256 samples of the 32^3 cantilever, lognormal KL modulus (COV 0.2),
+    FP64 element integration, assembly and Jacobi-PCG to 1e-10, one batch."""
+    from msolve_amd.stochastic import kl_table, standard_normal
+    from oracle import oracle as O
+    ns, n = 256, 32
+    model = msa.cantilever(n)
+    mesh = msa.Mesh(ctx, model.xyz, model.conn, model.dof, model.n_free)
+    table = kl_table([1.0, 1.0, 1.0], args.corr, args.terms)
+    xi = standard_normal(args.seed + 3, ns, args.terms)
+    b = make_batch(msa, ctx, [mesh], [model], [ns], msa.MSA_FP64, 1, table, xi, args,
+                   precond="jacobi", cov=0.2)
+    kw = dict(nu=0.3, tol=1e-10, max_it=args.max_it, graph_iters=args.graph_iters)
+    b.run(**kw)
+    ctx.synchronize()
+    t = time.perf_counter()
+    b.run(**kw)
+    ctx.synchronize()
+    el = time.perf_counter() - t
+    st = b.stats(0)
+    b.set_kernel_timing(True)
+    b.run(**kw)
+    kt = b.kernel_timing()
+    b.set_kernel_timing(False)
+    roof = spmv_roofline(b, kt, 8, "k_pcg_a<double,1> (SpMV + p update + p.q), 256 x 32^3",
+                         "k_pcg_a_fp64_counters.json")
+    # independent check of two samples: the oracle's own CSR operator
+    otab = O.kl_table([1.0, 1.0, 1.0], args.corr, args.terms)
+    res = {}
+    for s in (0, ns - 1):
+        E, bad = O.kl_field(model.conn, model.xyz, otab, xi[s], model.domain, 2.1e5, 0.2)
+        res[str(s)] = oracle_true_residual(O, model, E, b.solution(0, s))
+    b.close()
+    mesh.close()
+    out = dict(workload="256 lognormal-KL samples (COV 0.2) of the 32^3 cantilever, FP64 "
+                        "Jacobi-PCG to |f-Ku|/|f| < 1e-10, one batch",
+               solves_per_s=ns / el, s_per_batch=el, iters_mean=float(st["iters"].mean()),
+               iters_max=int(st["iters"].max()),
+               all_ok=bool((st["status"] == msa.MSA_OK).all()),
+               oracle_true_residual=res, roofline=roof,
+               kernel_timing_ms={k: v["ms"] for k, v in kt.items()})
+    log(f"[bench] C3 256x32^3 FP64: {el:.2f} s -> {ns / el:.1f} solves/s, "
+        f"k_pcg_a<double> {roof['achieved']:.0f} GB/s, residuals {res}")
+    return out
+
+
+def leg_c2(msa, ctx, args):
+    """One FP64 Jacobi-PCG solve of the 64^3 cantilever (E = 2.1e5) to 1e-10."""
+    from oracle import oracle as O
+    n = 64
+    model = msa.cantilever(n)
+    mesh = msa.Mesh(ctx, model.xyz, model.conn, model.dof, model.n_free)
+    b = msa.Batch(ctx, [mesh], [1], msa.MSA_FP64, 1)
+    E = np.full(model.n_elem, 2.1e5)
+    b.set_modulus(0, E)
+    b.set_rhs(0, model.f)
+    kw = dict(nu=0.3, tol=1e-10, max_it=args.max_it, graph_iters=args.graph_iters)
+    b.run(**kw)
+    ctx.synchronize()
+    t = time.perf_counter()
+    b.run(**kw)
+    ctx.synchronize()
+    el = time.perf_counter() - t
+    st = b.stats(0)
+    b.set_kernel_timing(True)
+    b.run(**kw)
+    kt = b.kernel_timing()
+    b.set_kernel_timing(False)
+    roof = spmv_roofline(b, kt, 8, "k_pcg_a<double,1> (SpMV + p update + p.q), one 64^3 sample",
+                         "k_pcg_a_fp64_counters.json")
+    u = b.solution(0, 0)
+    res = oracle_true_residual(O, model, E, u)
+    b.close()
+    mesh.close()
+    out = dict(workload="64^3 cantilever (811200 DOFs), E 2.1e5, FP64 Jacobi-PCG to 1e-10",
+               s_per_solve=el, iters=int(st["iters"][0]), status=int(st["status"][0]),
+               ms_per_iteration=1e3 * el / max(int(st["iters"][0]), 1),
+               oracle_true_residual=res, roofline=roof,
+               kernel_timing_ms={k: v["ms"] for k, v in kt.items()})
+    log(f"[bench] C2 64^3 FP64: {el:.3f} s, {int(st['iters'][0])} its, k_pcg_a<double> "
+        f"{roof['achieved']:.0f} GB/s, oracle residual {res:.2e}")
+    return out
+
+
+def leg_newmark(msa, ctx, args, nsteps):
     """Batched Newmark dynamics: 48x48x24 Hexa8 block (1 x 1 x 0.5, clamped
     at x = 0), consistent mass, density 7.85e-9, Newmark beta 0.25 / gamma
     0.5, dt 1e-4, base acceleration 1.0 sin(2 pi 20 t) in Z, lognormal KL
-    modulus (COV 0.2), mixed-precision PCG warm-started every step.  Rate =
-    samples x steps / s."""
-    import msolve_amd as msa
+    modulus (COV 0.2), mixed-precision PCG warm-started from the previous
+    displacement every step.  Rate = samples x steps / s."""
     from msolve_amd.stochastic import kl_table, standard_normal
-    ns, nsteps = args.newmark_samples, args.newmark
-    ctx = msa.Context(0)
+    ns = args.newmark_samples
     model = msa.cantilever(48, 48, 24, lz=0.5)
     mesh = msa.Mesh(ctx, model.xyz, model.conn, model.dof, model.n_free)
     table = kl_table(list(model.domain), args.corr, args.terms)
     xi = standard_normal(args.seed, ns, args.terms)
     acc = msa.sinusoidal_history(0.05, 1e-4, 0.05, 1.0, nsteps)
 
-    def make(precision):
-        b = msa.Batch(ctx, [mesh], [ns], precision, 1)
+    def make(precision, nsamp):
+        b = msa.Batch(ctx, [mesh], [nsamp], precision, 1)
         if args.precond == "mg":
             b.set_preconditioner(msa.MSA_PRECOND_MULTIGRID)
         if precision == msa.MSA_MIXED:
             period, drop = refinement(args, args.precond)
             b.set_refinement(msa.MSA_REFINE_RELIABLE, period, drop)
         b.set_kl(0, table, model.domain, 2.1e5, 0.2)
-        b.set_xi(0, xi)
+        b.set_xi(0, xi[:nsamp])
         b.eval_kl()
         b.set_dynamics(7.85e-9, 0.0, 0.0, 0.25, 0.5, 1e-4)
         b.set_probe(0, model.n_free - 1)
@@ -228,7 +425,7 @@ def bench_newmark(args):
         ctx.synchronize()
         return b
 
-    b = make(msa.MSA_MIXED)
+    b = make(msa.MSA_MIXED, ns)
     b.newmark(acc[:min(2, nsteps)], 2, args.newmark_tol, args.max_it, args.graph_iters)
     ctx.synchronize()
     t = time.perf_counter()
@@ -249,35 +446,135 @@ def bench_newmark(args):
             [b.dynamic_state(0, s)[3] for s in range(ns)])) / nsteps
     b.close()
     # accuracy: the same steps in FP64, PCG to 1e-12, for the first 4 samples
-    b64 = make(msa.MSA_FP64)
+    nchk = min(4, ns)
+    b64 = make(msa.MSA_FP64, nchk)
     b64.newmark(acc, 2, 1e-12, args.max_it, args.graph_iters)
     errs = [float(np.linalg.norm(hist[s] - b64.history(0, s)) / np.linalg.norm(b64.history(0, s)))
-            for s in range(min(4, ns))]
+            for s in range(nchk)]
     b64.close()
+    mesh.close()
+    log(f"[bench] Newmark: {nsteps} steps x {ns} samples in {el:.2f} s -> "
+        f"{ns * nsteps / el:.1f} sample-steps/s, probe error vs FP64 {max(errs):.2e}")
+    return dict(
+        workload=f"batched Newmark: {ns} lognormal-KL samples (COV 0.2) of a 48x48x24 Hexa8 "
+                 f"block, rho 7.85e-9, beta 0.25, gamma 0.5, dt 1e-4, 20 Hz base acceleration, "
+                 f"{'multigrid' if args.precond == 'mg' else 'Jacobi'}-preconditioned mixed PCG "
+                 f"to {args.newmark_tol} warm-started from the previous displacement",
+        sample_steps_per_s=ns * nsteps / el, samples=ns, steps=nsteps,
+        ms_per_time_step=1e3 * el / nsteps, n_free=model.n_free,
+        all_ok=bool((st["status"] == msa.MSA_OK).all()),
+        iters_per_step_mean=float(np.mean(its)) / nsteps,
+        probe_history_rel_err_vs_fp64=max(errs), **extra)
+
+
+def bench_newmark(args):
+    import msolve_amd as msa
+    ctx = msa.Context(0)
+    r = leg_newmark(msa, ctx, args, args.newmark)
+    ctx.close()
     out = {
         "metric": "Newmark sample-steps/sec, batched mixed warm-started PCG, 1 MI355X",
-        "value": ns * nsteps / el, "unit": "sample-steps/s", "n_gpus": 1, "steps": nsteps,
-        "ms_per_step": 1e3 * el / nsteps, "higher_is_better": True, "dtype": "f32+f64",
-        "data": "synthetic",
-        "config": {"workload": f"batched Newmark: {ns} lognormal-KL samples (COV 0.2) of a "
-                               f"48x48x24 Hexa8 block, rho 7.85e-9, beta 0.25, gamma 0.5, "
-                               f"dt 1e-4, 20 Hz base acceleration, "
-                               f"{'multigrid' if args.precond == 'mg' else 'Jacobi'}-preconditioned "
-                               f"PCG to {args.newmark_tol}",
-                   "precond": args.precond,
-                   "n_free": model.n_free, "all_ok": bool((st["status"] == msa.MSA_OK).all()),
-                   "iters_per_step_mean": float(np.mean(its)) / nsteps,
-                   "probe_history_rel_err_vs_fp64": max(errs)},
-        "extra": extra,
+        "value": r["sample_steps_per_s"], "unit": "sample-steps/s", "n_gpus": 1,
+        "steps": args.newmark, "ms_per_step": r["ms_per_time_step"], "higher_is_better": True,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": r["workload"], "precond": args.precond},
+        "extra": r,
     }
-    mesh.close()
-    ctx.close()
     print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(args, sizes, counts, xi, u_mixed):
+    """The oracle's CSR/Jacobi restatement of the reference per-sample loop
+    (KL modulus, Ke, assembly, PCG to the headline's tolerance), one sample
+    per mesh size: pinned to one core, then on every core of the host
+    share.  Also the oracle's own tightly converged solve of each class's
+    first sample, against which the headline's displacements are compared."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import oracle as O
+    otab = O.kl_table([1.0, 1.0, 1.0], args.corr, args.terms)
+    host = host_info()
+    aff = sorted(os.sched_getaffinity(0))
+    core = aff[0]
+    per, u_base, t_meas = [], [], 0.0
+    off = 0
+    os.sched_setaffinity(0, {core})
+    try:
+        for c, n in enumerate(sizes):
+            out = O.mc_reference(n, otab, xi[off:off + 1], cov=args.cov, tol=args.tol,
+                                 keep_u=True, storage="csr")
+            per.append(out["times"]["all"])
+            u_base.append(out["u"][0])
+            t_meas += out["times"]["all"]
+            off += counts[c]
+            log(f"[bench] cpu {n}^3 (core {core}): {out['times']['all']:.2f} s, "
+                f"{int(out['iters'][0])} its")
+    finally:
+        os.sched_setaffinity(0, set(aff))
+    N = sum(counts)
+    t_cpu = sum(counts[c] * per[c] for c in range(len(sizes)))
+    cpu = dict(value=N / t_cpu, unit="solves/s", cores=1, kind="port",
+               sample=f"one sample per mesh size {sizes} (first of each class), oracle "
+                      f"CSR/Jacobi-PCG FP64 restatement of the reference per-sample loop "
+                      f"(KL modulus, Ke, assembly, PCG to {args.tol}), serial, pinned to host "
+                      f"core {core}; rate = {N} / sum(class count x measured time), "
+                      f"{t_meas:.1f} s measured",
+               host=host, per_size_s=dict(zip(map(str, sizes), per)))
+    # the oracle's tight solves (one thread per class, in parallel)
+    offs = np.cumsum([0] + counts[:-1]).tolist()
+
+    def tight(c):
+        return O.mc_reference(sizes[c], otab, xi[offs[c]:offs[c] + 1], cov=args.cov,
+                              tol=args.oracle_tol, keep_u=True, storage="csr")["u"][0]
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(len(sizes)) as ex:
+        u_ref = list(ex.map(tight, range(len(sizes))))
+    log(f"[bench] oracle solves to {args.oracle_tol}: {time.perf_counter() - t0:.1f} s wall")
+    rel = lambda a, r: float(np.linalg.norm(a - r) / np.linalg.norm(r))
+    indep = {str(n): rel(u_mixed[c], u_ref[c]) for c, n in enumerate(sizes)}
+    cpu_err = {str(n): rel(u_base[c], u_ref[c]) for c, n in enumerate(sizes)}
+    cpu["rel_err_vs_oracle_tight"] = cpu_err
+    parallel = None
+    nthr = args.cpu_threads
+    if nthr < 0:
+        nthr = min(len(aff), HOST_CORE_SHARE)
+    if nthr > 1:
+        def cpu_job(tid):
+            res, o = [], 0
+            for c, n in enumerate(sizes):
+                row = o + (1 + tid) % counts[c]
+                r = O.mc_reference(n, otab, xi[row:row + 1], cov=args.cov, tol=args.tol,
+                                   keep_u=False, storage="csr")
+                res.append(r["times"]["all"])
+                o += counts[c]
+            return res
+
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(nthr) as ex:
+            res = list(ex.map(cpu_job, range(nthr)))
+        wall = time.perf_counter() - t0
+        per_par = [sum(r[c] for r in res) / nthr for c in range(len(sizes))]
+        rate = nthr * N / sum(counts[c] * per_par[c] for c in range(len(sizes)))
+        parallel = dict(
+            value=rate, unit="solves/s", cores=nthr, kind="port",
+            sample=f"{nthr} threads (every core of the host share one GPU gets; the host shows "
+                   f"nproc {host['nproc']}, {host['affinity_cpus']} usable), each one further "
+                   f"sample per mesh size {sizes}, the same oracle restatement; rate = {nthr} x "
+                   f"{N} / sum(class count x mean per-size time under load), {wall:.1f} s wall")
+        log(f"[bench] cpu x{nthr}: {rate:.2f} solves/s ({wall:.1f} s wall)")
+    return cpu, parallel, indep
 
 
 def main():
     args = parse()
-    if args.newmark and args.gpus == 1:
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: the GPU box has one, and the "
+                                   "sharding of the Monte Carlo samples over several GPUs is a "
+                                   "design, not a measured path",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    if args.newmark:
         return bench_newmark(args)
     if args.copy_calibration:
         import torch
@@ -289,14 +586,17 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_each_way": 4 * n, "launches": 3}))
         return
+    import msolve_amd as msa
+    if args.only:
+        ctx = msa.Context(0)
+        fn = {"c2": leg_c2, "c3": leg_c3}.get(args.only)
+        r = fn(msa, ctx, args) if fn else leg_newmark(msa, ctx, args, args.newmark_steps)
+        ctx.close()
+        print(json.dumps({"only": args.only, "result": r}), flush=True)
+        return
     if args.only_mixed:
         args.no_fp64 = args.no_cpu = args.no_single = True
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of the "
-                                   "Monte Carlo samples is described in DESIGN.md",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
-    import msolve_amd as msa
+        args.no_c2 = args.no_c3 = args.no_c5 = True
     from msolve_amd.stochastic import kl_table, standard_normal
 
     sizes, counts = workload(args)
@@ -310,8 +610,19 @@ def main():
     run_kw = dict(nu=0.3, tol=args.tol, max_it=args.max_it, inner_tol=args.inner_tol,
                   max_outer=args.max_outer, graph_iters=args.graph_iters)
 
+    # ------------------------------------------------ schedule: autotuner
+    tuned = None
+    if args.tile <= 0:
+        t = time.perf_counter()
+        pk = msa.MSA_PRECOND_MULTIGRID if args.precond == "mg" else msa.MSA_PRECOND_JACOBI
+        tuned = msa.autotune_schedule(ctx, meshes, args.tune_samples, msa.MSA_MIXED, pk)
+        args.tile, args.unroll = tuned["tile"], tuned["unroll"]
+        log(f"[bench] autotuner ({time.perf_counter() - t:.1f} s): tile {args.tile}, unroll "
+            f"{args.unroll}; ms per sample {tuned['table']}")
+
     # ------------------------------------------------ headline: mixed, batched
     bm = make_batch(msa, ctx, meshes, models, counts, msa.MSA_MIXED, args.tile, table, xi, args)
+    bm.set_schedule(args.unroll)
     log(f"[bench] mixed batch: {bm.device_bytes / 1e9:.1f} GB on device")
     for w in range(args.warmup):
         t = time.perf_counter()
@@ -329,68 +640,34 @@ def main():
     stats = [bm.stats(c) for c in range(len(models))]
     all_ok = all((s["status"] == msa.MSA_OK).all() for s in stats)
 
-    # roofline of the dominant kernel: one more step with HIP events around
-    # every launch on the context stream
+    # rooflines: one more step with HIP events around every launch on the
+    # context stream; the kernel with the most time per step is `roofline`
     bm.set_kernel_timing(True)
     bm.run(**run_kw)
     kt = bm.kernel_timing()
     bm.set_kernel_timing(False)
     log(f"[bench] kernel timing: {json.dumps(kt)}")
-    a_launches = kt["spmv_dot"]["launches"]
-    a_bytes = kernel_a_bytes(bm, models, counts, 4, a_launches) / max(a_launches, 1)
-    a_s = kt["spmv_dot"]["ms"] / 1e3 / max(a_launches, 1)
-    roof_a = dict(kernel=f"k_pcg_a<float,{args.tile}> (SpMV + p update + p.q)",
-                  launches=a_launches, avg_launch_ms=1e3 * a_s, avg_bytes_per_launch=a_bytes,
-                  achieved=a_bytes / a_s / 1e9 if a_s > 0 else 0.0,
-                  ms_total=kt["spmv_dot"]["ms"])
-    roof_a["frac"] = roof_a["achieved"] / HBM_PEAK_GBS
-    # the dominant kernel: the finest level's smoothing products under multigrid
-    mf = mf_work(bm, models) if args.precond == "mg" else None
-    bound, unit, peak, flops_per_launch = "hbm", "GB/s", HBM_PEAK_GBS, None
-    if args.precond == "mg" and kt["multigrid"]["launches"] > 0:
-        launches = kt["multigrid"]["launches"]
-        avg_s = kt["multigrid"]["ms"] / 1e3 / launches
-        if mf is not None:
-            flops_per_launch = mf[0] / launches
-            avg_bytes = mf[1] / launches
-            bound, unit, peak = "mfma", "TFLOP/s", FP32_PEAK_TFLOPS
-            kname = (f"k_mg_smooth<float,1,STEP,2> on the finest level: matrix-free FP32 "
-                     f"products on box grids (FP32 VALU, no MFMA; priced against the FP32 peak) "
-                     f"of the cycle's four steps PRE, RESID, POST1, POST2")
-        else:
-            avg_bytes = smooth_bytes(bm, models, 4) / launches
-            kname = (f"k_mg_smooth<float,{args.tile},STEP> on the finest level (the cycle's four "
-                     f"products: PRE, RESID, POST1, POST2)")
+    roof_a = spmv_roofline(bm, kt, 4, f"k_pcg_a<float,{args.tile}> (SpMV + p update + p.q)",
+                           "k_pcg_a_counters.json")
+    roof_s = smoother_roofline(bm, models, kt, args) if args.precond == "mg" else None
+    if roof_s is not None and roof_s["ms_total"] > roof_a["ms_total"]:
+        roofline, secondary = roof_s, roof_a
     else:
-        launches, avg_bytes, avg_s = a_launches, a_bytes, a_s
-        kname = roof_a["kernel"]
-    if flops_per_launch is not None:
-        achieved = flops_per_launch / avg_s / 1e12 if avg_s > 0 else 0.0
-    else:
-        achieved = avg_bytes / avg_s / 1e9 if avg_s > 0 else 0.0
-    # HBM traffic of k_pcg_a: rocprofv3 FETCH_SIZE/WRITE_SIZE passes (scaled by a
-    # device-copy calibration) measured bytes / algorithmic bytes, applied to
-    # this run's algorithmic bytes per launch (profiles/k_pcg_a_mixed_montecarlo.md)
-    traffic, traffic_ratio = None, None
-    cpath = os.path.join(ROOT, "profiles", "k_pcg_a_counters.json" if args.precond == "jacobi"
-                         else ("k_mg_smooth_mf_counters.json" if mf is not None
-                               else "k_mg_smooth_counters.json"))
-    if os.path.exists(cpath):
-        cj = json.load(open(cpath))
-        traffic_ratio = (cj["fetch_bytes"] + cj["write_bytes"]) / (cj["algorithmic_read"] +
-                                                                   cj["algorithmic_write"])
-        traffic = avg_bytes * traffic_ratio
+        roofline, secondary = roof_a, roof_s
     u_mixed = [[bm.solution(c, s) for s in range(counts[c])] for c in range(len(models))]
     iters_mixed = [s["iters"].tolist() for s in stats]
     outer_mixed = [s["outer"].tolist() for s in stats]
     bm.close()
 
-    extra = {"k_pcg_a": roof_a, "kernel_timing_ms": {k: v["ms"] for k, v in kt.items()}}
+    extra = {"roofline_second": secondary,
+             "kernel_timing_ms_per_step": {k: v["ms"] for k, v in kt.items()},
+             "kernel_launches_per_step": {k: v["launches"] for k, v in kt.items()}}
     # ------------------------------------------------ the other preconditioner
     other = "jacobi" if args.precond == "mg" else "mg"
     if not args.no_compare and not args.only_mixed:
         bo = make_batch(msa, ctx, meshes, models, counts, msa.MSA_MIXED, args.tile, table, xi,
                         args, precond=other)
+        bo.set_schedule(args.unroll)
         bo.run(**run_kw)
         ctx.synchronize()
         t = time.perf_counter()
@@ -402,10 +679,11 @@ def main():
             str(n): float(np.mean(bo.stats(c)["iters"])) for c, n in enumerate(sizes)}
         log(f"[bench] mixed with {other}: {to:.2f} s -> {N / to:.1f} solves/s")
         bo.close()
-    # ------------------------------------------------ FP64: batched + oracle
+    # ------------------------------------------------ FP64 batched
     max_err = None
     if not args.no_fp64:
         b64 = make_batch(msa, ctx, meshes, models, counts, msa.MSA_FP64, args.tile, table, xi, args)
+        b64.run(**run_kw)
         ctx.synchronize()
         t = time.perf_counter()
         b64.run(**run_kw)
@@ -416,22 +694,26 @@ def main():
             str(n): float(np.mean(b64.stats(c)["iters"])) for c, n in enumerate(sizes)}
         log(f"[bench] fp64 batched: {t64:.2f} s -> {N / t64:.1f} solves/s, iters "
             f"{extra['fp64_iters_mean_per_size']}")
+        b64.close()
+        # every sample against a tightly converged FP64 batch (a batch of its
+        # own, one captured graph per batch)
+        bt = make_batch(msa, ctx, meshes, models, counts, msa.MSA_FP64, args.tile, table, xi, args)
         kw = dict(run_kw)
-        kw["tol"] = args.oracle_tol
-        b64.run(**kw)
+        kw["tol"] = args.fp64_tol
+        bt.run(**kw)
         ctx.synchronize()
         errs = []
         for c in range(len(models)):
-            st = b64.stats(c)
-            assert (st["status"] == msa.MSA_OK).all(), "FP64 oracle solve did not converge"
+            st = bt.stats(c)
+            assert (st["status"] == msa.MSA_OK).all(), "tight FP64 solve did not converge"
             for s in range(counts[c]):
-                ur = b64.solution(c, s)
+                ur = bt.solution(c, s)
                 errs.append(np.linalg.norm(u_mixed[c][s] - ur) / np.linalg.norm(ur))
         max_err = float(np.max(errs))
-        extra["rel_err_vs_fp64"] = dict(max=max_err, median=float(np.median(errs)),
-                                        oracle_tol=args.oracle_tol)
-        log(f"[bench] mixed vs FP64 oracle: max rel err {max_err:.3e}")
-        b64.close()
+        extra["rel_err_vs_fp64_gpu"] = dict(max=max_err, median=float(np.median(errs)),
+                                            fp64_tol=args.fp64_tol)
+        log(f"[bench] mixed vs tight FP64 batch: max rel err {max_err:.3e}")
+        bt.close()
 
     # ------------------------------------------------ single-sample FP64 GPU PCG
     if not args.no_single:
@@ -452,59 +734,28 @@ def main():
         extra["fp64_single_solves_per_s"] = N / t_single
         extra["fp64_single_s_per_size"] = dict(zip(map(str, sizes), per))
         log(f"[bench] fp64 single-sample: {N / t_single:.1f} solves/s ({per})")
+    for m in meshes:
+        m.close()
 
-    # ------------------------------------------------ CPU baseline
-    cpu = None
+    # ------------------------------------------------ the other configurations
+    if not args.no_c3:
+        extra["c3_fp64_batch_256x32"] = leg_c3(msa, ctx, args)
+    if not args.no_c2:
+        extra["c2_fp64_single_64"] = leg_c2(msa, ctx, args)
+    if not args.no_c5:
+        extra["c5_newmark"] = leg_newmark(msa, ctx, args, args.newmark_steps)
+    ctx.close()
+
+    # ------------------------------------------------ CPU baseline + oracle check
+    cpu, indep = None, None
     if not args.no_cpu:
-        from oracle import oracle as O
-        otab = O.kl_table([1.0, 1.0, 1.0], args.corr, args.terms)
-        per, t_meas = [], 0.0
-        off = 0
-        for c, n in enumerate(sizes):
-            out = O.mc_reference(n, otab, xi[off:off + 1], cov=args.cov, tol=args.tol,
-                                 keep_u=False, storage="csr")
-            per.append(out["times"]["all"])
-            t_meas += out["times"]["all"]
-            off += counts[c]
-            log(f"[bench] cpu {n}^3: {out['times']['all']:.2f} s, {int(out['iters'][0])} its")
-        t_cpu = sum(counts[c] * per[c] for c in range(len(sizes)))
-        cpu = dict(value=N / t_cpu, unit="solves/s", cores=1, kind="port",
-                   sample=f"one sample per mesh size {sizes} (first of each class), oracle "
-                          f"CSR/Jacobi-PCG FP64 restatement of the reference per-sample loop "
-                          f"(KL modulus, Ke, assembly, PCG to {args.tol}), serial; rate = "
-                          f"{N} / sum(class count x measured time), {t_meas:.1f} s measured")
-        # the same restatement on several host cores at once (samples are
-        # independent; the oracle's C calls release the interpreter lock):
-        # every thread solves one other sample of each size, the per-size
-        # times under that load weight the class counts as above
-        nthr = args.cpu_threads
-        if nthr < 0:
-            nthr = min(16, len(os.sched_getaffinity(0)))
-        if nthr > 1:
-            from concurrent.futures import ThreadPoolExecutor
-
-            def cpu_job(tid):
-                res, o = [], 0
-                for c, n in enumerate(sizes):
-                    row = o + (1 + tid) % counts[c]
-                    r = O.mc_reference(n, otab, xi[row:row + 1], cov=args.cov, tol=args.tol,
-                                       keep_u=False, storage="csr")
-                    res.append(r["times"]["all"])
-                    o += counts[c]
-                return res
-
-            t0 = time.perf_counter()
-            with ThreadPoolExecutor(nthr) as ex:
-                res = list(ex.map(cpu_job, range(nthr)))
-            wall = time.perf_counter() - t0
-            per_par = [sum(r[c] for r in res) / nthr for c in range(len(sizes))]
-            rate = nthr * N / sum(counts[c] * per_par[c] for c in range(len(sizes)))
-            extra["cpu_baseline_parallel"] = dict(
-                value=rate, unit="solves/s", cores=nthr, kind="port",
-                sample=f"{nthr} threads, each one further sample per mesh size {sizes}, the same "
-                       f"oracle restatement; rate = {nthr} x {N} / sum(class count x mean "
-                       f"per-size time under load), {wall:.1f} s wall")
-            log(f"[bench] cpu x{nthr}: {rate:.2f} solves/s ({wall:.1f} s wall)")
+        firsts = [u_mixed[c][0] for c in range(len(sizes))]
+        cpu, par, indep = cpu_baseline(args, sizes, counts, xi, firsts)
+        extra["cpu_baseline_parallel"] = par
+        extra["rel_err_vs_cpu_oracle"] = dict(per_size=indep, max=max(indep.values()),
+                                              oracle_tol=args.oracle_tol,
+                                              samples="first sample of each class")
+        log(f"[bench] mixed vs CPU oracle: {indep}")
 
     out = {
         "metric": METRIC,
@@ -532,18 +783,16 @@ def main():
                             f"inner Jacobi-PCG (tol {args.inner_tol}) + restarted FP64 refinement")
                          + f" to |f-Ku|/|f| < {args.tol}"),
             "samples": N, "sizes": sizes, "counts": counts, "tile": args.tile,
+            "unroll": args.unroll,
+            "schedule_from": "autotuner" if tuned else "command line",
+            "autotune_ms_per_sample": ([dict(tile=k[0], unroll=k[1], ms=v)
+                                        for k, v in tuned["table"].items()] if tuned else None),
             "precond": args.precond,
             "graph_iters": args.graph_iters, "all_converged": bool(all_ok),
-            "max_rel_err_vs_fp64": max_err,
+            "max_rel_err_vs_cpu_oracle": max(indep.values()) if indep else None,
+            "max_rel_err_vs_fp64_gpu": max_err,
         },
-        "roofline": {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
-                     "frac": achieved / peak, "traffic": traffic,
-                     "traffic_over_algorithmic": traffic_ratio,
-                     "kernel": kname,
-                     "launches": launches, "avg_launch_ms": 1e3 * avg_s,
-                     "avg_bytes_per_launch": avg_bytes,
-                     "avg_flop_per_launch": flops_per_launch,
-                     "hbm_gbs": avg_bytes / avg_s / 1e9 if avg_s > 0 else 0.0},
+        "roofline": roofline,
         "cpu_baseline": cpu,
         "extra": extra,
         "iters_mean_per_size": {str(n): float(np.mean(iters_mixed[c])) for c, n in enumerate(sizes)},
