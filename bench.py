@@ -83,6 +83,9 @@ def parse():
                    help="FP32 residual drop that triggers a replacement (default 1e-3 with "
                         "multigrid, 0.1 with Jacobi)")
     p.add_argument("--refine", choices=["reliable", "restart"], default="reliable")
+    p.add_argument("--matrix-cores", type=int, default=0,
+                   help="1: box-grid smoothing products on the matrix cores, 0: on the vector ALU "
+                        "(the other form is timed beside the headline)")
     p.add_argument("--precond", choices=["mg", "jacobi"], default="mg",
                    help="PCG preconditioner: multigrid V-cycle or Jacobi")
     p.add_argument("--no-compare", action="store_true",
@@ -267,8 +270,9 @@ def spmv_roofline(b, kt, rs, name, counter_file):
                 ms_total=kt["spmv_dot"]["ms"])
 
 
-def smoother_roofline(bm, models, kt, args):
-    """Roofline entry of the finest level's smoothing products."""
+def smoother_roofline(bm, models, kt, args, matrix_cores=True):
+    """Roofline entry of the finest level's smoothing products (matrix-core
+    or vector-ALU element products: the algorithmic flops are the same)."""
     launches = kt["multigrid"]["launches"]
     if launches == 0:
         return None
@@ -279,13 +283,19 @@ def smoother_roofline(bm, models, kt, args):
         ach = flop / avg_s / 1e12 if avg_s > 0 else 0.0
         ratio = counters("k_mg_smooth_mf_counters.json")
         byts = mf[1] / launches
-        return dict(bound="valu", achieved=ach, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+        if matrix_cores:
+            bound, name = "mfma", ("k_mg_smooth_mfma<STEP> on the finest level: matrix-free FP32 "
+                                   "products on box grids (PRE, RESID, POST1, POST2), element "
+                                   "products by v_mfma_f32_16x16x4f32")
+        else:
+            bound, name = "valu", ("k_mg_smooth<float,1,STEP,2> on the finest level: matrix-free "
+                                   "FP32 products on box grids (PRE, RESID, POST1, POST2), FP32 VALU")
+        return dict(bound=bound, achieved=ach, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
                     frac=ach / FP32_PEAK_TFLOPS,
                     frac_of_valu_without_packed_fp32=ach / FP32_VALU_NOPACK_TFLOPS,
                     traffic=byts * ratio if ratio is not None else None,
                     traffic_over_algorithmic=ratio,
-                    kernel="k_mg_smooth<float,1,STEP,2> on the finest level: matrix-free FP32 "
-                           "products on box grids (PRE, RESID, POST1, POST2), FP32 VALU",
+                    kernel=name,
                     launches=launches, avg_launch_ms=1e3 * avg_s, avg_flop_per_launch=flop,
                     avg_bytes_per_launch=byts, hbm_gbs=byts / avg_s / 1e9,
                     ms_total=kt["multigrid"]["ms"])
@@ -623,6 +633,7 @@ def main():
     # ------------------------------------------------ headline: mixed, batched
     bm = make_batch(msa, ctx, meshes, models, counts, msa.MSA_MIXED, args.tile, table, xi, args)
     bm.set_schedule(args.unroll)
+    bm.set_matrix_cores(bool(args.matrix_cores))
     log(f"[bench] mixed batch: {bm.device_bytes / 1e9:.1f} GB on device")
     for w in range(args.warmup):
         t = time.perf_counter()
@@ -649,7 +660,8 @@ def main():
     log(f"[bench] kernel timing: {json.dumps(kt)}")
     roof_a = spmv_roofline(bm, kt, 4, f"k_pcg_a<float,{args.tile}> (SpMV + p update + p.q)",
                            "k_pcg_a_counters.json")
-    roof_s = smoother_roofline(bm, models, kt, args) if args.precond == "mg" else None
+    roof_s = (smoother_roofline(bm, models, kt, args, bool(args.matrix_cores))
+              if args.precond == "mg" else None)
     if roof_s is not None and roof_s["ms_total"] > roof_a["ms_total"]:
         roofline, secondary = roof_s, roof_a
     else:
@@ -657,9 +669,30 @@ def main():
     u_mixed = [[bm.solution(c, s) for s in range(counts[c])] for c in range(len(models))]
     iters_mixed = [s["iters"].tolist() for s in stats]
     outer_mixed = [s["outer"].tolist() for s in stats]
+    other_form = None
+    if args.precond == "mg" and not args.only_mixed:
+        # the same batch with the smoothing products in the other form
+        mc = not bool(args.matrix_cores)
+        bm.set_matrix_cores(mc)
+        bm.run(**run_kw)
+        ctx.synchronize()
+        t = time.perf_counter()
+        bm.run(**run_kw)
+        ctx.synchronize()
+        tv = time.perf_counter() - t
+        bm.set_kernel_timing(True)
+        bm.run(**run_kw)
+        ktv = bm.kernel_timing()
+        bm.set_kernel_timing(False)
+        other_form = dict(matrix_cores=mc, solves_per_s=N / tv,
+                          roofline=smoother_roofline(bm, models, ktv, args, mc),
+                          kernel_timing_ms_per_step={k: v["ms"] for k, v in ktv.items()})
+        log(f"[bench] mixed with the {'matrix-core' if mc else 'vector-ALU'} smoother: "
+            f"{tv:.2f} s -> {N / tv:.1f} solves/s")
+        bm.set_matrix_cores(bool(args.matrix_cores))
     bm.close()
 
-    extra = {"roofline_second": secondary,
+    extra = {"roofline_second": secondary, "mixed_other_smoother_form": other_form,
              "kernel_timing_ms_per_step": {k: v["ms"] for k, v in kt.items()},
              "kernel_launches_per_step": {k: v["launches"] for k, v in kt.items()}}
     # ------------------------------------------------ the other preconditioner
@@ -787,7 +820,7 @@ def main():
             "schedule_from": "autotuner" if tuned else "command line",
             "autotune_ms_per_sample": ([dict(tile=k[0], unroll=k[1], ms=v)
                                         for k, v in tuned["table"].items()] if tuned else None),
-            "precond": args.precond,
+            "precond": args.precond, "matrix_cores": bool(args.matrix_cores),
             "graph_iters": args.graph_iters, "all_converged": bool(all_ok),
             "max_rel_err_vs_cpu_oracle": max(indep.values()) if indep else None,
             "max_rel_err_vs_fp64_gpu": max_err,

@@ -335,6 +335,13 @@ int msa_batch_direct_info(msa_batch *b, int cls, long long *info);
  * residual replacements every 2 iterations with a 1e-3 trigger under
  * multigrid and the first-sample factor (8 and 0.1 under Jacobi). */
 int msa_batch_set_preconditioner(msa_batch *b, int kind, int max_levels, double ratio);
+/* Implementation of the matrix-free smoothing products on box grids of
+ * fully free blocks (no counterpart in the reference): enable != 0 (the
+ * default) computes each cell's element product on the matrix cores
+ * (v_mfma_f32_16x16x4f32 over 16 cells at a time, for grids of at most 64
+ * cells in x), 0 keeps the FP32 vector-ALU form.  Both are FP32 and agree to
+ * FP32 rounding; other classes are unaffected. */
+int msa_batch_set_matrix_cores(msa_batch *b, int enable);
 /* info[0] coarse levels of class cls (0: Jacobi), [1] free DOFs of the
  * coarsest level, [2] free DOFs summed over the coarse levels, [3] half
  * bandwidth of the coarsest level, [4] 1 if the finest level's smoothing

@@ -42,7 +42,7 @@ EXPORTS = [
     "msa_batch_factorize", "msa_batch_direct_solve", "msa_batch_set_solver",
     "msa_batch_direct_info", "msa_batch_set_preconditioner", "msa_batch_mg_info",
     "msa_batch_mg_apply", "msa_batch_graph_info", "msa_autotune_schedule",
-    "msa_autotune_candidates", "msa_batch_set_schedule",
+    "msa_autotune_candidates", "msa_batch_set_schedule", "msa_batch_set_matrix_cores",
 ]
 
 
@@ -137,6 +137,7 @@ def lib():
         "msa_autotune_schedule": [_vp, _i, C.POINTER(_vp), _i, _i, _i, _ip, _ip, _dp],
         "msa_autotune_candidates": [_ip, _ip],
         "msa_batch_set_schedule": [_vp, _i],
+        "msa_batch_set_matrix_cores": [_vp, _i],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

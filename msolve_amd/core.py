@@ -309,6 +309,11 @@ class Batch:
         """Slots per group of the SpMV's rows (1..4 at lane tile 1; 3 otherwise)."""
         check(lib().msa_batch_set_schedule(self.h, int(unroll)))
 
+    def set_matrix_cores(self, enable: bool):
+        """Matrix-free smoothing on box grids: element products on the matrix
+        cores (default) or on the vector ALU."""
+        check(lib().msa_batch_set_matrix_cores(self.h, 1 if enable else 0))
+
     def graph_info(self):
         """Captured PCG iteration blocks: graphs captured (each validated on
         the host), kernel nodes of the current one, iterations per replay,

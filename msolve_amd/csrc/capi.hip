@@ -118,6 +118,11 @@ struct msa_batch {
     std::vector<int4 *> mg_sm_a, mg_sm_f, mg_sm_b;  // the same split: stored-value / matrix-free
                                                    // products on general / box grids
     std::vector<int> mg_nsm_a, mg_nsm_f, mg_nsm_b;
+    // box grids whose products can run on the matrix cores: their items for
+    // the MFMA kernel (class, sample, y block) and for the VALU box kernel
+    std::vector<int4 *> mg_sm_m, mg_sm_e;
+    std::vector<int> mg_nsm_m, mg_nsm_e;
+    bool matrix_cores = false;          // msa_batch_set_matrix_cores
     std::vector<int4 *> mg_restr;       // per depth >= 1: items of every level there
     std::vector<int> mg_nrestr;
     int4 *mg_lv_items = nullptr;        // every coarse level
@@ -823,6 +828,26 @@ int mf_setup(msa_batch *b, ClassHost &C)
     return MSA_OK;
 }
 
+// A box grid of fully free blocks whose smoothing products fit the MFMA
+// kernel: one-sample MIXED tiles, at most kMfmaMaxCells cells in x, and at
+// most as many y blocks as the class has PCG partial slots per sample.
+bool mfma_ok(const msa_batch *b, const ClassHost &C)
+{
+    const ClassDev &D = C.dev;
+    if (!is_mixed(b) || b->T != 1 || !D.mf || !D.mf_box || !D.mf_allfree) return false;
+    const int cxl = std::max(D.mf_box_lo[0] - 1, 0), cxh = std::min(D.mf_box_hi[0], D.mf_nx - 2);
+    const int nyb = (D.mf_box_hi[1] - D.mf_box_lo[1] + kMfmaRows) / kMfmaRows;
+    return cxh - cxl + 1 <= kMfmaMaxCells && nyb <= C.nchunks;
+}
+
+// Items of the MFMA smoother: (class, sample, y block of kMfmaRows rows).
+void mfma_items(const ClassHost &C, int ci, std::vector<int4> &out)
+{
+    const int nyb = (C.dev.mf_box_hi[1] - C.dev.mf_box_lo[1] + kMfmaRows) / kMfmaRows;
+    for (int smp = 0; smp < C.nsamp; smp++)
+        for (int y = 0; y < nyb; y++) out.push_back(make_int4(ci, smp, y, 0));
+}
+
 // Coarse hierarchies of every class whose mesh is a structured grid that
 // halves at least once and whose coarsest level stays small; the other
 // classes keep the Jacobi preconditioner.
@@ -935,12 +960,16 @@ int mg_build(msa_batch *b, int max_levels)
     b->mg_sm_f.assign(b->mg_depth, nullptr);
     b->mg_sm_b.assign(b->mg_depth, nullptr);
     b->mg_nsm_b.assign(b->mg_depth, 0);
+    b->mg_sm_m.assign(b->mg_depth, nullptr);
+    b->mg_nsm_m.assign(b->mg_depth, 0);
+    b->mg_sm_e.assign(b->mg_depth, nullptr);
+    b->mg_nsm_e.assign(b->mg_depth, 0);
     b->mg_nsm_f.assign(b->mg_depth, 0);
     b->mg_restr.assign(b->mg_depth + 1, nullptr);
     b->mg_nrestr.assign(b->mg_depth + 1, 0);
     std::vector<int4> all, co;
     for (int d = 0; d <= b->mg_depth; d++) {
-        std::vector<int4> sm, sma, smf, smb, re;
+        std::vector<int4> sm, sma, smf, smb, smm, sme, re;
         for (int ci : depth_classes[d]) {
             ClassHost &C = tab(b, ci);
             if (d > 0) {
@@ -949,7 +978,12 @@ int mg_build(msa_batch *b, int max_levels)
             }
             if (C.dev.mg_coarse >= 0) {
                 class_items(C, ci, sm);
-                class_items(C, ci, !C.dev.mf ? sma : (C.dev.mf_box ? smb : smf));
+                if (mfma_ok(b, C)) {
+                    mfma_items(C, ci, smm);
+                    class_items(C, ci, sme);
+                } else {
+                    class_items(C, ci, !C.dev.mf ? sma : (C.dev.mf_box ? smb : smf));
+                }
             } else if (d > 0) {
                 class_items(C, ci, co);
             }
@@ -959,6 +993,8 @@ int mg_build(msa_batch *b, int max_levels)
             MSA_TRY(items_to_device(b, sma, &b->mg_sm_a[d], &b->mg_nsm_a[d]));
             MSA_TRY(items_to_device(b, smf, &b->mg_sm_f[d], &b->mg_nsm_f[d]));
             MSA_TRY(items_to_device(b, smb, &b->mg_sm_b[d], &b->mg_nsm_b[d]));
+            MSA_TRY(items_to_device(b, smm, &b->mg_sm_m[d], &b->mg_nsm_m[d]));
+            MSA_TRY(items_to_device(b, sme, &b->mg_sm_e[d], &b->mg_nsm_e[d]));
         }
         if (d > 0) MSA_TRY(items_to_device(b, re, &b->mg_restr[d], &b->mg_nrestr[d]));
     }
@@ -1050,6 +1086,9 @@ int mg_values(msa_batch *b)
         }));
         MSA_TRY(timed(b, KK_OTHER, [&] {
             return launch_mg_gersh_mf(s, b->d_cls, b->mg_sm_b[d], b->mg_nsm_b[d]);
+        }));
+        MSA_TRY(timed(b, KK_OTHER, [&] {
+            return launch_mg_gersh_mf(s, b->d_cls, b->mg_sm_e[d], b->mg_nsm_e[d]);
         }));
     }
     for (int ci : b->mg_coarsest) {
@@ -1203,6 +1242,16 @@ int mg_vcycle(msa_batch *b, int pbuf, int mode)
         if (b->mg_nsm_b[d] > 0)
             MSA_TRY(timed(b, d == 0 ? KK_MG : KK_OTHER, [&] {
                 return launch_mg_smooth(s, mx, T, b->d_cls, b->mg_sm_b[d], b->mg_nsm_b[d], step,
+                                        pbuf, mode, ratio, 2);
+            }));
+        if (b->matrix_cores && b->mg_nsm_m[d] > 0)
+            MSA_TRY(timed(b, d == 0 ? KK_MG : KK_OTHER, [&] {
+                return launch_mg_smooth_mfma(s, b->d_cls, b->mg_sm_m[d], b->mg_nsm_m[d], step, pbuf,
+                                             mode, ratio);
+            }));
+        if (!b->matrix_cores && b->mg_nsm_e[d] > 0)
+            MSA_TRY(timed(b, d == 0 ? KK_MG : KK_OTHER, [&] {
+                return launch_mg_smooth(s, mx, T, b->d_cls, b->mg_sm_e[d], b->mg_nsm_e[d], step,
                                         pbuf, mode, ratio, 2);
             }));
         return (int)MSA_OK;
@@ -2585,6 +2634,18 @@ int msa_batch_set_refinement(msa_batch *b, int method, int period, double drop)
     return MSA_OK;
 }
 
+int msa_batch_set_matrix_cores(msa_batch *b, int enable)
+{
+    MSA_CHECK(b, MSA_ERR_ARG, "null batch");
+    const bool on = enable != 0;
+    if (b->matrix_cores != on && b->graph) {  // the captured iteration block holds the kernels
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    b->matrix_cores = on;
+    return MSA_OK;
+}
+
 int msa_batch_graph_info(msa_batch *b, long long *info)
 {
     MSA_CHECK(b && info, MSA_ERR_ARG, "bad arguments");
@@ -2609,7 +2670,7 @@ int msa_batch_set_kernel_timing(msa_batch *b, int enable)
 {
     MSA_CHECK(b, MSA_ERR_ARG, "null batch");
     b->timing.on = enable != 0;
-    for (int k = 0; k < 5; k++) {
+    for (int k = 0; k < kKernelKinds; k++) {
         b->timing.ms[k] = 0;
         b->timing.n[k] = 0;
     }

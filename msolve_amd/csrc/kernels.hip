@@ -3144,6 +3144,47 @@ __global__ __launch_bounds__(256) void k_residual64_mf(const ClassDev *__restric
     }
 }
 
+// Row update of one smoothing step at one-sample tiles, given y = K src of
+// block row br: the row's three components of each vector move as one
+// 12- or 24-byte access.  PRE A = (1 + c1) z/theta + c2 D^-1 (b - K z/theta),
+// RESID B = b - K A, POST1 B = A + D^-1 (b - K A)/theta, POST2
+// z = B + c1 (B - A) + c2 D^-1 (b - K B) (and r.z).
+template <typename real, int STEP>
+__device__ __forceinline__ void mg_update1(const MSA_GLOBAL real *b, const MSA_GLOBAL real *dinv,
+                                           const MSA_GLOBAL real *z, const MSA_GLOBAL real *A,
+                                           const MSA_GLOBAL real *B, MSA_GLOBAL real *dst, int br,
+                                           const real (&y)[3], real ith, real c1, real c2,
+                                           real &rz)
+{
+    real bv[3], dv[3], xa[3], xb[3], out[3];
+    auto get = [&](const MSA_GLOBAL real *p, real (&v)[3]) {
+        const vec3<real> t = ld3(p, br);
+        v[0] = t.x;
+        v[1] = t.y;
+        v[2] = t.z;
+    };
+    get(b, bv);
+    if constexpr (STEP != MG_RESID) get(dinv, dv);
+    if constexpr (STEP == MG_PRE) get(z, xa);
+    if constexpr (STEP == MG_POST1 || STEP == MG_POST2) get(A, xa);
+    if constexpr (STEP == MG_POST2) get(B, xb);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        if constexpr (STEP == MG_PRE) {
+            const real x1 = xa[d] * ith;
+            out[d] = x1 + c1 * x1 + c2 * dv[d] * (bv[d] - y[d] * ith);
+        } else if constexpr (STEP == MG_RESID) {
+            out[d] = bv[d] - y[d];
+        } else if constexpr (STEP == MG_POST1) {
+            out[d] = xa[d] + ith * dv[d] * (bv[d] - y[d]);
+        } else {
+            out[d] = xb[d] + c1 * (xb[d] - xa[d]) + c2 * dv[d] * (bv[d] - y[d]);
+            rz += bv[d] * out[d];
+        }
+    }
+    st3(dst, br, out[0], out[1], out[2]);
+}
+
 // One smoothing-level product with its row update (STEP, see MgStep):
 //   MG_PRE   A = (1 + c1) x1 + c2 D^-1 (b - K x1),  x1 = z / theta
 //   MG_RESID B = b - K A
@@ -3194,35 +3235,7 @@ __global__ __launch_bounds__(256) void k_mg_smooth(const ClassDev *__restrict__ 
             else
                 row_product<real, T>(val, cidx, src, base, W, rr, s, y[0], y[1], y[2]);
             if constexpr (T == 1) {
-                // the row's three components of each vector move as one
-                // 12- or 24-byte access; the same arithmetic as below
-                real bv[3], dv[3], xa[3], xb[3], out[3];
-                auto get = [&](const MSA_GLOBAL real *p, real (&v)[3]) {
-                    const vec3<real> t = ld3(p, br);
-                    v[0] = t.x;
-                    v[1] = t.y;
-                    v[2] = t.z;
-                };
-                get(b, bv);
-                if constexpr (STEP != MG_RESID) get(dinv, dv);
-                if constexpr (STEP == MG_PRE) get(z, xa);
-                if constexpr (STEP == MG_POST1 || STEP == MG_POST2) get(A, xa);
-                if constexpr (STEP == MG_POST2) get(B, xb);
-#pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    if constexpr (STEP == MG_PRE) {
-                        const real x1 = xa[d] * ith;
-                        out[d] = x1 + c1 * x1 + c2 * dv[d] * (bv[d] - y[d] * ith);
-                    } else if constexpr (STEP == MG_RESID) {
-                        out[d] = bv[d] - y[d];
-                    } else if constexpr (STEP == MG_POST1) {
-                        out[d] = xa[d] + ith * dv[d] * (bv[d] - y[d]);
-                    } else {
-                        out[d] = xb[d] + c1 * (xb[d] - xa[d]) + c2 * dv[d] * (bv[d] - y[d]);
-                        rz += bv[d] * out[d];
-                    }
-                }
-                st3(dst, br, out[0], out[1], out[2]);
+                mg_update1<real, STEP>(b, dinv, z, A, B, dst, br, y, ith, c1, c2, rz);
                 continue;
             }
 #pragma unroll
@@ -3251,6 +3264,366 @@ __global__ __launch_bounds__(256) void k_mg_smooth(const ClassDev *__restrict__ 
             double acc[1] = {(double)rz};
             block_partials<T, 1>(acc, C.partial + ((size_t)tile * C.nchunks + chunk) * 3 * T);
         }
+    }
+}
+
+// The matrix-free smoothing products of a box grid of fully free blocks
+// with the element contraction on the matrix cores (MIXED, one-sample
+// tiles).  A workgroup owns kMfBY rows of nodes (all x) of one sample and
+// marches through the grid's cell layers in z.  Per layer:
+//   1. the layer's top node plane (of the product's input vector) and the
+//      cells' moduli go to LDS (the bottom plane is already there);
+//   2. every cell's 24 outputs Y_c = Khat (cK E_c U_c) (+ Mhat (cM U_c) in
+//      dynamics) by v_mfma_f32_16x16x4f32: 16 cells per product, Khat's 24
+//      rows as two 16-row blocks (the second half-used), K in 6 (12) steps
+//      of 4; U_c, the cell's 8 corners x 3 components, gathered from LDS;
+//   3. each owned node of the layer's bottom plane sums the 4 cells below it
+//      (kept from the previous layer) and the 4 above, in a fixed order, and
+//      takes the step's row update (mg_update1, as k_mg_smooth); the sums
+//      of the 4 cells below the next plane are kept for the next layer.
+// Cells in the rows next to the owned ones are computed by both
+// neighbouring workgroups, so every node's sum is formed in one place
+// (no atomics; results do not depend on scheduling).  Level 0's POST2 writes
+// its r.z partial in slot (sample, y block); y block 0 clears the slots
+// past the last block.
+constexpr int kMfBY = kMfmaRows;          // owned node rows per workgroup
+constexpr int kMfMaxCX = kMfmaMaxCells;   // cells in x a workgroup handles
+constexpr int kMfMaxCY = kMfBY + 1;
+constexpr int kMfNNX = kMfMaxCX + 1, kMfNNY = kMfMaxCY + 1;
+constexpr int kMfYS = 25;                 // LDS floats per cell (odd: conflict-free node reads)
+
+template <int STEP>
+__global__ __launch_bounds__(256) void k_mg_smooth_mfma(const ClassDev *__restrict__ cls,
+                                                        const int4 *__restrict__ items, int nitems,
+                                                        int pbuf, int mode, double ratio)
+{
+    __shared__ float Ul[2][kMfNNX * kMfNNY * 3];
+    __shared__ float El[kMfMaxCX * kMfMaxCY];
+    __shared__ float Yl[kMfMaxCX * kMfMaxCY * kMfYS];
+    __shared__ float Al[(kMfMaxCX + 1) * kMfBY * 3];
+    __shared__ float red[4];
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const int sample = it.y, yb = it.z;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    // box of free blocks, cell ranges touching this workgroup's nodes
+    const int nx = C.mf_nx, ny = C.mf_ny, nz = C.mf_nz;
+    const int b0x = C.mf_box_lo[0], b0y = C.mf_box_lo[1], b0z = C.mf_box_lo[2];
+    const int b1x = C.mf_box_hi[0], b1y = C.mf_box_hi[1], b1z = C.mf_box_hi[2];
+    const int nyb = (b1y - b0y + kMfBY) / kMfBY;
+    const int y0 = b0y + yb * kMfBY, y1 = min(y0 + kMfBY - 1, b1y);
+    const int cxl = max(b0x - 1, 0), cxh = min(b1x, nx - 2);
+    const int cyl = max(y0 - 1, 0), cyh = min(y1, ny - 2);
+    const int czl = max(b0z - 1, 0), czh = min(b1z, nz - 2);
+    const int NCX = cxh - cxl + 1, NCY = cyh - cyl + 1, NNX = NCX + 1, NNY = NCY + 1;
+    const int NOX = b1x - b0x + 1, NOY = y1 - y0 + 1, ncell = NCX * NCY;
+    const int nnode = NNX * NNY, nown = NOX * NOY;
+    const int sy = b1x - b0x + 1, sz = sy * (b1y - b0y + 1);
+    const bool part = STEP == MG_POST2 && C.mg_depth == 0;
+    double *pout = C.partial + (size_t)sample * C.nchunks * 3;
+    if (part && yb == 0)
+        for (int k = nyb + tid; k < C.nchunks; k += 256) pout[(size_t)k * 3] = 0.0;
+    const bool act = sample < C.nsamp && mg_act(cls[C.mg_root].st[sample], mode);
+    if (!act) {
+        if (part && tid == 0) pout[(size_t)yb * 3] = 0.0;
+        return;
+    }
+    const ChebCoef cc = cheb_coef(C.mg_lam[sample], ratio);
+    const float ith = (float)cc.ith, c1 = (float)cc.c1, c2 = (float)cc.c2;
+    const size_t vb = (size_t)sample * C.tile_vec;
+    const MSA_GLOBAL float *__restrict__ b = gbl((const float *)C.r + vb);
+    const MSA_GLOBAL float *__restrict__ dinv = gbl((const float *)C.dinv + vb);
+    MSA_GLOBAL float *__restrict__ z = gbl((float *)C.z + vb);
+    MSA_GLOBAL float *__restrict__ A = gbl((float *)C.q + vb);
+    MSA_GLOBAL float *__restrict__ B = gbl((float *)(pbuf ? C.p1 : C.p0) + vb);
+    const MSA_GLOBAL float *src = STEP == MG_PRE ? z : (STEP == MG_POST2 ? B : A);
+    MSA_GLOBAL float *dst = STEP == MG_PRE ? A : (STEP == MG_POST2 ? z : B);
+    const MSA_GLOBAL float *xsa = STEP == MG_PRE ? z : A;  // the update's first iterate
+    const double *Es = C.E + (size_t)sample * C.n_elem;
+    const float cK = C.dyn ? (float)C.cK : 1.f, cM = C.dyn ? (float)C.cM : 0.f;
+    const bool mass = cM != 0.f;
+    const bool cellid = C.mf_cell_id != 0;
+    const int *celltab = C.mf_cell;
+    // matrix-core operands of Khat (and Mhat x I3): block m, step s holds
+    // row 16 m + (lane & 15), column 4 s + (lane >> 4) in corner order
+    const int j16 = lane & 15, g = lane >> 4;
+    float ak[2][6], am[2][6];
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int st = 0; st < 6; st++) {
+            const int i = 16 * m + j16, k = 4 * st + g;
+            const bool in = i < 24;
+            ak[m][st] = in ? C.mf_k[((i / 3) * 8 + k / 3) * 9 + 3 * (i % 3) + k % 3] : 0.f;
+            am[m][st] = (in && mass && i % 3 == k % 3) ? C.mf_k[576 + (i / 3) * 8 + k / 3] : 0.f;
+        }
+    // Loads for the next layer and the row updates are issued before the
+    // products of the current layer and consumed after them, so their
+    // latency overlaps the matrix-core work: a thread holds up to 3 plane
+    // nodes, 2 cells and 2 owned nodes (nnode <= 585, ncell <= 512,
+    // nown <= 455).
+    float pv[3][3], ev[2], fb[2][3], fd[2][3], fa[2][3], fx[2][3];
+    // the thread's layer-invariant indices: plane nodes (block index in the
+    // plane, -1 outside the box or past the plane), cells (index in the
+    // layer, -1 past it), owned nodes (block index in the plane, -1 past)
+    // and their cells in the layer (-1: no such cell); quotients by the
+    // float reciprocal (exact here: t < 1024, divisors <= 65)
+    auto divmod = [](int t, int d, float inv, int &qq, int &rr) {
+        qq = (int)(((float)t + 0.5f) * inv);
+        rr = t - qq * d;
+    };
+    const float inx = 1.f / (float)NNX, icx = 1.f / (float)NCX, iox = 1.f / (float)NOX;
+    int pnode[3], pcell[2], obr[2], oq[2][4];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const int t = tid + 256 * i;
+        int ly, lx;
+        divmod(t, NNX, inx, ly, lx);
+        const int gx = cxl + lx, gy = cyl + ly;
+        pnode[i] = (t < nnode && gx >= b0x && gx <= b1x && gy >= b0y && gy <= b1y)
+                       ? (gx - b0x) + sy * (gy - b0y) : -1;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const int t = tid + 256 * i;
+        int ly, lx;
+        divmod(t, NCX, icx, ly, lx);
+        pcell[i] = t < ncell ? (cxl + lx) + (nx - 1) * (cyl + ly) : -1;
+        divmod(t, NOX, iox, ly, lx);
+        const int gx = b0x + lx, gy = y0 + ly;
+        obr[i] = t < nown ? lx + sy * (y0 - b0y + ly) : -1;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int cx = gx - 1 + (c & 1), cy = gy - 1 + (c >> 1);
+            oq[i][c] = (t < nown && cx >= cxl && cx <= cxh && cy >= cyl && cy <= cyh)
+                           ? (cx - cxl) + NCX * (cy - cyl) : -1;
+        }
+    }
+    auto fetch_plane = [&](int zz) {
+        const bool zin = zz >= b0z && zz <= b1z;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            pv[i][0] = pv[i][1] = pv[i][2] = 0.f;
+            if (zin && pnode[i] >= 0) {
+                const vec3<float> w = ld3(src, (size_t)pnode[i] + (size_t)sz * (zz - b0z));
+                pv[i][0] = w.x;
+                pv[i][1] = w.y;
+                pv[i][2] = w.z;
+            }
+        }
+    };
+    auto store_plane = [&](int slot) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const int t = tid + 256 * i;
+            if (t < nnode) {
+                Ul[slot][3 * t] = pv[i][0];
+                Ul[slot][3 * t + 1] = pv[i][1];
+                Ul[slot][3 * t + 2] = pv[i][2];
+            }
+        }
+    };
+    const int lstride = (nx - 1) * (ny - 1);
+    auto fetch_E = [&](int cz) {
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            ev[i] = 0.f;
+            if (pcell[i] >= 0) {
+                const int cell = pcell[i] + lstride * cz;
+                ev[i] = (float)Es[cellid ? cell : celltab[cell]] * cK;
+            }
+        }
+    };
+    auto store_E = [&]() {
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const int t = tid + 256 * i;
+            if (t < ncell) El[t] = ev[i];
+        }
+    };
+    auto fetch_rows = [&](int zz) {
+        if (zz < b0z || zz > b1z) return;
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            if (obr[i] < 0) continue;
+            const int br = obr[i] + sz * (zz - b0z);
+            const vec3<float> vb3 = ld3(b, br);
+            fb[i][0] = vb3.x;
+            fb[i][1] = vb3.y;
+            fb[i][2] = vb3.z;
+            if constexpr (STEP != MG_RESID) {
+                const vec3<float> w = ld3(dinv, br);
+                fd[i][0] = w.x;
+                fd[i][1] = w.y;
+                fd[i][2] = w.z;
+            }
+            if constexpr (STEP != MG_RESID) {
+                const vec3<float> w = ld3(xsa, br);
+                fa[i][0] = w.x;
+                fa[i][1] = w.y;
+                fa[i][2] = w.z;
+            }
+            if constexpr (STEP == MG_POST2) {
+                const vec3<float> w = ld3(B, br);
+                fx[i][0] = w.x;
+                fx[i][1] = w.y;
+                fx[i][2] = w.z;
+            }
+        }
+    };
+    float rz = 0.f;
+    // the owned nodes of plane zz: the sums of the cells below (Al) and, when
+    // `above`, of the current layer's cells, then the row update (the
+    // arithmetic of mg_update1); `keep` stores the current layer's sums for
+    // the plane above
+    auto finalize = [&](int zz, bool above, bool keep) {
+        const bool own = zz >= b0z && zz <= b1z;
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const int t = tid + 256 * i;
+            if (obr[i] < 0) continue;
+            float yv[3] = {Al[3 * t], Al[3 * t + 1], Al[3 * t + 2]};
+            float at[3] = {0.f, 0.f, 0.f};
+            if (above) {
+#pragma unroll
+                for (int c = 0; c < 4; c++) {  // cells (x - 1 + dx, y - 1 + dy), c = dx + 2 dy
+                    if (oq[i][c] < 0) continue;
+                    const float *yq = Yl + oq[i][c] * kMfYS;
+                    const int ob = (1 - (c & 1)) + 2 * (1 - (c >> 1));  // the node's corner, below
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        yv[d] += yq[3 * ob + d];
+                        at[d] += yq[3 * (ob + 4) + d];  // as the cell's top corner
+                    }
+                }
+            }
+            if (own) {
+                float out[3];
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    const float bv = fb[i][d];
+                    if constexpr (STEP == MG_PRE) {
+                        const float x1 = fa[i][d] * ith;
+                        out[d] = x1 + c1 * x1 + c2 * fd[i][d] * (bv - yv[d] * ith);
+                    } else if constexpr (STEP == MG_RESID) {
+                        out[d] = bv - yv[d];
+                    } else if constexpr (STEP == MG_POST1) {
+                        out[d] = fa[i][d] + ith * fd[i][d] * (bv - yv[d]);
+                    } else {
+                        out[d] = fx[i][d] + c1 * (fx[i][d] - fa[i][d]) + c2 * fd[i][d] * (bv - yv[d]);
+                        rz += bv * out[d];
+                    }
+                }
+                st3(dst, obr[i] + sz * (zz - b0z), out[0], out[1], out[2]);
+            }
+            if (keep) {
+                Al[3 * t] = at[0];
+                Al[3 * t + 1] = at[1];
+                Al[3 * t + 2] = at[2];
+            }
+        }
+    };
+    // products of one layer's cells: groups of 16 cells, two groups per
+    // wave in flight (four independent accumulators)
+    auto products = [&](int cur, int nxt) {
+        for (int gi = wave; gi * 16 < ncell; gi += 8) {
+            const int q0 = gi * 16 + j16, q1 = q0 + 64;
+            const bool v0 = q0 < ncell, v1 = q1 < ncell;
+            float w0[6], w1[6];
+            const float e0 = v0 ? El[q0] : 0.f, e1 = v1 ? El[q1] : 0.f;
+            int y0c, x0, y1c, x1;
+            divmod(v0 ? q0 : 0, NCX, icx, y0c, x0);
+            divmod(v1 ? q1 : 0, NCX, icx, y1c, x1);
+#pragma unroll
+            for (int st = 0; st < 6; st++) {
+                const int k = 4 * st + g, o = k / 3, p = k - 3 * (k / 3);
+                const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
+                const float *U = Ul[oz ? nxt : cur];
+                w0[st] = v0 ? U[((y0c + oy) * NNX + x0 + ox) * 3 + p] : 0.f;
+                w1[st] = v1 ? U[((y1c + oy) * NNX + x1 + ox) * 3 + p] : 0.f;
+            }
+            f32x4 d00 = {0.f, 0.f, 0.f, 0.f}, d01 = d00, d10 = d00, d11 = d00;
+#pragma unroll
+            for (int st = 0; st < 6; st++) {
+                const float a = e0 * w0[st], c = e1 * w1[st];
+                d00 = __builtin_amdgcn_mfma_f32_16x16x4f32(ak[0][st], a, d00, 0, 0, 0);
+                d01 = __builtin_amdgcn_mfma_f32_16x16x4f32(ak[1][st], a, d01, 0, 0, 0);
+                d10 = __builtin_amdgcn_mfma_f32_16x16x4f32(ak[0][st], c, d10, 0, 0, 0);
+                d11 = __builtin_amdgcn_mfma_f32_16x16x4f32(ak[1][st], c, d11, 0, 0, 0);
+            }
+            if (mass) {
+#pragma unroll
+                for (int st = 0; st < 6; st++) {
+                    const float a = cM * w0[st], c = cM * w1[st];
+                    d00 = __builtin_amdgcn_mfma_f32_16x16x4f32(am[0][st], a, d00, 0, 0, 0);
+                    d01 = __builtin_amdgcn_mfma_f32_16x16x4f32(am[1][st], a, d01, 0, 0, 0);
+                    d10 = __builtin_amdgcn_mfma_f32_16x16x4f32(am[0][st], c, d10, 0, 0, 0);
+                    d11 = __builtin_amdgcn_mfma_f32_16x16x4f32(am[1][st], c, d11, 0, 0, 0);
+                }
+            }
+            if (v0) {
+                float *yq = Yl + q0 * kMfYS;
+#pragma unroll
+                for (int r = 0; r < 4; r++) yq[4 * g + r] = d00[r];
+                if (g < 2) {
+#pragma unroll
+                    for (int r = 0; r < 4; r++) yq[16 + 4 * g + r] = d01[r];
+                }
+            }
+            if (v1) {
+                float *yq = Yl + q1 * kMfYS;
+#pragma unroll
+                for (int r = 0; r < 4; r++) yq[4 * g + r] = d10[r];
+                if (g < 2) {
+#pragma unroll
+                    for (int r = 0; r < 4; r++) yq[16 + 4 * g + r] = d11[r];
+                }
+            }
+        }
+    };
+    fetch_plane(czl);
+    store_plane(0);
+    fetch_plane(czl + 1);
+    store_plane(1);
+    fetch_E(czl);
+    store_E();
+    for (int t = tid; t < nown * 3; t += 256) Al[t] = 0.f;
+    __syncthreads();
+    int cur = 0;
+    for (int cz = czl; cz <= czh; cz++) {
+        const int nxt = cur ^ 1;
+        const bool more = cz + 1 <= czh;
+        if (more) {
+            fetch_plane(cz + 2);
+            fetch_E(cz + 1);
+        }
+        fetch_rows(cz);
+        products(cur, nxt);
+        __syncthreads();
+        finalize(cz, true, true);
+        if (more) {
+            store_plane(cur);  // plane cz + 2 takes plane cz's slot
+            store_E();
+        }
+        __syncthreads();
+        cur = nxt;
+    }
+    // the top plane of the grid has no cell layer above it
+    if (czh + 1 <= b1z) {
+        fetch_rows(czh + 1);
+        finalize(czh + 1, false, false);
+    }
+    if (part) {
+        float v = rz;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) red[tid >> 6] = v;
+        __syncthreads();
+        if (tid == 0)
+            pout[(size_t)yb * 3] = (double)red[0] + (double)red[1] + (double)red[2] + (double)red[3];
     }
 }
 
@@ -4299,6 +4672,21 @@ hipError_t launch_mg_smooth(hipStream_t s, bool mixed, int T, const ClassDev *d_
     case MG_POST2: return mg_smooth_step<MG_POST2>(s, mixed, T, d_cls, items, nitems, pbuf, mode, ratio, mf);
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_mg_smooth_mfma(hipStream_t s, const ClassDev *d_cls, const int4 *items,
+                                 int nitems, int step, int pbuf, int mode, double ratio)
+{
+    if (nitems <= 0) return hipSuccess;
+    const dim3 grid(xcd_grid(nitems)), block(256);
+    switch (step) {
+    case MG_PRE: hipLaunchKernelGGL(k_mg_smooth_mfma<MG_PRE>, grid, block, 0, s, d_cls, items, nitems, pbuf, mode, ratio); break;
+    case MG_RESID: hipLaunchKernelGGL(k_mg_smooth_mfma<MG_RESID>, grid, block, 0, s, d_cls, items, nitems, pbuf, mode, ratio); break;
+    case MG_POST1: hipLaunchKernelGGL(k_mg_smooth_mfma<MG_POST1>, grid, block, 0, s, d_cls, items, nitems, pbuf, mode, ratio); break;
+    case MG_POST2: hipLaunchKernelGGL(k_mg_smooth_mfma<MG_POST2>, grid, block, 0, s, d_cls, items, nitems, pbuf, mode, ratio); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_mg_restrict(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,

@@ -362,6 +362,16 @@ enum MgStep { MG_PRE = 0, MG_RESID = 1, MG_POST1 = 2, MG_POST2 = 3 };
 hipError_t launch_mg_smooth(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
                             const int4 *items, int nitems, int step, int pbuf, int mode,
                             double ratio, int mf);
+// matrix-free smoothing with the element products on the matrix cores (box
+// grids of fully free blocks, MIXED, one-sample tiles); items: (class,
+// sample, y block of kMfmaRows node rows, 0)
+hipError_t launch_mg_smooth_mfma(hipStream_t s, const ClassDev *d_cls, const int4 *items,
+                                 int nitems, int step, int pbuf, int mode, double ratio);
+#ifndef MSA_MFMA_ROWS
+#define MSA_MFMA_ROWS 7
+#endif
+constexpr int kMfmaRows = MSA_MFMA_ROWS;  // node rows (y) per workgroup of the MFMA smoother
+constexpr int kMfmaMaxCells = 64;  // cells in x it handles
 hipError_t launch_mg_restrict(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
                               const int4 *items, int nitems, int pbuf, int mode);
 hipError_t launch_mg_prolong(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,

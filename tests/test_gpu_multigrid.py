@@ -324,3 +324,85 @@ def test_matrix_free_forms_match_oracle_cycle(ctx, kind):
         b.close()
         assert rel(z, ref[prec]) < bar[prec], (prec, rel(z, ref[prec]))
     mesh.close()
+
+
+def _box_variant(kind):
+    """Box grids of fully free blocks for the matrix-core smoother:
+    "cube" 12 x 12 x 8 (two y blocks of 7 and 6 node rows), "wide" 64 x 6 x 4
+    (64 cells in x, the kernel's limit; one y block of 7 rows), "clamped2"
+    10 x 8 x 6 with the x = 0 and z = 0 faces clamped (the box starts at
+    z = 1, so the first cell layer's bottom plane is not owned) and "rows"
+    10 x 16 x 4 (17 node rows: 7 + 7 + 3).  Every count is even, so each
+    grid coarsens at least once."""
+    from types import SimpleNamespace
+    from msolve_amd.core import enumerate_dofs
+    if kind == "cube":
+        m = msa.cantilever(12, 12, 8, lz=0.5)
+    elif kind == "wide":
+        m = msa.cantilever(64, 6, 4, lx=4.0, ly=0.4, lz=0.3)
+    elif kind == "rows":
+        m = msa.cantilever(10, 16, 4, ly=1.6, lz=0.4)
+    else:
+        m = msa.cantilever(10, 8, 6, lz=0.6)
+        cons = np.asarray(m.constrained, np.uint8).reshape(-1, 3).copy()
+        cons[np.arange(11 * 9), :] = 1            # the z = 0 face
+        dof, n_free = enumerate_dofs(cons)
+        xyz = np.asarray(m.xyz).reshape(-1, 3)
+        f = np.zeros(n_free)
+        tip = np.nonzero(np.isclose(xyz[:, 0], xyz[:, 0].max()))[0]
+        for nd in tip:
+            if dof[3 * nd + 2] >= 0:
+                f[dof[3 * nd + 2]] = -100.0 / tip.size
+        return SimpleNamespace(xyz=m.xyz, conn=m.conn, dof=dof, n_free=n_free,
+                               n_elem=m.n_elem, f=f)
+    return SimpleNamespace(xyz=m.xyz, conn=m.conn, dof=m.dof, n_free=m.n_free,
+                           n_elem=m.n_elem, f=m.f)
+
+
+@pytest.mark.parametrize("kind,dynamics", [("cube", False), ("wide", False), ("clamped2", False),
+                                           ("rows", False), ("cube", True)])
+def test_matrix_core_smoother_matches_valu_form_and_oracle(ctx, kind, dynamics):
+    """One V-cycle of a MIXED batch with the finest (and coarse) box-grid
+    products on the matrix cores against the same cycle with the vector-ALU
+    products (both FP32: bar 1e-6) and, in statics, against the oracle's
+    cycle (bar 1e-6, as the other matrix-free forms); then a mixed solve to
+    |f-Ku|/|f| < 1e-10 on the matrix cores reaches the oracle's direct
+    solution to 1e-8."""
+    print(f"\n[case] matrix-core smoother grid={kind} dynamics={dynamics}", flush=True)
+    model = _box_variant(kind)
+    mesh = msa.Mesh(ctx, model.xyz, model.conn, model.dof, model.n_free)
+    rng = np.random.default_rng(29)
+    E = np.stack([lognormal(rng, model.n_elem) for _ in range(3)])
+    r = rng.standard_normal(model.n_free)
+    z = {}
+    for mc in (True, False):
+        b = msa.Batch(ctx, [mesh], [3], L.MSA_MIXED, 1)
+        b.set_modulus(0, E)
+        b.set_rhs(0, model.f)
+        if dynamics:
+            b.set_dynamics(7.85e-9, 5.0, 1e-5, 0.25, 0.5, 1e-4)
+        b.set_matrix_cores(mc)
+        b.assemble(0.3)
+        b.set_preconditioner(msa.MSA_PRECOND_MULTIGRID)
+        assert b.mg_info(0)["matrix_free"]
+        z[mc] = [b.mg_apply(0, s, r) for s in range(3)]
+        if mc and not dynamics:
+            b.solve(tol=1e-10, max_it=2000, graph_iters=8)
+            st = b.stats(0)
+            assert (st["status"] == msa.MSA_OK).all(), st
+            for s in range(3):
+                ke, bad = O.ke_mesh(model.conn, model.xyz, E[s], 0.3)
+                ri, data = O.skyline_assemble(model.conn, model.dof, model.n_free, ke)
+                fac, rc = O.skyline_factorize(ri, data, 1e-15)
+                assert bad == 0 and rc == 0
+                assert rel(b.solution(0, s), O.skyline_solve(ri, fac, model.f)) < 1e-8
+        b.close()
+    mesh.close()
+    for s in range(3):
+        d = rel(z[True][s], z[False][s])
+        print(f"[matrix cores] sample {s}: cycle vs vector-ALU form {d:.2e}", flush=True)
+        assert d < 1e-6
+    if not dynamics:
+        levels, Ps = MG.hierarchy(np.asarray(model.xyz).reshape(-1, 3), model.conn, model.dof,
+                                  model.n_free, E[1])
+        assert rel(z[True][1], MG.vcycle(levels, Ps, r)) < 1e-6
