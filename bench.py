@@ -255,6 +255,26 @@ def counters(name):
                                                      cj["algorithmic_write"])
 
 
+def back_to_back(roof, b, reps=20):
+    """Adds the SpMV+dot kernel timed over `reps` back-to-back launches with
+    every sample active (no per-launch event overhead) to a roofline entry
+    and makes it the entry's achieved figure; the per-launch HIP-event
+    figure of the solve stays beside it.  Overwrites the batch's PCG
+    iterates."""
+    ms = b.time_spmv(reps)
+    byts = b.spmv_bytes()["kernel"]
+    roof["per_launch_events"] = dict(achieved=roof["achieved"], avg_launch_ms=roof["avg_launch_ms"],
+                                     avg_bytes_per_launch=roof["avg_bytes_per_launch"])
+    roof["achieved"] = byts / (ms / 1e3) / 1e9
+    roof["frac"] = roof["achieved"] / HBM_PEAK_GBS
+    roof["avg_launch_ms"] = ms
+    roof["avg_bytes_per_launch"] = byts
+    roof["timing"] = f"{reps} back-to-back launches, every sample active, HIP events"
+    if roof.get("traffic_over_algorithmic"):
+        roof["traffic"] = byts * roof["traffic_over_algorithmic"]
+    return roof
+
+
 def spmv_roofline(b, kt, rs, name, counter_file):
     """Roofline entry of k_pcg_a from one kernel-timing solve."""
     launches = kt["spmv_dot"]["launches"]
@@ -347,12 +367,14 @@ def leg_c3(msa, ctx, args):
     b.set_kernel_timing(False)
     roof = spmv_roofline(b, kt, 8, "k_pcg_a<double,1> (SpMV + p update + p.q), 256 x 32^3",
                          "k_pcg_a_fp64_counters.json")
+    u = {s: b.solution(0, s) for s in (0, ns - 1)}
+    back_to_back(roof, b)
     # independent check of two samples: the oracle's own CSR operator
     otab = O.kl_table([1.0, 1.0, 1.0], args.corr, args.terms)
     res = {}
     for s in (0, ns - 1):
         E, bad = O.kl_field(model.conn, model.xyz, otab, xi[s], model.domain, 2.1e5, 0.2)
-        res[str(s)] = oracle_true_residual(O, model, E, b.solution(0, s))
+        res[str(s)] = oracle_true_residual(O, model, E, u[s])
     b.close()
     mesh.close()
     out = dict(workload="256 lognormal-KL samples (COV 0.2) of the 32^3 cantilever, FP64 "
@@ -392,6 +414,7 @@ def leg_c2(msa, ctx, args):
     roof = spmv_roofline(b, kt, 8, "k_pcg_a<double,1> (SpMV + p update + p.q), one 64^3 sample",
                          "k_pcg_a_fp64_counters.json")
     u = b.solution(0, 0)
+    back_to_back(roof, b)
     res = oracle_true_residual(O, model, E, u)
     b.close()
     mesh.close()

@@ -274,6 +274,14 @@ int msa_batch_get_kernel_timing(msa_batch *b, double *ms_total,
 /* Algorithmic bytes of one launch of the SpMV+dot kernel over every sample
  * of the batch (values + shared indices + vectors), and its element count. */
 int msa_batch_spmv_bytes(msa_batch *b, double *bytes_per_launch);
+/* Measurement (no counterpart in the reference): times `reps` back-to-back
+ * launches of the batch's SpMV + dot kernel with every sample active (the
+ * state at the start of a PCG solve, which it sets up: the PCG iterates, and
+ * so an FP64 batch's solution, are overwritten) between one pair of events
+ * on the context stream; *ms_per_launch = elapsed / reps.  Free of the
+ * per-launch event overhead of msa_batch_set_kernel_timing, which matters for
+ * launches of about 100 microseconds. */
+int msa_batch_time_spmv(msa_batch *b, int reps, double *ms_per_launch);
 
 /* ----------------------------------------------------------- direct solver
  * Banded Cholesky K = L L^T of every sample's assembled FP64 stiffness on the

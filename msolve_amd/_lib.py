@@ -43,6 +43,7 @@ EXPORTS = [
     "msa_batch_direct_info", "msa_batch_set_preconditioner", "msa_batch_mg_info",
     "msa_batch_mg_apply", "msa_batch_graph_info", "msa_autotune_schedule",
     "msa_autotune_candidates", "msa_batch_set_schedule", "msa_batch_set_matrix_cores",
+    "msa_batch_time_spmv",
 ]
 
 
@@ -138,6 +139,7 @@ def lib():
         "msa_autotune_candidates": [_ip, _ip],
         "msa_batch_set_schedule": [_vp, _i],
         "msa_batch_set_matrix_cores": [_vp, _i],
+        "msa_batch_time_spmv": [_vp, _i, _dp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

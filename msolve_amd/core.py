@@ -323,6 +323,13 @@ class Batch:
         return dict(captures=int(info[0]), kernel_nodes=int(info[1]),
                     iters_per_replay=int(info[2]), replays=int(info[3]))
 
+    def time_spmv(self, reps: int = 20) -> float:
+        """ms per launch of the SpMV + dot kernel over `reps` back-to-back
+        launches with every sample active (overwrites the PCG iterates)."""
+        ms = C.c_double()
+        check(lib().msa_batch_time_spmv(self.h, int(reps), C.byref(ms)))
+        return ms.value
+
     def spmv_bytes(self):
         b = np.zeros(2)
         check(lib().msa_batch_spmv_bytes(self.h, ptr(b, C.c_double)))

@@ -2634,6 +2634,31 @@ int msa_batch_set_refinement(msa_batch *b, int method, int period, double drop)
     return MSA_OK;
 }
 
+int msa_batch_time_spmv(msa_batch *b, int reps, double *ms_per_launch)
+{
+    MSA_CHECK(b && ms_per_launch && reps >= 1, MSA_ERR_ARG, "msa_batch_time_spmv: bad arguments");
+    MSA_CHECK(b->assembled, MSA_ERR_STATE, "batch not assembled");
+    MSA_HIP(hipSetDevice(b->ctx->device));
+    hipStream_t s = b->ctx->stream;
+    const bool mx = is_mixed(b);
+    const int ni = (int)b->items.size();
+    // the start of a PCG solve: every sample active, p = 0, z = D^-1 f
+    MSA_TRY(timed(b, KK_OTHER, [&] { return launch_set_int(s, b->d_counts, 1, 0); }));
+    MSA_TRY(timed(b, KK_OTHER, [&] { return launch_pcg_init(s, mx, b->T, b->d_cls, b->d_items, ni, 0, b->d_counts); }));
+    MSA_TRY(reduce(b, OP_INIT0));
+    // pbuf 0 reads z and p1, writes p0 and q: every launch does the same work
+    MSA_HIP(launch_pcg_a(s, mx, b->T, b->unroll, b->d_cls, b->d_items, ni, 0, b->d_counts));
+    MSA_HIP(hipEventRecord(b->timing.e0, s));
+    for (int r = 0; r < reps; r++)
+        MSA_HIP(launch_pcg_a(s, mx, b->T, b->unroll, b->d_cls, b->d_items, ni, 0, b->d_counts));
+    MSA_HIP(hipEventRecord(b->timing.e1, s));
+    MSA_HIP(hipEventSynchronize(b->timing.e1));
+    float ms = 0;
+    MSA_HIP(hipEventElapsedTime(&ms, b->timing.e0, b->timing.e1));
+    *ms_per_launch = ms / reps;
+    return MSA_OK;
+}
+
 int msa_batch_set_matrix_cores(msa_batch *b, int enable)
 {
     MSA_CHECK(b, MSA_ERR_ARG, "null batch");

@@ -74,3 +74,26 @@ def test_autotuned_schedule_solves_to_oracle(ctx, precond):
             bb.close()
     for m in meshes:
         m.close()
+
+
+def test_back_to_back_spmv_timing(ctx):
+    """msa_batch_time_spmv times the SpMV + dot kernel with every sample
+    active; the batch then solves as before (it only resets the iterates)."""
+    print("\n[case] back-to-back spmv timing", flush=True)
+    model = msa.cantilever(10)
+    mesh = msa.Mesh(ctx, model.xyz, model.conn, model.dof, model.n_free)
+    rng = np.random.default_rng(2)
+    E = lognormal(rng, (2, model.n_elem))
+    b = msa.Batch(ctx, [mesh], [2], L.MSA_FP64, 1)
+    b.set_modulus(0, E)
+    b.set_rhs(0, model.f)
+    b.assemble(0.3)
+    b.solve(tol=1e-12, max_it=5000, graph_iters=8)
+    u0 = [b.solution(0, s) for s in range(2)]
+    ms = b.time_spmv(10)
+    assert 0 < ms < 100
+    b.solve(tol=1e-12, max_it=5000, graph_iters=8)
+    for s in range(2):
+        assert np.array_equal(b.solution(0, s), u0[s])
+    b.close()
+    mesh.close()
