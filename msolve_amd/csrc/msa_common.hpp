@@ -306,7 +306,10 @@ hipError_t launch_assemble_gather(hipStream_t s, bool mixed, int T, const ClassD
                                   const int4 *items, int nitems);
 // one-sample tiles: gather assembly of kAsmGroup samples per work item
 // (class, tile group, chunk)
-constexpr int kAsmGroup = 8;
+#ifndef MSA_ASM_GROUP
+#define MSA_ASM_GROUP 8
+#endif
+constexpr int kAsmGroup = MSA_ASM_GROUP;
 hipError_t launch_assemble_gather_g(hipStream_t s, bool mixed, const ClassDev *d_cls,
                                     const int4 *gitems, int nitems);
 hipError_t launch_residual64(hipStream_t s, int T, const ClassDev *d_cls, const ColourLaunch &cl,
