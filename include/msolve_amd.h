@@ -259,6 +259,30 @@ int msa_batch_get_modulus(msa_batch *b, int cls, int sample, double *E);
  * msa_mesh_info info[2]); MIXED batches return the FP32 values widened. */
 int msa_batch_get_matrix_csr(msa_batch *b, int cls, int sample,
                              long long *rowptr, int *col, double *val);
+/* Sets the stiffness of one sample from a full symmetric CSR in the
+ * reference free-DOF numbering, in place of assembling it: the GPU side of
+ * MonteCarloAnalyzerWithStochasticMaterial.ReadMatricesFromFile
+ * (MonteCarloAnalyzerWithStochasticMaterial.cs:381-402, the
+ * LoadFromDiskAndCalculate mode), whose SkylineMatrix2D.ReadFromFile image
+ * a host converts to CSR.  MSA_FP64 batches without dynamics only (the
+ * mixed path's FP64 residual and the multigrid levels are formed from the
+ * moduli: a batch of loaded matrices solves with the Jacobi or the
+ * first-sample preconditioner, or the direct solver).  Every nonzero entry
+ * must lie in the mesh's sparsity pattern (MSA_ERR_ARG otherwise; zeros
+ * outside it, such as a skyline profile's, are skipped); duplicates add.
+ * The sample's other stored values become zero; the first call on a batch
+ * never assembled zeroes every sample, so every sample must be set before
+ * a solve.  D^-1 is refreshed; the preconditioner follows at the solve. */
+int msa_batch_set_matrix_csr(msa_batch *b, int cls, int sample,
+                             const long long *rowptr, const int *col,
+                             const double *val);
+/* Forms the preconditioner's values from the current stiffness now (the
+ * first-sample band factor, or the multigrid levels and their coarsest
+ * factors) instead of at the next solve: the counterpart of the
+ * factorization in SolverPCG's Initialize
+ * (MonteCarloSolverPCGDirectMatrixCalculator.cs:38-80), so a caller can time
+ * it apart.  No-op when they are current or with the Jacobi preconditioner. */
+int msa_batch_prepare_preconditioner(msa_batch *b);
 /* y = K x for one sample with the batch's SpMV kernel (x, y in the
  * reference numbering, FP64 on the host). */
 int msa_batch_spmv(msa_batch *b, int cls, int sample, const double *x,
@@ -295,6 +319,25 @@ int msa_batch_time_spmv(msa_batch *b, int reps, double *ms_per_launch);
  * sample: n * W doubles (msa_batch_direct_info).  A non-positive pivot gives
  * the sample MSA_BREAKDOWN. */
 int msa_batch_factorize(msa_batch *b);
+/* Zero-energy modes of the factorization (SkylineMatrix2D.Factorize with
+ * its zems / zemCols lists, SkylineMatrix2D.cs:63-158): with tol > 0, a
+ * pivot of magnitude under tol is a zero-energy mode: the pivot becomes 1,
+ * its row and column of the factor are cleared, and the mode (the cleared
+ * row, back-substituted through the factor) is kept, up to max_modes (1..64)
+ * per sample; a sample with more gets MSA_BREAKDOWN.  The solve then holds
+ * the mode's DOF at its load value as the reference's does.  tol = 0
+ * (default) turns it off: a non-positive pivot is MSA_BREAKDOWN.  Takes
+ * effect at the next factorization. */
+int msa_batch_set_zem(msa_batch *b, double tol, int max_modes);
+/* The zero-energy modes of one sample's last factorization: *n_modes found,
+ * cols[min(n, max_modes)] their DOFs (the reference's zemCols) and
+ * modes[min(n, max_modes)][n_free] the vectors (its zems), both in the
+ * reference free-DOF numbering.  The factorization runs in the band order:
+ * the modes span the same null space as the reference's, and equal its
+ * vectors where the band order is the DOF order (the natural order wins
+ * ties, as on box grids). */
+int msa_batch_get_zem(msa_batch *b, int cls, int sample, int max_modes, int *n_modes,
+                      int *cols, double *modes);
 /* u = K^-1 f for every sample from the factors (factorizing first when the
  * assembled values changed); msa_batch_get_stats then reports 0 iterations,
  * |f - K u|/|f| and the status. */

@@ -34,7 +34,9 @@ EXPORTS = [
     "msa_batch_set_modulus", "msa_batch_set_kl", "msa_batch_set_xi", "msa_batch_eval_kl",
     "msa_batch_set_rhs", "msa_batch_assemble", "msa_batch_solve", "msa_batch_run",
     "msa_batch_get_solution", "msa_batch_get_stats", "msa_batch_get_modulus",
-    "msa_batch_get_matrix_csr", "msa_batch_spmv", "msa_batch_set_kernel_timing",
+    "msa_batch_get_matrix_csr", "msa_batch_set_matrix_csr", "msa_batch_prepare_preconditioner",
+    "msa_batch_set_zem", "msa_batch_get_zem",
+    "msa_batch_spmv", "msa_batch_set_kernel_timing",
     "msa_batch_get_kernel_timing", "msa_batch_spmv_bytes", "msa_autotune_tile",
     "msa_batch_set_refinement", "msa_batch_set_integration",
     "msa_batch_set_dynamics", "msa_batch_set_probe", "msa_batch_newmark",
@@ -115,6 +117,10 @@ def lib():
         "msa_batch_get_modulus": [_vp, _i, _i, _dp],
         "msa_batch_element_results": [_vp, _i, _i, _dp, _dp, _dp],
         "msa_batch_get_matrix_csr": [_vp, _i, _i, _llp, _ip, _dp],
+        "msa_batch_set_matrix_csr": [_vp, _i, _i, _llp, _ip, _dp],
+        "msa_batch_prepare_preconditioner": [_vp],
+        "msa_batch_set_zem": [_vp, _d, _i],
+        "msa_batch_get_zem": [_vp, _i, _i, _i, _ip, _ip, _dp],
         "msa_batch_spmv": [_vp, _i, _i, _dp, _dp],
         "msa_batch_set_kernel_timing": [_vp, _i],
         "msa_batch_get_kernel_timing": [_vp, _dp, _llp],

@@ -22,13 +22,16 @@ MsaError instead of the reference's InvalidOperationException.
 """
 from __future__ import annotations
 
+import enum
+import os
 import time
 
 import numpy as np
 
 from . import _lib as L
 from .core import Batch, Context, Mesh, ke_packed
-from .stochastic import standard_normal, read_randoms_file
+from .matrix_io import read_skyline, write_skyline
+from .stochastic import read_randoms_file, standard_normal, write_randoms_file
 
 
 class ElementStructuralStiffnessProvider:
@@ -122,17 +125,84 @@ class SolverSkyline:
         return self.VectorX
 
 
+class StiffnessMatrixProductionMode(enum.IntEnum):
+    """MonteCarloAnalyzerWithStochasticMaterial.cs:18-24."""
+    Normal = 0
+    StoreToDisk = 1
+    LoadFromDiskAndCalculate = 2
+    StoreToDiskAndCalculate = 3
+
+
+def _timespan(seconds: float) -> str:
+    """.NET TimeSpan.ToString(): [d.]hh:mm:ss[.fffffff]."""
+    ticks = int(round(seconds * 1e7))
+    sec, frac = divmod(ticks, 10 ** 7)
+    d, sec = divmod(sec, 86400)
+    h, sec = divmod(sec, 3600)
+    m, sec = divmod(sec, 60)
+    out = (f"{d}." if d else "") + f"{h:02d}:{m:02d}:{sec:02d}"
+    return out + (f".{frac:07d}" if frac else "")
+
+
+def _dotnet_double(x: float) -> str:
+    """.NET Framework Double.ToString(): 15 significant digits, "E+XX" exponent."""
+    t = format(float(x), ".15G")
+    if "E" in t:
+        mant, e = t.split("E")
+        t = f"{mant}E{e[0]}{abs(int(e)):02d}"
+    return t
+
+
 class MonteCarloAnalyzerWithStochasticMaterial:
     """Monte Carlo over lognormal KL modulus samples, all samples solved
-    together.  `models` is one model per mesh class; `samples` the number of
-    samples of each class; `table` the KL table (stochastic.kl_table or
-    read_kl_file); `randomsReadFileName` optionally supplies xi in the
-    reference's file layout."""
+    together (MonteCarloAnalyzerWithStochasticMaterial.cs:26-405).
+
+    `models` is one model per mesh class; `samples` the number of samples of
+    each class; `table` the KL table (stochastic.kl_table or read_kl_file);
+    `randomsReadFileName` optionally supplies xi in the reference's file
+    layout, read from `simulationStartFrom` on.
+
+    Outputs (:232-296), written when `fileNameForLogging` is given (None: no
+    files, the results stay in memory):
+      "<f>-<M>-<start>.txt"        the solution at the output DOF, one
+                                   simulation per line (.NET Double.ToString);
+      "<f>-<M>-<start>-Iters.txt"  PCG iterations per simulation (the
+                                   reference writes an array it never fills:
+                                   zeros; these are the counts);
+      "<f>-<M>-<start>-Times.txt"  Elements / Factorize / Solution / All /
+                                   Total iterations (TimeSpan strings);
+      "randoms<M>.txt"             beside <f>, the generated xi, variable-major
+                                   (Initialize, :186-205), when not read.
+    `outputDOF` is one free-DOF index for every class or one per class; the
+    reference hard-codes node 1, X (:236); None takes each class's last free
+    DOF.  The phase buckets are batch totals: "element" = KL modulus + Ke +
+    assembly, "factorize" = the preconditioner's values (first-sample band
+    factor or multigrid levels; ~0 for Jacobi), "solution" = the PCG.
+
+    `stiffnessMatrixProductionMode` (:18-24, :368-402):
+      StoreToDisk                writes every simulation's K with
+                                 SkylineMatrix2D.WriteToFile as
+                                 "<path>/<name>Sub<c+1>Sim<i><ext>" and solves
+                                 nothing; StoreToDiskAndCalculate also solves;
+      LoadFromDiskAndCalculate   reads "<f>-Order.txt" (one matrix number a
+                                 line), and for simulation i solves the stored
+                                 matrix Order[i] with PCG preconditioned by the
+                                 band factor of matrix Order[b + blockSize/2],
+                                 b = the start of i's block of `blockSize`
+                                 (SolveWithOrder / MakePreconditioner,
+                                 :298-366); one model, an MSA_FP64 batch (the
+                                 stored matrices replace the assembly), times
+                                 in "<f>-Times-<blockSize>-<start>.txt" (all,
+                                 element, factorize, solution).  Stored files
+                                 are read in either layout (the reference
+                                 writes ASCII and reads binary)."""
 
     def __init__(self, ctx: Context, models, samples, table, expansionOrder: int,
                  meanE=2.1e5, cov=0.5, nu=0.3, precision=L.MSA_FP64, tile=0, seed=20261202,
                  randomsReadFileName: str | None = None, simulationStartFrom: int = 0,
-                 preconditioner=L.MSA_PRECOND_JACOBI):
+                 preconditioner=L.MSA_PRECOND_JACOBI, fileNameForLogging: str | None = None,
+                 stiffnessMatrixProductionMode=StiffnessMatrixProductionMode.Normal,
+                 stiffnessMatrixPath: str = "", blockSize: int = 5, outputDOF=None):
         self.ctx = ctx
         self.preconditioner = preconditioner
         self.models = list(models)
@@ -143,10 +213,58 @@ class MonteCarloAnalyzerWithStochasticMaterial:
         self.precision, self.tile, self.seed = precision, tile, seed
         self.randomsReadFileName = randomsReadFileName
         self.simulationStartFrom = simulationStartFrom
+        self.fileNameForLogging = fileNameForLogging
+        self.stiffnessMatrixProductionMode = StiffnessMatrixProductionMode(
+            stiffnessMatrixProductionMode)
+        self.stiffnessMatrixPath = stiffnessMatrixPath
+        self.blockSize = int(blockSize)
+        if len(self.samples) != len(self.models):
+            raise L.MsaError(L.MSA_ERR_ARG, "one sample count per model")
+        if outputDOF is None:
+            self.outputDOF = [m.n_free - 1 for m in self.models]
+        elif np.ndim(outputDOF) == 0:
+            self.outputDOF = [int(outputDOF)] * len(self.models)
+        else:
+            self.outputDOF = [int(d) for d in outputDOF]
+        for d, m in zip(self.outputDOF, self.models):
+            if not 0 <= d < m.n_free:
+                raise L.MsaError(L.MSA_ERR_ARG, f"output DOF {d} outside [0, {m.n_free})")
+        if self.stiffnessMatrixProductionMode == StiffnessMatrixProductionMode.LoadFromDiskAndCalculate:
+            if not fileNameForLogging:
+                raise L.MsaError(L.MSA_ERR_ARG, "LoadFromDiskAndCalculate reads "
+                                                "<fileNameForLogging>-Order.txt")
+            if len(self.models) != 1 or precision != L.MSA_FP64:
+                raise L.MsaError(L.MSA_ERR_ARG, "LoadFromDiskAndCalculate takes one model and "
+                                                "an MSA_FP64 batch")
+            if self.blockSize < 1:
+                raise L.MsaError(L.MSA_ERR_ARG, f"blockSize {blockSize}")
         self.batch = None
         self.randomNumbers = None
+        self.matrixOrder: list[int] = []
         self.times = {}
+        self.values = None
+        self.iterations = None
 
+    # -------------------------------------------------------------- files
+    def _log_name(self, suffix: str) -> str:
+        return (f"{self.fileNameForLogging}-{self.expansionOrder}-{self.simulationStartFrom}"
+                f"{suffix}.txt")
+
+    def _matrix_name(self, cls: int, simulation: int) -> str:
+        """"<path>/<name>Sub<id>Sim<i><ext>" of WriteMatricesToFile (:368-379)."""
+        name = self.stiffnessMatrixPath or "K"
+        path = os.path.dirname(name)
+        base, ext = os.path.splitext(os.path.basename(name))
+        return os.path.join(path, f"{base}Sub{cls + 1}Sim{simulation}{ext}")
+
+    def _write_lines(self, path: str, lines):
+        with open(path, "w", newline="\r\n") as fh:
+            fh.write("".join(f"{l}\n" for l in lines))
+
+    def _write_times(self, path: str, lines):
+        self._write_lines(path, lines)
+
+    # ----------------------------------------------------------- analyzer
     def Initialize(self):
         total = sum(self.samples)
         M = self.expansionOrder
@@ -155,6 +273,12 @@ class MonteCarloAnalyzerWithStochasticMaterial:
             self.randomNumbers = xi[self.simulationStartFrom:self.simulationStartFrom + total]
         else:
             self.randomNumbers = standard_normal(self.seed, total, M)
+            if self.fileNameForLogging:
+                write_randoms_file(os.path.join(os.path.dirname(self.fileNameForLogging),
+                                                f"randoms{M}.txt"), self.randomNumbers)
+        if self.stiffnessMatrixProductionMode == StiffnessMatrixProductionMode.LoadFromDiskAndCalculate:
+            self.BuildMatrices()
+            return
         self.meshes = [Mesh(self.ctx, m.xyz, m.conn, m.dof, m.n_free) for m in self.models]
         self.batch = Batch(self.ctx, self.meshes, self.samples, self.precision, self.tile)
         if self.preconditioner != L.MSA_PRECOND_JACOBI:
@@ -167,15 +291,146 @@ class MonteCarloAnalyzerWithStochasticMaterial:
             off += self.samples[c]
         self.ctx.synchronize()
 
+    def BuildMatrices(self):
+        """LoadFromDiskAndCalculate: reads the matrix order (:128-139)."""
+        if self.stiffnessMatrixProductionMode == StiffnessMatrixProductionMode.LoadFromDiskAndCalculate:
+            with open(f"{self.fileNameForLogging}-Order.txt") as fh:
+                self.matrixOrder = [int(l) for l in fh.read().split()]
+
     def Solve(self, tolerance=1e-10, maxIterations=20000, inner_tol=1e-4, max_outer=20,
               graph_iters=0):
+        mode = self.stiffnessMatrixProductionMode
+        if mode == StiffnessMatrixProductionMode.LoadFromDiskAndCalculate:
+            if not self.matrixOrder:
+                self.Initialize()
+            return self._solve_with_order(tolerance, maxIterations, graph_iters)
         if self.batch is None:
             self.Initialize()
-        t0 = time.perf_counter()
-        self.batch.run(self.nu, tolerance, maxIterations, inner_tol, max_outer, graph_iters)
-        self.ctx.synchronize()
-        self.times["all"] = time.perf_counter() - t0
+        return self._solve_normal(tolerance, maxIterations, inner_tol, max_outer, graph_iters)
+
+    def _solve_normal(self, tolerance, maxIterations, inner_tol, max_outer, graph_iters):
+        """SolveNormal (:232-296) over the whole batch."""
+        mode = self.stiffnessMatrixProductionMode
+        b, ctx = self.batch, self.ctx
+        t_all = time.perf_counter()
+        t = time.perf_counter()
+        b.eval_kl()
+        b.assemble(self.nu)
+        ctx.synchronize()
+        self.times = {"element": time.perf_counter() - t, "factorize": 0.0, "solution": 0.0}
+        if mode in (StiffnessMatrixProductionMode.StoreToDisk,
+                    StiffnessMatrixProductionMode.StoreToDiskAndCalculate):
+            off = self.simulationStartFrom
+            for c in range(len(self.models)):
+                for s in range(self.samples[c]):
+                    write_skyline(self._matrix_name(c, off + s), *b.matrix_skyline(c, s))
+                off += self.samples[c]
+        if mode == StiffnessMatrixProductionMode.StoreToDisk:
+            self.times["all"] = time.perf_counter() - t_all
+            return self
+        t = time.perf_counter()
+        b.prepare_preconditioner()
+        ctx.synchronize()
+        self.times["factorize"] = time.perf_counter() - t
+        t = time.perf_counter()
+        b.solve(tolerance, maxIterations, inner_tol, max_outer, graph_iters)
+        ctx.synchronize()
+        self.times["solution"] = time.perf_counter() - t
+        self.times["all"] = time.perf_counter() - t_all
+        values, iters = [], []
+        for c in range(len(self.models)):
+            st = b.stats(c)
+            iters.extend(int(i) for i in st["iters"])
+            values.extend(float(b.solution(c, s)[self.outputDOF[c]])
+                          for s in range(self.samples[c]))
+        self.values, self.iterations = np.array(values), np.array(iters)
+        if self.fileNameForLogging:
+            self._write_lines(self._log_name(""), (_dotnet_double(v) for v in values))
+            self._write_lines(self._log_name("-Iters"), iters)
+            self._write_times(self._log_name("-Times"), [
+                f"Elements: {_timespan(self.times['element'])}",
+                f"Factorize: {_timespan(self.times['factorize'])}",
+                f"Solution: {_timespan(self.times['solution'])}",
+                f"All: {_timespan(self.times['all'])}",
+                f"Total iterations: {int(np.sum(iters))}"])
         return self
+
+    def _solve_with_order(self, tolerance, maxIterations, graph_iters):
+        """SolveWithOrder + MakePreconditioner (:298-366): one class per block
+        of `blockSize` simulations; sample 0 of a class is the block's
+        preconditioner matrix, whose band factor (MSA_PRECOND_FIRST_SAMPLE)
+        preconditions every sample of the class."""
+        bs, start, nsim = self.blockSize, self.simulationStartFrom, self.samples[0]
+        order = self.matrixOrder
+        model = self.models[0]
+        if start + nsim > len(order):
+            raise L.MsaError(L.MSA_ERR_ARG, f"{len(order)} matrices in the order file for "
+                                            f"simulations {start}..{start + nsim - 1}")
+        blocks = []   # (matrix numbers with the preconditioner's first, simulation per sample)
+        for kb in range(start // bs, (start + nsim - 1) // bs + 1):
+            pidx = kb * bs + bs // 2
+            if pidx >= len(order):
+                raise L.MsaError(L.MSA_ERR_ARG, f"block at {kb * bs}: preconditioner matrix "
+                                                f"index {pidx} beyond the order file")
+            sims = [i for i in range(max(kb * bs, start), min(kb * bs + bs, start + nsim))]
+            if pidx in sims:
+                sims.remove(pidx)
+                blocks.append(([order[pidx]] + [order[i] for i in sims], [pidx] + sims))
+            else:   # a partial block: the preconditioner matrix is solved too, not reported
+                blocks.append(([order[pidx]] + [order[i] for i in sims], [None] + sims))
+        t_all = time.perf_counter()
+        self.meshes = [Mesh(self.ctx, model.xyz, model.conn, model.dof, model.n_free)]
+        b = Batch(self.ctx, self.meshes * len(blocks), [len(mb) for mb, _ in blocks],
+                  L.MSA_FP64, self.tile)
+        self.batch = b
+        b.set_preconditioner(L.MSA_PRECOND_FIRST_SAMPLE)
+        t = time.perf_counter()
+        for c, (mats, _) in enumerate(blocks):
+            b.set_rhs(c, model.f)
+            for s, mat in enumerate(mats):
+                b.set_matrix_skyline(c, s, *read_skyline(self._matrix_name(0, mat), None))
+        self.ctx.synchronize()
+        self.times = {"element": time.perf_counter() - t}
+        t = time.perf_counter()
+        b.prepare_preconditioner()
+        self.times["factorize"] = time.perf_counter() - t
+        t = time.perf_counter()
+        b.solve(tolerance, maxIterations, graph_iters=graph_iters)
+        self.ctx.synchronize()
+        self.times["solution"] = time.perf_counter() - t
+        self.times["all"] = time.perf_counter() - t_all
+        values = np.zeros(nsim)
+        iters = np.zeros(nsim, np.int64)
+        for c, (_, sims) in enumerate(blocks):
+            st = b.stats(c)
+            for s, i in enumerate(sims):
+                if i is None:
+                    continue
+                values[i - start] = b.solution(c, s)[self.outputDOF[0]]
+                iters[i - start] = st["iters"][s]
+        self.values, self.iterations = values, iters
+        self._blocks = blocks
+        if self.fileNameForLogging:
+            self._write_lines(self._log_name(""), (_dotnet_double(v) for v in values))
+            self._write_times(f"{self.fileNameForLogging}-Times-{bs}-{start}.txt",
+                              [_timespan(self.times[k])
+                               for k in ("all", "element", "factorize", "solution")])
+        return self
+
+    def SimulationSolution(self, simulation: int):
+        """The full solution of one simulation (index from simulationStartFrom)."""
+        i = simulation
+        if self.stiffnessMatrixProductionMode == StiffnessMatrixProductionMode.LoadFromDiskAndCalculate:
+            for c, (_, sims) in enumerate(self._blocks):
+                if i in sims:
+                    return self.batch.solution(c, sims.index(i))
+            raise L.MsaError(L.MSA_ERR_ARG, f"simulation {i} not solved")
+        off = self.simulationStartFrom
+        for c, n in enumerate(self.samples):
+            if off <= i < off + n:
+                return self.batch.solution(c, i - off)
+            off += n
+        raise L.MsaError(L.MSA_ERR_ARG, f"simulation {i} not solved")
 
     def Solution(self, cls: int, sample: int):
         return self.batch.solution(cls, sample)

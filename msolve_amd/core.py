@@ -209,6 +209,30 @@ class Batch:
         rp, col, val = self.matrix_csr(cls, sample)
         return csr_to_skyline(rp, col, val, self.meshes[cls].n_free)
 
+    def set_matrix_csr(self, cls: int, sample: int, rowptr, col, val):
+        """Sets one sample's K from full symmetric CSR in the free-DOF
+        numbering instead of assembling it (MSA_FP64 batches; every entry in
+        the mesh's sparsity pattern)."""
+        rp = np.ascontiguousarray(rowptr, np.int64)
+        cc = np.ascontiguousarray(col, np.int32)
+        vv = f64(val)
+        n = self.meshes[cls].n_free
+        if rp.size != n + 1 or cc.size < rp[-1] or vv.size < rp[-1]:
+            raise L.MsaError(L.MSA_ERR_ARG, f"CSR of {rp.size - 1} rows / {cc.size} entries "
+                                          f"for {n} free DOFs")
+        check(lib().msa_batch_set_matrix_csr(self.h, cls, sample, ptr(rp, C.c_longlong),
+                                             ptr(cc, C.c_int), ptr(vv, C.c_double)))
+
+    def set_matrix_skyline(self, cls: int, sample: int, rowIndex, data):
+        """Sets one sample's K from SkylineMatrix2D arrays (as read by
+        matrix_io.read_skyline)."""
+        from .matrix_io import skyline_to_csr
+        self.set_matrix_csr(cls, sample, *skyline_to_csr(rowIndex, data, drop_zeros=True))
+
+    def prepare_preconditioner(self):
+        """Forms the preconditioner's values now instead of at the next solve."""
+        check(lib().msa_batch_prepare_preconditioner(self.h))
+
     def spmv(self, cls: int, sample: int, x):
         x = f64(x)
         y = np.zeros(self.meshes[cls].n_free)
@@ -225,6 +249,21 @@ class Batch:
     def factorize(self):
         """Band Cholesky of every sample's assembled FP64 K (SkylineMatrix2D.Factorize)."""
         check(lib().msa_batch_factorize(self.h))
+
+    def set_zem(self, tol: float, max_modes: int = 16):
+        """Zero-energy-mode extraction in the next factorization (tol 0: off)."""
+        check(lib().msa_batch_set_zem(self.h, tol, max_modes))
+
+    def zem(self, cls: int, sample: int, max_modes: int = 64):
+        """(cols, modes[k][n_free]) of one sample's last factorization."""
+        n = self.meshes[cls].n_free
+        cnt = C.c_int(0)
+        cols = np.zeros(max_modes, np.int32)
+        modes = np.zeros((max_modes, n))
+        check(lib().msa_batch_get_zem(self.h, cls, sample, max_modes, C.byref(cnt),
+                                      ptr(cols, C.c_int), ptr(modes, C.c_double)))
+        k = min(cnt.value, max_modes)
+        return cols[:k].copy(), modes[:k].copy()
 
     def direct_solve(self):
         check(lib().msa_batch_direct_solve(self.h))

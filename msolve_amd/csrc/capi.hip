@@ -45,6 +45,12 @@ struct ClassHost {
     std::vector<int> bperm, binv;
     long long bw = -1;       // half bandwidth, -1: order not built
     size_t band_cap = 0;     // doubles allocated for the factors
+    // zero-energy modes of the direct solver (msa_batch_set_zem): buffers
+    // sized for zem_alloc modes per sample
+    double *zv = nullptr;
+    int *zcol = nullptr, *zcount = nullptr;
+    unsigned long long *zmask = nullptr;
+    int zem_alloc = 0;
     int nsamp = 0, ntiles = 0, nslices = 0, nchunks = 0, chunk = 0;
     std::vector<int> slice_ptr, cidx;
     long long tile_vals = 0, tile_vec = 0;
@@ -79,7 +85,10 @@ struct msa_batch {
     std::vector<void *> allocs;
     long long bytes = 0;
     bool assembled = false;
+    bool loaded = false;      // values set by msa_batch_set_matrix_csr, not assembled
     bool factored = false;    // band Cholesky factors match the assembled values
+    double zem_tol = 0.0;     // zero-energy-mode pivot tolerance (0: off)
+    int zem_max = 0;          // modes kept per sample
     int solver = MSA_SOLVER_PCG;
     double nu = 0.3;
     // mixed-precision refinement: restarted inner solves or periodic
@@ -681,6 +690,8 @@ int prepare_band(msa_batch *b)
     return MSA_OK;
 }
 
+int band_backward(msa_batch *b, const ClassDev *cp, int n, int W, int ns, int mode);
+
 // Forward and backward sweeps with the band factor of every sample (in
 // place on by): one launch for bands of at most kBandOneMax rows, else two
 // launch pairs per 64-column block.  mode >= 0: coarsest multigrid level,
@@ -694,10 +705,85 @@ int band_sweeps(msa_batch *b, const ClassDev *cp, int n, int W, int ns, int mode
         const int nk = std::min(kBandNB, n - k0), r0 = k0 + nk, m = std::min(n - r0, W - kBandNB);
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_band_fwd(s, cp, ns, k0, nk, r0, m, mode); }));
     }
+    return band_backward(b, cp, n, W, ns, mode);
+}
+
+// The backward sweep L^T x = y alone (in place on by), block by block.
+int band_backward(msa_batch *b, const ClassDev *cp, int n, int W, int ns, int mode)
+{
+    hipStream_t s = b->ctx->stream;
     const int last = ((n - 1) / kBandNB) * kBandNB;
     for (int k0 = last; k0 >= 0; k0 -= kBandNB) {
         const int nk = std::min(kBandNB, n - k0), r0 = k0 + nk, m = std::min(n - r0, W - kBandNB);
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_band_bwd(s, cp, ns, k0, nk, r0, m, mode); }));
+    }
+    return MSA_OK;
+}
+
+// Zero-energy-mode buffers and switches of one class for the next
+// factorization (on when b->zem_tol > 0); *changed when its ClassDev changed.
+int zem_setup(msa_batch *b, ClassHost &C, bool *changed)
+{
+    hipStream_t s = b->ctx->stream;
+    ClassDev &D = C.dev;
+    if (b->zem_tol <= 0.0) {
+        if (D.zv) {
+            D.zv = nullptr;
+            D.zcol = D.zcount = nullptr;
+            D.zmask = nullptr;
+            *changed = true;
+        }
+        return MSA_OK;
+    }
+    const int nblk = (D.bn + kBandNB - 1) / kBandNB;
+    if (C.zem_alloc < b->zem_max) {
+        void *p = nullptr;
+        MSA_TRY(dalloc(b, &p, (size_t)C.nsamp * b->zem_max * D.bn * sizeof(double)));
+        C.zv = (double *)p;
+        MSA_TRY(dalloc(b, &p, (size_t)C.nsamp * b->zem_max * sizeof(int)));
+        C.zcol = (int *)p;
+        MSA_TRY(dalloc(b, &p, (size_t)C.nsamp * sizeof(int)));
+        C.zcount = (int *)p;
+        MSA_TRY(dalloc(b, &p, (size_t)C.nsamp * nblk * sizeof(unsigned long long)));
+        C.zmask = (unsigned long long *)p;
+        C.zem_alloc = b->zem_max;
+    }
+    if (D.zv != C.zv || D.zem_tol != b->zem_tol || D.zem_max != b->zem_max) {
+        D.zv = C.zv;
+        D.zcol = C.zcol;
+        D.zcount = C.zcount;
+        D.zmask = C.zmask;
+        D.zem_tol = b->zem_tol;
+        D.zem_max = b->zem_max;
+        *changed = true;
+    }
+    MSA_HIP(hipMemsetAsync(C.zv, 0, (size_t)C.nsamp * b->zem_max * D.bn * sizeof(double), s));
+    MSA_HIP(hipMemsetAsync(C.zcol, 0xff, (size_t)C.nsamp * b->zem_max * sizeof(int), s));
+    MSA_HIP(hipMemsetAsync(C.zcount, 0, (size_t)C.nsamp * sizeof(int), s));
+    MSA_HIP(hipMemsetAsync(C.zmask, 0, (size_t)C.nsamp * nblk * sizeof(unsigned long long), s));
+    return MSA_OK;
+}
+
+// The modes of every sample of class c after its factorization: mode k of a
+// sample is L^-T of its saved row (SkylineMatrix2D.cs:138-157), by the
+// backward sweep on by (free after a factorization), mode by mode.
+int zem_modes(msa_batch *b, size_t c)
+{
+    hipStream_t s = b->ctx->stream;
+    ClassHost &C = b->cls[c];
+    const int n = C.dev.bn, ns = C.nsamp, zmax = b->zem_max;
+    std::vector<int> cnt(ns);
+    MSA_HIP(hipMemcpyAsync(cnt.data(), C.zcount, ns * sizeof(int), hipMemcpyDeviceToHost, s));
+    MSA_HIP(hipStreamSynchronize(s));
+    int most = 0;
+    for (int v : cnt) most = std::max(most, std::min(v, zmax));
+    const size_t row = (size_t)n * sizeof(double);
+    for (int k = 0; k < most; k++) {
+        MSA_HIP(hipMemcpy2DAsync(C.dev.by, row, C.zv + (size_t)k * n, zmax * row, row, ns,
+                                 hipMemcpyDeviceToDevice, s));
+        MSA_TRY(band_backward(b, b->d_cls + c, n, C.dev.bW, ns, -1));
+        MSA_HIP(hipMemcpy2DAsync(C.zv + (size_t)k * n, zmax * row, C.dev.by, row, row, ns,
+                                 hipMemcpyDeviceToDevice, s));
     }
     return MSA_OK;
 }
@@ -1149,6 +1235,12 @@ int first_values(msa_batch *b)
     bool upload = false;
     for (auto &C : b->cls) {
         MSA_TRY(band_setup(b, C, &upload, true));
+        if (C.dev.zv) {  // no zero-energy modes in the shared preconditioner factor
+            C.dev.zv = nullptr;
+            C.dev.zcol = C.dev.zcount = nullptr;
+            C.dev.zmask = nullptr;
+            upload = true;
+        }
         if (C.dev.mg_st != C.dev.st) {
             C.dev.mg_st = C.dev.st;  // the sweeps select samples by the cycle's mode
             upload = true;
@@ -1190,6 +1282,10 @@ int first_values(msa_batch *b)
 // Values of the chosen preconditioner from the assembled classes.
 int precond_values(msa_batch *b)
 {
+    MSA_CHECK(!(b->loaded && b->precond == MSA_PRECOND_MULTIGRID && b->mg_depth > 0),
+              MSA_ERR_STATE,
+              "the multigrid levels are built from the moduli: a batch of loaded matrices takes "
+              "the Jacobi or first-sample preconditioner");
     if (b->precond != MSA_PRECOND_FIRST_SAMPLE) return mg_values(b);
     b->mg_ready = false;
     MSA_TRY(first_values(b));
@@ -1852,9 +1948,19 @@ int msa_batch_assemble(msa_batch *b, double nu)
     }));
     b->nu = nu;
     b->assembled = true;
+    b->loaded = false;
     b->factored = false;
-    b->mg_ready = false;
-    if (b->precond != MSA_PRECOND_JACOBI) MSA_TRY(precond_values(b));
+    b->mg_ready = false;  // preconditioner values follow at the solve or msa_batch_prepare_preconditioner
+    return MSA_OK;
+}
+
+int msa_batch_prepare_preconditioner(msa_batch *b)
+{
+    MSA_CHECK(b, MSA_ERR_ARG, "null batch");
+    MSA_CHECK(b->assembled, MSA_ERR_STATE, "batch not assembled");
+    MSA_HIP(hipSetDevice(b->ctx->device));
+    if (!b->mg_ready) MSA_TRY(precond_values(b));
+    MSA_HIP(hipStreamSynchronize(b->ctx->stream));
     return MSA_OK;
 }
 
@@ -2260,6 +2366,11 @@ int msa_batch_factorize(msa_batch *b)
     MSA_HIP(hipSetDevice(b->ctx->device));
     hipStream_t s = b->ctx->stream;
     MSA_TRY(prepare_band(b));
+    {
+        bool upload = false;
+        for (auto &C : b->cls) MSA_TRY(zem_setup(b, C, &upload));
+        if (upload) MSA_TRY(upload_class_table(b));
+    }
     if (b->precond == MSA_PRECOND_FIRST_SAMPLE) b->mg_ready = false;  // its shared factor is overwritten
     for (auto &C : b->cls) {
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_fill_f64(s, C.dev.band, C.band_cap, 0.0); }));
@@ -2279,9 +2390,59 @@ int msa_batch_factorize(msa_batch *b)
             MSA_TRY(timed(b, KK_ASSEMBLE, [&] { return launch_chol_panel(s, cp, ns, k0, nk, r0, m); }));
             MSA_TRY(timed(b, KK_ASSEMBLE, [&] { return launch_chol_update(s, cp, ns, k0, nk, r0, m); }));
         }
+        if (b->zem_tol > 0.0) MSA_TRY(zem_modes(b, c));
     }
     MSA_HIP(hipStreamSynchronize(s));
     b->factored = true;
+    return MSA_OK;
+}
+
+int msa_batch_set_zem(msa_batch *b, double tol, int max_modes)
+{
+    MSA_CHECK(b, MSA_ERR_ARG, "null batch");
+    MSA_CHECK(tol >= 0.0 && std::isfinite(tol), MSA_ERR_ARG, "zero-energy tolerance %g", tol);
+    MSA_CHECK(tol == 0.0 || (max_modes >= 1 && max_modes <= 64), MSA_ERR_ARG,
+              "max_modes=%d outside [1, 64]", max_modes);
+    b->zem_tol = tol;
+    b->zem_max = tol > 0.0 ? max_modes : 0;
+    b->factored = false;
+    return MSA_OK;
+}
+
+int msa_batch_get_zem(msa_batch *b, int cls, int sample, int max_modes, int *n_modes, int *cols,
+                      double *modes)
+{
+    MSA_CHECK(b && n_modes && cls >= 0 && cls < (int)b->cls.size(), MSA_ERR_ARG, "bad arguments");
+    ClassHost &C = b->cls[cls];
+    MSA_CHECK(sample >= 0 && sample < C.nsamp, MSA_ERR_ARG, "sample %d out of range", sample);
+    MSA_CHECK(b->factored && b->zem_tol > 0.0 && C.zcount && C.dev.zv == C.zv, MSA_ERR_STATE,
+              "no factorization with zero-energy modes (msa_batch_set_zem, msa_batch_factorize)");
+    MSA_HIP(hipSetDevice(b->ctx->device));
+    hipStream_t s = b->ctx->stream;
+    int cnt = 0;
+    MSA_HIP(hipMemcpyAsync(&cnt, C.zcount + sample, sizeof(int), hipMemcpyDeviceToHost, s));
+    MSA_HIP(hipStreamSynchronize(s));
+    *n_modes = cnt;
+    MSA_CHECK(cnt <= b->zem_max, MSA_ERR_STATE,
+              "sample %d: %d zero-energy modes, more than the %d kept", sample, cnt, b->zem_max);
+    const int n = C.dev.bn, k = std::min(cnt, max_modes);
+    if (k <= 0) return MSA_OK;
+    std::vector<double> h((size_t)k * n);
+    std::vector<int> hc(k);
+    MSA_HIP(hipMemcpyAsync(h.data(), C.zv + (size_t)sample * b->zem_max * n, h.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, s));
+    MSA_HIP(hipMemcpyAsync(hc.data(), C.zcol + (size_t)sample * b->zem_max, k * sizeof(int),
+                           hipMemcpyDeviceToHost, s));
+    MSA_HIP(hipStreamSynchronize(s));
+    // band index -> reference free DOF: internal row bperm[i], ref2int^-1
+    const MeshHost &m = C.mesh->h;
+    std::vector<int> int2ref(3 * (size_t)m.nb, -1);
+    for (int g = 0; g < m.n_free; g++) int2ref[m.ref2int[g]] = g;
+    for (int j = 0; j < k; j++) {
+        if (cols) cols[j] = int2ref[C.bperm[hc[j]]];
+        if (modes)
+            for (int i = 0; i < n; i++) modes[(size_t)j * m.n_free + int2ref[C.bperm[i]]] = h[(size_t)j * n + i];
+    }
     return MSA_OK;
 }
 
@@ -2574,6 +2735,86 @@ int msa_batch_get_matrix_csr(msa_batch *b, int cls, int sample, long long *rowpt
         }
         rowptr[g + 1] = k;
     }
+    return MSA_OK;
+}
+
+int msa_batch_set_matrix_csr(msa_batch *b, int cls, int sample, const long long *rowptr,
+                             const int *col, const double *val)
+{
+    MSA_CHECK(b && rowptr && col && val && cls >= 0 && cls < (int)b->cls.size(), MSA_ERR_ARG,
+              "bad arguments");
+    ClassHost &C = b->cls[cls];
+    MSA_CHECK(sample >= 0 && sample < C.nsamp, MSA_ERR_ARG, "sample %d out of range", sample);
+    MSA_CHECK(!is_mixed(b), MSA_ERR_STATE,
+              "a loaded matrix needs an MSA_FP64 batch (the mixed path's FP64 residual is "
+              "formed from the moduli)");
+    MSA_CHECK(!b->dyn, MSA_ERR_STATE, "a loaded matrix is a static stiffness: dynamics is set");
+    const MeshHost &m = C.mesh->h;
+    const int nf = m.n_free;
+    MSA_CHECK(rowptr[0] == 0, MSA_ERR_ARG, "rowptr[0] = %lld", rowptr[0]);
+    for (int g = 0; g < nf; g++)
+        MSA_CHECK(rowptr[g + 1] >= rowptr[g], MSA_ERR_ARG, "rowptr decreases at row %d", g);
+    MSA_HIP(hipSetDevice(b->ctx->device));
+    hipStream_t s = b->ctx->stream;
+    const int tile = sample / b->T, lane = sample % b->T, R = b->R;
+    const size_t n = (size_t)C.slice_ptr[C.nslices] * 9 * R;
+    std::vector<double> hv(n, 0.0);
+    std::vector<std::pair<int, int>> rows(nf);
+    for (int nd = 0; nd < m.n_nodes; nd++)
+        for (int p = 0; p < 3; p++)
+            if (m.dof[3 * nd + p] >= 0) rows[m.dof[3 * nd + p]] = {nd, p};
+    // position of every column of a row in the stored image (the same
+    // addressing as msa_batch_get_matrix_csr); -1 outside the row's blocks
+    std::vector<long long> where(nf, -1);
+    for (int g = 0; g < nf; g++) {
+        const int nd = rows[g].first, p = rows[g].second;
+        const int br = m.node_block[nd], sl = br / R, rr = br % R;
+        const int nbj = m.brow_ptr[br + 1] - m.brow_ptr[br];
+        for (int j = 0; j < nbj; j++) {
+            const int mc = m.block_node[m.bcol[m.brow_ptr[br] + j]];
+            const size_t slot = (size_t)C.slice_ptr[sl] + j;
+            for (int q = 0; q < 3; q++) {
+                const int gc = m.dof[3 * mc + q];
+                if (gc >= 0)
+                    where[gc] = b->T > 1 ? (long long)((slot * 9 + 3 * p + q) * R + rr)
+                                         : (long long)val_at<double, 1>(slot, 3 * p + q, rr, 0);
+            }
+        }
+        for (long long k = rowptr[g]; k < rowptr[g + 1]; k++) {
+            const int c = col[k];
+            MSA_CHECK(c >= 0 && c < nf, MSA_ERR_ARG, "row %d: column %d outside [0, %d)", g, c, nf);
+            MSA_CHECK(std::isfinite(val[k]), MSA_ERR_ARG, "entry (%d, %d) is not finite", g, c);
+            if (where[c] < 0) {  // zeros of a profile may lie outside the pattern
+                MSA_CHECK(val[k] == 0.0, MSA_ERR_ARG,
+                          "entry (%d, %d) = %g lies outside the mesh's sparsity pattern", g, c,
+                          val[k]);
+                continue;
+            }
+            hv[(size_t)where[c]] += val[k];
+        }
+        for (int j = 0; j < nbj; j++) {
+            const int mc = m.block_node[m.bcol[m.brow_ptr[br] + j]];
+            for (int q = 0; q < 3; q++)
+                if (m.dof[3 * mc + q] >= 0) where[m.dof[3 * mc + q]] = -1;
+        }
+    }
+    if (!b->assembled && !b->loaded)  // first values of a new batch: unset samples hold zeros
+        for (auto &Ck : b->cls)
+            MSA_HIP(hipMemsetAsync(Ck.dev.val, 0, (size_t)Ck.ntiles * Ck.tile_vals * sizeof(double), s));
+    double *d = nullptr;
+    MSA_HIP(hipMalloc(&d, std::max<size_t>(n, 1) * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(d, hv.data(), n * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = launch_insert_sample(s, b->T, d, n, lane, (double *)C.dev.val + (size_t)tile * C.tile_vals);
+    if (e == hipSuccess)
+        e = launch_dinv(s, false, b->T, b->d_cls, b->d_items, (int)b->items.size());
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(d);
+    MSA_HIP(e);
+    b->assembled = true;
+    b->loaded = true;
+    b->factored = false;
+    b->mg_ready = false;
     return MSA_OK;
 }
 

@@ -2005,29 +2005,73 @@ __global__ __launch_bounds__(256) void k_band_scatter(const ClassDev *__restrict
 
 // In-place Cholesky of the nk x nk diagonal block at column k0; one
 // workgroup of 64 threads (thread = block row) per sample.
+//
+// Zero-energy modes (C.zv set; SkylineMatrix2D.Factorize's kFix branch,
+// SkylineMatrix2D.cs:110-130): a pivot d = K_nn - sum_k L_nk^2 with |d| <
+// zem_tol is the LDL^T pivot of a singular direction.  The factor then takes
+// L_nn = 1 and clears row n (saved first: w_n = 1, w_k = -L_nk; in LDL^T terms
+// -L'_nk sqrt(D_k), which is what the reference's back-substitution through
+// D^1/2 L'^T amounts to) and column n below the diagonal (the reference
+// zeroes K's column n before the later rows use it, so their L entries in
+// column n come out zero).  Row n's entries left of this block were used
+// only for updates of row and column n, both discarded, so clearing them
+// afterwards gives the reference's factor.  The panel kernel clears column
+// n of its rows from zmask.
 __global__ __launch_bounds__(64) void k_chol_diag(const ClassDev *__restrict__ cp, int k0, int nk)
 {
     __shared__ double A[kBandNB][kBandNB + 1];
-    __shared__ int fail;
+    __shared__ int fail, zem, zk;
+    __shared__ unsigned long long zbits;
     const ClassDev &C = *cp;
     const int smp = blockIdx.y;
     const int t = threadIdx.x;
     double *AB = C.band + (size_t)smp * C.band_ss;
-    if (t == 0) fail = 0;
+    const bool zon = C.zv != nullptr;
+    if (t == 0) {
+        fail = 0;
+        zbits = 0ull;
+    }
     if (t < nk)
         for (int c = 0; c <= t; c++) A[t][c] = AB[(size_t)(k0 + c) * C.bW + (t - c)];
     __syncthreads();
     for (int c = 0; c < nk; c++) {
         if (t == c) {
             double d = A[c][c];
-            if (!(d > 0.0) || !isfinite(d)) {
+            zem = 0;
+            if (zon && isfinite(d) && fabs(d) < C.zem_tol) {
+                zem = 1;
+                zk = C.zcount[smp]++;
+                if (zk >= C.zem_max) fail = 1;  // more modes than kept
+                zbits |= 1ull << c;
+                d = 1.0;
+            } else if (!(d > 0.0) || !isfinite(d)) {
                 fail = 1;
                 d = 1.0;
             }
             A[c][c] = sqrt(d);
         }
         __syncthreads();
-        if (t > c && t < nk) A[t][c] /= A[c][c];
+        if (zem) {
+            const int n = k0 + c;
+            double *w = zk < C.zem_max ? C.zv + ((size_t)smp * C.zem_max + zk) * C.bn : nullptr;
+            if (w && t == 0) {
+                w[n] = 1.0;
+                C.zcol[(size_t)smp * C.zem_max + zk] = n;
+            }
+            // row n left of this block: columns k in (n - bW, k0)
+            for (int k = max(0, n - C.bW + 1) + t; k < k0; k += 64) {
+                double *e = AB + (size_t)k * C.bW + (n - k);
+                if (w) w[k] = -*e;
+                *e = 0.0;
+            }
+            if (t < c) {
+                if (w) w[k0 + t] = -A[c][t];
+                A[c][t] = 0.0;
+            }
+            if (t > c && t < nk) A[t][c] = 0.0;
+        } else if (t > c && t < nk) {
+            A[t][c] /= A[c][c];
+        }
         __syncthreads();
         if (t > c && t < nk)
             for (int c2 = c + 1; c2 <= t; c2++) A[t][c2] -= A[t][c] * A[c2][c];
@@ -2036,6 +2080,7 @@ __global__ __launch_bounds__(64) void k_chol_diag(const ClassDev *__restrict__ c
     if (t < nk)
         for (int c = 0; c <= t; c++) AB[(size_t)(k0 + c) * C.bW + (t - c)] = A[t][c];
     if (t == 0 && fail) C.bfail[smp] = 1;
+    if (t == 0 && zon) C.zmask[(size_t)smp * ((C.bn + kBandNB - 1) / kBandNB) + k0 / kBandNB] = zbits;
 }
 
 // Panel rows i in [r0, r0 + m): L21 = A21 L11^-T, 64 rows per workgroup of
@@ -2063,9 +2108,12 @@ __global__ __launch_bounds__(256) void k_chol_panel(const ClassDev *__restrict__
         const int i = row0 + r;
         P[r][c] = (c < nk && i < r0 + m) ? AB[(size_t)(k0 + c) * C.bW + (i - k0 - c)] : 0.0;
     }
+    const unsigned long long zbits =
+        C.zv ? C.zmask[(size_t)smp * ((C.bn + kBandNB - 1) / kBandNB) + k0 / kBandNB] : 0ull;
     __syncthreads();
     for (int c = 0; c < nk; c++) {
-        if (tid < kBandNB) P[tid][c] /= Lp[c * (c + 1) / 2 + c];
+        // a zero-energy column of L is zero below the diagonal
+        if (tid < kBandNB) P[tid][c] = ((zbits >> c) & 1ull) ? 0.0 : P[tid][c] / Lp[c * (c + 1) / 2 + c];
         __syncthreads();
         const int rest = kBandNB * (nk - c - 1);
         for (int idx = tid; idx < rest; idx += 256) {

@@ -185,6 +185,16 @@ struct ClassDev {
     double *by;           // [nsamp][bn] right-hand side / solution in band order
     double *bpart;        // [nsamp][kBandChunks][64] partial sums of the backward solve
     int *bfail;           // [nsamp] non-positive pivot met
+    // zero-energy modes of the direct solver's factorization (zv null: off):
+    // a pivot of magnitude under zem_tol is set to 1 and its row and column
+    // of L cleared; zv[s][k] holds the k-th mode in band order (first the
+    // cleared row, then L^-T of it), zcol[s][k] its band column
+    double zem_tol;
+    int zem_max;          // modes kept per sample (more: bfail)
+    double *zv;           // [nsamp][zem_max][bn]
+    int *zcol;            // [nsamp][zem_max]
+    int *zcount;          // [nsamp] modes found
+    unsigned long long *zmask;  // [nsamp][ceil(bn/64)] mode columns of each 64-column block
     double *binvd;        // coarsest multigrid level: [nsamp][bn][bn] dense inverse, band order
     // geometric multigrid preconditioner (msa_batch_set_preconditioner).
     // Every class of a hierarchy is an entry of the class table; mg_* are

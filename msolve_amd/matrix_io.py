@@ -45,6 +45,31 @@ def csr_to_skyline(rowptr, col, val, n: int):
     return rowIndex, data
 
 
+def skyline_to_csr(rowIndex, data, drop_zeros: bool = False):
+    """SkylineMatrix2D arrays -> full symmetric CSR (rowptr int64[n+1],
+    col int32, val float64), columns ascending per row.  Stored zeros of the
+    profile stay entries unless drop_zeros (the diagonal always stays)."""
+    rowIndex = np.asarray(rowIndex, np.int64)
+    data = np.asarray(data, np.float64)
+    n = rowIndex.size - 1
+    h = np.diff(rowIndex)                      # entries of column j: K[j-k, j], k < h_j
+    cols = np.repeat(np.arange(n, dtype=np.int64), h)
+    k = np.arange(int(rowIndex[n]), dtype=np.int64) - np.repeat(rowIndex[:-1], h)
+    if drop_zeros:
+        keep = (data != 0) | (k == 0)
+        cols, k, data = cols[keep], k[keep], data[keep]
+    rows = cols - k
+    off = k > 0                                # mirror the strictly upper entries
+    r = np.concatenate([rows, cols[off]])
+    c = np.concatenate([cols, rows[off]])
+    v = np.concatenate([data, data[off]])
+    order = np.lexsort((c, r))
+    r, c, v = r[order], c[order], v[order]
+    rowptr = np.zeros(n + 1, np.int64)
+    np.cumsum(np.bincount(r, minlength=n), out=rowptr[1:])
+    return rowptr, c.astype(np.int32), v
+
+
 def skyline_to_dense(rowIndex, data):
     n = len(rowIndex) - 1
     K = np.zeros((n, n))
@@ -91,10 +116,25 @@ def write_skyline(name: str, rowIndex, data, binary: bool = False):
     return ix_name, data_name
 
 
-def read_skyline(name: str, binary: bool = True):
-    """ReadFromFile's binary layout (default) or the ASCII one WriteToFile
-    produces; returns (rowIndex int64, data float64)."""
+def _is_binary(path: str) -> bool:
+    """ReadFromFile's layout: an int32 count, then exactly that many items."""
+    size = os.path.getsize(path)
+    if size < 4:
+        return False
+    with open(path, "rb") as fh:
+        (m,) = struct.unpack("<i", fh.read(4))
+    return m >= 0 and size in (4 + 4 * m, 4 + 8 * m)
+
+
+def read_skyline(name: str, binary: bool | None = True):
+    """ReadFromFile's binary layout (default), the ASCII one WriteToFile
+    produces, or (binary=None) whichever the files hold; returns (rowIndex
+    int64, data float64).  The reference writes ASCII (WriteToFile) and
+    reads binary (ReadFromFile), so its own stored matrices do not read
+    back; None accepts both."""
     ix_name, data_name = _names(name)
+    if binary is None:
+        binary = _is_binary(ix_name)
     if binary:
         with open(ix_name, "rb") as fh:
             (m,) = struct.unpack("<i", fh.read(4))

@@ -527,13 +527,16 @@ void orc_skyline_multiply(int n, const int64_t *rowIndex, const double *d,
     }
 }
 
-/* SkylineMatrix2D.Factorize (SkylineMatrix2D.cs:63-136): in-place active-
+/* SkylineMatrix2D.Factorize (SkylineMatrix2D.cs:63-158): in-place active-
  * column LDL^T.  A pivot under `tol` used as a divisor raises (returns -1);
  * a pivot under `tol` on the diagonal being formed is a zero-energy mode:
- * it is set to 1 and its column/row cleared as the reference does (the
- * zero-energy vectors themselves are not formed).  Returns the number of
- * zero-energy modes found. */
-int orc_skyline_factorize(int n, const int64_t *rowIndex, double *d, double tol)
+ * it is set to 1, its column/row cleared, and (for the first `max_modes`)
+ * its column index kept in cols[] and its vector in zems[k*n ..]: 1 at the
+ * column, minus the cleared row of L before it (:114-124), then
+ * back-substituted through L^T after the factorization (:138-157).
+ * Returns the number of zero-energy modes found. */
+int orc_skyline_factorize_zem(int n, const int64_t *rowIndex, double *d, double tol,
+                              int max_modes, int *cols, double *zems)
 {
     int kfix = 0;
     const int64_t len = rowIndex[n];
@@ -573,7 +576,18 @@ int orc_skyline_factorize(int n, const int64_t *rowIndex, double *d, double tol)
         d[KN] -= B;
         if (fabs(d[KN]) < tol) {
             d[KN] = 1;
-            for (int64_t i1 = KN + 1; i1 <= KU; i1++) d[i1] = 0;
+            double *z = kfix < max_modes ? zems + (size_t)kfix * n : NULL;
+            if (z) {
+                memset(z, 0, sizeof(double) * (size_t)n);
+                z[nn] = 1;
+                cols[kfix] = nn;
+            }
+            int j1 = nn;
+            for (int64_t i1 = KN + 1; i1 <= KU; i1++) {
+                j1--;
+                if (z) z[j1] = -d[i1];
+                d[i1] = 0;
+            }
             for (int irest = nn + 1; irest < n; irest++) {
                 int64_t m1 = rowIndex[irest] + irest - nn;
                 if (m1 <= rowIndex[irest + 1] && m1 < len) d[m1] = 0;
@@ -581,7 +595,24 @@ int orc_skyline_factorize(int n, const int64_t *rowIndex, double *d, double tol)
             kfix++;
         }
     }
+    const int kept = kfix < max_modes ? kfix : max_modes;
+    for (int f = 0; f < kept && n > 1; f++) {
+        double *z = zems + (size_t)f * n;
+        for (int nn = n - 1; nn >= 1; nn--) {
+            int64_t KL = rowIndex[nn] + 1, KU = rowIndex[nn + 1] - 1;
+            int k = nn;
+            for (int64_t KK = KL; KK <= KU; KK++) {
+                k--;
+                z[k] -= d[KK] * z[nn];
+            }
+        }
+    }
     return kfix;
+}
+
+int orc_skyline_factorize(int n, const int64_t *rowIndex, double *d, double tol)
+{
+    return orc_skyline_factorize_zem(n, rowIndex, d, tol, 0, NULL, NULL);
 }
 
 /* SkylineMatrix2D.Solve (SkylineMatrix2D.cs:161-212): forward reduction,

@@ -57,6 +57,8 @@ void orc_skyline_multiply(int n, const int64_t *rowIndex, const double *data,
                           const double *x, double *y);
 int orc_skyline_factorize(int n, const int64_t *rowIndex, double *data,
                           double tol);
+int orc_skyline_factorize_zem(int n, const int64_t *rowIndex, double *data,
+                              double tol, int max_modes, int *cols, double *zems);
 void orc_skyline_solve(int n, const int64_t *rowIndex, const double *data,
                        const double *f, double *x);
 

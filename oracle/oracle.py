@@ -68,6 +68,7 @@ def lib():
         "orc_skyline_assemble": (V, [I, _i32p, _i32p, _f64p, _i64p, _f64p]),
         "orc_skyline_multiply": (V, [I, _i64p, _f64p, _f64p, _f64p]),
         "orc_skyline_factorize": (I, [I, _i64p, _f64p, D]),
+        "orc_skyline_factorize_zem": (I, [I, _i64p, _f64p, D, I, _i32p, _f64p]),
         "orc_skyline_solve": (V, [I, _i64p, _f64p, _f64p, _f64p]),
         "orc_csr_count": (C.c_int64, [I, _i32p, _i32p, I, _i64p]),
         "orc_csr_fill": (V, [I, _i32p, _i32p, I, _i64p, _i32p]),
@@ -260,6 +261,17 @@ def skyline_factorize(ri, data, tol=1e-15):
     d = data.copy()
     rc = lib().orc_skyline_factorize(ri.size - 1, ri, d, tol)
     return d, rc
+
+
+def skyline_factorize_zem(ri, data, tol, max_modes=16):
+    """Factorize with zero-energy modes: (factor, n_modes, cols, zems[k][n])."""
+    n = ri.size - 1
+    d = data.copy()
+    cols = np.zeros(max_modes, np.int32)
+    zems = np.zeros((max_modes, n))
+    k = lib().orc_skyline_factorize_zem(n, ri, d, tol, max_modes, cols, zems)
+    kept = min(max(k, 0), max_modes)
+    return d, k, cols[:kept].copy(), zems[:kept].copy()
 
 
 def skyline_solve(ri, factored, f):
