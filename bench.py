@@ -341,6 +341,23 @@ def oracle_true_residual(O, model, E, u, nu=0.3):
 
 
 # ------------------------------------------------------------------ legs
+def persistent_run(b, ctx, kw, ref):
+    """The same solve as one cooperative persistent launch: wall time of one
+    run (after one warm-up) and whether it gives the graph path's bits."""
+    b.set_persistent(True)
+    b.run(**kw)
+    ctx.synchronize()
+    t = time.perf_counter()
+    b.run(**kw)
+    ctx.synchronize()
+    el = time.perf_counter() - t
+    same = all(np.array_equal(b.solution(0, s), u) for s, u in ref.items())
+    info = b.persistent_info()
+    b.set_persistent(False)
+    return dict(s_per_solve=el, same_bits_as_graph=bool(same), workgroups=info["workgroups"],
+                launches=info["launches"])
+
+
 def leg_c3(msa, ctx, args):
     """256 samples of the 32^3 cantilever, lognormal KL modulus (COV 0.2),
     FP64 element integration, assembly and Jacobi-PCG to 1e-10, one batch."""
@@ -368,6 +385,7 @@ def leg_c3(msa, ctx, args):
     roof = spmv_roofline(b, kt, 8, "k_pcg_a<double,1> (SpMV + p update + p.q), 256 x 32^3",
                          "k_pcg_a_fp64_counters.json")
     u = {s: b.solution(0, s) for s in (0, ns - 1)}
+    pers = persistent_run(b, ctx, kw, u)
     back_to_back(roof, b)
     # independent check of two samples: the oracle's own CSR operator
     otab = O.kl_table([1.0, 1.0, 1.0], args.corr, args.terms)
@@ -382,10 +400,11 @@ def leg_c3(msa, ctx, args):
                solves_per_s=ns / el, s_per_batch=el, iters_mean=float(st["iters"].mean()),
                iters_max=int(st["iters"].max()),
                all_ok=bool((st["status"] == msa.MSA_OK).all()),
-               oracle_true_residual=res, roofline=roof,
+               oracle_true_residual=res, roofline=roof, persistent_launch=pers,
                kernel_timing_ms={k: v["ms"] for k, v in kt.items()})
     log(f"[bench] C3 256x32^3 FP64: {el:.2f} s -> {ns / el:.1f} solves/s, "
-        f"k_pcg_a<double> {roof['achieved']:.0f} GB/s, residuals {res}")
+        f"k_pcg_a<double> {roof['achieved']:.0f} GB/s, residuals {res}; persistent launch "
+        f"{pers['s_per_solve']:.2f} s (same bits: {pers['same_bits_as_graph']})")
     return out
 
 
@@ -414,6 +433,7 @@ def leg_c2(msa, ctx, args):
     roof = spmv_roofline(b, kt, 8, "k_pcg_a<double,1> (SpMV + p update + p.q), one 64^3 sample",
                          "k_pcg_a_fp64_counters.json")
     u = b.solution(0, 0)
+    pers = persistent_run(b, ctx, kw, {0: u})
     back_to_back(roof, b)
     res = oracle_true_residual(O, model, E, u)
     b.close()
@@ -421,10 +441,11 @@ def leg_c2(msa, ctx, args):
     out = dict(workload="64^3 cantilever (811200 DOFs), E 2.1e5, FP64 Jacobi-PCG to 1e-10",
                s_per_solve=el, iters=int(st["iters"][0]), status=int(st["status"][0]),
                ms_per_iteration=1e3 * el / max(int(st["iters"][0]), 1),
-               oracle_true_residual=res, roofline=roof,
+               oracle_true_residual=res, roofline=roof, persistent_launch=pers,
                kernel_timing_ms={k: v["ms"] for k, v in kt.items()})
     log(f"[bench] C2 64^3 FP64: {el:.3f} s, {int(st['iters'][0])} its, k_pcg_a<double> "
-        f"{roof['achieved']:.0f} GB/s, oracle residual {res:.2e}")
+        f"{roof['achieved']:.0f} GB/s, oracle residual {res:.2e}; persistent launch "
+        f"{pers['s_per_solve']:.3f} s (same bits: {pers['same_bits_as_graph']})")
     return out
 
 

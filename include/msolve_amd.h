@@ -462,6 +462,22 @@ int msa_batch_get_dynamic_state(msa_batch *b, int cls, int sample, double *u,
 int msa_autotune_tile(msa_ctx *ctx, msa_mesh *mesh, int n_samples,
                       int precision, int *best_tile, double *ms_per_tile);
 
+/* Persistent launch (the batched-PCG launch of the solve path; no
+ * counterpart in the reference, whose SolverPCG.Solve loops on the host,
+ * SolverPCG.cs:105-149): with enable, a Jacobi-preconditioned PCG solve
+ * (MSA_FP64 batches, and the inner solves of MSA_REFINE_RESTART) runs as
+ * ONE cooperative launch of co-resident workgroups that loops over the
+ * iterations itself: the four phases of an iteration (search direction +
+ * product, alpha, update, test + beta) are separated by grid-wide barriers,
+ * every workgroup leaves when no sample is active.  The per-item work and
+ * the summation order are those of the graph-captured iteration, so the
+ * results are the same bits.  One-sample tiles only (MSA_ERR_STATE
+ * otherwise); other preconditioners and the kernel-timing mode keep the
+ * captured iteration blocks.  info[0] persistent launches so far, info[1]
+ * workgroups of the last one. */
+int msa_batch_set_persistent(msa_batch *b, int enable);
+int msa_batch_persistent_info(msa_batch *b, long long *info);
+
 /* Whole-solve schedule autotuner (no counterpart in the reference; the batch
  * schedule of the product replacing SkylineMatrix2D.Multiply inside
  * SolverPCG.Solve, SolverPCG.cs:105-149).  For each candidate (lane tile T,

@@ -362,6 +362,15 @@ class Batch:
         return dict(captures=int(info[0]), kernel_nodes=int(info[1]),
                     iters_per_replay=int(info[2]), replays=int(info[3]))
 
+    def set_persistent(self, enable: bool):
+        """Jacobi-PCG solves as one cooperative launch (one-sample tiles)."""
+        check(lib().msa_batch_set_persistent(self.h, 1 if enable else 0))
+
+    def persistent_info(self):
+        info = np.zeros(2, np.int64)
+        check(lib().msa_batch_persistent_info(self.h, ptr(info, C.c_longlong)))
+        return dict(launches=int(info[0]), workgroups=int(info[1]))
+
     def time_spmv(self, reps: int = 20) -> float:
         """ms per launch of the SpMV + dot kernel over `reps` back-to-back
         launches with every sample active (overwrites the PCG iterates)."""

@@ -88,6 +88,9 @@ struct msa_batch {
     bool loaded = false;      // values set by msa_batch_set_matrix_csr, not assembled
     bool factored = false;    // band Cholesky factors match the assembled values
     double zem_tol = 0.0;     // zero-energy-mode pivot tolerance (0: off)
+    bool persistent = false;  // Jacobi PCG as one cooperative launch (msa_batch_set_persistent)
+    long long persist_launches = 0;
+    int persist_grid = 0;
     int zem_max = 0;          // modes kept per sample
     int solver = MSA_SOLVER_PCG;
     double nu = 0.3;
@@ -539,6 +542,27 @@ int build_graph(msa_batch *b, int kind, int iters, double tol, int max_it)
 // refining) reaches zero.  kind 1 interleaves residual replacements.
 int pcg_loop(msa_batch *b, int kind, double tol, int max_it, int graph_iters)
 {
+    if (b->persistent && kind == 0 && b->precond == MSA_PRECOND_JACOBI && !b->timing.on) {
+        MSA_CHECK(b->T == 1, MSA_ERR_STATE,
+                  "the persistent launch takes one-sample tiles (the batch has lane tile %d)", b->T);
+        ReduceArgs ra{}, rb{};
+        ra.op = OP_A;
+        rb.op = OP_B;
+        rb.tol = tol;
+        rb.max_it = max_it;
+        ra.active_count = rb.active_count = b->d_counts;
+        ra.ir_count = rb.ir_count = b->d_counts + 1;
+        int G = 0;
+        MSA_HIP(launch_pcg_persist(b->ctx->stream, is_mixed(b), b->unroll, b->d_cls, b->d_items,
+                                   (int)b->items.size(), b->d_tiles, (int)b->tiles.size(), ra, rb,
+                                   max_it + 1, &G));
+        b->persist_launches++;
+        b->persist_grid = G;
+        int active = 0;
+        MSA_TRY(read_count(b, 0, &active));
+        MSA_CHECK(active <= 0, MSA_ERR_STATE, "persistent PCG: %d samples still active", active);
+        return MSA_OK;
+    }
     const bool use_graph = graph_iters > 0 && !b->timing.on;
     int block = graph_iters > 0 ? graph_iters : 8;
     if (kind == 1) block = ((block + b->period - 1) / b->period) * b->period;
@@ -2394,6 +2418,23 @@ int msa_batch_factorize(msa_batch *b)
     }
     MSA_HIP(hipStreamSynchronize(s));
     b->factored = true;
+    return MSA_OK;
+}
+
+int msa_batch_set_persistent(msa_batch *b, int enable)
+{
+    MSA_CHECK(b, MSA_ERR_ARG, "null batch");
+    MSA_CHECK(!enable || b->T == 1, MSA_ERR_STATE,
+              "the persistent launch takes one-sample tiles (the batch has lane tile %d)", b->T);
+    b->persistent = enable != 0;
+    return MSA_OK;
+}
+
+int msa_batch_persistent_info(msa_batch *b, long long *info)
+{
+    MSA_CHECK(b && info, MSA_ERR_ARG, "bad arguments");
+    info[0] = b->persist_launches;
+    info[1] = b->persist_grid;
     return MSA_OK;
 }
 

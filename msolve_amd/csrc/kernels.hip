@@ -12,6 +12,8 @@
 #include <algorithm>
 #include <cmath>
 
+#include <hip/hip_cooperative_groups.h>
+
 #include "msa_common.hpp"
 
 namespace msa {
@@ -980,14 +982,10 @@ __global__ __launch_bounds__(256) void k_pcg_init(const ClassDev *__restrict__ c
 // U: slots per group of a row (the loads of a group issue together; the
 // schedule the autotuner picks); every U sums the slots in the same order.
 template <typename real, int T, int U = 3>
-__global__ __launch_bounds__(256) void k_pcg_a(const ClassDev *__restrict__ cls,
-                                               const int4 *__restrict__ items, int nitems,
-                                               int pbuf, int *active_count)
+__device__ __forceinline__ void pcg_a_item(const ClassDev *__restrict__ cls, const int4 it,
+                                           int pbuf)
 {
     constexpr int R = 64 / T;
-    const int item = xcd_item(nitems);
-    if (item < 0) return;
-    const int4 it = items[item];
     const ClassDev &C = cls[it.x];
     const int tile = it.y, chunk = it.z;
     const int lane = threadIdx.x & 63;
@@ -1094,19 +1092,24 @@ __global__ __launch_bounds__(256) void k_pcg_a(const ClassDev *__restrict__ cls,
     block_partials<T, 1>(acc, C.partial + ((size_t)tile * C.nchunks + chunk) * 3 * T);
 }
 
+template <typename real, int T, int U = 3>
+__global__ __launch_bounds__(256) void k_pcg_a(const ClassDev *__restrict__ cls,
+                                               const int4 *__restrict__ items, int nitems,
+                                               int pbuf, int *active_count)
+{
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    pcg_a_item<real, T, U>(cls, items[item], pbuf);
+}
+
 // Update + preconditioner + stopping test (SolverPCG.cs:122-136):
 // x += alpha p, r -= alpha q, z = D^-1 r, rz = r.z, rr = r.r; the last
 // workgroup of each tile checks |r|/|b| < tol and forms beta.
 template <typename real, int T>
-__global__ __launch_bounds__(256) void k_pcg_b(const ClassDev *__restrict__ cls,
-                                               const int4 *__restrict__ items, int nitems,
-                                               int pbuf, double tol, int max_it,
-                                               int *active_count)
+__device__ __forceinline__ void pcg_b_item(const ClassDev *__restrict__ cls, const int4 it,
+                                           int pbuf)
 {
     constexpr int R = 64 / T;
-    const int item = xcd_item(nitems);
-    if (item < 0) return;
-    const int4 it = items[item];
     const ClassDev &C = cls[it.x];
     const int tile = it.y, chunk = it.z;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1158,6 +1161,17 @@ __global__ __launch_bounds__(256) void k_pcg_b(const ClassDev *__restrict__ cls,
     }
     double acc[2] = {(double)rz, (double)rr2};
     block_partials<T, 2>(acc, C.partial + ((size_t)tile * C.nchunks + chunk) * 3 * T);
+}
+
+template <typename real, int T>
+__global__ __launch_bounds__(256) void k_pcg_b(const ClassDev *__restrict__ cls,
+                                               const int4 *__restrict__ items, int nitems,
+                                               int pbuf, double tol, int max_it,
+                                               int *active_count)
+{
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    pcg_b_item<real, T>(cls, items[item], pbuf);
 }
 
 // y = K x for every lane of one tile (tests and the single-product entry).
@@ -2656,44 +2670,56 @@ __device__ void apply_op(const ClassDev &C, int smp, const double *v, const Redu
     }
 }
 
-template <int T>
-__global__ __launch_bounds__(1024) void k_reduce(const ClassDev *__restrict__ cls,
-                                                 const int2 *__restrict__ tiles, ReduceArgs a)
+// The partial sums of one tile (chunk by chunk) reduced and applied.  The
+// summation is that of 1024 threads (thread v sums chunks v/T, +2G, ... and
+// +G, ..., then a tree over G = 1024/T); NTHR threads play 1024/NTHR of them
+// each, in any order (no step reads what it writes), so every NTHR gives the
+// same bits.  red: [2][1024] in LDS.
+template <int T, int NTHR>
+__device__ __forceinline__ void reduce_tile(const ClassDev *__restrict__ cls, const int2 ct,
+                                            const ReduceArgs &a, double (*red)[1024])
 {
-    constexpr int G = 1024 / T;
-    __shared__ double red[2][1024];
-    const int2 ct = tiles[blockIdx.x];
+    constexpr int G = 1024 / T, VPT = 1024 / NTHR;
     const ClassDev &C = cls[ct.x];
     const int tile = ct.y;
-    const int t = threadIdx.x % T, g = threadIdx.x / T;
     const bool two = a.op == OP_INIT0 || a.op == OP_INIT1 || a.op == OP_INIT2 || a.op == OP_B ||
                      a.op == OP_DIRECT;
-    const double *base = C.partial + (size_t)tile * C.nchunks * 3 * T + t;
-    double a0 = 0, a1 = 0, b0 = 0, b1 = 0;
-    int c = g;
-    for (; c + G < C.nchunks; c += 2 * G) {
-        a0 += base[(size_t)c * 3 * T];
-        b0 += base[(size_t)(c + G) * 3 * T];
-        if (two) {
-            a1 += base[(size_t)c * 3 * T + T];
-            b1 += base[(size_t)(c + G) * 3 * T + T];
+#pragma unroll
+    for (int q = 0; q < VPT; q++) {
+        const int v = threadIdx.x + NTHR * q;
+        const int t = v % T, g = v / T;
+        const double *base = C.partial + (size_t)tile * C.nchunks * 3 * T + t;
+        double a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+        int c = g;
+        for (; c + G < C.nchunks; c += 2 * G) {
+            a0 += base[(size_t)c * 3 * T];
+            b0 += base[(size_t)(c + G) * 3 * T];
+            if (two) {
+                a1 += base[(size_t)c * 3 * T + T];
+                b1 += base[(size_t)(c + G) * 3 * T + T];
+            }
         }
+        if (c < C.nchunks) {
+            a0 += base[(size_t)c * 3 * T];
+            if (two) a1 += base[(size_t)c * 3 * T + T];
+        }
+        red[0][v] = a0 + b0;
+        red[1][v] = a1 + b1;
     }
-    if (c < C.nchunks) {
-        a0 += base[(size_t)c * 3 * T];
-        if (two) a1 += base[(size_t)c * 3 * T + T];
-    }
-    red[0][threadIdx.x] = a0 + b0;
-    red[1][threadIdx.x] = a1 + b1;
     __syncthreads();
     for (int stride = G / 2; stride >= 1; stride >>= 1) {
-        if (g < stride) {
-            red[0][threadIdx.x] += red[0][threadIdx.x + stride * T];
-            red[1][threadIdx.x] += red[1][threadIdx.x + stride * T];
+#pragma unroll
+        for (int q = 0; q < VPT; q++) {
+            const int v = threadIdx.x + NTHR * q;
+            if (v / T < stride) {
+                red[0][v] += red[0][v + stride * T];
+                red[1][v] += red[1][v + stride * T];
+            }
         }
         __syncthreads();
     }
-    if (g == 0) {
+    if (threadIdx.x < T) {
+        const int t = threadIdx.x;
         const int smp = tile * T + t;
         const double v[2] = {red[0][t], red[1][t]};
         if (smp < C.nsamp) apply_op(C, smp, v, a);
@@ -2702,6 +2728,67 @@ __global__ __launch_bounds__(1024) void k_reduce(const ClassDev *__restrict__ cl
     if ((a.op == OP_INIT0 || a.op == OP_INIT1 || a.op == OP_INIT2 || a.op == OP_RURESTART) &&
         tile == 0 && threadIdx.x == 0)
         *C.due = 0;
+}
+
+template <int T>
+__global__ __launch_bounds__(1024) void k_reduce(const ClassDev *__restrict__ cls,
+                                                 const int2 *__restrict__ tiles, ReduceArgs a)
+{
+    __shared__ double red[2][1024];
+    reduce_tile<T, 1024>(cls, tiles[blockIdx.x], a, red);
+}
+
+// The whole Jacobi-PCG of a batch in one cooperative launch: every
+// iteration's four phases (search direction + product, alpha, update, test
+// and beta) separated by grid-wide barriers, the same per-item and per-tile
+// work and summation order as the four kernels of the captured iteration
+// (so the same bits).  Work items go to XCDs in contiguous ranges, as
+// xcd_item does.  Every workgroup reads the active-sample count after the
+// same barrier, so all leave the loop together; `iters` bounds it.
+template <typename real, int T, int U>
+__global__ __launch_bounds__(256) void k_pcg_persist(const ClassDev *__restrict__ cls,
+                                                     const int4 *__restrict__ items, int nitems,
+                                                     const int2 *__restrict__ tiles, int ntiles,
+                                                     ReduceArgs ra, ReduceArgs rb, int pbuf0,
+                                                     int iters)
+{
+    __shared__ double red[2][1024];
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    const int G = gridDim.x, gx = G >> 3, xcd = blockIdx.x & 7, m = blockIdx.x >> 3;
+    const int per = (nitems + 7) >> 3;
+    const int i0 = xcd * per + m, i1 = min((xcd + 1) * per, nitems);
+    auto barrier = [&] {
+        __threadfence();
+        grid.sync();
+        __threadfence();
+    };
+    for (int k = 0; k < iters; k++) {
+        // the same value in every workgroup (read between the same two barriers)
+        if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(rb.active_count, __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT)) <= 0)
+            break;
+        const int pbuf = (pbuf0 + k) & 1;
+        for (int i = i0; i < i1; i += gx) {
+            pcg_a_item<real, T, U>(cls, items[i], pbuf);
+            __syncthreads();
+        }
+        barrier();
+        for (int t = blockIdx.x; t < ntiles; t += G) {
+            reduce_tile<T, 256>(cls, tiles[t], ra, red);
+            __syncthreads();
+        }
+        barrier();
+        for (int i = i0; i < i1; i += gx) {
+            pcg_b_item<real, T>(cls, items[i], pbuf);
+            __syncthreads();
+        }
+        barrier();
+        for (int t = blockIdx.x; t < ntiles; t += G) {
+            reduce_tile<T, 256>(cls, tiles[t], rb, red);
+            __syncthreads();
+        }
+        barrier();
+    }
 }
 
 // ------------------------------------------------------------ multigrid
@@ -4394,6 +4481,52 @@ hipError_t launch_pcg_a(hipStream_t s, bool mixed, int T, int U, const ClassDev 
     }
     if (U != 3) return hipErrorInvalidValue;  // lane tiles T > 1 use groups of 3 slots
     MSA_PCG_LAUNCH(k_pcg_a, d_cls, items, nitems, pbuf, active_count);
+}
+
+template <typename real, int U>
+static hipError_t pcg_persist_u(hipStream_t s, const ClassDev *d_cls, const int4 *items, int nitems,
+                                const int2 *tiles, int ntiles, const ReduceArgs &ra,
+                                const ReduceArgs &rb, int iters, int *grid_out)
+{
+    const void *fn = (const void *)k_pcg_persist<real, 1, U>;
+    int dev = 0, ncu = 0, coop = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0);
+    if (e != hipSuccess) return e;
+    if (!coop || per_cu <= 0) return hipErrorNotSupported;
+    // co-resident workgroups, a multiple of the 8 XCDs, no more than the items need
+    int G = (per_cu * ncu) & ~7;
+    G = std::min(G, ((nitems + 7) / 8) * 8);
+    if (G < 8) return hipErrorInvalidConfiguration;
+    *grid_out = G;
+    int pbuf0 = 0;
+    ReduceArgs a = ra, b = rb;
+    void *args[] = {(void *)&d_cls, (void *)&items, (void *)&nitems, (void *)&tiles, (void *)&ntiles,
+                    (void *)&a,     (void *)&b,     (void *)&pbuf0,  (void *)&iters};
+    return hipLaunchCooperativeKernel(fn, dim3(G), dim3(256), args, 0, s);
+}
+
+hipError_t launch_pcg_persist(hipStream_t s, bool mixed, int U, const ClassDev *d_cls,
+                              const int4 *items, int nitems, const int2 *tiles, int ntiles,
+                              const ReduceArgs &ra, const ReduceArgs &rb, int iters, int *grid_out)
+{
+    if (nitems <= 0) return hipSuccess;
+#define MSA_PERSIST_U(real)                                                                   \
+    switch (U) {                                                                             \
+    case 1: return pcg_persist_u<real, 1>(s, d_cls, items, nitems, tiles, ntiles, ra, rb, iters, grid_out); \
+    case 2: return pcg_persist_u<real, 2>(s, d_cls, items, nitems, tiles, ntiles, ra, rb, iters, grid_out); \
+    case 3: return pcg_persist_u<real, 3>(s, d_cls, items, nitems, tiles, ntiles, ra, rb, iters, grid_out); \
+    case 4: return pcg_persist_u<real, 4>(s, d_cls, items, nitems, tiles, ntiles, ra, rb, iters, grid_out); \
+    default: return hipErrorInvalidValue;                                                    \
+    }
+    if (mixed) {
+        MSA_PERSIST_U(float)
+    } else {
+        MSA_PERSIST_U(double)
+    }
+#undef MSA_PERSIST_U
 }
 
 hipError_t launch_pcg_b(hipStream_t s, bool mixed, int T, const ClassDev *d_cls, const int4 *items,

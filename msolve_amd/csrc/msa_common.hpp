@@ -438,6 +438,11 @@ hipError_t launch_kl_eval(hipStream_t s, int T, const ClassDev *d_cls, int ntile
                           int M);
 hipError_t launch_extract_sample(hipStream_t s, bool mixed, int T, const void *src, size_t n_rows,
                                  int lane, double *dst);
+// The whole Jacobi-PCG (one-sample tiles) in one cooperative launch of at
+// most `iters` iterations; *grid_out = workgroups launched.
+hipError_t launch_pcg_persist(hipStream_t s, bool mixed, int U, const ClassDev *d_cls,
+                              const int4 *items, int nitems, const int2 *tiles, int ntiles,
+                              const ReduceArgs &ra, const ReduceArgs &rb, int iters, int *grid_out);
 hipError_t launch_insert_sample(hipStream_t s, int T, const double *src, size_t n_rows, int lane,
                                 double *dst);
 hipError_t launch_extract_values(hipStream_t s, bool mixed, int T, const void *val, size_t nslots9R,
