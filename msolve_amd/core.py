@@ -128,7 +128,10 @@ class Batch:
 
     def set_modulus(self, cls: int, E):
         E = f64(E)
-        assert E.size == self.n_samples[cls] * self.meshes[cls].n_elem
+        if E.size != self.n_samples[cls] * self.meshes[cls].n_elem:
+            raise L.MsaError(L.MSA_ERR_ARG, f"class {cls}: {E.size} moduli for "
+                                            f"{self.n_samples[cls]} samples x "
+                                            f"{self.meshes[cls].n_elem} elements")
         check(lib().msa_batch_set_modulus(self.h, cls, ptr(E, C.c_double)))
 
     def set_kl(self, cls: int, table, domain, meanE: float, cov: float):
@@ -150,7 +153,9 @@ class Batch:
 
     def set_rhs(self, cls: int, f):
         f = f64(f)
-        assert f.size == self.meshes[cls].n_free
+        if f.size != self.meshes[cls].n_free:
+            raise L.MsaError(L.MSA_ERR_ARG, f"class {cls}: load vector of {f.size} entries "
+                                            f"for {self.meshes[cls].n_free} free DOFs")
         check(lib().msa_batch_set_rhs(self.h, cls, ptr(f, C.c_double)))
 
     def assemble(self, nu: float = 0.3):

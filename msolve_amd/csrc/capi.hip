@@ -1735,6 +1735,7 @@ int msa_batch_create(msa_ctx *ctx, int n_classes, msa_mesh *const *meshes, const
     for (int c = 0; c < n_classes; c++) {
         MSA_CHECK(meshes[c] && meshes[c]->ctx == ctx, MSA_ERR_ARG, "mesh %d invalid", c);
         MSA_CHECK(n_samples[c] >= 1, MSA_ERR_ARG, "n_samples[%d]=%d", c, n_samples[c]);
+        MSA_CHECK(meshes[c]->h.n_free >= 1, MSA_ERR_ARG, "mesh %d has no free DOF to solve for", c);
     }
     if (tile == 0) tile = 1;  // one sample per lane tile: no lane waits for a slower sample
     MSA_CHECK(tile == 1 || tile == 2 || tile == 4 || tile == 8 || tile == 16 || tile == 32 ||
