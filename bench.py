@@ -86,6 +86,9 @@ def parse():
     p.add_argument("--matrix-cores", type=int, default=0,
                    help="1: box-grid smoothing products on the matrix cores, 0: on the vector ALU "
                         "(the other form is timed beside the headline)")
+    p.add_argument("--lanczos-steps", type=int, default=None,
+                   help="Lanczos steps of the multigrid smoothing interval (0: Gershgorin bound; "
+                        "default: the library's)")
     p.add_argument("--precond", choices=["mg", "jacobi"], default="mg",
                    help="PCG preconditioner: multigrid V-cycle or Jacobi")
     p.add_argument("--no-compare", action="store_true",
@@ -141,6 +144,8 @@ def make_batch(msa, ctx, meshes, models, counts, precision, tile, table, xi, arg
     precond = precond or args.precond
     if precond == "mg":
         b.set_preconditioner(msa.MSA_PRECOND_MULTIGRID)
+        if args.lanczos_steps is not None:
+            b.set_smoothing_interval(args.lanczos_steps)
     if precision == msa.MSA_MIXED:
         period, drop = refinement(args, precond)
         b.set_refinement(msa.MSA_REFINE_RELIABLE if args.refine == "reliable"
@@ -467,6 +472,8 @@ def leg_newmark(msa, ctx, args, nsteps):
         b = msa.Batch(ctx, [mesh], [nsamp], precision, 1)
         if args.precond == "mg":
             b.set_preconditioner(msa.MSA_PRECOND_MULTIGRID)
+            if args.lanczos_steps is not None:
+                b.set_smoothing_interval(args.lanczos_steps)
         if precision == msa.MSA_MIXED:
             period, drop = refinement(args, args.precond)
             b.set_refinement(msa.MSA_REFINE_RELIABLE, period, drop)

@@ -402,6 +402,18 @@ int msa_batch_set_matrix_cores(msa_batch *b, int enable);
  * matrix): K v from the cell moduli and that matrix in FP32, without
  * reading the stored values. */
 int msa_batch_mg_info(msa_batch *b, int cls, long long *info);
+/* Upper end lam of each level's Chebyshev smoothing interval [lam/ratio,
+ * lam] (no counterpart in the reference, whose PCG preconditioner is Jacobi
+ * or the first sample's factor): with lanczos_steps > 0 (default 8,
+ * margin 1.2) lam = min(Gershgorin bound, margin x the top Ritz value of
+ * lanczos_steps Lanczos steps on K x = lambda D x), per sample and level,
+ * formed with the preconditioner's values; 0 keeps the Gershgorin bound.
+ * The Ritz value converges to the top of the spectrum from below, so the
+ * margin keeps lam above it; the Gershgorin bound caps it. */
+int msa_batch_set_smoothing_interval(msa_batch *b, int lanczos_steps, double margin);
+/* lam[max_levels] of one sample: level 0 (the class) first, 0 past the
+ * coarsest smoothed level. */
+int msa_batch_mg_lambda(msa_batch *b, int cls, int sample, int max_levels, double *lam);
 /* z = B r (one V-cycle) for one sample of class cls, r and z in the
  * reference numbering.  Uses the batch's PCG work vectors: call it between
  * solves. */

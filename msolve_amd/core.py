@@ -295,6 +295,16 @@ class Batch:
                     coarsest_bandwidth=int(info[3]), matrix_free=bool(info[4]),
                     matrix_free_coarse_levels=int(info[5]))
 
+    def set_smoothing_interval(self, lanczos_steps: int = 8, margin: float = 1.2):
+        """Upper end of the Chebyshev smoothing interval: Lanczos estimate
+        (lanczos_steps > 0) or the Gershgorin bound (0)."""
+        check(lib().msa_batch_set_smoothing_interval(self.h, lanczos_steps, margin))
+
+    def mg_lambda(self, cls: int, sample: int, max_levels: int = 8):
+        lam = np.zeros(max_levels)
+        check(lib().msa_batch_mg_lambda(self.h, cls, sample, max_levels, ptr(lam, C.c_double)))
+        return lam[lam > 0]
+
     def mg_apply(self, cls: int, sample: int, r):
         """One V-cycle z = B r for one sample (reference DOF numbering)."""
         r = np.ascontiguousarray(r, np.float64)

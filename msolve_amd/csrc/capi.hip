@@ -121,6 +121,10 @@ struct msa_batch {
     // geometric multigrid preconditioner (msa_batch_set_preconditioner)
     int precond = MSA_PRECOND_JACOBI;
     double mg_ratio = 12.0;             // smoothing interval [lam/ratio, lam]
+    int mg_lz_steps = 8;                // Lanczos steps of lam (0: the Gershgorin bound)
+    double mg_lz_margin = 1.2;          // lam = min(Gershgorin, margin x top Ritz value)
+    int *d_lz_list = nullptr;           // the smoothed classes (device), for the Lanczos sums
+    std::vector<int> lz_list;
     std::vector<ClassHost> lv;          // coarse levels: class table entry cls.size() + i
     std::vector<msa_mesh *> mg_meshes;  // their meshes (owned by the batch)
     std::vector<int> mg_nlev;           // per solved class: coarse levels (0: Jacobi)
@@ -387,6 +391,7 @@ int read_count(msa_batch *b, int which, int *out)
 
 int mg_vcycle(msa_batch *b, int pbuf, int mode);
 int mg_values(msa_batch *b);
+int mg_lanczos(msa_batch *b);
 int precond_values(msa_batch *b);
 
 // ru_tol > 0: reliable-update mode (inner test off, replacement triggers on).
@@ -1017,6 +1022,9 @@ int mg_build(msa_batch *b, int max_levels)
             void *p = nullptr;
             MSA_TRY(dalloc(b, &p, (size_t)tab(b, parent).ntiles * b->T * sizeof(double)));
             P.mg_lam = (double *)p;
+            MSA_TRY(dalloc(b, &p, (size_t)tab(b, parent).ntiles * b->T * (2 * kLzMax + 1) *
+                                      sizeof(double)));
+            P.mg_lz = (double *)p;
             if ((int)depth_classes.size() <= D.mg_depth) depth_classes.resize(D.mg_depth + 1);
             depth_classes[D.mg_depth].push_back(ci);
             if (l + 1 == meshes.size()) {
@@ -1201,6 +1209,7 @@ int mg_values(msa_batch *b)
             return launch_mg_gersh_mf(s, b->d_cls, b->mg_sm_e[d], b->mg_nsm_e[d]);
         }));
     }
+    MSA_TRY(mg_lanczos(b));
     for (int ci : b->mg_coarsest) {
         ClassHost &C = tab(b, ci);
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_fill_f64(s, C.dev.band, C.band_cap, 0.0); }));
@@ -1339,6 +1348,99 @@ int first_apply(msa_batch *b, int mode)
 // z = B r for the classes with a hierarchy (B: one V-cycle); their z held
 // D^-1 r on entry.  pbuf selects the free search-direction buffer, mode the
 // samples served (see k_mg_smooth).
+// One smoothing step (MG_PRE ... MG_POST2) of every class at depth d.
+int mg_smooth_level(msa_batch *b, int d, int step, int pbuf, int mode)
+{
+    hipStream_t s = b->ctx->stream;
+    const bool mx = is_mixed(b);
+    const int T = b->T;
+    const double ratio = b->mg_ratio;
+    // timing kind KK_MG: the finest level's smoothing products only
+    if (b->mg_nsm_a[d] > 0)
+        MSA_TRY(timed(b, d == 0 ? KK_MG : KK_OTHER, [&] {
+            return launch_mg_smooth(s, mx, T, b->d_cls, b->mg_sm_a[d], b->mg_nsm_a[d], step, pbuf,
+                                    mode, ratio, 0);
+        }));
+    if (b->mg_nsm_f[d] > 0)
+        MSA_TRY(timed(b, d == 0 ? KK_MG : KK_OTHER, [&] {
+            return launch_mg_smooth(s, mx, T, b->d_cls, b->mg_sm_f[d], b->mg_nsm_f[d], step, pbuf,
+                                    mode, ratio, 1);
+        }));
+    if (b->mg_nsm_b[d] > 0)
+        MSA_TRY(timed(b, d == 0 ? KK_MG : KK_OTHER, [&] {
+            return launch_mg_smooth(s, mx, T, b->d_cls, b->mg_sm_b[d], b->mg_nsm_b[d], step, pbuf,
+                                    mode, ratio, 2);
+        }));
+    if (b->matrix_cores && b->mg_nsm_m[d] > 0)
+        MSA_TRY(timed(b, d == 0 ? KK_MG : KK_OTHER, [&] {
+            return launch_mg_smooth_mfma(s, b->d_cls, b->mg_sm_m[d], b->mg_nsm_m[d], step, pbuf,
+                                         mode, ratio);
+        }));
+    if (!b->matrix_cores && b->mg_nsm_e[d] > 0)
+        MSA_TRY(timed(b, d == 0 ? KK_MG : KK_OTHER, [&] {
+            return launch_mg_smooth(s, mx, T, b->d_cls, b->mg_sm_e[d], b->mg_nsm_e[d], step, pbuf,
+                                    mode, ratio, 2);
+        }));
+    return MSA_OK;
+}
+
+// The top of each sample's spectrum of D^-1 K on every smoothed level by
+// mg_lz_steps Lanczos steps (k_lz_step), all levels in lock-step: each
+// step's product is the RESID smoothing step with b = 0 (so the product is
+// the level's own, matrix-free on box grids); mg_lam then holds
+// min(Gershgorin, margin x top Ritz value).  Uses the levels' q, z, r and
+// p0, which are free before a solve.
+int mg_lanczos(msa_batch *b)
+{
+    const int K = b->mg_lz_steps;
+    if (K <= 0) return MSA_OK;
+    hipStream_t s = b->ctx->stream;
+    const bool mx = is_mixed(b);
+    const int T = b->T;
+    std::vector<int> smoothed;
+    int max_samp = 0;
+    for (int ci = 0; ci < (int)(b->cls.size() + b->lv.size()); ci++)
+        if (tab(b, ci).dev.mg_coarse >= 0) {
+            smoothed.push_back(ci);
+            max_samp = std::max(max_samp, tab(b, ci).nsamp);
+        }
+    if (smoothed.empty()) return MSA_OK;
+    if (smoothed != b->lz_list) {
+        void *p = nullptr;
+        MSA_TRY(dalloc(b, &p, smoothed.size() * sizeof(int)));
+        MSA_HIP(hipMemcpyAsync(p, smoothed.data(), smoothed.size() * sizeof(int),
+                               hipMemcpyHostToDevice, s));
+        b->d_lz_list = (int *)p;
+        b->lz_list = smoothed;
+    }
+    const int nl = (int)smoothed.size();
+    auto step = [&](int op, int k) {
+        for (int d = 0; d < b->mg_depth; d++)
+            MSA_TRY(timed(b, KK_OTHER, [&] {
+                return launch_lz_step(s, mx, T, b->d_cls, b->mg_smooth[d], b->mg_nsmooth[d], op, k);
+            }));
+        return (int)MSA_OK;
+    };
+    auto sums = [&](int which, int k) {
+        return timed(b, KK_OTHER, [&] {
+            return launch_lz_sum(s, T, b->d_cls, b->d_lz_list, nl, max_samp, which, k);
+        });
+    };
+    MSA_TRY(step(0, 0));
+    MSA_TRY(sums(0, 0));
+    for (int k = 0; k < K; k++) {
+        for (int d = 0; d < b->mg_depth; d++) MSA_TRY(mg_smooth_level(b, d, MG_RESID, 0, 0));
+        MSA_TRY(step(2, k));
+        MSA_TRY(sums(1, k));
+        if (k + 1 == K) break;
+        MSA_TRY(step(3, k));
+        MSA_TRY(sums(2, k));
+    }
+    return timed(b, KK_OTHER, [&] {
+        return launch_lz_eig(s, b->d_cls, b->d_lz_list, nl, max_samp, K, b->mg_lz_margin);
+    });
+}
+
 int mg_vcycle(msa_batch *b, int pbuf, int mode)
 {
     if (b->precond == MSA_PRECOND_FIRST_SAMPLE) return first_apply(b, mode);
@@ -1346,36 +1448,7 @@ int mg_vcycle(msa_batch *b, int pbuf, int mode)
     hipStream_t s = b->ctx->stream;
     const bool mx = is_mixed(b);
     const int T = b->T;
-    const double ratio = b->mg_ratio;
-    // timing kind KK_MG: the finest level's smoothing products only
-    auto smooth = [&](int d, int step) {
-        if (b->mg_nsm_a[d] > 0)
-            MSA_TRY(timed(b, d == 0 ? KK_MG : KK_OTHER, [&] {
-                return launch_mg_smooth(s, mx, T, b->d_cls, b->mg_sm_a[d], b->mg_nsm_a[d], step,
-                                        pbuf, mode, ratio, 0);
-            }));
-        if (b->mg_nsm_f[d] > 0)
-            MSA_TRY(timed(b, d == 0 ? KK_MG : KK_OTHER, [&] {
-                return launch_mg_smooth(s, mx, T, b->d_cls, b->mg_sm_f[d], b->mg_nsm_f[d], step,
-                                        pbuf, mode, ratio, 1);
-            }));
-        if (b->mg_nsm_b[d] > 0)
-            MSA_TRY(timed(b, d == 0 ? KK_MG : KK_OTHER, [&] {
-                return launch_mg_smooth(s, mx, T, b->d_cls, b->mg_sm_b[d], b->mg_nsm_b[d], step,
-                                        pbuf, mode, ratio, 2);
-            }));
-        if (b->matrix_cores && b->mg_nsm_m[d] > 0)
-            MSA_TRY(timed(b, d == 0 ? KK_MG : KK_OTHER, [&] {
-                return launch_mg_smooth_mfma(s, b->d_cls, b->mg_sm_m[d], b->mg_nsm_m[d], step, pbuf,
-                                             mode, ratio);
-            }));
-        if (!b->matrix_cores && b->mg_nsm_e[d] > 0)
-            MSA_TRY(timed(b, d == 0 ? KK_MG : KK_OTHER, [&] {
-                return launch_mg_smooth(s, mx, T, b->d_cls, b->mg_sm_e[d], b->mg_nsm_e[d], step,
-                                        pbuf, mode, ratio, 2);
-            }));
-        return (int)MSA_OK;
-    };
+    auto smooth = [&](int d, int step) { return mg_smooth_level(b, d, step, pbuf, mode); };
     for (int d = 0; d < b->mg_depth; d++) {
         MSA_TRY(smooth(d, MG_PRE));
         MSA_TRY(smooth(d, MG_RESID));
@@ -2560,6 +2633,34 @@ int msa_batch_mg_info(msa_batch *b, int cls, long long *info)
             info[3] = L.bw;
         }
     }
+    return MSA_OK;
+}
+
+int msa_batch_set_smoothing_interval(msa_batch *b, int lanczos_steps, double margin)
+{
+    MSA_CHECK(b, MSA_ERR_ARG, "null batch");
+    MSA_CHECK(lanczos_steps == 0 || (lanczos_steps >= 2 && lanczos_steps <= kLzMax), MSA_ERR_ARG,
+              "lanczos_steps=%d (0, or 2..%d)", lanczos_steps, kLzMax);
+    MSA_CHECK(margin >= 1.0 && margin <= 2.0, MSA_ERR_ARG, "margin %g outside [1, 2]", margin);
+    b->mg_lz_steps = lanczos_steps;
+    b->mg_lz_margin = margin;
+    b->mg_ready = false;
+    return MSA_OK;
+}
+
+int msa_batch_mg_lambda(msa_batch *b, int cls, int sample, int max_levels, double *lam)
+{
+    MSA_CHECK(b && lam && cls >= 0 && cls < (int)b->cls.size(), MSA_ERR_ARG, "bad arguments");
+    MSA_CHECK(sample >= 0 && sample < b->cls[cls].nsamp, MSA_ERR_ARG, "sample %d out of range", sample);
+    MSA_CHECK(b->precond == MSA_PRECOND_MULTIGRID && b->cls[cls].dev.mg_coarse >= 0, MSA_ERR_STATE,
+              "class %d has no multigrid hierarchy", cls);
+    MSA_CHECK(b->assembled, MSA_ERR_STATE, "batch not assembled");
+    MSA_HIP(hipSetDevice(b->ctx->device));
+    if (!b->mg_ready) MSA_TRY(mg_values(b));
+    int l = 0;
+    for (int ci = cls; ci >= 0 && tab(b, ci).dev.mg_coarse >= 0 && l < max_levels; ci = tab(b, ci).dev.mg_coarse, l++)
+        MSA_HIP(hipMemcpy(lam + l, tab(b, ci).dev.mg_lam + sample, sizeof(double), hipMemcpyDeviceToHost));
+    for (int k = l; k < max_levels; k++) lam[k] = 0.0;
     return MSA_OK;
 }
 

@@ -3402,6 +3402,170 @@ __global__ __launch_bounds__(256) void k_mg_smooth(const ClassDev *__restrict__ 
     }
 }
 
+// ------------------------------------------------ smoothing-interval estimate
+// The Chebyshev smoother works on [lam/ratio, lam], lam the top of the
+// spectrum of D^-1 K.  The Gershgorin bound is rigorous but loose on these
+// grids (5.36 against a spectrum top near 3.1), which puts the interval
+// above the spectrum and weakens the cycle.  A few Lanczos steps on
+// K x = lambda D x per sample and level give the top Ritz value, a lower
+// bound that converges from below; the interval then takes
+// min(Gershgorin, margin x Ritz).  Level buffers (free before a solve):
+// v = q, v_prev = z, y = p0 = b - K v with b = r = 0 (the RESID smoothing
+// step), so K v = -p0.  All vectors are D-normalised: v.D.v = 1, D = 1/dinv
+// on the free components (dinv = 0 marks a fixed one).
+__device__ __forceinline__ float lz_hash(unsigned x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return (float)(x & 0xffffffu) * (1.0f / 8388608.0f) - 1.0f;  // [-1, 1)
+}
+
+// Unnormalised vectors u_k (|u_k|_D = n_k, v_k = u_k / n_k), so that a step
+// takes two passes besides its product y = b - K u_k = -K u_k:
+// op 0: u_0 = hashed values on the free components, u_prev = 0, b = 0;
+//       partials of u.D.u
+// op 2: partials of u_k.K u_k = -(u.y)   (alpha_k = that / n_k^2)
+// op 3: w = (D^-1 K u_k - alpha_k u_k) / n_k - (n_k / n_k-1) u_k-1,
+//       u_prev = u_k, u = w (= u_k+1); partials of w.D.w (n_k+1^2)
+// lz[k] = alpha_k, lz[kLzMax + k] = n_k: the tridiagonal's off-diagonal
+// between k - 1 and k is n_k.
+template <typename real, int T>
+__global__ __launch_bounds__(256) void k_lz_step(const ClassDev *__restrict__ cls,
+                                                 const int4 *__restrict__ items, int nitems,
+                                                 int op, int k)
+{
+    constexpr int R = 64 / T;
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const int tile = it.y, chunk = it.z;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rr = lane / T, s = lane % T;
+    const int sample = tile * T + s;
+    const bool act = sample < C.nsamp;
+    const size_t vb = (size_t)tile * C.tile_vec;
+    real *__restrict__ u = (real *)C.q + vb;
+    real *__restrict__ y = (real *)C.p0 + vb;
+    real *__restrict__ up = (real *)C.z + vb;
+    real *__restrict__ b = (real *)C.r + vb;
+    const real *__restrict__ dinv = (const real *)C.dinv + vb;
+    real alpha = 0, inv_n = 0, ratio = 0;
+    if (act && op == 3) {
+        const double *lz = C.mg_lz + (size_t)sample * (2 * kLzMax + 1);
+        const double n = lz[kLzMax + k];
+        alpha = (real)lz[k];
+        inv_n = n > 0.0 ? (real)(1.0 / n) : (real)0;
+        ratio = (k > 0 && lz[kLzMax + k - 1] > 0.0) ? (real)(n / lz[kLzMax + k - 1]) : (real)0;
+    }
+    double acc = 0.0;
+    const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+    if (act) {
+        for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+            const int br = slice * R + rr;
+            if (br >= C.nb) continue;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const size_t o = ((size_t)br * 3 + d) * T + s;
+                const real dv = dinv[o];
+                if (op == 0) {
+                    const real h = dv != (real)0 ? (real)lz_hash((unsigned)(br * 3 + d) * 2654435761u +
+                                                                 (unsigned)sample * 40503u)
+                                                 : (real)0;
+                    u[o] = h;
+                    up[o] = 0;
+                    b[o] = 0;
+                    if (dv != (real)0) acc += (double)h * (double)h / (double)dv;
+                } else if (op == 2) {
+                    acc -= (double)u[o] * (double)y[o];
+                } else {
+                    const real uk = u[o];
+                    const real w = dv != (real)0 ? (-dv * y[o] - alpha * uk) * inv_n - ratio * up[o]
+                                                 : (real)0;
+                    up[o] = uk;
+                    u[o] = w;
+                    if (dv != (real)0) acc += (double)w * (double)w / (double)dv;
+                }
+            }
+        }
+    }
+    double a1[1] = {acc};
+    block_partials<T, 1>(a1, C.partial + ((size_t)tile * C.nchunks + chunk) * 3 * T);
+}
+
+// Per sample of the classes in `list` (grid.y): the chunk partials summed
+// in chunk order.  which 0: n_0 = sqrt(u.D.u); 1: alpha_k = sum / n_k^2;
+// 2: n_k+1 = sqrt(w.D.w).
+template <int T>
+__global__ void k_lz_sum(const ClassDev *__restrict__ cls, const int *__restrict__ list,
+                         int which, int k)
+{
+    const ClassDev &C = cls[list[blockIdx.y]];
+    const int smp = blockIdx.x * blockDim.x + threadIdx.x;
+    if (smp >= C.nsamp) return;
+    const int tile = smp / T, t = smp % T;
+    const double *p = C.partial + (size_t)tile * C.nchunks * 3 * T + t;
+    double sum = 0.0;
+    for (int c = 0; c < C.nchunks; c++) sum += p[(size_t)c * 3 * T];
+    double *lz = C.mg_lz + (size_t)smp * (2 * kLzMax + 1);
+    if (which == 0) {
+        lz[kLzMax] = sqrt(fmax(sum, 0.0));
+    } else if (which == 1) {
+        const double n = lz[kLzMax + k];
+        lz[k] = n > 0.0 ? sum / (n * n) : 0.0;
+    } else {
+        lz[kLzMax + k + 1] = sqrt(fmax(sum, 0.0));
+    }
+}
+
+// Top eigenvalue of the Lanczos tridiagonal (alpha_0..alpha_m-1, off-
+// diagonals n_1..n_m-1; m = steps, or fewer where an n vanishes: an
+// invariant subspace) by bisection on the Sturm count; the interval's top
+// becomes min(Gershgorin bound, margin x Ritz value).  A Ritz value that is
+// not finite and positive leaves the Gershgorin bound.
+__global__ void k_lz_eig(const ClassDev *__restrict__ cls, const int *__restrict__ list, int steps,
+                         double margin)
+{
+    const ClassDev &C = cls[list[blockIdx.y]];
+    const int smp = blockIdx.x * blockDim.x + threadIdx.x;
+    if (smp >= C.nsamp) return;
+    const double *lz = C.mg_lz + (size_t)smp * (2 * kLzMax + 1);
+    const double *al = lz, *nb = lz + kLzMax;
+    if (!(nb[0] > 0.0)) return;
+    int m = steps;
+    for (int j = 1; j < steps; j++)
+        if (!(nb[j] > 1e-12 * fabs(al[0]))) {
+            m = j;
+            break;
+        }
+    double lo = 1e300, hi = -1e300;
+    for (int j = 0; j < m; j++) {
+        const double bl = j > 0 ? fabs(nb[j]) : 0.0;
+        const double bu = j + 1 < m ? fabs(nb[j + 1]) : 0.0;
+        lo = fmin(lo, al[j] - bl - bu);
+        hi = fmax(hi, al[j] + bl + bu);
+    }
+    if (!(hi > lo) || !isfinite(hi) || !isfinite(lo)) return;
+    for (int it = 0; it < 200 && hi - lo > 1e-13 * fabs(hi); it++) {
+        const double x = 0.5 * (lo + hi);
+        int below = 0;  // eigenvalues below x: negative pivots of T - x I
+        double d = 1.0;
+        for (int j = 0; j < m; j++) {
+            d = (al[j] - x) - (j > 0 ? nb[j] * nb[j] / d : 0.0);
+            if (d == 0.0) d = -1e-300;
+            if (d < 0.0) below++;
+        }
+        if (below == m)
+            hi = x;
+        else
+            lo = x;
+    }
+    if (isfinite(hi) && hi > 0.0) C.mg_lam[smp] = fmin(C.mg_lam[smp], margin * hi);
+}
+
 // The matrix-free smoothing products of a box grid of fully free blocks
 // with the element contraction on the matrix cores (MIXED, one-sample
 // tiles).  A workgroup owns kMfBY rows of nodes (all x) of one sample and
@@ -4989,6 +5153,30 @@ hipError_t launch_mg_band_out(hipStream_t s, bool mixed, int T, const ClassDev *
         MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_mg_band_out<double, TT>), grid, dim3(256), 0, s, cp,
                                              mode));
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_lz_step(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
+                          const int4 *items, int nitems, int op, int k)
+{
+    MSA_PCG_LAUNCH(k_lz_step, d_cls, items, nitems, op, k);
+}
+
+hipError_t launch_lz_sum(hipStream_t s, int T, const ClassDev *d_cls, const int *list, int nlist,
+                         int max_samp, int which, int k)
+{
+    if (nlist <= 0 || max_samp <= 0) return hipSuccess;
+    MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_lz_sum<TT>), dim3((max_samp + 255) / 256, nlist), dim3(256),
+                                         0, s, d_cls, list, which, k));
+    return hipGetLastError();
+}
+
+hipError_t launch_lz_eig(hipStream_t s, const ClassDev *d_cls, const int *list, int nlist,
+                         int max_samp, int steps, double margin)
+{
+    if (nlist <= 0 || max_samp <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_lz_eig, dim3((max_samp + 255) / 256, nlist), dim3(256), 0, s, d_cls, list,
+                       steps, margin);
     return hipGetLastError();
 }
 

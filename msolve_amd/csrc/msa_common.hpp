@@ -202,7 +202,8 @@ struct ClassDev {
     // drives the cycle; coarse: the next coarser level; fine: the finer one.
     int mg_root, mg_coarse, mg_fine, mg_depth;
     const SampleState *mg_st;  // sample states of the root class
-    double *mg_lam;       // [ntiles*T] upper bound of the spectrum of D^-1 K (Gershgorin)
+    double *mg_lam;       // [ntiles*T] upper end of the smoothing interval (spectrum of D^-1 K)
+    double *mg_lz;        // [ntiles*T][2*kLzMax+1] Lanczos alpha, beta, start scale
     // transfer rows in ELL layout (entry k of row i at k*nb + i, padding:
     // column 0 with weight 0): prolongation rows of this class's blocks
     // (columns: blocks of mg_coarse), restriction rows of this coarse
@@ -237,6 +238,7 @@ struct ClassDev {
     double *mf_k64;       // [640] the same in FP64 (replacement residuals)
 };
 
+constexpr int kLzMax = 16;         // Lanczos steps of the smoothing-interval estimate, at most
 constexpr int kBandNB = 64;        // block size of the band Cholesky
 constexpr int kBandChunkRows = 512;  // rows per partial of the backward solve
 constexpr int kDenseMax = 256;       // coarsest multigrid levels up to this size use a dense inverse
@@ -443,6 +445,16 @@ hipError_t launch_extract_sample(hipStream_t s, bool mixed, int T, const void *s
 hipError_t launch_pcg_persist(hipStream_t s, bool mixed, int U, const ClassDev *d_cls,
                               const int4 *items, int nitems, const int2 *tiles, int ntiles,
                               const ReduceArgs &ra, const ReduceArgs &rb, int iters, int *grid_out);
+// Lanczos estimate of the top of the spectrum of D^-1 K per sample and
+// level (the upper end of the Chebyshev smoothing interval).  op: 0 start
+// vector, 2 alpha partials, 3 next vector and its D-norm partials; sum
+// (over the classes in list): which 0 start norm, 1 alpha_k, 2 n_k+1.
+hipError_t launch_lz_step(hipStream_t s, bool mixed, int T, const ClassDev *d_cls,
+                          const int4 *items, int nitems, int op, int k);
+hipError_t launch_lz_sum(hipStream_t s, int T, const ClassDev *d_cls, const int *list, int nlist,
+                         int max_samp, int which, int k);
+hipError_t launch_lz_eig(hipStream_t s, const ClassDev *d_cls, const int *list, int nlist,
+                         int max_samp, int steps, double margin);
 hipError_t launch_insert_sample(hipStream_t s, int T, const double *src, size_t n_rows, int lane,
                                 double *dst);
 hipError_t launch_extract_values(hipStream_t s, bool mixed, int T, const void *val, size_t nslots9R,

@@ -47,6 +47,17 @@ def oracle_levels(m, E):
     return MG.hierarchy(m.xyz.reshape(-1, 3), m.conn, m.dof, m.n_free, E)
 
 
+def with_gpu_interval(levels, b, c, s):
+    """The oracle's levels smoothing on the GPU's interval [lam/ratio, lam]
+    of class c, sample s (the Lanczos estimate; pinned against the true
+    spectrum by test_lanczos_interval_brackets_the_spectrum)."""
+    lam = b.mg_lambda(c, s)
+    assert len(lam) == len(levels) - 1
+    for L_, l in zip(levels[:-1], lam):
+        L_.lam = float(l)
+    return levels
+
+
 def close(b, meshes):
     b.close()
     for m in meshes:
@@ -74,6 +85,7 @@ def test_vcycle_matches_oracle(ctx, precision, bar):
     rng = np.random.default_rng(9)
     for c, m in enumerate(models):
         levels, Ps = oracle_levels(m, Es[c][1])
+        with_gpu_interval(levels, b, c, 1)
         r = rng.standard_normal(m.n_free)
         z = b.mg_apply(c, 1, r)
         zr = MG.vcycle(levels, Ps, r)
@@ -93,6 +105,7 @@ def test_fp64_solves_match_direct_and_oracle_iterations(ctx):
             ud = spl.spsolve(levels[0].K.tocsc(), m.f)
             assert rel(b.solution(c, s), ud) < 1e-9
             if c < 2:
+                with_gpu_interval(levels, b, c, s)
                 _, it = MG.mg_pcg(levels, Ps, m.f, tol=1e-11)
                 assert abs(int(st["iters"][s]) - it) <= 1, (c, s, st["iters"][s], it)
                 assert it < 25
@@ -243,6 +256,7 @@ def test_matrix_free_cycle_matches_stored_value_cycle(ctx, dynamics):
             b.set_dynamics(7.85e-9, 5.0, 1e-5, 0.25, 0.5, 1e-4)
         b.assemble(0.3)
         b.set_preconditioner(msa.MSA_PRECOND_MULTIGRID)
+        b.set_smoothing_interval(0)   # the same (Gershgorin) interval in both precisions
         info = b.mg_info(0)
         assert info["levels"] == 2
         assert info["matrix_free"] == (prec == L.MSA_MIXED)
@@ -301,16 +315,21 @@ def test_matrix_free_forms_match_oracle_cycle(ctx, kind):
     rng = np.random.default_rng(17)
     E = lognormal(rng, model.n_elem)
     r = rng.standard_normal(model.n_free)
-    levels, Ps = MG.hierarchy(np.asarray(model.xyz).reshape(-1, 3), model.conn, model.dof,
-                              model.n_free, E)
-    zref = MG.vcycle(levels, Ps, r)
-    ref = {L.MSA_FP64: zref, L.MSA_MIXED: zref}
+    def oracle(lam=None):
+        levels, Ps = MG.hierarchy(np.asarray(model.xyz).reshape(-1, 3), model.conn, model.dof,
+                                  model.n_free, E)
+        if lam is not None:
+            for L_, l in zip(levels[:-1], lam):
+                L_.lam = float(l)
+        return levels, Ps
     bar = {L.MSA_FP64: 1e-9, L.MSA_MIXED: 1e-6}
     if kind == "long":  # the bar is tight enough to see an FP32 coarsest factor
+        levels, Ps = oracle()
+        zg = MG.vcycle(levels, Ps, r)
         Kc = levels[-1].K
         levels[-1].K = sp.csr_matrix((Kc.data.astype(np.float32).astype(np.float64), Kc.indices,
                                       Kc.indptr), shape=Kc.shape)
-        assert rel(MG.vcycle(levels, Ps, r), zref) > 10 * bar[L.MSA_MIXED]
+        assert rel(MG.vcycle(levels, Ps, r), zg) > 10 * bar[L.MSA_MIXED]
     for prec in (L.MSA_FP64, L.MSA_MIXED):
         b = msa.Batch(ctx, [mesh], [1], prec, 0)
         b.set_modulus(0, E[None, :])
@@ -318,11 +337,13 @@ def test_matrix_free_forms_match_oracle_cycle(ctx, kind):
         b.assemble(0.3)
         b.set_preconditioner(msa.MSA_PRECOND_MULTIGRID)
         info = b.mg_info(0)
+        z = b.mg_apply(0, 0, r)
+        levels, Ps = oracle(b.mg_lambda(0, 0))   # the oracle's cycle on the GPU's interval
+        zref = MG.vcycle(levels, Ps, r)
+        b.close()
         assert info["levels"] == len(Ps)  # coarse levels
         assert info["matrix_free"] == (prec == L.MSA_MIXED)
-        z = b.mg_apply(0, 0, r)
-        b.close()
-        assert rel(z, ref[prec]) < bar[prec], (prec, rel(z, ref[prec]))
+        assert rel(z, zref) < bar[prec], (prec, rel(z, zref))
     mesh.close()
 
 
@@ -384,6 +405,7 @@ def test_matrix_core_smoother_matches_valu_form_and_oracle(ctx, kind, dynamics):
         b.set_matrix_cores(mc)
         b.assemble(0.3)
         b.set_preconditioner(msa.MSA_PRECOND_MULTIGRID)
+        b.set_smoothing_interval(0)   # the Gershgorin interval: both forms and the oracle share it
         assert b.mg_info(0)["matrix_free"]
         z[mc] = [b.mg_apply(0, s, r) for s in range(3)]
         if mc and not dynamics:
@@ -406,3 +428,45 @@ def test_matrix_core_smoother_matches_valu_form_and_oracle(ctx, kind, dynamics):
         levels, Ps = MG.hierarchy(np.asarray(model.xyz).reshape(-1, 3), model.conn, model.dof,
                                   model.n_free, E[1])
         assert rel(z[True][1], MG.vcycle(levels, Ps, r)) < 1e-6
+
+
+def test_lanczos_interval_brackets_the_spectrum(ctx):
+    """The smoothing interval's upper end from 8 Lanczos steps (margin 1.2,
+    the defaults) against the spectrum of D^-1 K of each level, from the oracle's
+    level operators (scipy eigsh): it lies above the top eigenvalue and
+    within the margin of it, and below the Gershgorin bound (which is what
+    0 Lanczos steps gives, to 1e-12 in FP64).  MIXED batches (FP32 vectors)
+    find the FP64 value to 1e-3.  The tighter interval saves PCG
+    iterations (FP64, 12^3)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    print("\n[case] Lanczos smoothing interval", flush=True)
+    out = {}
+    for prec in (L.MSA_FP64, L.MSA_MIXED):
+        for steps in (8, 0):
+            b, models, meshes, Es = make(ctx, [8, 12], 2, prec, seed=21)
+            b.set_smoothing_interval(steps)
+            b.solve(1e-11, 5000)
+            for c in range(2):
+                assert (b.stats(c)["status"] == msa.MSA_OK).all()
+                for s in range(2):
+                    out[(prec, steps, c, s)] = (b.mg_lambda(c, s), int(b.stats(c)["iters"][s]))
+            close(b, meshes)
+    for c, m in enumerate(models):
+        for s in range(2):
+            levels, _ = oracle_levels(m, Es[c][s])
+            lam8, it8 = out[(L.MSA_FP64, 8, c, s)]
+            lam0, it0 = out[(L.MSA_FP64, 0, c, s)]
+            assert len(lam8) == len(levels) - 1
+            for l, L_ in enumerate(levels[:-1]):
+                dh = sp.diags(np.sqrt(L_.dinv))
+                top = sla.eigsh(dh @ L_.K @ dh, k=1, which="LA", return_eigenvectors=False,
+                                tol=1e-10)[0]
+                print(f"[lanczos] class {c} sample {s} level {l}: top {top:.5f} "
+                      f"lam {lam8[l]:.5f} gershgorin {L_.lam:.5f}", flush=True)
+                assert abs(lam0[l] - L_.lam) <= 1e-12 * L_.lam
+                assert top * (1 - 1e-9) <= lam8[l] <= min(1.2 * top, L_.lam) * (1 + 1e-12)
+                assert abs(out[(L.MSA_MIXED, 8, c, s)][0][l] - lam8[l]) <= 1e-3 * lam8[l]
+            if c == 1:
+                assert it8 < it0, (it8, it0)
+
