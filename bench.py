@@ -193,8 +193,10 @@ def mf_work(b, models):
     rows of the unit element matrix times the cell's 24 values) and 3 FMA
     (modulus scaling): 150 flop.  Bytes per product and sample: the moduli
     (8 per element) and 4 vectors of 4*3*nb on average over the four steps;
-    the grid tables once per launch."""
+    the grid tables once per launch.  The set-up's Lanczos steps (one RESID
+    product each, every sample) count as products too."""
     flops = byts = 0.0
+    lz_steps = b.smoothing_interval()[0]
     for c, m in enumerate(models):
         info = b.mg_info(c)
         if info["levels"] == 0:
@@ -207,9 +209,10 @@ def mf_work(b, models):
         free_node = (np.asarray(m.dof).reshape(-1, 3) >= 0).any(axis=1)
         inc = float(free_node[conn].sum())
         nb = float(free_node.sum())
-        flops += cyc.sum() * 4 * inc * 150.0
-        byts += cyc.sum() * 4 * (8.0 * conn.shape[0] + 4 * 4 * 3 * nb)
-        byts += float(cyc.max()) * 4 * 4.0 * (len(free_node) + conn.shape[0] + nb)
+        prods = cyc.sum() * 4 + lz_steps * len(cyc)
+        flops += prods * inc * 150.0
+        byts += prods * (8.0 * conn.shape[0] + 4 * 4 * 3 * nb)
+        byts += (float(cyc.max()) * 4 + lz_steps) * 4.0 * (len(free_node) + conn.shape[0] + nb)
     return flops, byts
 
 

@@ -411,6 +411,7 @@ int msa_batch_mg_info(msa_batch *b, int cls, long long *info);
  * The Ritz value converges to the top of the spectrum from below, so the
  * margin keeps lam above it; the Gershgorin bound caps it. */
 int msa_batch_set_smoothing_interval(msa_batch *b, int lanczos_steps, double margin);
+int msa_batch_get_smoothing_interval(msa_batch *b, int *lanczos_steps, double *margin);
 /* lam[max_levels] of one sample: level 0 (the class) first, 0 past the
  * coarsest smoothed level. */
 int msa_batch_mg_lambda(msa_batch *b, int cls, int sample, int max_levels, double *lam);

@@ -37,7 +37,7 @@ EXPORTS = [
     "msa_batch_get_matrix_csr", "msa_batch_set_matrix_csr", "msa_batch_prepare_preconditioner",
     "msa_batch_set_zem", "msa_batch_get_zem",
     "msa_batch_set_persistent", "msa_batch_persistent_info",
-    "msa_batch_set_smoothing_interval", "msa_batch_mg_lambda",
+    "msa_batch_set_smoothing_interval", "msa_batch_get_smoothing_interval", "msa_batch_mg_lambda",
     "msa_batch_spmv", "msa_batch_set_kernel_timing",
     "msa_batch_get_kernel_timing", "msa_batch_spmv_bytes", "msa_autotune_tile",
     "msa_batch_set_refinement", "msa_batch_set_integration",
@@ -124,6 +124,7 @@ def lib():
         "msa_batch_set_zem": [_vp, _d, _i],
         "msa_batch_set_persistent": [_vp, _i],
         "msa_batch_set_smoothing_interval": [_vp, _i, _d],
+        "msa_batch_get_smoothing_interval": [_vp, _ip, _dp],
         "msa_batch_mg_lambda": [_vp, _i, _i, _i, _dp],
         "msa_batch_persistent_info": [_vp, _llp],
         "msa_batch_get_zem": [_vp, _i, _i, _i, _ip, _ip, _dp],

@@ -300,6 +300,13 @@ class Batch:
         (lanczos_steps > 0) or the Gershgorin bound (0)."""
         check(lib().msa_batch_set_smoothing_interval(self.h, lanczos_steps, margin))
 
+    def smoothing_interval(self):
+        """(Lanczos steps, margin) of the smoothing interval."""
+        k = C.c_int(0)
+        m = C.c_double(0.0)
+        check(lib().msa_batch_get_smoothing_interval(self.h, C.byref(k), C.byref(m)))
+        return k.value, m.value
+
     def mg_lambda(self, cls: int, sample: int, max_levels: int = 8):
         lam = np.zeros(max_levels)
         check(lib().msa_batch_mg_lambda(self.h, cls, sample, max_levels, ptr(lam, C.c_double)))

@@ -2648,6 +2648,14 @@ int msa_batch_set_smoothing_interval(msa_batch *b, int lanczos_steps, double mar
     return MSA_OK;
 }
 
+int msa_batch_get_smoothing_interval(msa_batch *b, int *lanczos_steps, double *margin)
+{
+    MSA_CHECK(b && lanczos_steps && margin, MSA_ERR_ARG, "bad arguments");
+    *lanczos_steps = b->mg_lz_steps;
+    *margin = b->mg_lz_margin;
+    return MSA_OK;
+}
+
 int msa_batch_mg_lambda(msa_batch *b, int cls, int sample, int max_levels, double *lam)
 {
     MSA_CHECK(b && lam && cls >= 0 && cls < (int)b->cls.size(), MSA_ERR_ARG, "bad arguments");
