@@ -89,6 +89,8 @@ def parse():
     p.add_argument("--lanczos-steps", type=int, default=None,
                    help="Lanczos steps of the multigrid smoothing interval (0: Gershgorin bound; "
                         "default: the library's)")
+    p.add_argument("--lanczos-margin", type=float, default=None,
+                   help="margin over the top Ritz value (default: the library's)")
     p.add_argument("--precond", choices=["mg", "jacobi"], default="mg",
                    help="PCG preconditioner: multigrid V-cycle or Jacobi")
     p.add_argument("--no-compare", action="store_true",
@@ -144,8 +146,10 @@ def make_batch(msa, ctx, meshes, models, counts, precision, tile, table, xi, arg
     precond = precond or args.precond
     if precond == "mg":
         b.set_preconditioner(msa.MSA_PRECOND_MULTIGRID)
-        if args.lanczos_steps is not None:
-            b.set_smoothing_interval(args.lanczos_steps)
+        if args.lanczos_steps is not None or args.lanczos_margin is not None:
+            k0, m0 = b.smoothing_interval()
+            b.set_smoothing_interval(k0 if args.lanczos_steps is None else args.lanczos_steps,
+                                     m0 if args.lanczos_margin is None else args.lanczos_margin)
     if precision == msa.MSA_MIXED:
         period, drop = refinement(args, precond)
         b.set_refinement(msa.MSA_REFINE_RELIABLE if args.refine == "reliable"
@@ -475,8 +479,10 @@ def leg_newmark(msa, ctx, args, nsteps):
         b = msa.Batch(ctx, [mesh], [nsamp], precision, 1)
         if args.precond == "mg":
             b.set_preconditioner(msa.MSA_PRECOND_MULTIGRID)
-            if args.lanczos_steps is not None:
-                b.set_smoothing_interval(args.lanczos_steps)
+            if args.lanczos_steps is not None or args.lanczos_margin is not None:
+                k0, m0 = b.smoothing_interval()
+                b.set_smoothing_interval(k0 if args.lanczos_steps is None else args.lanczos_steps,
+                                         m0 if args.lanczos_margin is None else args.lanczos_margin)
         if precision == msa.MSA_MIXED:
             period, drop = refinement(args, args.precond)
             b.set_refinement(msa.MSA_REFINE_RELIABLE, period, drop)

@@ -3496,20 +3496,25 @@ __global__ __launch_bounds__(256) void k_lz_step(const ClassDev *__restrict__ cl
     block_partials<T, 1>(a1, C.partial + ((size_t)tile * C.nchunks + chunk) * 3 * T);
 }
 
-// Per sample of the classes in `list` (grid.y): the chunk partials summed
-// in chunk order.  which 0: n_0 = sqrt(u.D.u); 1: alpha_k = sum / n_k^2;
+// Per sample of the classes in `list` (grid.y), one wave per sample: the
+// chunk partials summed lane-strided, then across the wave in a fixed
+// order.  which 0: n_0 = sqrt(u.D.u); 1: alpha_k = sum / n_k^2;
 // 2: n_k+1 = sqrt(w.D.w).
 template <int T>
-__global__ void k_lz_sum(const ClassDev *__restrict__ cls, const int *__restrict__ list,
-                         int which, int k)
+__global__ __launch_bounds__(256) void k_lz_sum(const ClassDev *__restrict__ cls,
+                                                const int *__restrict__ list, int which, int k)
 {
     const ClassDev &C = cls[list[blockIdx.y]];
-    const int smp = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int smp = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (smp >= C.nsamp) return;
     const int tile = smp / T, t = smp % T;
     const double *p = C.partial + (size_t)tile * C.nchunks * 3 * T + t;
     double sum = 0.0;
-    for (int c = 0; c < C.nchunks; c++) sum += p[(size_t)c * 3 * T];
+    for (int c = lane; c < C.nchunks; c += 64) sum += p[(size_t)c * 3 * T];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) sum += __shfl_down(sum, off, 64);
+    if (lane != 0) return;
     double *lz = C.mg_lz + (size_t)smp * (2 * kLzMax + 1);
     if (which == 0) {
         lz[kLzMax] = sqrt(fmax(sum, 0.0));
@@ -5166,7 +5171,7 @@ hipError_t launch_lz_sum(hipStream_t s, int T, const ClassDev *d_cls, const int 
                          int max_samp, int which, int k)
 {
     if (nlist <= 0 || max_samp <= 0) return hipSuccess;
-    MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_lz_sum<TT>), dim3((max_samp + 255) / 256, nlist), dim3(256),
+    MSA_DISPATCH_T(T, hipLaunchKernelGGL((k_lz_sum<TT>), dim3((max_samp + 3) / 4, nlist), dim3(256),
                                          0, s, d_cls, list, which, k));
     return hipGetLastError();
 }
