@@ -268,6 +268,70 @@ class SolverPCG {
     std::unique_ptr<Batch> batch_;
 };
 
+// SolverSkyline (SolverSkyline.cs:55-205) for one model and one modulus
+// field on the GPU band Cholesky: Initialize() assembles, Factorize() is
+// SkylineMatrix2D.Factorize(tolerance, zems, zemCols) (SkylineMatrix2D.cs:
+// 63-158: a pivot under tolerance is a zero-energy mode; tolerance 0 makes
+// a non-positive pivot an error instead), Solve() the forward/backward
+// substitution into VectorX.
+class SolverSkyline {
+  public:
+    SolverSkyline(Context &ctx, const Model &m, std::vector<double> E, double nu = 0.3)
+        : ctx_(ctx), model_(m), E_(std::move(E)), nu_(nu) {}
+
+    void Initialize()
+    {
+        mesh_.reset(new Mesh(ctx_, model_));
+        batch_.reset(new Batch(ctx_, {mesh_.get()}, {1}, MSA_FP64, 1));
+        check(msa_batch_set_solver(batch_->get(), MSA_SOLVER_DIRECT));
+        check(msa_batch_set_modulus(batch_->get(), 0, E_.data()));
+        check(msa_batch_set_rhs(batch_->get(), 0, model_.Forces.data()));
+        check(msa_batch_assemble(batch_->get(), nu_));
+    }
+
+    void Factorize(double tolerance, std::vector<std::vector<double>> &zems,
+                   std::vector<int> &zemCols, int maxModes = 64)
+    {
+        if (!batch_) Initialize();
+        check(msa_batch_set_zem(batch_->get(), tolerance, tolerance > 0 ? maxModes : 0));
+        check(msa_batch_factorize(batch_->get()));
+        zems.clear();
+        zemCols.clear();
+        if (tolerance <= 0) return;
+        int n = 0;
+        std::vector<int> cols(maxModes);
+        std::vector<double> modes((size_t)maxModes * model_.TotalDOFs);
+        check(msa_batch_get_zem(batch_->get(), 0, 0, maxModes, &n, cols.data(), modes.data()));
+        for (int k = 0; k < n; k++) {
+            zemCols.push_back(cols[k]);
+            zems.emplace_back(modes.begin() + (size_t)k * model_.TotalDOFs,
+                              modes.begin() + (size_t)(k + 1) * model_.TotalDOFs);
+        }
+    }
+
+    void Solve()
+    {
+        if (!batch_) Initialize();
+        check(msa_batch_direct_solve(batch_->get()));
+        int status = 0;
+        check(msa_batch_get_stats(batch_->get(), 0, nullptr, nullptr, nullptr, &status));
+        VectorX.assign(model_.TotalDOFs, 0.0);
+        check(msa_batch_get_solution(batch_->get(), 0, 0, VectorX.data()));
+        Status = status;
+    }
+
+    std::vector<double> VectorX;
+    int Status = -1;
+
+  private:
+    Context &ctx_;
+    const Model &model_;
+    std::vector<double> E_;
+    double nu_;
+    std::unique_ptr<Mesh> mesh_;
+    std::unique_ptr<Batch> batch_;
+};
+
 // Karhunen-Loeve table in the fields of the reference's KL file
 // (LognormalFileStochasticCoefficientsProvider.cs:90-109).
 struct KLTable {

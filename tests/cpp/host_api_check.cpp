@@ -1,5 +1,6 @@
 // Exercises the C++ host layer (include/msolve_amd.hpp) end to end on the
-// GPU: a plain SolverPCG solve (Jacobi and multigrid) and a two-class Monte Carlo batch, printing
+// GPU: a plain SolverPCG solve (Jacobi and multigrid), a two-class Monte Carlo batch, a Newmark
+// run and SolverSkyline's zero-energy modes on a floating box, printing
 // the results as JSON for tests/test_gpu_host_cpp.py to compare with the
 // oracle.  Usage: host_api_check <kl_table.txt> (M, then rows of w a per
 // axis, evpPos, eigs) ; xi and moduli are derived deterministically.
@@ -90,6 +91,25 @@ int main(int argc, char **argv)
         mg.Solve(20000, 1e-12);
         std::printf("\"mg_products\": %d, \"mg_status\": %d,\n", mg.Products, mg.Status);
         dump("u_mg8", mg.VectorX);
+        // SolverSkyline with zero-energy modes on a floating 3^3 box (no
+        // supports: six rigid-body modes), unit load on the last DOF
+        msolve::Model fl = msolve::Hexa8MeshGenerator::Box(3, 3, 3);
+        fl.Loads.push_back({fl.NodeCount() - 1, msolve::DOFType::X, 1.0});
+        fl.ConnectDataStructures();
+        std::vector<double> Efl(fl.ElementCount(), 2.1e5);
+        msolve::SolverSkyline sky(ctx, fl, Efl, 0.3);
+        std::vector<std::vector<double>> zems;
+        std::vector<int> zemCols;
+        sky.Factorize(1e-3, zems, zemCols);
+        sky.Solve();
+        std::printf("\"zem_cols\": [");
+        for (size_t k = 0; k < zemCols.size(); k++) std::printf(k ? ",%d" : "%d", zemCols[k]);
+        std::printf("],\n\"sky_status\": %d,\n", sky.Status);
+        for (size_t k = 0; k < zems.size(); k++) {
+            std::string key = "zem_" + std::to_string(k);
+            dump(key.c_str(), zems[k]);
+        }
+        dump("u_sky", sky.VectorX);
         std::vector<double> ke0(ke.begin(), ke.begin() + 300);
         dump("ke0", ke0, true);
         std::printf("}\n");

@@ -1,5 +1,7 @@
 """GPU: the C++ host layer (include/msolve_amd.hpp) driven end to end by
-tests/cpp/host_api_check.cpp, compared with the oracle."""
+tests/cpp/host_api_check.cpp, compared with the oracle: SolverPCG (Jacobi
+and multigrid), the Monte Carlo analyzer, Newmark, the element provider and
+SolverSkyline with zero-energy modes."""
 import json
 import os
 import subprocess
@@ -59,5 +61,19 @@ def test_cpp_host_layer(tmp_path):
     c8 = O.Cantilever(8)
     assert out["mg_status"] == 0 and out["mg_products"] < 25
     assert rel(out["u_mg8"], direct(c8, np.full(c8.n_elem, 2.1e5))) < 1e-10
+    # SolverSkyline.Factorize(1e-3, zems, zemCols) on the floating 3^3 box
+    # against the oracle's restatement (SkylineMatrix2D.Factorize + Solve)
+    from msolve_amd.model import box_mesh
+    xyz, conn = box_mesh(3, 3, 3)
+    dof, nf = O.enumerate_dofs(np.zeros((xyz.shape[0], 3), np.uint8))   # no supports
+    ke, _ = O.ke_mesh(conn, xyz, np.full(conn.shape[0], 2.1e5), 0.3)
+    ri, data = O.skyline_assemble(conn, dof, nf, ke)
+    fac, k, cols, Z = O.skyline_factorize_zem(ri, data, 1e-3, 16)
+    assert out["zem_cols"] == cols.tolist() and k == 6 and out["sky_status"] == 0
+    for j in range(6):
+        assert np.abs(np.array(out[f"zem_{j}"]) - Z[j]).max() <= 1e-8 * np.abs(Z[j]).max()
+    f = np.zeros(nf)
+    f[nf - 3] = 1.0
+    assert rel(out["u_sky"], O.skyline_solve(ri, fac, f)) < 1e-9
     ke, _ = O.ke_mesh(c4.conn[:8], c4.xyz, np.ones(1), 0.3)
     assert np.abs(np.array(out["ke0"]) - ke[0]).max() < 1e-13 * np.abs(ke[0]).max()
