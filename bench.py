@@ -80,7 +80,7 @@ def parse():
     p.add_argument("--period", type=int, default=None,
                    help="replacement check period (iterations; default 2 with multigrid, 8 with Jacobi)")
     p.add_argument("--drop", type=float, default=None,
-                   help="FP32 residual drop that triggers a replacement (default 1e-3 with "
+                   help="FP32 residual drop that triggers a replacement (default 1e-4 with "
                         "multigrid, 0.1 with Jacobi)")
     p.add_argument("--refine", choices=["reliable", "restart"], default="reliable")
     p.add_argument("--matrix-cores", type=int, default=0,
@@ -134,10 +134,10 @@ def workload(args):
 def refinement(args, precond):
     """Replacement check period and trigger: a multigrid iteration cuts the
     residual by about 5x, so checks come every 2 iterations and a
-    replacement waits for a 1e-3 drop; Jacobi-PCG keeps 8 and 0.1."""
+    replacement waits for a 1e-4 drop; Jacobi-PCG keeps 8 and 0.1."""
     mg = precond == "mg"
     return (args.period if args.period is not None else (2 if mg else 8),
-            args.drop if args.drop is not None else (1e-3 if mg else 0.1))
+            args.drop if args.drop is not None else (1e-4 if mg else 0.1))
 
 
 def make_batch(msa, ctx, meshes, models, counts, precision, tile, table, xi, args, precond=None,

@@ -200,9 +200,10 @@ int msa_batch_solve(msa_batch *b, double tol, int max_it, double inner_tol,
 
 /* Refinement scheme of MIXED solves (no counterpart in the reference, which
  * is FP64 only).  MSA_REFINE_RELIABLE (default): the FP32 PCG never restarts;
- * a sample whose FP32 residual has fallen by `drop` (default 0.1) since its
- * last replacement -- or whose estimated FP64 residual passes tol -- gets,
- * at the next check (every `period` iterations, even, default 8), its FP32
+ * a sample whose FP32 residual has fallen by `drop` (default 0.1; 1e-4 with
+ * the multigrid or first-sample preconditioner) since its last replacement
+ * -- or whose estimated FP64 residual passes tol -- gets, at the next check
+ * (every `period` iterations, even, default 8; 2 with those), its FP32
  * correction folded into the FP64 solution, its residual replaced by the
  * FP64 f - K u, the stopping test |f - K u|/|f| < tol, and the FP32
  * recurrence continued from that residual (search direction rescaled);

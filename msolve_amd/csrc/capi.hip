@@ -2602,10 +2602,10 @@ int msa_batch_set_preconditioner(msa_batch *b, int kind, int max_levels, double 
     b->mg_ratio = ratio > 0 ? ratio : 12.0;
     if (!b->refine_set) {
         // a V-cycle iteration cuts the residual about 5x: check for a
-        // replacement every 2 iterations, replace after a 1e-3 drop
+        // replacement every 2 iterations, replace after a 1e-4 drop
         // (the same for the first-sample factor: a strong preconditioner)
         b->period = kind != MSA_PRECOND_JACOBI ? 2 : 8;
-        b->drop = kind != MSA_PRECOND_JACOBI ? 1e-3 : 0.1;
+        b->drop = kind != MSA_PRECOND_JACOBI ? 1e-4 : 0.1;
     }
     if (b->graph) {  // the captured iteration block holds the preconditioner
         (void)hipGraphExecDestroy(b->graph);
