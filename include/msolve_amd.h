@@ -222,6 +222,16 @@ int msa_batch_set_refinement(msa_batch *b, int method, int period, double drop);
  * preconditioner re-captures at the next solve. */
 int msa_batch_graph_info(msa_batch *b, long long *info);
 
+/* Stored layout of class cls (no counterpart in the reference; the
+ * reference's SkylineMatrix2D keeps the upper profile only): info[0] 1 if
+ * the class is a box grid whose block rows hold their 27 neighbours in
+ * stencil slots (offset (dx, dy, dz) in slot (dz+1)*9 + (dy+1)*3 + dx+1,
+ * missing neighbours as zero blocks), [1] 1 if the PCG product reads only
+ * the upper triangle of that image (one-sample tiles: the lower blocks are
+ * the transposes of the upper blocks of the rows above, as in the skyline
+ * product), [2] stored block slots per sample, [3] block rows. */
+int msa_batch_layout_info(msa_batch *b, int cls, long long *info);
+
 /* Element integration precision of a MIXED batch, whose stiffness values are
  * stored in FP32: MSA_FP64 (default) integrates in FP64 and rounds each
  * assembled value; MSA_MIXED integrates in FP32.  FP32 integration leaves
@@ -297,7 +307,10 @@ int msa_batch_set_kernel_timing(msa_batch *b, int enable);
 int msa_batch_get_kernel_timing(msa_batch *b, double *ms_total,
                                 long long *launches);
 /* Algorithmic bytes of one launch of the SpMV+dot kernel over every sample
- * of the batch (values + shared indices + vectors), and its element count. */
+ * of the batch: bytes_per_launch[0] the kernel's (values + shared indices +
+ * vectors; on box grids with one-sample tiles the upper triangle of the
+ * values, diagonal included, and no indices), bytes_per_launch[1] a plain
+ * SpMV's (all values + indices + one vector read and one written). */
 int msa_batch_spmv_bytes(msa_batch *b, double *bytes_per_launch);
 /* Measurement (no counterpart in the reference): times `reps` back-to-back
  * launches of the batch's SpMV + dot kernel with every sample active (the

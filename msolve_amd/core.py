@@ -384,6 +384,15 @@ class Batch:
         return dict(captures=int(info[0]), kernel_nodes=int(info[1]),
                     iters_per_replay=int(info[2]), replays=int(info[3]))
 
+    def layout_info(self, cls: int):
+        """Stored layout of a class: stencil slot order (box grids), PCG
+        product from the upper triangle, stored block slots per sample,
+        block rows."""
+        info = np.zeros(4, np.int64)
+        check(lib().msa_batch_layout_info(self.h, cls, ptr(info, C.c_longlong)))
+        return dict(stencil=bool(info[0]), upper_triangle_product=bool(info[1]),
+                    slots=int(info[2]), block_rows=int(info[3]))
+
     def set_persistent(self, enable: bool):
         """Jacobi-PCG solves as one cooperative launch (one-sample tiles)."""
         check(lib().msa_batch_set_persistent(self.h, 1 if enable else 0))

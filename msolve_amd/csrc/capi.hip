@@ -53,6 +53,14 @@ struct ClassHost {
     int zem_alloc = 0;
     int nsamp = 0, ntiles = 0, nslices = 0, nchunks = 0, chunk = 0;
     std::vector<int> slice_ptr, cidx;
+    // slot within its row's slice of every adjacency entry (brow_ptr/bcol
+    // order): its rank in the row, or its stencil slot on box grids
+    std::vector<uint8_t> srank;
+    int R = 64;  // block rows per slice
+    int slot_of(const MeshHost &h, int br, int j) const
+    {
+        return slice_ptr[br / R] + srank[h.brow_ptr[br] + j];
+    }
     long long tile_vals = 0, tile_vec = 0;
     ClassDev dev{};
     bool has_rhs = false, has_E = false, has_kl = false, has_xi = false;
@@ -183,6 +191,46 @@ int dalloc(msa_batch *b, void **p, size_t bytes)
         if (rc_ != MSA_OK) return rc_;  \
     } while (0)
 
+// Slot of every adjacency entry within its row (ClassHost::srank): the
+// stencil slot on a box grid (ClassDev::stencil), with sy, sz the block
+// strides in y and z; otherwise the entry's rank in the row, and false.
+// A box thinner than 3 points in x or y keeps the ranks: there the stencil
+// order is not the column order.
+bool stencil_ranks(const MeshHost &h, std::vector<uint8_t> &srank, int &sy, int &sz)
+{
+    srank.resize(h.bcol.size());
+    for (int br = 0; br < h.nb; br++)
+        for (int j = h.brow_ptr[br]; j < h.brow_ptr[br + 1]; j++)
+            srank[j] = (uint8_t)(j - h.brow_ptr[br]);
+    sy = sz = 0;
+    GridInfo g;
+    if (grid_info(h, g) != MSA_OK || !g.box) {
+        set_error("");
+        return false;
+    }
+    const int nx = g.box_hi[0] - g.box_lo[0] + 1, ny = g.box_hi[1] - g.box_lo[1] + 1;
+    if (nx < 3 || ny < 3) return false;
+    std::vector<uint8_t> st(h.bcol.size());
+    for (int br = 0; br < h.nb; br++) {
+        const int pr = g.bpos[br];
+        for (int j = h.brow_ptr[br]; j < h.brow_ptr[br + 1]; j++) {
+            const int bc = h.bcol[j], pc = g.bpos[bc];
+            const int dx = (pc & 1023) - (pr & 1023);
+            const int dy = ((pc >> 10) & 1023) - ((pr >> 10) & 1023);
+            const int dz = (pc >> 20) - (pr >> 20);
+            if (std::abs(dx) > 1 || std::abs(dy) > 1 || std::abs(dz) > 1) return false;
+            if (bc != br + dx + nx * dy + nx * ny * dz) return false;
+            const int s = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+            if ((s > 13) != (bc > br) || (s == 13) != (bc == br)) return false;
+            st[j] = (uint8_t)s;
+        }
+    }
+    srank.swap(st);
+    sy = nx;
+    sz = nx * ny;
+    return true;
+}
+
 // Device arrays and block-ELL layout of one class (a solved class or a
 // multigrid level) for nsamp samples.
 int setup_class(msa_batch *b, ClassHost &C, msa_mesh *mesh, int nsamp)
@@ -198,6 +246,9 @@ int setup_class(msa_batch *b, ClassHost &C, msa_mesh *mesh, int nsamp)
         C.nslices = (h.nb + R - 1) / R;
         C.chunk = kWaves * std::max(1, 16 / R);
         C.nchunks = (C.nslices + C.chunk - 1) / C.chunk;
+        C.R = R;
+        int sy = 0, sz = 0;
+        const bool stencil = stencil_ranks(h, C.srank, sy, sz);
         C.slice_ptr.assign(C.nslices + 1, 0);
         for (int sl = 0; sl < C.nslices; sl++) {
             int wmax = 0;
@@ -205,14 +256,31 @@ int setup_class(msa_batch *b, ClassHost &C, msa_mesh *mesh, int nsamp)
                 int br = sl * R + rr;
                 if (br < h.nb) wmax = std::max(wmax, h.brow_ptr[br + 1] - h.brow_ptr[br]);
             }
-            C.slice_ptr[sl + 1] = C.slice_ptr[sl] + wmax;
+            C.slice_ptr[sl + 1] = C.slice_ptr[sl] + (stencil ? 27 : wmax);
         }
         const long long nslots = C.slice_ptr[C.nslices];
         C.cidx.assign((size_t)nslots * R, -1);
         for (int br = 0; br < h.nb; br++) {
-            int sl = br / R, rr = br % R;
+            const int rr = br % R;
             for (int j = 0; j < h.brow_ptr[br + 1] - h.brow_ptr[br]; j++)
-                C.cidx[(size_t)(C.slice_ptr[sl] + j) * R + rr] = h.bcol[h.brow_ptr[br] + j];
+                C.cidx[(size_t)C.slot_of(h, br, j) * R + rr] = h.bcol[h.brow_ptr[br] + j];
+        }
+        // the class's diagonal slots and element-pair slots (the mesh's
+        // ranks unless the stencil order replaces them)
+        std::vector<int> dpos(h.nb);
+        for (int br = 0; br < h.nb; br++) dpos[br] = C.srank[h.brow_ptr[br] + h.diag_pos[br]];
+        std::vector<uint8_t> emap_c;
+        if (stencil) {
+            emap_c.assign(h.emap.size(), 255);
+            for (int e = 0; e < h.n_elem; e++)
+                for (int a = 0; a < 8; a++) {
+                    const int br = h.node_block[h.conn[8 * e + a]];
+                    for (int bb = 0; bb < 8; bb++) {
+                        const uint8_t j = h.emap[(size_t)e * 64 + a * 8 + bb];
+                        if (j != 255 && br >= 0)
+                            emap_c[(size_t)e * 64 + a * 8 + bb] = C.srank[h.brow_ptr[br] + j];
+                    }
+                }
         }
         std::vector<uint8_t> bfree(h.nb);
         for (int br = 0; br < h.nb; br++) bfree[br] = h.node_free[h.block_node[br]];
@@ -236,6 +304,10 @@ int setup_class(msa_batch *b, ClassHost &C, msa_mesh *mesh, int nsamp)
         D.node_free = mesh->d_node_free;
         D.emap = mesh->d_emap;
         D.colour_elems = mesh->d_colour_elems;
+        D.stencil = stencil ? 1 : 0;
+        D.sym = stencil && T == 1 ? 1 : 0;
+        D.sym_sy = sy;
+        D.sym_sz = sz;
         MSA_CHECK(h.ncolours <= kMaxColours, MSA_ERR_ARG, "too many colours");
         for (int k = 0; k <= h.ncolours; k++) D.colour_ptr[k] = h.colour_ptr[k];
         auto A = [&](void **p, size_t bytes) {
@@ -248,9 +320,14 @@ int setup_class(msa_batch *b, ClassHost &C, msa_mesh *mesh, int nsamp)
         A(&p, C.cidx.size() * sizeof(int));
         D.cidx = (const int *)p;
         if (rc == MSA_OK) MSA_HIP(hipMemcpy(p, C.cidx.data(), C.cidx.size() * sizeof(int), hipMemcpyHostToDevice));
-        A(&p, h.diag_pos.size() * sizeof(int));
+        A(&p, dpos.size() * sizeof(int));
         D.diag_pos = (const int *)p;
-        if (rc == MSA_OK) MSA_HIP(hipMemcpy(p, h.diag_pos.data(), h.diag_pos.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (rc == MSA_OK) MSA_HIP(hipMemcpy(p, dpos.data(), dpos.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (stencil) {
+            A(&p, emap_c.size());
+            D.emap = (const uint8_t *)p;
+            if (rc == MSA_OK) MSA_HIP(hipMemcpy(p, emap_c.data(), emap_c.size(), hipMemcpyHostToDevice));
+        }
         A(&p, bfree.size());
         D.block_free = (const uint8_t *)p;
         if (rc == MSA_OK) MSA_HIP(hipMemcpy(p, bfree.data(), bfree.size(), hipMemcpyHostToDevice));
@@ -266,15 +343,15 @@ int setup_class(msa_batch *b, ClassHost &C, msa_mesh *mesh, int nsamp)
                     for (int bb = 0; bb < 8; bb++) {
                         const int j = h.emap[(size_t)e * 64 + a * 8 + bb];
                         if (j == 255) continue;
-                        const size_t sr = (size_t)(C.slice_ptr[br / R] + j) * R + br % R;
+                        const size_t slot = (size_t)C.slot_of(h, br, j);
+                        const size_t sr = slot * R + br % R;
                         const int k = cnt[sr]++;
                         if (k >= 8) {
                             set_error("mesh: more than 8 elements share a node pair");
                             rc = MSA_ERR_ARG;
                             break;
                         }
-                        gmap[((size_t)(C.slice_ptr[br / R] + j) * 8 + k) * R + br % R] =
-                            e * 64 + a * 8 + bb;
+                        gmap[(slot * 8 + k) * R + br % R] = e * 64 + a * 8 + bb;
                     }
                 }
             A(&p, gmap.size() * sizeof(int));
@@ -2867,7 +2944,7 @@ int msa_batch_get_matrix_csr(msa_batch *b, int cls, int sample, long long *rowpt
             if (m.dof[3 * nd + p] >= 0) rows[m.dof[3 * nd + p]] = {nd, p};
     for (int g = 0; g < m.n_free; g++) {
         const int nd = rows[g].first, p = rows[g].second;
-        const int br = m.node_block[nd], sl = br / R, rr = br % R;
+        const int br = m.node_block[nd], rr = br % R;
         for (int j = 0; j < m.brow_ptr[br + 1] - m.brow_ptr[br]; j++) {
             const int bc = m.bcol[m.brow_ptr[br] + j];
             const int mc = m.block_node[bc];
@@ -2875,7 +2952,7 @@ int msa_batch_get_matrix_csr(msa_batch *b, int cls, int sample, long long *rowpt
                 const int gc = m.dof[3 * mc + q];
                 if (gc < 0) continue;
                 col[k] = gc;
-                const size_t slot = (size_t)C.slice_ptr[sl] + j;
+                const size_t slot = (size_t)C.slot_of(m, br, j);
                 // the extracted lane keeps the one-sample-tile layout at
                 // T = 1, and is entry-major per row lane otherwise
                 val[k] = hv[b->T > 1 ? (slot * 9 + 3 * p + q) * R + rr
@@ -2919,11 +2996,11 @@ int msa_batch_set_matrix_csr(msa_batch *b, int cls, int sample, const long long 
     std::vector<long long> where(nf, -1);
     for (int g = 0; g < nf; g++) {
         const int nd = rows[g].first, p = rows[g].second;
-        const int br = m.node_block[nd], sl = br / R, rr = br % R;
+        const int br = m.node_block[nd], rr = br % R;
         const int nbj = m.brow_ptr[br + 1] - m.brow_ptr[br];
         for (int j = 0; j < nbj; j++) {
             const int mc = m.block_node[m.bcol[m.brow_ptr[br] + j]];
-            const size_t slot = (size_t)C.slice_ptr[sl] + j;
+            const size_t slot = (size_t)C.slot_of(m, br, j);
             for (int q = 0; q < 3; q++) {
                 const int gc = m.dof[3 * mc + q];
                 if (gc >= 0)
@@ -3073,6 +3150,17 @@ int msa_batch_graph_info(msa_batch *b, long long *info)
     return MSA_OK;
 }
 
+int msa_batch_layout_info(msa_batch *b, int cls, long long *info)
+{
+    MSA_CHECK(b && info && cls >= 0 && cls < (int)b->cls.size(), MSA_ERR_ARG, "bad arguments");
+    const ClassHost &C = b->cls[cls];
+    info[0] = C.dev.stencil;
+    info[1] = C.dev.sym;
+    info[2] = C.slice_ptr[C.nslices];
+    info[3] = C.dev.nb;
+    return MSA_OK;
+}
+
 int msa_batch_set_integration(msa_batch *b, int precision)
 {
     MSA_CHECK(b, MSA_ERR_ARG, "null batch");
@@ -3113,7 +3201,10 @@ int msa_batch_spmv_bytes(msa_batch *b, double *bytes)
         const MeshHost &m = C.mesh->h;
         const double S = C.nsamp, nnz = (double)m.nnz_scalar, n = 3.0 * m.nb;
         const double idx = 4.0 * (double)m.bcol.size() + 4.0 * (C.nslices + 1);
-        kernel += S * rs * nnz + idx + S * 4.0 * rs * n;
+        if (C.dev.sym)  // the upper triangle with the diagonal; offsets, not indices
+            kernel += S * rs * 0.5 * (nnz + n) + S * 4.0 * rs * n;
+        else
+            kernel += S * rs * nnz + idx + S * 4.0 * rs * n;
         survey += S * rs * nnz + idx + S * 2.0 * rs * n;
     }
     bytes[0] = kernel;

@@ -975,10 +975,105 @@ __global__ __launch_bounds__(256) void k_pcg_init(const ClassDev *__restrict__ c
     block_partials<T, 2>(acc, C.partial + ((size_t)tile * C.nchunks + chunk) * 3 * T);
 }
 
+// Rows [s0, s1) slices of a stencil class (one-sample tiles) in the
+// symmetric product q = K p, p = z + beta p_old gathered per column: the
+// row's 14 upper blocks (slots 13..26, diagonal first) times p at br + off,
+// and the 13 lower blocks as the transposes of the upper blocks stored in
+// the rows br - off (slot s of row j holds K(j, j + off_s)).  The values
+// move once from HBM: a row's upper blocks are loaded through the caches
+// and re-read transposed by the rows above them, which the same and the
+// neighbouring workgroups run at about the same time (forward and backward
+// slots alternate, so the re-reads follow the first reads closely).  Missing
+// neighbours hold zero blocks; gathered columns are clamped into [0, nb)
+// and a row above the first is masked.  Returns the lane's p.q partial.
+// (SkylineMatrix2D.Multiply, SkylineMatrix2D.cs:253-269, likewise applies
+// each stored entry of the upper profile to both triangles, by scattering
+// into the column's rows; here every row gathers, with no write conflicts.)
+template <typename real>
+__device__ __forceinline__ real pcg_a_sym_rows(const ClassDev &C, int s0, int s1, int wave,
+                                               int rr, bool act, real beta,
+                                               const MSA_GLOBAL real *__restrict__ z,
+                                               const MSA_GLOBAL real *__restrict__ po,
+                                               MSA_GLOBAL real *__restrict__ pn,
+                                               MSA_GLOBAL real *__restrict__ q,
+                                               const MSA_GLOBAL real *__restrict__ val)
+{
+    using V3 = vec3<real>;
+    const int sy = C.sym_sy, sz = C.sym_sz, nb1 = C.nb - 1;
+    real pq = 0;
+    for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+        const int br = slice * 64 + rr;
+        if (!(act && br < C.nb)) continue;
+        // task t < 26: for k = t/2 + 1, t even the upper block k of this row,
+        // t odd the transpose of block k of row br - off; t = 26 the diagonal.
+        // Summation: each pair (+off, -off) first, the pairs in order of k,
+        // the diagonal last.  The terms of a row nearly cancel (rigid-body
+        // modes); this order keeps the FP32 rounding of the sum at or below
+        // that of the column-ordered sum, where a diagonal-first order raised
+        // it by about half and cost multigrid-PCG iterations.
+        real a0 = 0, a1 = 0, a2 = 0, f0 = 0, f1 = 0, f2 = 0;
+#pragma unroll 1
+        for (int t0 = 0; t0 < 27; t0 += 3) {
+            real v[3][3], m[3][9];
+            bool bwd[3];
+#pragma unroll
+            for (int u = 0; u < 3; u++) {
+                const int t = t0 + u;
+                const int su = t == 26 ? 13 : 14 + (t >> 1);
+                bwd[u] = t < 26 && (t & 1) != 0;
+                const int off = (su % 3 - 1) + sy * ((su / 3) % 3 - 1) + sz * (su / 9 - 1);
+                const int row = bwd[u] ? br - off : br;
+                const int rowc = max(row, 0);
+                const int col = bwd[u] ? rowc : min(br + off, nb1);
+                const V3 zc = ld3(z, col), pc = ld3(po, col);
+                const bool live = row >= 0;
+                v[u][0] = live ? zc.x + beta * pc.x : (real)0;
+                v[u][1] = live ? zc.y + beta * pc.y : (real)0;
+                v[u][2] = live ? zc.z + beta * pc.z : (real)0;
+                load_block<real, 1, false>(val, (size_t)(rowc >> 6) * 27 + su, rowc & 63, 0, m[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < 3; u++) {
+                const int t = t0 + u;
+                real r0, r1, r2;
+                if (bwd[u]) {
+                    r0 = m[u][0] * v[u][0] + m[u][3] * v[u][1] + m[u][6] * v[u][2];
+                    r1 = m[u][1] * v[u][0] + m[u][4] * v[u][1] + m[u][7] * v[u][2];
+                    r2 = m[u][2] * v[u][0] + m[u][5] * v[u][1] + m[u][8] * v[u][2];
+                } else {
+                    r0 = m[u][0] * v[u][0] + m[u][1] * v[u][1] + m[u][2] * v[u][2];
+                    r1 = m[u][3] * v[u][0] + m[u][4] * v[u][1] + m[u][5] * v[u][2];
+                    r2 = m[u][6] * v[u][0] + m[u][7] * v[u][1] + m[u][8] * v[u][2];
+                }
+                if (t == 26) {  // the diagonal
+                    a0 += r0;
+                    a1 += r1;
+                    a2 += r2;
+                } else if (!bwd[u]) {  // upper block k: held until its pair
+                    f0 = r0;
+                    f1 = r1;
+                    f2 = r2;
+                } else {
+                    a0 += f0 + r0;
+                    a1 += f1 + r1;
+                    a2 += f2 + r2;
+                }
+            }
+        }
+        const V3 zn = ld3(z, br), pnv = ld3(po, br);
+        const real n0 = zn.x + beta * pnv.x, n1 = zn.y + beta * pnv.y, n2 = zn.z + beta * pnv.z;
+        st3(pn, br, n0, n1, n2);
+        st3(q, br, a0, a1, a2);
+        pq += n0 * a0 + n1 * a1 + n2 * a2;
+    }
+    return pq;
+}
+
 // Search direction + product + curvature (SolverPCGSimpleSearchVectorCalculator
 // .cs:18-41, MCPCGDirect.cs:48-51): p = z + beta p_old on the fly for every
 // gathered column, q = K p, own rows of p written to the other buffer,
 // pq = p.q per sample; the last workgroup of each tile sets alpha = rz/pq.
+// Stencil classes of one-sample tiles take the symmetric product above.
 // U: slots per group of a row (the loads of a group issue together; the
 // schedule the autotuner picks); every U sums the slots in the same order.
 template <typename real, int T, int U = 3>
@@ -1025,7 +1120,11 @@ __device__ __forceinline__ void pcg_a_item(const ClassDev *__restrict__ cls, con
     // stiffness values are streamed once per product: load them
     // non-temporally so L2 keeps the reused vector gathers
     auto block = [&](int slot, real (&m)[9]) { load_block<real, T, true>(val, slot, rr, s, m); };
-    if (__ballot(act)) {
+    bool sym = false;
+    if constexpr (T == 1) sym = C.sym != 0;
+    if (sym) {
+        if (__ballot(act)) pq = pcg_a_sym_rows<real>(C, s0, s1, wave, rr, act, beta, z, po, pn, q, val);
+    } else if (__ballot(act)) {
         for (int slice = s0 + wave; slice < s1; slice += kWaves) {
             const int br = slice * R + rr;
             const int base = sptr[slice];

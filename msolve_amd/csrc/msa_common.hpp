@@ -142,6 +142,14 @@ struct ClassDev {
     long long tile_vals, tile_vec;
     double fnorm, c0;
     const int *slice_ptr, *cidx, *diag_pos;
+    // box grids (blocks = the points of a box, numbered lexicographically
+    // with strides 1, sym_sy, sym_sz): stencil slot order, 27 slots per
+    // row, slot s = (dz+1)*9 + (dy+1)*3 + (dx+1) holding the neighbour at
+    // offset (dx, dy, dz) or a zero block, so the diagonal is slot 13 and
+    // slots 13..26 hold the upper triangle.  sym: one-sample tiles whose PCG
+    // product reads only those 14 slots (the lower blocks are the
+    // transposes of upper blocks stored in the rows above).
+    int stencil, sym, sym_sy, sym_sz;
     const uint8_t *block_free;
     const double *xyz;
     const int *conn, *node_block;
