@@ -108,6 +108,8 @@ def parse():
     p.add_argument("--no-c2", action="store_true", help="skip the 64^3 FP64 single solve")
     p.add_argument("--no-c3", action="store_true", help="skip the 256 x 32^3 FP64 batch")
     p.add_argument("--no-c5", action="store_true", help="skip the batched Newmark dynamics")
+    p.add_argument("--no-persistent", action="store_true",
+                   help="skip the cooperative persistent-launch runs of the C2/C3 legs")
     p.add_argument("--only", choices=["c2", "c3", "c5"], default=None,
                    help="run only this comparison workload (profiling)")
     p.add_argument("--newmark", type=int, default=0, metavar="STEPS",
@@ -366,9 +368,12 @@ def oracle_true_residual(O, model, E, u, nu=0.3):
 
 
 # ------------------------------------------------------------------ legs
-def persistent_run(b, ctx, kw, ref):
+def persistent_run(b, ctx, kw, ref, args):
     """The same solve as one cooperative persistent launch: wall time of one
-    run (after one warm-up) and whether it gives the graph path's bits."""
+    run (after one warm-up) and whether it gives the graph path's bits
+    (None with --no-persistent)."""
+    if args.no_persistent:
+        return None
     b.set_persistent(True)
     b.run(**kw)
     ctx.synchronize()
@@ -410,7 +415,7 @@ def leg_c3(msa, ctx, args):
     roof = spmv_roofline(b, kt, 8, spmv_name(b, "double", 1) + ", 256 x 32^3",
                          "k_pcg_a_fp64_counters.json")
     u = {s: b.solution(0, s) for s in (0, ns - 1)}
-    pers = persistent_run(b, ctx, kw, u)
+    pers = persistent_run(b, ctx, kw, u, args)
     back_to_back(roof, b)
     # independent check of two samples: the oracle's own CSR operator
     otab = O.kl_table([1.0, 1.0, 1.0], args.corr, args.terms)
@@ -428,8 +433,9 @@ def leg_c3(msa, ctx, args):
                oracle_true_residual=res, roofline=roof, persistent_launch=pers,
                kernel_timing_ms={k: v["ms"] for k, v in kt.items()})
     log(f"[bench] C3 256x32^3 FP64: {el:.2f} s -> {ns / el:.1f} solves/s, "
-        f"k_pcg_a<double> {roof['achieved']:.0f} GB/s, residuals {res}; persistent launch "
-        f"{pers['s_per_solve']:.2f} s (same bits: {pers['same_bits_as_graph']})")
+        f"k_pcg_a<double> {roof['achieved']:.0f} GB/s, residuals {res}" +
+        (f"; persistent launch {pers['s_per_solve']:.2f} s (same bits: "
+         f"{pers['same_bits_as_graph']})" if pers else ""))
     return out
 
 
@@ -458,7 +464,7 @@ def leg_c2(msa, ctx, args):
     roof = spmv_roofline(b, kt, 8, spmv_name(b, "double", 1) + ", one 64^3 sample",
                          "k_pcg_a_fp64_counters.json")
     u = b.solution(0, 0)
-    pers = persistent_run(b, ctx, kw, {0: u})
+    pers = persistent_run(b, ctx, kw, {0: u}, args)
     back_to_back(roof, b)
     res = oracle_true_residual(O, model, E, u)
     b.close()
@@ -469,8 +475,9 @@ def leg_c2(msa, ctx, args):
                oracle_true_residual=res, roofline=roof, persistent_launch=pers,
                kernel_timing_ms={k: v["ms"] for k, v in kt.items()})
     log(f"[bench] C2 64^3 FP64: {el:.3f} s, {int(st['iters'][0])} its, k_pcg_a<double> "
-        f"{roof['achieved']:.0f} GB/s, oracle residual {res:.2e}; persistent launch "
-        f"{pers['s_per_solve']:.3f} s (same bits: {pers['same_bits_as_graph']})")
+        f"{roof['achieved']:.0f} GB/s, oracle residual {res:.2e}" +
+        (f"; persistent launch {pers['s_per_solve']:.3f} s (same bits: "
+         f"{pers['same_bits_as_graph']})" if pers else ""))
     return out
 
 
