@@ -975,20 +975,57 @@ __global__ __launch_bounds__(256) void k_pcg_init(const ClassDev *__restrict__ c
     block_partials<T, 2>(acc, C.partial + ((size_t)tile * C.nchunks + chunk) * 3 * T);
 }
 
+// Whole-wave DPP shifts of one value: lane l takes lane l - 1's (shr) or
+// l + 1's (shl) src; lane 0 (shr) / 63 (shl) keeps old.  Doubles move as
+// two 32-bit halves.
+template <int CTRL, typename real>
+__device__ __forceinline__ real dpp_wave(real old, real src)
+{
+    if constexpr (sizeof(real) == 4) {
+        return __builtin_bit_cast(real, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
+                                                                    __builtin_bit_cast(int, src),
+                                                                    CTRL, 0xf, 0xf, false));
+    } else {
+        const unsigned long long o = __builtin_bit_cast(unsigned long long, old);
+        const unsigned long long v = __builtin_bit_cast(unsigned long long, src);
+        const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)v, CTRL, 0xf,
+                                                   0xf, false);
+        const int hi = __builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32), (int)(unsigned)(v >> 32),
+                                                   CTRL, 0xf, 0xf, false);
+        return __builtin_bit_cast(real, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+    }
+}
+constexpr int kDppWaveShl1 = 0x130, kDppWaveShr1 = 0x138;
+
 // Rows [s0, s1) slices of a stencil class (one-sample tiles) in the
-// symmetric product q = K p, p = z + beta p_old gathered per column: the
-// row's 14 upper blocks (slots 13..26, diagonal first) times p at br + off,
-// and the 13 lower blocks as the transposes of the upper blocks stored in
-// the rows br - off (slot s of row j holds K(j, j + off_s)).  The values
-// move once from HBM: a row's upper blocks are loaded through the caches
-// and re-read transposed by the rows above them, which the same and the
-// neighbouring workgroups run at about the same time (forward and backward
-// slots alternate, so the re-reads follow the first reads closely).  Missing
-// neighbours hold zero blocks; gathered columns are clamped into [0, nb)
-// and a row above the first is masked.  Returns the lane's p.q partial.
-// (SkylineMatrix2D.Multiply, SkylineMatrix2D.cs:253-269, likewise applies
-// each stored entry of the upper profile to both triangles, by scattering
-// into the column's rows; here every row gathers, with no write conflicts.)
+// symmetric product q = K p, p = z + beta p_old: every row's 14 upper
+// blocks (slots 13..26) times p at br + off, and its 13 lower blocks as the
+// transposes of the upper blocks stored in the rows br - off (slot s of row
+// j holds K(j, j + off_s)).  The values move once from HBM: a row's upper
+// blocks are loaded through the caches and re-read transposed by the rows
+// above them, which the same and the neighbouring workgroups run at about
+// the same time.  (SkylineMatrix2D.Multiply, SkylineMatrix2D.cs:253-269,
+// likewise applies each stored entry of the upper profile to both
+// triangles, by scattering into the column's rows; here every row gathers,
+// with no write conflicts.)
+//
+// The 27 positions a row needs, forward and backward, are its 3x3x3
+// stencil: stencil position n = (dz+1)*9 + (dy+1)*3 + dx+1 at row
+// br + off(n); n > 13 is upper slot n of this row, n < 13 the transpose of
+// upper slot 26 - n of row br + off(n), n = 13 the diagonal.  Lane l holds
+// row br = 64 * slice + l, so each lane loads z and p_old only at its own x
+// for the 9 (dy, dz) columns and takes x - 1 / x + 1 from lanes l - 1 / l + 1
+// by DPP shifts (lanes 0 / 63 load theirs): 18 gathers per row instead of
+// 54.  Column m and its mirror 8 - m go together, m from 4 down to 0.
+// Summation: each (+off, -off) pair first, the pairs in ascending distance
+// k (n = 13 +- k), the diagonal last.  A row's terms nearly cancel
+// (rigid-body modes); this order keeps the FP32 rounding of the sum at or
+// below that of the column-ordered sum, where a diagonal-first order raised
+// it by about half and cost multigrid-PCG iterations.  The whole wave runs
+// every slice (DPP needs every lane): rows past nb compute on the last row
+// and store nothing.  Missing neighbours hold zero blocks; positions
+// outside [0, nb) are clamped (their blocks are zero) and a backward row
+// above the first is masked.  Returns the lane's p.q partial.
 template <typename real>
 __device__ __forceinline__ real pcg_a_sym_rows(const ClassDev &C, int s0, int s1, int wave,
                                                int rr, bool act, real beta,
@@ -1001,70 +1038,88 @@ __device__ __forceinline__ real pcg_a_sym_rows(const ClassDev &C, int s0, int s1
     using V3 = vec3<real>;
     const int sy = C.sym_sy, sz = C.sym_sz, nb1 = C.nb - 1;
     real pq = 0;
+    if (!act) return pq;  // one sample per tile: uniform over the wave
     for (int slice = s0 + wave; slice < s1; slice += kWaves) {
         const int br = slice * 64 + rr;
-        if (!(act && br < C.nb)) continue;
-        // task t < 26: for k = t/2 + 1, t even the upper block k of this row,
-        // t odd the transpose of block k of row br - off; t = 26 the diagonal.
-        // Summation: each pair (+off, -off) first, the pairs in order of k,
-        // the diagonal last.  The terms of a row nearly cancel (rigid-body
-        // modes); this order keeps the FP32 rounding of the sum at or below
-        // that of the column-ordered sum, where a diagonal-first order raised
-        // it by about half and cost multigrid-PCG iterations.
-        real a0 = 0, a1 = 0, a2 = 0, f0 = 0, f1 = 0, f2 = 0;
-#pragma unroll 1
-        for (int t0 = 0; t0 < 27; t0 += 3) {
-            real v[3][3], m[3][9];
-            bool bwd[3];
-#pragma unroll
-            for (int u = 0; u < 3; u++) {
-                const int t = t0 + u;
-                const int su = t == 26 ? 13 : 14 + (t >> 1);
-                bwd[u] = t < 26 && (t & 1) != 0;
-                const int off = (su % 3 - 1) + sy * ((su / 3) % 3 - 1) + sz * (su / 9 - 1);
-                const int row = bwd[u] ? br - off : br;
-                const int rowc = max(row, 0);
-                const int col = bwd[u] ? rowc : min(br + off, nb1);
-                const V3 zc = ld3(z, col), pc = ld3(po, col);
-                const bool live = row >= 0;
-                v[u][0] = live ? zc.x + beta * pc.x : (real)0;
-                v[u][1] = live ? zc.y + beta * pc.y : (real)0;
-                v[u][2] = live ? zc.z + beta * pc.z : (real)0;
-                load_block<real, 1, false>(val, (size_t)(rowc >> 6) * 27 + su, rowc & 63, 0, m[u]);
+        const int brc = min(br, nb1);
+        // p at stencil column m = (dy+1) + 3 (dz+1): c[0] at x - 1, c[1] at x,
+        // c[2] at x + 1
+        auto column = [&](int m, real (&c)[3][3]) {
+            const int mo = sy * (m % 3 - 1) + sz * (m / 3 - 1);
+            const V3 zc = ld3(z, min(max(brc + mo, 0), nb1));
+            const V3 pc = ld3(po, min(max(brc + mo, 0), nb1));
+            const real own[3] = {zc.x + beta * pc.x, zc.y + beta * pc.y, zc.z + beta * pc.z};
+            real edge[3] = {0, 0, 0};
+            if (rr == 0 || rr == 63) {
+                const int r1 = min(max(brc + mo + (rr == 0 ? -1 : 1), 0), nb1);
+                const V3 ze = ld3(z, r1), pe = ld3(po, r1);
+                edge[0] = ze.x + beta * pe.x;
+                edge[1] = ze.y + beta * pe.y;
+                edge[2] = ze.z + beta * pe.z;
             }
 #pragma unroll
-            for (int u = 0; u < 3; u++) {
-                const int t = t0 + u;
-                real r0, r1, r2;
-                if (bwd[u]) {
-                    r0 = m[u][0] * v[u][0] + m[u][3] * v[u][1] + m[u][6] * v[u][2];
-                    r1 = m[u][1] * v[u][0] + m[u][4] * v[u][1] + m[u][7] * v[u][2];
-                    r2 = m[u][2] * v[u][0] + m[u][5] * v[u][1] + m[u][8] * v[u][2];
-                } else {
-                    r0 = m[u][0] * v[u][0] + m[u][1] * v[u][1] + m[u][2] * v[u][2];
-                    r1 = m[u][3] * v[u][0] + m[u][4] * v[u][1] + m[u][5] * v[u][2];
-                    r2 = m[u][6] * v[u][0] + m[u][7] * v[u][1] + m[u][8] * v[u][2];
-                }
-                if (t == 26) {  // the diagonal
-                    a0 += r0;
-                    a1 += r1;
-                    a2 += r2;
-                } else if (!bwd[u]) {  // upper block k: held until its pair
-                    f0 = r0;
-                    f1 = r1;
-                    f2 = r2;
-                } else {
-                    a0 += f0 + r0;
-                    a1 += f1 + r1;
-                    a2 += f2 + r2;
-                }
+            for (int d = 0; d < 3; d++) {
+                c[1][d] = own[d];
+                c[0][d] = dpp_wave<kDppWaveShr1>(edge[d], own[d]);
+                c[2][d] = dpp_wave<kDppWaveShl1>(edge[d], own[d]);
+            }
+        };
+        // the block of stencil position n times v (transposed for n < 13)
+        auto term = [&](int n, const real (&v)[3], real (&r)[3]) {
+            const int su = n >= 13 ? n : 26 - n;
+            const int off = (n % 3 - 1) + sy * ((n / 3) % 3 - 1) + sz * (n / 9 - 1);
+            const int j = n >= 13 ? brc : brc + off;
+            const int jc = max(j, 0);
+            real m[9];
+            load_block<real, 1, false>(val, (size_t)(jc >> 6) * 27 + su, jc & 63, 0, m);
+            const real w0 = j >= 0 ? v[0] : (real)0, w1 = j >= 0 ? v[1] : (real)0,
+                       w2 = j >= 0 ? v[2] : (real)0;
+            if (n < 13) {
+                r[0] = m[0] * w0 + m[3] * w1 + m[6] * w2;
+                r[1] = m[1] * w0 + m[4] * w1 + m[7] * w2;
+                r[2] = m[2] * w0 + m[5] * w1 + m[8] * w2;
+            } else {
+                r[0] = m[0] * w0 + m[1] * w1 + m[2] * w2;
+                r[1] = m[3] * w0 + m[4] * w1 + m[5] * w2;
+                r[2] = m[6] * w0 + m[7] * w1 + m[8] * w2;
+            }
+        };
+        real a[3], d[3], own[3];
+        {
+            real c[3][3], t12[3], t14[3];
+            column(4, c);
+            term(14, c[2], t14);
+            term(12, c[0], t12);
+            term(13, c[1], d);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                a[k] = 0;
+                a[k] += t14[k] + t12[k];
+                own[k] = c[1][k];
             }
         }
-        const V3 zn = ld3(z, br), pnv = ld3(po, br);
-        const real n0 = zn.x + beta * pnv.x, n1 = zn.y + beta * pnv.y, n2 = zn.z + beta * pnv.z;
-        st3(pn, br, n0, n1, n2);
-        st3(q, br, a0, a1, a2);
-        pq += n0 * a0 + n1 * a1 + n2 * a2;
+#pragma unroll 1
+        for (int m = 3; m >= 0; m--) {
+            real cl[3][3], ch[3][3], rl[3][3], rh[3][3];
+            column(m, cl);
+            column(8 - m, ch);
+#pragma unroll
+            for (int dx = 2; dx >= 0; dx--) {
+                term(3 * m + dx, cl[dx], rl[dx]);
+                term(26 - 3 * m - dx, ch[2 - dx], rh[dx]);
+            }
+#pragma unroll
+            for (int dx = 2; dx >= 0; dx--)
+#pragma unroll
+                for (int k = 0; k < 3; k++) a[k] += rh[dx][k] + rl[dx][k];
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) a[k] += d[k];
+        if (br < C.nb) {
+            st3(pn, br, own[0], own[1], own[2]);
+            st3(q, br, a[0], a[1], a[2]);
+            pq += own[0] * a[0] + own[1] * a[1] + own[2] * a[2];
+        }
     }
     return pq;
 }
