@@ -418,8 +418,8 @@ int msa_batch_set_matrix_cores(msa_batch *b, int enable);
 int msa_batch_mg_info(msa_batch *b, int cls, long long *info);
 /* Upper end lam of each level's Chebyshev smoothing interval [lam/ratio,
  * lam] (no counterpart in the reference, whose PCG preconditioner is Jacobi
- * or the first sample's factor): with lanczos_steps > 0 (default 6,
- * margin 1.2) lam = min(Gershgorin bound, margin x the top Ritz value of
+ * or the first sample's factor): with lanczos_steps > 0 (default 4,
+ * margin 1.3) lam = min(Gershgorin bound, margin x the top Ritz value of
  * lanczos_steps Lanczos steps on K x = lambda D x), per sample and level,
  * formed with the preconditioner's values; 0 keeps the Gershgorin bound.
  * The Ritz value converges to the top of the spectrum from below, so the

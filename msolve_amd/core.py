@@ -295,7 +295,7 @@ class Batch:
                     coarsest_bandwidth=int(info[3]), matrix_free=bool(info[4]),
                     matrix_free_coarse_levels=int(info[5]))
 
-    def set_smoothing_interval(self, lanczos_steps: int = 6, margin: float = 1.2):
+    def set_smoothing_interval(self, lanczos_steps: int = 4, margin: float = 1.3):
         """Upper end of the Chebyshev smoothing interval: Lanczos estimate
         (lanczos_steps > 0) or the Gershgorin bound (0)."""
         check(lib().msa_batch_set_smoothing_interval(self.h, lanczos_steps, margin))

@@ -129,8 +129,8 @@ struct msa_batch {
     // geometric multigrid preconditioner (msa_batch_set_preconditioner)
     int precond = MSA_PRECOND_JACOBI;
     double mg_ratio = 12.0;             // smoothing interval [lam/ratio, lam]
-    int mg_lz_steps = 6;                // Lanczos steps of lam (0: the Gershgorin bound)
-    double mg_lz_margin = 1.2;          // lam = min(Gershgorin, margin x top Ritz value)
+    int mg_lz_steps = 4;                // Lanczos steps of lam (0: the Gershgorin bound)
+    double mg_lz_margin = 1.3;          // lam = min(Gershgorin, margin x top Ritz value)
     int *d_lz_list = nullptr;           // the smoothed classes (device), for the Lanczos sums
     std::vector<int> lz_list;
     std::vector<ClassHost> lv;          // coarse levels: class table entry cls.size() + i
