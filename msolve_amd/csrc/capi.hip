@@ -362,6 +362,9 @@ int setup_class(msa_batch *b, ClassHost &C, msa_mesh *mesh, int nsamp)
         }
         const size_t nvec = (size_t)C.ntiles * C.tile_vec;
         A(&D.val, (size_t)C.ntiles * C.tile_vals * rs);
+        // upper-triangle classes never write their lower slots: zeros, so
+        // that a consumer reading a padding slot sees a zero block
+        if (rc == MSA_OK && D.sym) MSA_HIP(hipMemset(D.val, 0, (size_t)C.ntiles * C.tile_vals * rs));
         A(&D.x, nvec * rs);
         A(&D.r, nvec * rs);
         A(&D.z, nvec * rs);
@@ -1116,6 +1119,8 @@ int mg_build(msa_batch *b, int max_levels)
                     void *p = nullptr;
                     MSA_TRY(dalloc(b, &p, (size_t)Cc.ntiles * Cc.tile_vals * sizeof(double)));
                     Cc.dev.val64 = (double *)p;
+                    if (Cc.dev.sym)  // lower slots are not written (as the FP32 values)
+                        MSA_HIP(hipMemset(p, 0, (size_t)Cc.ntiles * Cc.tile_vals * sizeof(double)));
                 }
                 // small coarsest operators: dense inverse, one launch per cycle
                 if (Cc.dev.bn <= kDenseMax) {
@@ -2952,12 +2957,21 @@ int msa_batch_get_matrix_csr(msa_batch *b, int cls, int sample, long long *rowpt
                 const int gc = m.dof[3 * mc + q];
                 if (gc < 0) continue;
                 col[k] = gc;
-                const size_t slot = (size_t)C.slot_of(m, br, j);
+                size_t slot = (size_t)C.slot_of(m, br, j);
+                int rl = rr, e = 3 * p + q;
+                // upper-triangle classes: a lower block is the transpose of
+                // the upper block of its column's row
+                const int sj = C.srank[m.brow_ptr[br] + j];
+                if (C.dev.sym && sj < 13) {
+                    slot = (size_t)C.slice_ptr[bc / R] + (26 - sj);
+                    rl = bc % R;
+                    e = 3 * q + p;
+                }
                 // the extracted lane keeps the one-sample-tile layout at
                 // T = 1, and is entry-major per row lane otherwise
-                val[k] = hv[b->T > 1 ? (slot * 9 + 3 * p + q) * R + rr
-                            : (rs == 4 ? val_at<float, 1>(slot, 3 * p + q, rr, 0)
-                                       : val_at<double, 1>(slot, 3 * p + q, rr, 0))];
+                val[k] = hv[b->T > 1 ? (slot * 9 + e) * R + rl
+                            : (rs == 4 ? val_at<float, 1>(slot, e, rl, 0)
+                                       : val_at<double, 1>(slot, e, rl, 0))];
                 k++;
             }
         }

@@ -158,6 +158,43 @@ __device__ __forceinline__ void store_block(MSA_GLOBAL real *val, size_t slot, i
     b[512 + rr] = m[8];
 }
 
+// Stored block j of a block row (slot base + j) whose block column is col.
+// Classes that keep only the upper triangle (ClassDev::sym: stencil order,
+// one-sample tiles) do not store their lower slots: lower slot j < 13 is
+// read as the transpose of upper slot 26 - j of row col (which holds
+// K(col, br)).  col < 0 (padding) reads the slot's own zeros.
+template <typename real, int T, bool NT = false>
+__device__ __forceinline__ void load_row_block(const ClassDev &C, const MSA_GLOBAL real *val,
+                                               size_t base, int j, int col, int rr, int s,
+                                               real (&m)[9])
+{
+    if constexpr (T == 1) {
+        if (C.sym && j < 13 && col >= 0) {
+            real t[9];
+            load_block<real, 1, NT>(val, (size_t)(col >> 6) * 27 + (26 - j), col & 63, 0, t);
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+#pragma unroll
+                for (int q = 0; q < 3; q++) m[3 * p + q] = t[3 * q + p];
+            return;
+        }
+    }
+    load_block<real, T, NT>(val, base + j, rr, s, m);
+}
+
+// Position of entry (p, q) of block j of row (rr, s) in a tile's values,
+// with the lower-slot rule of load_row_block.
+template <typename real, int T>
+__device__ __forceinline__ size_t row_entry(const ClassDev &C, size_t base, int j, int col, int rr,
+                                            int s, int p, int q)
+{
+    if constexpr (T == 1) {
+        if (C.sym && j < 13 && col >= 0)
+            return val_at<real, 1>((size_t)(col >> 6) * 27 + (26 - j), 3 * q + p, col & 63, 0);
+    }
+    return val_at<real, T>(base + j, 3 * p + q, rr, s);
+}
+
 template <typename real>
 __device__ __forceinline__ void st3(MSA_GLOBAL real *p, size_t i, real x, real y, real z)
 {
@@ -790,12 +827,15 @@ __global__ __launch_bounds__(256) void k_assemble_gather_g(const ClassDev *__res
         __syncthreads();
     }
     const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+    // upper-triangle classes store slots 13..26 only (load_row_block reads
+    // a lower slot from the upper one of its column's row)
+    const int j0 = C.sym ? 13 : 0;
     for (int slice = s0 + wave; slice < s1; slice += kWaves) {
         const int br = slice * R + rr;
         if (br >= C.nb) continue;
         const int base = C.slice_ptr[slice], W = C.slice_ptr[slice + 1] - base;
         const uint8_t fr = C.block_free[br];
-        for (int j = 0; j < W; j++) {
+        for (int j = j0; j < W; j++) {
             const int slot = base + j;
             const int col = cidx[(size_t)slot * R + rr];
             double acc[G][9];
@@ -1357,7 +1397,7 @@ __global__ __launch_bounds__(256) void k_spmv(const ClassDev *__restrict__ cls,
             const size_t xo = (size_t)col * 3 * T + s;
             const double v0 = xin[xo], v1 = xin[xo + T], v2 = xin[xo + 2 * T];
             real m[9];
-            load_block<real, T>(gbl(val), slot, rr, s, m);
+            load_row_block<real, T>(C, gbl(val), base, j, col, rr, s, m);
             a0 += (double)m[0] * v0 + (double)m[1] * v1 + (double)m[2] * v2;
             a1 += (double)m[3] * v0 + (double)m[4] * v1 + (double)m[5] * v2;
             a2 += (double)m[6] * v0 + (double)m[7] * v1 + (double)m[8] * v2;
@@ -2050,7 +2090,7 @@ __global__ __launch_bounds__(256) void k_pcg_init_warm(const ClassDev *__restric
                 const size_t xo = (size_t)col * 3 * T + s;
                 const double v0 = x[xo], v1 = x[xo + T], v2 = x[xo + 2 * T];
                 double m[9];
-                load_block<double, T>(gbl(val), slot, rr, s, m);
+                load_row_block<double, T>(C, gbl(val), base, j, col, rr, s, m);
                 kx[0] += m[0] * v0 + m[1] * v1 + m[2] * v2;
                 kx[1] += m[3] * v0 + m[4] * v1 + m[5] * v2;
                 kx[2] += m[6] * v0 + m[7] * v1 + m[8] * v2;
@@ -2163,8 +2203,8 @@ __global__ __launch_bounds__(256) void k_band_scatter(const ClassDev *__restrict
                     const int jj = C.binv[3 * col + q];
                     if (jj < 0 || jj > i) continue;
                     AB[(size_t)jj * C.bW + (i - jj)] =
-                        C.val64 ? C.val64[(size_t)tile * C.tile_vals + val_at<double, T>(slot, 3 * p + q, rr, s)]
-                                : (double)val[val_at<real, T>(slot, 3 * p + q, rr, s)];
+                        C.val64 ? C.val64[(size_t)tile * C.tile_vals + row_entry<double, T>(C, base, j, col, rr, s, p, q)]
+                                : (double)val[row_entry<real, T>(C, base, j, col, rr, s, p, q)];
                 }
             }
         }
@@ -2990,11 +3030,13 @@ __device__ __forceinline__ void gather3(const MSA_GLOBAL real *v, int col, int s
 }
 
 // y = K v for block row br (slots [base, base + W) of its slice): groups of
-// three slots, padding columns clamped to block 0 (their values are zero).
+// three slots, padding columns clamped to block 0 (their values are zero);
+// lower slots of upper-triangle classes by load_row_block.
 template <typename real, int T>
-__device__ __forceinline__ void row_product(const MSA_GLOBAL real *val, const MSA_GLOBAL int *cidx,
-                                            const MSA_GLOBAL real *v, int base, int W, int rr,
-                                            int s, real &a0, real &a1, real &a2)
+__device__ __forceinline__ void row_product(const ClassDev &C, const MSA_GLOBAL real *val,
+                                            const MSA_GLOBAL int *cidx, const MSA_GLOBAL real *v,
+                                            int base, int W, int rr, int s, real &a0, real &a1,
+                                            real &a2)
 {
     constexpr int R = 64 / T;
     a0 = a1 = a2 = 0;
@@ -3002,12 +3044,12 @@ __device__ __forceinline__ void row_product(const MSA_GLOBAL real *val, const MS
     for (; j + 3 <= W; j += 3) {
         int col[3];
 #pragma unroll
-        for (int u = 0; u < 3; u++) col[u] = max(cidx[(size_t)(base + j + u) * R + rr], 0);
+        for (int u = 0; u < 3; u++) col[u] = cidx[(size_t)(base + j + u) * R + rr];
         real x[3][3], m[3][9];
 #pragma unroll
         for (int u = 0; u < 3; u++) {
-            gather3<real, T>(v, col[u], s, x[u][0], x[u][1], x[u][2]);
-            load_block<real, T, true>(val, base + j + u, rr, s, m[u]);
+            gather3<real, T>(v, max(col[u], 0), s, x[u][0], x[u][1], x[u][2]);
+            load_row_block<real, T, true>(C, val, base, j + u, col[u], rr, s, m[u]);
         }
 #pragma unroll
         for (int u = 0; u < 3; u++) {
@@ -3017,10 +3059,10 @@ __device__ __forceinline__ void row_product(const MSA_GLOBAL real *val, const MS
         }
     }
     for (; j < W; j++) {
-        const int col = max(cidx[(size_t)(base + j) * R + rr], 0);
+        const int col = cidx[(size_t)(base + j) * R + rr];
         real x0, x1, x2, m[9];
-        gather3<real, T>(v, col, s, x0, x1, x2);
-        load_block<real, T, true>(val, base + j, rr, s, m);
+        gather3<real, T>(v, max(col, 0), s, x0, x1, x2);
+        load_row_block<real, T, true>(C, val, base, j, col, rr, s, m);
         a0 += m[0] * x0 + m[1] * x1 + m[2] * x2;
         a1 += m[3] * x0 + m[4] * x1 + m[5] * x2;
         a2 += m[6] * x0 + m[7] * x1 + m[8] * x2;
@@ -3522,7 +3564,7 @@ __global__ __launch_bounds__(256) void k_mg_smooth(const ClassDev *__restrict__ 
                 mf_product<MF == 2>(C, (const MSA_GLOBAL float *)src, C.E + (size_t)tile * C.n_elem,
                                     br, y[0], y[1], y[2]);
             else
-                row_product<real, T>(val, cidx, src, base, W, rr, s, y[0], y[1], y[2]);
+                row_product<real, T>(C, val, cidx, src, base, W, rr, s, y[0], y[1], y[2]);
             if constexpr (T == 1) {
                 mg_update1<real, STEP>(b, dinv, z, A, B, dst, br, y, ith, c1, c2, rz);
                 continue;
@@ -4337,7 +4379,8 @@ __global__ __launch_bounds__(256) void k_mg_gersh(const ClassDev *__restrict__ c
             double sum[3] = {0, 0, 0};
             for (int j = 0; j < W; j++) {
                 real m[9];
-                load_block<real, T, true>(gbl(val), base + j, rr, s, m);
+                load_row_block<real, T, true>(C, gbl(val), base, j, C.cidx[(size_t)(base + j) * R + rr],
+                                              rr, s, m);
 #pragma unroll
                 for (int p = 0; p < 3; p++)
                     sum[p] += fabs((double)m[3 * p]) + fabs((double)m[3 * p + 1]) +
