@@ -283,7 +283,10 @@ int msa_batch_get_matrix_csr(msa_batch *b, int cls, int sample,
  * outside it, such as a skyline profile's, are skipped); duplicates add.
  * The sample's other stored values become zero; the first call on a batch
  * never assembled zeroes every sample, so every sample must be set before
- * a solve.  D^-1 is refreshed; the preconditioner follows at the solve. */
+ * a solve.  D^-1 is refreshed; the preconditioner follows at the solve.
+ * On box-grid classes with one-sample tiles (msa_batch_layout_info) the
+ * products use the upper triangle of the loaded matrix, as the skyline
+ * image it came from holds only the upper profile. */
 int msa_batch_set_matrix_csr(msa_batch *b, int cls, int sample,
                              const long long *rowptr, const int *col,
                              const double *val);
