@@ -241,20 +241,20 @@ def smooth_bytes(b, models, rs):
     return total
 
 
-def kernel_a_bytes(b, rs):
+def kernel_a_bytes(b, rs, full=False):
     """Algorithmic bytes moved by all SpMV+dot (k_pcg_a) launches of one
     solve: per active sample and iteration the matrix values and the four
     vectors (z, p_old read; p_new, q written); the shared block-column
     indices and slice pointers once per launch of a class with active
     samples.  Classes whose product reads the upper triangle (box grids,
-    one-sample tiles) count its values, diagonal included, and no
-    indices."""
+    one-sample tiles) count its values, diagonal included, and no indices;
+    full=True prices every class at the full-storage bytes."""
     total = 0.0
     for c, mesh in enumerate(b.meshes):
         st = b.stats(c)
         nnz, n = mesh.info["nnz"], 3 * mesh.info["blocks"]
         its = st["iters"].astype(np.float64)
-        if b.layout_info(c)["upper_triangle_product"]:
+        if b.layout_info(c)["upper_triangle_product"] and not full:
             total += its.sum() * (rs * 0.5 * (nnz + n) + 4 * rs * n)
             continue
         total += its.sum() * (rs * nnz + 4 * rs * n)
@@ -274,7 +274,7 @@ def counters(name):
                                                      cj["algorithmic_write"])
 
 
-def back_to_back(roof, b, reps=20):
+def back_to_back(roof, b, reps=20, rs=8):
     """Adds the SpMV+dot kernel timed over `reps` back-to-back launches with
     every sample active (no per-launch event overhead) to a roofline entry
     and makes it the entry's achieved figure; the per-launch HIP-event
@@ -291,6 +291,14 @@ def back_to_back(roof, b, reps=20):
     roof["timing"] = f"{reps} back-to-back launches, every sample active, HIP events"
     if roof.get("traffic_over_algorithmic"):
         roof["traffic"] = byts * roof["traffic_over_algorithmic"]
+    if "full_storage_equivalent" in roof:
+        full = 0.0
+        for c, mesh in enumerate(b.meshes):
+            nnz, n = mesh.info["nnz"], 3 * mesh.info["blocks"]
+            full += b.n_samples[c] * (rs * nnz + 4 * rs * n)
+            full += 4.0 * mesh.info["block_nnz"] + 4.0 * (mesh.info["blocks"] + 1)
+        roof["full_storage_equivalent"].update(gbs=full / (ms / 1e3) / 1e9,
+                                               avg_bytes_per_launch=full)
     return roof
 
 
@@ -309,12 +317,24 @@ def spmv_roofline(b, kt, rs, name, counter_file):
     a_bytes = kernel_a_bytes(b, rs) / max(launches, 1)
     ratio = counters(counter_file)
     ach = a_bytes / a_s / 1e9 if a_s > 0 else 0.0
-    return dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=ach / HBM_PEAK_GBS,
-                traffic=a_bytes * ratio if ratio is not None else None,
-                traffic_over_algorithmic=ratio, kernel=name, launches=launches,
-                avg_launch_ms=1e3 * a_s, avg_bytes_per_launch=a_bytes,
-                ms_total=kt["spmv_dot"]["ms"])
+    out = dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
+               frac=ach / HBM_PEAK_GBS,
+               traffic=a_bytes * ratio if ratio is not None else None,
+               traffic_over_algorithmic=ratio, kernel=name, launches=launches,
+               avg_launch_ms=1e3 * a_s, avg_bytes_per_launch=a_bytes,
+               ms_total=kt["spmv_dot"]["ms"])
+    full = kernel_a_bytes(b, rs, full=True) / max(launches, 1)
+    if full != a_bytes and a_s > 0:
+        # the symmetric product: the same launches priced at the bytes a
+        # full-storage product moves (both triangles and the indices), to
+        # compare with the full-storage kernel; `achieved` stays the
+        # product's own bytes
+        out["full_storage_equivalent"] = dict(
+            gbs=full / a_s / 1e9, avg_bytes_per_launch=full,
+            note="the same launches priced at full-storage bytes (both triangles + indices), "
+                 "to compare with a full-storage SpMV; not the kernel's own traffic and not "
+                 "a fraction of the HBM peak")
+    return out
 
 
 def smoother_roofline(bm, models, kt, args, matrix_cores=True):
