@@ -768,6 +768,32 @@ def main():
     u_mixed = [[bm.solution(c, s) for s in range(counts[c])] for c in range(len(models))]
     iters_mixed = [s["iters"].tolist() for s in stats]
     outer_mixed = [s["outer"].tolist() for s in stats]
+    full_storage = None
+    if not args.only_mixed and all(bm.layout_info(c)["upper_triangle_product"]
+                                   for c in range(len(models))):
+        # the same batch with the full image (both triangles stored and
+        # streamed, msa_batch_set_symmetric_product(b, 0)): the plain SpMV's
+        # rate beside the symmetric product's
+        bm.set_symmetric_product(False)
+        bm.run(**run_kw)
+        ctx.synchronize()
+        t = time.perf_counter()
+        bm.run(**run_kw)
+        ctx.synchronize()
+        tf = time.perf_counter() - t
+        bm.set_kernel_timing(True)
+        bm.run(**run_kw)
+        ktf = bm.kernel_timing()
+        bm.set_kernel_timing(False)
+        full_storage = dict(
+            solves_per_s=N / tf,
+            roofline=spmv_roofline(bm, ktf, 4, spmv_name(bm, "float", args.tile),
+                                   "k_pcg_a_full_counters.json"),
+            kernel_timing_ms_per_step={k: v["ms"] for k, v in ktf.items()})
+        log(f"[bench] mixed with the full-storage SpMV: {tf:.2f} s -> {N / tf:.1f} solves/s, "
+            f"k_pcg_a {full_storage['roofline']['achieved']:.0f} GB/s "
+            f"({full_storage['roofline']['frac']:.3f} of peak)")
+        bm.set_symmetric_product(True)
     other_form = None
     if args.precond == "mg" and not args.only_mixed:
         # the same batch with the smoothing products in the other form
@@ -792,6 +818,7 @@ def main():
     bm.close()
 
     extra = {"roofline_second": secondary, "mixed_other_smoother_form": other_form,
+             "mixed_full_storage_spmv": full_storage,
              "kernel_timing_ms_per_step": {k: v["ms"] for k, v in kt.items()},
              "kernel_launches_per_step": {k: v["launches"] for k, v in kt.items()}}
     # ------------------------------------------------ the other preconditioner

@@ -232,6 +232,15 @@ int msa_batch_graph_info(msa_batch *b, long long *info);
  * product), [2] stored block slots per sample, [3] block rows. */
 int msa_batch_layout_info(msa_batch *b, int cls, long long *info);
 
+/* The upper-triangle storage and product of box-grid classes (enable = 1,
+ * the default) or the full image: both triangles stored and streamed, the
+ * products as on other meshes (enable = 0), for instance for a loaded
+ * matrix that is not symmetric.  No counterpart in the reference (its
+ * SkylineMatrix2D always keeps the upper profile).  A change of form
+ * needs the values again: the batch counts as not assembled (assemble,
+ * run, or load the matrices again). */
+int msa_batch_set_symmetric_product(msa_batch *b, int enable);
+
 /* Element integration precision of a MIXED batch, whose stiffness values are
  * stored in FP32: MSA_FP64 (default) integrates in FP64 and rounds each
  * assembled value; MSA_MIXED integrates in FP32.  FP32 integration leaves

@@ -393,6 +393,12 @@ class Batch:
         return dict(stencil=bool(info[0]), upper_triangle_product=bool(info[1]),
                     slots=int(info[2]), block_rows=int(info[3]))
 
+    def set_symmetric_product(self, enable: bool):
+        """Upper-triangle storage and product on box grids (True, the
+        default) or the full image (False); a change needs assembling
+        again."""
+        check(lib().msa_batch_set_symmetric_product(self.h, 1 if enable else 0))
+
     def set_persistent(self, enable: bool):
         """Jacobi-PCG solves as one cooperative launch (one-sample tiles)."""
         check(lib().msa_batch_set_persistent(self.h, 1 if enable else 0))

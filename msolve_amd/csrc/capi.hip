@@ -147,6 +147,7 @@ struct msa_batch {
     std::vector<int4 *> mg_sm_m, mg_sm_e;
     std::vector<int> mg_nsm_m, mg_nsm_e;
     bool matrix_cores = false;          // msa_batch_set_matrix_cores
+    bool full_storage = false;          // msa_batch_set_symmetric_product(b, 0)
     std::vector<int4 *> mg_restr;       // per depth >= 1: items of every level there
     std::vector<int> mg_nrestr;
     int4 *mg_lv_items = nullptr;        // every coarse level
@@ -305,7 +306,7 @@ int setup_class(msa_batch *b, ClassHost &C, msa_mesh *mesh, int nsamp)
         D.emap = mesh->d_emap;
         D.colour_elems = mesh->d_colour_elems;
         D.stencil = stencil ? 1 : 0;
-        D.sym = stencil && T == 1 ? 1 : 0;
+        D.sym = stencil && T == 1 && !b->full_storage ? 1 : 0;
         D.sym_sy = sy;
         D.sym_sz = sz;
         MSA_CHECK(h.ncolours <= kMaxColours, MSA_ERR_ARG, "too many colours");
@@ -363,8 +364,9 @@ int setup_class(msa_batch *b, ClassHost &C, msa_mesh *mesh, int nsamp)
         const size_t nvec = (size_t)C.ntiles * C.tile_vec;
         A(&D.val, (size_t)C.ntiles * C.tile_vals * rs);
         // upper-triangle classes never write their lower slots: zeros, so
-        // that a consumer reading a padding slot sees a zero block
-        if (rc == MSA_OK && D.sym) MSA_HIP(hipMemset(D.val, 0, (size_t)C.ntiles * C.tile_vals * rs));
+        // that a consumer reading a padding slot sees a zero block (every
+        // stencil class, as msa_batch_set_symmetric_product may switch it)
+        if (rc == MSA_OK && D.stencil) MSA_HIP(hipMemset(D.val, 0, (size_t)C.ntiles * C.tile_vals * rs));
         A(&D.x, nvec * rs);
         A(&D.r, nvec * rs);
         A(&D.z, nvec * rs);
@@ -1119,7 +1121,7 @@ int mg_build(msa_batch *b, int max_levels)
                     void *p = nullptr;
                     MSA_TRY(dalloc(b, &p, (size_t)Cc.ntiles * Cc.tile_vals * sizeof(double)));
                     Cc.dev.val64 = (double *)p;
-                    if (Cc.dev.sym)  // lower slots are not written (as the FP32 values)
+                    if (Cc.dev.stencil)  // lower slots may not be written (as the FP32 values)
                         MSA_HIP(hipMemset(p, 0, (size_t)Cc.ntiles * Cc.tile_vals * sizeof(double)));
                 }
                 // small coarsest operators: dense inverse, one launch per cycle
@@ -3151,6 +3153,31 @@ int msa_batch_set_matrix_cores(msa_batch *b, int enable)
         b->graph = nullptr;
     }
     b->matrix_cores = on;
+    return MSA_OK;
+}
+
+int msa_batch_set_symmetric_product(msa_batch *b, int enable)
+{
+    MSA_CHECK(b, MSA_ERR_ARG, "null batch");
+    b->full_storage = enable == 0;
+    bool changed = false;
+    auto apply = [&](ClassHost &C) {
+        const int want = C.dev.stencil && b->T == 1 && !b->full_storage ? 1 : 0;
+        if (C.dev.sym != want) {
+            C.dev.sym = want;
+            changed = true;
+        }
+    };
+    for (auto &C : b->cls) apply(C);
+    for (auto &L : b->lv) apply(L);
+    if (changed) {
+        // the stored image changes form: the values are assembled again
+        // (padding slots of either form hold zeros, so the other form's
+        // leftovers are never read)
+        b->assembled = false;
+        b->mg_ready = false;
+        MSA_TRY(upload_class_table(b));
+    }
     return MSA_OK;
 }
 

@@ -17,7 +17,9 @@ transposes of upper blocks of the rows above (the way SkylineMatrix2D.cs:
   solve to 1e-9, on a box grid and on a mesh with partly fixed nodes;
 * MIXED solves with the symmetric product reach the 1e-8 bar against
   the oracle's direct solve with the Jacobi and the multigrid
-  preconditioners."""
+  preconditioners;
+* msa_batch_set_symmetric_product switches a box-grid batch to the full
+  image and back (assembly required, same bars, bit-equal on return)."""
 from types import SimpleNamespace
 
 import numpy as np
@@ -177,5 +179,60 @@ def test_mixed_symmetric_product_meets_bar(ctx, precond):
     assert (st["status"] == msa.MSA_OK).all(), st
     for s in range(ns):
         assert rel(b.solution(0, s), direct(model, E[s])) < 1e-8
+    b.close()
+    mesh.close()
+
+
+@pytest.mark.parametrize("precision,precond", [(L.MSA_FP64, L.MSA_PRECOND_JACOBI),
+                                               (L.MSA_MIXED, L.MSA_PRECOND_MULTIGRID)])
+def test_full_storage_switch(ctx, precision, precond):
+    """msa_batch_set_symmetric_product(b, 0) stores and streams both
+    triangles on a box grid: the batch must be assembled again, its solves
+    meet the same bars (FP64: to 1e-11 of the symmetric product's), its
+    export is the oracle's matrix, and switching back reproduces the first
+    solve bit for bit.  An unchanged setting keeps the batch assembled."""
+    print(f"\n[case] full-storage switch precision={precision} precond={precond}", flush=True)
+    model = msa.cantilever(6)
+    rng = np.random.default_rng(14)
+    ns = 3
+    E = moduli(rng, ns, model.n_elem)
+    mesh = msa.Mesh(ctx, model.xyz, model.conn, model.dof, model.n_free)
+    b = msa.Batch(ctx, [mesh], [ns], precision, 1)
+    if precond != L.MSA_PRECOND_JACOBI:
+        b.set_preconditioner(precond)
+    b.set_modulus(0, E)
+    b.set_rhs(0, model.f)
+    tol = 1e-13 if precision == L.MSA_FP64 else 1e-9
+    sols = {}
+    for sym in (True, False, True):
+        b.set_symmetric_product(sym)
+        li = b.layout_info(0)
+        assert li["stencil"] and li["upper_triangle_product"] == sym
+        if sym is False:
+            with pytest.raises(msa.MsaError, match="assembled"):
+                b.solve(tol=tol, max_it=20000, graph_iters=8)
+        b.assemble(0.3)
+        b.set_symmetric_product(sym)  # unchanged: stays assembled
+        b.solve(tol=tol, max_it=20000, graph_iters=8)
+        st = b.stats(0)
+        assert (st["status"] == msa.MSA_OK).all(), st
+        u = [b.solution(0, s) for s in range(ns)]
+        if sym in sols:
+            for s in range(ns):
+                assert np.array_equal(u[s], sols[sym][s])
+        sols[sym] = u
+        if precision == L.MSA_FP64:
+            rp, col, val = b.matrix_csr(0, 0)
+            Kg = np.zeros((model.n_free, model.n_free))
+            for r in range(model.n_free):
+                Kg[r, col[rp[r]:rp[r + 1]]] = val[rp[r]:rp[r + 1]]
+            Ko = skyline_to_dense(*oracle_K(model, E[0]))
+            assert np.abs(Kg - Ko).max() <= 1e-13 * np.abs(Ko).max()
+    for s in range(ns):
+        ud = direct(model, E[s])
+        bar = 1e-9 if precision == L.MSA_FP64 else 1e-8
+        assert rel(sols[True][s], ud) < bar and rel(sols[False][s], ud) < bar
+        if precision == L.MSA_FP64:
+            assert rel(sols[True][s], sols[False][s]) < 1e-11
     b.close()
     mesh.close()
