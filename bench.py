@@ -769,8 +769,8 @@ def main():
     iters_mixed = [s["iters"].tolist() for s in stats]
     outer_mixed = [s["outer"].tolist() for s in stats]
     full_storage = None
-    if not args.only_mixed and all(bm.layout_info(c)["upper_triangle_product"]
-                                   for c in range(len(models))):
+    sym_all = all(bm.layout_info(c)["upper_triangle_product"] for c in range(len(models)))
+    if not args.only_mixed and sym_all:
         # the same batch with the full image (both triangles stored and
         # streamed, msa_batch_set_symmetric_product(b, 0)): the plain SpMV's
         # rate beside the symmetric product's
@@ -943,6 +943,8 @@ def main():
                          + f" to |f-Ku|/|f| < {args.tol}"),
             "samples": N, "sizes": sizes, "counts": counts, "tile": args.tile,
             "unroll": args.unroll,
+            "spmv_form": ("symmetric: stored upper triangle, lower blocks as transposes "
+                          "(the slot unroll does not apply)" if sym_all else "full storage"),
             "schedule_from": "autotuner" if tuned else "command line",
             "autotune_ms_per_sample": ([dict(tile=k[0], unroll=k[1], ms=v)
                                         for k, v in tuned["table"].items()] if tuned else None),
