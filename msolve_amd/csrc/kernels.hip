@@ -1041,10 +1041,10 @@ constexpr int kDppWaveShl1 = 0x130, kDppWaveShr1 = 0x138;
 // symmetric product q = K p, p = z + beta p_old: every row's 14 upper
 // blocks (slots 13..26) times p at br + off, and its 13 lower blocks as the
 // transposes of the upper blocks stored in the rows br - off (slot s of row
-// j holds K(j, j + off_s)).  The values move once from HBM: a row's upper
-// blocks are loaded through the caches and re-read transposed by the rows
-// above them, which the same and the neighbouring workgroups run at about
-// the same time.  (SkylineMatrix2D.Multiply, SkylineMatrix2D.cs:253-269,
+// j holds K(j, j + off_s)).  The values cross HBM about once: a row's
+// upper blocks are loaded through the caches and re-read transposed by the
+// rows above them, which the same and the neighbouring workgroups run at
+// about the same time, so most re-reads hit L2.  (SkylineMatrix2D.Multiply, SkylineMatrix2D.cs:253-269,
 // likewise applies each stored entry of the upper profile to both
 // triangles, by scattering into the column's rows; here every row gathers,
 // with no write conflicts.)
