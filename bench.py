@@ -71,7 +71,7 @@ def parse():
                    help="tolerance of the tightly converged FP64 GPU batch")
     p.add_argument("--max-it", type=int, default=20000)
     p.add_argument("--max-outer", type=int, default=30)
-    p.add_argument("--graph-iters", type=int, default=16)
+    p.add_argument("--graph-iters", type=int, default=12)
     p.add_argument("--tile", type=int, default=0,
                    help="lane tile of the batches (0: tile and slot unroll from the autotuner)")
     p.add_argument("--unroll", type=int, default=3, help="slot unroll when --tile is given")

@@ -194,7 +194,11 @@ int msa_batch_assemble(msa_batch *b, double nu);
  * max_it PCG iterations.  MIXED: tol on the FP64 residual |f - K u|/|f|,
  * inner FP32 PCG to inner_tol (relative) with at most max_it iterations,
  * at most max_outer refinements.  graph_iters > 0 captures that many PCG
- * iterations per hipGraph launch (even number), 0 launches eagerly. */
+ * iterations per hipGraph launch (even number), 0 launches eagerly.  Once
+ * a sample of the batch has converged, the remaining iterations go in
+ * blocks of 2 (MIXED with residual replacement: one replacement period),
+ * polled after each, so the solve ends at most one short block after its
+ * last sample. */
 int msa_batch_solve(msa_batch *b, double tol, int max_it, double inner_tol,
                     int max_outer, int graph_iters);
 
@@ -213,8 +217,10 @@ int msa_batch_solve(msa_batch *b, double tol, int max_it, double inner_tol,
 int msa_batch_set_refinement(msa_batch *b, int method, int period, double drop);
 
 /* Captured PCG iteration blocks (no counterpart in the reference): info[0]
- * graphs captured so far, [1] kernel nodes of the current graph (0: none),
- * [2] PCG iterations per replay, [3] graph launches so far.  Every captured
+ * captures so far (a solve's capture records the caller's block and the
+ * short tail block together; a Newmark run adds its three step graphs),
+ * [1] kernel nodes of the current block graph (0: none), [2] PCG
+ * iterations per replay of it, [3] graph launches so far.  Every captured
  * graph is checked on the host before it is instantiated: kernel nodes only,
  * non-empty grids, workgroups within each kernel's launch bounds, LDS within
  * a CU; a failed check fails the solve with MSA_ERR_STATE.  Changing the

@@ -117,9 +117,11 @@ struct msa_batch {
     double *d_acc = nullptr;  // base acceleration history of the current run
     int acc_cap = 0;
     int *d_step = nullptr;    // device step counter
-    // captured block of PCG iterations (kind 0 plain, 1 with replacement)
-    hipGraphExec_t graph = nullptr;
-    int graph_iters = 0, graph_kind = -1;
+    // captured blocks of PCG iterations (kind 0 plain, 1 with replacement):
+    // the caller's block and the short tail block replayed once a sample
+    // has converged
+    hipGraphExec_t graph = nullptr, graph_tail = nullptr;
+    int graph_iters = 0, graph_tail_iters = 0, graph_kind = -1;
     double graph_tol = -1;
     int graph_maxit = -1;
     long long graph_kernels = 0;   // kernel nodes of the current graph
@@ -438,6 +440,16 @@ int reduce(msa_batch *b, int op, double tol = 0.0, int max_it = 0, int max_outer
     });
 }
 
+void drop_graph(msa_batch *b)
+{
+    for (hipGraphExec_t *g : {&b->graph, &b->graph_tail})
+        if (*g) {
+            (void)hipGraphExecDestroy(*g);
+            *g = nullptr;
+        }
+    b->graph_tail_iters = 0;
+}
+
 // Class table: the solved classes, then the multigrid levels.  Growing it
 // moves d_cls, which invalidates a captured graph.
 int upload_class_table(msa_batch *b)
@@ -451,10 +463,7 @@ int upload_class_table(msa_batch *b)
         MSA_TRY(dalloc(b, &p, sizeof(ClassDev) * n));
         b->d_cls = (ClassDev *)p;
         b->d_cls_cap = n;
-        if (b->graph) {
-            (void)hipGraphExecDestroy(b->graph);
-            b->graph = nullptr;
-        }
+        drop_graph(b);
     }
     MSA_HIP(hipMemcpyAsync(b->d_cls, h.data(), sizeof(ClassDev) * h.size(),
                            hipMemcpyHostToDevice, b->ctx->stream));
@@ -597,26 +606,42 @@ int validate_graph(hipGraph_t g, long long *n_kernels)
     return MSA_OK;
 }
 
-int build_graph(msa_batch *b, int kind, int iters, double tol, int max_it)
+// Records the launches of fn() on the context stream into an instantiated,
+// host-validated graph (n_kernels: its kernel nodes).
+template <typename F>
+int capture(msa_batch *b, hipGraphExec_t *out, F &&fn, long long *n_kernels = nullptr)
 {
-    if (b->graph) {
-        (void)hipGraphExecDestroy(b->graph);
-        b->graph = nullptr;
-    }
     hipStream_t s = b->ctx->stream;
     MSA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int rc = launch_block(b, kind, iters, tol, max_it);
+    int rc = fn();
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(s, &g);
-    if (rc == MSA_OK && e == hipSuccess) rc = validate_graph(g, &b->graph_kernels);
-    if (rc != MSA_OK) {
-        if (g) (void)hipGraphDestroy(g);
-        return rc;
+    long long nk = 0;
+    if (rc == MSA_OK && e == hipSuccess) rc = validate_graph(g, &nk);
+    if (rc == MSA_OK && e == hipSuccess) e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    if (rc != MSA_OK) return rc;
+    MSA_HIP(e);
+    if (n_kernels) *n_kernels = nk;
+    return MSA_OK;
+}
+
+// Smallest block the iteration loop replays: two iterations (the product
+// buffers alternate), or one replacement period.
+int tail_iters(const msa_batch *b, int kind) { return kind == 1 ? b->period : 2; }
+
+// The caller's block of `iters` iterations and, when shorter, the tail
+// block, recorded together (one capture in the batch's count).
+int build_graph(msa_batch *b, int kind, int iters, double tol, int max_it)
+{
+    drop_graph(b);
+    MSA_TRY(capture(b, &b->graph, [&] { return launch_block(b, kind, iters, tol, max_it); },
+                    &b->graph_kernels));
+    const int tail = tail_iters(b, kind);
+    if (tail < iters) {
+        MSA_TRY(capture(b, &b->graph_tail, [&] { return launch_block(b, kind, tail, tol, max_it); }));
+        b->graph_tail_iters = tail;
     }
-    MSA_HIP(e);
-    e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    MSA_HIP(e);
     b->graph_iters = iters;
     b->graph_kind = kind;
     b->graph_tol = tol;
@@ -657,6 +682,13 @@ int pcg_loop(msa_batch *b, int kind, double tol, int max_it, int graph_iters)
                       b->graph_tol != tol || b->graph_maxit != max_it))
         MSA_TRY(build_graph(b, kind, block, tol, max_it));
     const int which = kind == 1 ? 1 : 0;
+    // Once a sample has converged, the rest follow within a few iterations
+    // in a Monte Carlo batch, and every launch of a block still dispatches
+    // the tiles of the converged samples: the loop then replays the tail
+    // block, which stops at most tail - 1 iterations after the last sample.
+    const int tail = tail_iters(b, kind);
+    int total = 0;
+    for (auto &C : b->cls) total += C.nsamp;
     long long done = 0;
     for (;;) {
         int active = 0;
@@ -666,13 +698,14 @@ int pcg_loop(msa_batch *b, int kind, double tol, int max_it, int graph_iters)
             set_error("PCG driver: samples still active after %lld iterations", done);
             return MSA_ERR_STATE;
         }
+        const int it = active < total && tail < block ? tail : block;
         if (use_graph) {
-            MSA_HIP(hipGraphLaunch(b->graph, b->ctx->stream));
+            MSA_HIP(hipGraphLaunch(it == block ? b->graph : b->graph_tail, b->ctx->stream));
             b->graph_replays++;
         } else {
-            MSA_TRY(launch_block(b, kind, block, tol, max_it));
+            MSA_TRY(launch_block(b, kind, it, tol, max_it));
         }
-        done += block;
+        done += it;
     }
     return MSA_OK;
 }
@@ -1872,7 +1905,7 @@ int msa_batch_destroy(msa_batch *b)
     if (!b) return MSA_OK;
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
-    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    drop_graph(b);
     if (b->timing.e0) (void)hipEventDestroy(b->timing.e0);
     if (b->timing.e1) (void)hipEventDestroy(b->timing.e1);
     for (void *p : b->allocs) (void)hipFree(p);
@@ -2338,26 +2371,6 @@ int msa_batch_set_probe(msa_batch *b, int cls, int dof)
 
 namespace {
 
-// Records the launches of fn() on the context stream into an instantiated,
-// host-validated graph.
-template <typename F>
-int capture(msa_batch *b, hipGraphExec_t *out, F &&fn)
-{
-    hipStream_t s = b->ctx->stream;
-    MSA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int rc = fn();
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(s, &g);
-    long long nk = 0;
-    if (rc == MSA_OK && e == hipSuccess) rc = validate_graph(g, &nk);
-    if (rc == MSA_OK && e == hipSuccess) e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
-    if (g) (void)hipGraphDestroy(g);
-    if (rc != MSA_OK) return rc;
-    MSA_HIP(e);
-    b->graph_captures++;
-    return MSA_OK;
-}
-
 // One Newmark step up to its PCG iterations (NewmarkDynamicAnalyzer.Solve,
 // NewmarkDynamicAnalyzer.cs:280-299): the effective right-hand side
 // (:189-206, the mass-acceleration load of the step), its norm, and the
@@ -2512,13 +2525,15 @@ int msa_batch_newmark(msa_batch *b, int nsteps, const double *acc, int acc_dir, 
             if (g) (void)hipGraphExecDestroy(g);
     };
     int rc = MSA_OK;
-    for (int k = 0; k < 3 && rc == MSA_OK; k++)
+    for (int k = 0; k < 3 && rc == MSA_OK; k++) {
         rc = capture(b, &G[k], [&]() -> int {
             if (k >= 1) MSA_TRY(newmark_tail(b, dc));
             if (k == 2) return MSA_OK;
             MSA_TRY(newmark_head(b, dc, tol, max_it));
             return direct ? MSA_OK : launch_block(b, kind, block, tol, max_it);
         });
+        if (rc == MSA_OK) b->graph_captures++;
+    }
     for (int step = 0; step < nsteps && rc == MSA_OK; step++) {
         hipError_t e = hipGraphLaunch(G[step == 0 ? 0 : 1], s);
         if (e != hipSuccess) {
@@ -2691,10 +2706,7 @@ int msa_batch_set_preconditioner(msa_batch *b, int kind, int max_levels, double 
         b->period = kind != MSA_PRECOND_JACOBI ? 2 : 8;
         b->drop = kind != MSA_PRECOND_JACOBI ? 1e-4 : 0.1;
     }
-    if (b->graph) {  // the captured iteration block holds the preconditioner
-        (void)hipGraphExecDestroy(b->graph);
-        b->graph = nullptr;
-    }
+    drop_graph(b);  // the captured iteration blocks hold the preconditioner
     b->mg_ready = false;
     if (b->assembled) MSA_TRY(precond_values(b));
     return MSA_OK;
@@ -3112,10 +3124,7 @@ int msa_batch_set_refinement(msa_batch *b, int method, int period, double drop)
     b->period = period;
     b->drop = drop;
     b->refine_set = true;
-    if (b->graph) {  // the captured iteration block holds the period and the trigger
-        (void)hipGraphExecDestroy(b->graph);
-        b->graph = nullptr;
-    }
+    drop_graph(b);  // the captured iteration blocks hold the period and the trigger
     return MSA_OK;
 }
 
@@ -3148,10 +3157,7 @@ int msa_batch_set_matrix_cores(msa_batch *b, int enable)
 {
     MSA_CHECK(b, MSA_ERR_ARG, "null batch");
     const bool on = enable != 0;
-    if (b->matrix_cores != on && b->graph) {  // the captured iteration block holds the kernels
-        (void)hipGraphExecDestroy(b->graph);
-        b->graph = nullptr;
-    }
+    if (b->matrix_cores != on) drop_graph(b);  // the captured iteration blocks hold the kernels
     b->matrix_cores = on;
     return MSA_OK;
 }
@@ -3315,10 +3321,7 @@ int msa_batch_set_schedule(msa_batch *b, int unroll)
     MSA_CHECK(unroll >= 1 && unroll <= 4, MSA_ERR_ARG, "unroll=%d (1..4)", unroll);
     MSA_CHECK(b->T == 1 || unroll == 3, MSA_ERR_ARG,
               "lane tiles T > 1 use groups of 3 slots (unroll=%d, T=%d)", unroll, b->T);
-    if (b->unroll != unroll && b->graph) {  // the captured iteration block holds the kernel
-        (void)hipGraphExecDestroy(b->graph);
-        b->graph = nullptr;
-    }
+    if (b->unroll != unroll) drop_graph(b);  // the captured iteration blocks hold the kernel
     b->unroll = unroll;
     return MSA_OK;
 }
