@@ -1010,6 +1010,16 @@ void class_items(const ClassHost &C, int ci, std::vector<int4> &out)
         for (int k = 0; k < C.nchunks; k++) out.push_back(make_int4(ci, t, k, 0));
 }
 
+// Work items of the box-grid smoothing lists: runs of kMfRun chunks of one
+// tile, the run length in the item's w.
+constexpr int kMfRun = 4;
+void class_run_items(const ClassHost &C, int ci, std::vector<int4> &out)
+{
+    for (int t = 0; t < C.ntiles; t++)
+        for (int k = 0; k < C.nchunks; k += kMfRun)
+            out.push_back(make_int4(ci, t, k, std::min(kMfRun, C.nchunks - k)));
+}
+
 // Work items of the grouped gather assembly: tiles in groups of kAsmGroup.
 void class_group_items(const ClassHost &C, int ci, std::vector<int4> &out)
 {
@@ -1215,7 +1225,9 @@ int mg_build(msa_batch *b, int max_levels)
                 class_items(C, ci, sm);
                 if (mfma_ok(b, C)) {
                     mfma_items(C, ci, smm);
-                    class_items(C, ci, sme);
+                    class_run_items(C, ci, sme);
+                } else if (C.dev.mf && C.dev.mf_box && b->T == 1) {
+                    class_run_items(C, ci, smb);
                 } else {
                     class_items(C, ci, !C.dev.mf ? sma : (C.dev.mf_box ? smb : smf));
                 }

@@ -3479,38 +3479,54 @@ __global__ __launch_bounds__(256) void k_residual64_mf(const ClassDev *__restric
 // block row br: the row's three components of each vector move as one
 // 12- or 24-byte access.  PRE A = (1 + c1) z/theta + c2 D^-1 (b - K z/theta),
 // RESID B = b - K A, POST1 B = A + D^-1 (b - K A)/theta, POST2
-// z = B + c1 (B - A) + c2 D^-1 (b - K B) (and r.z).
+// z = B + c1 (B - A) + c2 D^-1 (b - K B) (and r.z).  mg_load1 reads the
+// row's operands, which do not depend on y: the matrix-free product issues
+// it before its own loads, so these reads overlap its arithmetic.
+template <typename real>
+struct MgRow {
+    real bv[3], dv[3], xa[3], xb[3];
+};
+
 template <typename real, int STEP>
-__device__ __forceinline__ void mg_update1(const MSA_GLOBAL real *b, const MSA_GLOBAL real *dinv,
-                                           const MSA_GLOBAL real *z, const MSA_GLOBAL real *A,
-                                           const MSA_GLOBAL real *B, MSA_GLOBAL real *dst, int br,
-                                           const real (&y)[3], real ith, real c1, real c2,
-                                           real &rz)
+__device__ __forceinline__ MgRow<real> mg_load1(const MSA_GLOBAL real *b,
+                                                const MSA_GLOBAL real *dinv,
+                                                const MSA_GLOBAL real *z,
+                                                const MSA_GLOBAL real *A,
+                                                const MSA_GLOBAL real *B, int br)
 {
-    real bv[3], dv[3], xa[3], xb[3], out[3];
+    MgRow<real> w;
     auto get = [&](const MSA_GLOBAL real *p, real (&v)[3]) {
         const vec3<real> t = ld3(p, br);
         v[0] = t.x;
         v[1] = t.y;
         v[2] = t.z;
     };
-    get(b, bv);
-    if constexpr (STEP != MG_RESID) get(dinv, dv);
-    if constexpr (STEP == MG_PRE) get(z, xa);
-    if constexpr (STEP == MG_POST1 || STEP == MG_POST2) get(A, xa);
-    if constexpr (STEP == MG_POST2) get(B, xb);
+    get(b, w.bv);
+    if constexpr (STEP != MG_RESID) get(dinv, w.dv);
+    if constexpr (STEP == MG_PRE) get(z, w.xa);
+    if constexpr (STEP == MG_POST1 || STEP == MG_POST2) get(A, w.xa);
+    if constexpr (STEP == MG_POST2) get(B, w.xb);
+    return w;
+}
+
+template <typename real, int STEP>
+__device__ __forceinline__ void mg_finish1(const MgRow<real> &w, MSA_GLOBAL real *dst, int br,
+                                           const real (&y)[3], real ith, real c1, real c2,
+                                           real &rz)
+{
+    real out[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         if constexpr (STEP == MG_PRE) {
-            const real x1 = xa[d] * ith;
-            out[d] = x1 + c1 * x1 + c2 * dv[d] * (bv[d] - y[d] * ith);
+            const real x1 = w.xa[d] * ith;
+            out[d] = x1 + c1 * x1 + c2 * w.dv[d] * (w.bv[d] - y[d] * ith);
         } else if constexpr (STEP == MG_RESID) {
-            out[d] = bv[d] - y[d];
+            out[d] = w.bv[d] - y[d];
         } else if constexpr (STEP == MG_POST1) {
-            out[d] = xa[d] + ith * dv[d] * (bv[d] - y[d]);
+            out[d] = w.xa[d] + ith * w.dv[d] * (w.bv[d] - y[d]);
         } else {
-            out[d] = xb[d] + c1 * (xb[d] - xa[d]) + c2 * dv[d] * (bv[d] - y[d]);
-            rz += bv[d] * out[d];
+            out[d] = w.xb[d] + c1 * (w.xb[d] - w.xa[d]) + c2 * w.dv[d] * (w.bv[d] - y[d]);
+            rz += w.bv[d] * out[d];
         }
     }
     st3(dst, br, out[0], out[1], out[2]);
@@ -3551,49 +3567,58 @@ __global__ __launch_bounds__(256) void k_mg_smooth(const ClassDev *__restrict__ 
     const MSA_GLOBAL real *__restrict__ val = gbl((const real *)C.val + (size_t)tile * C.tile_vals);
     const MSA_GLOBAL int *__restrict__ cidx = gbl(C.cidx);
     const MSA_GLOBAL int *__restrict__ sptr = gbl(C.slice_ptr);
-    real rz = 0;
-    const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
-    if (__ballot(act)) {
-        for (int slice = s0 + wave; slice < s1; slice += kWaves) {
-            const int br = slice * R + rr;
-            const int base = sptr[slice];
-            const int W = sptr[slice + 1] - base;
-            if (!act || br >= C.nb) continue;
-            real y[3];
-            if constexpr (MF != 0)
-                mf_product<MF == 2>(C, (const MSA_GLOBAL float *)src, C.E + (size_t)tile * C.n_elem,
-                                    br, y[0], y[1], y[2]);
-            else
-                row_product<real, T>(C, val, cidx, src, base, W, rr, s, y[0], y[1], y[2]);
-            if constexpr (T == 1) {
-                mg_update1<real, STEP>(b, dinv, z, A, B, dst, br, y, ith, c1, c2, rz);
-                continue;
-            }
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                const size_t o = ((size_t)br * 3 + d) * T + s;
-                const real bv = b[o], dv = dinv[o];
-                real out;
-                if constexpr (STEP == MG_PRE) {
-                    const real x1 = z[o] * ith;
-                    out = x1 + c1 * x1 + c2 * dv * (bv - y[d] * ith);
-                } else if constexpr (STEP == MG_RESID) {
-                    out = bv - y[d];
-                } else if constexpr (STEP == MG_POST1) {
-                    out = A[o] + ith * dv * (bv - y[d]);
-                } else {
-                    const real xb = B[o];
-                    out = xb + c1 * (xb - A[o]) + c2 * dv * (bv - y[d]);
-                    rz += bv * out;
+    // an item covers max(1, it.w) consecutive chunks (box-grid lists group
+    // them, so one workgroup's set-up serves several slices per wave)
+    const int nch = max(1, it.w);
+    for (int ck = chunk; ck < chunk + nch; ck++) {
+        real rz = 0;
+        const int s0 = ck * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+        if (__ballot(act)) {
+            for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+                const int br = slice * R + rr;
+                const int base = sptr[slice];
+                const int W = sptr[slice + 1] - base;
+                if (!act || br >= C.nb) continue;
+                real y[3];
+                if constexpr (T == 1) {
+                    const MgRow<real> w = mg_load1<real, STEP>(b, dinv, z, A, B, br);
+                    if constexpr (MF != 0)
+                        mf_product<MF == 2>(C, (const MSA_GLOBAL float *)src,
+                                            C.E + (size_t)tile * C.n_elem, br, y[0], y[1], y[2]);
+                    else
+                        row_product<real, T>(C, val, cidx, src, base, W, rr, s, y[0], y[1], y[2]);
+                    mg_finish1<real, STEP>(w, dst, br, y, ith, c1, c2, rz);
+                    continue;
                 }
-                dst[o] = out;
+                row_product<real, T>(C, val, cidx, src, base, W, rr, s, y[0], y[1], y[2]);
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    const size_t o = ((size_t)br * 3 + d) * T + s;
+                    const real bv = b[o], dv = dinv[o];
+                    real out;
+                    if constexpr (STEP == MG_PRE) {
+                        const real x1 = z[o] * ith;
+                        out = x1 + c1 * x1 + c2 * dv * (bv - y[d] * ith);
+                    } else if constexpr (STEP == MG_RESID) {
+                        out = bv - y[d];
+                    } else if constexpr (STEP == MG_POST1) {
+                        out = A[o] + ith * dv * (bv - y[d]);
+                    } else {
+                        const real xb = B[o];
+                        out = xb + c1 * (xb - A[o]) + c2 * dv * (bv - y[d]);
+                        rz += bv * out;
+                    }
+                    dst[o] = out;
+                }
             }
         }
-    }
-    if constexpr (STEP == MG_POST2) {
-        if (C.mg_depth == 0) {
-            double acc[1] = {(double)rz};
-            block_partials<T, 1>(acc, C.partial + ((size_t)tile * C.nchunks + chunk) * 3 * T);
+        if constexpr (STEP == MG_POST2) {
+            if (C.mg_depth == 0) {
+                double acc[1] = {(double)rz};
+                block_partials<T, 1>(acc, C.partial + ((size_t)tile * C.nchunks + ck) * 3 * T);
+                // the next chunk's partials reuse the same LDS rows
+                if (ck + 1 < chunk + nch) __syncthreads();
+            }
         }
     }
 }
@@ -3779,7 +3804,7 @@ __global__ void k_lz_eig(const ClassDev *__restrict__ cls, const int *__restrict
 //      of 4; U_c, the cell's 8 corners x 3 components, gathered from LDS;
 //   3. each owned node of the layer's bottom plane sums the 4 cells below it
 //      (kept from the previous layer) and the 4 above, in a fixed order, and
-//      takes the step's row update (mg_update1, as k_mg_smooth); the sums
+//      takes the step's row update (mg_finish1, as k_mg_smooth); the sums
 //      of the 4 cells below the next plane are kept for the next layer.
 // Cells in the rows next to the owned ones are computed by both
 // neighbouring workgroups, so every node's sum is formed in one place
@@ -3977,7 +4002,7 @@ __global__ __launch_bounds__(256) void k_mg_smooth_mfma(const ClassDev *__restri
     float rz = 0.f;
     // the owned nodes of plane zz: the sums of the cells below (Al) and, when
     // `above`, of the current layer's cells, then the row update (the
-    // arithmetic of mg_update1); `keep` stores the current layer's sums for
+    // arithmetic of mg_finish1); `keep` stores the current layer's sums for
     // the plane above
     auto finalize = [&](int zz, bool above, bool keep) {
         const bool own = zz >= b0z && zz <= b1z;
@@ -4423,7 +4448,7 @@ __global__ __launch_bounds__(256) void k_mg_gersh_mf(const ClassDev *__restrict_
         const int b1x = C.mf_box_hi[0], b1y = C.mf_box_hi[1], b1z = C.mf_box_hi[2];
         const float cK = C.dyn ? (float)C.cK : 1.f, cM = C.dyn ? (float)C.cM : 0.f;
         const double *Es = C.E + (size_t)tile * C.n_elem;
-        const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+        const int s0 = chunk * C.chunk, s1 = min(s0 + C.chunk * max(1, it.w), C.nslices);
         for (int slice = s0 + wave; slice < s1; slice += kWaves) {
             const int br = slice * 64 + lane;
             if (br >= C.nb) continue;
