@@ -263,6 +263,27 @@ def kernel_a_bytes(b, rs, full=False):
     return total
 
 
+def mf_pcg_work(b, models):
+    """FP32 operations and algorithmic HBM bytes of all matrix-free PCG
+    product launches (k_pcg_a_mf) of one solve, None unless every class
+    takes that product.  Per iteration and active sample: the same 150
+    flop per (free node, incident cell) pair as the matrix-free smoothing
+    products; bytes: z and p_old read, p and q written (4 vectors of
+    4*3*nb) and the moduli (8 per element)."""
+    flops = byts = 0.0
+    for c, m in enumerate(models):
+        if b.product_form(c) != "matrix_free":
+            return None
+        its = b.stats(c)["iters"].astype(np.float64).sum()
+        conn = np.asarray(m.conn).reshape(-1, 8)
+        free_node = (np.asarray(m.dof).reshape(-1, 3) >= 0).any(axis=1)
+        inc = float(free_node[conn].sum())
+        nb = float(free_node.sum())
+        flops += its * inc * 150.0
+        byts += its * (4 * 4 * 3 * nb + 8.0 * conn.shape[0])
+    return flops, byts
+
+
 def counters(name):
     """Measured/algorithmic HBM bytes of a kernel from its rocprofv3
     FETCH_SIZE/WRITE_SIZE passes (profiles/<name>), or None."""
@@ -304,16 +325,31 @@ def back_to_back(roof, b, reps=20, rs=8):
 
 def spmv_name(b, real, tile):
     """Name of the SpMV+dot kernel entry, saying which product it runs."""
+    if all(b.product_form(c) == "matrix_free" for c in range(len(b.meshes))):
+        return ("k_pcg_a_mf (matrix-free FP32 PCG product K p on box grids from the cell "
+                "moduli, the corners' differences from the node's value; + p update + p.q)")
     if all(b.layout_info(c)["upper_triangle_product"] for c in range(len(b.meshes))):
         return (f"k_pcg_a<{real},{tile}> (symmetric SpMV from the stored upper triangle "
                 "+ p update + p.q)")
     return f"k_pcg_a<{real},{tile}> (SpMV + p update + p.q)"
 
 
-def spmv_roofline(b, kt, rs, name, counter_file):
-    """Roofline entry of k_pcg_a from one kernel-timing solve."""
+def spmv_roofline(b, kt, rs, name, counter_file, models=None):
+    """Roofline entry of k_pcg_a from one kernel-timing solve (the
+    matrix-free product: FP32 VALU, like the smoothing products)."""
     launches = kt["spmv_dot"]["launches"]
     a_s = kt["spmv_dot"]["ms"] / 1e3 / max(launches, 1)
+    mfw = mf_pcg_work(b, models) if models is not None else None
+    if mfw is not None:
+        flop, byts = mfw[0] / max(launches, 1), mfw[1] / max(launches, 1)
+        ach = flop / a_s / 1e12 if a_s > 0 else 0.0
+        return dict(bound="valu", achieved=ach, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+                    frac=ach / FP32_PEAK_TFLOPS,
+                    frac_of_valu_without_packed_fp32=ach / FP32_VALU_NOPACK_TFLOPS,
+                    traffic=None, kernel=name, launches=launches, avg_launch_ms=1e3 * a_s,
+                    avg_flop_per_launch=flop, avg_bytes_per_launch=byts,
+                    hbm_gbs=byts / a_s / 1e9 if a_s > 0 else 0.0,
+                    ms_total=kt["spmv_dot"]["ms"])
     a_bytes = kernel_a_bytes(b, rs) / max(launches, 1)
     ratio = counters(counter_file)
     ach = a_bytes / a_s / 1e9 if a_s > 0 else 0.0
@@ -758,7 +794,8 @@ def main():
     kt = bm.kernel_timing()
     bm.set_kernel_timing(False)
     log(f"[bench] kernel timing: {json.dumps(kt)}")
-    roof_a = spmv_roofline(bm, kt, 4, spmv_name(bm, "float", args.tile), "k_pcg_a_counters.json")
+    roof_a = spmv_roofline(bm, kt, 4, spmv_name(bm, "float", args.tile), "k_pcg_a_counters.json",
+                           models)
     roof_s = (smoother_roofline(bm, models, kt, args, bool(args.matrix_cores))
               if args.precond == "mg" else None)
     if roof_s is not None and roof_s["ms_total"] > roof_a["ms_total"]:
@@ -768,8 +805,34 @@ def main():
     u_mixed = [[bm.solution(c, s) for s in range(counts[c])] for c in range(len(models))]
     iters_mixed = [s["iters"].tolist() for s in stats]
     outer_mixed = [s["outer"].tolist() for s in stats]
-    full_storage = None
+    full_storage = stored = None
+    mf_all = all(bm.product_form(c) == "matrix_free" for c in range(len(models)))
     sym_all = all(bm.layout_info(c)["upper_triangle_product"] for c in range(len(models)))
+    if not args.only_mixed and mf_all:
+        # the same batch with the PCG product from the stored upper triangle
+        # (msa_batch_set_matrix_free_product(b, 0)): its HBM roofline and
+        # the solve rate beside the matrix-free product's
+        bm.set_matrix_free_product(False)
+        bm.run(**run_kw)
+        ctx.synchronize()
+        t = time.perf_counter()
+        bm.run(**run_kw)
+        ctx.synchronize()
+        ts_ = time.perf_counter() - t
+        bm.set_kernel_timing(True)
+        bm.run(**run_kw)
+        kts = bm.kernel_timing()
+        bm.set_kernel_timing(False)
+        stored = dict(
+            solves_per_s=N / ts_,
+            iters_mean_per_size={str(n): float(np.mean(bm.stats(c)["iters"]))
+                                 for c, n in enumerate(sizes)},
+            roofline=spmv_roofline(bm, kts, 4, spmv_name(bm, "float", args.tile),
+                                   "k_pcg_a_counters.json"),
+            kernel_timing_ms_per_step={k: v["ms"] for k, v in kts.items()})
+        log(f"[bench] mixed with the stored upper-triangle product: {ts_:.2f} s -> "
+            f"{N / ts_:.1f} solves/s, k_pcg_a {stored['roofline']['achieved']:.0f} GB/s "
+            f"({stored['roofline']['frac']:.3f} of peak)")
     if not args.only_mixed and sym_all:
         # the same batch with the full image (both triangles stored and
         # streamed, msa_batch_set_symmetric_product(b, 0)): the plain SpMV's
@@ -794,6 +857,7 @@ def main():
             f"k_pcg_a {full_storage['roofline']['achieved']:.0f} GB/s "
             f"({full_storage['roofline']['frac']:.3f} of peak)")
         bm.set_symmetric_product(True)
+    bm.set_matrix_free_product(True)
     other_form = None
     if args.precond == "mg" and not args.only_mixed:
         # the same batch with the smoothing products in the other form
@@ -818,6 +882,7 @@ def main():
     bm.close()
 
     extra = {"roofline_second": secondary, "mixed_other_smoother_form": other_form,
+             "mixed_stored_upper_triangle_product": stored,
              "mixed_full_storage_spmv": full_storage,
              "kernel_timing_ms_per_step": {k: v["ms"] for k, v in kt.items()},
              "kernel_launches_per_step": {k: v["launches"] for k, v in kt.items()}}
@@ -943,7 +1008,10 @@ def main():
                          + f" to |f-Ku|/|f| < {args.tol}"),
             "samples": N, "sizes": sizes, "counts": counts, "tile": args.tile,
             "unroll": args.unroll,
-            "spmv_form": ("symmetric: stored upper triangle, lower blocks as transposes "
+            "spmv_form": ("matrix-free FP32 product from the cell moduli (differences from "
+                          "the node's value); the stored upper triangle is still assembled"
+                          if mf_all else
+                          "symmetric: stored upper triangle, lower blocks as transposes "
                           "(the slot unroll does not apply)" if sym_all else "full storage"),
             "schedule_from": "autotuner" if tuned else "command line",
             "autotune_ms_per_sample": ([dict(tile=k[0], unroll=k[1], ms=v)

@@ -247,6 +247,23 @@ int msa_batch_layout_info(msa_batch *b, int cls, long long *info);
  * run, or load the matrices again). */
 int msa_batch_set_symmetric_product(msa_batch *b, int enable);
 
+/* The PCG product q = K p of box-grid classes computed matrix-free (enable
+ * = 1, the default): MIXED batches of one-sample tiles with the multigrid
+ * preconditioner, on the classes it coarsens (a class without coarser
+ * levels keeps the stored product), whose finest smoothing products compute K v from
+ * the cell moduli and the unit element matrix in FP32.  The product then
+ * reads z, p_old and the moduli instead of the stored FP32 values; the
+ * FP64 residual replacements (also matrix-free) fix the accuracy, and the
+ * stored values are still assembled (export, coarse levels, warm starts).
+ * enable = 0: the stored-value product.  Takes effect at the next solve.
+ * Replaces, for those classes, the role of SkylineMatrix2D.Multiply
+ * (SkylineMatrix2D.cs:253-269) in SolverPCG.cs:117. */
+int msa_batch_set_matrix_free_product(msa_batch *b, int enable);
+
+/* The PCG product form of class cls at the last solve: 0 the stored
+ * values (full image), 1 the stored upper triangle, 2 matrix-free. */
+int msa_batch_product_form(msa_batch *b, int cls, int *form);
+
 /* Element integration precision of a MIXED batch, whose stiffness values are
  * stored in FP32: MSA_FP64 (default) integrates in FP64 and rounds each
  * assembled value; MSA_MIXED integrates in FP32.  FP32 integration leaves
@@ -327,8 +344,9 @@ int msa_batch_get_kernel_timing(msa_batch *b, double *ms_total,
 /* Algorithmic bytes of one launch of the SpMV+dot kernel over every sample
  * of the batch: bytes_per_launch[0] the kernel's (values + shared indices +
  * vectors; on box grids with one-sample tiles the upper triangle of the
- * values, diagonal included, and no indices), bytes_per_launch[1] a plain
- * SpMV's (all values + indices + one vector read and one written). */
+ * values, diagonal included, and no indices; matrix-free classes: the four
+ * vectors and the FP64 cell moduli), bytes_per_launch[1] a plain SpMV's
+ * (all values + indices + one vector read and one written). */
 int msa_batch_spmv_bytes(msa_batch *b, double *bytes_per_launch);
 /* Measurement (no counterpart in the reference): times `reps` back-to-back
  * launches of the batch's SpMV + dot kernel with every sample active (the

@@ -46,6 +46,7 @@ EXPORTS = [
     "msa_batch_factorize", "msa_batch_direct_solve", "msa_batch_set_solver",
     "msa_batch_direct_info", "msa_batch_set_preconditioner", "msa_batch_mg_info",
     "msa_batch_mg_apply", "msa_batch_graph_info", "msa_batch_layout_info", "msa_batch_set_symmetric_product",
+    "msa_batch_set_matrix_free_product", "msa_batch_product_form",
     "msa_autotune_schedule",
     "msa_autotune_candidates", "msa_batch_set_schedule", "msa_batch_set_matrix_cores",
     "msa_batch_time_spmv",
@@ -151,6 +152,8 @@ def lib():
         "msa_batch_graph_info": [_vp, _llp],
         "msa_batch_layout_info": [_vp, _i, _llp],
         "msa_batch_set_symmetric_product": [_vp, _i],
+        "msa_batch_set_matrix_free_product": [_vp, _i],
+        "msa_batch_product_form": [_vp, _i, _ip],
         "msa_autotune_schedule": [_vp, _i, C.POINTER(_vp), _i, _i, _i, _ip, _ip, _dp],
         "msa_autotune_candidates": [_ip, _ip],
         "msa_batch_set_schedule": [_vp, _i],

@@ -399,6 +399,19 @@ class Batch:
         again."""
         check(lib().msa_batch_set_symmetric_product(self.h, 1 if enable else 0))
 
+    def set_matrix_free_product(self, enable: bool):
+        """PCG product of box-grid classes matrix-free (True, the default:
+        MIXED, one-sample tiles, multigrid preconditioner) or from the
+        stored values (False); takes effect at the next solve."""
+        check(lib().msa_batch_set_matrix_free_product(self.h, 1 if enable else 0))
+
+    def product_form(self, cls: int) -> str:
+        """The PCG product of a class at the last solve: "stored",
+        "upper_triangle" or "matrix_free"."""
+        f = C.c_int(0)
+        check(lib().msa_batch_product_form(self.h, cls, C.byref(f)))
+        return ("stored", "upper_triangle", "matrix_free")[f.value]
+
     def set_persistent(self, enable: bool):
         """Jacobi-PCG solves as one cooperative launch (one-sample tiles)."""
         check(lib().msa_batch_set_persistent(self.h, 1 if enable else 0))

@@ -150,6 +150,11 @@ struct msa_batch {
     std::vector<int> mg_nsm_m, mg_nsm_e;
     bool matrix_cores = false;          // msa_batch_set_matrix_cores
     bool full_storage = false;          // msa_batch_set_symmetric_product(b, 0)
+    bool mf_product = true;             // msa_batch_set_matrix_free_product
+    // PCG product work items while some class takes the matrix-free product:
+    // the stored-value classes, and the matrix-free ones (runs of chunks)
+    int4 *d_items_st = nullptr, *d_items_mf = nullptr;
+    int n_items_st = 0, n_items_mf = 0;
     std::vector<int4 *> mg_restr;       // per depth >= 1: items of every level there
     std::vector<int> mg_nrestr;
     int4 *mg_lv_items = nullptr;        // every coarse level
@@ -485,15 +490,25 @@ int mg_values(msa_batch *b);
 int mg_lanczos(msa_batch *b);
 int precond_values(msa_batch *b);
 
+// q = K p with p = z + beta p_old for every class: the stored-value product
+// and, for the classes whose product is matrix-free, k_pcg_a_mf.
+hipError_t launch_product(msa_batch *b, int pbuf)
+{
+    hipStream_t s = b->ctx->stream;
+    if (b->n_items_mf == 0)
+        return launch_pcg_a(s, is_mixed(b), b->T, b->unroll, b->d_cls, b->d_items,
+                            (int)b->items.size(), pbuf, b->d_counts);
+    const hipError_t e = launch_pcg_a(s, is_mixed(b), b->T, b->unroll, b->d_cls, b->d_items_st,
+                                      b->n_items_st, pbuf, b->d_counts);
+    return e != hipSuccess ? e : launch_pcg_a_mf(s, b->d_cls, b->d_items_mf, b->n_items_mf, pbuf);
+}
+
 // ru_tol > 0: reliable-update mode (inner test off, replacement triggers on).
 int launch_iteration_pair(msa_batch *b, int pbuf, double tol, int max_it, double ru_tol = 0.0)
 {
     const bool mx = is_mixed(b);
     hipStream_t s = b->ctx->stream;
-    MSA_TRY(timed(b, KK_SPMV, [&] {
-        return launch_pcg_a(s, mx, b->T, b->unroll, b->d_cls, b->d_items, (int)b->items.size(),
-                            pbuf, b->d_counts);
-    }));
+    MSA_TRY(timed(b, KK_SPMV, [&] { return launch_product(b, pbuf); }));
     MSA_TRY(reduce(b, OP_A));
     MSA_TRY(timed(b, KK_UPDATE, [&] {
         return launch_pcg_b(s, mx, b->T, b->d_cls, b->d_items, (int)b->items.size(), pbuf, tol,
@@ -1018,6 +1033,40 @@ void class_run_items(const ClassHost &C, int ci, std::vector<int4> &out)
     for (int t = 0; t < C.ntiles; t++)
         for (int k = 0; k < C.nchunks; k += kMfRun)
             out.push_back(make_int4(ci, t, k, std::min(kMfRun, C.nchunks - k)));
+}
+
+// Which box-grid classes take the matrix-free PCG product (the setting,
+// MIXED, one-sample tiles, the multigrid preconditioner set up: its unit
+// element matrix and grid tables serve the product), and their work items.
+// Called after the preconditioner values of a solve; a change re-uploads
+// the class table and drops the captured graphs (they hold the items).
+int sync_mf_pcg(msa_batch *b)
+{
+    bool changed = false, any = false;
+    for (auto &C : b->cls) {
+        const int want = b->mf_product && is_mixed(b) && b->T == 1 &&
+                         b->precond == MSA_PRECOND_MULTIGRID && b->mg_ready && C.dev.mf &&
+                         C.dev.mf_box;
+        changed |= C.dev.mf_pcg != want;
+        C.dev.mf_pcg = want;
+        any |= want != 0;
+    }
+    if (!changed) return MSA_OK;
+    if (any) {
+        std::vector<int4> st, mf;
+        for (int c = 0; c < (int)b->cls.size(); c++) {
+            if (b->cls[c].dev.mf_pcg)
+                class_run_items(b->cls[c], c, mf);
+            else
+                class_items(b->cls[c], c, st);
+        }
+        MSA_TRY(items_to_device(b, st, &b->d_items_st, &b->n_items_st));
+        MSA_TRY(items_to_device(b, mf, &b->d_items_mf, &b->n_items_mf));
+    } else {
+        b->n_items_st = b->n_items_mf = 0;
+    }
+    drop_graph(b);
+    return upload_class_table(b);
 }
 
 // Work items of the grouped gather assembly: tiles in groups of kAsmGroup.
@@ -2207,6 +2256,7 @@ int msa_batch_solve(msa_batch *b, double tol, int max_it, double inner_tol, int 
     const bool mx = is_mixed(b);
     const int T = b->T, ni = (int)b->items.size();
     if (!b->mg_ready) MSA_TRY(precond_values(b));
+    MSA_TRY(sync_mf_pcg(b));
     if (b->dyn) {
         // static solve with the effective matrix: per-sample right-hand side = f
         MSA_TRY(timed(b, KK_OTHER, [&] { return launch_fs_static(s, T, b->d_cls, b->d_items, ni); }));
@@ -2469,6 +2519,7 @@ int msa_batch_newmark(msa_batch *b, int nsteps, const double *acc, int acc_dir, 
     hipStream_t s = b->ctx->stream;
     const int T = b->T, ni = (int)b->items.size();
     if (b->solver != MSA_SOLVER_DIRECT && !b->mg_ready) MSA_TRY(precond_values(b));
+    MSA_TRY(sync_mf_pcg(b));
     if (acc_dir >= 0) {
         if (b->acc_cap < nsteps) {
             void *p = nullptr;
@@ -3153,10 +3204,9 @@ int msa_batch_time_spmv(msa_batch *b, int reps, double *ms_per_launch)
     MSA_TRY(timed(b, KK_OTHER, [&] { return launch_pcg_init(s, mx, b->T, b->d_cls, b->d_items, ni, 0, b->d_counts); }));
     MSA_TRY(reduce(b, OP_INIT0));
     // pbuf 0 reads z and p1, writes p0 and q: every launch does the same work
-    MSA_HIP(launch_pcg_a(s, mx, b->T, b->unroll, b->d_cls, b->d_items, ni, 0, b->d_counts));
+    MSA_HIP(launch_product(b, 0));
     MSA_HIP(hipEventRecord(b->timing.e0, s));
-    for (int r = 0; r < reps; r++)
-        MSA_HIP(launch_pcg_a(s, mx, b->T, b->unroll, b->d_cls, b->d_items, ni, 0, b->d_counts));
+    for (int r = 0; r < reps; r++) MSA_HIP(launch_product(b, 0));
     MSA_HIP(hipEventRecord(b->timing.e1, s));
     MSA_HIP(hipEventSynchronize(b->timing.e1));
     float ms = 0;
@@ -3196,6 +3246,22 @@ int msa_batch_set_symmetric_product(msa_batch *b, int enable)
         b->mg_ready = false;
         MSA_TRY(upload_class_table(b));
     }
+    return MSA_OK;
+}
+
+int msa_batch_set_matrix_free_product(msa_batch *b, int enable)
+{
+    MSA_CHECK(b, MSA_ERR_ARG, "null batch");
+    b->mf_product = enable != 0;  // takes effect at the next solve
+    return MSA_OK;
+}
+
+int msa_batch_product_form(msa_batch *b, int cls, int *form)
+{
+    MSA_CHECK(b && form, MSA_ERR_ARG, "bad arguments");
+    MSA_CHECK(cls >= 0 && cls < (int)b->cls.size(), MSA_ERR_ARG, "class %d", cls);
+    const ClassDev &D = b->cls[cls].dev;
+    *form = D.mf_pcg ? 2 : (D.sym ? 1 : 0);
     return MSA_OK;
 }
 
@@ -3260,7 +3326,9 @@ int msa_batch_spmv_bytes(msa_batch *b, double *bytes)
         const MeshHost &m = C.mesh->h;
         const double S = C.nsamp, nnz = (double)m.nnz_scalar, n = 3.0 * m.nb;
         const double idx = 4.0 * (double)m.bcol.size() + 4.0 * (C.nslices + 1);
-        if (C.dev.sym)  // the upper triangle with the diagonal; offsets, not indices
+        if (C.dev.mf_pcg)  // matrix-free: the four vectors and the cell moduli
+            kernel += S * 4.0 * rs * n + S * 8.0 * m.n_elem;
+        else if (C.dev.sym)  // the upper triangle with the diagonal; offsets, not indices
             kernel += S * rs * 0.5 * (nnz + n) + S * 4.0 * rs * n;
         else
             kernel += S * rs * nnz + idx + S * 4.0 * rs * n;

@@ -1164,6 +1164,14 @@ __device__ __forceinline__ real pcg_a_sym_rows(const ClassDev &C, int s0, int s1
     return pq;
 }
 
+// The matrix-free product of a uniform-grid class (defined with the
+// smoothing kernels below).
+template <bool BOX, bool PG = false>
+__device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL float *v,
+                                           const double *Es, int br, float &y0, float &y1,
+                                           float &y2, const MSA_GLOBAL float *v2 = nullptr,
+                                           float beta = 0.f, float *pown = nullptr);
+
 // Search direction + product + curvature (SolverPCGSimpleSearchVectorCalculator
 // .cs:18-41, MCPCGDirect.cs:48-51): p = z + beta p_old on the fly for every
 // gathered column, q = K p, own rows of p written to the other buffer,
@@ -1294,6 +1302,52 @@ __global__ __launch_bounds__(256) void k_pcg_a(const ClassDev *__restrict__ cls,
     const int item = xcd_item(nitems);
     if (item < 0) return;
     pcg_a_item<real, T, U>(cls, items[item], pbuf);
+}
+
+// The same step for box-grid classes whose product is matrix-free
+// (mf_pcg: MIXED, one-sample tiles, multigrid set up): q = K p from the
+// cell moduli and the unit element matrix, the operator of the finest
+// smoothing products, with p = z + beta p_old formed at every gathered node;
+// no stored value is read.  An item covers max(1, it.w) consecutive chunks
+// and writes each chunk's p.q partial.
+__global__ __launch_bounds__(256, 4) void k_pcg_a_mf(const ClassDev *__restrict__ cls,
+                                                  const int4 *__restrict__ items, int nitems,
+                                                  int pbuf)
+{
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const int tile = it.y, chunk = it.z, nch = max(1, it.w);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const bool act = tile < C.nsamp && C.st[tile].active;  // one sample per tile
+    const float beta = act ? (float)C.st[tile].beta : 0.f;
+    const size_t vb = (size_t)tile * C.tile_vec;
+    const MSA_GLOBAL float *__restrict__ z = gbl((const float *)C.z + vb);
+    const MSA_GLOBAL float *__restrict__ po = gbl((const float *)(pbuf ? C.p1 : C.p0) + vb);
+    MSA_GLOBAL float *__restrict__ pn = gbl((float *)(pbuf ? C.p0 : C.p1) + vb);
+    MSA_GLOBAL float *__restrict__ q = gbl((float *)C.q + vb);
+    const double *Es = C.E + (size_t)tile * C.n_elem;
+    for (int ck = chunk; ck < chunk + nch; ck++) {
+        float pq = 0.f;
+        const int s0 = ck * C.chunk, s1 = min(s0 + C.chunk, C.nslices);
+        if (act) {
+            for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+                const int br = slice * 64 + lane;
+                if (br >= C.nb) continue;
+                float y[3], own[3];
+                mf_product<true, true>(C, z, Es, br, y[0], y[1], y[2], po, beta, own);
+                st3(pn, br, own[0], own[1], own[2]);
+                st3(q, br, y[0], y[1], y[2]);
+                pq += own[0] * y[0] + own[1] * y[1] + own[2] * y[2];
+            }
+        }
+        double acc[1] = {(double)pq};
+        block_partials<1, 1>(acc, C.partial + ((size_t)tile * C.nchunks + ck) * 3);
+        // the next chunk's partials reuse the same LDS rows
+        if (ck + 1 < chunk + nch) __syncthreads();
+    }
 }
 
 // Update + preconditioner + stopping test (SolverPCG.cs:122-136):
@@ -3186,11 +3240,17 @@ __device__ __forceinline__ void mf_mass(cfloat *mh, const float (&xv)[27][3], co
 // BOX selects the box-grid form at compile time: kernels holding both
 // forms would share one scalar-register budget for their element-matrix
 // loads.
-template <bool BOX>
+//
+// PG (box grids): the product of p = v + beta v2 with p formed at every
+// gathered position (the PCG search direction, gathered from z and p_old);
+// pown receives the node's own p.
+template <bool BOX, bool PG>
 __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL float *v,
                                            const double *Es, int br, float &y0, float &y1,
-                                           float &y2)
+                                           float &y2, const MSA_GLOBAL float *v2, float beta,
+                                           float *pown)
 {
+    static_assert(BOX || !PG, "the gathered search direction is a box-grid form");
     const int g = gbl(C.mf_bpos)[br];
     const int i = g & 1023, j = (g >> 10) & 1023, k = g >> 20;
     const int nx = __builtin_amdgcn_readfirstlane(C.mf_nx);
@@ -3203,6 +3263,19 @@ __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL f
     const float cK = C.dyn ? (float)C.cK : 1.f, cM = C.dyn ? (float)C.cM : 0.f;
     cfloat *kh = mf_khat(C);
     float acc[3] = {0.f, 0.f, 0.f};
+    // one node's three components at byte offset o (past the end: zeros)
+    const __amdgpu_buffer_rsrc_t vb2 =
+        PG ? wave_buffer((const void *)v2, (unsigned)nb * 12u) : vb;
+    auto node = [&](unsigned o) -> f32x3 {
+        f32x3 w = buffer_load_f32x3(vb, (int)o, 0, 0);
+        if constexpr (PG) {
+            const f32x3 u = buffer_load_f32x3(vb2, (int)o, 0, 0);
+            w.x = w.x + beta * u.x;
+            w.y = w.y + beta * u.y;
+            w.z = w.z + beta * u.z;
+        }
+        return w;
+    };
     if constexpr (BOX) {
         const int b0x = __builtin_amdgcn_readfirstlane(C.mf_box_lo[0]);
         const int b0y = __builtin_amdgcn_readfirstlane(C.mf_box_lo[1]);
@@ -3227,10 +3300,15 @@ __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL f
         }
 #pragma unroll
         for (int m = 0; m < 9; m++) {
-            const f32x3 w = buffer_load_f32x3(vb, (int)off[m], 0, 0);
+            const f32x3 w = node(off[m]);
             x0[m][0] = w.x;
             x0[m][1] = w.y;
             x0[m][2] = w.z;
+        }
+        if constexpr (PG) {
+            pown[0] = x0[4][0];
+            pown[1] = x0[4][1];
+            pown[2] = x0[4][2];
         }
         // lane 0: the column at x - 1, lane 63: at x + 1 (zeros past the
         // box ends; other lanes take them from their neighbours)
@@ -3241,8 +3319,7 @@ __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL f
             const unsigned sh = lane == 0 ? (unsigned)-12 : 12u;
 #pragma unroll
             for (int m = 0; m < 9; m++) {
-                const f32x3 w =
-                    buffer_load_f32x3(vb, (int)((okx && off[m] != vout) ? off[m] + sh : vout), 0, 0);
+                const f32x3 w = node((okx && off[m] != vout) ? off[m] + sh : vout);
                 xf[m][0] = w.x;
                 xf[m][1] = w.y;
                 xf[m][2] = w.z;
@@ -3261,19 +3338,39 @@ __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL f
                                   : __builtin_amdgcn_update_dpp(o, s, 0x130, 0xf, 0xf, false);
             return __builtin_bit_cast(float, v);
         };
+        // PG: each cell's block row times the corners' differences from the
+        // node's own value, Khat[a][.] (x_o - x_a), which equals Khat[a][.] x
+        // since the blocks of a row sum to zero (a translation stores no
+        // energy).  Smooth search directions make K p a small difference of
+        // large terms; the differences are small, so the FP32 product keeps
+        // the accuracy the PCG needs.  A corner outside the box counts as
+        // x_o = 0 (its difference -x_a).
+        float xn[3] = {0.f, 0.f, 0.f};
+        if constexpr (PG) {
+            xn[0] = x0[4][0];
+            xn[1] = x0[4][1];
+            xn[2] = x0[4][2];
+        }
 #pragma unroll
         for (int c = 0; c < 8; c++) {
             const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
             const int a = (1 - cx) + 2 * (1 - cy) + 4 * (1 - cz);  // the node's corner
+            const float f = cx ? fp : fm;
             float t0[3] = {0.f, 0.f, 0.f}, ts[3] = {0.f, 0.f, 0.f};
 #pragma unroll
             for (int o = 0; o < 8; o++) {
+                if (PG && o == a) continue;  // the node itself: difference 0
                 const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
                 const int m = (cy + oy) + 3 * (cz + oz);
                 cfloat *kab = kh + (a * 8 + o) * 9;
                 const int dx = cx + ox;  // 0: x - 1, 1: own column, 2: x + 1
-                const float xb[3] = {col(m, dx, 0), col(m, dx, 1), col(m, dx, 2)};
-                float(&tt)[3] = dx == 1 ? t0 : ts;
+                float xb[3];
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    const float v = col(m, dx, q);
+                    xb[q] = !PG ? v : (dx == 1 ? v - xn[q] : __builtin_fmaf(f, v, -xn[q]));
+                }
+                float(&tt)[3] = (PG || dx == 1) ? t0 : ts;
 #pragma unroll
                 for (int p = 0; p < 3; p++) {
                     tt[p] = __builtin_fmaf(kab[3 * p], xb[0], tt[p]);
@@ -3281,10 +3378,10 @@ __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL f
                     tt[p] = __builtin_fmaf(kab[3 * p + 2], xb[2], tt[p]);
                 }
             }
-            const float f = cx ? fp : fm;
 #pragma unroll
             for (int p = 0; p < 3; p++)
-                acc[p] = __builtin_fmaf(Ec[c], __builtin_fmaf(f, ts[p], t0[p]), acc[p]);
+                acc[p] = PG ? __builtin_fmaf(Ec[c], t0[p], acc[p])
+                            : __builtin_fmaf(Ec[c], __builtin_fmaf(f, ts[p], t0[p]), acc[p]);
             // keep each cell's 72 scalar loads next to their use
             __builtin_amdgcn_sched_barrier(0);
         }
@@ -4872,6 +4969,15 @@ hipError_t launch_pcg_a(hipStream_t s, bool mixed, int T, int U, const ClassDev 
     }
     if (U != 3) return hipErrorInvalidValue;  // lane tiles T > 1 use groups of 3 slots
     MSA_PCG_LAUNCH(k_pcg_a, d_cls, items, nitems, pbuf, active_count);
+}
+
+hipError_t launch_pcg_a_mf(hipStream_t s, const ClassDev *d_cls, const int4 *items, int nitems,
+                           int pbuf)
+{
+    if (nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pcg_a_mf, dim3(xcd_grid(nitems)), dim3(256), 0, s, d_cls, items, nitems,
+                       pbuf);
+    return hipGetLastError();
 }
 
 template <typename real, int U>

@@ -233,6 +233,9 @@ struct ClassDev {
     // sub-box of the grid (mf_box; neighbour block = own block + offset)
     int mf_cell_id, mf_box;
     int mf_box_lo[3], mf_box_hi[3];
+    // the PCG product q = K p of this box-grid class matrix-free as well
+    // (set per solve: MIXED, one-sample tiles, multigrid set up)
+    int mf_pcg;
     // FP64 copy of the stored values (same layout as a one-sample-tile FP64
     // batch) for the coarsest multigrid level of a MIXED batch: the band
     // factor is computed from these, not from the FP32 values (null: none)
@@ -341,6 +344,8 @@ hipError_t launch_pcg_init(hipStream_t s, bool mixed, int T, const ClassDev *d_c
 // U: slots per group of a row (1..4 at T = 1, 3 for T > 1)
 hipError_t launch_pcg_a(hipStream_t s, bool mixed, int T, int U, const ClassDev *d_cls,
                         const int4 *items, int nitems, int pbuf, int *active_count);
+hipError_t launch_pcg_a_mf(hipStream_t s, const ClassDev *d_cls, const int4 *items, int nitems,
+                           int pbuf);
 hipError_t launch_pcg_b(hipStream_t s, bool mixed, int T, const ClassDev *d_cls, const int4 *items,
                         int nitems, int pbuf, double tol, int max_it, int *active_count);
 hipError_t launch_ir_begin(hipStream_t s, int T, const ClassDev *d_cls, const int4 *items,

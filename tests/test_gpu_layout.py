@@ -236,3 +236,53 @@ def test_full_storage_switch(ctx, precision, precond):
             assert rel(sols[True][s], sols[False][s]) < 1e-11
     b.close()
     mesh.close()
+
+
+@pytest.mark.parametrize("sizes", [(8,), (8, 12)])
+def test_matrix_free_pcg_product(ctx, sizes):
+    """MIXED multigrid batches of box grids take the matrix-free PCG product
+    (msa_batch_product_form 2) by default, on the classes the multigrid
+    coarsens (those hold its grid tables): the FP32 operator of the finest
+    smoothing products, with the FP64 residual replacements fixing the
+    accuracy.  Its solves meet the 1e-8 bar against the oracle's direct
+    solve, as the stored upper-triangle product's do (same iteration counts
+    within 2); a Jacobi batch keeps the stored product."""
+    print(f"\n[case] matrix-free PCG product sizes={sizes}", flush=True)
+    models = [msa.cantilever(n) for n in sizes]
+    meshes = [msa.Mesh(ctx, m.xyz, m.conn, m.dof, m.n_free) for m in models]
+    rng = np.random.default_rng(15)
+    ns = 3
+    E = [moduli(rng, ns, m.n_elem) for m in models]
+    sols, iters = {}, {}
+    for mf in (True, False):
+        b = msa.Batch(ctx, meshes, [ns] * len(models), L.MSA_MIXED, 1)
+        b.set_preconditioner(L.MSA_PRECOND_MULTIGRID)
+        b.set_matrix_free_product(mf)
+        for c, m in enumerate(models):
+            b.set_modulus(c, E[c])
+            b.set_rhs(c, m.f)
+        b.assemble(0.3)
+        b.solve(tol=1e-9, max_it=20000, graph_iters=8)
+        for c in range(len(models)):
+            assert b.product_form(c) == ("matrix_free" if mf else "upper_triangle")
+            st = b.stats(c)
+            assert (st["status"] == msa.MSA_OK).all(), st
+        sols[mf] = [[b.solution(c, s) for s in range(ns)] for c in range(len(models))]
+        iters[mf] = [b.stats(c)["iters"].copy() for c in range(len(models))]
+        b.close()
+    for c, m in enumerate(models):
+        for s in range(ns):
+            ud = direct(m, E[c][s])
+            print(f"  class {c} sample {s}: iterations {iters[True][c][s]} / {iters[False][c][s]}")
+            assert rel(sols[True][c][s], ud) < 1e-8
+            assert rel(sols[False][c][s], ud) < 1e-8
+            assert abs(int(iters[True][c][s]) - int(iters[False][c][s])) <= 2
+    b = msa.Batch(ctx, meshes[:1], [2], L.MSA_MIXED, 1)
+    b.set_modulus(0, E[0][:2])
+    b.set_rhs(0, models[0].f)
+    b.assemble(0.3)
+    b.solve(tol=1e-9, max_it=20000, graph_iters=8)
+    assert b.product_form(0) == "upper_triangle"
+    b.close()
+    for mesh in meshes:
+        mesh.close()
