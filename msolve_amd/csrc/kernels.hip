@@ -1169,8 +1169,7 @@ __device__ __forceinline__ real pcg_a_sym_rows(const ClassDev &C, int s0, int s1
 template <bool BOX, bool PG = false>
 __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL float *v,
                                            const double *Es, int br, float &y0, float &y1,
-                                           float &y2, const MSA_GLOBAL float *v2 = nullptr,
-                                           float beta = 0.f, float *pown = nullptr);
+                                           float &y2, float *pown = nullptr);
 
 // Search direction + product + curvature (SolverPCGSimpleSearchVectorCalculator
 // .cs:18-41, MCPCGDirect.cs:48-51): p = z + beta p_old on the fly for every
@@ -1305,12 +1304,35 @@ __global__ __launch_bounds__(256) void k_pcg_a(const ClassDev *__restrict__ cls,
 }
 
 // The same step for box-grid classes whose product is matrix-free
-// (mf_pcg: MIXED, one-sample tiles, multigrid set up): q = K p from the
-// cell moduli and the unit element matrix, the operator of the finest
-// smoothing products, with p = z + beta p_old formed at every gathered node;
-// no stored value is read.  An item covers max(1, it.w) consecutive chunks
-// and writes each chunk's p.q partial.
-__global__ __launch_bounds__(256, 4) void k_pcg_a_mf(const ClassDev *__restrict__ cls,
+// (mf_pcg: MIXED, one-sample tiles, multigrid set up), in two passes over
+// an item's rows.  k_pcg_p writes p = z + beta p_old; k_pcg_a_mf then
+// computes q = K p from the cell moduli and the unit element matrix (the
+// operator of the finest smoothing products) with one gather per node,
+// and the p.q partials.  No stored value is read.  An item covers
+// max(1, it.w) consecutive chunks.
+__global__ __launch_bounds__(256) void k_pcg_p(const ClassDev *__restrict__ cls,
+                                               const int4 *__restrict__ items, int nitems,
+                                               int pbuf)
+{
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const int tile = it.y, nch = max(1, it.w);
+    if (tile >= C.nsamp || !C.st[tile].active) return;  // one sample per tile
+    const float beta = (float)C.st[tile].beta;
+    const size_t vb = (size_t)tile * C.tile_vec;
+    const MSA_GLOBAL float *__restrict__ z = gbl((const float *)C.z + vb);
+    const MSA_GLOBAL float *__restrict__ po = gbl((const float *)(pbuf ? C.p1 : C.p0) + vb);
+    MSA_GLOBAL float *__restrict__ pn = gbl((float *)(pbuf ? C.p0 : C.p1) + vb);
+    const int r0 = it.z * C.chunk * 64, r1 = min((it.z + nch) * C.chunk * 64, C.nb);
+    for (int br = r0 + (int)threadIdx.x; br < r1; br += blockDim.x) {
+        const vec3<float> zc = ld3(z, br), pc = ld3(po, br);
+        st3(pn, br, zc.x + beta * pc.x, zc.y + beta * pc.y, zc.z + beta * pc.z);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pcg_a_mf(const ClassDev *__restrict__ cls,
                                                   const int4 *__restrict__ items, int nitems,
                                                   int pbuf)
 {
@@ -1322,11 +1344,8 @@ __global__ __launch_bounds__(256, 4) void k_pcg_a_mf(const ClassDev *__restrict_
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const bool act = tile < C.nsamp && C.st[tile].active;  // one sample per tile
-    const float beta = act ? (float)C.st[tile].beta : 0.f;
     const size_t vb = (size_t)tile * C.tile_vec;
-    const MSA_GLOBAL float *__restrict__ z = gbl((const float *)C.z + vb);
-    const MSA_GLOBAL float *__restrict__ po = gbl((const float *)(pbuf ? C.p1 : C.p0) + vb);
-    MSA_GLOBAL float *__restrict__ pn = gbl((float *)(pbuf ? C.p0 : C.p1) + vb);
+    const MSA_GLOBAL float *__restrict__ p = gbl((const float *)(pbuf ? C.p0 : C.p1) + vb);
     MSA_GLOBAL float *__restrict__ q = gbl((float *)C.q + vb);
     const double *Es = C.E + (size_t)tile * C.n_elem;
     for (int ck = chunk; ck < chunk + nch; ck++) {
@@ -1337,8 +1356,7 @@ __global__ __launch_bounds__(256, 4) void k_pcg_a_mf(const ClassDev *__restrict_
                 const int br = slice * 64 + lane;
                 if (br >= C.nb) continue;
                 float y[3], own[3];
-                mf_product<true, true>(C, z, Es, br, y[0], y[1], y[2], po, beta, own);
-                st3(pn, br, own[0], own[1], own[2]);
+                mf_product<true, true>(C, p, Es, br, y[0], y[1], y[2], own);
                 st3(q, br, y[0], y[1], y[2]);
                 pq += own[0] * y[0] + own[1] * y[1] + own[2] * y[2];
             }
@@ -3241,16 +3259,15 @@ __device__ __forceinline__ void mf_mass(cfloat *mh, const float (&xv)[27][3], co
 // forms would share one scalar-register budget for their element-matrix
 // loads.
 //
-// PG (box grids): the product of p = v + beta v2 with p formed at every
-// gathered position (the PCG search direction, gathered from z and p_old);
-// pown receives the node's own p.
+// PG (box grids): the PCG product K p, summed over the corners'
+// differences from the node's value (below); pown receives the node's own
+// value of v.
 template <bool BOX, bool PG>
 __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL float *v,
                                            const double *Es, int br, float &y0, float &y1,
-                                           float &y2, const MSA_GLOBAL float *v2, float beta,
-                                           float *pown)
+                                           float &y2, float *pown)
 {
-    static_assert(BOX || !PG, "the gathered search direction is a box-grid form");
+    static_assert(BOX || !PG, "the difference form is a box-grid form");
     const int g = gbl(C.mf_bpos)[br];
     const int i = g & 1023, j = (g >> 10) & 1023, k = g >> 20;
     const int nx = __builtin_amdgcn_readfirstlane(C.mf_nx);
@@ -3264,18 +3281,7 @@ __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL f
     cfloat *kh = mf_khat(C);
     float acc[3] = {0.f, 0.f, 0.f};
     // one node's three components at byte offset o (past the end: zeros)
-    const __amdgpu_buffer_rsrc_t vb2 =
-        PG ? wave_buffer((const void *)v2, (unsigned)nb * 12u) : vb;
-    auto node = [&](unsigned o) -> f32x3 {
-        f32x3 w = buffer_load_f32x3(vb, (int)o, 0, 0);
-        if constexpr (PG) {
-            const f32x3 u = buffer_load_f32x3(vb2, (int)o, 0, 0);
-            w.x = w.x + beta * u.x;
-            w.y = w.y + beta * u.y;
-            w.z = w.z + beta * u.z;
-        }
-        return w;
-    };
+    auto node = [&](unsigned o) -> f32x3 { return buffer_load_f32x3(vb, (int)o, 0, 0); };
     if constexpr (BOX) {
         const int b0x = __builtin_amdgcn_readfirstlane(C.mf_box_lo[0]);
         const int b0y = __builtin_amdgcn_readfirstlane(C.mf_box_lo[1]);
@@ -4975,6 +4981,8 @@ hipError_t launch_pcg_a_mf(hipStream_t s, const ClassDev *d_cls, const int4 *ite
                            int pbuf)
 {
     if (nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pcg_p, dim3(xcd_grid(nitems)), dim3(256), 0, s, d_cls, items, nitems,
+                       pbuf);
     hipLaunchKernelGGL(k_pcg_a_mf, dim3(xcd_grid(nitems)), dim3(256), 0, s, d_cls, items, nitems,
                        pbuf);
     return hipGetLastError();
