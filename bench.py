@@ -265,11 +265,12 @@ def kernel_a_bytes(b, rs, full=False):
 
 def mf_pcg_work(b, models):
     """FP32 operations and algorithmic HBM bytes of all matrix-free PCG
-    product launches (k_pcg_a_mf) of one solve, None unless every class
-    takes that product.  Per iteration and active sample: the same 150
-    flop per (free node, incident cell) pair as the matrix-free smoothing
-    products; bytes: z and p_old read, p and q written (4 vectors of
-    4*3*nb) and the moduli (8 per element)."""
+    product launches (k_pcg_p + k_pcg_a_mf) of one solve, None unless
+    every class takes that product.  Per iteration and active sample: the
+    same 150 flop per (free node, incident cell) pair as the matrix-free
+    smoothing products; bytes: z and p_old read and p written by the p
+    pass, p gathered and q written by the product (5 vectors of 4*3*nb),
+    and the moduli (8 per element)."""
     flops = byts = 0.0
     for c, m in enumerate(models):
         if b.product_form(c) != "matrix_free":
@@ -280,7 +281,7 @@ def mf_pcg_work(b, models):
         inc = float(free_node[conn].sum())
         nb = float(free_node.sum())
         flops += its * inc * 150.0
-        byts += its * (4 * 4 * 3 * nb + 8.0 * conn.shape[0])
+        byts += its * (5 * 4 * 3 * nb + 8.0 * conn.shape[0])
     return flops, byts
 
 
@@ -326,8 +327,9 @@ def back_to_back(roof, b, reps=20, rs=8):
 def spmv_name(b, real, tile):
     """Name of the SpMV+dot kernel entry, saying which product it runs."""
     if all(b.product_form(c) == "matrix_free" for c in range(len(b.meshes))):
-        return ("k_pcg_a_mf (matrix-free FP32 PCG product K p on box grids from the cell "
-                "moduli, the corners' differences from the node's value; + p update + p.q)")
+        return ("k_pcg_p + k_pcg_a_mf (p = z + beta p_old, then the matrix-free FP32 PCG "
+                "product K p on box grids from the cell moduli, the corners' differences from "
+                "the node's value, + p.q; time of both kernels)")
     if all(b.layout_info(c)["upper_triangle_product"] for c in range(len(b.meshes))):
         return (f"k_pcg_a<{real},{tile}> (symmetric SpMV from the stored upper triangle "
                 "+ p update + p.q)")

@@ -344,8 +344,9 @@ int msa_batch_get_kernel_timing(msa_batch *b, double *ms_total,
 /* Algorithmic bytes of one launch of the SpMV+dot kernel over every sample
  * of the batch: bytes_per_launch[0] the kernel's (values + shared indices +
  * vectors; on box grids with one-sample tiles the upper triangle of the
- * values, diagonal included, and no indices; matrix-free classes: the four
- * vectors and the FP64 cell moduli), bytes_per_launch[1] a plain SpMV's
+ * values, diagonal included, and no indices; matrix-free classes: z and
+ * p_old read and p written by the p pass, p and q by the product, and the
+ * FP64 cell moduli), bytes_per_launch[1] a plain SpMV's
  * (all values + indices + one vector read and one written). */
 int msa_batch_spmv_bytes(msa_batch *b, double *bytes_per_launch);
 /* Measurement (no counterpart in the reference): times `reps` back-to-back

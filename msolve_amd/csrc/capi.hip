@@ -3326,8 +3326,8 @@ int msa_batch_spmv_bytes(msa_batch *b, double *bytes)
         const MeshHost &m = C.mesh->h;
         const double S = C.nsamp, nnz = (double)m.nnz_scalar, n = 3.0 * m.nb;
         const double idx = 4.0 * (double)m.bcol.size() + 4.0 * (C.nslices + 1);
-        if (C.dev.mf_pcg)  // matrix-free: the four vectors and the cell moduli
-            kernel += S * 4.0 * rs * n + S * 8.0 * m.n_elem;
+        if (C.dev.mf_pcg)  // matrix-free: the p pass and the product's vectors, the moduli
+            kernel += S * 5.0 * rs * n + S * 8.0 * m.n_elem;
         else if (C.dev.sym)  // the upper triangle with the diagonal; offsets, not indices
             kernel += S * rs * 0.5 * (nnz + n) + S * 4.0 * rs * n;
         else
