@@ -238,7 +238,7 @@ def test_full_storage_switch(ctx, precision, precond):
     mesh.close()
 
 
-@pytest.mark.parametrize("sizes", [(8,), (8, 12)])
+@pytest.mark.parametrize("sizes", [(8,), (8, 12), ("partly_fixed", 8)])
 def test_matrix_free_pcg_product(ctx, sizes):
     """MIXED multigrid batches of box grids take the matrix-free PCG product
     (msa_batch_product_form 2) by default, on the classes the multigrid
@@ -248,7 +248,11 @@ def test_matrix_free_pcg_product(ctx, sizes):
     solve, as the stored upper-triangle product's do (same iteration counts
     within 2); a Jacobi batch keeps the stored product."""
     print(f"\n[case] matrix-free PCG product sizes={sizes}", flush=True)
-    models = [msa.cantilever(n) for n in sizes]
+    if sizes[0] == "partly_fixed":  # a box grid with partly constrained nodes
+        models = [model_of("partly_fixed", sizes[1])]
+        models[0].n_elem = models[0].conn.shape[0]
+    else:
+        models = [msa.cantilever(n) for n in sizes]
     meshes = [msa.Mesh(ctx, m.xyz, m.conn, m.dof, m.n_free) for m in models]
     rng = np.random.default_rng(15)
     ns = 3
