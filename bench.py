@@ -345,10 +345,13 @@ def spmv_roofline(b, kt, rs, name, counter_file, models=None):
     if mfw is not None:
         flop, byts = mfw[0] / max(launches, 1), mfw[1] / max(launches, 1)
         ach = flop / a_s / 1e12 if a_s > 0 else 0.0
+        ratio = counters("k_pcg_a_mf_counters.json")
         return dict(bound="valu", achieved=ach, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
                     frac=ach / FP32_PEAK_TFLOPS,
                     frac_of_valu_without_packed_fp32=ach / FP32_VALU_NOPACK_TFLOPS,
-                    traffic=None, kernel=name, launches=launches, avg_launch_ms=1e3 * a_s,
+                    traffic=byts * ratio if ratio is not None else None,
+                    traffic_over_algorithmic=ratio,
+                    kernel=name, launches=launches, avg_launch_ms=1e3 * a_s,
                     avg_flop_per_launch=flop, avg_bytes_per_launch=byts,
                     hbm_gbs=byts / a_s / 1e9 if a_s > 0 else 0.0,
                     ms_total=kt["spmv_dot"]["ms"])
