@@ -290,3 +290,34 @@ def test_matrix_free_pcg_product(ctx, sizes):
     b.close()
     for mesh in meshes:
         mesh.close()
+
+
+def test_matrix_free_product_switch_on_one_batch(ctx):
+    """msa_batch_set_matrix_free_product on one batch between solves: the
+    next solve rebuilds the product's work items and captures its graphs
+    again; switching back reproduces the first solve bit for bit, and both
+    forms meet the 1e-8 bar."""
+    print("\n[case] matrix-free product switch on one batch", flush=True)
+    model = msa.cantilever(8)
+    rng = np.random.default_rng(16)
+    ns = 3
+    E = moduli(rng, ns, model.n_elem)
+    mesh = msa.Mesh(ctx, model.xyz, model.conn, model.dof, model.n_free)
+    b = msa.Batch(ctx, [mesh], [ns], L.MSA_MIXED, 1)
+    b.set_preconditioner(L.MSA_PRECOND_MULTIGRID)
+    b.set_modulus(0, E)
+    b.set_rhs(0, model.f)
+    b.assemble(0.3)
+    sols = []
+    for mf in (True, False, True):
+        b.set_matrix_free_product(mf)
+        b.solve(tol=1e-9, max_it=20000, graph_iters=8)
+        assert b.product_form(0) == ("matrix_free" if mf else "upper_triangle")
+        assert (b.stats(0)["status"] == msa.MSA_OK).all()
+        sols.append([b.solution(0, s) for s in range(ns)])
+    for s in range(ns):
+        assert np.array_equal(sols[0][s], sols[2][s])
+        ud = direct(model, E[s])
+        assert rel(sols[0][s], ud) < 1e-8 and rel(sols[1][s], ud) < 1e-8
+    b.close()
+    mesh.close()
