@@ -3318,9 +3318,11 @@ __device__ __forceinline__ void mf_product(const ClassDev &C, const MSA_GLOBAL f
             x0[m][2] = w.z;
         }
         if constexpr (PG) {
-            pown[0] = x0[4][0];
-            pown[1] = x0[4][1];
-            pown[2] = x0[4][2];
+            if (pown) {
+                pown[0] = x0[4][0];
+                pown[1] = x0[4][1];
+                pown[2] = x0[4][2];
+            }
         }
         // lane 0: the column at x - 1, lane 63: at x + 1 (zeros past the
         // box ends; other lanes take them from their neighbours)
@@ -3692,8 +3694,9 @@ __global__ __launch_bounds__(256) void k_mg_smooth(const ClassDev *__restrict__ 
                 if constexpr (T == 1) {
                     const MgRow<real> w = mg_load1<real, STEP>(b, dinv, z, A, B, br);
                     if constexpr (MF != 0)
-                        mf_product<MF == 2>(C, (const MSA_GLOBAL float *)src,
-                                            C.E + (size_t)tile * C.n_elem, br, y[0], y[1], y[2]);
+                        mf_product<MF == 2, MF == 2>(C, (const MSA_GLOBAL float *)src,
+                                                     C.E + (size_t)tile * C.n_elem, br, y[0],
+                                                     y[1], y[2]);
                     else
                         row_product<real, T>(C, val, cidx, src, base, W, rr, s, y[0], y[1], y[2]);
                     mg_finish1<real, STEP>(w, dst, br, y, ith, c1, c2, rz);
