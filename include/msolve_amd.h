@@ -257,7 +257,7 @@ int msa_batch_set_symmetric_product(msa_batch *b, int enable);
  * stored values are still assembled (export, coarse levels, warm starts).
  * enable = 0: the stored-value product.  Takes effect at the next solve.
  * Replaces, for those classes, the role of SkylineMatrix2D.Multiply
- * (SkylineMatrix2D.cs:253-269) in SolverPCG.cs:117. */
+ * (SkylineMatrix2D.cs:253-269) in SolverPCG.cs:120. */
 int msa_batch_set_matrix_free_product(msa_batch *b, int enable);
 
 /* The PCG product form of class cls at the last solve: 0 the stored
