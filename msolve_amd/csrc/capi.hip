@@ -164,7 +164,9 @@ struct msa_batch {
     int4 *mg_co_items = nullptr;        // the coarsest levels
     int mg_nco_items = 0;
     int4 *mf_items = nullptr;           // solved classes with matrix-free products
-    int n_mf_items = 0;
+    int n_mf_items = 0;                 //   (general grids; box grids: runs of chunks below)
+    int4 *mf_box_items = nullptr;
+    int n_mf_box_items = 0;
     std::vector<int> mg_coarsest;       // class table indices of the coarsest levels
     int mg_max_levels = 0;              // max_levels the hierarchy was built with
     size_t d_cls_cap = 0;               // ClassDev entries allocated at d_cls
@@ -541,10 +543,14 @@ int launch_replacement(msa_batch *b, double tol, int max_it)
     MSA_TRY(timed(b, KK_OTHER, [&] { return launch_ru_fold(s, T, b->d_cls, b->d_items, ni); }));
     // matrix-free residual of the uniform-grid classes (multigrid set up),
     // the coloured element scatter for the others
-    const bool mfres = b->precond == MSA_PRECOND_MULTIGRID && b->mg_ready && b->n_mf_items > 0;
+    const bool mfres = b->precond == MSA_PRECOND_MULTIGRID && b->mg_ready &&
+                       b->n_mf_items + b->n_mf_box_items > 0;
     if (mfres)
         MSA_TRY(timed(b, KK_RES64, [&] {
-            return launch_residual64_mf(s, b->d_cls, b->mf_items, b->n_mf_items);
+            const hipError_t e = launch_residual64_mf(s, b->d_cls, b->mf_items, b->n_mf_items);
+            return e != hipSuccess
+                       ? e
+                       : launch_residual64_mf_box(s, b->d_cls, b->mf_box_items, b->n_mf_box_items);
         }));
     const int ncol = max_colours(b);
     for (int k = 0; k < ncol; k++) {
@@ -1101,7 +1107,7 @@ int mf_setup(msa_batch *b, ClassHost &C)
     void *p = nullptr;
     MSA_TRY(dalloc(b, &p, 640 * sizeof(float)));
     D.mf_k = (float *)p;
-    MSA_TRY(dalloc(b, &p, 640 * sizeof(double)));
+    MSA_TRY(dalloc(b, &p, kMfK64 * sizeof(double)));
     D.mf_k64 = (double *)p;
     D.mf_nx = g.nx;
     D.mf_ny = g.ny;
@@ -1296,10 +1302,16 @@ int mg_build(msa_batch *b, int max_levels)
     }
     MSA_TRY(items_to_device(b, all, &b->mg_lv_items, &b->mg_nlv_items));
     {
-        std::vector<int4> mfi;
+        std::vector<int4> mfi, mfb;
         for (int c = 0; c < ncls; c++)
-            if (b->cls[c].dev.mf) class_items(b->cls[c], c, mfi);
+            if (b->cls[c].dev.mf) {
+                if (b->cls[c].dev.mf_box)
+                    class_run_items(b->cls[c], c, mfb);
+                else
+                    class_items(b->cls[c], c, mfi);
+            }
         MSA_TRY(items_to_device(b, mfi, &b->mf_items, &b->n_mf_items));
+        MSA_TRY(items_to_device(b, mfb, &b->mf_box_items, &b->n_mf_box_items));
     }
     std::vector<int4> gall;
     for (size_t l = 0; l < b->lv.size(); l++)

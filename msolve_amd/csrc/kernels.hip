@@ -3480,6 +3480,8 @@ __global__ void k_mf_khat(const ClassDev *__restrict__ cp)
         const int blk = i / 9, r = i - 9 * (i / 9);
         const int a = C.mf_corner[blk >> 3], b = C.mf_corner[blk & 7];
         v = C.kc[(a * 8 + b) * 9 + r];
+        const int p = r / 3, q = r - 3 * (r / 3);
+        C.mf_k64[kMfK64T + (((blk >> 3) * 3 + q) * 8 + (blk & 7)) * 3 + p] = v;
     } else if (i < 640) {
         const int blk = i - 576;
         const int a = C.mf_corner[blk >> 3], b = C.mf_corner[blk & 7];
@@ -3580,6 +3582,156 @@ __global__ __launch_bounds__(256) void k_residual64_mf(const ClassDev *__restric
         const uint8_t fr = C.block_free[br];
         // the row's three components move as one 24-byte access; fixed
         // components are written back unchanged
+        const vec3<double> rv = ld3((const MSA_GLOBAL double *)r, br);
+        st3(r, br, (fr & 1) ? rv.x - y[0] : rv.x, (fr & 2) ? rv.y - y[1] : rv.y,
+            (fr & 4) ? rv.z - y[2] : rv.z);
+    }
+}
+
+// The same residual on box grids (free blocks = a lexicographic sub-box of
+// the grid, one-sample tiles), one input component q at a time: each lane
+// loads component q of the 9 (dy, dz) columns at its own x once and takes
+// the x - 1 / x + 1 columns from lanes l - 1 / l + 1 by DPP shifts (lanes
+// 0 and 63 load theirs), zero past a row end; the element matrix comes by
+// scalar loads of its transposed copy, 24 + 8 values per (cell, q).  The
+// general form above gathers the 8 corners of every cell (64 loads per
+// row).  Items cover max(1, it.w) chunks.
+using cdouble = __attribute__((address_space(4))) const double;
+__device__ __forceinline__ cdouble *mf_khat64(const ClassDev &C)
+{
+    unsigned long long kp = (unsigned long long)C.mf_k64;
+    asm volatile("" : "+s"(kp));
+    return (cdouble *)kp;
+}
+
+__global__ __launch_bounds__(256) void k_residual64_mf_box(const ClassDev *__restrict__ cls,
+                                                           const int4 *__restrict__ items,
+                                                           int nitems)
+{
+    const int item = xcd_item(nitems);
+    if (item < 0) return;
+    const int4 it = items[item];
+    const ClassDev &C = cls[it.x];
+    const int tile = it.y, chunk = it.z, nch = max(1, it.w);
+    const int sample = tile;
+    if (*C.due == 0 || sample >= C.nsamp || !C.st[sample].ir_active || !C.st[sample].ru_due) return;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nx = __builtin_amdgcn_readfirstlane(C.mf_nx);
+    const int ny = __builtin_amdgcn_readfirstlane(C.mf_ny);
+    const int nz = __builtin_amdgcn_readfirstlane(C.mf_nz);
+    const int nb = __builtin_amdgcn_readfirstlane(C.nb);
+    const int b0x = __builtin_amdgcn_readfirstlane(C.mf_box_lo[0]);
+    const int b0y = __builtin_amdgcn_readfirstlane(C.mf_box_lo[1]);
+    const int b0z = __builtin_amdgcn_readfirstlane(C.mf_box_lo[2]);
+    const int b1x = __builtin_amdgcn_readfirstlane(C.mf_box_hi[0]);
+    const int b1y = __builtin_amdgcn_readfirstlane(C.mf_box_hi[1]);
+    const int b1z = __builtin_amdgcn_readfirstlane(C.mf_box_hi[2]);
+    const int sy = b1x - b0x + 1, sz = sy * (b1y - b0y + 1);
+    const MSA_GLOBAL double *u = gbl((const double *)C.u64 + (size_t)tile * C.tile_vec);
+    MSA_GLOBAL double *r = gbl(C.r64 + (size_t)tile * C.tile_vec);
+    const double *Es = C.E + (size_t)tile * C.n_elem;
+    const __amdgpu_buffer_rsrc_t ub = wave_buffer((const void *)u, (unsigned)nb * 24u);
+    const unsigned uout = (unsigned)nb * 24u;
+    const int ne = __builtin_amdgcn_readfirstlane(C.n_elem);
+    const __amdgpu_buffer_rsrc_t eb = wave_buffer((const void *)Es, (unsigned)ne * 8u);
+    const unsigned eout = (unsigned)ne * 8u;
+    const double cK = C.dyn ? C.cK : 1.0, cM = C.dyn ? C.cM : 0.0;
+    const int s0 = chunk * C.chunk, s1 = min((chunk + nch) * C.chunk, C.nslices);
+    for (int slice = s0 + wave; slice < s1; slice += kWaves) {
+        const int br = slice * 64 + lane;
+        if (br >= nb) continue;
+        const int g = gbl(C.mf_bpos)[br];
+        const int i = g & 1023, j = (g >> 10) & 1023, k = g >> 20;
+        const bool lo[3] = {i > 0, j > 0, k > 0}, hi[3] = {i < nx - 1, j < ny - 1, k < nz - 1};
+        const bool blo[3] = {i > b0x, j > b0y, k > b0z}, bhi[3] = {i < b1x, j < b1y, k < b1z};
+        const unsigned own = (unsigned)br * 24u;
+        unsigned off[9];
+#pragma unroll
+        for (int m = 0; m < 9; m++) {
+            const int dy = m % 3, dz = m / 3;
+            const bool in = (dy != 0 || blo[1]) && (dy != 2 || bhi[1]) && (dz != 0 || blo[2]) &&
+                            (dz != 2 || bhi[2]);
+            off[m] = in ? own + (unsigned)(24 * ((dy - 1) * sy + (dz - 1) * sz)) : uout;
+        }
+        // the 8 cells' moduli (cells outside the grid: 0) and mass weights
+        double Ec[8], Mc[8];
+        {
+            const int cy1 = nx - 1, cz1 = (nx - 1) * (ny - 1);
+            const int c0 = (i - 1) + cy1 * (j - 1) + cz1 * (k - 1);
+            const bool ident = __builtin_amdgcn_readfirstlane(C.mf_cell_id) != 0;
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+                const bool cin = (cx ? hi[0] : lo[0]) && (cy ? hi[1] : lo[1]) && (cz ? hi[2] : lo[2]);
+                int ce = cin ? c0 + cx + cy * cy1 + cz * cz1 : 0;
+                if (!ident) ce = gbl(C.mf_cell)[ce];
+                const double e = buffer_load_f64(eb, (int)(cin ? (unsigned)ce * 8u : eout), 0, 0);
+                Ec[c] = e * cK;
+                Mc[c] = cin ? cM : 0.0;
+            }
+        }
+        // x - 1 / x + 1 values count where the neighbour is in the box
+        const double fm = blo[0] ? 1.0 : 0.0, fp = bhi[0] ? 1.0 : 0.0;
+        double y[3] = {0.0, 0.0, 0.0};
+#pragma unroll 1
+        for (int q = 0; q < 3; q++) {
+            double xq[9], xm[9], xp[9], ym = 0.0;
+#pragma unroll
+            for (int m = 0; m < 9; m++)
+                xq[m] = buffer_load_f64(ub, (int)(off[m] != uout ? off[m] + 8u * q : uout), 0, 0);
+            // lane 0: the column at x - 1, lane 63: at x + 1 (the other
+            // lanes' values are replaced by the shifts)
+#pragma unroll
+            for (int m = 0; m < 9; m++) xm[m] = 0.0;
+            if (lane == 0 || lane == 63) {
+                const bool okx = lane == 0 ? blo[0] : bhi[0];
+                const unsigned sh = lane == 0 ? (unsigned)-24 : 24u;
+#pragma unroll
+                for (int m = 0; m < 9; m++)
+                    xm[m] = buffer_load_f64(
+                        ub, (int)((okx && off[m] != uout) ? off[m] + sh + 8u * q : uout), 0, 0);
+            }
+#pragma unroll
+            for (int m = 0; m < 9; m++) {
+                xp[m] = fp * dpp_wave<kDppWaveShl1>(xm[m], xq[m]);
+                xm[m] = fm * dpp_wave<kDppWaveShr1>(xm[m], xq[m]);
+            }
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+                const int a = (1 - cx) + 2 * (1 - cy) + 4 * (1 - cz);  // the node's corner
+                double t[3] = {0.0, 0.0, 0.0}, mq = 0.0;
+#pragma unroll
+                for (int h = 0; h < 2; h++) {  // 4 corners (16 scalar values) at a time
+                    cdouble *kt = mf_khat64(C) + kMfK64T + (a * 3 + q) * 24;
+                    cdouble *mh = mf_khat64(C) + 576 + a * 8;
+#pragma unroll
+                    for (int o = 4 * h; o < 4 * h + 4; o++) {
+                        const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
+                        const int m = (cy + oy) + 3 * (cz + oz), dx = cx + ox;
+                        const double v = dx == 0 ? xm[m] : (dx == 1 ? xq[m] : xp[m]);
+#pragma unroll
+                        for (int p = 0; p < 3; p++) t[p] = __builtin_fma(kt[o * 3 + p], v, t[p]);
+                        mq = __builtin_fma(mh[o], v, mq);
+                    }
+                    if (h == 0) __builtin_amdgcn_sched_barrier(0);
+                }
+#pragma unroll
+                for (int p = 0; p < 3; p++) y[p] = __builtin_fma(Ec[c], t[p], y[p]);
+                ym = __builtin_fma(Mc[c], mq, ym);
+                // keep each (cell, component)'s 32 scalar loads next to their use
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            // the mass couples each component to itself only
+            if (q == 0)
+                y[0] += ym;
+            else if (q == 1)
+                y[1] += ym;
+            else
+                y[2] += ym;
+        }
+        const uint8_t fr = C.block_free[br];
         const vec3<double> rv = ld3((const MSA_GLOBAL double *)r, br);
         st3(r, br, (fr & 1) ? rv.x - y[0] : rv.x, (fr & 2) ? rv.y - y[1] : rv.y,
             (fr & 4) ? rv.z - y[2] : rv.z);
@@ -5438,6 +5590,15 @@ hipError_t launch_residual64_mf(hipStream_t s, const ClassDev *d_cls, const int4
 {
     if (nitems <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_residual64_mf, dim3(xcd_grid(nitems)), dim3(256), 0, s, d_cls, items, nitems);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual64_mf_box(hipStream_t s, const ClassDev *d_cls, const int4 *items,
+                                   int nitems)
+{
+    if (nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_residual64_mf_box, dim3(xcd_grid(nitems)), dim3(256), 0, s, d_cls, items,
+                       nitems);
     return hipGetLastError();
 }
 

@@ -136,6 +136,11 @@ struct GridInfo {
 int grid_info(const MeshHost &m, GridInfo &g);
 
 // Device view of one class of a batch (copied to device memory).
+// FP64 unit element matrix of matrix-free classes: Khat (576) + Mhat (64),
+// then Khat transposed by input component (576)
+constexpr int kMfK64T = 640;
+constexpr int kMfK64 = 640 + 576;
+
 struct ClassDev {
     int nb, nslices, ntiles, nsamp;
     int n_elem, nchunks, chunk, M;
@@ -227,6 +232,9 @@ struct ClassDev {
     // matrix-free smoothing products (mixed precision, one-sample tiles) on
     // a uniform structured grid: K v from the cell moduli and the shared
     // unit element matrix mf_k[576] (+ unit mass mf_k[576 + 64])
+    // (mf_k64: the same in FP64, then Khat again at kMfK64T with the 3x3
+    // blocks transposed and grouped by (a, input component q, b):
+    // [((a * 3 + q) * 8 + b) * 3 + p] = Khat[a][b][p][q])
     int mf, mf_nx, mf_ny, mf_nz;
     int mf_corner[8];
     // index shortcuts: element = cell (mf_cell_id); blocks = a lexicographic
@@ -344,6 +352,8 @@ hipError_t launch_pcg_init(hipStream_t s, bool mixed, int T, const ClassDev *d_c
 // U: slots per group of a row (1..4 at T = 1, 3 for T > 1)
 hipError_t launch_pcg_a(hipStream_t s, bool mixed, int T, int U, const ClassDev *d_cls,
                         const int4 *items, int nitems, int pbuf, int *active_count);
+hipError_t launch_residual64_mf_box(hipStream_t s, const ClassDev *d_cls, const int4 *items,
+                                   int nitems);
 hipError_t launch_pcg_a_mf(hipStream_t s, const ClassDev *d_cls, const int4 *items, int nitems,
                            int pbuf);
 hipError_t launch_pcg_b(hipStream_t s, bool mixed, int T, const ClassDev *d_cls, const int4 *items,
