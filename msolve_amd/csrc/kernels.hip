@@ -1358,11 +1358,13 @@ __global__ __launch_bounds__(256) void k_pcg_a_mf(const ClassDev *__restrict__ c
                 float y[3], own[3];
                 mf_product<true, true>(C, p, Es, br, y[0], y[1], y[2], own);
                 // a constrained component's row is zero in K (as in the
-                // stored product)
-                const uint8_t fr = C.block_free[br];
-                y[0] = (fr & 1) ? y[0] : 0.f;
-                y[1] = (fr & 2) ? y[1] : 0.f;
-                y[2] = (fr & 4) ? y[2] : 0.f;
+                // stored product); classes of fully free blocks have none
+                if (!__builtin_amdgcn_readfirstlane(C.mf_allfree)) {
+                    const uint8_t fr = C.block_free[br];
+                    y[0] = (fr & 1) ? y[0] : 0.f;
+                    y[1] = (fr & 2) ? y[1] : 0.f;
+                    y[2] = (fr & 4) ? y[2] : 0.f;
+                }
                 st3(q, br, y[0], y[1], y[2]);
                 pq += own[0] * y[0] + own[1] * y[1] + own[2] * y[2];
             }
